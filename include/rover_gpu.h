@@ -1,0 +1,276 @@
+/*
+ * rover_gpu.h -- C-ABI of the MI355X quantized graph search.
+ *
+ * Three faces, all plain C (pointers and sizes, no torch or HIP types):
+ *
+ *  1. The reference's FastRPC operator surface (calculator.idl), with the
+ *     qaic C mapping: `in sequence<T> x` -> `const T *x, int xLen`, `rout`
+ *     sequences -> `T *x, int xLen`, `rout int32` -> `int32_t *`.  Ops that
+ *     take file descriptors of shared buffers in the reference take host
+ *     pointers here (the data is copied into HBM once, at load).
+ *  2. The reference's library call for one query,
+ *     retrieval_pure_pq_search_scalar[_with_policy]
+ *     (retrieval_sqmd_pure_pq.h:264-278), same structs and return contract.
+ *  3. Batch extensions with no reference counterpart: many queries with
+ *     per-query beam width L and k in one persistent kernel, from host or
+ *     device buffers, plus the learned early-termination model.
+ *
+ * Return values of calculator_* and rover_* ops are AEEResult-style codes
+ * (ROVER_SUCCESS == AEE_SUCCESS == 0); the library search returns a count or
+ * -1, as the reference does.
+ */
+#ifndef ROVER_GPU_H
+#define ROVER_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int AEEResult;
+typedef uint64_t remote_handle64;
+
+#define ROVER_SUCCESS 0    /* AEE_SUCCESS   */
+#define ROVER_EFAILED 1    /* AEE_EFAILED   */
+#define ROVER_ENOMEMORY 2  /* AEE_ENOMEMORY */
+#define ROVER_EBADPARM 14  /* AEE_EBADPARM  */
+
+#define ROVER_ABI_VERSION 1        /* search-only, as calculator_rover_configure */
+#define ROVER_MAX_POOL 2000        /* rover_limits.h:40 DSP_SEARCH_MAX_POOL */
+#define ROVER_ENTRY_POINT_SEED 42u /* rover_limits.h:38, config.c:18 */
+
+/* ---------------------------------------------------------------- 1. IDL */
+
+/* Session open/close (src/app/dsp.c:18-31).  `uri` is ignored beyond
+ * selecting device 0; each handle owns its own loaded indexes. */
+int calculator_open(const char *uri, remote_handle64 *handle);
+int calculator_close(remote_handle64 handle);
+
+/* calculator.idl:15; main.c:28 -> dsp.c:8-17.  abi_version must be
+ * ROVER_ABI_VERSION; l1_bits (4 or 8) selects the SQ code width. */
+AEEResult calculator_rover_configure(remote_handle64 h, int32_t abi_version,
+                                     int32_t l1_bits);
+
+/* calculator.idl:36-45 (retrieval_load): SQ index = dense graph
+ * int32[num_vectors*graph_k] and global-scale l1 codes. */
+AEEResult calculator_retrieval_load(remote_handle64 h, const int32_t *graph,
+                                    int graphLen, const uint8_t *l1_codes,
+                                    int l1_codesLen, int32_t num_vectors,
+                                    int32_t dim, int32_t graph_k,
+                                    int32_t graph_entry_point, float l1_scale,
+                                    float l1_zero_point);
+
+/* calculator.idl:46-55 (retrieval_search) on the SQ index.  Seeds are the
+ * graph entry plus the LCG of retrieval_async_dsp_search.c:219-230 with
+ * seed 42.  Candidates are ordered by (distance, id). */
+AEEResult calculator_retrieval_search(remote_handle64 h, const float *query,
+                                      int queryLen, int32_t entry_points,
+                                      int32_t expand_count, int32_t pool_size,
+                                      int32_t num_out, int32_t *candidate_ids,
+                                      int candidate_idsLen,
+                                      float *candidate_dists,
+                                      int candidate_distsLen,
+                                      int32_t *actual_count);
+
+/* calculator.idl:56 */
+AEEResult calculator_retrieval_unload(remote_handle64 h);
+
+/* calculator.idl:119-131 (retrieval_pure_pq_load_dma): the IDL's
+ * `fd_pq_codes` sequence becomes the chunk base pointers. */
+AEEResult calculator_retrieval_pure_pq_load_dma(
+    remote_handle64 h, const uint8_t *const *pq_codes, int pq_codesLen,
+    const int32_t *pq_chunk_vector_counts, int pq_chunk_vector_countsLen,
+    int32_t pq_chunk_vectors, const float *pq_codebook, int pq_codebookLen,
+    int32_t num_vectors, int32_t dim, int32_t pq_m, int32_t pq_ks,
+    int32_t pq_subdim, int32_t pq_code_bytes, const uint8_t *index_uuid,
+    int index_uuidLen);
+
+/* calculator.idl:132 */
+AEEResult calculator_retrieval_pure_pq_unload_dma(remote_handle64 h);
+
+/* The pure-PQ graph stays host-owned in the reference
+ * (rpc_demo_pure_pq.c:491-497 demo_resolve_graph_index); on the GPU it is
+ * resident in HBM next to the codes. */
+AEEResult rover_pure_pq_graph_load(remote_handle64 h, const int32_t *graph,
+                                   int64_t graphLen, int32_t num_vectors,
+                                   int32_t graph_k, int32_t graph_entry_point);
+
+/* Residual index for online correction and final refine
+ * (retrieval_pq_residual_index_open, retrieval_pq_residual.c:716-835),
+ * given as flat arrays: codes [num_vectors][res_code_bytes] and
+ * qparams [num_vectors][2] = (scale, zero_point). */
+AEEResult rover_pure_pq_residual_load(remote_handle64 h,
+                                      const uint8_t *residual_codes,
+                                      int64_t residual_codesLen,
+                                      const float *qparams, int64_t qparamsLen,
+                                      int32_t residual_bits);
+
+/* Optional graph->original id map used by the refine tie-break
+ * (retrieval_final_refine.c:212-219); NULL restores the identity. */
+AEEResult rover_pure_pq_original_ids(remote_handle64 h, const int32_t *map,
+                                     int mapLen);
+
+/* ---------------------------------------------------- 2. library search */
+
+typedef struct PurePqSearchIndexView {
+    const int32_t *graph;
+    int graph_degree;
+    int num_vectors;
+    const uint8_t *const *code_chunks;
+    const int *chunk_vector_counts;
+    int chunk_count;
+    int chunk_vectors;
+    const float *codebook;
+    int dim;
+    int m;
+    int ks;
+    int subdim;
+} PurePqSearchIndexView;
+
+typedef struct PurePqSearchRequest {
+    const float *query;
+    const int *entry_ids;
+    int entry_count;
+    int expand_count;
+    int pool_capacity;
+    int num_out;
+} PurePqSearchRequest;
+
+typedef struct PurePqSearchStats {
+    uint64_t pq_scored_candidates;
+    uint64_t corrected_candidates;
+    uint64_t expanded_nodes;
+    uint64_t visited_dedup_skips;
+} PurePqSearchStats;
+
+typedef enum PurePqCorrectionMode {
+    PURE_PQ_CORRECTION_OFF = 0,
+    PURE_PQ_CORRECTION_ONLINE = 1
+} PurePqCorrectionMode;
+
+typedef int (*PurePqDistanceEvaluatorFn)(void *context, int vector_id,
+                                         float coarse_distance,
+                                         float *out_distance);
+
+typedef struct PurePqDistancePolicy {
+    PurePqCorrectionMode mode;
+    PurePqDistanceEvaluatorFn evaluator;
+    void *context;
+} PurePqDistancePolicy;
+
+/* The one ONLINE evaluator the GPU recognises: set policy.evaluator to
+ * rover_pq_residual_evaluator and policy.context to a RoverResidualView.
+ * The function itself only marks the policy; called on the host it
+ * returns -1.  Any other evaluator makes the search return -1. */
+typedef struct RoverResidualView {
+    const uint8_t *codes;   /* [num_vectors][res_code_bytes] */
+    const float *qparams;   /* [num_vectors][2] */
+    int bits;               /* 4 or 8 */
+} RoverResidualView;
+int rover_pq_residual_evaluator(void *context, int vector_id,
+                                float coarse_distance, float *out_distance);
+
+/* retrieval_sqmd_pure_pq.h:264-278.  Runs on the GPU of a process-wide
+ * session; the index view is copied to HBM on first use and kept while the
+ * same view (pointers and shape) is passed again.  Call
+ * rover_pure_pq_view_release() before changing an index in place. */
+int retrieval_pure_pq_search_scalar_with_policy(
+    const PurePqSearchIndexView *index, const PurePqSearchRequest *request,
+    const PurePqDistancePolicy *policy, int *out_ids, float *out_distances,
+    int out_capacity, PurePqSearchStats *out_stats);
+int retrieval_pure_pq_search_scalar(const PurePqSearchIndexView *index,
+                                    const PurePqSearchRequest *request,
+                                    int *out_ids, float *out_distances,
+                                    int out_capacity,
+                                    PurePqSearchStats *out_stats);
+void rover_pure_pq_view_release(void);
+
+/* ------------------------------------------------------------ 3. batch */
+
+enum { ROVER_METHOD_PQ = 0, ROVER_METHOD_SQ = 1 };
+
+/* Per-query status in out_counts when negative. */
+enum {
+    ROVER_QUERY_BAD_ROW = -2,
+    ROVER_QUERY_BAD_ENTRY = -3,
+    ROVER_QUERY_NONFINITE = -4,
+    ROVER_QUERY_BAD_PARAM = -5
+};
+
+typedef struct RoverSearchBatch {
+    const float *queries;        /* [num_queries][dim]                      */
+    int num_queries;
+    const int32_t *pool_sizes;   /* per-query L, or NULL -> pool_size       */
+    const int32_t *num_outs;     /* per-query k, or NULL -> num_out         */
+    int pool_size;
+    int num_out;
+    int method;                  /* ROVER_METHOD_PQ / ROVER_METHOD_SQ       */
+    int correction;              /* PQ: 0 off (ADC ranking), 1 online       */
+    int refine_candidates;       /* PQ: 0 off, else refine the top F        */
+    int entry_points;            /* seeds: graph entry + LCG                */
+    uint32_t seed;               /* 0 -> ROVER_ENTRY_POINT_SEED             */
+    int expand_count;
+    int32_t *out_ids;            /* [num_queries][out_stride]               */
+    float *out_dists;            /* [num_queries][out_stride]               */
+    int32_t *out_counts;         /* [num_queries]                           */
+    int out_stride;
+    uint64_t *out_stats;         /* [num_queries][4] or NULL                */
+    /* launch shape; 0 -> tuned default */
+    int hash_log2;
+    int probe_limit;
+    int grid_waves;
+    /* learned early termination (needs rover_learned_stop_set) */
+    int learned_stop;
+    float stop_threshold;        /* 0 -> 0.9 */
+} RoverSearchBatch;
+
+/* Host buffers in and out; returns when results are in place. */
+AEEResult rover_search_batch(remote_handle64 h, const RoverSearchBatch *b);
+
+/* Every pointer of `b` is a device (HBM) address; asynchronous on `stream`
+ * (a hipStream_t, NULL = default stream). */
+AEEResult rover_search_batch_device(remote_handle64 h,
+                                    const RoverSearchBatch *b, void *stream);
+
+/* Seeds of one query: index 0 is graph_entry (when in range), the rest the
+ * LCG s = s*1103515245 + 12345, id = (s * n) >> 32
+ * (rpc_demo_pure_pq.c:192-208).  Returns the count. */
+int rover_make_seed_ids(int num_vectors, int entry_count, int graph_entry,
+                        uint32_t seed, int32_t *out_ids);
+
+/* Index building on the device, bit-identical to the reference encoders:
+ * PQ codes (retrieval_pq_core.c:226-268) and per-vector residual codes and
+ * qparams (retrieval_pq_residual.c:314-385).  Device pointers; synchronous
+ * with respect to the default stream. */
+AEEResult rover_build_pq_encode(const float *d_x, int n, int dim, int m, int ks,
+                                const float *d_codebook, uint8_t *d_codes);
+AEEResult rover_build_residual_encode(const float *d_x, const uint8_t *d_codes,
+                                      int n, int dim, int m, int ks,
+                                      const float *d_codebook, int bits,
+                                      uint8_t *d_res, float *d_qparams);
+
+/* Device memory helpers, so callers need no HIP headers. */
+AEEResult rover_device_alloc(void **d_ptr, size_t bytes);
+AEEResult rover_device_free(void *d_ptr);
+AEEResult rover_device_upload(void *d_dst, const void *src, size_t bytes);
+AEEResult rover_device_download(void *dst, const void *d_src, size_t bytes);
+AEEResult rover_device_synchronize(void);
+int rover_device_count(void);
+const char *rover_last_error(void);
+
+/* Timing and streams (hipEvent_t / hipStream_t as opaque pointers).
+ * Streams are created at the default priority. */
+AEEResult rover_event_create(void **event);
+AEEResult rover_event_record(void *event, void *stream);
+AEEResult rover_event_elapsed_ms(void *start, void *stop, float *ms);
+AEEResult rover_event_destroy(void *event);
+AEEResult rover_stream_create(void **stream);
+AEEResult rover_stream_destroy(void *stream);
+AEEResult rover_stream_synchronize(void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,757 @@
+/*
+ * rover_oracle.c -- CPU restatement of Rover's scalar quantized graph search.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker, never the product).  Compile with
+ * -ffp-contract=off: every float operation below is rounded separately, in
+ * the order the reference performs it.  Citations are to the Rover source
+ * tree (src/dsp_demo/... unless stated).
+ */
+#define _GNU_SOURCE
+#include "rover_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef struct {
+    int id;
+    float distance;
+} OrcCand;
+
+/* retrieval_pq_core.c:46-52 (retrieval_pq_code_bytes) */
+int orc_code_bytes(int m, int ks) {
+    if (m <= 0 || ks <= 0 || ks > 256) return -1;
+    return ks <= 16 ? m / 2 + m % 2 : m;
+}
+
+/* retrieval_pq_residual.c:173-175 (pq_residual_code_bytes) */
+int orc_residual_code_bytes(int dim, int bits) {
+    return bits == 4 ? dim / 2 + dim % 2 : dim;
+}
+
+/* retrieval_pq_core.c:54-64 (retrieval_pq_code_get): low nibble = even m */
+static int orc_code_get(const uint8_t *code, int subspace, int ks) {
+    if (ks <= 16) {
+        uint8_t packed = code[subspace >> 1];
+        return (subspace & 1) ? (packed >> 4) : (packed & 0x0f);
+    }
+    return code[subspace];
+}
+
+/* retrieval_pq_core.c:16-25 (pq_sub_l2): sequential f32 sum of squares */
+static float orc_sub_l2(const float *a, const float *b, int subdim) {
+    float acc = 0.0f;
+    for (int d = 0; d < subdim; d++) {
+        float diff = a[d] - b[d];
+        acc += diff * diff;
+    }
+    return acc;
+}
+
+/* retrieval_pq_core.c:362-390 (retrieval_pq_build_lut) */
+int orc_build_lut(const float *query, const float *codebook, int m, int ks,
+                  int subdim, float *lut) {
+    if (!query || !codebook || !lut || m <= 0 || ks <= 0 || ks > 256 ||
+        subdim <= 0)
+        return -1;
+    for (int mi = 0; mi < m; mi++) {
+        const float *q = query + (size_t)mi * subdim;
+        for (int k = 0; k < ks; k++) {
+            const float *c = codebook + ((size_t)mi * ks + k) * subdim;
+            float dist = orc_sub_l2(q, c, subdim);
+            if (!isfinite(dist)) return -1;
+            lut[(size_t)mi * ks + k] = dist;
+        }
+    }
+    return 0;
+}
+
+/* retrieval_pq_core.c:392-416 (retrieval_pq_adc_scalar): sequential m order,
+ * FLT_MAX on any non-finite value. */
+float orc_adc(const uint8_t *code, const float *lut, int m, int ks) {
+    float acc = 0.0f;
+    for (int mi = 0; mi < m; mi++) {
+        int c = orc_code_get(code, mi, ks);
+        if (c >= ks) return FLT_MAX;
+        float v = lut[(size_t)mi * ks + c];
+        if (!isfinite(v)) return FLT_MAX;
+        acc += v;
+        if (!isfinite(acc)) return FLT_MAX;
+    }
+    return acc;
+}
+
+/* retrieval_pq_residual.c:314-385 (pq_residual_quantize_vector): per-vector
+ * min/max of (x - centroid), scale=(max-min)/levels (floored at 1e-12),
+ * code=clamp(roundf((r-zero)*(1/scale))). */
+int orc_residual_encode(const float *vector, const uint8_t *pq_code,
+                        const float *codebook, int m, int ks, int subdim,
+                        int bits, uint8_t *out_code, float *out_qparams) {
+    float minimum = FLT_MAX, maximum = -FLT_MAX, scale, inv;
+    int levels = bits == 4 ? 15 : 255;
+    int dim = m * subdim;
+    int d = 0;
+    if (bits != 4 && bits != 8) return -1;
+    for (int s = 0; s < m; s++) {
+        const float *c = codebook +
+            ((size_t)s * ks + orc_code_get(pq_code, s, ks)) * subdim;
+        for (int i = 0; i < subdim; i++, d++) {
+            float r = vector[d] - c[i];
+            if (!isfinite(r)) return -1;
+            if (r < minimum) minimum = r;
+            if (r > maximum) maximum = r;
+        }
+    }
+    scale = (maximum - minimum) / (float)levels;
+    if (scale < 1.0e-12f) scale = 1.0e-12f;
+    inv = 1.0f / scale;
+    memset(out_code, 0, (size_t)orc_residual_code_bytes(dim, bits));
+    d = 0;
+    for (int s = 0; s < m; s++) {
+        const float *c = codebook +
+            ((size_t)s * ks + orc_code_get(pq_code, s, ks)) * subdim;
+        for (int i = 0; i < subdim; i++, d++) {
+            float r = vector[d] - c[i];
+            int q = (int)roundf((r - minimum) * inv);
+            if (q < 0) q = 0;
+            if (q > levels) q = levels;
+            if (bits == 4) {
+                if (d & 1) out_code[d >> 1] |= (uint8_t)(q << 4);
+                else out_code[d >> 1] = (uint8_t)q;
+            } else {
+                out_code[d] = (uint8_t)q;
+            }
+        }
+    }
+    out_qparams[0] = scale;
+    out_qparams[1] = minimum;
+    return 0;
+}
+
+/* retrieval_pq_residual.c:946-952 (pq_residual_decode_value): three
+ * separately rounded f32 operations, ((c + code*scale) + zero). */
+static float orc_decode_value(float centroid, int code, float scale,
+                              float zero) {
+    volatile float product = (float)code * scale;
+    volatile float sum = centroid + product;
+    volatile float value = sum + zero;
+    return value;
+}
+
+static int orc_res_code(const uint8_t *res, int bits, int d) {
+    if (bits == 4) {
+        uint8_t packed = res[d >> 1];
+        return (d & 1) ? (packed >> 4) : (packed & 0x0f);
+    }
+    return res[d];
+}
+
+/* retrieval_pq_residual.c:983-1042 (retrieval_pq_residual_query_evaluate) */
+int orc_residual_evaluate(const OrcIndex *ix, const float *query,
+                          int vector_id, float *out_distance) {
+    const uint8_t *pq, *res;
+    const float *qp;
+    float distance = 0.0f;
+    int d = 0;
+    if (!ix || !query || !out_distance || vector_id < 0 ||
+        vector_id >= ix->num_vectors || !ix->res_codes || !ix->res_qparams)
+        return -1;
+    pq = ix->codes + (size_t)vector_id * orc_code_bytes(ix->m, ix->ks);
+    res = ix->res_codes +
+          (size_t)vector_id * orc_residual_code_bytes(ix->dim, ix->res_bits);
+    qp = ix->res_qparams + 2 * (size_t)vector_id;
+    if (!isfinite(qp[0]) || !isfinite(qp[1]) || qp[0] <= 0.0f) return -1;
+    for (int s = 0; s < ix->m; s++) {
+        const float *c = ix->codebook +
+            ((size_t)s * ix->ks + orc_code_get(pq, s, ix->ks)) * ix->subdim;
+        for (int i = 0; i < ix->subdim; i++, d++) {
+            float rec = orc_decode_value(c[i], orc_res_code(res, ix->res_bits, d),
+                                         qp[0], qp[1]);
+            float diff = query[d] - rec;
+            distance += diff * diff;
+        }
+    }
+    if (!isfinite(distance)) return -1;
+    *out_distance = distance;
+    return 0;
+}
+
+/* retrieval_pq_residual.c:954-981 (retrieval_pq_residual_query_decode) */
+int orc_residual_decode(const OrcIndex *ix, int vector_id, float *out) {
+    const uint8_t *pq, *res;
+    const float *qp;
+    int d = 0;
+    if (!ix || !out || vector_id < 0 || vector_id >= ix->num_vectors ||
+        !ix->res_codes || !ix->res_qparams)
+        return -1;
+    qp = ix->res_qparams + 2 * (size_t)vector_id;
+    if (!isfinite(qp[0]) || qp[0] <= 0 || !isfinite(qp[1])) return -1;
+    pq = ix->codes + (size_t)vector_id * orc_code_bytes(ix->m, ix->ks);
+    res = ix->res_codes +
+          (size_t)vector_id * orc_residual_code_bytes(ix->dim, ix->res_bits);
+    for (int s = 0; s < ix->m; s++) {
+        const float *c = ix->codebook +
+            ((size_t)s * ix->ks + orc_code_get(pq, s, ix->ks)) * ix->subdim;
+        for (int i = 0; i < ix->subdim; i++, d++) {
+            out[d] = orc_decode_value(c[i], orc_res_code(res, ix->res_bits, d),
+                                      qp[0], qp[1]);
+            if (!isfinite(out[d])) return -1;
+        }
+    }
+    return 0;
+}
+
+/* retrieval_rpc_demo_support.c:573-627 (demo_set_qparams_from_range and
+ * demo_compute_range): one global scale/zero over all values. */
+int orc_sq_range_params(const float *data, long count, int bits,
+                        float *out_scale, float *out_zero) {
+    float lo = FLT_MAX, hi = -FLT_MAX, range;
+    if (!data || count <= 0 || (bits != 4 && bits != 8)) return -1;
+    for (long i = 0; i < count; i++) {
+        if (data[i] < lo) lo = data[i];
+        if (data[i] > hi) hi = data[i];
+    }
+    range = hi - lo;
+    if (range < 1e-12f) range = 1e-12f;
+    *out_scale = range / (float)((1 << bits) - 1);
+    *out_zero = lo;
+    return 0;
+}
+
+/* retrieval_rpc_demo_support.c:664-700 (demo_quantize_l1_vector): 4-bit
+ * codes packed low nibble first. */
+int orc_sq_encode(const float *v, int dim, int bits, float scale, float zero,
+                  uint8_t *out) {
+    const float inv = 1.0f / scale;
+    const int levels = (1 << bits) - 1;
+    if (bits != 4 && bits != 8) return -1;
+    memset(out, 0, (size_t)(bits == 4 ? (dim + 1) / 2 : dim));
+    for (int i = 0; i < dim; i++) {
+        int q = (int)roundf((v[i] - zero) * inv);
+        if (q < 0) q = 0;
+        else if (q > levels) q = levels;
+        if (bits == 4) {
+            if (i & 1) out[i >> 1] |= (uint8_t)(q << 4);
+            else out[i >> 1] = (uint8_t)q;
+        } else {
+            out[i] = (uint8_t)q;
+        }
+    }
+    return 0;
+}
+
+/* retrieval_l1_int8_core.c:13-60 (retrieval_quantize_query_u8/_u4) */
+int orc_sq_quantize_query(const float *query, int dim, int bits, float scale,
+                          float zero, uint8_t *out) {
+    int hi = bits == 4 ? 15 : 255;
+    float inv;
+    if (!query || !out || dim <= 0 || scale == 0.0f) return -1;
+    inv = 1.0f / scale;
+    for (int i = 0; i < dim; i++) {
+        int v = (int)roundf((query[i] - zero) * inv);
+        if (v < 0) v = 0;
+        else if (v > hi) v = hi;
+        out[i] = (uint8_t)v;
+    }
+    return 0;
+}
+
+/* retrieval_l1_int8_core.c:119-165: integer sum of squared code
+ * differences, then (float)acc * (scale*scale) with scale*scale in f32. */
+float orc_sq_distance(const uint8_t *q, const uint8_t *code, int dim, int bits,
+                      float scale) {
+    int64_t acc = 0;
+    float scale_sq = scale * scale;
+    for (int i = 0; i < dim; i++) {
+        int c = bits == 4 ? ((i & 1) ? (code[i >> 1] >> 4) : (code[i >> 1] & 15))
+                          : code[i];
+        int qv = bits == 4 ? (q[i] & 15) : q[i];
+        int diff = qv - c;
+        acc += (int64_t)(diff * diff);
+    }
+    return (float)acc * scale_sq;
+}
+
+/* retrieval_rpc_demo_pure_pq.c:192-208 (pure_pq_make_seed_ids) */
+int orc_seed_ids(int num_vectors, int entry_count, int graph_entry,
+                 uint32_t seed, int *out) {
+    int count = entry_count < num_vectors ? entry_count : num_vectors;
+    for (int i = 0; i < count; i++) {
+        if (i == 0 && graph_entry >= 0 && graph_entry < num_vectors) {
+            out[i] = graph_entry;
+        } else {
+            seed = seed * 1103515245u + 12345u;
+            out[i] = (int)(((uint64_t)seed * (uint64_t)num_vectors) >> 32);
+        }
+    }
+    return count;
+}
+
+/* retrieval_sqmd_pure_pq.c:97-112 (pure_pq_order): distance, then id */
+static int orc_order(float ld, int lid, float rd, int rid) {
+    if (ld < rd) return -1;
+    if (ld > rd) return 1;
+    if (lid < rid) return -1;
+    if (lid > rid) return 1;
+    return 0;
+}
+
+static int orc_cand_cmp(const void *a, const void *b) {
+    const OrcCand *x = a, *y = b;
+    return orc_order(x->distance, x->id, y->distance, y->id);
+}
+
+/* retrieval_sqmd_pure_pq.c:121-176: bounded max-heap of the best L */
+static void orc_pool_sift_up(OrcCand *pool, int pos) {
+    OrcCand c = pool[pos];
+    while (pos > 0) {
+        int parent = (pos - 1) / 2;
+        if (orc_order(c.distance, c.id, pool[parent].distance,
+                      pool[parent].id) <= 0)
+            break;
+        pool[pos] = pool[parent];
+        pos = parent;
+    }
+    pool[pos] = c;
+}
+
+static void orc_pool_sift_down(OrcCand *pool, int size, int pos) {
+    OrcCand c = pool[pos];
+    for (;;) {
+        int child = 2 * pos + 1;
+        if (child >= size) break;
+        if (child + 1 < size &&
+            orc_order(pool[child + 1].distance, pool[child + 1].id,
+                      pool[child].distance, pool[child].id) > 0)
+            child++;
+        if (orc_order(c.distance, c.id, pool[child].distance,
+                      pool[child].id) >= 0)
+            break;
+        pool[pos] = pool[child];
+        pos = child;
+    }
+    pool[pos] = c;
+}
+
+static int orc_pool_insert(OrcCand *pool, int *size, int cap, int id,
+                           float dist) {
+    if (*size < cap) {
+        pool[*size].id = id;
+        pool[*size].distance = dist;
+        orc_pool_sift_up(pool, *size);
+        (*size)++;
+        return 1;
+    }
+    if (orc_order(dist, id, pool[0].distance, pool[0].id) < 0) {
+        pool[0].id = id;
+        pool[0].distance = dist;
+        orc_pool_sift_down(pool, *size, 0);
+        return 1;
+    }
+    return 0;
+}
+
+/* retrieval_sqmd_pure_pq.c:178-238: unbounded min-heap frontier */
+static void orc_front_push(OrcCand *f, int *size, int id, float dist) {
+    int pos = (*size)++;
+    OrcCand c = {id, dist};
+    while (pos > 0) {
+        int parent = (pos - 1) / 2;
+        if (orc_order(c.distance, c.id, f[parent].distance, f[parent].id) >= 0)
+            break;
+        f[pos] = f[parent];
+        pos = parent;
+    }
+    f[pos] = c;
+}
+
+static OrcCand orc_front_pop(OrcCand *f, int *size) {
+    OrcCand top = f[0];
+    (*size)--;
+    if (*size > 0) {
+        OrcCand c = f[*size];
+        int pos = 0;
+        for (;;) {
+            int child = 2 * pos + 1;
+            if (child >= *size) break;
+            if (child + 1 < *size &&
+                orc_order(f[child + 1].distance, f[child + 1].id,
+                          f[child].distance, f[child].id) < 0)
+                child++;
+            if (orc_order(c.distance, c.id, f[child].distance,
+                          f[child].id) <= 0)
+                break;
+            f[pos] = f[child];
+            pos = child;
+        }
+        f[pos] = c;
+    }
+    return top;
+}
+
+/* retrieval_sqmd_pure_pq.c:240-249 (pure_pq_pool_contains) */
+static int orc_pool_contains(const OrcCand *pool, int size, int id,
+                             float dist) {
+    for (int i = 0; i < size; i++)
+        if (pool[i].id == id && pool[i].distance == dist) return 1;
+    return 0;
+}
+
+/* retrieval_sqmd_pure_pq.c:59-87 (pure_pq_graph_row_count): -1 padding only
+ * as a suffix; any other out-of-range id makes the row invalid. */
+static int orc_row_count(const int32_t *row, int slots, int n) {
+    for (int i = 0; i < slots; i++) {
+        int id = row[i];
+        if (id == -1) {
+            for (int j = i + 1; j < slots; j++)
+                if (row[j] != -1) return -1;
+            return i;
+        }
+        if (id < 0 || id >= n) return -1;
+    }
+    return slots;
+}
+
+typedef struct {
+    const OrcIndex *ix;
+    int scorer;
+    const float *query;
+    const float *lut;
+    uint8_t qsq[4096];
+} OrcScorer;
+
+/* One candidate's final distance: ADC (retrieval_pq_core.c:280-330, FLT_MAX
+ * or non-finite is an error), then the distance policy
+ * (retrieval_sqmd_pure_pq.c:40-57).  For SQ the policy evaluator returns the
+ * SQ distance and the coarse ADC is discarded. */
+static int orc_score(const OrcScorer *s, int id, float *out) {
+    const OrcIndex *ix = s->ix;
+    if (s->scorer == ORC_SCORE_SQ) {
+        int cb = ix->sq_bits == 4 ? (ix->dim + 1) / 2 : ix->dim;
+        float d = orc_sq_distance(s->qsq, ix->sq_codes + (size_t)id * cb,
+                                  ix->dim, ix->sq_bits, ix->sq_scale);
+        if (!isfinite(d)) return -1;
+        *out = d;
+        return 0;
+    }
+    float d = orc_adc(ix->codes + (size_t)id * orc_code_bytes(ix->m, ix->ks),
+                      s->lut, ix->m, ix->ks);
+    if (!isfinite(d) || d == FLT_MAX) return -1;
+    if (s->scorer == ORC_SCORE_ADC_ONLINE) {
+        if (orc_residual_evaluate(ix, s->query, id, &d) != 0 || !isfinite(d))
+            return -1;
+    }
+    *out = d;
+    return 0;
+}
+
+/* retrieval_sqmd_pure_pq.c:793-969
+ * (retrieval_pure_pq_search_scalar_with_policy) */
+int orc_search(const OrcIndex *ix, int scorer, const float *query,
+               const int *entry_ids, int entry_count, int expand_count,
+               int pool_capacity, int num_out, int *out_ids,
+               float *out_distances, int out_capacity, OrcStats *out_stats) {
+    OrcCand *pool = NULL, *front = NULL;
+    uint32_t *visited = NULL;
+    float *lut = NULL;
+    int *batch = NULL;
+    int pool_size = 0, front_size = 0, result = -1;
+    OrcStats st;
+    OrcScorer sc;
+    memset(&st, 0, sizeof(st));
+    if (out_stats) memset(out_stats, 0, sizeof(*out_stats));
+    if (!ix || !query || !entry_ids || entry_count <= 0 || expand_count <= 0 ||
+        pool_capacity <= 0 || num_out <= 0 || num_out > pool_capacity ||
+        !out_ids || !out_distances || out_capacity <= 0 || !ix->graph ||
+        ix->degree <= 0 || ix->num_vectors <= 0)
+        return -1;
+    if (scorer != ORC_SCORE_SQ &&
+        (!ix->codes || !ix->codebook || ix->m <= 0 ||
+         (ix->ks != 16 && ix->ks != 256) || ix->subdim <= 0 ||
+         ix->m * ix->subdim != ix->dim))
+        return -1;
+    if (scorer == ORC_SCORE_ADC_ONLINE && (!ix->res_codes || !ix->res_qparams))
+        return -1;
+    if (scorer == ORC_SCORE_SQ &&
+        (!ix->sq_codes || ix->dim > 4096 || (ix->sq_bits != 4 && ix->sq_bits != 8)))
+        return -1;
+    if (expand_count > ix->degree) expand_count = ix->degree;
+    memset(&sc, 0, sizeof(sc));
+    sc.ix = ix;
+    sc.scorer = scorer;
+    sc.query = query;
+    pool = malloc((size_t)pool_capacity * sizeof(*pool));
+    front = malloc((size_t)ix->num_vectors * sizeof(*front));
+    visited = calloc(((size_t)ix->num_vectors + 31) / 32, sizeof(*visited));
+    batch = malloc((size_t)expand_count * sizeof(*batch));
+    if (!pool || !front || !visited || !batch) goto done;
+    if (scorer == ORC_SCORE_SQ) {
+        if (orc_sq_quantize_query(query, ix->dim, ix->sq_bits, ix->sq_scale,
+                                  ix->sq_zero, sc.qsq) != 0)
+            goto done;
+    } else {
+        lut = malloc((size_t)ix->m * ix->ks * sizeof(*lut));
+        if (!lut || orc_build_lut(query, ix->codebook, ix->m, ix->ks,
+                                  ix->subdim, lut) != 0)
+            goto done;
+        sc.lut = lut;
+    }
+
+    for (int e = 0; e < entry_count; e++) {
+        int id = entry_ids[e];
+        float dist;
+        if (id < 0 || id >= ix->num_vectors) goto done;
+        if (visited[id >> 5] & (1u << (id & 31))) {
+            st.visited_dedup_skips++;
+            continue;
+        }
+        visited[id >> 5] |= 1u << (id & 31);
+        if (orc_score(&sc, id, &dist) != 0) goto done;
+        st.pq_scored_candidates++;
+        if (scorer != ORC_SCORE_ADC) st.corrected_candidates++;
+        if (orc_pool_insert(pool, &pool_size, pool_capacity, id, dist))
+            orc_front_push(front, &front_size, id, dist);
+    }
+
+    while (front_size > 0) {
+        OrcCand src;
+        const int32_t *row;
+        int nb = 0, valid;
+        if (pool_size == pool_capacity &&
+            orc_order(front[0].distance, front[0].id, pool[0].distance,
+                      pool[0].id) > 0)
+            break;
+        src = orc_front_pop(front, &front_size);
+        if (!orc_pool_contains(pool, pool_size, src.id, src.distance)) continue;
+        row = ix->graph + (size_t)src.id * ix->degree;
+        valid = orc_row_count(row, expand_count, ix->num_vectors);
+        if (valid < 0) goto done;
+        for (int j = 0; j < valid; j++) {
+            int id = row[j];
+            if (visited[id >> 5] & (1u << (id & 31))) {
+                st.visited_dedup_skips++;
+                continue;
+            }
+            visited[id >> 5] |= 1u << (id & 31);
+            batch[nb++] = id;
+        }
+        st.expanded_nodes++;
+        if (nb == 0) continue;
+        /* scores of the whole batch are computed before any insertion */
+        float dists[256];
+        float *bd = nb <= 256 ? dists : malloc((size_t)nb * sizeof(float));
+        if (!bd) goto done;
+        for (int j = 0; j < nb; j++) {
+            if (orc_score(&sc, batch[j], &bd[j]) != 0) {
+                if (bd != dists) free(bd);
+                goto done;
+            }
+        }
+        st.pq_scored_candidates += (uint64_t)nb;
+        for (int j = 0; j < nb; j++) {
+            if (scorer != ORC_SCORE_ADC) st.corrected_candidates++;
+            if (orc_pool_insert(pool, &pool_size, pool_capacity, batch[j], bd[j]))
+                orc_front_push(front, &front_size, batch[j], bd[j]);
+        }
+        if (bd != dists) free(bd);
+    }
+
+    qsort(pool, (size_t)pool_size, sizeof(*pool), orc_cand_cmp);
+    result = num_out < pool_size ? num_out : pool_size;
+    if (result > out_capacity) result = out_capacity;
+    for (int i = 0; i < result; i++) {
+        out_ids[i] = pool[i].id;
+        out_distances[i] = pool[i].distance;
+    }
+    if (out_stats) *out_stats = st;
+done:
+    free(pool);
+    free(front);
+    free(visited);
+    free(lut);
+    free(batch);
+    return result;
+}
+
+typedef struct {
+    int graph, original;
+    double distance;
+} OrcRefineItem;
+
+static int orc_refine_cmp(const void *a, const void *b) {
+    const OrcRefineItem *x = a, *y = b;
+    if (x->distance != y->distance) return x->distance < y->distance ? -1 : 1;
+    return (x->original > y->original) - (x->original < y->original);
+}
+
+/* retrieval_final_refine.c:256-292 (refine_impl) with the PQ residual
+ * decoder (retrieval_pq_residual.c:954-981) as the vector source:
+ * dist = sum_d ((double)q - (double)v)^2 sequentially in f64, sorted by
+ * (dist, original id), top output_k returned as (float)dist. */
+int orc_refine(const OrcIndex *ix, const float *query, const int32_t *ids,
+               int count, int output_k, const int32_t *graph_to_original,
+               int32_t *out_ids, float *out_distances) {
+    OrcRefineItem *items;
+    float *vec;
+    int rc;
+    if (!ix || !query || !ids || count < 0 || output_k <= 0) return -1;
+    for (int d = 0; d < ix->dim; d++)
+        if (!isfinite(query[d])) return -4;
+    items = malloc((size_t)(count > 0 ? count : 1) * sizeof(*items));
+    vec = malloc((size_t)ix->dim * sizeof(float));
+    if (!items || !vec) {
+        free(items);
+        free(vec);
+        return -1;
+    }
+    for (int i = 0; i < count; i++) {
+        int id = ids[i];
+        if (id < 0 || id >= ix->num_vectors) {
+            rc = -1;
+            goto out;
+        }
+        for (int j = 0; j < i; j++)
+            if (ids[j] == id) {
+                rc = -1; /* duplicate storage row */
+                goto out;
+            }
+        items[i].graph = id;
+        items[i].original = graph_to_original ? graph_to_original[id] : id;
+        if (orc_residual_decode(ix, id, vec) != 0) {
+            rc = -1;
+            goto out;
+        }
+        double acc = 0.0;
+        for (int d = 0; d < ix->dim; d++) {
+            double delta = (double)query[d] - (double)vec[d];
+            acc += delta * delta;
+        }
+        if (!isfinite(acc) || acc >= FLT_MAX) {
+            rc = -4;
+            goto out;
+        }
+        items[i].distance = acc;
+    }
+    qsort(items, (size_t)count, sizeof(*items), orc_refine_cmp);
+    rc = count < output_k ? count : output_k;
+    for (int i = 0; i < rc; i++) {
+        out_ids[i] = items[i].graph;
+        out_distances[i] = (float)items[i].distance;
+    }
+out:
+    free(items);
+    free(vec);
+    return rc;
+}
+
+/* retrieval_rpc_demo_support.c:1709-1728 (demo_recall_hits_at_k) */
+int orc_recall_hits(const int *res, int res_count, const int32_t *gt,
+                    int gt_k, int k) {
+    int hits = 0;
+    for (int i = 0; i < k && i < res_count; i++)
+        for (int j = 0; j < gt_k && j < k; j++)
+            if (res[i] == gt[j]) {
+                hits++;
+                break;
+            }
+    return hits;
+}
+
+/* retrieval_rpc_demo_pure_pq.c:387-400 (pure_pq_result_fingerprint):
+ * FNV-1a 64 over (id, distance bits) little-endian bytes. */
+uint64_t orc_fingerprint(uint64_t hash, const int *ids, const float *dists,
+                         int count) {
+    for (int i = 0; i < count; i++) {
+        uint32_t v[2];
+        v[0] = (uint32_t)ids[i];
+        memcpy(&v[1], &dists[i], 4);
+        for (int b = 0; b < 8; b++) {
+            hash ^= (v[b / 4] >> (8 * (b % 4))) & 255u;
+            hash *= UINT64_C(1099511628211);
+        }
+    }
+    return hash;
+}
+
+typedef struct {
+    const OrcIndex *ix;
+    int scorer;
+    const float *queries;
+    int nq;
+    const int *entries;
+    int n_entries, expand;
+    const int *pool, *num_out;
+    int stride, refine;
+    int *out_ids;
+    float *out_d;
+    int *out_counts;
+    OrcStats *stats;
+    int next;
+    int failed;
+} OrcBatch;
+
+static void *orc_batch_worker(void *arg) {
+    OrcBatch *b = arg;
+    int *ids = NULL;
+    float *dists = NULL;
+    int cap = 0;
+    for (;;) {
+        int q = __atomic_fetch_add(&b->next, 1, __ATOMIC_RELAXED);
+        if (q >= b->nq) break;
+        int L = b->pool[q];
+        if (L > cap) {
+            free(ids);
+            free(dists);
+            cap = L;
+            ids = malloc((size_t)cap * sizeof(int));
+            dists = malloc((size_t)cap * sizeof(float));
+            if (!ids || !dists) {
+                __atomic_store_n(&b->failed, 1, __ATOMIC_RELAXED);
+                b->out_counts[q] = -1;
+                cap = 0;
+                continue;
+            }
+        }
+        const float *qv = b->queries + (size_t)q * b->ix->dim;
+        int want = b->refine > 0 ? (b->refine < L ? b->refine : L) : b->num_out[q];
+        OrcStats st;
+        int n = orc_search(b->ix, b->scorer, qv, b->entries, b->n_entries,
+                           b->expand, L, want, ids, dists, cap, &st);
+        if (n >= 0 && b->refine > 0) {
+            n = orc_refine(b->ix, qv, ids, n, b->num_out[q], NULL, ids, dists);
+        }
+        if (n < 0) __atomic_store_n(&b->failed, 1, __ATOMIC_RELAXED);
+        b->out_counts[q] = n;
+        for (int i = 0; i < n && i < b->stride; i++) {
+            b->out_ids[(size_t)q * b->stride + i] = ids[i];
+            b->out_d[(size_t)q * b->stride + i] = dists[i];
+        }
+        if (b->stats) b->stats[q] = st;
+    }
+    free(ids);
+    free(dists);
+    return NULL;
+}
+
+int orc_search_batch(const OrcIndex *ix, int scorer, const float *queries,
+                     int nq, const int *entries, int n_entries, int expand,
+                     const int *pool, const int *num_out, int stride,
+                     int refine, int threads, int *out_ids, float *out_d,
+                     int *out_counts, OrcStats *stats) {
+    OrcBatch b = {ix, scorer, queries, nq, entries, n_entries, expand, pool,
+                  num_out, stride, refine, out_ids, out_d, out_counts, stats,
+                  0, 0};
+    pthread_t tid[256];
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    for (int t = 0; t < threads; t++)
+        if (pthread_create(&tid[t], NULL, orc_batch_worker, &b) != 0) {
+            threads = t;
+            b.failed = 1;
+            break;
+        }
+    for (int t = 0; t < threads; t++) pthread_join(tid[t], NULL);
+    return b.failed ? -1 : 0;
+}

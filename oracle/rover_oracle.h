@@ -1,0 +1,108 @@
+/*
+ * rover_oracle.h -- CPU restatement of Rover's scalar quantized graph search.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker that the HIP search path is
+ * compared against; only tests/, __graft_entry__.smoke() and bench.py's CPU
+ * baseline leg may load it.  Nothing in rover_amd/ links or calls it.
+ *
+ * Every function restates one reference function (cited file:line below,
+ * paths relative to the Rover source tree) with the same floating-point
+ * operation order, so results are bit-identical when both are compiled
+ * without FMA contraction (-ffp-contract=off).
+ */
+#ifndef ROVER_ORACLE_H
+#define ROVER_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* How a scored candidate's distance is produced. */
+enum {
+    ORC_SCORE_ADC = 0,        /* PQ ADC, policy OFF                         */
+    ORC_SCORE_ADC_ONLINE = 1, /* PQ ADC replaced by residual reconstruction */
+    ORC_SCORE_SQ = 2          /* global-scale SQ8/SQ4 code distance         */
+};
+
+typedef struct OrcIndex {
+    const int32_t *graph; /* [num_vectors][degree], -1 suffix padding */
+    int degree;
+    int num_vectors;
+    /* PQ */
+    const uint8_t *codes;    /* [num_vectors][code_bytes] */
+    const float *codebook;   /* [m][ks][subdim] */
+    int dim, m, ks, subdim;
+    /* residual (ONLINE correction) */
+    const uint8_t *res_codes;  /* [num_vectors][res_code_bytes] */
+    const float *res_qparams;  /* [num_vectors][2] = (scale, zero_point) */
+    int res_bits;              /* 4 or 8 */
+    /* SQ (global scale) */
+    const uint8_t *sq_codes;   /* [num_vectors][sq_code_bytes] */
+    int sq_bits;               /* 4 or 8 */
+    float sq_scale, sq_zero;
+} OrcIndex;
+
+typedef struct OrcStats {
+    uint64_t pq_scored_candidates;
+    uint64_t corrected_candidates;
+    uint64_t expanded_nodes;
+    uint64_t visited_dedup_skips;
+} OrcStats;
+
+int orc_code_bytes(int m, int ks);
+int orc_residual_code_bytes(int dim, int bits);
+
+int orc_build_lut(const float *query, const float *codebook, int m, int ks,
+                  int subdim, float *lut);
+float orc_adc(const uint8_t *code, const float *lut, int m, int ks);
+
+int orc_residual_encode(const float *vector, const uint8_t *pq_code,
+                        const float *codebook, int m, int ks, int subdim,
+                        int bits, uint8_t *out_code, float *out_qparams);
+int orc_residual_evaluate(const OrcIndex *index, const float *query,
+                          int vector_id, float *out_distance);
+int orc_residual_decode(const OrcIndex *index, int vector_id,
+                        float *out_vector);
+
+int orc_sq_range_params(const float *data, long count, int bits,
+                        float *out_scale, float *out_zero);
+int orc_sq_encode(const float *vector, int dim, int bits, float scale,
+                  float zero, uint8_t *out_code);
+int orc_sq_quantize_query(const float *query, int dim, int bits, float scale,
+                          float zero, uint8_t *out_query);
+float orc_sq_distance(const uint8_t *query_q, const uint8_t *code, int dim,
+                      int bits, float scale);
+
+int orc_seed_ids(int num_vectors, int entry_count, int graph_entry,
+                 uint32_t seed, int *out_ids);
+
+int orc_search(const OrcIndex *index, int scorer, const float *query,
+               const int *entry_ids, int entry_count, int expand_count,
+               int pool_capacity, int num_out, int *out_ids,
+               float *out_distances, int out_capacity, OrcStats *out_stats);
+
+int orc_refine(const OrcIndex *index, const float *query, const int32_t *ids,
+               int count, int output_k, const int32_t *graph_to_original,
+               int32_t *out_ids, float *out_distances);
+
+int orc_recall_hits(const int *result_ids, int result_count,
+                    const int32_t *gt_row, int gt_k, int k);
+uint64_t orc_fingerprint(uint64_t hash, const int *ids,
+                         const float *distances, int count);
+
+/* Many queries on a pool of host threads (one query per thread at a time).
+ * refine_k > 0 applies orc_refine to the first refine_candidates results.
+ * Returns 0, or -1 if any query failed (its count is then -1). */
+int orc_search_batch(const OrcIndex *index, int scorer, const float *queries,
+                     int num_queries, const int *entry_ids, int entry_count,
+                     int expand_count, const int *pool_capacity,
+                     const int *num_out, int out_stride, int refine_candidates,
+                     int threads, int *out_ids, float *out_distances,
+                     int *out_counts, OrcStats *out_stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,154 @@
+/*
+ * rover_hip.h -- the thin C-ABI between the C host library (rover_host.c)
+ * and the HIP device code (rover_search.hip).  Plain pointers and sizes
+ * only.  Pointers named d_* are device (HBM) addresses.
+ */
+#ifndef ROVER_HIP_H
+#define ROVER_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    RV_SCORE_ADC = 0,        /* PQ ADC ranking (correction off)           */
+    RV_SCORE_ADC_ONLINE = 1, /* residual-reconstructed L2 (online)        */
+    RV_SCORE_SQ = 2          /* global-scale SQ8/SQ4 integer code distance */
+};
+
+/* Status written per query to out_counts when negative. */
+enum {
+    RV_QUERY_BAD_ROW = -2,     /* graph row with -1 not as suffix / bad id  */
+    RV_QUERY_BAD_ENTRY = -3,   /* entry id out of range                     */
+    RV_QUERY_NONFINITE = -4,   /* non-finite or FLT_MAX distance            */
+    RV_QUERY_BAD_PARAM = -5    /* L/k outside the launch's limits           */
+};
+
+/* Device-resident index.  All arrays are row-major and dense. */
+typedef struct RvIndex {
+    const int32_t *graph;       /* [n][degree], -1 suffix padding         */
+    int degree, n, graph_entry;
+    const uint8_t *codes;       /* [n][code_bytes] PQ codes               */
+    int code_bytes;
+    const float *codebook;      /* [m][ks][subdim]                        */
+    int dim, m, ks, subdim;
+    const uint8_t *res_codes;   /* [n][res_code_bytes]                    */
+    const float *res_qparams;   /* [n][2] (scale, zero_point)             */
+    int res_bits, res_code_bytes;
+    const uint8_t *sq_codes;    /* [n][sq_code_bytes]                     */
+    int sq_bits, sq_code_bytes;
+    float sq_scale, sq_zero;
+    const int32_t *graph_to_original; /* optional permutation for refine  */
+} RvIndex;
+
+/* Learned early-termination predictor: 16 -> 64 -> 64 -> 1 MLP,
+ * BF16 weights (stored as raw uint16), FP32 biases, FP32 accumulation. */
+#define RV_MLP_IN 16
+#define RV_MLP_HIDDEN 64
+typedef struct RvMlp {
+    const uint16_t *w1;    /* [64][16]  bf16 */
+    const float *b1;       /* [64]           */
+    const uint16_t *w2;    /* [64][64]  bf16 */
+    const float *b2;       /* [64]           */
+    const uint16_t *w3;    /* [64]      bf16 */
+    const float *b3;       /* [1]            */
+    const float *feat_mean;/* [16]           */
+    const float *feat_inv_std; /* [16]       */
+} RvMlp;
+
+typedef struct RvBatch {
+    const float *queries;        /* [nq][dim]                               */
+    int nq;
+    const int32_t *pool_sizes;   /* [nq] per-query L, or NULL -> pool_size  */
+    const int32_t *num_outs;     /* [nq] per-query k, or NULL -> num_out    */
+    int pool_size, num_out;
+    int max_pool;                /* LDS is sized for this L                 */
+    const int32_t *entries;      /* [n_entries] seed ids, shared by all     */
+    int n_entries;
+    int expand_count;
+    int scorer;                  /* RV_SCORE_*                              */
+    int refine_candidates;       /* 0: off; F: refine the top F, output k   */
+    int hash_log2;               /* visited hash has 1<<hash_log2 slots     */
+    int probe_limit;
+    int32_t *out_ids;            /* [nq][out_stride]                        */
+    float *out_dists;            /* [nq][out_stride]                        */
+    int32_t *out_counts;         /* [nq] count or RV_QUERY_* status         */
+    int out_stride;
+    uint64_t *out_stats;         /* [nq][4] scored, corrected, expanded,
+                                    dedup; may be NULL                      */
+    int32_t *queue;              /* device work counter, zeroed per launch  */
+    /* learned stop (stop_mode 0 = off) */
+    int stop_mode;               /* 1: infer and stop; 2: record traces     */
+    RvMlp mlp;
+    float stop_threshold;        /* stop when sigmoid(out) >= threshold     */
+    int stop_every;              /* evaluate every stop_every expansions    */
+    int stop_min_steps;
+    int stop_label_k;            /* top-K whose stability is predicted      */
+    float *trace_feats;          /* [nq][trace_cap][16]                     */
+    int32_t *trace_steps;        /* [nq][trace_cap]                         */
+    int32_t *trace_meta;         /* [nq][2]: records, last top-K change step */
+    int trace_cap;
+} RvBatch;
+
+int rv_device_count(void);
+int rv_set_device(int device);
+int rv_malloc(void **d_ptr, size_t bytes);
+int rv_free(void *d_ptr);
+int rv_memcpy_h2d(void *d_dst, const void *src, size_t bytes);
+int rv_memcpy_d2h(void *dst, const void *d_src, size_t bytes);
+int rv_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes);
+int rv_synchronize(void);
+const char *rv_last_error(void);
+int rv_event_create(void **ev);
+int rv_event_record(void *ev, void *stream);
+int rv_event_elapsed(void *start, void *stop, float *ms);
+int rv_event_destroy(void *ev);
+int rv_stream_create(void **stream);
+int rv_stream_destroy(void *stream);
+int rv_stream_synchronize(void *stream);
+
+/* Bytes of LDS one query slot needs, and how many slots fit per CU. */
+size_t rv_search_lds_bytes(const RvIndex *index, const RvBatch *batch);
+int rv_search_slots_per_cu(const RvIndex *index, const RvBatch *batch);
+
+/* Launches the persistent search over the batch on `stream` (NULL = the
+ * default stream).  grid_waves <= 0 picks the occupancy-derived count.  The
+ * call is asynchronous; it zeroes batch->queue with a kernel first. */
+int rv_search_launch(const RvIndex *index, const RvBatch *batch,
+                     int grid_waves, void *stream);
+
+/* Encoders for index building on the device (bit-identical to the
+ * reference encoders). */
+int rv_pq_encode(const float *d_x, int n, int dim, int m, int ks,
+                 const float *d_codebook, uint8_t *d_codes, void *stream);
+int rv_residual_encode(const float *d_x, const uint8_t *d_codes, int n,
+                       int dim, int m, int ks, const float *d_codebook,
+                       int bits, uint8_t *d_res, float *d_qparams,
+                       void *stream);
+
+/* MLP training kernels (FP32 master weights). */
+typedef struct RvMlpTrain {
+    float *w1, *b1, *w2, *b2, *w3, *b3;          /* params  [5185+..]     */
+    float *g;                                    /* grads, flat [P]       */
+    float *m1, *m2;                              /* Adam moments [P]      */
+    float *loss;                                 /* [1] summed loss        */
+} RvMlpTrain;
+int rv_mlp_param_count(void);
+/* params/grads/moments are flat arrays in the order w1,b1,w2,b2,w3,b3. */
+int rv_mlp_forward_backward(const float *d_params, const float *d_x,
+                            const float *d_y, const float *d_weight, int n,
+                            float *d_grad, float *d_loss, float *d_out,
+                            void *stream);
+int rv_mlp_adam(float *d_params, const float *d_grad, float *d_m1,
+                float *d_m2, int step, float lr, float beta1, float beta2,
+                float eps, float grad_scale, void *stream);
+int rv_mlp_to_bf16(const float *d_params, uint16_t *d_bf16, void *stream);
+int rv_fill_zero(void *d_ptr, size_t bytes, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

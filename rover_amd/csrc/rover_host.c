@@ -1,0 +1,925 @@
+/*
+ * rover_host.c -- C host side of the MI355X quantized graph search.
+ *
+ * Implements the calculator.idl operator surface, the reference's library
+ * search call and the batch extensions of include/rover_gpu.h on top of the
+ * thin HIP ABI in rover_hip.h.  There is no CPU search path here: every
+ * search runs the HIP kernels, and a missing or failing device is an error.
+ */
+#define _GNU_SOURCE
+#include "rover_gpu.h"
+#include "rover_hip.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ROVER_CTX_MAGIC 0x52564731u /* "RVG1" */
+
+static __thread char g_err[512];
+
+static void set_err(const char *what) {
+    const char *detail = rv_last_error();
+    snprintf(g_err, sizeof(g_err), "%s%s%s", what,
+             detail && detail[0] ? ": " : "", detail ? detail : "");
+}
+
+const char *rover_last_error(void) { return g_err; }
+
+typedef struct RoverLearned {
+    int ready;
+    uint16_t *d_w;     /* bf16 w1 | w2 | w3 */
+    float *d_f;        /* b1 | b2 | b3 | mean | inv_std */
+} RoverLearned;
+
+typedef struct RoverCtx {
+    uint32_t magic;
+    int device;
+    int l1_bits;
+    /* pure-PQ index */
+    int pq_loaded;
+    int n, dim, m, ks, subdim, code_bytes;
+    uint8_t *d_codes;
+    float *d_codebook;
+    int32_t *d_graph;
+    int graph_n, degree, entry;
+    uint8_t *d_res;
+    float *d_qparams;
+    int res_bits;
+    int32_t *d_orig;
+    /* SQ index (retrieval_load) */
+    int sq_loaded;
+    int32_t *d_sq_graph;
+    uint8_t *d_sq_codes;
+    int sq_n, sq_dim, sq_degree, sq_entry, sq_bits;
+    float sq_scale, sq_zero;
+    /* launch workspace */
+    int32_t *d_queue;
+    int32_t *d_entries;
+    int entries_n, entries_count, entries_graph_entry;
+    uint32_t entries_seed;
+    int entries_cap;
+    void *d_stage;
+    size_t stage_cap;
+    RoverLearned learned;
+    pthread_mutex_t lock;
+} RoverCtx;
+
+static RoverCtx *ctx_of(remote_handle64 h) {
+    RoverCtx *c = (RoverCtx *)(uintptr_t)h;
+    if (!c || c->magic != ROVER_CTX_MAGIC) return NULL;
+    return c;
+}
+
+static void dfree(void *p) {
+    if (p) rv_free(p);
+}
+
+static int dalloc_copy(void **d, const void *src, size_t bytes) {
+    *d = NULL;
+    if (rv_malloc(d, bytes)) return -1;
+    if (src && rv_memcpy_h2d(*d, src, bytes)) {
+        rv_free(*d);
+        *d = NULL;
+        return -1;
+    }
+    return 0;
+}
+
+int rover_device_count(void) { return rv_device_count(); }
+
+int calculator_open(const char *uri, remote_handle64 *handle) {
+    RoverCtx *c;
+    (void)uri;
+    if (!handle) return ROVER_EBADPARM;
+    *handle = 0;
+    if (rv_device_count() <= 0) {
+        snprintf(g_err, sizeof(g_err), "calculator_open: no HIP device");
+        return ROVER_EFAILED;
+    }
+    c = calloc(1, sizeof(*c));
+    if (!c) return ROVER_ENOMEMORY;
+    c->magic = ROVER_CTX_MAGIC;
+    c->device = 0;
+    c->l1_bits = 8;
+    pthread_mutex_init(&c->lock, NULL);
+    if (rv_set_device(0) || rv_malloc((void **)&c->d_queue, 64)) {
+        set_err("calculator_open");
+        free(c);
+        return ROVER_EFAILED;
+    }
+    *handle = (remote_handle64)(uintptr_t)c;
+    return ROVER_SUCCESS;
+}
+
+static void learned_free(RoverLearned *l) {
+    dfree(l->d_w);
+    dfree(l->d_f);
+    memset(l, 0, sizeof(*l));
+}
+
+AEEResult calculator_retrieval_pure_pq_unload_dma(remote_handle64 h) {
+    RoverCtx *c = ctx_of(h);
+    if (!c) return ROVER_EBADPARM;
+    dfree(c->d_codes);
+    dfree(c->d_codebook);
+    dfree(c->d_graph);
+    dfree(c->d_res);
+    dfree(c->d_qparams);
+    dfree(c->d_orig);
+    c->d_codes = NULL;
+    c->d_codebook = NULL;
+    c->d_graph = NULL;
+    c->d_res = NULL;
+    c->d_qparams = NULL;
+    c->d_orig = NULL;
+    c->pq_loaded = 0;
+    c->graph_n = 0;
+    c->res_bits = 0;
+    return ROVER_SUCCESS;
+}
+
+AEEResult calculator_retrieval_unload(remote_handle64 h) {
+    RoverCtx *c = ctx_of(h);
+    if (!c) return ROVER_EBADPARM;
+    dfree(c->d_sq_graph);
+    dfree(c->d_sq_codes);
+    c->d_sq_graph = NULL;
+    c->d_sq_codes = NULL;
+    c->sq_loaded = 0;
+    return ROVER_SUCCESS;
+}
+
+int calculator_close(remote_handle64 h) {
+    RoverCtx *c = ctx_of(h);
+    if (!c) return ROVER_EBADPARM;
+    calculator_retrieval_pure_pq_unload_dma(h);
+    calculator_retrieval_unload(h);
+    learned_free(&c->learned);
+    dfree(c->d_queue);
+    dfree(c->d_entries);
+    dfree(c->d_stage);
+    c->magic = 0;
+    pthread_mutex_destroy(&c->lock);
+    free(c);
+    return ROVER_SUCCESS;
+}
+
+AEEResult calculator_rover_configure(remote_handle64 h, int32_t abi_version,
+                                     int32_t l1_bits) {
+    RoverCtx *c = ctx_of(h);
+    if (!c) return ROVER_EBADPARM;
+    if (abi_version != ROVER_ABI_VERSION || (l1_bits != 4 && l1_bits != 8))
+        return ROVER_EBADPARM;
+    c->l1_bits = l1_bits;
+    return ROVER_SUCCESS;
+}
+
+/* retrieval_async_dsp_search.c:698-799: validates the chunk table and the
+ * PQ shape, copies the codebook; the code chunks are concatenated into one
+ * dense [num_vectors][code_bytes] array in HBM. */
+AEEResult calculator_retrieval_pure_pq_load_dma(
+    remote_handle64 h, const uint8_t *const *pq_codes, int pq_codesLen,
+    const int32_t *counts, int countsLen, int32_t chunk_vectors,
+    const float *codebook, int codebookLen, int32_t num_vectors, int32_t dim,
+    int32_t m, int32_t ks, int32_t subdim, int32_t code_bytes,
+    const uint8_t *uuid, int uuidLen) {
+    RoverCtx *c = ctx_of(h);
+    long total = 0;
+    int expect_bytes;
+    if (!c) return ROVER_EBADPARM;
+    (void)uuid;
+    expect_bytes = (ks == 16) ? m / 2 + m % 2 : m;
+    if (!pq_codes || !counts || !codebook || pq_codesLen <= 0 ||
+        pq_codesLen != countsLen || chunk_vectors <= 0 || num_vectors <= 0 ||
+        dim <= 0 || dim > 4096 || m <= 0 || (ks != 16 && ks != 256) ||
+        subdim <= 0 || m * subdim != dim || code_bytes != expect_bytes ||
+        codebookLen != m * ks * subdim || (uuid && uuidLen != 16))
+        return ROVER_EBADPARM;
+    for (int i = 0; i < pq_codesLen; i++) {
+        if (!pq_codes[i] || counts[i] <= 0 || counts[i] > chunk_vectors ||
+            (i + 1 < pq_codesLen && counts[i] != chunk_vectors))
+            return ROVER_EBADPARM;
+        total += counts[i];
+    }
+    if (total != num_vectors) return ROVER_EBADPARM;
+    calculator_retrieval_pure_pq_unload_dma(h);
+    if (rv_malloc((void **)&c->d_codes, (size_t)num_vectors * code_bytes) ||
+        dalloc_copy((void **)&c->d_codebook, codebook,
+                    (size_t)codebookLen * sizeof(float))) {
+        set_err("pure_pq_load");
+        calculator_retrieval_pure_pq_unload_dma(h);
+        return ROVER_ENOMEMORY;
+    }
+    {
+        size_t off = 0;
+        for (int i = 0; i < pq_codesLen; i++) {
+            size_t bytes = (size_t)counts[i] * code_bytes;
+            if (rv_memcpy_h2d(c->d_codes + off, pq_codes[i], bytes)) {
+                set_err("pure_pq_load");
+                calculator_retrieval_pure_pq_unload_dma(h);
+                return ROVER_EFAILED;
+            }
+            off += bytes;
+        }
+    }
+    c->n = num_vectors;
+    c->dim = dim;
+    c->m = m;
+    c->ks = ks;
+    c->subdim = subdim;
+    c->code_bytes = code_bytes;
+    c->pq_loaded = 1;
+    return ROVER_SUCCESS;
+}
+
+AEEResult rover_pure_pq_graph_load(remote_handle64 h, const int32_t *graph,
+                                   int64_t graphLen, int32_t num_vectors,
+                                   int32_t graph_k, int32_t entry) {
+    RoverCtx *c = ctx_of(h);
+    if (!c) return ROVER_EBADPARM;
+    if (!c->pq_loaded || !graph || num_vectors != c->n || graph_k <= 0 ||
+        graph_k > 64 || graphLen != (int64_t)num_vectors * graph_k ||
+        entry >= num_vectors)
+        return ROVER_EBADPARM;
+    dfree(c->d_graph);
+    if (dalloc_copy((void **)&c->d_graph, graph,
+                    (size_t)graphLen * sizeof(int32_t))) {
+        set_err("graph_load");
+        c->d_graph = NULL;
+        return ROVER_ENOMEMORY;
+    }
+    c->graph_n = num_vectors;
+    c->degree = graph_k;
+    c->entry = entry;
+    c->entries_count = 0; /* seed cache depends on the entry point */
+    return ROVER_SUCCESS;
+}
+
+AEEResult rover_pure_pq_residual_load(remote_handle64 h, const uint8_t *codes,
+                                      int64_t codesLen, const float *qparams,
+                                      int64_t qparamsLen, int32_t bits) {
+    RoverCtx *c = ctx_of(h);
+    int64_t rb;
+    if (!c) return ROVER_EBADPARM;
+    if (!c->pq_loaded || !codes || !qparams || (bits != 4 && bits != 8))
+        return ROVER_EBADPARM;
+    rb = bits == 4 ? c->dim / 2 + c->dim % 2 : c->dim;
+    if (codesLen != rb * c->n || qparamsLen != 2 * (int64_t)c->n)
+        return ROVER_EBADPARM;
+    dfree(c->d_res);
+    dfree(c->d_qparams);
+    c->d_res = NULL;
+    c->d_qparams = NULL;
+    if (dalloc_copy((void **)&c->d_res, codes, (size_t)codesLen) ||
+        dalloc_copy((void **)&c->d_qparams, qparams,
+                    (size_t)qparamsLen * sizeof(float))) {
+        set_err("residual_load");
+        return ROVER_ENOMEMORY;
+    }
+    c->res_bits = bits;
+    return ROVER_SUCCESS;
+}
+
+AEEResult rover_pure_pq_original_ids(remote_handle64 h, const int32_t *map,
+                                     int mapLen) {
+    RoverCtx *c = ctx_of(h);
+    if (!c) return ROVER_EBADPARM;
+    dfree(c->d_orig);
+    c->d_orig = NULL;
+    if (!map) return ROVER_SUCCESS;
+    if (!c->pq_loaded || mapLen != c->n) return ROVER_EBADPARM;
+    {
+        /* must be a permutation (retrieval_final_refine.c:68-79) */
+        uint8_t *seen = calloc((size_t)c->n, 1);
+        if (!seen) return ROVER_ENOMEMORY;
+        for (int i = 0; i < c->n; i++) {
+            if (map[i] < 0 || map[i] >= c->n || seen[map[i]]) {
+                free(seen);
+                return ROVER_EBADPARM;
+            }
+            seen[map[i]] = 1;
+        }
+        free(seen);
+    }
+    if (dalloc_copy((void **)&c->d_orig, map, (size_t)mapLen * sizeof(int32_t)))
+        return ROVER_ENOMEMORY;
+    return ROVER_SUCCESS;
+}
+
+AEEResult calculator_retrieval_load(remote_handle64 h, const int32_t *graph,
+                                    int graphLen, const uint8_t *l1_codes,
+                                    int l1_codesLen, int32_t num_vectors,
+                                    int32_t dim, int32_t graph_k,
+                                    int32_t entry, float l1_scale,
+                                    float l1_zero) {
+    RoverCtx *c = ctx_of(h);
+    int cb;
+    if (!c) return ROVER_EBADPARM;
+    cb = c->l1_bits == 4 ? (dim + 1) / 2 : dim;
+    if (!graph || !l1_codes || num_vectors <= 0 || dim <= 0 || dim > 4096 ||
+        graph_k <= 0 || graph_k > 64 ||
+        (int64_t)graphLen != (int64_t)num_vectors * graph_k ||
+        (int64_t)l1_codesLen != (int64_t)num_vectors * cb ||
+        !isfinite(l1_scale) || !(l1_scale > 0.0f) || !isfinite(l1_zero) ||
+        entry >= num_vectors)
+        return ROVER_EBADPARM;
+    calculator_retrieval_unload(h);
+    if (dalloc_copy((void **)&c->d_sq_graph, graph, (size_t)graphLen * 4) ||
+        dalloc_copy((void **)&c->d_sq_codes, l1_codes, (size_t)l1_codesLen)) {
+        set_err("retrieval_load");
+        calculator_retrieval_unload(h);
+        return ROVER_ENOMEMORY;
+    }
+    c->sq_n = num_vectors;
+    c->sq_dim = dim;
+    c->sq_degree = graph_k;
+    c->sq_entry = entry;
+    c->sq_bits = c->l1_bits;
+    c->sq_scale = l1_scale;
+    c->sq_zero = l1_zero;
+    c->sq_loaded = 1;
+    c->entries_count = 0;
+    return ROVER_SUCCESS;
+}
+
+/* rpc_demo_pure_pq.c:192-208 (pure_pq_make_seed_ids); the DSP search uses
+ * the same recurrence (retrieval_async_dsp_search.c:219-230). */
+int rover_make_seed_ids(int n, int entry_count, int graph_entry, uint32_t seed,
+                        int32_t *out) {
+    int count = entry_count < n ? entry_count : n;
+    for (int i = 0; i < count; i++) {
+        if (i == 0 && graph_entry >= 0 && graph_entry < n) {
+            out[i] = graph_entry;
+        } else {
+            seed = seed * 1103515245u + 12345u;
+            out[i] = (int32_t)(((uint64_t)seed * (uint64_t)n) >> 32);
+        }
+    }
+    return count;
+}
+
+/* Uploads the seed list once per (n, entry_points, graph entry, seed). */
+static int ensure_entries(RoverCtx *c, int n, int entry_points, int graph_entry,
+                          uint32_t seed) {
+    int32_t *ids;
+    int count;
+    if (c->entries_count > 0 && c->entries_n == n &&
+        c->entries_graph_entry == graph_entry && c->entries_seed == seed &&
+        c->entries_count == (entry_points < n ? entry_points : n))
+        return 0;
+    if (entry_points > c->entries_cap) {
+        dfree(c->d_entries);
+        c->d_entries = NULL;
+        c->entries_cap = 0;
+        if (rv_malloc((void **)&c->d_entries, (size_t)entry_points * 4)) {
+            set_err("entries");
+            return -1;
+        }
+        c->entries_cap = entry_points;
+    }
+    ids = malloc((size_t)entry_points * sizeof(int32_t));
+    if (!ids) return -1;
+    count = rover_make_seed_ids(n, entry_points, graph_entry, seed, ids);
+    if (rv_memcpy_h2d(c->d_entries, ids, (size_t)count * 4)) {
+        free(ids);
+        set_err("entries upload");
+        return -1;
+    }
+    free(ids);
+    c->entries_n = n;
+    c->entries_graph_entry = graph_entry;
+    c->entries_seed = seed;
+    c->entries_count = count;
+    return 0;
+}
+
+static int ilog2_ceil(int v) {
+    int l = 0;
+    while ((1 << l) < v) l++;
+    return l;
+}
+
+/* Default visited-hash size: about 16 slots per pool entry, 256..4096. */
+static int default_hash_log2(int max_pool) {
+    int l = ilog2_ceil(max_pool * 16);
+    if (l < 8) l = 8;
+    if (l > 12) l = 12;
+    return l;
+}
+
+static int fill_index(const RoverCtx *c, int method, int correction,
+                      int refine, RvIndex *ix) {
+    memset(ix, 0, sizeof(*ix));
+    if (method == ROVER_METHOD_SQ) {
+        if (!c->sq_loaded) return -1;
+        ix->graph = c->d_sq_graph;
+        ix->degree = c->sq_degree;
+        ix->n = c->sq_n;
+        ix->graph_entry = c->sq_entry;
+        ix->dim = c->sq_dim;
+        ix->sq_codes = c->d_sq_codes;
+        ix->sq_bits = c->sq_bits;
+        ix->sq_code_bytes = c->sq_bits == 4 ? (c->sq_dim + 1) / 2 : c->sq_dim;
+        ix->sq_scale = c->sq_scale;
+        ix->sq_zero = c->sq_zero;
+        return 0;
+    }
+    if (!c->pq_loaded || !c->d_graph) return -1;
+    if ((correction || refine) && !c->d_res) return -1;
+    ix->graph = c->d_graph;
+    ix->degree = c->degree;
+    ix->n = c->n;
+    ix->graph_entry = c->entry;
+    ix->codes = c->d_codes;
+    ix->code_bytes = c->code_bytes;
+    ix->codebook = c->d_codebook;
+    ix->dim = c->dim;
+    ix->m = c->m;
+    ix->ks = c->ks;
+    ix->subdim = c->subdim;
+    ix->res_codes = c->d_res;
+    ix->res_qparams = c->d_qparams;
+    ix->res_bits = c->res_bits;
+    ix->res_code_bytes = c->res_bits == 4 ? c->dim / 2 + c->dim % 2 : c->dim;
+    ix->graph_to_original = c->d_orig;
+    return 0;
+}
+
+/* Largest per-query L of a batch; per-query arrays live on the device for
+ * rover_search_batch_device, so the caller states the bound in pool_size. */
+static AEEResult launch_batch(RoverCtx *c, const RoverSearchBatch *b,
+                              int max_pool, int explicit_entries,
+                              void *stream) {
+    RvIndex ix;
+    RvBatch rb;
+    uint32_t seed = b->seed ? b->seed : ROVER_ENTRY_POINT_SEED;
+    if (b->num_queries < 0 || b->entry_points <= 0 || b->expand_count <= 0 ||
+        max_pool <= 0 || max_pool > ROVER_MAX_POOL || b->out_stride <= 0 ||
+        !b->out_ids || !b->out_dists || !b->out_counts ||
+        (b->num_queries > 0 && !b->queries) || b->refine_candidates < 0 ||
+        (b->method != ROVER_METHOD_PQ && b->method != ROVER_METHOD_SQ) ||
+        (b->method == ROVER_METHOD_SQ && (b->correction || b->refine_candidates)))
+        return ROVER_EBADPARM;
+    if (fill_index(c, b->method, b->correction, b->refine_candidates, &ix)) {
+        snprintf(g_err, sizeof(g_err), "search: index not loaded");
+        return ROVER_EFAILED;
+    }
+    if (!explicit_entries &&
+        ensure_entries(c, ix.n, b->entry_points, ix.graph_entry, seed))
+        return ROVER_EFAILED;
+    memset(&rb, 0, sizeof(rb));
+    rb.queries = b->queries;
+    rb.nq = b->num_queries;
+    rb.pool_sizes = b->pool_sizes;
+    rb.num_outs = b->num_outs;
+    rb.pool_size = b->pool_size;
+    rb.num_out = b->num_out;
+    rb.max_pool = max_pool;
+    rb.entries = c->d_entries;
+    rb.n_entries = c->entries_count;
+    rb.expand_count = b->expand_count;
+    rb.scorer = b->method == ROVER_METHOD_SQ
+                    ? RV_SCORE_SQ
+                    : (b->correction ? RV_SCORE_ADC_ONLINE : RV_SCORE_ADC);
+    rb.refine_candidates = b->refine_candidates;
+    rb.hash_log2 = b->hash_log2 > 0 ? b->hash_log2 : default_hash_log2(max_pool);
+    rb.probe_limit = b->probe_limit > 0 ? b->probe_limit : 8;
+    rb.out_ids = b->out_ids;
+    rb.out_dists = b->out_dists;
+    rb.out_counts = b->out_counts;
+    rb.out_stride = b->out_stride;
+    rb.out_stats = b->out_stats;
+    rb.queue = c->d_queue;
+    rb.stop_label_k = 10;
+    if (b->learned_stop) {
+        if (!c->learned.ready) {
+            snprintf(g_err, sizeof(g_err), "search: learned stop model not set");
+            return ROVER_EFAILED;
+        }
+        rb.stop_mode = 1;
+        rb.stop_threshold = b->stop_threshold > 0.0f ? b->stop_threshold : 0.9f;
+        rb.stop_every = 1;
+        rb.mlp.w1 = c->learned.d_w;
+        rb.mlp.w2 = c->learned.d_w + RV_MLP_HIDDEN * RV_MLP_IN;
+        rb.mlp.w3 = c->learned.d_w + RV_MLP_HIDDEN * RV_MLP_IN +
+                    RV_MLP_HIDDEN * RV_MLP_HIDDEN;
+        rb.mlp.b1 = c->learned.d_f;
+        rb.mlp.b2 = c->learned.d_f + RV_MLP_HIDDEN;
+        rb.mlp.b3 = c->learned.d_f + 2 * RV_MLP_HIDDEN;
+        rb.mlp.feat_mean = c->learned.d_f + 2 * RV_MLP_HIDDEN + 1;
+        rb.mlp.feat_inv_std = c->learned.d_f + 2 * RV_MLP_HIDDEN + 1 + RV_MLP_IN;
+    }
+    if (rv_search_launch(&ix, &rb, b->grid_waves, stream)) {
+        set_err("search launch");
+        return ROVER_EFAILED;
+    }
+    return ROVER_SUCCESS;
+}
+
+AEEResult rover_search_batch_device(remote_handle64 h, const RoverSearchBatch *b,
+                                    void *stream) {
+    RoverCtx *c = ctx_of(h);
+    AEEResult rc;
+    if (!c || !b) return ROVER_EBADPARM;
+    pthread_mutex_lock(&c->lock);
+    rc = launch_batch(c, b, b->pool_size, 0, stream);
+    pthread_mutex_unlock(&c->lock);
+    return rc;
+}
+
+static int ensure_stage(RoverCtx *c, size_t bytes) {
+    if (bytes <= c->stage_cap) return 0;
+    dfree(c->d_stage);
+    c->d_stage = NULL;
+    c->stage_cap = 0;
+    if (rv_malloc(&c->d_stage, bytes)) {
+        set_err("stage alloc");
+        return -1;
+    }
+    c->stage_cap = bytes;
+    return 0;
+}
+
+static size_t up256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+static AEEResult search_host(RoverCtx *c, const RoverSearchBatch *b,
+                             int explicit_entries) {
+    RoverSearchBatch d;
+    int dim, max_pool;
+    size_t oq, ol, ok, oi, od, oc, os, total;
+    AEEResult rc = ROVER_EFAILED;
+    if (!c || !b) return ROVER_EBADPARM;
+    if (b->num_queries < 0 || (b->num_queries > 0 && !b->queries) ||
+        b->out_stride <= 0 || !b->out_ids || !b->out_dists || !b->out_counts)
+        return ROVER_EBADPARM;
+    if (b->num_queries == 0) return ROVER_SUCCESS;
+    dim = b->method == ROVER_METHOD_SQ ? c->sq_dim : c->dim;
+    max_pool = b->pool_size;
+    if (b->pool_sizes)
+        for (int i = 0; i < b->num_queries; i++)
+            if (b->pool_sizes[i] > max_pool) max_pool = b->pool_sizes[i];
+    pthread_mutex_lock(&c->lock);
+    oq = 0;
+    ol = up256(oq + (size_t)b->num_queries * dim * sizeof(float));
+    ok = up256(ol + (b->pool_sizes ? (size_t)b->num_queries * 4 : 0));
+    oi = up256(ok + (b->num_outs ? (size_t)b->num_queries * 4 : 0));
+    od = up256(oi + (size_t)b->num_queries * b->out_stride * 4);
+    oc = up256(od + (size_t)b->num_queries * b->out_stride * 4);
+    os = up256(oc + (size_t)b->num_queries * 4);
+    total = up256(os + (b->out_stats ? (size_t)b->num_queries * 32 : 0));
+    if (ensure_stage(c, total)) goto out;
+    {
+        char *base = (char *)c->d_stage;
+        d = *b;
+        d.queries = (const float *)(base + oq);
+        d.pool_sizes = b->pool_sizes ? (const int32_t *)(base + ol) : NULL;
+        d.num_outs = b->num_outs ? (const int32_t *)(base + ok) : NULL;
+        d.out_ids = (int32_t *)(base + oi);
+        d.out_dists = (float *)(base + od);
+        d.out_counts = (int32_t *)(base + oc);
+        d.out_stats = b->out_stats ? (uint64_t *)(base + os) : NULL;
+        if (rv_memcpy_h2d(base + oq, b->queries,
+                          (size_t)b->num_queries * dim * sizeof(float)) ||
+            (b->pool_sizes &&
+             rv_memcpy_h2d(base + ol, b->pool_sizes, (size_t)b->num_queries * 4)) ||
+            (b->num_outs &&
+             rv_memcpy_h2d(base + ok, b->num_outs, (size_t)b->num_queries * 4))) {
+            set_err("batch upload");
+            goto out;
+        }
+        rc = launch_batch(c, &d, max_pool, explicit_entries, NULL);
+        if (rc != ROVER_SUCCESS) goto out;
+        rc = ROVER_EFAILED;
+        if (rv_synchronize() ||
+            rv_memcpy_d2h(b->out_ids, base + oi, (size_t)b->num_queries * b->out_stride * 4) ||
+            rv_memcpy_d2h(b->out_dists, base + od, (size_t)b->num_queries * b->out_stride * 4) ||
+            rv_memcpy_d2h(b->out_counts, base + oc, (size_t)b->num_queries * 4) ||
+            (b->out_stats &&
+             rv_memcpy_d2h(b->out_stats, base + os, (size_t)b->num_queries * 32))) {
+            set_err("batch download");
+            goto out;
+        }
+        rc = ROVER_SUCCESS;
+    }
+out:
+    pthread_mutex_unlock(&c->lock);
+    return rc;
+}
+
+AEEResult rover_search_batch(remote_handle64 h, const RoverSearchBatch *b) {
+    RoverCtx *c = ctx_of(h);
+    if (!c) return ROVER_EBADPARM;
+    return search_host(c, b, 0);
+}
+
+/* calculator.idl:46-55 on the SQ index: one query through the batch path. */
+AEEResult calculator_retrieval_search(remote_handle64 h, const float *query,
+                                      int queryLen, int32_t entry_points,
+                                      int32_t expand_count, int32_t pool_size,
+                                      int32_t num_out, int32_t *ids, int idsLen,
+                                      float *dists, int distsLen,
+                                      int32_t *actual_count) {
+    RoverCtx *c = ctx_of(h);
+    RoverSearchBatch b;
+    int32_t count = 0;
+    AEEResult rc;
+    if (!actual_count) return ROVER_EBADPARM;
+    *actual_count = 0;
+    if (!c) return ROVER_EBADPARM;
+    if (!c->sq_loaded) return ROVER_EFAILED;
+    if (!query || queryLen < c->sq_dim || pool_size <= 0 || num_out <= 0 ||
+        !ids || !dists || idsLen < num_out || distsLen < num_out ||
+        entry_points <= 0 || expand_count <= 0)
+        return ROVER_EBADPARM;
+    if (pool_size > ROVER_MAX_POOL) pool_size = ROVER_MAX_POOL;
+    if (num_out > pool_size) num_out = pool_size;
+    memset(&b, 0, sizeof(b));
+    b.queries = query;
+    b.num_queries = 1;
+    b.pool_size = pool_size;
+    b.num_out = num_out;
+    b.method = ROVER_METHOD_SQ;
+    b.entry_points = entry_points;
+    b.seed = ROVER_ENTRY_POINT_SEED;
+    b.expand_count = expand_count;
+    b.out_ids = ids;
+    b.out_dists = dists;
+    b.out_counts = &count;
+    b.out_stride = num_out;
+    rc = rover_search_batch(h, &b);
+    if (rc != ROVER_SUCCESS) return rc;
+    if (count < 0) return ROVER_EFAILED;
+    *actual_count = count;
+    return ROVER_SUCCESS;
+}
+
+/* ------------------------------------------------ library search (face 2) */
+
+int rover_pq_residual_evaluator(void *context, int vector_id,
+                                float coarse_distance, float *out_distance) {
+    (void)context;
+    (void)vector_id;
+    (void)coarse_distance;
+    (void)out_distance;
+    return -1;
+}
+
+static pthread_mutex_t g_view_lock = PTHREAD_MUTEX_INITIALIZER;
+static remote_handle64 g_view_handle;
+static struct {
+    const int32_t *graph;
+    const void *chunk0;
+    const float *codebook;
+    const uint8_t *res_codes;
+    int n, degree, dim, m, ks, chunk_count, chunk_vectors, res_bits;
+} g_view_key;
+
+void rover_pure_pq_view_release(void) {
+    pthread_mutex_lock(&g_view_lock);
+    if (g_view_handle) calculator_close(g_view_handle);
+    g_view_handle = 0;
+    memset(&g_view_key, 0, sizeof(g_view_key));
+    pthread_mutex_unlock(&g_view_lock);
+}
+
+int retrieval_pure_pq_search_scalar_with_policy(
+    const PurePqSearchIndexView *ix, const PurePqSearchRequest *rq,
+    const PurePqDistancePolicy *policy, int *out_ids, float *out_distances,
+    int out_capacity, PurePqSearchStats *out_stats) {
+    const RoverResidualView *rv = NULL;
+    int online = 0, result = -1, ks_bytes;
+    if (out_stats) memset(out_stats, 0, sizeof(*out_stats));
+    /* retrieval_sqmd_pure_pq.c:251-278 (pure_pq_validate_search) */
+    if (!ix || !rq || !out_ids || !out_distances || out_capacity <= 0 ||
+        !ix->graph || ix->graph_degree <= 0 || ix->graph_degree > 64 ||
+        ix->num_vectors <= 0 || !ix->code_chunks || !ix->chunk_vector_counts ||
+        ix->chunk_count <= 0 || ix->chunk_vectors <= 0 || !ix->codebook ||
+        ix->dim <= 0 || ix->m <= 0 || (ix->ks != 16 && ix->ks != 256) ||
+        ix->subdim <= 0 || ix->m * ix->subdim != ix->dim || !rq->query ||
+        !rq->entry_ids || rq->entry_count <= 0 || rq->expand_count <= 0 ||
+        rq->pool_capacity <= 0 || rq->num_out <= 0 ||
+        rq->num_out > rq->pool_capacity || rq->pool_capacity > ROVER_MAX_POOL)
+        return -1;
+    for (int i = 0; i < ix->chunk_count; i++)
+        if (!ix->code_chunks[i] || ix->chunk_vector_counts[i] <= 0 ||
+            ix->chunk_vector_counts[i] > ix->chunk_vectors)
+            return -1;
+    /* retrieval_sqmd_pure_pq.c:31-38: OFF, or ONLINE with an evaluator;
+     * the GPU runs only the residual evaluator it knows. */
+    if (policy && policy->mode != PURE_PQ_CORRECTION_OFF) {
+        if (policy->mode != PURE_PQ_CORRECTION_ONLINE ||
+            policy->evaluator != rover_pq_residual_evaluator || !policy->context)
+            return -1;
+        rv = (const RoverResidualView *)policy->context;
+        if (!rv->codes || !rv->qparams || (rv->bits != 4 && rv->bits != 8))
+            return -1;
+        online = 1;
+    }
+    ks_bytes = ix->ks == 16 ? ix->m / 2 + ix->m % 2 : ix->m;
+    pthread_mutex_lock(&g_view_lock);
+    if (!g_view_handle || g_view_key.graph != ix->graph ||
+        g_view_key.chunk0 != ix->code_chunks[0] ||
+        g_view_key.codebook != ix->codebook || g_view_key.n != ix->num_vectors ||
+        g_view_key.degree != ix->graph_degree || g_view_key.dim != ix->dim ||
+        g_view_key.m != ix->m || g_view_key.ks != ix->ks ||
+        g_view_key.chunk_count != ix->chunk_count ||
+        g_view_key.chunk_vectors != ix->chunk_vectors ||
+        (online && (g_view_key.res_codes != rv->codes ||
+                    g_view_key.res_bits != rv->bits))) {
+        if (g_view_handle) calculator_close(g_view_handle);
+        g_view_handle = 0;
+        memset(&g_view_key, 0, sizeof(g_view_key));
+        if (calculator_open(NULL, &g_view_handle) != ROVER_SUCCESS ||
+            calculator_retrieval_pure_pq_load_dma(
+                g_view_handle, ix->code_chunks, ix->chunk_count,
+                ix->chunk_vector_counts, ix->chunk_count, ix->chunk_vectors,
+                ix->codebook, ix->m * ix->ks * ix->subdim, ix->num_vectors,
+                ix->dim, ix->m, ix->ks, ix->subdim, ks_bytes, NULL, 0) ||
+            rover_pure_pq_graph_load(g_view_handle, ix->graph,
+                                     (int64_t)ix->num_vectors * ix->graph_degree,
+                                     ix->num_vectors, ix->graph_degree, -1) ||
+            (online && rover_pure_pq_residual_load(
+                           g_view_handle, rv->codes,
+                           (int64_t)ix->num_vectors *
+                               (rv->bits == 4 ? ix->dim / 2 + ix->dim % 2 : ix->dim),
+                           rv->qparams, 2 * (int64_t)ix->num_vectors, rv->bits))) {
+            if (g_view_handle) calculator_close(g_view_handle);
+            g_view_handle = 0;
+            goto out;
+        }
+        g_view_key.graph = ix->graph;
+        g_view_key.chunk0 = ix->code_chunks[0];
+        g_view_key.codebook = ix->codebook;
+        g_view_key.n = ix->num_vectors;
+        g_view_key.degree = ix->graph_degree;
+        g_view_key.dim = ix->dim;
+        g_view_key.m = ix->m;
+        g_view_key.ks = ix->ks;
+        g_view_key.chunk_count = ix->chunk_count;
+        g_view_key.chunk_vectors = ix->chunk_vectors;
+        g_view_key.res_codes = online ? rv->codes : NULL;
+        g_view_key.res_bits = online ? rv->bits : 0;
+    }
+    {
+        /* explicit entry ids (not the LCG): upload them as the seed list */
+        RoverCtx *c = ctx_of(g_view_handle);
+        RoverSearchBatch b;
+        int32_t count = 0;
+        uint64_t st[4] = {0, 0, 0, 0};
+        int num_out = rq->num_out < out_capacity ? rq->num_out : out_capacity;
+        if (rq->entry_count > c->entries_cap) {
+            dfree(c->d_entries);
+            c->d_entries = NULL;
+            c->entries_cap = 0;
+            if (rv_malloc((void **)&c->d_entries, (size_t)rq->entry_count * 4))
+                goto out;
+            c->entries_cap = rq->entry_count;
+        }
+        if (rv_memcpy_h2d(c->d_entries, rq->entry_ids, (size_t)rq->entry_count * 4))
+            goto out;
+        /* pin the cache key to something ensure_entries never matches */
+        c->entries_n = -1;
+        c->entries_count = rq->entry_count;
+        memset(&b, 0, sizeof(b));
+        b.queries = rq->query;
+        b.num_queries = 1;
+        b.pool_size = rq->pool_capacity;
+        b.num_out = num_out;
+        b.method = ROVER_METHOD_PQ;
+        b.correction = online;
+        b.entry_points = rq->entry_count;
+        b.expand_count = rq->expand_count;
+        b.out_ids = out_ids;
+        b.out_dists = out_distances;
+        b.out_counts = &count;
+        b.out_stride = num_out;
+        b.out_stats = st;
+        if (search_host(c, &b, 1) != ROVER_SUCCESS || count < 0)
+            goto out;
+        if (out_stats) {
+            out_stats->pq_scored_candidates = st[0];
+            out_stats->corrected_candidates = st[1];
+            out_stats->expanded_nodes = st[2];
+            out_stats->visited_dedup_skips = st[3];
+        }
+        result = count;
+    }
+out:
+    pthread_mutex_unlock(&g_view_lock);
+    return result;
+}
+
+int retrieval_pure_pq_search_scalar(const PurePqSearchIndexView *index,
+                                    const PurePqSearchRequest *request,
+                                    int *out_ids, float *out_distances,
+                                    int out_capacity,
+                                    PurePqSearchStats *out_stats) {
+    return retrieval_pure_pq_search_scalar_with_policy(
+        index, request, NULL, out_ids, out_distances, out_capacity, out_stats);
+}
+
+/* ---------------------------------------------------------- device memory */
+
+AEEResult rover_device_alloc(void **d_ptr, size_t bytes) {
+    if (!d_ptr) return ROVER_EBADPARM;
+    if (rv_malloc(d_ptr, bytes)) {
+        set_err("rover_device_alloc");
+        return ROVER_ENOMEMORY;
+    }
+    return ROVER_SUCCESS;
+}
+AEEResult rover_device_free(void *d_ptr) {
+    return rv_free(d_ptr) ? ROVER_EFAILED : ROVER_SUCCESS;
+}
+AEEResult rover_device_upload(void *d_dst, const void *src, size_t bytes) {
+    if (rv_memcpy_h2d(d_dst, src, bytes)) {
+        set_err("rover_device_upload");
+        return ROVER_EFAILED;
+    }
+    return ROVER_SUCCESS;
+}
+AEEResult rover_device_download(void *dst, const void *d_src, size_t bytes) {
+    if (rv_memcpy_d2h(dst, d_src, bytes)) {
+        set_err("rover_device_download");
+        return ROVER_EFAILED;
+    }
+    return ROVER_SUCCESS;
+}
+AEEResult rover_device_synchronize(void) {
+    if (rv_synchronize()) {
+        set_err("rover_device_synchronize");
+        return ROVER_EFAILED;
+    }
+    return ROVER_SUCCESS;
+}
+
+AEEResult rover_build_pq_encode(const float *d_x, int n, int dim, int m, int ks,
+                                const float *d_codebook, uint8_t *d_codes) {
+    if (!d_x || !d_codebook || !d_codes || n <= 0 || m <= 0 || dim % m)
+        return ROVER_EBADPARM;
+    if (rv_pq_encode(d_x, n, dim, m, ks, d_codebook, d_codes, NULL)) {
+        set_err("rover_build_pq_encode");
+        return ROVER_EFAILED;
+    }
+    return ROVER_SUCCESS;
+}
+
+AEEResult rover_build_residual_encode(const float *d_x, const uint8_t *d_codes,
+                                      int n, int dim, int m, int ks,
+                                      const float *d_codebook, int bits,
+                                      uint8_t *d_res, float *d_qparams) {
+    if (!d_x || !d_codes || !d_codebook || !d_res || !d_qparams || n <= 0 ||
+        m <= 0 || dim % m || (bits != 4 && bits != 8))
+        return ROVER_EBADPARM;
+    if (rv_residual_encode(d_x, d_codes, n, dim, m, ks, d_codebook, bits, d_res,
+                           d_qparams, NULL)) {
+        set_err("rover_build_residual_encode");
+        return ROVER_EFAILED;
+    }
+    return ROVER_SUCCESS;
+}
+
+AEEResult rover_event_create(void **event) {
+    if (!event || rv_event_create(event)) {
+        set_err("rover_event_create");
+        return ROVER_EFAILED;
+    }
+    return ROVER_SUCCESS;
+}
+AEEResult rover_event_record(void *event, void *stream) {
+    if (rv_event_record(event, stream)) {
+        set_err("rover_event_record");
+        return ROVER_EFAILED;
+    }
+    return ROVER_SUCCESS;
+}
+AEEResult rover_event_elapsed_ms(void *start, void *stop, float *ms) {
+    if (!ms || rv_event_elapsed(start, stop, ms)) {
+        set_err("rover_event_elapsed_ms");
+        return ROVER_EFAILED;
+    }
+    return ROVER_SUCCESS;
+}
+AEEResult rover_event_destroy(void *event) {
+    return rv_event_destroy(event) ? ROVER_EFAILED : ROVER_SUCCESS;
+}
+AEEResult rover_stream_create(void **stream) {
+    if (!stream || rv_stream_create(stream)) {
+        set_err("rover_stream_create");
+        return ROVER_EFAILED;
+    }
+    return ROVER_SUCCESS;
+}
+AEEResult rover_stream_destroy(void *stream) {
+    return rv_stream_destroy(stream) ? ROVER_EFAILED : ROVER_SUCCESS;
+}
+AEEResult rover_stream_synchronize(void *stream) {
+    if (rv_stream_synchronize(stream)) {
+        set_err("rover_stream_synchronize");
+        return ROVER_EFAILED;
+    }
+    return ROVER_SUCCESS;
+}

@@ -1,0 +1,1013 @@
+// rover_search.hip -- CDNA4 (gfx950) kernels for Rover's quantized graph
+// search: batched best-first beam search, one wavefront per query, run as a
+// persistent kernel over a device work counter.
+//
+// Semantics follow the reference scalar search
+// (retrieval_sqmd_pure_pq.c:793-969, retrieval_pure_pq_search_scalar_with_policy):
+//   * candidates are ordered by (distance, id) (pure_pq_order, :97-112);
+//   * the result pool keeps the best L candidates seen;
+//   * the search repeatedly expands the best pool member that has not been
+//     expanded yet and stops when none is left.  This expansion sequence is
+//     the one the reference's frontier heap produces: a frontier entry that
+//     was evicted from the pool is worse than the pool's worst member from
+//     then on, so when it reaches the heap top the reference terminates
+//     (:875-888); every other heap top is the best unexpanded pool member.
+//   * a node is scored once; re-scoring a node that is not in the pool can
+//     never change the pool (its key is worse than the pool's worst for the
+//     rest of the search), so the LDS visited hash may drop entries when it
+//     is full without changing results.  Duplicates of pool members are
+//     removed exactly in the merge (identical key).
+//
+// All float arithmetic keeps the reference's operation order with no
+// contraction (the file is compiled with -ffp-contract=off):
+//   LUT    retrieval_pq_core.c:362-390 + 16-25 (sequential over subdim)
+//   ADC    retrieval_pq_core.c:392-416 (sequential over m)
+//   online retrieval_pq_residual.c:983-1042 (((c + code*s) + z), sequential)
+//   SQ     retrieval_l1_int8_core.c:119-165 (integer sum, one f32 multiply)
+//   refine retrieval_final_refine.c:256-292 (f64 sequential, (dist, orig))
+//
+// LDS holds, per query: the query vector, the ADC table (m*ks f32) or the
+// quantized query, the sorted pool as two ping-pong arrays of 64-bit keys
+// (float bits << 32 | id; distances are >= +0 so unsigned order equals
+// (distance, id) order), expanded flags, and the visited hash.
+#include <hip/hip_runtime.h>
+
+#include <float.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "rover_hip.h"
+
+#define RV_WAVE 64
+#define RV_EMPTY_SLOT 0xffffffffu
+#define RV_KEY_MAX 0xffffffffffffffffull
+
+static __thread char g_rv_error[256];
+
+static int rv_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) {
+        snprintf(g_rv_error, sizeof(g_rv_error), "%s: %s", what,
+                 hipGetErrorString(e));
+        return -1;
+    }
+    return 0;
+}
+
+extern "C" const char *rv_last_error(void) { return g_rv_error; }
+
+// ---------------------------------------------------------------- wave utils
+
+__device__ __forceinline__ int rv_lane() { return (int)__lane_id(); }
+
+__device__ __forceinline__ uint64_t rv_key(float d, int id) {
+    return ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)id;
+}
+__device__ __forceinline__ float rv_key_dist(uint64_t k) {
+    return __uint_as_float((uint32_t)(k >> 32));
+}
+__device__ __forceinline__ int rv_key_id(uint64_t k) {
+    return (int)(uint32_t)(k & 0xffffffffu);
+}
+
+__device__ __forceinline__ uint64_t rv_uniform64(uint64_t v) {
+    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ int rv_uniform(int v) {
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+// Ascending bitonic sort of one 64-bit key per lane across the wavefront.
+__device__ __forceinline__ uint64_t rv_wave_sort(uint64_t key) {
+    const int lane = rv_lane();
+#pragma unroll
+    for (int k = 2; k <= RV_WAVE; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            uint64_t other = __shfl_xor(key, j);
+            bool up = (lane & k) == 0;
+            bool lower = (lane & j) == 0;
+            uint64_t lo = key < other ? key : other;
+            uint64_t hi = key < other ? other : key;
+            key = (lower == up) ? lo : hi;
+        }
+    }
+    return key;
+}
+
+// Number of keys in sorted arr[0..n) strictly smaller than key.
+__device__ __forceinline__ int rv_lower_bound(const uint64_t *arr, int n,
+                                              uint64_t key) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (arr[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Open-addressing visited set in LDS.  Returns true when id was not
+// recorded before (newly inserted, or the probe budget ran out: then the
+// node is simply scored again, which cannot change results).
+__device__ __forceinline__ bool rv_visit(uint32_t *table, int log2n,
+                                         int probe_limit, uint32_t id) {
+    const uint32_t mask = (1u << log2n) - 1u;
+    uint32_t h = (id * 0x9E3779B1u) >> (32 - log2n);
+    for (int p = 0; p < probe_limit; p++) {
+        uint32_t old = atomicCAS(&table[h], RV_EMPTY_SLOT, id);
+        if (old == RV_EMPTY_SLOT) return true;
+        if (old == id) return false;
+        h = (h + 1u) & mask;
+    }
+    return true;
+}
+
+// ----------------------------------------------------------- distance ops
+
+__device__ __forceinline__ int rv_code_get(const uint8_t *code, int s, int ks) {
+    if (ks <= 16) {
+        uint8_t packed = code[s >> 1];
+        return (s & 1) ? (packed >> 4) : (packed & 0x0f);
+    }
+    return code[s];
+}
+
+// ADC: sequential f32 sum over subspaces of LUT[m][code_m].
+__device__ __forceinline__ float rv_adc(const float *lut, const uint8_t *row,
+                                        int m, int ks, int code_bytes) {
+    float acc = 0.0f;
+    if (ks == 256 && (code_bytes & 15) == 0) {
+        for (int c = 0; c < code_bytes; c += 16) {
+            const uint4 w = *reinterpret_cast<const uint4 *>(row + c);
+            const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+            for (int b = 0; b < 16; b++) {
+                const int code = (ws[b >> 2] >> (8 * (b & 3))) & 255;
+                acc += lut[(c + b) * 256 + code];
+            }
+        }
+    } else {
+        for (int s = 0; s < m; s++) acc += lut[s * ks + rv_code_get(row, s, ks)];
+    }
+    return acc;
+}
+
+__device__ __forceinline__ int rv_res_code(const uint8_t *res, int bits, int d) {
+    if (bits == 4) {
+        uint8_t packed = res[d >> 1];
+        return (d & 1) ? (packed >> 4) : (packed & 0x0f);
+    }
+    return res[d];
+}
+
+// Residual-reconstructed value ((centroid + code*scale) + zero).
+__device__ __forceinline__ float rv_decode_value(float c, int code, float s,
+                                                 float z) {
+    float product = (float)code * s;
+    float sum = c + product;
+    return sum + z;
+}
+
+// Online correction distance; returns false for invalid qparams.
+__device__ bool rv_online(const RvIndex &ix, const float *q, int id,
+                          float *out) {
+    const uint8_t *pq = ix.codes + (size_t)id * ix.code_bytes;
+    const uint8_t *res = ix.res_codes + (size_t)id * ix.res_code_bytes;
+    const float s = ix.res_qparams[2 * (size_t)id];
+    const float z = ix.res_qparams[2 * (size_t)id + 1];
+    if (!isfinite(s) || !isfinite(z) || !(s > 0.0f)) return false;
+    float dist = 0.0f;
+    int d = 0;
+    for (int sub = 0; sub < ix.m; sub++) {
+        const float *cent = ix.codebook +
+            ((size_t)sub * ix.ks + rv_code_get(pq, sub, ix.ks)) * ix.subdim;
+        for (int i = 0; i < ix.subdim; i++, d++) {
+            float rec = rv_decode_value(cent[i], rv_res_code(res, ix.res_bits, d), s, z);
+            float diff = q[d] - rec;
+            dist += diff * diff;
+        }
+    }
+    *out = dist;
+    return true;
+}
+
+__device__ __forceinline__ float rv_sq(const uint8_t *qsq, const uint8_t *code,
+                                       int dim, int bits, float scale_sq) {
+    int acc = 0;
+    if (bits == 8) {
+        int i = 0;
+        if ((dim & 15) == 0) {
+            for (; i < dim; i += 16) {
+                const uint4 w = *reinterpret_cast<const uint4 *>(code + i);
+                const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+                for (int b = 0; b < 16; b++) {
+                    int c = (ws[b >> 2] >> (8 * (b & 3))) & 255;
+                    int diff = (int)qsq[i + b] - c;
+                    acc += diff * diff;
+                }
+            }
+        }
+        for (; i < dim; i++) {
+            int diff = (int)qsq[i] - (int)code[i];
+            acc += diff * diff;
+        }
+    } else {
+        for (int i = 0; i < dim; i++) {
+            int c = (i & 1) ? (code[i >> 1] >> 4) : (code[i >> 1] & 15);
+            int diff = (int)(qsq[i] & 15) - c;
+            acc += diff * diff;
+        }
+    }
+    return (float)acc * scale_sq;
+}
+
+// ----------------------------------------------------------- LDS layout
+
+struct RvLayout {
+    int off_q, off_tab, off_keys0, off_keys1, off_flags0, off_flags1;
+    int off_cand, off_hash, total;
+};
+
+__host__ __device__ static inline int rv_align(int v, int a) {
+    return (v + a - 1) / a * a;
+}
+
+__host__ __device__ static inline RvLayout rv_layout(int dim, int scorer, int m,
+                                                    int ks, int max_pool,
+                                                    int hash_log2) {
+    RvLayout l;
+    int off = 0;
+    l.off_q = off;
+    off = rv_align(off + 4 * dim, 16);
+    l.off_tab = off;
+    if (scorer == RV_SCORE_ADC) off = rv_align(off + 4 * m * ks, 16);
+    else if (scorer == RV_SCORE_SQ) off = rv_align(off + dim, 16);
+    l.off_keys0 = off;
+    off = rv_align(off + 8 * max_pool, 16);
+    l.off_keys1 = off;
+    off = rv_align(off + 8 * max_pool, 16);
+    l.off_flags0 = off;
+    off = rv_align(off + max_pool, 16);
+    l.off_flags1 = off;
+    off = rv_align(off + max_pool, 16);
+    l.off_cand = off;
+    off = rv_align(off + 8 * RV_WAVE, 16);
+    l.off_hash = off;
+    off = rv_align(off + 4 * (1 << hash_log2), 16);
+    l.total = off;
+    return l;
+}
+
+// ------------------------------------------------------------ learned stop
+
+__device__ __forceinline__ float rv_bf16(uint16_t v) {
+    return __uint_as_float((uint32_t)v << 16);
+}
+
+// Features of the search state after `step` expansions (identical code
+// path for trace recording and inference).  All inputs are wave-uniform.
+struct RvStopState {
+    float d_entry, top1_0, topk_0;
+    int last_change, admitted_last;
+};
+
+__device__ __forceinline__ void rv_features(const RvStopState &st,
+                                            const uint64_t *keys, int n, int L,
+                                            int label_k, int step, int next_pos,
+                                            int degree, float *f) {
+    const float eps = 1e-30f;
+    float top1 = rv_key_dist(keys[0]);
+    int kk = (n < label_k ? n : label_k) - 1;
+    float topk = rv_key_dist(keys[kk]);
+    float worst = rv_key_dist(keys[n - 1]);
+    float fmin = next_pos < n ? rv_key_dist(keys[next_pos]) : worst;
+    f[0] = __logf(st.d_entry + 1.0f);
+    f[1] = __logf(top1 + 1.0f);
+    f[2] = __logf(topk + 1.0f);
+    f[3] = top1 / (st.d_entry + eps);
+    f[4] = topk / (st.d_entry + eps);
+    f[5] = top1 / (st.top1_0 + eps);
+    f[6] = topk / (st.topk_0 + eps);
+    f[7] = __log2f((float)step + 1.0f);
+    f[8] = __logf(fmin + 1.0f);
+    f[9] = fmin / (topk + eps);
+    f[10] = __log2f((float)(step - st.last_change) + 1.0f);
+    f[11] = (float)st.admitted_last / (float)degree;
+    f[12] = (float)n / (float)L;
+    f[13] = (float)next_pos / (float)L;
+    f[14] = worst / (topk + eps);
+    f[15] = fmin / (worst + eps);
+}
+
+// 16 -> 64 -> 64 -> 1 MLP, lane j owns hidden unit j.  FP32 accumulation
+// over BF16 weights; returns the logit (wave-uniform).
+__device__ float rv_mlp_logit(const RvMlp &mlp, const float *f_raw) {
+    const int j = rv_lane();
+    float x[RV_MLP_IN];
+#pragma unroll
+    for (int i = 0; i < RV_MLP_IN; i++)
+        x[i] = (f_raw[i] - mlp.feat_mean[i]) * mlp.feat_inv_std[i];
+    float h1 = mlp.b1[j];
+#pragma unroll
+    for (int i = 0; i < RV_MLP_IN; i++) h1 += rv_bf16(mlp.w1[j * RV_MLP_IN + i]) * x[i];
+    h1 = h1 > 0.0f ? h1 : 0.0f;
+    float h2 = mlp.b2[j];
+#pragma unroll 8
+    for (int i = 0; i < RV_MLP_HIDDEN; i++) {
+        float hi = __shfl(h1, i);
+        h2 += rv_bf16(mlp.w2[j * RV_MLP_HIDDEN + i]) * hi;
+    }
+    h2 = h2 > 0.0f ? h2 : 0.0f;
+    float o = rv_bf16(mlp.w3[j]) * h2;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) o += __shfl_xor(o, off);
+    return o + mlp.b3[0];
+}
+
+// ------------------------------------------------------------- the kernel
+
+template <int SCORER>
+__global__ __launch_bounds__(RV_WAVE) void rv_search_kernel(RvIndex ix, RvBatch b,
+                                                            RvLayout lay) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    float *qv = reinterpret_cast<float *>(smem + lay.off_q);
+    float *lut = reinterpret_cast<float *>(smem + lay.off_tab);
+    uint8_t *qsq = reinterpret_cast<uint8_t *>(smem + lay.off_tab);
+    uint64_t *keys_buf[2] = {reinterpret_cast<uint64_t *>(smem + lay.off_keys0),
+                             reinterpret_cast<uint64_t *>(smem + lay.off_keys1)};
+    uint8_t *flags_buf[2] = {smem + lay.off_flags0, smem + lay.off_flags1};
+    uint64_t *cand = reinterpret_cast<uint64_t *>(smem + lay.off_cand);
+    uint32_t *hash = reinterpret_cast<uint32_t *>(smem + lay.off_hash);
+    const int lane = rv_lane();
+    const int hash_n = 1 << b.hash_log2;
+    const int expand = b.expand_count < ix.degree ? b.expand_count : ix.degree;
+    const float sq_scale_sq = ix.sq_scale * ix.sq_scale;
+
+    for (;;) {
+        int q = 0;
+        if (lane == 0) q = atomicAdd(b.queue, 1);
+        q = rv_uniform(__shfl(q, 0));
+        if (q >= b.nq) break;
+
+        const int L = b.pool_sizes ? b.pool_sizes[q] : b.pool_size;
+        const int K = b.num_outs ? b.num_outs[q] : b.num_out;
+        int32_t *oid = b.out_ids + (size_t)q * b.out_stride;
+        float *odist = b.out_dists + (size_t)q * b.out_stride;
+        const int refine = b.refine_candidates;
+        if (L <= 0 || L > b.max_pool || K <= 0 || K > L || K > b.out_stride) {
+            if (lane == 0) b.out_counts[q] = RV_QUERY_BAD_PARAM;
+            continue;
+        }
+
+        // ---- per-query setup: query vector, table, visited hash
+        const float *qg = b.queries + (size_t)q * ix.dim;
+        for (int d = lane; d < ix.dim; d += RV_WAVE) qv[d] = qg[d];
+        for (int i = lane; i < hash_n; i += RV_WAVE) hash[i] = RV_EMPTY_SLOT;
+        __syncthreads();
+        if (SCORER == RV_SCORE_ADC) {
+            const int entries = ix.m * ix.ks;
+            for (int e = lane; e < entries; e += RV_WAVE) {
+                const int sub = e / ix.ks;
+                const float *qs = qv + sub * ix.subdim;
+                const float *c = ix.codebook + (size_t)e * ix.subdim;
+                float acc = 0.0f;
+                for (int d = 0; d < ix.subdim; d++) {
+                    float diff = qs[d] - c[d];
+                    acc += diff * diff;
+                }
+                lut[e] = acc;
+            }
+        } else if (SCORER == RV_SCORE_SQ) {
+            const float inv = 1.0f / ix.sq_scale;
+            const int hi = ix.sq_bits == 4 ? 15 : 255;
+            for (int d = lane; d < ix.dim; d += RV_WAVE) {
+                int v = (int)roundf((qv[d] - ix.sq_zero) * inv);
+                v = v < 0 ? 0 : (v > hi ? hi : v);
+                qsq[d] = (uint8_t)v;
+            }
+        }
+        __syncthreads();
+
+        int cur = 0;        // which ping-pong buffer holds the pool
+        int n = 0;          // pool size
+        int status = 0;
+        uint64_t st_scored = 0, st_expanded = 0, st_dedup = 0;
+        RvStopState ss;
+        ss.d_entry = 0.0f;
+        ss.top1_0 = 0.0f;
+        ss.topk_0 = 0.0f;
+        ss.last_change = 0;
+        ss.admitted_last = 0;
+        int trace_n = 0;
+
+        // Merge one candidate per lane (key, or RV_KEY_MAX for none) into
+        // the pool.  Returns the lowest position a new candidate landed at
+        // (or INT_MAX) via *first_new, the count admitted, and the new
+        // position of the pool entry at old position `track` via *tracked.
+        auto merge = [&](uint64_t key, int track, int *first_new, int *admitted,
+                         int *tracked) {
+            uint64_t *src = keys_buf[cur];
+            uint64_t *dst = keys_buf[cur ^ 1];
+            uint8_t *fsrc = flags_buf[cur];
+            uint8_t *fdst = flags_buf[cur ^ 1];
+            if (n == L && key != RV_KEY_MAX && key >= src[L - 1]) key = RV_KEY_MAX;
+            *first_new = 0x7fffffff;
+            *admitted = 0;
+            *tracked = track;
+            if (__ballot(key != RV_KEY_MAX) == 0ull) return;
+            key = rv_wave_sort(key);
+            uint64_t prev = __shfl_up(key, 1);
+            bool valid = key != RV_KEY_MAX && !(lane > 0 && prev == key);
+            int lb = 0;
+            if (valid) {
+                lb = rv_lower_bound(src, n, key);
+                if (lb < n && src[lb] == key) valid = false;  // already pooled
+            }
+            const uint64_t vmask = __ballot(valid);
+            const int cv = __popcll(vmask);
+            if (cv == 0) return;
+            const int rank = __popcll(vmask & ((1ull << lane) - 1ull));
+            // sorted valid candidates, compacted, for the pool-side search
+            if (valid) cand[rank] = key;
+            if (track >= 0 && track < n) {
+                const uint64_t tk = src[track];
+                *tracked = track + rv_uniform(__popcll(__ballot(valid && key < tk)));
+            }
+            __syncthreads();
+            for (int j = lane; j < n; j += RV_WAVE) {
+                const uint64_t pk = src[j];
+                int lo = 0, hi = cv;
+                while (lo < hi) {
+                    int mid = (lo + hi) >> 1;
+                    if (cand[mid] < pk) lo = mid + 1;
+                    else hi = mid;
+                }
+                const int np = j + lo;
+                if (np < L) {
+                    dst[np] = pk;
+                    fdst[np] = fsrc[j];
+                }
+            }
+            int pos = 0x7fffffff;
+            if (valid) {
+                pos = lb + rank;
+                if (pos < L) {
+                    dst[pos] = key;
+                    fdst[pos] = 0;
+                } else {
+                    pos = 0x7fffffff;
+                }
+            }
+            // wave min of pos
+            int mn = pos;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                int o = __shfl_xor(mn, off);
+                mn = o < mn ? o : mn;
+            }
+            *first_new = rv_uniform(mn);
+            *admitted = rv_uniform(__popcll(__ballot(pos != 0x7fffffff)));
+            n = (n + cv) < L ? (n + cv) : L;
+            cur ^= 1;
+            __syncthreads();
+        };
+
+        // ---- seeds (rpc_demo_pure_pq.c:192-208 seeds; :843-873 scoring)
+        for (int e0 = 0; e0 < b.n_entries && status == 0; e0 += RV_WAVE) {
+            const int e = e0 + lane;
+            uint64_t key = RV_KEY_MAX;
+            bool bad = false, nonfinite = false, dup = false;
+            if (e < b.n_entries) {
+                const int id = b.entries[e];
+                if (id < 0 || id >= ix.n) {
+                    bad = true;
+                } else if (!rv_visit(hash, b.hash_log2, b.probe_limit, (uint32_t)id)) {
+                    dup = true;
+                } else {
+                    float d = 0.0f;
+                    bool ok = true;
+                    if (SCORER == RV_SCORE_ADC) {
+                        d = rv_adc(lut, ix.codes + (size_t)id * ix.code_bytes, ix.m,
+                                   ix.ks, ix.code_bytes);
+                    } else if (SCORER == RV_SCORE_ADC_ONLINE) {
+                        ok = rv_online(ix, qv, id, &d);
+                    } else {
+                        d = rv_sq(qsq, ix.sq_codes + (size_t)id * ix.sq_code_bytes,
+                                  ix.dim, ix.sq_bits, sq_scale_sq);
+                    }
+                    if (!ok || !isfinite(d) || d == FLT_MAX) nonfinite = true;
+                    else key = rv_key(d + 0.0f, id);
+                }
+            }
+            // the reference validates entries one at a time: an invalid entry
+            // fails the query whatever follows it
+            if (__ballot(bad) != 0ull) status = RV_QUERY_BAD_ENTRY;
+            else if (__ballot(nonfinite) != 0ull) status = RV_QUERY_NONFINITE;
+            if (status) break;
+            st_scored += __popcll(__ballot(key != RV_KEY_MAX));
+            st_dedup += __popcll(__ballot(dup));
+            if (e0 == 0 && lane == 0 && key != RV_KEY_MAX) ss.d_entry = rv_key_dist(key);
+            int fn, adm, tr;
+            merge(key, -1, &fn, &adm, &tr);
+        }
+        ss.d_entry = __shfl(ss.d_entry, 0);
+        if (status == 0 && n == 0) status = RV_QUERY_BAD_ENTRY;
+        if (status == 0) {
+            const uint64_t *k0 = keys_buf[cur];
+            ss.top1_0 = rv_key_dist(k0[0]);
+            ss.topk_0 = rv_key_dist(k0[(n < b.stop_label_k ? n : b.stop_label_k) - 1]);
+        }
+
+        // ---- main loop: expand the best unexpanded pool member
+        int scan = 0;
+        int step = 0;
+        bool stopped = false;
+        while (status == 0) {
+            const uint64_t *keys = keys_buf[cur];
+            uint8_t *flags = flags_buf[cur];
+            int next = -1;
+            while (scan < n) {
+                const int p = scan + lane;
+                const bool open = p < n && flags[p] == 0;
+                const uint64_t m = __ballot(open);
+                if (m) {
+                    next = scan + __ffsll((unsigned long long)m) - 1;
+                    break;
+                }
+                scan += RV_WAVE;
+            }
+            next = rv_uniform(next);
+            if (next < 0) break;
+
+            // learned stop: features of the current state
+            if (b.stop_mode && step > 0 && step % b.stop_every == 0) {
+                float f[RV_MLP_IN];
+                rv_features(ss, keys, n, L, b.stop_label_k, step, next, expand, f);
+                if (b.stop_mode == 2) {
+                    if (trace_n < b.trace_cap) {
+                        if (lane < RV_MLP_IN) {
+                            float v = f[0];
+#pragma unroll
+                            for (int i = 1; i < RV_MLP_IN; i++) v = lane == i ? f[i] : v;
+                            b.trace_feats[((size_t)q * b.trace_cap + trace_n) * RV_MLP_IN + lane] = v;
+                        }
+                        if (lane == 0) b.trace_steps[(size_t)q * b.trace_cap + trace_n] = step;
+                        trace_n++;
+                    }
+                } else if (step >= b.stop_min_steps) {
+                    // the reduction leaves the same logit in every lane
+                    const float logit = rv_mlp_logit(b.mlp, f);
+                    const float p = 1.0f / (1.0f + __expf(-logit));
+                    if (rv_uniform(p >= b.stop_threshold ? 1 : 0)) {
+                        stopped = true;
+                        break;
+                    }
+                }
+            }
+
+            const uint64_t src_key = keys[next];
+            const int src = rv_key_id(src_key);
+            if (lane == 0) flags[next] = 1;
+
+            // neighbor row and its validity (retrieval_sqmd_pure_pq.c:59-87)
+            int nb = -1;
+            if (lane < expand) nb = ix.graph[(size_t)src * ix.degree + lane];
+            const uint64_t pad = __ballot(lane < expand && nb == -1);
+            const int valid_n = pad ? (__ffsll((unsigned long long)pad) - 1) : expand;
+            const uint64_t after = __ballot(lane >= valid_n && lane < expand && nb != -1);
+            const uint64_t oob = __ballot(lane < valid_n && (nb < 0 || nb >= ix.n));
+            if (after | oob) {
+                status = RV_QUERY_BAD_ROW;
+                break;
+            }
+            uint64_t key = RV_KEY_MAX;
+            bool dup = false, nonfinite = false;
+            if (lane < valid_n) {
+                if (!rv_visit(hash, b.hash_log2, b.probe_limit, (uint32_t)nb)) {
+                    dup = true;
+                } else {
+                    float d = 0.0f;
+                    bool ok = true;
+                    if (SCORER == RV_SCORE_ADC) {
+                        d = rv_adc(lut, ix.codes + (size_t)nb * ix.code_bytes, ix.m,
+                                   ix.ks, ix.code_bytes);
+                    } else if (SCORER == RV_SCORE_ADC_ONLINE) {
+                        ok = rv_online(ix, qv, nb, &d);
+                    } else {
+                        d = rv_sq(qsq, ix.sq_codes + (size_t)nb * ix.sq_code_bytes,
+                                  ix.dim, ix.sq_bits, sq_scale_sq);
+                    }
+                    if (!ok || !isfinite(d) || d == FLT_MAX) nonfinite = true;
+                    else key = rv_key(d + 0.0f, nb);
+                }
+            }
+            if (__ballot(nonfinite) != 0ull) {
+                status = RV_QUERY_NONFINITE;
+                break;
+            }
+            st_expanded++;
+            st_scored += __popcll(__ballot(key != RV_KEY_MAX));
+            st_dedup += __popcll(__ballot(dup));
+            step++;
+
+            int first_new, admitted, src_pos;
+            merge(key, next, &first_new, &admitted, &src_pos);
+            ss.admitted_last = admitted;
+            if (first_new < b.stop_label_k) ss.last_change = step;
+            // everything left of the expanded source is expanded, except the
+            // new candidates; resume the scan at the first of either
+            scan = first_new < src_pos + 1 ? first_new : src_pos + 1;
+        }
+        (void)stopped;
+
+        if (b.stop_mode == 2 && lane == 0) {
+            b.trace_meta[2 * (size_t)q] = trace_n;
+            b.trace_meta[2 * (size_t)q + 1] = ss.last_change;
+        }
+
+        // ---- output
+        const uint64_t *keys = keys_buf[cur];
+        if (status != 0) {
+            if (lane == 0) b.out_counts[q] = status;
+        } else if (refine == 0) {
+            const int cnt = K < n ? K : n;
+            for (int i = lane; i < cnt; i += RV_WAVE) {
+                oid[i] = rv_key_id(keys[i]);
+                odist[i] = rv_key_dist(keys[i]);
+            }
+            if (lane == 0) b.out_counts[q] = cnt;
+        } else {
+            // final refine of the first F pool members
+            // (retrieval_final_refine.c:256-292 with the residual decoder)
+            const int F = refine < n ? refine : n;
+            uint64_t *dbits = keys_buf[cur ^ 1];
+            bool bad = false;
+            for (int i = lane; i < F; i += RV_WAVE) {
+                const int id = rv_key_id(keys[i]);
+                const uint8_t *pq = ix.codes + (size_t)id * ix.code_bytes;
+                const uint8_t *res = ix.res_codes + (size_t)id * ix.res_code_bytes;
+                const float s = ix.res_qparams[2 * (size_t)id];
+                const float z = ix.res_qparams[2 * (size_t)id + 1];
+                double acc = 0.0;
+                if (!isfinite(s) || !isfinite(z) || !(s > 0.0f)) bad = true;
+                int d = 0;
+                for (int sub = 0; sub < ix.m; sub++) {
+                    const float *cent = ix.codebook +
+                        ((size_t)sub * ix.ks + rv_code_get(pq, sub, ix.ks)) * ix.subdim;
+                    for (int t = 0; t < ix.subdim; t++, d++) {
+                        float v = rv_decode_value(cent[t], rv_res_code(res, ix.res_bits, d), s, z);
+                        if (!isfinite(v)) bad = true;
+                        double delta = (double)qv[d] - (double)v;
+                        acc += delta * delta;
+                    }
+                }
+                if (!isfinite(acc) || acc >= (double)FLT_MAX) bad = true;
+                dbits[i] = (uint64_t)__double_as_longlong(acc);
+            }
+            if (__ballot(bad) != 0ull) {
+                if (lane == 0) b.out_counts[q] = RV_QUERY_NONFINITE;
+            } else {
+                __syncthreads();
+                for (int i = lane; i < F; i += RV_WAVE) {
+                    const uint64_t di = dbits[i];
+                    const int gi = rv_key_id(keys[i]);
+                    const int oi = ix.graph_to_original ? ix.graph_to_original[gi] : gi;
+                    int rank = 0;
+                    for (int j = 0; j < F; j++) {
+                        const uint64_t dj = dbits[j];
+                        const int gj = rv_key_id(keys[j]);
+                        const int oj = ix.graph_to_original ? ix.graph_to_original[gj] : gj;
+                        rank += (dj < di) || (dj == di && oj < oi);
+                    }
+                    if (rank < K) {
+                        oid[rank] = gi;
+                        odist[rank] = (float)__longlong_as_double((long long)di);
+                    }
+                }
+                if (lane == 0) b.out_counts[q] = K < F ? K : F;
+            }
+        }
+        if (b.out_stats && lane == 0) {
+            uint64_t *o = b.out_stats + 4 * (size_t)q;
+            o[0] = st_scored;
+            o[1] = SCORER == RV_SCORE_ADC ? 0 : st_scored;
+            o[2] = st_expanded;
+            o[3] = st_dedup;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void rv_zero_kernel(uint32_t *p, size_t words) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < words;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+__global__ void rv_zero_bytes_kernel(uint8_t *p, size_t bytes) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < bytes;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+// -------------------------------------------------------------- encoders
+
+// retrieval_pq_core.c:226-268 (retrieval_pq_encode_batch): argmin over k of
+// the sequential sub-L2, first minimum wins.  One thread per (vector, m).
+__global__ void rv_pq_encode_kernel(const float *x, int n, int dim, int m,
+                                    int ks, const float *codebook,
+                                    uint8_t *codes, int code_bytes) {
+    const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (t >= (size_t)n * m) return;
+    const int v = (int)(t / m), s = (int)(t % m);
+    const int sub = dim / m;
+    const float *xv = x + (size_t)v * dim + (size_t)s * sub;
+    const float *cb = codebook + (size_t)s * ks * sub;
+    float best = FLT_MAX;
+    int best_k = 0;
+    for (int k = 0; k < ks; k++) {
+        float acc = 0.0f;
+        for (int d = 0; d < sub; d++) {
+            float diff = xv[d] - cb[(size_t)k * sub + d];
+            acc += diff * diff;
+        }
+        if (acc < best) {
+            best = acc;
+            best_k = k;
+        }
+    }
+    if (ks <= 16) {
+        // two subspaces share a byte: the even thread writes both nibbles
+        if ((s & 1) == 0) {
+            int hi = 0;
+            if (s + 1 < m) {
+                const float *xv2 = xv + sub;
+                const float *cb2 = cb + (size_t)ks * sub;
+                float best2 = FLT_MAX;
+                for (int k = 0; k < ks; k++) {
+                    float acc = 0.0f;
+                    for (int d = 0; d < sub; d++) {
+                        float diff = xv2[d] - cb2[(size_t)k * sub + d];
+                        acc += diff * diff;
+                    }
+                    if (acc < best2) {
+                        best2 = acc;
+                        hi = k;
+                    }
+                }
+            }
+            codes[(size_t)v * code_bytes + (s >> 1)] = (uint8_t)(best_k | (hi << 4));
+        }
+    } else {
+        codes[(size_t)v * code_bytes + s] = (uint8_t)best_k;
+    }
+}
+
+// retrieval_pq_residual.c:314-385 (pq_residual_quantize_vector).  One
+// wavefront per vector: lanes reduce min/max, then quantize.
+__global__ __launch_bounds__(RV_WAVE) void rv_residual_encode_kernel(
+    const float *x, const uint8_t *codes, int n, int dim, int m, int ks,
+    const float *codebook, int code_bytes, int bits, uint8_t *res,
+    int res_bytes, float *qparams) {
+    const int v = blockIdx.x;
+    const int lane = rv_lane();
+    if (v >= n) return;
+    const int sub = dim / m;
+    const uint8_t *code = codes + (size_t)v * code_bytes;
+    const float *xv = x + (size_t)v * dim;
+    float lo = FLT_MAX, hi = -FLT_MAX;
+    bool bad = false;
+    for (int d = lane; d < dim; d += RV_WAVE) {
+        const int s = d / sub;
+        float c = codebook[((size_t)s * ks + rv_code_get(code, s, ks)) * sub + (d % sub)];
+        float r = xv[d] - c;
+        if (!isfinite(r)) bad = true;
+        lo = r < lo ? r : lo;
+        hi = r > hi ? r : hi;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        float a = __shfl_xor(lo, off), b2 = __shfl_xor(hi, off);
+        lo = a < lo ? a : lo;
+        hi = b2 > hi ? b2 : hi;
+    }
+    const int levels = bits == 4 ? 15 : 255;
+    float scale = (hi - lo) / (float)levels;
+    if (scale < 1.0e-12f) scale = 1.0e-12f;
+    const float inv = 1.0f / scale;
+    uint8_t *out = res + (size_t)v * res_bytes;
+    if (bits == 4) {
+        for (int byte = lane; byte < res_bytes; byte += RV_WAVE) {
+            int packed = 0;
+            for (int h = 0; h < 2; h++) {
+                const int d = 2 * byte + h;
+                if (d >= dim) break;
+                const int s = d / sub;
+                float c = codebook[((size_t)s * ks + rv_code_get(code, s, ks)) * sub + (d % sub)];
+                float r = xv[d] - c;
+                int qv = (int)roundf((r - lo) * inv);
+                qv = qv < 0 ? 0 : (qv > levels ? levels : qv);
+                packed |= qv << (4 * h);
+            }
+            out[byte] = (uint8_t)packed;
+        }
+    } else {
+        for (int d = lane; d < dim; d += RV_WAVE) {
+            const int s = d / sub;
+            float c = codebook[((size_t)s * ks + rv_code_get(code, s, ks)) * sub + (d % sub)];
+            float r = xv[d] - c;
+            int qv = (int)roundf((r - lo) * inv);
+            qv = qv < 0 ? 0 : (qv > levels ? levels : qv);
+            out[d] = (uint8_t)qv;
+        }
+    }
+    if (lane == 0) {
+        qparams[2 * (size_t)v] = bad ? NAN : scale;
+        qparams[2 * (size_t)v + 1] = lo;
+    }
+}
+
+// ------------------------------------------------------------ host side
+
+extern "C" int rv_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+extern "C" int rv_set_device(int device) {
+    return rv_check(hipSetDevice(device), "hipSetDevice");
+}
+extern "C" int rv_malloc(void **p, size_t bytes) {
+    return rv_check(hipMalloc(p, bytes ? bytes : 16), "hipMalloc");
+}
+extern "C" int rv_free(void *p) { return rv_check(hipFree(p), "hipFree"); }
+extern "C" int rv_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+    return rv_check(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+}
+extern "C" int rv_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+    return rv_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+}
+extern "C" int rv_memcpy_d2d(void *dst, const void *src, size_t bytes) {
+    return rv_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice), "hipMemcpy D2D");
+}
+extern "C" int rv_synchronize(void) {
+    return rv_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+}
+
+extern "C" int rv_fill_zero(void *p, size_t bytes, void *stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (bytes % 4 == 0 && ((uintptr_t)p & 3) == 0) {
+        size_t words = bytes / 4;
+        int grid = (int)((words + 255) / 256);
+        if (grid > 4096) grid = 4096;
+        if (grid < 1) grid = 1;
+        rv_zero_kernel<<<grid, 256, 0, s>>>((uint32_t *)p, words);
+    } else {
+        int grid = (int)((bytes + 255) / 256);
+        if (grid > 4096) grid = 4096;
+        if (grid < 1) grid = 1;
+        rv_zero_bytes_kernel<<<grid, 256, 0, s>>>((uint8_t *)p, bytes);
+    }
+    return rv_check(hipGetLastError(), "zero kernel");
+}
+
+static RvLayout rv_layout_for(const RvIndex *ix, const RvBatch *b) {
+    return rv_layout(ix->dim, b->scorer, ix->m, ix->ks, b->max_pool, b->hash_log2);
+}
+
+extern "C" size_t rv_search_lds_bytes(const RvIndex *ix, const RvBatch *b) {
+    return (size_t)rv_layout_for(ix, b).total;
+}
+
+template <int S>
+static int rv_occupancy(int lds) {
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rv_search_kernel<S>,
+                                                     RV_WAVE, lds) != hipSuccess)
+        return 0;
+    return blocks;
+}
+
+extern "C" int rv_search_slots_per_cu(const RvIndex *ix, const RvBatch *b) {
+    const int lds = rv_layout_for(ix, b).total;
+    switch (b->scorer) {
+    case RV_SCORE_ADC: return rv_occupancy<RV_SCORE_ADC>(lds);
+    case RV_SCORE_ADC_ONLINE: return rv_occupancy<RV_SCORE_ADC_ONLINE>(lds);
+    case RV_SCORE_SQ: return rv_occupancy<RV_SCORE_SQ>(lds);
+    }
+    return 0;
+}
+
+template <int S>
+static int rv_launch_one(const RvIndex *ix, const RvBatch *b, int grid,
+                         RvLayout lay, hipStream_t s) {
+    static int configured = 0;
+    if (!configured) {
+        hipFuncSetAttribute((const void *)rv_search_kernel<S>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        configured = 1;
+    }
+    rv_search_kernel<S><<<grid, RV_WAVE, lay.total, s>>>(*ix, *b, lay);
+    return rv_check(hipGetLastError(), "rv_search_kernel launch");
+}
+
+extern "C" int rv_search_launch(const RvIndex *ix, const RvBatch *b,
+                                int grid_waves, void *stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (!ix || !b || !b->queue || b->nq < 0 || b->hash_log2 < 4 ||
+        b->hash_log2 > 16 || b->max_pool <= 0 || b->probe_limit <= 0 ||
+        b->n_entries <= 0 || b->expand_count <= 0 || ix->degree > RV_WAVE ||
+        ix->dim <= 0 || ix->dim > 4096 ||
+        (b->stop_mode && (b->stop_every <= 0 || b->stop_label_k <= 0))) {
+        snprintf(g_rv_error, sizeof(g_rv_error), "rv_search_launch: bad parameters");
+        return -1;
+    }
+    if (b->nq == 0) return 0;
+    const RvLayout lay = rv_layout_for(ix, b);
+    if (lay.total > 160 * 1024) {
+        snprintf(g_rv_error, sizeof(g_rv_error),
+                 "rv_search_launch: %d bytes of LDS per query exceed 160 KiB",
+                 lay.total);
+        return -1;
+    }
+    if (grid_waves <= 0) {
+        int dev = 0, cus = 0;
+        hipGetDevice(&dev);
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        int per = rv_search_slots_per_cu(ix, b);
+        if (per < 1) per = 1;
+        grid_waves = cus * per;
+    }
+    if (grid_waves > b->nq) grid_waves = b->nq;
+    if (rv_fill_zero(b->queue, sizeof(int32_t), stream)) return -1;
+    switch (b->scorer) {
+    case RV_SCORE_ADC: return rv_launch_one<RV_SCORE_ADC>(ix, b, grid_waves, lay, s);
+    case RV_SCORE_ADC_ONLINE:
+        return rv_launch_one<RV_SCORE_ADC_ONLINE>(ix, b, grid_waves, lay, s);
+    case RV_SCORE_SQ: return rv_launch_one<RV_SCORE_SQ>(ix, b, grid_waves, lay, s);
+    }
+    snprintf(g_rv_error, sizeof(g_rv_error), "rv_search_launch: unknown scorer");
+    return -1;
+}
+
+extern "C" int rv_pq_encode(const float *x, int n, int dim, int m, int ks,
+                            const float *codebook, uint8_t *codes, void *stream) {
+    if (n <= 0 || m <= 0 || dim % m || (ks != 16 && ks != 256)) return -1;
+    const int code_bytes = ks <= 16 ? m / 2 + m % 2 : m;
+    const size_t threads = (size_t)n * m;
+    rv_pq_encode_kernel<<<(unsigned)((threads + 255) / 256), 256, 0,
+                          (hipStream_t)stream>>>(x, n, dim, m, ks, codebook, codes,
+                                                 code_bytes);
+    return rv_check(hipGetLastError(), "rv_pq_encode");
+}
+
+extern "C" int rv_residual_encode(const float *x, const uint8_t *codes, int n,
+                                  int dim, int m, int ks, const float *codebook,
+                                  int bits, uint8_t *res, float *qparams,
+                                  void *stream) {
+    if (n <= 0 || m <= 0 || dim % m || (bits != 4 && bits != 8)) return -1;
+    const int code_bytes = ks <= 16 ? m / 2 + m % 2 : m;
+    const int res_bytes = bits == 4 ? dim / 2 + dim % 2 : dim;
+    rv_residual_encode_kernel<<<n, RV_WAVE, 0, (hipStream_t)stream>>>(
+        x, codes, n, dim, m, ks, codebook, code_bytes, bits, res, res_bytes, qparams);
+    return rv_check(hipGetLastError(), "rv_residual_encode");
+}
+
+// ------------------------------------------------------ events and streams
+
+extern "C" int rv_event_create(void **ev) {
+    hipEvent_t e;
+    if (rv_check(hipEventCreate(&e), "hipEventCreate")) return -1;
+    *ev = (void *)e;
+    return 0;
+}
+extern "C" int rv_event_record(void *ev, void *stream) {
+    return rv_check(hipEventRecord((hipEvent_t)ev, (hipStream_t)stream), "hipEventRecord");
+}
+extern "C" int rv_event_elapsed(void *start, void *stop, float *ms) {
+    if (rv_check(hipEventSynchronize((hipEvent_t)stop), "hipEventSynchronize")) return -1;
+    return rv_check(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop),
+                    "hipEventElapsedTime");
+}
+extern "C" int rv_event_destroy(void *ev) {
+    return rv_check(hipEventDestroy((hipEvent_t)ev), "hipEventDestroy");
+}
+// Streams are created at the default priority.
+extern "C" int rv_stream_create(void **stream) {
+    hipStream_t s;
+    if (rv_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate"))
+        return -1;
+    *stream = (void *)s;
+    return 0;
+}
+extern "C" int rv_stream_destroy(void *stream) {
+    return rv_check(hipStreamDestroy((hipStream_t)stream), "hipStreamDestroy");
+}
+extern "C" int rv_stream_synchronize(void *stream) {
+    return rv_check(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+}

@@ -1,0 +1,373 @@
+"""Index building: PQ codebooks and codes, 4/8-bit residuals, global SQ
+codes, and the degree-R RNG-pruned graph with a medoid entry point.
+
+Encoders are bit-identical to the reference's (checked against the compiled
+reference by the tests):
+  * PQ encode: argmin over k of the sequential f32 sub-L2, first minimum
+    wins (retrieval_pq_core.c:226-268);
+  * residual: per-vector min/max of (x - centroid), scale=(max-min)/levels,
+    code=clamp(roundf((r-min)/scale)) (retrieval_pq_residual.c:314-385);
+  * SQ: one global scale/zero over all values
+    (retrieval_rpc_demo_support.c:573-700).
+Codebook training follows the reference trainer's shape (uniform-stride
+initial centroids, Lloyd iterations; retrieval_pq_core.c:96-191) but its
+floating-point summation order is not the reference's, so codebooks are
+not bit-identical; search parity is unaffected because the GPU and the
+oracle read the same artifact.
+
+Graph construction follows retrieval_search_build_core.c: exact 2R nearest
+neighbours sorted by (distance, id) (:314-361), RNG occlusion pruning
+`d(kept, c) < d(u, c)` (:904-930), fill from the remaining candidates in
+order (:932-947), rows sorted by (distance, id).
+"""
+import numpy as np
+
+
+# ------------------------------------------------------------------ PQ
+
+def _sub_l2_f32(x, c):
+    """Sequential f32 sum over the last axis of (x - c)^2, x [..., s],
+    c [..., s] broadcastable (retrieval_pq_core.c:16-25)."""
+    diff = (x - c).astype(np.float32)
+    acc = np.zeros(diff.shape[:-1], dtype=np.float32)
+    for d in range(diff.shape[-1]):
+        acc = (acc + diff[..., d] * diff[..., d]).astype(np.float32)
+    return acc
+
+
+def uniform_stride_sample(n, count):
+    """Row indices k*(n-1)/(count-1) (the reference's uniform-stride
+    strategy, retrieval_pq_core.c:140-149 applied to training rows)."""
+    if count >= n:
+        return np.arange(n)
+    return (np.arange(count, dtype=np.int64) * (n - 1)) // max(count - 1, 1)
+
+
+def train_pq(x, m, ks=256, iters=20, train_max=100000, device=None):
+    """k-means codebook float32 [m, ks, dim/m]."""
+    n, dim = x.shape
+    sub = dim // m
+    tr = x[uniform_stride_sample(n, min(n, train_max))]
+    cnt = tr.shape[0]
+    init = (np.arange(ks, dtype=np.int64) * (cnt - 1)) // max(ks - 1, 1)
+    if device is not None:
+        import torch
+        t = torch.from_numpy(np.ascontiguousarray(tr)).to(device)
+        out = np.empty((m, ks, sub), dtype=np.float32)
+        for s in range(m):
+            xs = t[:, s * sub:(s + 1) * sub].contiguous()
+            cent = xs[torch.from_numpy(init).to(device)].clone()
+            for _ in range(iters):
+                d = (xs * xs).sum(1, keepdim=True) - 2.0 * xs @ cent.T + \
+                    (cent * cent).sum(1)[None, :]
+                a = torch.argmin(d, dim=1)
+                sums = torch.zeros_like(cent).index_add_(0, a, xs)
+                counts = torch.bincount(a, minlength=ks).to(xs.dtype)
+                nz = counts > 0
+                cent[nz] = sums[nz] / counts[nz, None]
+            out[s] = cent.cpu().numpy()
+        return out
+    out = np.empty((m, ks, sub), dtype=np.float32)
+    for s in range(m):
+        xs = np.ascontiguousarray(tr[:, s * sub:(s + 1) * sub])
+        cent = xs[init].copy()
+        for _ in range(iters):
+            d = (xs * xs).sum(1)[:, None] - 2.0 * xs @ cent.T + \
+                (cent * cent).sum(1)[None, :]
+            a = np.argmin(d, axis=1)
+            sums = np.zeros_like(cent, dtype=np.float64)
+            np.add.at(sums, a, xs)
+            counts = np.bincount(a, minlength=ks)
+            nz = counts > 0
+            cent[nz] = (sums[nz] / counts[nz, None]).astype(np.float32)
+        out[s] = cent
+    return out
+
+
+def code_bytes(m, ks):
+    return m // 2 + m % 2 if ks <= 16 else m
+
+
+def encode_pq(x, codebook, chunk=4096):
+    """uint8 [n, code_bytes]; 4-bit codes put even subspaces in the low
+    nibble (retrieval_pq_core.c:54-85)."""
+    m, ks, sub = codebook.shape
+    n = x.shape[0]
+    idx = np.empty((n, m), dtype=np.uint8)
+    for s0 in range(0, n, chunk):
+        xs = x[s0:s0 + chunk]
+        for s in range(m):
+            d = _sub_l2_f32(xs[:, None, s * sub:(s + 1) * sub], codebook[s][None])
+            idx[s0:s0 + chunk, s] = np.argmin(d, axis=1)
+    if ks > 16:
+        return idx
+    out = np.zeros((n, code_bytes(m, ks)), dtype=np.uint8)
+    for s in range(m):
+        if s & 1:
+            out[:, s >> 1] |= (idx[:, s] << 4).astype(np.uint8)
+        else:
+            out[:, s >> 1] |= idx[:, s]
+    return out
+
+
+def pq_indices(codes, m, ks):
+    if ks > 16:
+        return codes[:, :m].astype(np.int64)
+    lo = codes & 0x0F
+    hi = codes >> 4
+    out = np.empty((codes.shape[0], m), dtype=np.int64)
+    out[:, 0::2] = lo[:, : (m + 1) // 2]
+    out[:, 1::2] = hi[:, : m // 2]
+    return out
+
+
+def reconstruct_pq(codes, codebook):
+    m, ks, sub = codebook.shape
+    idx = pq_indices(codes, m, ks)
+    parts = [codebook[s][idx[:, s]] for s in range(m)]
+    return np.concatenate(parts, axis=1).astype(np.float32)
+
+
+def _roundf_nonneg_or_clamp(v, levels):
+    """clamp(roundf(v), 0, levels) for float32 v; roundf rounds half away
+    from zero, and anything below zero clamps to 0."""
+    r = np.floor(v.astype(np.float64) + 0.5)
+    return np.clip(r, 0, levels).astype(np.int64)
+
+
+def residual_code_bytes(dim, bits):
+    return dim // 2 + dim % 2 if bits == 4 else dim
+
+
+def encode_residual(x, codes, codebook, bits=4):
+    """(codes uint8 [n, res_code_bytes], qparams float32 [n, 2])."""
+    levels = 15 if bits == 4 else 255
+    recon = reconstruct_pq(codes, codebook)
+    r = (x.astype(np.float32) - recon).astype(np.float32)
+    lo = r.min(axis=1)
+    hi = r.max(axis=1)
+    scale = ((hi - lo) / np.float32(levels)).astype(np.float32)
+    scale = np.where(scale < np.float32(1e-12), np.float32(1e-12), scale).astype(np.float32)
+    inv = (np.float32(1.0) / scale).astype(np.float32)
+    v = ((r - lo[:, None]).astype(np.float32) * inv[:, None]).astype(np.float32)
+    q = _roundf_nonneg_or_clamp(v, levels).astype(np.uint8)
+    n, dim = x.shape
+    if bits == 8:
+        out = q
+    else:
+        out = np.zeros((n, residual_code_bytes(dim, 4)), dtype=np.uint8)
+        out[:, : (dim + 1) // 2] |= q[:, 0::2]
+        out[:, : dim // 2] |= (q[:, 1::2] << 4).astype(np.uint8)
+    qp = np.stack([scale, lo], axis=1).astype(np.float32)
+    return np.ascontiguousarray(out), np.ascontiguousarray(qp)
+
+
+def sq_params(x, bits):
+    """Global (scale, zero) over all values (demo_set_qparams_from_range)."""
+    lo = np.float32(x.min())
+    hi = np.float32(x.max())
+    rng = np.float32(hi - lo)
+    if rng < np.float32(1e-12):
+        rng = np.float32(1e-12)
+    return np.float32(rng / np.float32((1 << bits) - 1)), lo
+
+
+def encode_sq(x, bits, scale, zero):
+    levels = (1 << bits) - 1
+    inv = np.float32(np.float32(1.0) / np.float32(scale))
+    v = ((x.astype(np.float32) - np.float32(zero)).astype(np.float32) * inv).astype(np.float32)
+    q = _roundf_nonneg_or_clamp(v, levels).astype(np.uint8)
+    if bits == 8:
+        return np.ascontiguousarray(q)
+    n, dim = x.shape
+    out = np.zeros((n, (dim + 1) // 2), dtype=np.uint8)
+    out[:, : (dim + 1) // 2] |= q[:, 0::2]
+    out[:, : dim // 2] |= (q[:, 1::2] << 4).astype(np.uint8)
+    return out
+
+
+# ---------------------------------------------------------------- graph
+
+def _prune_rows(du, dpair, R):
+    """RNG occlusion over candidates sorted by (distance, id).
+    du [B, C] distance u->c, dpair [B, C, C] distance c_i->c_j.
+    Returns bool keep mask [B, C] with exactly min(R, C) True per row."""
+    B, Cn = du.shape
+    keep = np.zeros((B, Cn), dtype=bool)
+    count = np.zeros(B, dtype=np.int64)
+    for i in range(Cn):
+        occl = (keep[:, :i] & (dpair[:, :i, i] < du[:, i:i + 1])).any(axis=1)
+        acc = (~occl) & (count < R)
+        keep[:, i] = acc
+        count += acc
+    # fill from the remaining candidates in order
+    need = R - count
+    rest = ~keep
+    rank = np.cumsum(rest, axis=1)
+    keep |= rest & (rank <= need[:, None])
+    return keep
+
+
+def build_graph(x, R, candidates=None, device=None, batch=None):
+    """int32 [n, R] neighbour rows (distance, id order), -1 padded when n-1 < R."""
+    n, dim = x.shape
+    Cn = min(n - 1, candidates or 2 * R)
+    if device is not None:
+        return _build_graph_torch(x, R, Cn, device, batch or 8192)
+    graph = np.full((n, R), -1, dtype=np.int32)
+    xf = x.astype(np.float32)
+    norms = (xf * xf).sum(1)
+    step = batch or 256
+    for s in range(0, n, step):
+        e = min(n, s + step)
+        d = norms[s:e, None] - 2.0 * xf[s:e] @ xf.T + norms[None, :]
+        d[np.arange(e - s), np.arange(s, e)] = np.inf
+        kk = min(n - 1, Cn + 8)
+        cand = np.argpartition(d, kk - 1, axis=1)[:, :kk]
+        # exact f32 distances, sorted by (distance, id)
+        diff = xf[cand] - xf[s:e, None, :]
+        du = np.einsum("bcd,bcd->bc", diff, diff).astype(np.float32)
+        order = np.lexsort((cand, du), axis=1)
+        cand = np.take_along_axis(cand, order, 1)[:, :Cn]
+        du = np.take_along_axis(du, order, 1)[:, :Cn]
+        pts = xf[cand]
+        g = np.einsum("bid,bjd->bij", pts, pts)
+        sq = np.einsum("bid,bid->bi", pts, pts)
+        dpair = sq[:, :, None] + sq[:, None, :] - 2.0 * g
+        keep = _prune_rows(du, dpair, R)
+        for i in range(e - s):
+            row = cand[i][keep[i]][:R]
+            graph[s + i, :len(row)] = row
+    return graph
+
+
+def _build_graph_torch(x, R, Cn, device, batch):
+    import torch
+    n, dim = x.shape
+    xt = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).to(device)
+    norms = (xt * xt).sum(1)
+    graph = torch.full((n, R), -1, dtype=torch.int32, device=device)
+    kk = min(n - 1, Cn + 8)
+    for s in range(0, n, batch):
+        e = min(n, s + batch)
+        xb = xt[s:e]
+        d = norms[s:e, None] - 2.0 * (xb @ xt.T) + norms[None, :]
+        d[torch.arange(e - s, device=device), torch.arange(s, e, device=device)] = float("inf")
+        cand = torch.topk(d, kk, dim=1, largest=False).indices
+        del d
+        diff = xt[cand] - xb[:, None, :]
+        du = (diff * diff).sum(-1)
+        # sort by (distance, id): stable sort by id, then by distance
+        o1 = torch.argsort(cand, dim=1, stable=True)
+        cand = torch.gather(cand, 1, o1)
+        du = torch.gather(du, 1, o1)
+        o2 = torch.argsort(du, dim=1, stable=True)
+        cand = torch.gather(cand, 1, o2)[:, :Cn].contiguous()
+        du = torch.gather(du, 1, o2)[:, :Cn].contiguous()
+        pts = xt[cand].contiguous()
+        sq = (pts * pts).sum(-1)
+        g = torch.bmm(pts, pts.transpose(1, 2).contiguous())
+        dpair = sq[:, :, None] + sq[:, None, :] - 2.0 * g
+        B = e - s
+        keep = torch.zeros((B, Cn), dtype=torch.bool, device=device)
+        count = torch.zeros(B, dtype=torch.int64, device=device)
+        for i in range(Cn):
+            if i:
+                occl = (keep[:, :i] & (dpair[:, :i, i] < du[:, i:i + 1])).any(dim=1)
+            else:
+                occl = torch.zeros(B, dtype=torch.bool, device=device)
+            acc = (~occl) & (count < R)
+            keep[:, i] = acc
+            count += acc.to(torch.int64)
+        need = R - count
+        rest = ~keep
+        rank = torch.cumsum(rest.to(torch.int64), dim=1)
+        keep |= rest & (rank <= need[:, None])
+        # compact kept candidates, preserving order
+        pos = torch.cumsum(keep.to(torch.int64), dim=1) - 1
+        rows = torch.full((B, R), -1, dtype=torch.int32, device=device)
+        sel = keep & (pos < R)
+        bi = torch.arange(B, device=device)[:, None].expand(B, Cn)
+        rows[bi[sel], pos[sel]] = cand[sel].to(torch.int32)
+        graph[s:e] = rows
+    return graph.cpu().numpy()
+
+
+def medoid(x):
+    """Entry point: the base vector nearest the centroid (the standard
+    stand-in for the medoid, which is quadratic to compute exactly)."""
+    c = x.astype(np.float64).mean(0)
+    d = ((x.astype(np.float64) - c) ** 2).sum(1)
+    return int(np.argmin(d))
+
+
+class PqIndexData:
+    """Arrays of a pure-PQ index (all host numpy)."""
+
+    def __init__(self, graph, entry, codes, codebook, res_codes=None,
+                 res_qparams=None, res_bits=4):
+        self.graph = np.ascontiguousarray(graph, dtype=np.int32)
+        self.entry = int(entry)
+        self.codes = np.ascontiguousarray(codes, dtype=np.uint8)
+        self.codebook = np.ascontiguousarray(codebook, dtype=np.float32)
+        self.res_codes = None if res_codes is None else np.ascontiguousarray(res_codes)
+        self.res_qparams = None if res_qparams is None else np.ascontiguousarray(res_qparams)
+        self.res_bits = res_bits
+
+    @property
+    def n(self):
+        return self.graph.shape[0]
+
+    @property
+    def degree(self):
+        return self.graph.shape[1]
+
+    @property
+    def m(self):
+        return self.codebook.shape[0]
+
+    @property
+    def ks(self):
+        return self.codebook.shape[1]
+
+    @property
+    def dim(self):
+        return self.codebook.shape[0] * self.codebook.shape[2]
+
+
+def build_pq_index(x, m, R, ks=256, res_bits=4, iters=20, train_max=100000,
+                   device=None, graph=None):
+    codebook = train_pq(x, m, ks, iters=iters, train_max=train_max, device=device)
+    if device is not None and x.shape[0] > 100000:
+        codes, res, qp = encode_on_device(x, codebook, res_bits)
+    else:
+        codes = encode_pq(x, codebook)
+        res, qp = encode_residual(x, codes, codebook, res_bits)
+    if graph is None:
+        graph = build_graph(x, R, device=device)
+    return PqIndexData(graph, medoid(x), codes, codebook, res, qp, res_bits)
+
+
+def encode_on_device(x, codebook, res_bits):
+    """PQ codes and residuals with the library's HIP encoders."""
+    from . import native
+    L = native.lib()
+    m, ks, sub = codebook.shape
+    n, dim = x.shape
+    cb = code_bytes(m, ks)
+    rb = residual_code_bytes(dim, res_bits)
+    dx = native.DeviceBuffer.from_array(np.ascontiguousarray(x, dtype=np.float32))
+    dcb = native.DeviceBuffer.from_array(codebook)
+    dcodes = native.DeviceBuffer(n * cb)
+    dres = native.DeviceBuffer(n * rb)
+    dqp = native.DeviceBuffer(n * 8)
+    native.check(L.rover_build_pq_encode(dx.ptr, n, dim, m, ks, dcb.ptr, dcodes.ptr),
+                 "rover_build_pq_encode")
+    native.check(L.rover_build_residual_encode(dx.ptr, dcodes.ptr, n, dim, m, ks,
+                                               dcb.ptr, res_bits, dres.ptr, dqp.ptr),
+                 "rover_build_residual_encode")
+    native.synchronize()
+    codes = dcodes.download(np.uint8, n * cb).reshape(n, cb)
+    res = dres.download(np.uint8, n * rb).reshape(n, rb)
+    qp = dqp.download(np.float32, n * 2).reshape(n, 2)
+    return codes, res, qp

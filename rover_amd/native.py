@@ -1,0 +1,256 @@
+"""ctypes bindings to librover.so (include/rover_gpu.h).
+
+The library is the product: there is no Python or CPU fallback for search.
+Loading fails loudly when the library has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "librover.so")
+
+ROVER_SUCCESS = 0
+METHOD_PQ = 0
+METHOD_SQ = 1
+ENTRY_POINT_SEED = 42
+ABI_VERSION = 1
+
+
+class RoverError(RuntimeError):
+    pass
+
+
+class PurePqSearchIndexView(C.Structure):
+    _fields_ = [("graph", C.c_void_p), ("graph_degree", C.c_int),
+                ("num_vectors", C.c_int), ("code_chunks", C.c_void_p),
+                ("chunk_vector_counts", C.c_void_p), ("chunk_count", C.c_int),
+                ("chunk_vectors", C.c_int), ("codebook", C.c_void_p),
+                ("dim", C.c_int), ("m", C.c_int), ("ks", C.c_int),
+                ("subdim", C.c_int)]
+
+
+class PurePqSearchRequest(C.Structure):
+    _fields_ = [("query", C.c_void_p), ("entry_ids", C.c_void_p),
+                ("entry_count", C.c_int), ("expand_count", C.c_int),
+                ("pool_capacity", C.c_int), ("num_out", C.c_int)]
+
+
+class PurePqSearchStats(C.Structure):
+    _fields_ = [("pq_scored_candidates", C.c_uint64),
+                ("corrected_candidates", C.c_uint64),
+                ("expanded_nodes", C.c_uint64),
+                ("visited_dedup_skips", C.c_uint64)]
+
+
+class PurePqDistancePolicy(C.Structure):
+    _fields_ = [("mode", C.c_int), ("evaluator", C.c_void_p),
+                ("context", C.c_void_p)]
+
+
+class RoverResidualView(C.Structure):
+    _fields_ = [("codes", C.c_void_p), ("qparams", C.c_void_p),
+                ("bits", C.c_int)]
+
+
+class RoverSearchBatch(C.Structure):
+    _fields_ = [("queries", C.c_void_p), ("num_queries", C.c_int),
+                ("pool_sizes", C.c_void_p), ("num_outs", C.c_void_p),
+                ("pool_size", C.c_int), ("num_out", C.c_int),
+                ("method", C.c_int), ("correction", C.c_int),
+                ("refine_candidates", C.c_int), ("entry_points", C.c_int),
+                ("seed", C.c_uint32), ("expand_count", C.c_int),
+                ("out_ids", C.c_void_p), ("out_dists", C.c_void_p),
+                ("out_counts", C.c_void_p), ("out_stride", C.c_int),
+                ("out_stats", C.c_void_p), ("hash_log2", C.c_int),
+                ("probe_limit", C.c_int), ("grid_waves", C.c_int),
+                ("learned_stop", C.c_int), ("stop_threshold", C.c_float)]
+
+
+# (name, restype, argtypes); every symbol include/rover_gpu.h declares.
+_H = C.c_uint64
+_P = C.c_void_p
+_I = C.c_int
+_F = C.c_float
+SIGNATURES = [
+    ("calculator_open", _I, [C.c_char_p, C.POINTER(_H)]),
+    ("calculator_close", _I, [_H]),
+    ("calculator_rover_configure", _I, [_H, C.c_int32, C.c_int32]),
+    ("calculator_retrieval_load", _I,
+     [_H, _P, _I, _P, _I, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _F, _F]),
+    ("calculator_retrieval_search", _I,
+     [_H, _P, _I, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P, _I, _P, _I,
+      _P]),
+    ("calculator_retrieval_unload", _I, [_H]),
+    ("calculator_retrieval_pure_pq_load_dma", _I,
+     [_H, _P, _I, _P, _I, C.c_int32, _P, _I, C.c_int32, C.c_int32, C.c_int32,
+      C.c_int32, C.c_int32, C.c_int32, _P, _I]),
+    ("calculator_retrieval_pure_pq_unload_dma", _I, [_H]),
+    ("rover_pure_pq_graph_load", _I,
+     [_H, _P, C.c_int64, C.c_int32, C.c_int32, C.c_int32]),
+    ("rover_pure_pq_residual_load", _I,
+     [_H, _P, C.c_int64, _P, C.c_int64, C.c_int32]),
+    ("rover_pure_pq_original_ids", _I, [_H, _P, _I]),
+    ("rover_pq_residual_evaluator", _I, [_P, _I, _F, _P]),
+    ("retrieval_pure_pq_search_scalar_with_policy", _I,
+     [_P, _P, _P, _P, _P, _I, _P]),
+    ("retrieval_pure_pq_search_scalar", _I, [_P, _P, _P, _P, _I, _P]),
+    ("rover_pure_pq_view_release", None, []),
+    ("rover_search_batch", _I, [_H, _P]),
+    ("rover_search_batch_device", _I, [_H, _P, _P]),
+    ("rover_make_seed_ids", _I, [_I, _I, _I, C.c_uint32, _P]),
+    ("rover_build_pq_encode", _I, [_P, _I, _I, _I, _I, _P, _P]),
+    ("rover_build_residual_encode", _I, [_P, _P, _I, _I, _I, _I, _P, _I, _P, _P]),
+    ("rover_device_alloc", _I, [C.POINTER(_P), C.c_size_t]),
+    ("rover_device_free", _I, [_P]),
+    ("rover_device_upload", _I, [_P, _P, C.c_size_t]),
+    ("rover_device_download", _I, [_P, _P, C.c_size_t]),
+    ("rover_device_synchronize", _I, []),
+    ("rover_device_count", _I, []),
+    ("rover_last_error", C.c_char_p, []),
+    ("rover_event_create", _I, [C.POINTER(_P)]),
+    ("rover_event_record", _I, [_P, _P]),
+    ("rover_event_elapsed_ms", _I, [_P, _P, C.POINTER(_F)]),
+    ("rover_event_destroy", _I, [_P]),
+    ("rover_stream_create", _I, [C.POINTER(_P)]),
+    ("rover_stream_destroy", _I, [_P]),
+    ("rover_stream_synchronize", _I, [_P]),
+]
+
+_lib = None
+
+
+def lib():
+    """Loads librover.so once; raises if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RoverError(
+                "librover.so not built (%s): run __graft_entry__.build()"
+                % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            if not hasattr(L, name):
+                continue
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error():
+    msg = lib().rover_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(rc, what):
+    if rc != ROVER_SUCCESS:
+        raise RoverError("%s failed (rc=%d): %s" % (what, rc, last_error()))
+
+
+def ptr(a):
+    """Address of a contiguous numpy array (or None)."""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "array must be C-contiguous"
+    return a.ctypes.data
+
+
+class DeviceBuffer:
+    """HBM allocation owned by Python, freed on close/garbage collection."""
+
+    def __init__(self, nbytes):
+        p = C.c_void_p()
+        check(lib().rover_device_alloc(C.byref(p), max(int(nbytes), 16)),
+              "rover_device_alloc")
+        self.ptr = p.value
+        self.nbytes = int(nbytes)
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a)
+        buf = cls(a.nbytes)
+        if a.nbytes:
+            check(lib().rover_device_upload(buf.ptr, ptr(a), a.nbytes),
+                  "rover_device_upload")
+        return buf
+
+    def download(self, dtype, count):
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes:
+            check(lib().rover_device_download(ptr(out), self.ptr, out.nbytes),
+                  "rover_device_download")
+        return out
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        check(lib().rover_device_upload(self.ptr, ptr(a), a.nbytes),
+              "rover_device_upload")
+
+    def close(self):
+        if self.ptr is not None and _lib is not None:
+            _lib.rover_device_free(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_count():
+    return lib().rover_device_count()
+
+
+def synchronize():
+    check(lib().rover_device_synchronize(), "rover_device_synchronize")
+
+
+def make_seed_ids(n, entry_points, graph_entry, seed=ENTRY_POINT_SEED):
+    out = np.zeros(max(entry_points, 1), dtype=np.int32)
+    cnt = lib().rover_make_seed_ids(n, entry_points, graph_entry, seed, ptr(out))
+    return out[:cnt]
+
+
+class Event:
+    def __init__(self):
+        p = C.c_void_p()
+        check(lib().rover_event_create(C.byref(p)), "rover_event_create")
+        self.ptr = p.value
+
+    def record(self, stream=None):
+        check(lib().rover_event_record(self.ptr, stream), "rover_event_record")
+
+    def elapsed_ms(self, stop):
+        ms = C.c_float()
+        check(lib().rover_event_elapsed_ms(self.ptr, stop.ptr, C.byref(ms)),
+              "rover_event_elapsed_ms")
+        return ms.value
+
+    def __del__(self):
+        try:
+            if self.ptr and _lib is not None:
+                _lib.rover_event_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        p = C.c_void_p()
+        check(lib().rover_stream_create(C.byref(p)), "rover_stream_create")
+        self.ptr = p.value
+
+    def synchronize(self):
+        check(lib().rover_stream_synchronize(self.ptr), "rover_stream_synchronize")
+
+    def __del__(self):
+        try:
+            if self.ptr and _lib is not None:
+                _lib.rover_stream_destroy(self.ptr)
+        except Exception:
+            pass

@@ -1,0 +1,250 @@
+"""Writes the golden fixtures under tests/golden/ from the compiled reference.
+
+Run in a container that has the Rover source tree (oracle/Makefile builds
+oracle/_ref/librover_ref.so from it):
+
+    python tests/golden/make_golden.py
+
+Every expected output in the fixtures comes from the reference's own scalar
+C: retrieval_pure_pq_search_scalar_with_policy, retrieval_pq_build_lut,
+retrieval_pq_adc_ids_batch_scalar, retrieval_pq_train/encode_batch, the
+PQRES01 residual writer/reader and its evaluator/decoder,
+retrieval_final_refine, the SQ quantize/distance kernels and
+retrieval_sqmd_make_seed_ids.  Inputs are small synthetic indexes; base
+vectors are multiples of 1/16 so they are stored exactly as uint16.
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import ctypes as C  # noqa: E402
+
+import pyoracle  # noqa: E402
+from rover_amd import data, index  # noqa: E402
+
+
+def _p(a):
+    return a.ctypes.data
+
+
+def make_base(n, dim, comps, sigma, seed, nq):
+    base, queries = data.workload(n, dim, comps, sigma, nq, base_seed=seed,
+                                  query_seed=seed + 1)
+    base = (np.round(base * 16.0) / 16.0).astype(np.float32)
+    queries = (np.round(queries * 16.0) / 16.0).astype(np.float32)
+    return base, queries
+
+
+def ref_index(R, base, m, ks, res_bits, iters=8):
+    ref = pyoracle.reference()
+    n, dim = base.shape
+    cb = np.zeros((m, ks, dim // m), dtype=np.float32)
+    assert ref.ref_pq_train(_p(base), n, dim, m, ks, iters, _p(cb)) == 0
+    code_bytes = index.code_bytes(m, ks)
+    codes = np.zeros((n, code_bytes), dtype=np.uint8)
+    assert ref.ref_pq_encode(_p(base), n, dim, m, ks, _p(cb), _p(codes)) == 0
+    rb = index.residual_code_bytes(dim, res_bits)
+    res = np.zeros((n, rb), dtype=np.uint8)
+    qp = np.zeros((n, 2), dtype=np.float32)
+    handle = ref.ref_residual_open(_p(base), _p(codes), _p(cb), n, dim, m, ks,
+                                   res_bits, _p(res), _p(qp))
+    assert handle
+    graph = index.build_graph(base, R)
+    return cb, codes, res, qp, graph, handle
+
+
+def ref_search(handle, graph, codes, cb, query, entries, expand, L, num_out,
+               mode, chunk_vectors, sq=None):
+    ref = pyoracle.reference()
+    n, R = graph.shape
+    m, ks, sub = cb.shape
+    ids = np.zeros(num_out, dtype=np.int32)
+    d = np.zeros(num_out, dtype=np.float32)
+    st = np.zeros(4, dtype=np.uint64)
+    sq_codes, sq_bits, sq_scale, sq_zero = sq if sq else (None, 8, 1.0, 0.0)
+    rc = ref.ref_search(_p(graph), R, n, _p(codes), chunk_vectors, _p(cb),
+                        m * sub, m, ks, _p(query), _p(entries), len(entries),
+                        expand, L, num_out, mode, handle,
+                        None if sq_codes is None else _p(sq_codes), sq_bits,
+                        sq_scale, sq_zero, _p(ids), _p(d), num_out, _p(st))
+    return rc, ids, d, st
+
+
+def pq_case(name, n, dim, m, ks, R, res_bits, comps, nq, Ls, expands, seed):
+    ref = pyoracle.reference()
+    base, queries = make_base(n, dim, comps, 8.0, seed, nq)
+    cb, codes, res, qp, graph, handle = ref_index(R, base, m, ks, res_bits)
+    entry = index.medoid(base)
+    out = dict(base_x16=(base * 16).astype(np.uint16),
+               queries_x16=(queries * 16).astype(np.uint16),
+               codebook=cb, codes=codes, res_codes=res, res_qparams=qp,
+               res_bits=np.int32(res_bits), graph=graph, entry=np.int32(entry))
+    # LUT and ADC for every query, ADC of a sample of ids
+    sample = np.arange(0, n, max(1, n // 97), dtype=np.int32)
+    luts, adcs, evals, decs = [], [], [], []
+    for q in queries:
+        lut = np.zeros(m * ks, dtype=np.float32)
+        assert ref.ref_build_lut(_p(q), _p(cb), m, ks, dim // m, _p(lut)) == 0
+        a = np.zeros(len(sample), dtype=np.float32)
+        assert ref.ref_adc_ids(_p(codes), n, codes.shape[1], 300, _p(sample),
+                               len(sample), _p(lut), m, ks, _p(a)) == 0
+        e = np.zeros(len(sample), dtype=np.float32)
+        for i, vid in enumerate(sample):
+            v = np.zeros(1, dtype=np.float32)
+            assert ref.ref_residual_evaluate(handle, _p(q), int(vid), _p(v)) == 0
+            e[i] = v[0]
+        if len(luts) < 2:
+            luts.append(lut)
+        adcs.append(a)
+        evals.append(e)
+    for vid in sample[:16]:
+        v = np.zeros(dim, dtype=np.float32)
+        assert ref.ref_residual_decode(handle, int(vid), _p(v)) == 0
+        decs.append(v)
+    out.update(adc_sample=sample, luts=np.stack(luts), adcs=np.stack(adcs),
+               online_evals=np.stack(evals), decoded=np.stack(decs))
+    # searches: (mode, entry_points, expand, L) -> full pool, stats
+    cases = []
+    results = {}
+    for mode in (0, 1):
+        for ep in (1, 32):
+            entries = pyoracle.seed_ids(n, ep, entry, 42)
+            for expand in expands:
+                for L in Ls:
+                    key = "m%d_e%d_x%d_L%d" % (mode, ep, expand, L)
+                    ids_all, d_all, st_all, cnt_all = [], [], [], []
+                    for q in queries:
+                        rc, ids, d, st = ref_search(handle, graph, codes, cb, q,
+                                                    entries, expand, L, L, mode,
+                                                    300)
+                        assert rc >= 0, (key, rc)
+                        ids_all.append(ids)
+                        d_all.append(d)
+                        st_all.append(st)
+                        cnt_all.append(rc)
+                    results[key + "_ids"] = np.stack(ids_all)
+                    results[key + "_dists"] = np.stack(d_all)
+                    results[key + "_stats"] = np.stack(st_all)
+                    results[key + "_counts"] = np.array(cnt_all, dtype=np.int32)
+                    cases.append((mode, ep, expand, L))
+    out.update(results)
+    out["search_cases"] = np.array(cases, dtype=np.int32)
+    # final refine of the first F pool members -> top k
+    F, K = 20, 10
+    ref_ids, ref_d = [], []
+    for qi, q in enumerate(queries):
+        key = "m0_e1_x%d_L%d" % (expands[0], Ls[-1])
+        cand = np.ascontiguousarray(results[key + "_ids"][qi][:F])
+        ids = cand.copy()
+        d = np.zeros(F, dtype=np.float32)
+        rc = ref.ref_refine(handle, n, dim, _p(q), _p(ids), _p(d), F, K, None)
+        assert rc == K, rc
+        ref_ids.append(ids[:K].copy())
+        ref_d.append(d[:K].copy())
+    out.update(refine_F=np.int32(F), refine_K=np.int32(K),
+               refine_source=np.array([0, 1, expands[0], Ls[-1]], dtype=np.int32),
+               refine_ids=np.stack(ref_ids), refine_dists=np.stack(ref_d))
+    # error cases: -1 in the middle of a row; out-of-range entry
+    bad = graph.copy()
+    bad[entry, 3] = -1
+    rc_bad_row, _, _, _ = ref_search(handle, bad, codes, cb, queries[0],
+                                     np.array([entry], dtype=np.int32), R, 16, 16,
+                                     0, 300)
+    rc_bad_entry, _, _, _ = ref_search(handle, graph, codes, cb, queries[0],
+                                       np.array([entry, n], dtype=np.int32), R,
+                                       16, 16, 0, 300)
+    out.update(rc_bad_row=np.int32(rc_bad_row), rc_bad_entry=np.int32(rc_bad_entry))
+    ref.ref_residual_close(handle)
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path), "bytes")
+    return base, queries, graph, entry
+
+
+def sq_case(name, base, queries, graph, entry):
+    """SQ distances through the reference traversal's policy hook."""
+    ref = pyoracle.reference()
+    n, dim = base.shape
+    out = {}
+    dummy_cb = np.zeros((16, 256, dim // 16), dtype=np.float32)
+    dummy_codes = np.zeros((n, 16), dtype=np.uint8)
+    for bits in (8, 4):
+        scale, zero = index.sq_params(base, bits)
+        codes = index.encode_sq(base, bits, scale, zero)
+        out["sq%d_scale" % bits] = np.float32(scale)
+        out["sq%d_zero" % bits] = np.float32(zero)
+        out["sq%d_codes" % bits] = codes
+        # quantized queries and distances for a sample of ids
+        qq_all, dist_all = [], []
+        sample = np.arange(0, n, max(1, n // 61))
+        for q in queries:
+            qq = np.zeros(dim, dtype=np.uint8)
+            assert ref.ref_quantize_query(_p(q), dim, bits, scale, zero, _p(qq)) == 0
+            qq_all.append(qq)
+            dist_all.append(np.array(
+                [ref.ref_sq_distance(_p(qq), _p(np.ascontiguousarray(codes[i])), dim,
+                                     bits, scale) for i in sample], dtype=np.float32))
+        out["sq%d_qq" % bits] = np.stack(qq_all)
+        out["sq%d_sample" % bits] = sample.astype(np.int32)
+        out["sq%d_sample_dists" % bits] = np.stack(dist_all)
+        for ep in (1, 32):
+            entries = pyoracle.seed_ids(n, ep, entry, 42)
+            for L in (16, 64):
+                key = "sq%d_e%d_L%d" % (bits, ep, L)
+                ids_all, d_all, st_all = [], [], []
+                for q in queries:
+                    rc, ids, d, st = ref_search(None, graph, dummy_codes, dummy_cb, q,
+                                                entries, graph.shape[1], L, L, 2, n,
+                                                sq=(codes, bits, scale, zero))
+                    assert rc == L, (key, rc)
+                    ids_all.append(ids)
+                    d_all.append(d)
+                    st_all.append(st)
+                out[key + "_ids"] = np.stack(ids_all)
+                out[key + "_dists"] = np.stack(d_all)
+                out[key + "_stats"] = np.stack(st_all)
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
+def seeds_case(name):
+    """retrieval_sqmd_make_seed_ids with n = 65536: its id (s/65536)%n is the
+    LCG state's high half, which is also what the pure-PQ mapping
+    (s*n)>>32 gives for this n."""
+    ref = pyoracle.reference()
+    out = {}
+    for seed in (42, 1, 12345):
+        ids = np.zeros(64, dtype=np.int32)
+        cnt = ref.ref_sqmd_seed_ids(65536, 64, -1, seed, _p(ids), 64)
+        out["sqmd_seed%d" % seed] = ids[:cnt].copy()
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
+def main():
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "all"],
+                   check=True)
+    if not pyoracle.ref_available():
+        sys.exit("reference build missing: oracle/_ref/librover_ref.so")
+    base, queries, graph, entry = pq_case(
+        "pq_d128_m32", n=1500, dim=128, m=32, ks=256, R=32, res_bits=4,
+        comps=48, nq=12, Ls=(16, 64, 160), expands=(32, 12), seed=42)
+    sq_case("sq_d128", base, queries, graph, entry)
+    pq_case("pq_d96_m48", n=1000, dim=96, m=48, ks=256, R=48, res_bits=4,
+            comps=32, nq=8, Ls=(24, 96), expands=(48,), seed=52)
+    pq_case("pq_d64_ks16", n=800, dim=64, m=32, ks=16, R=16, res_bits=8,
+            comps=24, nq=8, Ls=(16, 64), expands=(16,), seed=62)
+    seeds_case("seeds")
+
+
+if __name__ == "__main__":
+    main()
