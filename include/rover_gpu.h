@@ -234,6 +234,57 @@ AEEResult rover_search_batch(remote_handle64 h, const RoverSearchBatch *b);
 AEEResult rover_search_batch_device(remote_handle64 h,
                                     const RoverSearchBatch *b, void *stream);
 
+/* ------------------------------------------- learned early termination */
+
+/* Trains the stop predictor (16 -> 64 -> 64 -> 1 MLP, BF16 weights, FP32
+ * accumulation) on traces of plain searches of `queries` at beam width
+ * pool_size, and installs it on the session.  A record taken every
+ * stop_every expansions is labelled 1 when the search's top label_k is
+ * already final.  No reference counterpart. */
+typedef struct RoverLearnedStopTrain {
+    const float *queries;     /* host [num_queries][dim] training queries */
+    int num_queries;
+    int pool_size;            /* L of the trace searches                  */
+    int label_k;              /* top-K whose stability is predicted       */
+    int entry_points;
+    int expand_count;
+    int stop_every;           /* feature/decision cadence (expansions)    */
+    int min_steps;            /* no decision before this many expansions  */
+    int trace_cap;            /* records kept per query (<= 4096)         */
+    int steps;                /* Adam steps                               */
+    int batch;                /* samples per step                         */
+    float lr;
+    float pos_weight;         /* weight of label-1 samples (0 -> 1)       */
+    uint32_t seed;
+    /* outputs */
+    float final_loss;         /* mean weighted BCE over the last steps    */
+    long samples;             /* trace records                            */
+    float positive_frac;
+} RoverLearnedStopTrain;
+AEEResult rover_learned_stop_train(remote_handle64 h, RoverLearnedStopTrain *t);
+
+/* Installs / reads back the predictor: FP32 parameters in the flat order
+ * w1[64][16] | b1[64] | w2[64][64] | b2[64] | w3[64] | b3[1] (rounded to
+ * BF16 for inference) and the feature normalization (x - mean) * inv_std. */
+AEEResult rover_learned_stop_set(remote_handle64 h, const float *params,
+                                 int paramsLen, const float *mean,
+                                 const float *inv_std, int stop_every,
+                                 int label_k, int min_steps);
+AEEResult rover_learned_stop_get(remote_handle64 h, float *params,
+                                 int paramsLen, float *mean, float *inv_std);
+
+/* The training kernels on their own (device pointers): adds the gradient of
+ * sum_i w_i * BCE(sigmoid(f(x_i)), y_i) to d_grad and the loss to *d_loss,
+ * logits to d_out (may be NULL); one Adam step scaled by grad_scale. */
+int rover_mlp_param_count(void);
+AEEResult rover_mlp_forward_backward(const float *d_params, const float *d_x,
+                                     const float *d_y, const float *d_weight,
+                                     int n, float *d_grad, float *d_loss,
+                                     float *d_out);
+AEEResult rover_mlp_adam_step(float *d_params, const float *d_grad, float *d_m1,
+                              float *d_m2, int step, float lr,
+                              float grad_scale);
+
 /* Seeds of one query: index 0 is graph_entry (when in range), the rest the
  * LCG s = s*1103515245 + 12345, id = (s * n) >> 32
  * (rpc_demo_pure_pq.c:192-208).  Returns the count. */

@@ -1,0 +1,68 @@
+/*
+ * rover_ctx.h -- session state shared by the C host sources.
+ */
+#ifndef ROVER_CTX_H
+#define ROVER_CTX_H
+
+#include <pthread.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "rover_gpu.h"
+#include "rover_hip.h"
+
+#define ROVER_CTX_MAGIC 0x52564731u /* "RVG1" */
+
+typedef struct RoverLearned {
+    int ready;
+    uint16_t *d_w;     /* bf16 w1 | w2 | w3 */
+    float *d_f;        /* b1 | b2 | b3 | mean | inv_std */
+    float *h_params;   /* fp32 master copy [rv_mlp_param_count()] */
+    float h_mean[RV_MLP_IN], h_inv_std[RV_MLP_IN];
+    int stop_every, label_k, min_steps;
+} RoverLearned;
+
+typedef struct RoverCtx {
+    uint32_t magic;
+    int device;
+    int l1_bits;
+    /* pure-PQ index */
+    int pq_loaded;
+    int n, dim, m, ks, subdim, code_bytes;
+    uint8_t *d_codes;
+    float *d_codebook;
+    int32_t *d_graph;
+    int graph_n, degree, entry;
+    uint8_t *d_res;
+    float *d_qparams;
+    int res_bits;
+    int32_t *d_orig;
+    /* SQ index (retrieval_load) */
+    int sq_loaded;
+    int32_t *d_sq_graph;
+    uint8_t *d_sq_codes;
+    int sq_n, sq_dim, sq_degree, sq_entry, sq_bits;
+    float sq_scale, sq_zero;
+    /* launch workspace */
+    int32_t *d_queue;
+    int32_t *d_entries;
+    int entries_n, entries_count, entries_graph_entry;
+    uint32_t entries_seed;
+    int entries_cap;
+    void *d_stage;
+    size_t stage_cap;
+    RoverLearned learned;
+    pthread_mutex_t lock;
+} RoverCtx;
+
+RoverCtx *rover_ctx_of(remote_handle64 h);
+void rover_set_err(const char *what);
+void rover_set_errf(const char *msg);
+void rover_dfree(void *p);
+/* Fills the kernel's index and batch descriptors for a batch request;
+ * explicit_entries keeps the seed list already on the device. */
+AEEResult rover_prepare_launch(RoverCtx *c, const RoverSearchBatch *b,
+                               int max_pool, int explicit_entries, RvIndex *ix,
+                               RvBatch *rb);
+
+#endif

@@ -129,23 +129,34 @@ int rv_residual_encode(const float *d_x, const uint8_t *d_codes, int n,
                        int bits, uint8_t *d_res, float *d_qparams,
                        void *stream);
 
-/* MLP training kernels (FP32 master weights). */
-typedef struct RvMlpTrain {
-    float *w1, *b1, *w2, *b2, *w3, *b3;          /* params  [5185+..]     */
-    float *g;                                    /* grads, flat [P]       */
-    float *m1, *m2;                              /* Adam moments [P]      */
-    float *loss;                                 /* [1] summed loss        */
-} RvMlpTrain;
+/* MLP training kernels (FP32 master weights, BF16-rounded forward).
+ * Flat parameter layout: w1[64][16] | b1[64] | w2[64][64] | b2[64] | w3[64] |
+ * b3[1].  Gradients and losses are accumulated (callers zero them). */
 int rv_mlp_param_count(void);
-/* params/grads/moments are flat arrays in the order w1,b1,w2,b2,w3,b3. */
 int rv_mlp_forward_backward(const float *d_params, const float *d_x,
                             const float *d_y, const float *d_weight, int n,
                             float *d_grad, float *d_loss, float *d_out,
                             void *stream);
+int rv_mlp_fwd_bwd_sampled(const float *d_params, const float *d_x,
+                           const float *d_y, const float *d_weight,
+                           const int32_t *d_idx, int n, float *d_grad,
+                           float *d_loss, float *d_out, void *stream);
 int rv_mlp_adam(float *d_params, const float *d_grad, float *d_m1,
                 float *d_m2, int step, float lr, float beta1, float beta2,
                 float eps, float grad_scale, void *stream);
 int rv_mlp_to_bf16(const float *d_params, uint16_t *d_bf16, void *stream);
+/* Labels/weights from search traces: y = step >= last top-K change,
+ * weight = pos_w or neg_w for valid records, 0 for padding. */
+int rv_trace_labels(const int32_t *d_steps, const int32_t *d_meta, int nq,
+                    int cap, float pos_w, float neg_w, float *d_y,
+                    float *d_weight, void *stream);
+int rv_normalize(float *d_x, size_t n, const float *d_mean,
+                 const float *d_inv_std, void *stream);
+/* n pseudo-random sample ids in [0, population) from (seed, step). */
+int rv_sample_indices(int32_t *d_idx, int n, size_t population, uint32_t seed,
+                      uint32_t step, void *stream);
+const char *rv_learned_error(void);
+uint16_t rv_bf16_round_host(float f);
 int rv_fill_zero(void *d_ptr, size_t bytes, void *stream);
 
 #ifdef __cplusplus

@@ -7,8 +7,7 @@
  * search runs the HIP kernels, and a missing or failing device is an error.
  */
 #define _GNU_SOURCE
-#include "rover_gpu.h"
-#include "rover_hip.h"
+#include "rover_ctx.h"
 
 #include <math.h>
 #include <pthread.h>
@@ -16,64 +15,29 @@
 #include <stdlib.h>
 #include <string.h>
 
-#define ROVER_CTX_MAGIC 0x52564731u /* "RVG1" */
-
 static __thread char g_err[512];
 
-static void set_err(const char *what) {
+void rover_set_err(const char *what) {
     const char *detail = rv_last_error();
     snprintf(g_err, sizeof(g_err), "%s%s%s", what,
              detail && detail[0] ? ": " : "", detail ? detail : "");
 }
 
+void rover_set_errf(const char *msg) { snprintf(g_err, sizeof(g_err), "%s", msg); }
+
 const char *rover_last_error(void) { return g_err; }
 
-typedef struct RoverLearned {
-    int ready;
-    uint16_t *d_w;     /* bf16 w1 | w2 | w3 */
-    float *d_f;        /* b1 | b2 | b3 | mean | inv_std */
-} RoverLearned;
+#define set_err rover_set_err
+#define ctx_of rover_ctx_of
+#define dfree rover_dfree
 
-typedef struct RoverCtx {
-    uint32_t magic;
-    int device;
-    int l1_bits;
-    /* pure-PQ index */
-    int pq_loaded;
-    int n, dim, m, ks, subdim, code_bytes;
-    uint8_t *d_codes;
-    float *d_codebook;
-    int32_t *d_graph;
-    int graph_n, degree, entry;
-    uint8_t *d_res;
-    float *d_qparams;
-    int res_bits;
-    int32_t *d_orig;
-    /* SQ index (retrieval_load) */
-    int sq_loaded;
-    int32_t *d_sq_graph;
-    uint8_t *d_sq_codes;
-    int sq_n, sq_dim, sq_degree, sq_entry, sq_bits;
-    float sq_scale, sq_zero;
-    /* launch workspace */
-    int32_t *d_queue;
-    int32_t *d_entries;
-    int entries_n, entries_count, entries_graph_entry;
-    uint32_t entries_seed;
-    int entries_cap;
-    void *d_stage;
-    size_t stage_cap;
-    RoverLearned learned;
-    pthread_mutex_t lock;
-} RoverCtx;
-
-static RoverCtx *ctx_of(remote_handle64 h) {
+RoverCtx *rover_ctx_of(remote_handle64 h) {
     RoverCtx *c = (RoverCtx *)(uintptr_t)h;
     if (!c || c->magic != ROVER_CTX_MAGIC) return NULL;
     return c;
 }
 
-static void dfree(void *p) {
+void rover_dfree(void *p) {
     if (p) rv_free(p);
 }
 
@@ -117,6 +81,7 @@ int calculator_open(const char *uri, remote_handle64 *handle) {
 static void learned_free(RoverLearned *l) {
     dfree(l->d_w);
     dfree(l->d_f);
+    free(l->h_params);
     memset(l, 0, sizeof(*l));
 }
 
@@ -448,13 +413,9 @@ static int fill_index(const RoverCtx *c, int method, int correction,
     return 0;
 }
 
-/* Largest per-query L of a batch; per-query arrays live on the device for
- * rover_search_batch_device, so the caller states the bound in pool_size. */
-static AEEResult launch_batch(RoverCtx *c, const RoverSearchBatch *b,
-                              int max_pool, int explicit_entries,
-                              void *stream) {
-    RvIndex ix;
-    RvBatch rb;
+AEEResult rover_prepare_launch(RoverCtx *c, const RoverSearchBatch *b,
+                               int max_pool, int explicit_entries, RvIndex *ix,
+                               RvBatch *rb) {
     uint32_t seed = b->seed ? b->seed : ROVER_ENTRY_POINT_SEED;
     if (b->num_queries < 0 || b->entry_points <= 0 || b->expand_count <= 0 ||
         max_pool <= 0 || max_pool > ROVER_MAX_POOL || b->out_stride <= 0 ||
@@ -463,55 +424,66 @@ static AEEResult launch_batch(RoverCtx *c, const RoverSearchBatch *b,
         (b->method != ROVER_METHOD_PQ && b->method != ROVER_METHOD_SQ) ||
         (b->method == ROVER_METHOD_SQ && (b->correction || b->refine_candidates)))
         return ROVER_EBADPARM;
-    if (fill_index(c, b->method, b->correction, b->refine_candidates, &ix)) {
+    if (fill_index(c, b->method, b->correction, b->refine_candidates, ix)) {
         snprintf(g_err, sizeof(g_err), "search: index not loaded");
         return ROVER_EFAILED;
     }
     if (!explicit_entries &&
-        ensure_entries(c, ix.n, b->entry_points, ix.graph_entry, seed))
+        ensure_entries(c, ix->n, b->entry_points, ix->graph_entry, seed))
         return ROVER_EFAILED;
-    memset(&rb, 0, sizeof(rb));
-    rb.queries = b->queries;
-    rb.nq = b->num_queries;
-    rb.pool_sizes = b->pool_sizes;
-    rb.num_outs = b->num_outs;
-    rb.pool_size = b->pool_size;
-    rb.num_out = b->num_out;
-    rb.max_pool = max_pool;
-    rb.entries = c->d_entries;
-    rb.n_entries = c->entries_count;
-    rb.expand_count = b->expand_count;
-    rb.scorer = b->method == ROVER_METHOD_SQ
-                    ? RV_SCORE_SQ
-                    : (b->correction ? RV_SCORE_ADC_ONLINE : RV_SCORE_ADC);
-    rb.refine_candidates = b->refine_candidates;
-    rb.hash_log2 = b->hash_log2 > 0 ? b->hash_log2 : default_hash_log2(max_pool);
-    rb.probe_limit = b->probe_limit > 0 ? b->probe_limit : 8;
-    rb.out_ids = b->out_ids;
-    rb.out_dists = b->out_dists;
-    rb.out_counts = b->out_counts;
-    rb.out_stride = b->out_stride;
-    rb.out_stats = b->out_stats;
-    rb.queue = c->d_queue;
-    rb.stop_label_k = 10;
+    memset(rb, 0, sizeof(*rb));
+    rb->queries = b->queries;
+    rb->nq = b->num_queries;
+    rb->pool_sizes = b->pool_sizes;
+    rb->num_outs = b->num_outs;
+    rb->pool_size = b->pool_size;
+    rb->num_out = b->num_out;
+    rb->max_pool = max_pool;
+    rb->entries = c->d_entries;
+    rb->n_entries = c->entries_count;
+    rb->expand_count = b->expand_count;
+    rb->scorer = b->method == ROVER_METHOD_SQ
+                     ? RV_SCORE_SQ
+                     : (b->correction ? RV_SCORE_ADC_ONLINE : RV_SCORE_ADC);
+    rb->refine_candidates = b->refine_candidates;
+    rb->hash_log2 = b->hash_log2 > 0 ? b->hash_log2 : default_hash_log2(max_pool);
+    rb->probe_limit = b->probe_limit > 0 ? b->probe_limit : 8;
+    rb->out_ids = b->out_ids;
+    rb->out_dists = b->out_dists;
+    rb->out_counts = b->out_counts;
+    rb->out_stride = b->out_stride;
+    rb->out_stats = b->out_stats;
+    rb->queue = c->d_queue;
+    rb->stop_label_k = c->learned.label_k > 0 ? c->learned.label_k : 10;
+    rb->stop_every = c->learned.stop_every > 0 ? c->learned.stop_every : 1;
     if (b->learned_stop) {
         if (!c->learned.ready) {
             snprintf(g_err, sizeof(g_err), "search: learned stop model not set");
             return ROVER_EFAILED;
         }
-        rb.stop_mode = 1;
-        rb.stop_threshold = b->stop_threshold > 0.0f ? b->stop_threshold : 0.9f;
-        rb.stop_every = 1;
-        rb.mlp.w1 = c->learned.d_w;
-        rb.mlp.w2 = c->learned.d_w + RV_MLP_HIDDEN * RV_MLP_IN;
-        rb.mlp.w3 = c->learned.d_w + RV_MLP_HIDDEN * RV_MLP_IN +
-                    RV_MLP_HIDDEN * RV_MLP_HIDDEN;
-        rb.mlp.b1 = c->learned.d_f;
-        rb.mlp.b2 = c->learned.d_f + RV_MLP_HIDDEN;
-        rb.mlp.b3 = c->learned.d_f + 2 * RV_MLP_HIDDEN;
-        rb.mlp.feat_mean = c->learned.d_f + 2 * RV_MLP_HIDDEN + 1;
-        rb.mlp.feat_inv_std = c->learned.d_f + 2 * RV_MLP_HIDDEN + 1 + RV_MLP_IN;
+        rb->stop_mode = 1;
+        rb->stop_threshold = b->stop_threshold > 0.0f ? b->stop_threshold : 0.9f;
+        rb->stop_min_steps = c->learned.min_steps;
+        rb->mlp.w1 = c->learned.d_w;
+        rb->mlp.w2 = c->learned.d_w + RV_MLP_HIDDEN * RV_MLP_IN;
+        rb->mlp.w3 = c->learned.d_w + RV_MLP_HIDDEN * RV_MLP_IN +
+                     RV_MLP_HIDDEN * RV_MLP_HIDDEN;
+        rb->mlp.b1 = c->learned.d_f;
+        rb->mlp.b2 = c->learned.d_f + RV_MLP_HIDDEN;
+        rb->mlp.b3 = c->learned.d_f + 2 * RV_MLP_HIDDEN;
+        rb->mlp.feat_mean = c->learned.d_f + 2 * RV_MLP_HIDDEN + 1;
+        rb->mlp.feat_inv_std = c->learned.d_f + 2 * RV_MLP_HIDDEN + 1 + RV_MLP_IN;
     }
+    return ROVER_SUCCESS;
+}
+
+static AEEResult launch_batch(RoverCtx *c, const RoverSearchBatch *b,
+                              int max_pool, int explicit_entries,
+                              void *stream) {
+    RvIndex ix;
+    RvBatch rb;
+    AEEResult rc = rover_prepare_launch(c, b, max_pool, explicit_entries, &ix, &rb);
+    if (rc != ROVER_SUCCESS) return rc;
     if (rv_search_launch(&ix, &rb, b->grid_waves, stream)) {
         set_err("search launch");
         return ROVER_EFAILED;

@@ -68,6 +68,17 @@ class RoverSearchBatch(C.Structure):
                 ("learned_stop", C.c_int), ("stop_threshold", C.c_float)]
 
 
+class RoverLearnedStopTrain(C.Structure):
+    _fields_ = [("queries", C.c_void_p), ("num_queries", C.c_int),
+                ("pool_size", C.c_int), ("label_k", C.c_int),
+                ("entry_points", C.c_int), ("expand_count", C.c_int),
+                ("stop_every", C.c_int), ("min_steps", C.c_int),
+                ("trace_cap", C.c_int), ("steps", C.c_int), ("batch", C.c_int),
+                ("lr", C.c_float), ("pos_weight", C.c_float),
+                ("seed", C.c_uint32), ("final_loss", C.c_float),
+                ("samples", C.c_long), ("positive_frac", C.c_float)]
+
+
 # (name, restype, argtypes); every symbol include/rover_gpu.h declares.
 _H = C.c_uint64
 _P = C.c_void_p
@@ -100,6 +111,12 @@ SIGNATURES = [
     ("rover_search_batch", _I, [_H, _P]),
     ("rover_search_batch_device", _I, [_H, _P, _P]),
     ("rover_make_seed_ids", _I, [_I, _I, _I, C.c_uint32, _P]),
+    ("rover_learned_stop_train", _I, [_H, _P]),
+    ("rover_learned_stop_set", _I, [_H, _P, _I, _P, _P, _I, _I, _I]),
+    ("rover_learned_stop_get", _I, [_H, _P, _I, _P, _P]),
+    ("rover_mlp_param_count", _I, []),
+    ("rover_mlp_forward_backward", _I, [_P, _P, _P, _P, _I, _P, _P, _P]),
+    ("rover_mlp_adam_step", _I, [_P, _P, _P, _P, _I, _F, _F]),
     ("rover_build_pq_encode", _I, [_P, _I, _I, _I, _I, _P, _P]),
     ("rover_build_residual_encode", _I, [_P, _P, _I, _I, _I, _I, _P, _I, _P, _P]),
     ("rover_device_alloc", _I, [C.POINTER(_P), C.c_size_t]),
