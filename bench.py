@@ -63,6 +63,13 @@ def main():
     ap.add_argument("--target", type=float, default=0.95)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--train-queries", type=int, default=200_000)
+    ap.add_argument("--train-L", type=int, default=512)
+    ap.add_argument("--label-k", type=int, default=10)
+    ap.add_argument("--stop-every", type=int, default=2)
+    ap.add_argument("--trace-cap", type=int, default=512)
+    ap.add_argument("--train-steps", type=int, default=2000)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -94,55 +101,101 @@ def main():
     d_cnt = native.DeviceBuffer(nq * 4)
     d_st = native.DeviceBuffer(nq * 32)
 
-    def run(L, stats=False):
+    def run(L, stats=False, learned=False):
         calc.search_batch_device(dq.ptr, nq, L, k, d_ids.ptr, d_d.ptr, d_cnt.ptr, k,
                                  refine_candidates=min(F, L), entry_points=1,
                                  expand_count=args.R,
-                                 d_stats=d_st.ptr if stats else None)
+                                 d_stats=d_st.ptr if stats else None,
+                                 learned_stop=1 if learned else 0,
+                                 stop_threshold=0.9)
 
-    sweep = []
-    chosen = None
-    for L in [int(v) for v in args.Ls.split(",")]:
-        run(L, stats=True)
+    def sweep_mode(Ls, learned):
+        rows, chosen = [], None
+        for L in Ls:
+            run(L, stats=True, learned=learned)
+            native.synchronize()
+            ids = d_ids.download(np.int32, nq * k).reshape(nq, k)
+            cnt = d_cnt.download(np.int32, nq)
+            st = d_st.download(np.uint64, nq * 4).reshape(nq, 4)
+            assert (cnt == k).all(), "search failed for some queries: %s" % np.unique(cnt)
+            rec = data.recall_at_k(ids, cnt, gt, k)
+            t0, t1 = native.Event(), native.Event()
+            t0.record()
+            run(L, learned=learned)
+            t1.record()
+            ms = t0.elapsed_ms(t1)
+            rows.append(dict(L=L, recall10=round(rec, 4), qps=round(nq / (ms / 1e3), 1),
+                             mean_expanded=round(float(st[:, 2].mean()), 2),
+                             mean_scored=round(float(st[:, 0].mean()), 2)))
+            log("%s L=%d recall@10=%.4f qps=%.0f expanded=%.1f scored=%.1f" %
+                ("learned" if learned else "plain", L, rec, nq / (ms / 1e3),
+                 st[:, 2].mean(), st[:, 0].mean()))
+            if chosen is None and rec >= args.target:
+                chosen = (L, st.copy())
+        if chosen is None:
+            chosen = (Ls[-1], st.copy())
+            log("no L reached the target recall; using the largest")
+        return rows, chosen
+
+    def timed(L, learned):
+        for _ in range(args.warmup):
+            run(L, learned=learned)
         native.synchronize()
-        ids = d_ids.download(np.int32, nq * k).reshape(nq, k)
-        cnt = d_cnt.download(np.int32, nq)
-        st = d_st.download(np.uint64, nq * 4).reshape(nq, 4)
-        assert (cnt == k).all(), "search failed for some queries: %s" % np.unique(cnt)
-        rec = data.recall_at_k(ids, cnt, gt, k)
-        t0 = native.Event()
-        t1 = native.Event()
-        t0.record()
-        run(L)
-        t1.record()
-        ms = t0.elapsed_ms(t1)
-        sweep.append(dict(L=L, recall10=round(rec, 4), qps=round(nq / (ms / 1e3), 1),
-                          mean_expanded=float(st[:, 2].mean()),
-                          mean_scored=float(st[:, 0].mean())))
-        log("plain L=%d recall@10=%.4f qps=%.0f expanded=%.1f scored=%.1f" %
-            (L, rec, nq / (ms / 1e3), st[:, 2].mean(), st[:, 0].mean()))
-        if chosen is None and rec >= args.target:
-            chosen = (L, st)
-    if chosen is None:
-        chosen = (sweep[-1]["L"], st)
-        log("no L reached the target recall; using the largest")
-    L, stats = chosen
+        ev = [native.Event() for _ in range(args.steps + 1)]
+        ev[0].record()
+        for i in range(args.steps):
+            run(L, learned=learned)
+            ev[i + 1].record()
+        native.synchronize()
+        per = [ev[i].elapsed_ms(ev[i + 1]) for i in range(args.steps)]
+        return sum(per) / args.steps
+
+    Ls = [int(v) for v in args.Ls.split(",")]
+    plain_rows, (L_plain, st_plain) = sweep_mode(Ls, False)
+
+    # ---- learned stop: traces of plain L=train_L searches of training queries
+    learned_rows, L_learned, st_learned, train_info = None, None, None, None
+    if not args.no_learned:
+        import ctypes as C
+        train_q = data.gaussian_mixture(args.train_queries, args.dim, args.components,
+                                        8.0, 44, centers_seed=42)
+        t = native.RoverLearnedStopTrain()
+        t.queries = train_q.ctypes.data
+        t.num_queries = train_q.shape[0]
+        t.pool_size = args.train_L
+        t.label_k = args.label_k
+        t.entry_points = 1
+        t.expand_count = args.R
+        t.stop_every = args.stop_every
+        t.min_steps = args.stop_every
+        t.trace_cap = args.trace_cap
+        t.steps = args.train_steps
+        t.batch = 8192
+        t.lr = 3e-3
+        t.seed = 44
+        t0 = time.time()
+        native.check(native.lib().rover_learned_stop_train(calc.h, C.byref(t)),
+                     "rover_learned_stop_train")
+        train_info = dict(train_queries=args.train_queries, train_L=args.train_L,
+                          label_k=args.label_k, stop_every=args.stop_every,
+                          samples=int(t.samples), positive_frac=round(t.positive_frac, 4),
+                          final_loss=round(t.final_loss, 5),
+                          seconds=round(time.time() - t0, 2), threshold=0.9)
+        log("learned stop trained: %s" % train_info)
+        learned_rows, (L_learned, st_learned) = sweep_mode(Ls, True)
 
     # ---- timed region: K steps of the full batch, queries resident in HBM
-    for _ in range(args.warmup):
-        run(L)
-    native.synchronize()
-    ev = [native.Event() for _ in range(args.steps + 1)]
-    ev[0].record()
-    for i in range(args.steps):
-        run(L)
-        ev[i + 1].record()
-    native.synchronize()
-    per = [ev[i].elapsed_ms(ev[i + 1]) for i in range(args.steps)]
-    total_ms = sum(per)
-    ms_per_step = total_ms / args.steps
-    qps = nq / (ms_per_step / 1e3)
-    log("timed: L=%d %.3f ms/step  %.0f QPS" % (L, ms_per_step, qps))
+    ms_plain = timed(L_plain, False)
+    qps_plain = nq / (ms_plain / 1e3)
+    log("timed plain: L=%d %.3f ms/step  %.0f QPS" % (L_plain, ms_plain, qps_plain))
+    if L_learned is not None:
+        ms_learned = timed(L_learned, True)
+        qps_learned = nq / (ms_learned / 1e3)
+        log("timed learned: L=%d %.3f ms/step  %.0f QPS (%.2fx plain)" %
+            (L_learned, ms_learned, qps_learned, qps_learned / qps_plain))
+        ms_per_step, qps, stats, L, mode = ms_learned, qps_learned, st_learned, L_learned, "learned-stop"
+    else:
+        ms_per_step, qps, stats, L, mode = ms_plain, qps_plain, st_plain, L_plain, "plain"
 
     code_bytes = index.code_bytes(args.m, 256)
     res_bytes = index.residual_code_bytes(args.dim, 4)
@@ -153,6 +206,7 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                 "traffic": None, "kernel": "rv_search_kernel<0>",
                 "algorithmic_bytes_per_launch": int(algo)}
+    L_cpu = L_plain
 
     cpu = None
     if not args.no_cpu:
@@ -163,22 +217,25 @@ def main():
         threads = min(16, os.cpu_count() or 1)
         probe = min(nq, 64 * threads)
         t = time.time()
-        pyoracle.search_batch(oix, 0, queries[:probe], entries, args.R, L, k,
-                              refine=min(F, L), threads=threads)
+        pyoracle.search_batch(oix, 0, queries[:probe], entries, args.R, L_cpu, k,
+                              refine=min(F, L_cpu), threads=threads)
         rate = probe / max(time.time() - t, 1e-6)
         sample = int(min(nq, max(probe, rate * args.cpu_seconds)))
         t = time.time()
         oids, od, ocnt, _ = pyoracle.search_batch(oix, 0, queries[:sample], entries,
-                                                  args.R, L, k, refine=min(F, L),
+                                                  args.R, L_cpu, k, refine=min(F, L_cpu),
                                                   threads=threads)
         el = time.time() - t
-        # spot parity on the sample: same ids as the GPU
+        # spot parity on the sample: the plain GPU search returns the same ids
+        run(L_cpu)
+        native.synchronize()
         gids = d_ids.download(np.int32, nq * k).reshape(nq, k)[:sample]
         same = float((gids == oids).all(axis=1).mean())
         cpu = {"value": round(sample / el, 1), "unit": "queries/s", "cores": threads,
                "kind": "port",
-               "sample": "%d queries of the same workload at L=%d, refine %d -> top-%d, "
-                         "one query per thread" % (sample, L, min(F, L), k),
+               "sample": "%d queries of the same workload, plain search at L=%d, refine "
+                         "%d -> top-%d, one query per thread" % (sample, L_cpu,
+                                                                 min(F, L_cpu), k),
                "gpu_ids_identical_frac": same}
         log("cpu baseline: %.0f QPS on %d threads (%d queries, %.1fs); GPU ids "
             "identical on %.4f of them" % (sample / el, threads, sample, el, same))
@@ -198,9 +255,15 @@ def main():
         "data": "synthetic",
         "config": {"workload": "1M x 128 Gaussian mixture (1024 comps, sigma 8), PQ 32x256 "
                                "+ 4-bit residual refine, R=32 RNG graph, %d queries" % nq,
-                   "mode": "plain", "L": L, "k": k, "refine_candidates": min(F, L),
-                   "entry_points": 1, "recall_at_10": [s for s in sweep if s["L"] == L][0]["recall10"],
-                   "sweep": sweep},
+                   "mode": mode, "L": L, "k": k, "refine_candidates": min(F, L),
+                   "entry_points": 1,
+                   "plain": {"L": L_plain, "qps": round(qps_plain, 1),
+                             "ms_per_step": round(ms_plain, 4), "sweep": plain_rows},
+                   "learned": None if L_learned is None else {
+                       "L": L_learned, "qps": round(qps_learned, 1),
+                       "ms_per_step": round(ms_learned, 4),
+                       "speedup_vs_plain": round(qps_learned / qps_plain, 3),
+                       "sweep": learned_rows, "training": train_info}},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -360,7 +360,13 @@ __global__ __launch_bounds__(RV_WAVE) void rv_search_kernel(RvIndex ix, RvBatch 
         float *odist = b.out_dists + (size_t)q * b.out_stride;
         const int refine = b.refine_candidates;
         if (L <= 0 || L > b.max_pool || K <= 0 || K > L || K > b.out_stride) {
-            if (lane == 0) b.out_counts[q] = RV_QUERY_BAD_PARAM;
+            if (lane == 0) {
+                b.out_counts[q] = RV_QUERY_BAD_PARAM;
+                if (b.stop_mode == 2) {
+                    b.trace_meta[2 * (size_t)q] = 0;
+                    b.trace_meta[2 * (size_t)q + 1] = 0;
+                }
+            }
             continue;
         }
 

@@ -134,8 +134,10 @@ def test_library_dropin_matches_reference(golden, name):
                                                g["queries"][qi], entries, int(expand),
                                                int(L), int(L), chunk_vectors=256,
                                                residual=res)
-            assert np.array_equal(ids, g[key + "_ids"][qi])
-            assert np.array_equal(bits(d), bits(g[key + "_dists"][qi]))
+            c = int(g[key + "_counts"][qi])
+            assert len(ids) == c
+            assert np.array_equal(ids, g[key + "_ids"][qi][:c])
+            assert np.array_equal(bits(d), bits(g[key + "_dists"][qi][:c]))
             assert int(st[2]) == int(g[key + "_stats"][qi][2])
 
 
