@@ -1,16 +1,24 @@
-"""Benchmark: QPS at recall@10 >= 0.95 on the 1M x 128 PQ index, one MI355X.
+"""Benchmark: QPS at recall@10 >= 0.95 on the 1M x 128 PQ index, one MI355X,
+learned early termination against the plain fixed-L search.
 
-Workload (N=1,000,000, d=128, 1024-component Gaussian mixture, sigma=8;
-PQ M=32 x 256 with 4-bit residuals; degree-32 graph from exact 64-NN and
-RNG pruning, medoid entry; 10,000 queries from the same mixture): the index
-is built on the GPU at start-up, the queries are resident in HBM, and one
-"step" is one batched search of all queries (plain ADC traversal of beam
-width L, then the fused final refine of the top F candidates to top-10).
-L is the smallest of the sweep reaching recall@10 >= 0.95.
+Workload: N=1,000,000 base vectors, d=128, from a 1024-component Gaussian
+mixture (centres U[0,100]^d, sigma 8, clipped to [0,255]); PQ M=32 x 256
+(8-bit codes) plus per-vector residual codes; degree-32 graph from exact
+64-NN, RNG pruning and backlinks; seed list = medoid + k-means
+representatives; 10,000 queries from the same mixture.  The index is built
+on the GPU at start-up and the queries are resident in HBM.
+
+One "step" is one batched search of all queries.  For each mode (online
+residual correction; ADC ranking + final refine) the beam width L is the
+smallest of the sweep reaching recall@10 >= 0.95 against exact float32
+ground truth; the plain search uses the faster mode.  The learned stop
+predictor is trained on traces of that mode's plain search of 200,000
+training queries at L=512 and then swept the same way.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -25,6 +33,7 @@ from rover_amd import data, index, native  # noqa: E402
 from rover_amd.search import Calculator  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+T0 = time.time()
 
 
 def log(*a):
@@ -32,10 +41,11 @@ def log(*a):
 
 
 def algorithmic_bytes(stats, R, code_bytes, res_bytes, dim, k, refine_F,
-                      codebook_bytes, online=False):
-    """Bytes one launch must move: graph rows of expanded nodes, codes of
-    scored candidates, codes+residual+qparams of refined candidates, the
-    query and its output, and the codebook once."""
+                      codebook_bytes, online):
+    """Bytes one launch must move: graph rows of expanded nodes (4R each),
+    codes of scored candidates, their residual codes and qparams in online
+    mode, codes+residual+qparams of refined candidates, the query vector
+    and the output; the codebook once."""
     E = stats[:, 2].astype(np.float64)
     Cq = stats[:, 0].astype(np.float64)
     b = 4.0 * R * E + code_bytes * Cq
@@ -56,10 +66,14 @@ def main():
     ap.add_argument("--m", type=int, default=32)
     ap.add_argument("--R", type=int, default=32)
     ap.add_argument("--components", type=int, default=1024)
-    ap.add_argument("--Ls", default="32,48,64,96,128,192,256")
+    ap.add_argument("--res-bits", type=int, default=8)
+    ap.add_argument("--entries", type=int, default=4096,
+                    help="k-means representatives in the shared seed list")
+    ap.add_argument("--Ls", default="16,24,32,48,64,96,128,192,256")
     ap.add_argument("--k", type=int, default=10)
-    ap.add_argument("--refine", type=int, default=40,
-                    help="refine candidates F (top-F of the pool)")
+    ap.add_argument("--refine", type=int, default=64,
+                    help="refine candidates F of the ADC mode")
+    ap.add_argument("--modes", default="online,adc")
     ap.add_argument("--target", type=float, default=0.95)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -73,20 +87,21 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs the search on a single "
-              "MI355X (queries are independent; see DESIGN.md for sharding)",
+        print("bench.py measures one GPU: the search runs on a single MI355X "
+              "(queries are independent; multi-GPU is data parallel over queries)",
               file=sys.stderr)
         sys.exit(2)
 
     import torch
     dev = torch.device("cuda:0")
     log("device", torch.cuda.get_device_name(0))
-
     base, queries = data.workload(args.n, args.dim, args.components, 8.0, args.nq)
     log("data", base.shape, queries.shape)
-    ix = index.build_pq_index(base, m=args.m, R=args.R, res_bits=4, iters=20,
-                              train_max=100000, device=dev)
-    log("index built: graph", ix.graph.shape, "entry", ix.entry)
+    ix = index.build_pq_index(base, m=args.m, R=args.R, res_bits=args.res_bits,
+                              iters=20, train_max=100000, device=dev,
+                              n_entries=args.entries)
+    log("index built: graph", ix.graph.shape, "medoid", ix.entry, "seed list",
+        len(ix.entries))
     gt = data.ground_truth(base, queries, k=100, device=dev)
     log("ground truth done")
     torch.cuda.synchronize()
@@ -94,77 +109,99 @@ def main():
 
     calc = Calculator()
     calc.load_pq_index(ix)
-    nq, k, F = args.nq, args.k, args.refine
+    nq, k = args.nq, args.k
     dq = native.DeviceBuffer.from_array(queries)
     d_ids = native.DeviceBuffer(nq * k * 4)
     d_d = native.DeviceBuffer(nq * k * 4)
     d_cnt = native.DeviceBuffer(nq * 4)
     d_st = native.DeviceBuffer(nq * 32)
+    d_ent = native.DeviceBuffer.from_array(ix.entries)
+    n_ent = len(ix.entries)
 
-    def run(L, stats=False, learned=False):
+    def refine_of(mode, L):
+        return 0 if mode == "online" else min(args.refine, L)
+
+    def run(mode, L, stats=False, learned=False):
         calc.search_batch_device(dq.ptr, nq, L, k, d_ids.ptr, d_d.ptr, d_cnt.ptr, k,
-                                 refine_candidates=min(F, L), entry_points=1,
+                                 correction=1 if mode == "online" else 0,
+                                 refine_candidates=refine_of(mode, L),
                                  expand_count=args.R,
                                  d_stats=d_st.ptr if stats else None,
                                  learned_stop=1 if learned else 0,
-                                 stop_threshold=0.9)
+                                 stop_threshold=0.9, d_entry_ids=d_ent.ptr,
+                                 entry_count=n_ent)
 
-    def sweep_mode(Ls, learned):
+    def sweep(mode, learned):
         rows, chosen = [], None
-        for L in Ls:
-            run(L, stats=True, learned=learned)
+        for L in [int(v) for v in args.Ls.split(",")]:
+            run(mode, L, stats=True, learned=learned)
             native.synchronize()
             ids = d_ids.download(np.int32, nq * k).reshape(nq, k)
             cnt = d_cnt.download(np.int32, nq)
             st = d_st.download(np.uint64, nq * 4).reshape(nq, 4)
-            assert (cnt == k).all(), "search failed for some queries: %s" % np.unique(cnt)
+            assert (cnt == k).all(), "search failed: %s" % np.unique(cnt)
             rec = data.recall_at_k(ids, cnt, gt, k)
-            t0, t1 = native.Event(), native.Event()
-            t0.record()
-            run(L, learned=learned)
-            t1.record()
-            ms = t0.elapsed_ms(t1)
+            e0, e1 = native.Event(), native.Event()
+            e0.record()
+            run(mode, L, learned=learned)
+            e1.record()
+            ms = e0.elapsed_ms(e1)
             rows.append(dict(L=L, recall10=round(rec, 4), qps=round(nq / (ms / 1e3), 1),
                              mean_expanded=round(float(st[:, 2].mean()), 2),
                              mean_scored=round(float(st[:, 0].mean()), 2)))
-            log("%s L=%d recall@10=%.4f qps=%.0f expanded=%.1f scored=%.1f" %
-                ("learned" if learned else "plain", L, rec, nq / (ms / 1e3),
+            log("%s%s L=%d recall@10=%.4f qps=%.0f expanded=%.1f scored=%.1f" %
+                (mode, "+learned" if learned else "", L, rec, nq / (ms / 1e3),
                  st[:, 2].mean(), st[:, 0].mean()))
             if chosen is None and rec >= args.target:
-                chosen = (L, st.copy())
-        if chosen is None:
-            chosen = (Ls[-1], st.copy())
-            log("no L reached the target recall; using the largest")
+                chosen = dict(L=L, stats=st.copy(), qps=nq / (ms / 1e3), recall=rec)
+                break
         return rows, chosen
 
-    def timed(L, learned):
+    def timed(mode, L, learned):
         for _ in range(args.warmup):
-            run(L, learned=learned)
+            run(mode, L, learned=learned)
         native.synchronize()
         ev = [native.Event() for _ in range(args.steps + 1)]
         ev[0].record()
         for i in range(args.steps):
-            run(L, learned=learned)
+            run(mode, L, learned=learned)
             ev[i + 1].record()
         native.synchronize()
-        per = [ev[i].elapsed_ms(ev[i + 1]) for i in range(args.steps)]
-        return sum(per) / args.steps
+        return sum(ev[i].elapsed_ms(ev[i + 1]) for i in range(args.steps)) / args.steps
 
-    Ls = [int(v) for v in args.Ls.split(",")]
-    plain_rows, (L_plain, st_plain) = sweep_mode(Ls, False)
+    # ---- plain search: best mode at the target recall
+    plain = {}
+    for mode in args.modes.split(","):
+        rows, chosen = sweep(mode, False)
+        plain[mode] = dict(sweep=rows, chosen=chosen)
+    reached = [(v["chosen"]["qps"], m) for m, v in plain.items() if v["chosen"]]
+    if reached:
+        mode = max(reached)[1]
+    else:
+        mode = args.modes.split(",")[0]
+        last = plain[mode]["sweep"][-1]
+        run(mode, last["L"], stats=True)
+        native.synchronize()
+        plain[mode]["chosen"] = dict(L=last["L"], recall=last["recall10"],
+                                     qps=last["qps"], stats=d_st.download(
+                                         np.uint64, nq * 4).reshape(nq, 4))
+        log("no L reached the target recall; using mode %s at its largest L" % mode)
+    L_plain = plain[mode]["chosen"]["L"]
 
-    # ---- learned stop: traces of plain L=train_L searches of training queries
-    learned_rows, L_learned, st_learned, train_info = None, None, None, None
+    # ---- learned stop on the same mode
+    learned = None
     if not args.no_learned:
-        import ctypes as C
         train_q = data.gaussian_mixture(args.train_queries, args.dim, args.components,
                                         8.0, 44, centers_seed=42)
         t = native.RoverLearnedStopTrain()
         t.queries = train_q.ctypes.data
         t.num_queries = train_q.shape[0]
         t.pool_size = args.train_L
+        t.correction = 1 if mode == "online" else 0
         t.label_k = args.label_k
         t.entry_points = 1
+        t.entry_ids = ix.entries.ctypes.data
+        t.entry_count = n_ent
         t.expand_count = args.R
         t.stop_every = args.stop_every
         t.min_steps = args.stop_every
@@ -173,72 +210,88 @@ def main():
         t.batch = 8192
         t.lr = 3e-3
         t.seed = 44
-        t0 = time.time()
+        tt = time.time()
         native.check(native.lib().rover_learned_stop_train(calc.h, C.byref(t)),
                      "rover_learned_stop_train")
-        train_info = dict(train_queries=args.train_queries, train_L=args.train_L,
-                          label_k=args.label_k, stop_every=args.stop_every,
-                          samples=int(t.samples), positive_frac=round(t.positive_frac, 4),
-                          final_loss=round(t.final_loss, 5),
-                          seconds=round(time.time() - t0, 2), threshold=0.9)
-        log("learned stop trained: %s" % train_info)
-        learned_rows, (L_learned, st_learned) = sweep_mode(Ls, True)
+        training = dict(train_queries=args.train_queries, train_L=args.train_L,
+                        mode=mode, label_k=args.label_k, stop_every=args.stop_every,
+                        samples=int(t.samples), positive_frac=round(t.positive_frac, 4),
+                        final_loss=round(t.final_loss, 5),
+                        seconds=round(time.time() - tt, 2), threshold=0.9)
+        log("learned stop trained:", training)
+        rows, chosen = sweep(mode, True)
+        if chosen is None:
+            last = rows[-1]
+            run(mode, last["L"], stats=True, learned=True)
+            native.synchronize()
+            chosen = dict(L=last["L"], recall=last["recall10"], qps=last["qps"],
+                          stats=d_st.download(np.uint64, nq * 4).reshape(nq, 4))
+        learned = dict(sweep=rows, chosen=chosen, training=training)
 
     # ---- timed region: K steps of the full batch, queries resident in HBM
-    ms_plain = timed(L_plain, False)
+    ms_plain = timed(mode, L_plain, False)
     qps_plain = nq / (ms_plain / 1e3)
-    log("timed plain: L=%d %.3f ms/step  %.0f QPS" % (L_plain, ms_plain, qps_plain))
-    if L_learned is not None:
-        ms_learned = timed(L_learned, True)
-        qps_learned = nq / (ms_learned / 1e3)
-        log("timed learned: L=%d %.3f ms/step  %.0f QPS (%.2fx plain)" %
-            (L_learned, ms_learned, qps_learned, qps_learned / qps_plain))
-        ms_per_step, qps, stats, L, mode = ms_learned, qps_learned, st_learned, L_learned, "learned-stop"
+    log("timed plain %s: L=%d %.3f ms/step %.0f QPS" % (mode, L_plain, ms_plain, qps_plain))
+    if learned:
+        L_l = learned["chosen"]["L"]
+        ms_l = timed(mode, L_l, True)
+        qps_l = nq / (ms_l / 1e3)
+        log("timed learned %s: L=%d %.3f ms/step %.0f QPS (%.2fx plain)" %
+            (mode, L_l, ms_l, qps_l, qps_l / qps_plain))
+        ms_per_step, qps, stats, L, label = ms_l, qps_l, learned["chosen"]["stats"], L_l, "learned-stop"
     else:
-        ms_per_step, qps, stats, L, mode = ms_plain, qps_plain, st_plain, L_plain, "plain"
+        ms_per_step, qps, stats, L, label = ms_plain, qps_plain, plain[mode]["chosen"]["stats"], L_plain, "plain"
 
     code_bytes = index.code_bytes(args.m, 256)
-    res_bytes = index.residual_code_bytes(args.dim, 4)
+    res_bytes = index.residual_code_bytes(args.dim, args.res_bits)
     algo = algorithmic_bytes(stats, args.R, code_bytes, res_bytes, args.dim, k,
-                             min(F, L), ix.codebook.nbytes)
+                             refine_of(mode, L), ix.codebook.nbytes, mode == "online")
     achieved = algo / (ms_per_step / 1e3) / 1e9
+    kernel = "rv_search_kernel<%d>" % (1 if mode == "online" else 0)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": None, "kernel": "rv_search_kernel<0>",
+                "traffic": None, "kernel": kernel,
                 "algorithmic_bytes_per_launch": int(algo)}
-    L_cpu = L_plain
 
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle
         oix = pyoracle.Index.from_pq(ix)
-        entries = pyoracle.seed_ids(ix.n, 1, ix.entry, 42)
+        scorer = pyoracle.SCORE_ONLINE if mode == "online" else pyoracle.SCORE_ADC
         threads = min(16, os.cpu_count() or 1)
-        probe = min(nq, 64 * threads)
-        t = time.time()
-        pyoracle.search_batch(oix, 0, queries[:probe], entries, args.R, L_cpu, k,
-                              refine=min(F, L_cpu), threads=threads)
-        rate = probe / max(time.time() - t, 1e-6)
+        probe = min(nq, 16 * threads)
+        t1 = time.time()
+        pyoracle.search_batch(oix, scorer, queries[:probe], ix.entries, args.R, L_plain,
+                              k, refine=refine_of(mode, L_plain), threads=threads)
+        rate = probe / max(time.time() - t1, 1e-6)
         sample = int(min(nq, max(probe, rate * args.cpu_seconds)))
-        t = time.time()
-        oids, od, ocnt, _ = pyoracle.search_batch(oix, 0, queries[:sample], entries,
-                                                  args.R, L_cpu, k, refine=min(F, L_cpu),
+        t1 = time.time()
+        oids, od, ocnt, _ = pyoracle.search_batch(oix, scorer, queries[:sample], ix.entries,
+                                                  args.R, L_plain, k,
+                                                  refine=refine_of(mode, L_plain),
                                                   threads=threads)
-        el = time.time() - t
-        # spot parity on the sample: the plain GPU search returns the same ids
-        run(L_cpu)
+        el = time.time() - t1
+        run(mode, L_plain)
         native.synchronize()
         gids = d_ids.download(np.int32, nq * k).reshape(nq, k)[:sample]
-        same = float((gids == oids).all(axis=1).mean())
+        gd = d_d.download(np.float32, nq * k).reshape(nq, k)[:sample]
+        same = float(((gids == oids).all(axis=1) &
+                      (gd.view(np.uint32) == od.view(np.uint32)).all(axis=1)).mean())
         cpu = {"value": round(sample / el, 1), "unit": "queries/s", "cores": threads,
                "kind": "port",
-               "sample": "%d queries of the same workload, plain search at L=%d, refine "
-                         "%d -> top-%d, one query per thread" % (sample, L_cpu,
-                                                                 min(F, L_cpu), k),
-               "gpu_ids_identical_frac": same}
-        log("cpu baseline: %.0f QPS on %d threads (%d queries, %.1fs); GPU ids "
-            "identical on %.4f of them" % (sample / el, threads, sample, el, same))
+               "sample": "%d of the queries, plain %s search at L=%d, one query per host "
+                         "thread (%.1f s)" % (sample, mode, L_plain, el),
+               "gpu_results_identical_frac": same}
+        log("cpu baseline: %.0f QPS on %d threads; GPU ids+distances identical on %.4f"
+            % (sample / el, threads, same))
+
+    def brief(v):
+        if not v:
+            return None
+        c = v["chosen"]
+        return {"L": c["L"], "recall10": round(float(c["recall"]), 4),
+                "sweep": v["sweep"]}
 
     out = {
         "metric": "QPS @ recall@10=0.95, 1M×128 PQ, learned-stop vs plain, 1×MI355X",
@@ -253,17 +306,20 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "1M x 128 Gaussian mixture (1024 comps, sigma 8), PQ 32x256 "
-                               "+ 4-bit residual refine, R=32 RNG graph, %d queries" % nq,
-                   "mode": mode, "L": L, "k": k, "refine_candidates": min(F, L),
-                   "entry_points": 1,
-                   "plain": {"L": L_plain, "qps": round(qps_plain, 1),
-                             "ms_per_step": round(ms_plain, 4), "sweep": plain_rows},
-                   "learned": None if L_learned is None else {
-                       "L": L_learned, "qps": round(qps_learned, 1),
-                       "ms_per_step": round(ms_learned, 4),
-                       "speedup_vs_plain": round(qps_learned / qps_plain, 3),
-                       "sweep": learned_rows, "training": train_info}},
+        "config": {
+            "workload": "1M x 128 Gaussian mixture (1024 comps, sigma 8); PQ 32x256 + "
+                        "%d-bit residuals; R=32 graph (exact 64-NN, RNG prune, backlinks); "
+                        "%d queries, k=%d" % (args.res_bits, nq, k),
+            "search": label, "mode": mode, "L": L,
+            "refine_candidates": refine_of(mode, L),
+            "seed_list": "medoid + %d k-means representatives" % (n_ent - 1),
+            "plain_qps": round(qps_plain, 1), "plain_L": L_plain,
+            "plain_ms_per_step": round(ms_plain, 4),
+            "learned_speedup_vs_plain": round(qps / qps_plain, 3) if learned else None,
+            "plain_modes": {m: brief(v) for m, v in plain.items()},
+            "learned": None if not learned else dict(brief(learned),
+                                                      training=learned["training"]),
+        },
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
@@ -272,5 +328,4 @@ def main():
 
 
 if __name__ == "__main__":
-    T0 = time.time()
     main()

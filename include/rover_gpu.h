@@ -211,6 +211,10 @@ typedef struct RoverSearchBatch {
     int refine_candidates;       /* PQ: 0 off, else refine the top F        */
     int entry_points;            /* seeds: graph entry + LCG                */
     uint32_t seed;               /* 0 -> ROVER_ENTRY_POINT_SEED             */
+    const int32_t *entry_ids;    /* explicit seed list shared by all queries
+                                    (PurePqSearchRequest.entry_ids); NULL ->
+                                    the LCG seeds above                     */
+    int entry_count;
     int expand_count;
     int32_t *out_ids;            /* [num_queries][out_stride]               */
     float *out_dists;            /* [num_queries][out_stride]               */
@@ -245,8 +249,11 @@ typedef struct RoverLearnedStopTrain {
     const float *queries;     /* host [num_queries][dim] training queries */
     int num_queries;
     int pool_size;            /* L of the trace searches                  */
+    int correction;           /* 0: ADC ranking, 1: online correction     */
     int label_k;              /* top-K whose stability is predicted       */
     int entry_points;
+    const int32_t *entry_ids; /* host seed list, or NULL -> LCG seeds      */
+    int entry_count;
     int expand_count;
     int stop_every;           /* feature/decision cadence (expansions)    */
     int min_steps;            /* no decision before this many expansions  */

@@ -417,7 +417,9 @@ AEEResult rover_prepare_launch(RoverCtx *c, const RoverSearchBatch *b,
                                int max_pool, int explicit_entries, RvIndex *ix,
                                RvBatch *rb) {
     uint32_t seed = b->seed ? b->seed : ROVER_ENTRY_POINT_SEED;
-    if (b->num_queries < 0 || b->entry_points <= 0 || b->expand_count <= 0 ||
+    if (b->entry_ids && b->entry_count <= 0) return ROVER_EBADPARM;
+    if (b->num_queries < 0 || (!b->entry_ids && b->entry_points <= 0) ||
+        b->expand_count <= 0 ||
         max_pool <= 0 || max_pool > ROVER_MAX_POOL || b->out_stride <= 0 ||
         !b->out_ids || !b->out_dists || !b->out_counts ||
         (b->num_queries > 0 && !b->queries) || b->refine_candidates < 0 ||
@@ -428,7 +430,7 @@ AEEResult rover_prepare_launch(RoverCtx *c, const RoverSearchBatch *b,
         snprintf(g_err, sizeof(g_err), "search: index not loaded");
         return ROVER_EFAILED;
     }
-    if (!explicit_entries &&
+    if (!explicit_entries && !b->entry_ids &&
         ensure_entries(c, ix->n, b->entry_points, ix->graph_entry, seed))
         return ROVER_EFAILED;
     memset(rb, 0, sizeof(*rb));
@@ -439,8 +441,8 @@ AEEResult rover_prepare_launch(RoverCtx *c, const RoverSearchBatch *b,
     rb->pool_size = b->pool_size;
     rb->num_out = b->num_out;
     rb->max_pool = max_pool;
-    rb->entries = c->d_entries;
-    rb->n_entries = c->entries_count;
+    rb->entries = b->entry_ids ? b->entry_ids : c->d_entries;
+    rb->n_entries = b->entry_ids ? b->entry_count : c->entries_count;
     rb->expand_count = b->expand_count;
     rb->scorer = b->method == ROVER_METHOD_SQ
                      ? RV_SCORE_SQ
@@ -521,7 +523,7 @@ static AEEResult search_host(RoverCtx *c, const RoverSearchBatch *b,
                              int explicit_entries) {
     RoverSearchBatch d;
     int dim, max_pool;
-    size_t oq, ol, ok, oi, od, oc, os, total;
+    size_t oq, ol, ok, oi, od, oc, os, oe, total;
     AEEResult rc = ROVER_EFAILED;
     if (!c || !b) return ROVER_EBADPARM;
     if (b->num_queries < 0 || (b->num_queries > 0 && !b->queries) ||
@@ -541,7 +543,8 @@ static AEEResult search_host(RoverCtx *c, const RoverSearchBatch *b,
     od = up256(oi + (size_t)b->num_queries * b->out_stride * 4);
     oc = up256(od + (size_t)b->num_queries * b->out_stride * 4);
     os = up256(oc + (size_t)b->num_queries * 4);
-    total = up256(os + (b->out_stats ? (size_t)b->num_queries * 32 : 0));
+    oe = up256(os + (b->out_stats ? (size_t)b->num_queries * 32 : 0));
+    total = up256(oe + (b->entry_ids ? (size_t)b->entry_count * 4 : 0));
     if (ensure_stage(c, total)) goto out;
     {
         char *base = (char *)c->d_stage;
@@ -553,6 +556,12 @@ static AEEResult search_host(RoverCtx *c, const RoverSearchBatch *b,
         d.out_dists = (float *)(base + od);
         d.out_counts = (int32_t *)(base + oc);
         d.out_stats = b->out_stats ? (uint64_t *)(base + os) : NULL;
+        d.entry_ids = b->entry_ids ? (const int32_t *)(base + oe) : NULL;
+        if (b->entry_ids && b->entry_count > 0 &&
+            rv_memcpy_h2d(base + oe, b->entry_ids, (size_t)b->entry_count * 4)) {
+            set_err("batch upload");
+            goto out;
+        }
         if (rv_memcpy_h2d(base + oq, b->queries,
                           (size_t)b->num_queries * dim * sizeof(float)) ||
             (b->pool_sizes &&

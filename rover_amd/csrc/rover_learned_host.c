@@ -154,7 +154,7 @@ AEEResult rover_learned_stop_train(remote_handle64 h, RoverLearnedStopTrain *t) 
     float *d_grad = NULL, *d_m1 = NULL, *d_m2 = NULL, *d_loss = NULL;
     float *d_mean = NULL, *d_istd = NULL, *d_od = NULL;
     int32_t *d_steps = NULL, *d_meta = NULL, *d_oid = NULL, *d_cnt = NULL;
-    int32_t *d_idx = NULL;
+    int32_t *d_idx = NULL, *d_ent = NULL;
     float *h_feat = NULL, *h_y = NULL, *h_w = NULL, *params = NULL;
     int32_t *h_meta = NULL;
     double sum[NF], sq[NF], cnt = 0.0, pos = 0.0;
@@ -184,7 +184,18 @@ AEEResult rover_learned_stop_train(remote_handle64 h, RoverLearnedStopTrain *t) 
     b.pool_size = t->pool_size;
     b.num_out = 1;
     b.method = ROVER_METHOD_PQ;
+    b.correction = t->correction ? 1 : 0;
     b.entry_points = t->entry_points > 0 ? t->entry_points : 1;
+    if (t->entry_ids) {
+        if (t->entry_count <= 0 ||
+            rv_malloc((void **)&d_ent, (size_t)t->entry_count * sizeof(int32_t)) ||
+            rv_memcpy_h2d(d_ent, t->entry_ids, (size_t)t->entry_count * sizeof(int32_t))) {
+            rover_set_err("learned stop: entry ids");
+            goto out;
+        }
+        b.entry_ids = d_ent;
+        b.entry_count = t->entry_count;
+    }
     b.expand_count = t->expand_count > 0 ? t->expand_count : c->degree;
     b.out_ids = d_oid;
     b.out_dists = d_od;
@@ -328,6 +339,7 @@ out:
     rover_dfree(d_m2);
     rover_dfree(d_loss);
     rover_dfree(d_idx);
+    rover_dfree(d_ent);
     free(h_feat);
     free(h_y);
     free(h_w);

@@ -523,6 +523,19 @@ __global__ __launch_bounds__(RV_WAVE) void rv_search_kernel(RvIndex ix, RvBatch 
         }
         ss.d_entry = __shfl(ss.d_entry, 0);
         if (status == 0 && n == 0) status = RV_QUERY_BAD_ENTRY;
+        // A long seed list would fill the visited hash with far-away seeds.
+        // Keep only the pool members then: a seed outside the pool that the
+        // search meets again is re-scored and rejected (its key is worse
+        // than the pool's worst), so results do not change; only the
+        // scored/dedup counters then differ from the reference's.
+        if (status == 0 && b.n_entries * 4 > hash_n) {
+            for (int i = lane; i < hash_n; i += RV_WAVE) hash[i] = RV_EMPTY_SLOT;
+            __syncthreads();
+            const uint64_t *k0 = keys_buf[cur];
+            for (int i = lane; i < n; i += RV_WAVE)
+                rv_visit(hash, b.hash_log2, b.probe_limit, (uint32_t)rv_key_id(k0[i]));
+            __syncthreads();
+        }
         if (status == 0) {
             const uint64_t *k0 = keys_buf[cur];
             ss.top1_0 = rv_key_dist(k0[0]);

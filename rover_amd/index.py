@@ -188,32 +188,96 @@ def encode_sq(x, bits, scale, zero):
 
 # ---------------------------------------------------------------- graph
 
-def _prune_rows(du, dpair, R):
+def _prune_rows(du, dpair, R, valid=None):
     """RNG occlusion over candidates sorted by (distance, id).
-    du [B, C] distance u->c, dpair [B, C, C] distance c_i->c_j.
-    Returns bool keep mask [B, C] with exactly min(R, C) True per row."""
+    du [B, C] distance u->c, dpair [B, C, C] distance c_i->c_j, valid [B, C]
+    marks real candidates (default all).  Returns bool keep mask [B, C] with
+    min(R, #valid) True per row: occlusion-survivors first
+    (retrieval_search_build_core.c:904-930), then filled from the remaining
+    candidates in order (:932-947)."""
     B, Cn = du.shape
+    if valid is None:
+        valid = np.ones((B, Cn), dtype=bool)
     keep = np.zeros((B, Cn), dtype=bool)
     count = np.zeros(B, dtype=np.int64)
     for i in range(Cn):
         occl = (keep[:, :i] & (dpair[:, :i, i] < du[:, i:i + 1])).any(axis=1)
-        acc = (~occl) & (count < R)
+        acc = (~occl) & (count < R) & valid[:, i]
         keep[:, i] = acc
         count += acc
-    # fill from the remaining candidates in order
     need = R - count
-    rest = ~keep
+    rest = ~keep & valid
     rank = np.cumsum(rest, axis=1)
     keep |= rest & (rank <= need[:, None])
     return keep
 
 
-def build_graph(x, R, candidates=None, device=None, batch=None):
-    """int32 [n, R] neighbour rows (distance, id order), -1 padded when n-1 < R."""
+def reverse_candidates(graph, x, R):
+    """For each node v, up to R nodes u with an edge u -> v, closest first
+    (the backlink lists of the reference's construction)."""
+    n, deg = graph.shape
+    src = np.repeat(np.arange(n, dtype=np.int64), deg)
+    dst = graph.ravel().astype(np.int64)
+    ok = dst >= 0
+    src, dst = src[ok], dst[ok]
+    diff = x[src].astype(np.float32) - x[dst].astype(np.float32)
+    dist = np.einsum("ij,ij->i", diff, diff)
+    order = np.lexsort((src, dist, dst))
+    src, dst = src[order], dst[order]
+    start = np.searchsorted(dst, np.arange(n))
+    rank = np.arange(len(dst)) - start[dst]
+    keep = rank < R
+    rev = np.full((n, R), -1, dtype=np.int32)
+    rev[dst[keep], rank[keep]] = src[keep]
+    return rev
+
+
+def _prune_candidate_rows(x, u_ids, cand, R):
+    """RNG-prunes candidate rows (may hold -1 and duplicates) for nodes
+    u_ids; returns int32 [len(u_ids), R] rows sorted by (distance, id)."""
+    B, Cn = cand.shape
+    xf = x
+    cand = cand.astype(np.int64)
+    valid = (cand >= 0) & (cand != u_ids[:, None])
+    # drop duplicates: sort by id, mark repeats
+    o = np.argsort(np.where(valid, cand, np.iinfo(np.int64).max), axis=1, kind="stable")
+    cs = np.take_along_axis(cand, o, 1)
+    vs = np.take_along_axis(valid, o, 1)
+    dup = np.zeros_like(vs)
+    dup[:, 1:] = cs[:, 1:] == cs[:, :-1]
+    vs &= ~dup
+    safe = np.where(vs, cs, 0)
+    diff = xf[safe] - xf[u_ids][:, None, :]
+    du = np.einsum("bcd,bcd->bc", diff, diff).astype(np.float32)
+    du = np.where(vs, du, np.float32(np.inf))
+    order = np.lexsort((np.where(vs, cs, np.iinfo(np.int64).max), du), axis=1)
+    cs = np.take_along_axis(safe, order, 1)
+    du = np.take_along_axis(du, order, 1)
+    vs = np.take_along_axis(vs, order, 1)
+    pts = xf[cs]
+    g = np.einsum("bid,bjd->bij", pts, pts)
+    sq = np.einsum("bid,bid->bi", pts, pts)
+    dpair = sq[:, :, None] + sq[:, None, :] - 2.0 * g
+    keep = _prune_rows(du, dpair, R, vs)
+    rows = np.full((B, R), -1, dtype=np.int32)
+    for i in range(B):
+        r = cs[i][keep[i]][:R]
+        rows[i, :len(r)] = r
+    return rows
+
+
+def build_graph(x, R, candidates=None, device=None, batch=None, backlinks=True):
+    """int32 [n, R] neighbour rows (distance, id order), -1 padded when fewer
+    candidates exist.  backlinks: after the kNN+RNG rows are built, each node
+    re-prunes its row together with up to R nodes that link to it, so that
+    nodes the occlusion rule left without in-edges become reachable."""
     n, dim = x.shape
     Cn = min(n - 1, candidates or 2 * R)
     if device is not None:
-        return _build_graph_torch(x, R, Cn, device, batch or 8192)
+        g = _build_graph_torch(x, R, Cn, device, batch or 8192)
+        if backlinks:
+            g = _backlinks_torch(x, g, R, device, batch or 8192)
+        return g
     graph = np.full((n, R), -1, dtype=np.int32)
     xf = x.astype(np.float32)
     norms = (xf * xf).sum(1)
@@ -238,7 +302,65 @@ def build_graph(x, R, candidates=None, device=None, batch=None):
         for i in range(e - s):
             row = cand[i][keep[i]][:R]
             graph[s + i, :len(row)] = row
-    return graph
+    if not backlinks:
+        return graph
+    rev = reverse_candidates(graph, xf, R)
+    out = np.full((n, R), -1, dtype=np.int32)
+    for s in range(0, n, step):
+        e = min(n, s + step)
+        cand = np.concatenate([graph[s:e], rev[s:e]], axis=1)
+        out[s:e] = _prune_candidate_rows(xf, np.arange(s, e), cand, R)
+    return out
+
+
+def _prune_torch(xt, u_ids, cand, R):
+    """GPU counterpart of _prune_candidate_rows: cand [B, C] int64 (may hold
+    -1 and duplicates), returns int32 rows [B, R] (distance, id order)."""
+    import torch
+    B, Cn = cand.shape
+    dev = xt.device
+    big = torch.iinfo(torch.int64).max
+    valid = (cand >= 0) & (cand != u_ids[:, None])
+    key = torch.where(valid, cand, torch.full_like(cand, big))
+    key, _ = torch.sort(key, dim=1)
+    dup = torch.zeros_like(valid)
+    dup[:, 1:] = key[:, 1:] == key[:, :-1]
+    valid = (key != big) & ~dup
+    cand = torch.where(valid, key, torch.zeros_like(key))
+    pts = xt[cand]
+    diff = pts - xt[u_ids][:, None, :]
+    du = (diff * diff).sum(-1)
+    du = torch.where(valid, du, torch.full_like(du, float("inf")))
+    # (distance, id) order: ids are already ascending, so a stable sort by
+    # distance breaks ties by id
+    o = torch.argsort(du, dim=1, stable=True)
+    cand = torch.gather(cand, 1, o)
+    du = torch.gather(du, 1, o)
+    valid = torch.gather(valid, 1, o)
+    pts = xt[cand].contiguous()
+    sq = (pts * pts).sum(-1)
+    g = torch.bmm(pts, pts.transpose(1, 2).contiguous())
+    dpair = sq[:, :, None] + sq[:, None, :] - 2.0 * g
+    keep = torch.zeros((B, Cn), dtype=torch.bool, device=dev)
+    count = torch.zeros(B, dtype=torch.int64, device=dev)
+    for i in range(Cn):
+        if i:
+            occl = (keep[:, :i] & (dpair[:, :i, i] < du[:, i:i + 1])).any(dim=1)
+        else:
+            occl = torch.zeros(B, dtype=torch.bool, device=dev)
+        acc = (~occl) & (count < R) & valid[:, i]
+        keep[:, i] = acc
+        count += acc.to(torch.int64)
+    need = R - count
+    rest = ~keep & valid
+    rank = torch.cumsum(rest.to(torch.int64), dim=1)
+    keep |= rest & (rank <= need[:, None])
+    pos = torch.cumsum(keep.to(torch.int64), dim=1) - 1
+    rows = torch.full((B, R), -1, dtype=torch.int32, device=dev)
+    sel = keep & (pos < R)
+    bi = torch.arange(B, device=dev)[:, None].expand(B, Cn)
+    rows[bi[sel], pos[sel]] = cand[sel].to(torch.int32)
+    return rows
 
 
 def _build_graph_torch(x, R, Cn, device, batch):
@@ -257,40 +379,50 @@ def _build_graph_torch(x, R, Cn, device, batch):
         del d
         diff = xt[cand] - xb[:, None, :]
         du = (diff * diff).sum(-1)
-        # sort by (distance, id): stable sort by id, then by distance
         o1 = torch.argsort(cand, dim=1, stable=True)
         cand = torch.gather(cand, 1, o1)
         du = torch.gather(du, 1, o1)
         o2 = torch.argsort(du, dim=1, stable=True)
         cand = torch.gather(cand, 1, o2)[:, :Cn].contiguous()
-        du = torch.gather(du, 1, o2)[:, :Cn].contiguous()
-        pts = xt[cand].contiguous()
-        sq = (pts * pts).sum(-1)
-        g = torch.bmm(pts, pts.transpose(1, 2).contiguous())
-        dpair = sq[:, :, None] + sq[:, None, :] - 2.0 * g
-        B = e - s
-        keep = torch.zeros((B, Cn), dtype=torch.bool, device=device)
-        count = torch.zeros(B, dtype=torch.int64, device=device)
-        for i in range(Cn):
-            if i:
-                occl = (keep[:, :i] & (dpair[:, :i, i] < du[:, i:i + 1])).any(dim=1)
-            else:
-                occl = torch.zeros(B, dtype=torch.bool, device=device)
-            acc = (~occl) & (count < R)
-            keep[:, i] = acc
-            count += acc.to(torch.int64)
-        need = R - count
-        rest = ~keep
-        rank = torch.cumsum(rest.to(torch.int64), dim=1)
-        keep |= rest & (rank <= need[:, None])
-        # compact kept candidates, preserving order
-        pos = torch.cumsum(keep.to(torch.int64), dim=1) - 1
-        rows = torch.full((B, R), -1, dtype=torch.int32, device=device)
-        sel = keep & (pos < R)
-        bi = torch.arange(B, device=device)[:, None].expand(B, Cn)
-        rows[bi[sel], pos[sel]] = cand[sel].to(torch.int32)
-        graph[s:e] = rows
+        graph[s:e] = _prune_torch(xt, torch.arange(s, e, device=device), cand, R)
     return graph.cpu().numpy()
+
+
+def _backlinks_torch(x, graph, R, device, batch):
+    """Each node re-prunes its row together with up to R closest nodes that
+    link to it (GPU)."""
+    import torch
+    n, deg = graph.shape
+    xt = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).to(device)
+    g = torch.from_numpy(graph).to(device).to(torch.int64)
+    src = torch.arange(n, device=device).repeat_interleave(deg)
+    dst = g.reshape(-1)
+    ok = dst >= 0
+    src, dst = src[ok], dst[ok]
+    dist = torch.empty(src.shape[0], device=device)
+    step = 1 << 22
+    for s in range(0, src.shape[0], step):
+        a, b = src[s:s + step], dst[s:s + step]
+        df = xt[a] - xt[b]
+        dist[s:s + step] = (df * df).sum(-1)
+    # order by (dst, dist, src) with stable sorts from the last key
+    o = torch.argsort(src, stable=True)
+    src, dst, dist = src[o], dst[o], dist[o]
+    o = torch.argsort(dist, stable=True)
+    src, dst, dist = src[o], dst[o], dist[o]
+    o = torch.argsort(dst, stable=True)
+    src, dst = src[o], dst[o]
+    start = torch.searchsorted(dst, torch.arange(n, device=device))
+    rank = torch.arange(dst.shape[0], device=device) - start[dst]
+    keep = rank < R
+    rev = torch.full((n, R), -1, dtype=torch.int64, device=device)
+    rev[dst[keep], rank[keep]] = src[keep]
+    out = torch.full((n, R), -1, dtype=torch.int32, device=device)
+    for s in range(0, n, batch):
+        e = min(n, s + batch)
+        cand = torch.cat([g[s:e], rev[s:e]], dim=1)
+        out[s:e] = _prune_torch(xt, torch.arange(s, e, device=device), cand, R)
+    return out.cpu().numpy()
 
 
 def medoid(x):
@@ -301,13 +433,73 @@ def medoid(x):
     return int(np.argmin(d))
 
 
+def entry_list(x, count, entry, device=None, iters=10, sample=None):
+    """Shared seed list: the medoid, then the base vector nearest each of
+    `count` k-means centroids (deduplicated, in centroid order).
+
+    A degree-R graph built from exact nearest neighbours of well-separated
+    clusters has no edges between clusters; seeding every query with one
+    node per k-means cell lets the best-first search start in the query's
+    cluster.  The list is passed as the request's entry ids, exactly like the
+    reference's seed list (PurePqSearchRequest.entry_ids)."""
+    n = x.shape[0]
+    count = min(count, n)
+    rows = uniform_stride_sample(n, min(n, sample or max(64 * count, 20000)))
+    tr = np.ascontiguousarray(x[rows], dtype=np.float32)
+    init = (np.arange(count, dtype=np.int64) * (len(tr) - 1)) // max(count - 1, 1)
+    if device is not None:
+        import torch
+        t = torch.from_numpy(tr).to(device)
+        cent = t[torch.from_numpy(init).to(device)].clone()
+        for _ in range(iters):
+            d = (t * t).sum(1, keepdim=True) - 2.0 * t @ cent.T + (cent * cent).sum(1)[None]
+            a = torch.argmin(d, dim=1)
+            sums = torch.zeros_like(cent).index_add_(0, a, t)
+            cnt = torch.bincount(a, minlength=count).to(t.dtype)
+            nz = cnt > 0
+            cent[nz] = sums[nz] / cnt[nz, None]
+        xt = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).to(device)
+        xn = (xt * xt).sum(1)
+        best = torch.full((count,), float("inf"), device=device)
+        arg = torch.zeros(count, dtype=torch.int64, device=device)
+        step = 1 << 18
+        for s in range(0, n, step):
+            d = xn[None, s:s + step] - 2.0 * cent @ xt[s:s + step].T
+            v, i = torch.min(d, dim=1)
+            upd = v < best
+            best = torch.where(upd, v, best)
+            arg = torch.where(upd, i + s, arg)
+        reps = arg.cpu().numpy()
+    else:
+        cent = tr[init].copy()
+        for _ in range(iters):
+            d = (tr * tr).sum(1)[:, None] - 2.0 * tr @ cent.T + (cent * cent).sum(1)[None]
+            a = np.argmin(d, axis=1)
+            sums = np.zeros_like(cent, dtype=np.float64)
+            np.add.at(sums, a, tr)
+            cnt = np.bincount(a, minlength=count)
+            nz = cnt > 0
+            cent[nz] = (sums[nz] / cnt[nz, None]).astype(np.float32)
+        xf = np.ascontiguousarray(x, dtype=np.float32)
+        d = (xf * xf).sum(1)[None, :] - 2.0 * cent @ xf.T
+        reps = np.argmin(d, axis=1)
+    out, seen = [int(entry)], {int(entry)}
+    for r in reps:
+        if int(r) not in seen:
+            seen.add(int(r))
+            out.append(int(r))
+    return np.array(out, dtype=np.int32)
+
+
 class PqIndexData:
     """Arrays of a pure-PQ index (all host numpy)."""
 
     def __init__(self, graph, entry, codes, codebook, res_codes=None,
-                 res_qparams=None, res_bits=4):
+                 res_qparams=None, res_bits=4, entries=None):
         self.graph = np.ascontiguousarray(graph, dtype=np.int32)
         self.entry = int(entry)
+        # shared seed list (medoid first); None -> the graph entry alone
+        self.entries = None if entries is None else np.ascontiguousarray(entries, dtype=np.int32)
         self.codes = np.ascontiguousarray(codes, dtype=np.uint8)
         self.codebook = np.ascontiguousarray(codebook, dtype=np.float32)
         self.res_codes = None if res_codes is None else np.ascontiguousarray(res_codes)
@@ -336,7 +528,9 @@ class PqIndexData:
 
 
 def build_pq_index(x, m, R, ks=256, res_bits=4, iters=20, train_max=100000,
-                   device=None, graph=None):
+                   device=None, graph=None, n_entries=0):
+    """n_entries > 0 adds a shared seed list of the medoid plus n_entries
+    k-means representatives (see entry_list)."""
     codebook = train_pq(x, m, ks, iters=iters, train_max=train_max, device=device)
     if device is not None and x.shape[0] > 100000:
         codes, res, qp = encode_on_device(x, codebook, res_bits)
@@ -345,7 +539,9 @@ def build_pq_index(x, m, R, ks=256, res_bits=4, iters=20, train_max=100000,
         res, qp = encode_residual(x, codes, codebook, res_bits)
     if graph is None:
         graph = build_graph(x, R, device=device)
-    return PqIndexData(graph, medoid(x), codes, codebook, res, qp, res_bits)
+    entry = medoid(x)
+    entries = entry_list(x, n_entries, entry, device=device) if n_entries > 0 else None
+    return PqIndexData(graph, entry, codes, codebook, res, qp, res_bits, entries)
 
 
 def encode_on_device(x, codebook, res_bits):

@@ -60,7 +60,8 @@ class RoverSearchBatch(C.Structure):
                 ("pool_size", C.c_int), ("num_out", C.c_int),
                 ("method", C.c_int), ("correction", C.c_int),
                 ("refine_candidates", C.c_int), ("entry_points", C.c_int),
-                ("seed", C.c_uint32), ("expand_count", C.c_int),
+                ("seed", C.c_uint32), ("entry_ids", C.c_void_p),
+                ("entry_count", C.c_int), ("expand_count", C.c_int),
                 ("out_ids", C.c_void_p), ("out_dists", C.c_void_p),
                 ("out_counts", C.c_void_p), ("out_stride", C.c_int),
                 ("out_stats", C.c_void_p), ("hash_log2", C.c_int),
@@ -70,8 +71,10 @@ class RoverSearchBatch(C.Structure):
 
 class RoverLearnedStopTrain(C.Structure):
     _fields_ = [("queries", C.c_void_p), ("num_queries", C.c_int),
-                ("pool_size", C.c_int), ("label_k", C.c_int),
-                ("entry_points", C.c_int), ("expand_count", C.c_int),
+                ("pool_size", C.c_int), ("correction", C.c_int),
+                ("label_k", C.c_int),
+                ("entry_points", C.c_int), ("entry_ids", C.c_void_p),
+                ("entry_count", C.c_int), ("expand_count", C.c_int),
                 ("stop_every", C.c_int), ("min_steps", C.c_int),
                 ("trace_cap", C.c_int), ("steps", C.c_int), ("batch", C.c_int),
                 ("lr", C.c_float), ("pos_weight", C.c_float),

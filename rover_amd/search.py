@@ -123,8 +123,10 @@ class Calculator:
                       entry_points, expand_count, out_ids, out_dists,
                       out_counts, stride, out_stats, hash_log2, probe_limit,
                       grid_waves, learned_stop, stop_threshold, pool_sizes,
-                      num_outs, seed):
+                      num_outs, seed, entry_ids=None, entry_count=0):
         b = native.RoverSearchBatch()
+        b.entry_ids = entry_ids
+        b.entry_count = int(entry_count)
         b.queries = queries
         b.num_queries = nq
         b.pool_sizes = pool_sizes
@@ -153,8 +155,9 @@ class Calculator:
                      correction=0, refine_candidates=0, entry_points=1,
                      expand_count=32, stats=False, hash_log2=0, probe_limit=0,
                      grid_waves=0, learned_stop=0, stop_threshold=0.0,
-                     seed=native.ENTRY_POINT_SEED):
-        """Host arrays in and out.  pool_size / num_out: int or per-query
+                     seed=native.ENTRY_POINT_SEED, entry_ids=None):
+        """Host arrays in and out.  entry_ids: explicit seed list shared by
+        all queries (otherwise graph entry + LCG seeds).  pool_size / num_out: int or per-query
         arrays.  Returns ids [nq, k_max], dists, counts (negative = per-query
         error status) and, with stats=True, [nq, 4] counters."""
         q = np.ascontiguousarray(queries, dtype=np.float32)
@@ -167,11 +170,13 @@ class Calculator:
         d = np.zeros((nq, Kmax), dtype=np.float32)
         cnt = np.zeros(nq, dtype=np.int32)
         st = np.zeros((nq, 4), dtype=np.uint64) if stats else None
+        e = None if entry_ids is None else np.ascontiguousarray(entry_ids, dtype=np.int32)
         b = self._batch_struct(ptr(q), nq, Lmax, Kmax, method, correction,
                                refine_candidates, entry_points, expand_count,
                                ptr(ids), ptr(d), ptr(cnt), Kmax, ptr(st),
                                hash_log2, probe_limit, grid_waves, learned_stop,
-                               stop_threshold, ptr(Ls), ptr(Ks), seed)
+                               stop_threshold, ptr(Ls), ptr(Ks), seed, ptr(e),
+                               0 if e is None else len(e))
         check(native.lib().rover_search_batch(self.h, C.byref(b)), "rover_search_batch")
         return (ids, d, cnt, st) if stats else (ids, d, cnt)
 
@@ -182,7 +187,8 @@ class Calculator:
                             probe_limit=0, grid_waves=0, learned_stop=0,
                             stop_threshold=0.0, d_pool_sizes=None,
                             d_num_outs=None, stream=None,
-                            seed=native.ENTRY_POINT_SEED):
+                            seed=native.ENTRY_POINT_SEED, d_entry_ids=None,
+                            entry_count=0):
         """Device pointers in and out; asynchronous.  With per-query device
         arrays, pool_size must be their maximum."""
         b = self._batch_struct(d_queries, nq, pool_size, num_out, method,
@@ -190,7 +196,7 @@ class Calculator:
                                expand_count, d_ids, d_dists, d_counts, stride,
                                d_stats, hash_log2, probe_limit, grid_waves,
                                learned_stop, stop_threshold, d_pool_sizes,
-                               d_num_outs, seed)
+                               d_num_outs, seed, d_entry_ids, entry_count)
         check(native.lib().rover_search_batch_device(self.h, C.byref(b), stream),
               "rover_search_batch_device")
 
