@@ -290,8 +290,9 @@ def main():
         if not v:
             return None
         c = v["chosen"]
-        return {"L": c["L"], "recall10": round(float(c["recall"]), 4),
-                "sweep": v["sweep"]}
+        return {"L": c["L"] if c else None,
+                "recall10": round(float(c["recall"]), 4) if c else None,
+                "reached_target": c is not None, "sweep": v["sweep"]}
 
     out = {
         "metric": "QPS @ recall@10=0.95, 1M×128 PQ, learned-stop vs plain, 1×MI355X",

@@ -172,9 +172,69 @@ __device__ __forceinline__ float rv_decode_value(float c, int code, float s,
     return sum + z;
 }
 
+// Online correction for the common shape: M subspaces of SUB dims with
+// 8-bit PQ codes (ks = 256).  All code bytes, residual bytes and centroid
+// vectors of the candidate are loaded before the sequential accumulation,
+// so a candidate costs one round of independent loads instead of M
+// dependent ones.  Arithmetic order is the reference's.
+template <int M, int SUB>
+__device__ __forceinline__ bool rv_online_fixed(const RvIndex &ix, const float *q,
+                                                int id, float *out) {
+    constexpr int D = M * SUB;
+    const float s = ix.res_qparams[2 * (size_t)id];
+    const float z = ix.res_qparams[2 * (size_t)id + 1];
+    uint32_t pw[M / 4];
+    const uint32_t *prow = reinterpret_cast<const uint32_t *>(ix.codes + (size_t)id * M);
+#pragma unroll
+    for (int w = 0; w < M / 4; w++) pw[w] = prow[w];
+    constexpr int RW = D / 4;  // 8-bit residual words; 4-bit uses half
+    uint32_t rw[RW];
+    const uint32_t *rrow = reinterpret_cast<const uint32_t *>(ix.res_codes + (size_t)id * ix.res_code_bytes);
+    const int rwords = ix.res_bits == 4 ? D / 8 : D / 4;
+#pragma unroll
+    for (int w = 0; w < RW; w++) rw[w] = w < rwords ? rrow[w] : 0u;
+    float c[D];
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        const int code = (pw[m >> 2] >> (8 * (m & 3))) & 255;
+        const float *cent = ix.codebook + ((size_t)m * 256 + code) * SUB;
+        if (SUB == 4) {
+            const float4 v = *reinterpret_cast<const float4 *>(cent);
+            c[m * 4 + 0] = v.x;
+            c[m * 4 + 1] = v.y;
+            c[m * 4 + 2] = v.z;
+            c[m * 4 + 3] = v.w;
+        } else if (SUB == 2) {
+            const float2 v = *reinterpret_cast<const float2 *>(cent);
+            c[m * 2 + 0] = v.x;
+            c[m * 2 + 1] = v.y;
+        } else {
+#pragma unroll
+            for (int i = 0; i < SUB; i++) c[m * SUB + i] = cent[i];
+        }
+    }
+    if (!isfinite(s) || !isfinite(z) || !(s > 0.0f)) return false;
+    float dist = 0.0f;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        int code;
+        if (ix.res_bits == 4) code = (rw[d >> 3] >> (4 * (d & 7))) & 15;
+        else code = (rw[d >> 2] >> (8 * (d & 3))) & 255;
+        const float rec = rv_decode_value(c[d], code, s, z);
+        const float diff = q[d] - rec;
+        dist += diff * diff;
+    }
+    *out = dist;
+    return true;
+}
+
 // Online correction distance; returns false for invalid qparams.
 __device__ bool rv_online(const RvIndex &ix, const float *q, int id,
                           float *out) {
+    if (ix.ks == 256 && (ix.res_bits == 8 || ix.res_bits == 4)) {
+        if (ix.m == 32 && ix.subdim == 4) return rv_online_fixed<32, 4>(ix, q, id, out);
+        if (ix.m == 48 && ix.subdim == 2) return rv_online_fixed<48, 2>(ix, q, id, out);
+    }
     const uint8_t *pq = ix.codes + (size_t)id * ix.code_bytes;
     const uint8_t *res = ix.res_codes + (size_t)id * ix.res_code_bytes;
     const float s = ix.res_qparams[2 * (size_t)id];
