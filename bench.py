@@ -84,6 +84,13 @@ def main():
     ap.add_argument("--stop-every", type=int, default=2)
     ap.add_argument("--trace-cap", type=int, default=512)
     ap.add_argument("--train-steps", type=int, default=2000)
+    ap.add_argument("--save-index", default="",
+                    help="write the built index, queries and ground truth (.npz)")
+    ap.add_argument("--load-index", default="",
+                    help="reuse an index written by --save-index (no rebuild)")
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="also run one plain copy kernel of this many MiB (a "
+                         "known-traffic reference for profiler byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -92,20 +99,46 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    import torch
-    dev = torch.device("cuda:0")
-    log("device", torch.cuda.get_device_name(0))
-    base, queries = data.workload(args.n, args.dim, args.components, 8.0, args.nq)
-    log("data", base.shape, queries.shape)
-    ix = index.build_pq_index(base, m=args.m, R=args.R, res_bits=args.res_bits,
-                              iters=20, train_max=100000, device=dev,
-                              n_entries=args.entries)
-    log("index built: graph", ix.graph.shape, "medoid", ix.entry, "seed list",
-        len(ix.entries))
-    gt = data.ground_truth(base, queries, k=100, device=dev)
-    log("ground truth done")
-    torch.cuda.synchronize()
-    torch.cuda.empty_cache()
+    if args.load_index:
+        z = np.load(args.load_index, allow_pickle=False)
+        ix = index.PqIndexData(z["graph"], int(z["entry"]), z["codes"], z["codebook"],
+                               z["res_codes"], z["res_qparams"], int(z["res_bits"]),
+                               z["entries"])
+        queries, gt = z["queries"], z["gt"]
+        assert queries.shape[0] == args.nq and ix.res_bits == args.res_bits
+        log("index loaded from", args.load_index)
+    else:
+        import torch
+        dev = torch.device("cuda:0")
+        log("device", torch.cuda.get_device_name(0))
+        base, queries = data.workload(args.n, args.dim, args.components, 8.0, args.nq)
+        log("data", base.shape, queries.shape)
+        ix = index.build_pq_index(base, m=args.m, R=args.R, res_bits=args.res_bits,
+                                  iters=20, train_max=100000, device=dev,
+                                  n_entries=args.entries)
+        log("index built: graph", ix.graph.shape, "medoid", ix.entry, "seed list",
+            len(ix.entries))
+        gt = data.ground_truth(base, queries, k=100, device=dev)
+        log("ground truth done")
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        if args.save_index:
+            np.savez(args.save_index, graph=ix.graph, entry=np.int32(ix.entry),
+                     codes=ix.codes, codebook=ix.codebook, res_codes=ix.res_codes,
+                     res_qparams=ix.res_qparams, res_bits=np.int32(ix.res_bits),
+                     entries=ix.entries, queries=queries, gt=gt)
+            log("index saved to", args.save_index)
+        del base
+
+    if args.calibrate_copy_mb:
+        nb = args.calibrate_copy_mb << 20
+        a, b2 = native.DeviceBuffer(nb), native.DeviceBuffer(nb)
+        native.check(native.lib().rover_device_copy(b2.ptr, a.ptr, nb), "copy")
+        native.synchronize()
+        log("calibration copy: %d MiB read + %d MiB written" % (args.calibrate_copy_mb,
+                                                                args.calibrate_copy_mb))
+        a.close()
+        b2.close()
 
     calc = Calculator()
     calc.load_pq_index(ix)
@@ -193,6 +226,7 @@ def main():
     if not args.no_learned:
         train_q = data.gaussian_mixture(args.train_queries, args.dim, args.components,
                                         8.0, 44, centers_seed=42)
+        # the mixture's centres depend only on centers_seed, as for the base
         t = native.RoverLearnedStopTrain()
         t.queries = train_q.ctypes.data
         t.num_queries = train_q.shape[0]

@@ -315,6 +315,9 @@ AEEResult rover_device_free(void *d_ptr);
 AEEResult rover_device_upload(void *d_dst, const void *src, size_t bytes);
 AEEResult rover_device_download(void *dst, const void *d_src, size_t bytes);
 AEEResult rover_device_synchronize(void);
+/* Device-to-device copy by a plain vector-copy kernel (16-byte aligned
+ * sizes and pointers); its traffic is exactly 2 * bytes. */
+AEEResult rover_device_copy(void *d_dst, const void *d_src, size_t bytes);
 int rover_device_count(void);
 const char *rover_last_error(void);
 

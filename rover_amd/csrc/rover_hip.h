@@ -158,6 +158,7 @@ int rv_sample_indices(int32_t *d_idx, int n, size_t population, uint32_t seed,
 const char *rv_learned_error(void);
 uint16_t rv_bf16_round_host(float f);
 int rv_fill_zero(void *d_ptr, size_t bytes, void *stream);
+int rv_copy(void *d_dst, const void *d_src, size_t bytes, void *stream);
 
 #ifdef __cplusplus
 }

@@ -904,3 +904,12 @@ AEEResult rover_stream_synchronize(void *stream) {
     }
     return ROVER_SUCCESS;
 }
+
+AEEResult rover_device_copy(void *d_dst, const void *d_src, size_t bytes) {
+    if (!d_dst || !d_src) return ROVER_EBADPARM;
+    if (rv_copy(d_dst, d_src, bytes, NULL)) {
+        set_err("rover_device_copy");
+        return ROVER_EFAILED;
+    }
+    return ROVER_SUCCESS;
+}

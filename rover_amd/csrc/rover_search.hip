@@ -1057,6 +1057,27 @@ extern "C" int rv_residual_encode(const float *x, const uint8_t *codes, int n,
     return rv_check(hipGetLastError(), "rv_residual_encode");
 }
 
+// ------------------------------------------------------------ copy kernel
+
+// Plain 16-byte-vector copy; a kernel of known traffic for calibrating the
+// HBM byte counters of a profiler.
+__global__ void rv_copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                               size_t n16) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n16;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+extern "C" int rv_copy(void *dst, const void *src, size_t bytes, void *stream) {
+    if ((bytes & 15) || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15)) {
+        snprintf(g_rv_error, sizeof(g_rv_error), "rv_copy: 16-byte alignment required");
+        return -1;
+    }
+    rv_copy_kernel<<<4096, 256, 0, (hipStream_t)stream>>>((const uint4 *)src,
+                                                          (uint4 *)dst, bytes / 16);
+    return rv_check(hipGetLastError(), "rv_copy");
+}
+
 // ------------------------------------------------------ events and streams
 
 extern "C" int rv_event_create(void **ev) {

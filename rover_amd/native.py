@@ -127,6 +127,7 @@ SIGNATURES = [
     ("rover_device_upload", _I, [_P, _P, C.c_size_t]),
     ("rover_device_download", _I, [_P, _P, C.c_size_t]),
     ("rover_device_synchronize", _I, []),
+    ("rover_device_copy", _I, [_P, _P, C.c_size_t]),
     ("rover_device_count", _I, []),
     ("rover_last_error", C.c_char_p, []),
     ("rover_event_create", _I, [C.POINTER(_P)]),
