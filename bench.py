@@ -67,9 +67,11 @@ def main():
     ap.add_argument("--R", type=int, default=32)
     ap.add_argument("--components", type=int, default=1024)
     ap.add_argument("--res-bits", type=int, default=8)
-    ap.add_argument("--entries", type=int, default=4096,
-                    help="k-means representatives in the shared seed list")
-    ap.add_argument("--Ls", default="16,24,32,48,64,96,128,192,256")
+    ap.add_argument("--router", type=int, default=4096,
+                    help="entry-router centroids (k-means over the base)")
+    ap.add_argument("--route", type=int, default=4,
+                    help="entries per query: representatives of its nearest centroids")
+    ap.add_argument("--Ls", default="8,10,12,16,20,24,32,48,64,96,128")
     ap.add_argument("--k", type=int, default=10)
     ap.add_argument("--refine", type=int, default=64,
                     help="refine candidates F of the ADC mode")
@@ -103,7 +105,8 @@ def main():
         z = np.load(args.load_index, allow_pickle=False)
         ix = index.PqIndexData(z["graph"], int(z["entry"]), z["codes"], z["codebook"],
                                z["res_codes"], z["res_qparams"], int(z["res_bits"]),
-                               z["entries"])
+                               router_centroids=z["router_centroids"],
+                               router_ids=z["router_ids"])
         queries, gt = z["queries"], z["gt"]
         assert queries.shape[0] == args.nq and ix.res_bits == args.res_bits
         log("index loaded from", args.load_index)
@@ -115,9 +118,9 @@ def main():
         log("data", base.shape, queries.shape)
         ix = index.build_pq_index(base, m=args.m, R=args.R, res_bits=args.res_bits,
                                   iters=20, train_max=100000, device=dev,
-                                  n_entries=args.entries)
-        log("index built: graph", ix.graph.shape, "medoid", ix.entry, "seed list",
-            len(ix.entries))
+                                  n_router=args.router)
+        log("index built: graph", ix.graph.shape, "medoid", ix.entry, "router",
+            ix.router_centroids.shape)
         gt = data.ground_truth(base, queries, k=100, device=dev)
         log("ground truth done")
         torch.cuda.synchronize()
@@ -126,7 +129,8 @@ def main():
             np.savez(args.save_index, graph=ix.graph, entry=np.int32(ix.entry),
                      codes=ix.codes, codebook=ix.codebook, res_codes=ix.res_codes,
                      res_qparams=ix.res_qparams, res_bits=np.int32(ix.res_bits),
-                     entries=ix.entries, queries=queries, gt=gt)
+                     router_centroids=ix.router_centroids, router_ids=ix.router_ids,
+                     queries=queries, gt=gt)
             log("index saved to", args.save_index)
         del base
 
@@ -148,8 +152,7 @@ def main():
     d_d = native.DeviceBuffer(nq * k * 4)
     d_cnt = native.DeviceBuffer(nq * 4)
     d_st = native.DeviceBuffer(nq * 32)
-    d_ent = native.DeviceBuffer.from_array(ix.entries)
-    n_ent = len(ix.entries)
+    S = args.route
 
     def refine_of(mode, L):
         return 0 if mode == "online" else min(args.refine, L)
@@ -161,8 +164,7 @@ def main():
                                  expand_count=args.R,
                                  d_stats=d_st.ptr if stats else None,
                                  learned_stop=1 if learned else 0,
-                                 stop_threshold=0.9, d_entry_ids=d_ent.ptr,
-                                 entry_count=n_ent)
+                                 stop_threshold=0.9, route_entries=S)
 
     def sweep(mode, learned):
         rows, chosen = [], None
@@ -234,8 +236,7 @@ def main():
         t.correction = 1 if mode == "online" else 0
         t.label_k = args.label_k
         t.entry_points = 1
-        t.entry_ids = ix.entries.ctypes.data
-        t.entry_count = n_ent
+        t.route_entries = S
         t.expand_count = args.R
         t.stop_every = args.stop_every
         t.min_steps = args.stop_every
@@ -294,15 +295,17 @@ def main():
         oix = pyoracle.Index.from_pq(ix)
         scorer = pyoracle.SCORE_ONLINE if mode == "online" else pyoracle.SCORE_ADC
         threads = min(16, os.cpu_count() or 1)
+        # the same per-query entries the GPU search uses (router output)
+        ents = calc.route_entries(queries, S)
         probe = min(nq, 16 * threads)
         t1 = time.time()
-        pyoracle.search_batch(oix, scorer, queries[:probe], ix.entries, args.R, L_plain,
+        pyoracle.search_batch(oix, scorer, queries[:probe], ents[:probe], args.R, L_plain,
                               k, refine=refine_of(mode, L_plain), threads=threads)
         rate = probe / max(time.time() - t1, 1e-6)
         sample = int(min(nq, max(probe, rate * args.cpu_seconds)))
         t1 = time.time()
-        oids, od, ocnt, _ = pyoracle.search_batch(oix, scorer, queries[:sample], ix.entries,
-                                                  args.R, L_plain, k,
+        oids, od, ocnt, _ = pyoracle.search_batch(oix, scorer, queries[:sample],
+                                                  ents[:sample], args.R, L_plain, k,
                                                   refine=refine_of(mode, L_plain),
                                                   threads=threads)
         el = time.time() - t1
@@ -347,7 +350,8 @@ def main():
                         "%d queries, k=%d" % (args.res_bits, nq, k),
             "search": label, "mode": mode, "L": L,
             "refine_candidates": refine_of(mode, L),
-            "seed_list": "medoid + %d k-means representatives" % (n_ent - 1),
+            "entries": "per query: representatives of its %d nearest of %d k-means "
+                       "router centroids" % (S, len(ix.router_ids)),
             "plain_qps": round(qps_plain, 1), "plain_L": L_plain,
             "plain_ms_per_step": round(ms_plain, 4),
             "learned_speedup_vs_plain": round(qps / qps_plain, 3) if learned else None,

@@ -107,6 +107,20 @@ AEEResult rover_pure_pq_residual_load(remote_handle64 h,
                                       const float *qparams, int64_t qparamsLen,
                                       int32_t residual_bits);
 
+/* Entry router (no reference counterpart): `count` centroids
+ * [count][dim] and one representative node id per centroid.  A batch with
+ * route_entries = S seeds each query with the representatives of its S
+ * nearest centroids (exact squared L2, ties by centroid index), passed to
+ * the search as that query's entry ids. */
+AEEResult rover_pure_pq_router_load(remote_handle64 h, const float *centroids,
+                                    int centroidsLen, const int32_t *rep_ids,
+                                    int count);
+
+/* The routing step alone, device pointers: out[nq][S] node ids. */
+AEEResult rover_route_entries(remote_handle64 h, const float *d_queries,
+                              int num_queries, int S, int32_t *d_out,
+                              void *stream);
+
 /* Optional graph->original id map used by the refine tie-break
  * (retrieval_final_refine.c:212-219); NULL restores the identity. */
 AEEResult rover_pure_pq_original_ids(remote_handle64 h, const int32_t *map,
@@ -211,10 +225,16 @@ typedef struct RoverSearchBatch {
     int refine_candidates;       /* PQ: 0 off, else refine the top F        */
     int entry_points;            /* seeds: graph entry + LCG                */
     uint32_t seed;               /* 0 -> ROVER_ENTRY_POINT_SEED             */
-    const int32_t *entry_ids;    /* explicit seed list shared by all queries
-                                    (PurePqSearchRequest.entry_ids); NULL ->
-                                    the LCG seeds above                     */
+    const int32_t *entry_ids;    /* explicit seeds (PurePqSearchRequest.
+                                    entry_ids): one list shared by all
+                                    queries, or [num_queries][entry_stride]
+                                    rows; NULL -> the LCG seeds above       */
     int entry_count;
+    int entry_stride;            /* 0: shared list; > 0: per-query rows     */
+    int route_entries;           /* > 0: each query's entries are the S
+                                    nearest router centroids' nodes
+                                    (rover_pure_pq_router_load); overrides
+                                    entry_ids                               */
     int expand_count;
     int32_t *out_ids;            /* [num_queries][out_stride]               */
     float *out_dists;            /* [num_queries][out_stride]               */
@@ -254,6 +274,8 @@ typedef struct RoverLearnedStopTrain {
     int entry_points;
     const int32_t *entry_ids; /* host seed list, or NULL -> LCG seeds      */
     int entry_count;
+    int route_entries;        /* > 0: route each training query (as in a
+                                 search batch); overrides entry_ids        */
     int expand_count;
     int stop_every;           /* feature/decision cadence (expansions)    */
     int min_steps;            /* no decision before this many expansions  */

@@ -67,8 +67,8 @@ def oracle():
         L.orc_recall_hits.argtypes = [P, I, P, I, I]
         L.orc_fingerprint.argtypes = [C.c_uint64, P, P, I]
         L.orc_fingerprint.restype = C.c_uint64
-        L.orc_search_batch.argtypes = [P, I, P, I, P, I, I, P, P, I, I, I, P, P,
-                                       P, P]
+        L.orc_search_batch.argtypes = [P, I, P, I, P, I, I, I, P, P, I, I, I, P,
+                                       P, P, P]
         _orc = L
     return _orc
 
@@ -160,7 +160,8 @@ def search(index, scorer, query, entries, expand, L, num_out):
 
 def search_batch(index, scorer, queries, entries, expand, L, k, refine=0,
                  threads=None):
-    """Many queries on host threads; L and k scalars or per-query arrays.
+    """Many queries on host threads; L and k scalars or per-query arrays;
+    entries a shared list or an [nq, S] array of per-query rows.
     Returns ids [nq, stride], dists, counts, stats [nq, 4]."""
     q = np.ascontiguousarray(queries, dtype=np.float32)
     nq = q.shape[0]
@@ -173,9 +174,11 @@ def search_batch(index, scorer, queries, entries, expand, L, k, refine=0,
     cnt = np.zeros(nq, dtype=np.int32)
     st = (OrcStats * max(nq, 1))()
     threads = threads or os.cpu_count() or 1
+    estride = e.shape[1] if e.ndim == 2 else 0
     oracle().orc_search_batch(C.byref(index.struct), scorer, _p(q), nq, _p(e),
-                              len(e), expand, _p(Ls), _p(ks), stride, refine,
-                              threads, _p(ids), _p(d), _p(cnt), st)
+                              e.shape[1] if estride else len(e), estride, expand,
+                              _p(Ls), _p(ks), stride, refine, threads, _p(ids),
+                              _p(d), _p(cnt), st)
     stats = np.array([(s.pq_scored_candidates, s.corrected_candidates,
                        s.expanded_nodes, s.visited_dedup_skips) for s in st[:nq]],
                      dtype=np.uint64).reshape(nq, 4)

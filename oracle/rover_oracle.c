@@ -681,7 +681,7 @@ typedef struct {
     const float *queries;
     int nq;
     const int *entries;
-    int n_entries, expand;
+    int n_entries, entry_stride, expand;
     const int *pool, *num_out;
     int stride, refine;
     int *out_ids;
@@ -717,8 +717,10 @@ static void *orc_batch_worker(void *arg) {
         const float *qv = b->queries + (size_t)q * b->ix->dim;
         int want = b->refine > 0 ? (b->refine < L ? b->refine : L) : b->num_out[q];
         OrcStats st;
-        int n = orc_search(b->ix, b->scorer, qv, b->entries, b->n_entries,
-                           b->expand, L, want, ids, dists, cap, &st);
+        const int *ents = b->entries + (b->entry_stride > 0 ? (size_t)q * b->entry_stride : 0);
+        int ne = b->entry_stride > 0 ? b->entry_stride : b->n_entries;
+        int n = orc_search(b->ix, b->scorer, qv, ents, ne, b->expand, L, want, ids,
+                           dists, cap, &st);
         if (n >= 0 && b->refine > 0) {
             n = orc_refine(b->ix, qv, ids, n, b->num_out[q], NULL, ids, dists);
         }
@@ -736,13 +738,14 @@ static void *orc_batch_worker(void *arg) {
 }
 
 int orc_search_batch(const OrcIndex *ix, int scorer, const float *queries,
-                     int nq, const int *entries, int n_entries, int expand,
-                     const int *pool, const int *num_out, int stride,
-                     int refine, int threads, int *out_ids, float *out_d,
-                     int *out_counts, OrcStats *stats) {
-    OrcBatch b = {ix, scorer, queries, nq, entries, n_entries, expand, pool,
-                  num_out, stride, refine, out_ids, out_d, out_counts, stats,
-                  0, 0};
+                     int nq, const int *entries, int n_entries,
+                     int entry_stride, int expand, const int *pool,
+                     const int *num_out, int stride, int refine, int threads,
+                     int *out_ids, float *out_d, int *out_counts,
+                     OrcStats *stats) {
+    OrcBatch b = {ix, scorer, queries, nq, entries, n_entries, entry_stride,
+                  expand, pool, num_out, stride, refine, out_ids, out_d,
+                  out_counts, stats, 0, 0};
     pthread_t tid[256];
     if (threads < 1) threads = 1;
     if (threads > 256) threads = 256;

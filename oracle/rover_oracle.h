@@ -94,9 +94,12 @@ uint64_t orc_fingerprint(uint64_t hash, const int *ids,
 
 /* Many queries on a pool of host threads (one query per thread at a time).
  * refine_k > 0 applies orc_refine to the first refine_candidates results.
+ * entry_stride > 0: entry_ids holds one row of entry_stride ids per query
+ * (entry_count is then ignored).
  * Returns 0, or -1 if any query failed (its count is then -1). */
 int orc_search_batch(const OrcIndex *index, int scorer, const float *queries,
                      int num_queries, const int *entry_ids, int entry_count,
+                     int entry_stride,
                      int expand_count, const int *pool_capacity,
                      const int *num_out, int out_stride, int refine_candidates,
                      int threads, int *out_ids, float *out_distances,

@@ -51,6 +51,14 @@ typedef struct RoverCtx {
     int entries_cap;
     void *d_stage;
     size_t stage_cap;
+    /* entry router: centroids [router_n][dim] and representative node ids */
+    float *d_router_cent;
+    int32_t *d_router_ids;
+    int router_n;
+    float *d_route_dist;      /* routing scratch */
+    size_t route_dist_cap;
+    int32_t *d_route_ent;     /* per-query entry rows of the last launch */
+    size_t route_ent_cap;
     RoverLearned learned;
     pthread_mutex_t lock;
 } RoverCtx;
@@ -64,5 +72,9 @@ void rover_dfree(void *p);
 AEEResult rover_prepare_launch(RoverCtx *c, const RoverSearchBatch *b,
                                int max_pool, int explicit_entries, RvIndex *ix,
                                RvBatch *rb);
+/* Routes device queries to S entries each (nearest router centroids) into
+ * d_out[nq][S]; chunks the scratch.  Asynchronous on `stream`. */
+AEEResult rover_route_into(RoverCtx *c, const float *d_queries, int nq, int S,
+                           int32_t *d_out, void *stream);
 
 #endif

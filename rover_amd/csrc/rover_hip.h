@@ -66,8 +66,10 @@ typedef struct RvBatch {
     const int32_t *num_outs;     /* [nq] per-query k, or NULL -> num_out    */
     int pool_size, num_out;
     int max_pool;                /* LDS is sized for this L                 */
-    const int32_t *entries;      /* [n_entries] seed ids, shared by all     */
+    const int32_t *entries;      /* [n_entries] seed ids, shared by all, or
+                                    [nq][entry_stride] per query            */
     int n_entries;
+    int entry_stride;            /* 0: shared list; else per-query rows     */
     int expand_count;
     int scorer;                  /* RV_SCORE_*                              */
     int refine_candidates;       /* 0: off; F: refine the top F, output k   */
@@ -159,6 +161,13 @@ const char *rv_learned_error(void);
 uint16_t rv_bf16_round_host(float f);
 int rv_fill_zero(void *d_ptr, size_t bytes, void *stream);
 int rv_copy(void *d_dst, const void *d_src, size_t bytes, void *stream);
+
+/* Entry router: for each query the S (<= 16) nearest of nc centroids by
+ * exact squared L2, written as their representative node ids
+ * (out[nq][S], nearest first).  scratch holds nq*nc floats. */
+int rv_route_entries(const float *d_queries, int nq, int dim,
+                     const float *d_centroids, int nc, const int32_t *d_rep_ids,
+                     int S, float *d_scratch, int32_t *d_out, void *stream);
 
 #ifdef __cplusplus
 }

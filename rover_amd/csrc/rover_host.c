@@ -94,6 +94,11 @@ AEEResult calculator_retrieval_pure_pq_unload_dma(remote_handle64 h) {
     dfree(c->d_res);
     dfree(c->d_qparams);
     dfree(c->d_orig);
+    dfree(c->d_router_cent);
+    dfree(c->d_router_ids);
+    c->d_router_cent = NULL;
+    c->d_router_ids = NULL;
+    c->router_n = 0;
     c->d_codes = NULL;
     c->d_codebook = NULL;
     c->d_graph = NULL;
@@ -126,6 +131,8 @@ int calculator_close(remote_handle64 h) {
     dfree(c->d_queue);
     dfree(c->d_entries);
     dfree(c->d_stage);
+    dfree(c->d_route_dist);
+    dfree(c->d_route_ent);
     c->magic = 0;
     pthread_mutex_destroy(&c->lock);
     free(c);
@@ -417,7 +424,8 @@ AEEResult rover_prepare_launch(RoverCtx *c, const RoverSearchBatch *b,
                                int max_pool, int explicit_entries, RvIndex *ix,
                                RvBatch *rb) {
     uint32_t seed = b->seed ? b->seed : ROVER_ENTRY_POINT_SEED;
-    if (b->entry_ids && b->entry_count <= 0) return ROVER_EBADPARM;
+    if (b->entry_ids && b->entry_count <= 0 && b->entry_stride <= 0)
+        return ROVER_EBADPARM;
     if (b->num_queries < 0 || (!b->entry_ids && b->entry_points <= 0) ||
         b->expand_count <= 0 ||
         max_pool <= 0 || max_pool > ROVER_MAX_POOL || b->out_stride <= 0 ||
@@ -442,7 +450,9 @@ AEEResult rover_prepare_launch(RoverCtx *c, const RoverSearchBatch *b,
     rb->num_out = b->num_out;
     rb->max_pool = max_pool;
     rb->entries = b->entry_ids ? b->entry_ids : c->d_entries;
-    rb->n_entries = b->entry_ids ? b->entry_count : c->entries_count;
+    rb->entry_stride = b->entry_ids && b->entry_stride > 0 ? b->entry_stride : 0;
+    rb->n_entries = b->entry_ids ? (rb->entry_stride ? rb->entry_stride : b->entry_count)
+                                 : c->entries_count;
     rb->expand_count = b->expand_count;
     rb->scorer = b->method == ROVER_METHOD_SQ
                      ? RV_SCORE_SQ
@@ -479,12 +489,105 @@ AEEResult rover_prepare_launch(RoverCtx *c, const RoverSearchBatch *b,
     return ROVER_SUCCESS;
 }
 
-static AEEResult launch_batch(RoverCtx *c, const RoverSearchBatch *b,
+static int grow(void **d, size_t *cap, size_t bytes) {
+    if (bytes <= *cap) return 0;
+    dfree(*d);
+    *d = NULL;
+    *cap = 0;
+    if (rv_malloc(d, bytes)) return -1;
+    *cap = bytes;
+    return 0;
+}
+
+AEEResult rover_route_into(RoverCtx *c, const float *d_queries, int nq, int S,
+                           int32_t *d_out, void *stream) {
+    const int chunk = 16384;
+    if (!c->d_router_cent || c->router_n <= 0) {
+        snprintf(g_err, sizeof(g_err), "route: no entry router loaded");
+        return ROVER_EFAILED;
+    }
+    if (S <= 0 || S > 16 || S > c->router_n || nq < 0) return ROVER_EBADPARM;
+    if (grow((void **)&c->d_route_dist, &c->route_dist_cap,
+             (size_t)(nq < chunk ? nq : chunk) * c->router_n * sizeof(float))) {
+        set_err("route scratch");
+        return ROVER_ENOMEMORY;
+    }
+    for (int q0 = 0; q0 < nq; q0 += chunk) {
+        int cnt = nq - q0 < chunk ? nq - q0 : chunk;
+        if (rv_route_entries(d_queries + (size_t)q0 * c->dim, cnt, c->dim,
+                             c->d_router_cent, c->router_n, c->d_router_ids, S,
+                             c->d_route_dist, d_out + (size_t)q0 * S, stream)) {
+            set_err("route");
+            return ROVER_EFAILED;
+        }
+    }
+    return ROVER_SUCCESS;
+}
+
+AEEResult rover_route_entries(remote_handle64 h, const float *d_queries,
+                              int num_queries, int S, int32_t *d_out,
+                              void *stream) {
+    RoverCtx *c = ctx_of(h);
+    AEEResult rc;
+    if (!c || !d_queries || !d_out) return ROVER_EBADPARM;
+    pthread_mutex_lock(&c->lock);
+    rc = rover_route_into(c, d_queries, num_queries, S, d_out, stream);
+    pthread_mutex_unlock(&c->lock);
+    return rc;
+}
+
+AEEResult rover_pure_pq_router_load(remote_handle64 h, const float *centroids,
+                                    int centroidsLen, const int32_t *rep_ids,
+                                    int count) {
+    RoverCtx *c = ctx_of(h);
+    if (!c) return ROVER_EBADPARM;
+    if (!c->pq_loaded || !centroids || !rep_ids || count <= 0 ||
+        (int64_t)centroidsLen != (int64_t)count * c->dim)
+        return ROVER_EBADPARM;
+    for (int i = 0; i < count; i++)
+        if (rep_ids[i] < 0 || rep_ids[i] >= c->n) return ROVER_EBADPARM;
+    dfree(c->d_router_cent);
+    dfree(c->d_router_ids);
+    c->d_router_cent = NULL;
+    c->d_router_ids = NULL;
+    c->router_n = 0;
+    if (dalloc_copy((void **)&c->d_router_cent, centroids,
+                    (size_t)centroidsLen * sizeof(float)) ||
+        dalloc_copy((void **)&c->d_router_ids, rep_ids, (size_t)count * sizeof(int32_t))) {
+        set_err("router_load");
+        return ROVER_ENOMEMORY;
+    }
+    c->router_n = count;
+    return ROVER_SUCCESS;
+}
+
+static AEEResult launch_batch(RoverCtx *c, const RoverSearchBatch *b_in,
                               int max_pool, int explicit_entries,
                               void *stream) {
     RvIndex ix;
     RvBatch rb;
-    AEEResult rc = rover_prepare_launch(c, b, max_pool, explicit_entries, &ix, &rb);
+    RoverSearchBatch routed;
+    const RoverSearchBatch *b = b_in;
+    AEEResult rc;
+    if (b_in->route_entries > 0) {
+        /* per-query entries from the router, written to a session buffer */
+        int S = b_in->route_entries;
+        if (b_in->num_queries > 0 && !b_in->queries) return ROVER_EBADPARM;
+        if (grow((void **)&c->d_route_ent, &c->route_ent_cap,
+                 (size_t)(b_in->num_queries > 0 ? b_in->num_queries : 1) * S * 4)) {
+            set_err("route entries");
+            return ROVER_ENOMEMORY;
+        }
+        rc = rover_route_into(c, b_in->queries, b_in->num_queries, S, c->d_route_ent,
+                              stream);
+        if (rc != ROVER_SUCCESS) return rc;
+        routed = *b_in;
+        routed.entry_ids = c->d_route_ent;
+        routed.entry_count = S;
+        routed.entry_stride = S;
+        b = &routed;
+    }
+    rc = rover_prepare_launch(c, b, max_pool, explicit_entries, &ix, &rb);
     if (rc != ROVER_SUCCESS) return rc;
     if (rv_search_launch(&ix, &rb, b->grid_waves, stream)) {
         set_err("search launch");
@@ -523,7 +626,7 @@ static AEEResult search_host(RoverCtx *c, const RoverSearchBatch *b,
                              int explicit_entries) {
     RoverSearchBatch d;
     int dim, max_pool;
-    size_t oq, ol, ok, oi, od, oc, os, oe, total;
+    size_t oq, ol, ok, oi, od, oc, os, oe, total, entries_bytes;
     AEEResult rc = ROVER_EFAILED;
     if (!c || !b) return ROVER_EBADPARM;
     if (b->num_queries < 0 || (b->num_queries > 0 && !b->queries) ||
@@ -544,7 +647,11 @@ static AEEResult search_host(RoverCtx *c, const RoverSearchBatch *b,
     oc = up256(od + (size_t)b->num_queries * b->out_stride * 4);
     os = up256(oc + (size_t)b->num_queries * 4);
     oe = up256(os + (b->out_stats ? (size_t)b->num_queries * 32 : 0));
-    total = up256(oe + (b->entry_ids ? (size_t)b->entry_count * 4 : 0));
+    entries_bytes = b->entry_ids ? (size_t)(b->entry_stride > 0
+                                                ? b->entry_stride * (size_t)b->num_queries
+                                                : (size_t)b->entry_count) * 4
+                                 : 0;
+    total = up256(oe + entries_bytes);
     if (ensure_stage(c, total)) goto out;
     {
         char *base = (char *)c->d_stage;
@@ -557,8 +664,7 @@ static AEEResult search_host(RoverCtx *c, const RoverSearchBatch *b,
         d.out_counts = (int32_t *)(base + oc);
         d.out_stats = b->out_stats ? (uint64_t *)(base + os) : NULL;
         d.entry_ids = b->entry_ids ? (const int32_t *)(base + oe) : NULL;
-        if (b->entry_ids && b->entry_count > 0 &&
-            rv_memcpy_h2d(base + oe, b->entry_ids, (size_t)b->entry_count * 4)) {
+        if (entries_bytes && rv_memcpy_h2d(base + oe, b->entry_ids, entries_bytes)) {
             set_err("batch upload");
             goto out;
         }

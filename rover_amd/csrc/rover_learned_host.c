@@ -186,7 +186,17 @@ AEEResult rover_learned_stop_train(remote_handle64 h, RoverLearnedStopTrain *t) 
     b.method = ROVER_METHOD_PQ;
     b.correction = t->correction ? 1 : 0;
     b.entry_points = t->entry_points > 0 ? t->entry_points : 1;
-    if (t->entry_ids) {
+    if (t->route_entries > 0) {
+        int S = t->route_entries;
+        if (rv_malloc((void **)&d_ent, (size_t)t->num_queries * S * sizeof(int32_t)) ||
+            rover_route_into(c, d_q, t->num_queries, S, d_ent, NULL) != ROVER_SUCCESS) {
+            rover_set_err("learned stop: routing");
+            goto out;
+        }
+        b.entry_ids = d_ent;
+        b.entry_count = S;
+        b.entry_stride = S;
+    } else if (t->entry_ids) {
         if (t->entry_count <= 0 ||
             rv_malloc((void **)&d_ent, (size_t)t->entry_count * sizeof(int32_t)) ||
             rv_memcpy_h2d(d_ent, t->entry_ids, (size_t)t->entry_count * sizeof(int32_t))) {

@@ -543,13 +543,16 @@ __global__ __launch_bounds__(RV_WAVE) void rv_search_kernel(RvIndex ix, RvBatch 
             __syncthreads();
         };
 
-        // ---- seeds (rpc_demo_pure_pq.c:192-208 seeds; :843-873 scoring)
+        // ---- seeds (rpc_demo_pure_pq.c:192-208 seeds; :843-873 scoring);
+        // one shared list, or a row of entry_stride ids per query
+        const int32_t *ents =
+            b.entries + (b.entry_stride > 0 ? (size_t)q * b.entry_stride : 0);
         for (int e0 = 0; e0 < b.n_entries && status == 0; e0 += RV_WAVE) {
             const int e = e0 + lane;
             uint64_t key = RV_KEY_MAX;
             bool bad = false, nonfinite = false, dup = false;
             if (e < b.n_entries) {
-                const int id = b.entries[e];
+                const int id = ents[e];
                 if (id < 0 || id >= ix.n) {
                     bad = true;
                 } else if (!rv_visit(hash, b.hash_log2, b.probe_limit, (uint32_t)id)) {
@@ -1055,6 +1058,124 @@ extern "C" int rv_residual_encode(const float *x, const uint8_t *codes, int n,
     rv_residual_encode_kernel<<<n, RV_WAVE, 0, (hipStream_t)stream>>>(
         x, codes, n, dim, m, ks, codebook, code_bytes, bits, res, res_bytes, qparams);
     return rv_check(hipGetLastError(), "rv_residual_encode");
+}
+
+// ----------------------------------------------------------- entry router
+//
+// Per-query entry selection: exact squared L2 from every query to every
+// routing centroid (a tiled [queries x centroids] kernel, each distance a
+// sequential f32 sum over the dimensions), then the S nearest centroids per
+// query by (distance, centroid index), mapped to their representative node
+// ids.  The ids become that query's entry list for the graph search.
+
+#define RT_TQ 64  // queries per tile
+#define RT_TC 64  // centroids per tile
+#define RT_KD 32  // dimensions staged per step
+
+__global__ __launch_bounds__(256) void rv_router_dist_kernel(
+    const float *__restrict__ q, int nq, const float *__restrict__ cent, int nc,
+    int dim, float *__restrict__ out) {
+    __shared__ float Qs[RT_TQ][RT_KD + 1];
+    __shared__ float Cs[RT_TC][RT_KD + 1];
+    const int t = threadIdx.x;
+    const int q0 = blockIdx.x * RT_TQ, c0 = blockIdx.y * RT_TC;
+    const int tq = (t / 16) * 4, tc = (t % 16) * 4;  // 4x4 outputs per thread
+    float acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = 0.0f;
+    for (int k0 = 0; k0 < dim; k0 += RT_KD) {
+        for (int i = t; i < RT_TQ * RT_KD; i += 256) {
+            const int r = i / RT_KD, k = i % RT_KD;
+            Qs[r][k] = (q0 + r < nq && k0 + k < dim) ? q[(size_t)(q0 + r) * dim + k0 + k] : 0.0f;
+            Cs[r][k] = (c0 + r < nc && k0 + k < dim) ? cent[(size_t)(c0 + r) * dim + k0 + k] : 0.0f;
+        }
+        __syncthreads();
+        const int kend = dim - k0 < RT_KD ? dim - k0 : RT_KD;
+        for (int k = 0; k < kend; k++) {
+            float a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) a[i] = Qs[tq + i][k];
+#pragma unroll
+            for (int j = 0; j < 4; j++) b[j] = Cs[tc + j][k];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const float d = a[i] - b[j];
+                    acc[i][j] += d * d;
+                }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int qq = q0 + tq + i, cc = c0 + tc + j;
+            if (qq < nq && cc < nc) out[(size_t)qq * nc + cc] = acc[i][j];
+        }
+}
+
+// One wavefront per query: the S smallest (distance, centroid) keys.
+__global__ __launch_bounds__(RV_WAVE) void rv_router_select_kernel(
+    const float *__restrict__ dist, int nq, int nc, const int32_t *__restrict__ rep_ids,
+    int S, int32_t *__restrict__ out) {
+    const int q = blockIdx.x;
+    const int lane = rv_lane();
+    if (q >= nq) return;
+    uint64_t best[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) best[i] = RV_KEY_MAX;
+    const float *row = dist + (size_t)q * nc;
+    for (int c = lane; c < nc; c += RV_WAVE) {
+        // insertion into the lane's sorted list: carry the larger key on
+        uint64_t v = rv_key(row[c] + 0.0f, c);
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            if (i < S) {
+                const uint64_t lo = v < best[i] ? v : best[i];
+                const uint64_t hi = v < best[i] ? best[i] : v;
+                best[i] = lo;
+                v = hi;
+            }
+        }
+    }
+    // S rounds of wave-wide minimum over the lanes' list heads
+    int head = 0;
+    for (int s = 0; s < S; s++) {
+        uint64_t mine = RV_KEY_MAX;
+#pragma unroll
+        for (int i = 0; i < 16; i++)
+            if (i == head) mine = best[i];
+        uint64_t m = mine;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            uint64_t o = __shfl_xor(m, off);
+            m = o < m ? o : m;
+        }
+        if (mine == m && m != RV_KEY_MAX) head++;
+        if (lane == 0)
+            out[(size_t)q * S + s] = m == RV_KEY_MAX ? rep_ids[0] : rep_ids[rv_key_id(m)];
+    }
+}
+
+extern "C" int rv_route_entries(const float *queries, int nq, int dim,
+                                const float *centroids, int nc,
+                                const int32_t *rep_ids, int S, float *scratch,
+                                int32_t *out, void *stream) {
+    if (nq <= 0) return 0;
+    if (nc <= 0 || S <= 0 || S > 16 || S > nc || dim <= 0) {
+        snprintf(g_rv_error, sizeof(g_rv_error), "rv_route_entries: bad parameters");
+        return -1;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    dim3 grid((nq + RT_TQ - 1) / RT_TQ, (nc + RT_TC - 1) / RT_TC);
+    rv_router_dist_kernel<<<grid, 256, 0, s>>>(queries, nq, centroids, nc, dim, scratch);
+    if (rv_check(hipGetLastError(), "rv_router_dist_kernel")) return -1;
+    rv_router_select_kernel<<<nq, RV_WAVE, 0, s>>>(scratch, nq, nc, rep_ids, S, out);
+    return rv_check(hipGetLastError(), "rv_router_select_kernel");
 }
 
 // ------------------------------------------------------------ copy kernel

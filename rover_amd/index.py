@@ -442,6 +442,20 @@ def entry_list(x, count, entry, device=None, iters=10, sample=None):
     node per k-means cell lets the best-first search start in the query's
     cluster.  The list is passed as the request's entry ids, exactly like the
     reference's seed list (PurePqSearchRequest.entry_ids)."""
+    _, reps = kmeans_representatives(x, count, device, iters, sample)
+    out, seen = [int(entry)], {int(entry)}
+    for r in reps:
+        if int(r) not in seen:
+            seen.add(int(r))
+            out.append(int(r))
+    return np.array(out, dtype=np.int32)
+
+
+def kmeans_representatives(x, count, device=None, iters=10, sample=None):
+    """(centroids float32 [count, dim], representative ids int32 [count]):
+    Lloyd k-means on a uniform-stride sample, then for every centroid the
+    base vector nearest to it.  Used by the entry router: each query starts
+    from the representatives of its nearest centroids."""
     n = x.shape[0]
     count = min(count, n)
     rows = uniform_stride_sample(n, min(n, sample or max(64 * count, 20000)))
@@ -470,6 +484,7 @@ def entry_list(x, count, entry, device=None, iters=10, sample=None):
             best = torch.where(upd, v, best)
             arg = torch.where(upd, i + s, arg)
         reps = arg.cpu().numpy()
+        cent = cent.cpu().numpy()
     else:
         cent = tr[init].copy()
         for _ in range(iters):
@@ -483,21 +498,23 @@ def entry_list(x, count, entry, device=None, iters=10, sample=None):
         xf = np.ascontiguousarray(x, dtype=np.float32)
         d = (xf * xf).sum(1)[None, :] - 2.0 * cent @ xf.T
         reps = np.argmin(d, axis=1)
-    out, seen = [int(entry)], {int(entry)}
-    for r in reps:
-        if int(r) not in seen:
-            seen.add(int(r))
-            out.append(int(r))
-    return np.array(out, dtype=np.int32)
+    return (np.ascontiguousarray(cent, dtype=np.float32),
+            np.ascontiguousarray(reps, dtype=np.int32))
 
 
 class PqIndexData:
     """Arrays of a pure-PQ index (all host numpy)."""
 
     def __init__(self, graph, entry, codes, codebook, res_codes=None,
-                 res_qparams=None, res_bits=4, entries=None):
+                 res_qparams=None, res_bits=4, entries=None,
+                 router_centroids=None, router_ids=None):
         self.graph = np.ascontiguousarray(graph, dtype=np.int32)
         self.entry = int(entry)
+        # entry router: centroids and their representative node ids
+        self.router_centroids = None if router_centroids is None else \
+            np.ascontiguousarray(router_centroids, dtype=np.float32)
+        self.router_ids = None if router_ids is None else \
+            np.ascontiguousarray(router_ids, dtype=np.int32)
         # shared seed list (medoid first); None -> the graph entry alone
         self.entries = None if entries is None else np.ascontiguousarray(entries, dtype=np.int32)
         self.codes = np.ascontiguousarray(codes, dtype=np.uint8)
@@ -528,9 +545,10 @@ class PqIndexData:
 
 
 def build_pq_index(x, m, R, ks=256, res_bits=4, iters=20, train_max=100000,
-                   device=None, graph=None, n_entries=0):
+                   device=None, graph=None, n_entries=0, n_router=0):
     """n_entries > 0 adds a shared seed list of the medoid plus n_entries
-    k-means representatives (see entry_list)."""
+    k-means representatives (see entry_list); n_router > 0 adds an entry
+    router of n_router centroids (see kmeans_representatives)."""
     codebook = train_pq(x, m, ks, iters=iters, train_max=train_max, device=device)
     if device is not None and x.shape[0] > 100000:
         codes, res, qp = encode_on_device(x, codebook, res_bits)
@@ -541,7 +559,8 @@ def build_pq_index(x, m, R, ks=256, res_bits=4, iters=20, train_max=100000,
         graph = build_graph(x, R, device=device)
     entry = medoid(x)
     entries = entry_list(x, n_entries, entry, device=device) if n_entries > 0 else None
-    return PqIndexData(graph, entry, codes, codebook, res, qp, res_bits, entries)
+    rc, ri = kmeans_representatives(x, n_router, device) if n_router > 0 else (None, None)
+    return PqIndexData(graph, entry, codes, codebook, res, qp, res_bits, entries, rc, ri)
 
 
 def encode_on_device(x, codebook, res_bits):

@@ -61,7 +61,8 @@ class RoverSearchBatch(C.Structure):
                 ("method", C.c_int), ("correction", C.c_int),
                 ("refine_candidates", C.c_int), ("entry_points", C.c_int),
                 ("seed", C.c_uint32), ("entry_ids", C.c_void_p),
-                ("entry_count", C.c_int), ("expand_count", C.c_int),
+                ("entry_count", C.c_int), ("entry_stride", C.c_int),
+                ("route_entries", C.c_int), ("expand_count", C.c_int),
                 ("out_ids", C.c_void_p), ("out_dists", C.c_void_p),
                 ("out_counts", C.c_void_p), ("out_stride", C.c_int),
                 ("out_stats", C.c_void_p), ("hash_log2", C.c_int),
@@ -74,7 +75,8 @@ class RoverLearnedStopTrain(C.Structure):
                 ("pool_size", C.c_int), ("correction", C.c_int),
                 ("label_k", C.c_int),
                 ("entry_points", C.c_int), ("entry_ids", C.c_void_p),
-                ("entry_count", C.c_int), ("expand_count", C.c_int),
+                ("entry_count", C.c_int), ("route_entries", C.c_int),
+                ("expand_count", C.c_int),
                 ("stop_every", C.c_int), ("min_steps", C.c_int),
                 ("trace_cap", C.c_int), ("steps", C.c_int), ("batch", C.c_int),
                 ("lr", C.c_float), ("pos_weight", C.c_float),
@@ -106,6 +108,8 @@ SIGNATURES = [
     ("rover_pure_pq_residual_load", _I,
      [_H, _P, C.c_int64, _P, C.c_int64, C.c_int32]),
     ("rover_pure_pq_original_ids", _I, [_H, _P, _I]),
+    ("rover_pure_pq_router_load", _I, [_H, _P, _I, _P, _I]),
+    ("rover_route_entries", _I, [_H, _P, _I, _I, _P, _P]),
     ("rover_pq_residual_evaluator", _I, [_P, _I, _F, _P]),
     ("retrieval_pure_pq_search_scalar_with_policy", _I,
      [_P, _P, _P, _P, _P, _I, _P]),

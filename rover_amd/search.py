@@ -90,6 +90,24 @@ class Calculator:
         self.pure_pq_graph_load(data.graph, data.entry)
         if data.res_codes is not None:
             self.pure_pq_residual_load(data.res_codes, data.res_qparams, data.res_bits)
+        if getattr(data, "router_centroids", None) is not None:
+            self.pure_pq_router_load(data.router_centroids, data.router_ids)
+
+    def pure_pq_router_load(self, centroids, rep_ids):
+        c = np.ascontiguousarray(centroids, dtype=np.float32)
+        r = np.ascontiguousarray(rep_ids, dtype=np.int32)
+        check(native.lib().rover_pure_pq_router_load(self.h, ptr(c), c.size, ptr(r),
+                                                      len(r)), "rover_pure_pq_router_load")
+
+    def route_entries(self, queries, S):
+        """Per-query entry ids [nq, S] chosen by the router (host arrays)."""
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        dq = native.DeviceBuffer.from_array(q)
+        dout = native.DeviceBuffer(q.shape[0] * S * 4)
+        check(native.lib().rover_route_entries(self.h, dq.ptr, q.shape[0], S, dout.ptr,
+                                               None), "rover_route_entries")
+        native.synchronize()
+        return dout.download(np.int32, q.shape[0] * S).reshape(q.shape[0], S)
 
     def retrieval_load(self, graph, l1_codes, dim, entry, scale, zero):
         """calculator.idl:36-45 (SQ index; bits from rover_configure)."""
@@ -123,10 +141,13 @@ class Calculator:
                       entry_points, expand_count, out_ids, out_dists,
                       out_counts, stride, out_stats, hash_log2, probe_limit,
                       grid_waves, learned_stop, stop_threshold, pool_sizes,
-                      num_outs, seed, entry_ids=None, entry_count=0):
+                      num_outs, seed, entry_ids=None, entry_count=0,
+                      entry_stride=0, route_entries=0):
         b = native.RoverSearchBatch()
         b.entry_ids = entry_ids
         b.entry_count = int(entry_count)
+        b.entry_stride = int(entry_stride)
+        b.route_entries = int(route_entries)
         b.queries = queries
         b.num_queries = nq
         b.pool_sizes = pool_sizes
@@ -155,9 +176,11 @@ class Calculator:
                      correction=0, refine_candidates=0, entry_points=1,
                      expand_count=32, stats=False, hash_log2=0, probe_limit=0,
                      grid_waves=0, learned_stop=0, stop_threshold=0.0,
-                     seed=native.ENTRY_POINT_SEED, entry_ids=None):
-        """Host arrays in and out.  entry_ids: explicit seed list shared by
-        all queries (otherwise graph entry + LCG seeds).  pool_size / num_out: int or per-query
+                     seed=native.ENTRY_POINT_SEED, entry_ids=None,
+                     route_entries=0):
+        """Host arrays in and out.  entry_ids: explicit seeds, a list shared
+        by all queries or a [nq, S] array of per-query rows (otherwise graph
+        entry + LCG seeds); route_entries=S: per-query seeds from the router.  pool_size / num_out: int or per-query
         arrays.  Returns ids [nq, k_max], dists, counts (negative = per-query
         error status) and, with stats=True, [nq, 4] counters."""
         q = np.ascontiguousarray(queries, dtype=np.float32)
@@ -171,12 +194,14 @@ class Calculator:
         cnt = np.zeros(nq, dtype=np.int32)
         st = np.zeros((nq, 4), dtype=np.uint64) if stats else None
         e = None if entry_ids is None else np.ascontiguousarray(entry_ids, dtype=np.int32)
+        stride = e.shape[1] if e is not None and e.ndim == 2 else 0
         b = self._batch_struct(ptr(q), nq, Lmax, Kmax, method, correction,
                                refine_candidates, entry_points, expand_count,
                                ptr(ids), ptr(d), ptr(cnt), Kmax, ptr(st),
                                hash_log2, probe_limit, grid_waves, learned_stop,
                                stop_threshold, ptr(Ls), ptr(Ks), seed, ptr(e),
-                               0 if e is None else len(e))
+                               0 if e is None else (stride or len(e)), stride,
+                               route_entries)
         check(native.lib().rover_search_batch(self.h, C.byref(b)), "rover_search_batch")
         return (ids, d, cnt, st) if stats else (ids, d, cnt)
 
@@ -188,7 +213,7 @@ class Calculator:
                             stop_threshold=0.0, d_pool_sizes=None,
                             d_num_outs=None, stream=None,
                             seed=native.ENTRY_POINT_SEED, d_entry_ids=None,
-                            entry_count=0):
+                            entry_count=0, entry_stride=0, route_entries=0):
         """Device pointers in and out; asynchronous.  With per-query device
         arrays, pool_size must be their maximum."""
         b = self._batch_struct(d_queries, nq, pool_size, num_out, method,
@@ -196,7 +221,8 @@ class Calculator:
                                expand_count, d_ids, d_dists, d_counts, stride,
                                d_stats, hash_log2, probe_limit, grid_waves,
                                learned_stop, stop_threshold, d_pool_sizes,
-                               d_num_outs, seed, d_entry_ids, entry_count)
+                               d_num_outs, seed, d_entry_ids, entry_count,
+                               entry_stride, route_entries)
         check(native.lib().rover_search_batch_device(self.h, C.byref(b), stream),
               "rover_search_batch_device")
 

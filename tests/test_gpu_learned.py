@@ -79,7 +79,7 @@ def trained():
     base, queries = data.workload(12000, 128, 64, 8.0, 300)
     train_q = data.gaussian_mixture(3000, 128, 64, 8.0, 44, centers_seed=42)
     ix = rindex.build_pq_index(base, m=32, R=32, res_bits=8, iters=6,
-                               train_max=12000, n_entries=128)
+                               train_max=12000, n_router=128)
     gt = data.ground_truth(base, queries, k=10)
     calc = Calculator()
     calc.load_pq_index(ix)
@@ -90,8 +90,7 @@ def trained():
     t.correction = 1
     t.label_k = 10
     t.entry_points = 1
-    t.entry_ids = ix.entries.ctypes.data
-    t.entry_count = len(ix.entries)
+    t.route_entries = 4
     t.expand_count = 32
     t.stop_every = 2
     t.min_steps = 4
@@ -122,9 +121,9 @@ def test_threshold_above_one_is_the_plain_search(trained):
     to the plain fixed-L search: identical ids, distances and expansions."""
     calc, ix, queries, gt, t = trained
     a = calc.search_batch(queries, 128, 10, stats=True, correction=1,
-                          entry_ids=ix.entries)
+                          route_entries=4)
     b = calc.search_batch(queries, 128, 10, stats=True, learned_stop=1,
-                          stop_threshold=1.5, correction=1, entry_ids=ix.entries)
+                          stop_threshold=1.5, correction=1, route_entries=4)
     assert np.array_equal(a[0], b[0])
     assert np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32))
     assert np.array_equal(a[3], b[3])
@@ -133,13 +132,13 @@ def test_threshold_above_one_is_the_plain_search(trained):
 def test_learned_stop_gain(trained):
     """The trained predictor cuts expansions at the same beam width while
     recall@10 stays within 0.02 of the plain search that judges it (online
-    correction, 8-bit residuals, medoid + k-means seed list)."""
+    correction, 8-bit residuals, routed per-query entries)."""
     calc, ix, queries, gt, t = trained
     ids_p, _, cnt_p, st_p = calc.search_batch(queries, 128, 10, stats=True,
-                                              correction=1, entry_ids=ix.entries)
+                                              correction=1, route_entries=4)
     ids_l, _, cnt_l, st_l = calc.search_batch(queries, 128, 10, stats=True,
                                               learned_stop=1, stop_threshold=0.9,
-                                              correction=1, entry_ids=ix.entries)
+                                              correction=1, route_entries=4)
     rp = data.recall_at_k(ids_p, cnt_p, gt, 10)
     rl = data.recall_at_k(ids_l, cnt_l, gt, 10)
     ep = st_p[:, 2].mean()
