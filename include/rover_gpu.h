@@ -270,10 +270,14 @@ typedef struct RoverLearnedStopTrain {
     int num_queries;
     int pool_size;            /* L of the trace searches                  */
     int correction;           /* 0: ADC ranking, 1: online correction     */
-    int label_k;              /* top-K whose stability is predicted       */
+    int label_k;              /* top-K whose stability is predicted (<= 16) */
+    int label_slack;          /* label 1 once all but label_slack of the
+                                 final top-K have been scored (0: final)   */
     int entry_points;
     const int32_t *entry_ids; /* host seed list, or NULL -> LCG seeds      */
     int entry_count;
+    int entry_stride;         /* 0: entry_ids is one shared list; > 0: rows
+                                 [num_queries][entry_stride]               */
     int route_entries;        /* > 0: route each training query (as in a
                                  search batch); overrides entry_ids        */
     int expand_count;

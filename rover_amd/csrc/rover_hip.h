@@ -47,11 +47,15 @@ typedef struct RvIndex {
 /* Learned early-termination predictor: 16 -> 64 -> 64 -> 1 MLP,
  * BF16 weights (stored as raw uint16), FP32 biases, FP32 accumulation. */
 #define RV_MLP_IN 16
+/* Trace records: per query 2 counters + the scored steps of the final top-K
+ * (K <= RV_TRACE_TOPK). */
+#define RV_TRACE_TOPK 16
+#define RV_TRACE_META (2 + RV_TRACE_TOPK)
 #define RV_MLP_HIDDEN 64
 typedef struct RvMlp {
-    const uint16_t *w1;    /* [64][16]  bf16 */
+    const uint16_t *w1;    /* [16][64]  bf16, input-major (w1t[i][j]) */
     const float *b1;       /* [64]           */
-    const uint16_t *w2;    /* [64][64]  bf16 */
+    const uint16_t *w2;    /* [64][64]  bf16, input-major (w2t[i][j]) */
     const float *b2;       /* [64]           */
     const uint16_t *w3;    /* [64]      bf16 */
     const float *b3;       /* [1]            */
@@ -91,7 +95,10 @@ typedef struct RvBatch {
     int stop_label_k;            /* top-K whose stability is predicted      */
     float *trace_feats;          /* [nq][trace_cap][16]                     */
     int32_t *trace_steps;        /* [nq][trace_cap]                         */
-    int32_t *trace_meta;         /* [nq][2]: records, last top-K change step */
+    int32_t *trace_meta;         /* [nq][RV_TRACE_META]: records, last top-K
+                                    change step, then the steps at which the
+                                    final top-K members were scored, ascending
+                                    (min(K, pool) values, padded with the last) */
     int trace_cap;
 } RvBatch;
 
@@ -147,11 +154,12 @@ int rv_mlp_adam(float *d_params, const float *d_grad, float *d_m1,
                 float *d_m2, int step, float lr, float beta1, float beta2,
                 float eps, float grad_scale, void *stream);
 int rv_mlp_to_bf16(const float *d_params, uint16_t *d_bf16, void *stream);
-/* Labels/weights from search traces: y = step >= last top-K change,
+/* Labels/weights from search traces: y = 1 once all but `slack` of the
+ * final top-K members have been scored (slack 0: the top-K is final),
  * weight = pos_w or neg_w for valid records, 0 for padding. */
 int rv_trace_labels(const int32_t *d_steps, const int32_t *d_meta, int nq,
-                    int cap, float pos_w, float neg_w, float *d_y,
-                    float *d_weight, void *stream);
+                    int cap, int label_k, int slack, float pos_w, float neg_w,
+                    float *d_y, float *d_weight, void *stream);
 int rv_normalize(float *d_x, size_t n, const float *d_mean,
                  const float *d_inv_std, void *stream);
 /* n pseudo-random sample ids in [0, population) from (seed, step). */

@@ -221,26 +221,37 @@ __global__ void rv_mlp_adam_kernel(float *p, const float *g, float *m1, float *m
 }
 
 __global__ void rv_mlp_bf16_kernel(const float *p, uint16_t *w) {
-    // w1 | w2 | w3 as bf16, contiguous
+    // Inference copy as bf16, input-major so that the lanes of a wavefront
+    // (one hidden unit each) read one contiguous row per input:
+    // w1t[16][64] | w2t[64][64] | w3[64].
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int n1 = MLP_H * MLP_IN, n2 = MLP_H * MLP_H, n3 = MLP_H;
-    if (i < n1) w[i] = rv_f32_to_bf16(p[OFF_W1 + i]);
-    else if (i < n1 + n2) w[i] = rv_f32_to_bf16(p[OFF_W2 + (i - n1)]);
-    else if (i < n1 + n2 + n3) w[i] = rv_f32_to_bf16(p[OFF_W3 + (i - n1 - n2)]);
+    if (i < n1) {
+        const int in = i / MLP_H, j = i % MLP_H;
+        w[i] = rv_f32_to_bf16(p[OFF_W1 + j * MLP_IN + in]);
+    } else if (i < n1 + n2) {
+        const int in = (i - n1) / MLP_H, j = (i - n1) % MLP_H;
+        w[i] = rv_f32_to_bf16(p[OFF_W2 + j * MLP_H + in]);
+    } else if (i < n1 + n2 + n3) {
+        w[i] = rv_f32_to_bf16(p[OFF_W3 + (i - n1 - n2)]);
+    }
 }
 
 // Turns search traces into samples: record r of query q is valid when
 // r < records[q]; its label is 1 when the record's step is at or after the
-// last change of the predicted top-K (the top-K is then final).
+// step at which all but `slack` of the final top-K members had been scored
+// (a final top-K member stays in the top-K from the step it is scored, so
+// with slack 0 this is the last change of the top-K: the top-K is final).
 __global__ void rv_trace_labels_kernel(const int32_t *steps, const int32_t *meta,
-                                       int nq, int cap, float pos_w, float neg_w,
-                                       float *y, float *weight) {
+                                       int nq, int cap, int need, float pos_w,
+                                       float neg_w, float *y, float *weight) {
     const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (i >= (size_t)nq * cap) return;
     const int q = (int)(i / cap), r = (int)(i % cap);
-    const int recs = meta[2 * (size_t)q], last = meta[2 * (size_t)q + 1];
+    const int32_t *mq = meta + (size_t)q * RV_TRACE_META;
+    const int recs = mq[0], ready = mq[2 + need];
     const bool ok = r < recs;
-    const bool pos = ok && steps[i] >= last;
+    const bool pos = ok && steps[i] >= ready;
     y[i] = pos ? 1.f : 0.f;
     weight[i] = ok ? (pos ? pos_w : neg_w) : 0.f;
 }
@@ -320,11 +331,16 @@ extern "C" int rv_mlp_to_bf16(const float *params, uint16_t *w, void *stream) {
 }
 
 extern "C" int rv_trace_labels(const int32_t *steps, const int32_t *meta, int nq,
-                               int cap, float pos_w, float neg_w, float *y,
-                               float *weight, void *stream) {
+                               int cap, int label_k, int slack, float pos_w,
+                               float neg_w, float *y, float *weight, void *stream) {
+    if (label_k <= 0 || label_k > RV_TRACE_TOPK || slack < 0 || slack >= label_k) {
+        snprintf(g_err, sizeof(g_err), "rv_trace_labels: bad label_k/slack");
+        return -1;
+    }
     const size_t n = (size_t)nq * cap;
+    if (n == 0) return 0;
     rv_trace_labels_kernel<<<(unsigned)((n + 255) / 256), 256, 0, (hipStream_t)stream>>>(
-        steps, meta, nq, cap, pos_w, neg_w, y, weight);
+        steps, meta, nq, cap, label_k - 1 - slack, pos_w, neg_w, y, weight);
     return chk(hipGetLastError(), "rv_trace_labels");
 }
 

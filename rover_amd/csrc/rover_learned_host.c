@@ -3,9 +3,12 @@
  * training and installation of the stop predictor.
  *
  * Traces come from the plain search itself (the search kernel in trace mode
- * records the 16 stop features every `stop_every` expansions and the last
- * step at which its top-K changed).  A record's label is 1 when the top-K is
- * already final at that step.  The MLP is trained by the library's own HIP
+ * records the 16 stop features every `stop_every` expansions and, per query,
+ * the steps at which the members of its final top-K were scored).  A
+ * record's label is 1 when all but `label_slack` of the final top-K had been
+ * scored by that step (slack 0: the top-K is already final; a final top-K
+ * member stays in the top-K from the step it is scored, since candidates
+ * better than it only accumulate).  The MLP is trained by the library's own HIP
  * forward/backward/Adam kernels and installed on the session; searches with
  * learned_stop set stop once the predicted probability reaches the
  * threshold, and otherwise run to the plain fixed-L termination.
@@ -161,7 +164,8 @@ AEEResult rover_learned_stop_train(remote_handle64 h, RoverLearnedStopTrain *t) 
     float mean[NF], istd[NF];
     if (!c || !t || !t->queries || t->num_queries <= 0 || t->pool_size <= 0 ||
         t->pool_size > ROVER_MAX_POOL || t->stop_every <= 0 || t->trace_cap <= 0 ||
-        t->label_k <= 0 || t->label_k > t->pool_size || t->steps <= 0 ||
+        t->label_k <= 0 || t->label_k > t->pool_size || t->label_k > RV_TRACE_TOPK ||
+        t->label_slack < 0 || t->label_slack >= t->label_k || t->steps <= 0 ||
         t->batch <= 0 || !(t->lr > 0.0f))
         return ROVER_EBADPARM;
     pthread_mutex_lock(&c->lock);
@@ -171,7 +175,7 @@ AEEResult rover_learned_stop_train(remote_handle64 h, RoverLearnedStopTrain *t) 
         rv_memcpy_h2d(d_q, t->queries, (size_t)t->num_queries * c->dim * sizeof(float)) ||
         rv_malloc((void **)&d_feat, nrec * NF * sizeof(float)) ||
         rv_malloc((void **)&d_steps, nrec * sizeof(int32_t)) ||
-        rv_malloc((void **)&d_meta, (size_t)t->num_queries * 2 * sizeof(int32_t)) ||
+        rv_malloc((void **)&d_meta, (size_t)t->num_queries * RV_TRACE_META * sizeof(int32_t)) ||
         rv_malloc((void **)&d_oid, (size_t)t->num_queries * sizeof(int32_t)) ||
         rv_malloc((void **)&d_od, (size_t)t->num_queries * sizeof(float)) ||
         rv_malloc((void **)&d_cnt, (size_t)t->num_queries * sizeof(int32_t))) {
@@ -197,14 +201,17 @@ AEEResult rover_learned_stop_train(remote_handle64 h, RoverLearnedStopTrain *t) 
         b.entry_count = S;
         b.entry_stride = S;
     } else if (t->entry_ids) {
-        if (t->entry_count <= 0 ||
-            rv_malloc((void **)&d_ent, (size_t)t->entry_count * sizeof(int32_t)) ||
-            rv_memcpy_h2d(d_ent, t->entry_ids, (size_t)t->entry_count * sizeof(int32_t))) {
+        /* one shared list, or one row of entry_stride ids per query */
+        size_t cnt = t->entry_stride > 0 ? (size_t)t->entry_stride * t->num_queries
+                                         : (size_t)(t->entry_count > 0 ? t->entry_count : 0);
+        if (cnt == 0 || rv_malloc((void **)&d_ent, cnt * sizeof(int32_t)) ||
+            rv_memcpy_h2d(d_ent, t->entry_ids, cnt * sizeof(int32_t))) {
             rover_set_err("learned stop: entry ids");
             goto out;
         }
         b.entry_ids = d_ent;
-        b.entry_count = t->entry_count;
+        b.entry_count = t->entry_stride > 0 ? t->entry_stride : t->entry_count;
+        b.entry_stride = t->entry_stride > 0 ? t->entry_stride : 0;
     }
     b.expand_count = t->expand_count > 0 ? t->expand_count : c->degree;
     b.out_ids = d_oid;
@@ -229,14 +236,16 @@ AEEResult rover_learned_stop_train(remote_handle64 h, RoverLearnedStopTrain *t) 
     {
         int stride = t->num_queries > 4096 ? t->num_queries / 4096 : 1;
         int nsamp = (t->num_queries + stride - 1) / stride;
-        h_meta = malloc((size_t)t->num_queries * 2 * sizeof(int32_t));
+        const int need = 2 + t->label_k - 1 - t->label_slack;
+        h_meta = malloc((size_t)t->num_queries * RV_TRACE_META * sizeof(int32_t));
         h_feat = malloc((size_t)t->trace_cap * NF * sizeof(float));
         if (!h_meta || !h_feat ||
-            rv_memcpy_d2h(h_meta, d_meta, (size_t)t->num_queries * 2 * sizeof(int32_t)))
+            rv_memcpy_d2h(h_meta, d_meta,
+                          (size_t)t->num_queries * RV_TRACE_META * sizeof(int32_t)))
             goto out;
         for (int f = 0; f < NF; f++) sum[f] = sq[f] = 0.0;
         for (int s = 0; s < nsamp; s++) {
-            int q = s * stride, recs = h_meta[2 * q];
+            int q = s * stride, recs = h_meta[(size_t)q * RV_TRACE_META];
             if (recs <= 0) continue;
             if (rv_memcpy_d2h(h_feat, d_feat + (size_t)q * t->trace_cap * NF,
                               (size_t)recs * NF * sizeof(float)))
@@ -253,7 +262,7 @@ AEEResult rover_learned_stop_train(remote_handle64 h, RoverLearnedStopTrain *t) 
                     sq[f] += v * v;
                 }
                 cnt += 1.0;
-                pos += steps_q[r] >= h_meta[2 * q + 1] ? 1.0 : 0.0;
+                pos += steps_q[r] >= h_meta[(size_t)q * RV_TRACE_META + need] ? 1.0 : 0.0;
             }
         }
         if (cnt < 1.0) {
@@ -266,7 +275,8 @@ AEEResult rover_learned_stop_train(remote_handle64 h, RoverLearnedStopTrain *t) 
             istd[f] = (float)(1.0 / sqrt(var > 1e-12 ? var : 1e-12));
         }
         t->samples = 0;
-        for (int q = 0; q < t->num_queries; q++) t->samples += h_meta[2 * q];
+        for (int q = 0; q < t->num_queries; q++)
+            t->samples += h_meta[(size_t)q * RV_TRACE_META];
         t->positive_frac = (float)(pos / cnt);
     }
     /* ---- labels, weights, normalized features */
@@ -281,8 +291,8 @@ AEEResult rover_learned_stop_train(remote_handle64 h, RoverLearnedStopTrain *t) 
             rv_malloc((void **)&d_istd, NF * sizeof(float)) ||
             rv_memcpy_h2d(d_mean, mean, sizeof(mean)) ||
             rv_memcpy_h2d(d_istd, istd, sizeof(istd)) ||
-            rv_trace_labels(d_steps, d_meta, t->num_queries, t->trace_cap, pos_w,
-                            neg_w, d_y, d_w, NULL) ||
+            rv_trace_labels(d_steps, d_meta, t->num_queries, t->trace_cap, t->label_k,
+                            t->label_slack, pos_w, neg_w, d_y, d_w, NULL) ||
             rv_normalize(d_feat, nrec, d_mean, d_istd, NULL)) {
             rover_set_err("learned stop: labels");
             goto out;

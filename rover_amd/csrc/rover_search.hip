@@ -290,6 +290,7 @@ __device__ __forceinline__ float rv_sq(const uint8_t *qsq, const uint8_t *code,
 
 struct RvLayout {
     int off_q, off_tab, off_keys0, off_keys1, off_flags0, off_flags1;
+    int off_born0, off_born1;  // trace mode: step each pool entry was scored
     int off_cand, off_hash, total;
 };
 
@@ -299,7 +300,7 @@ __host__ __device__ static inline int rv_align(int v, int a) {
 
 __host__ __device__ static inline RvLayout rv_layout(int dim, int scorer, int m,
                                                     int ks, int max_pool,
-                                                    int hash_log2) {
+                                                    int hash_log2, bool trace) {
     RvLayout l;
     int off = 0;
     l.off_q = off;
@@ -315,6 +316,13 @@ __host__ __device__ static inline RvLayout rv_layout(int dim, int scorer, int m,
     off = rv_align(off + max_pool, 16);
     l.off_flags1 = off;
     off = rv_align(off + max_pool, 16);
+    l.off_born0 = l.off_born1 = -1;
+    if (trace) {
+        l.off_born0 = off;
+        off = rv_align(off + 2 * max_pool, 16);
+        l.off_born1 = off;
+        off = rv_align(off + 2 * max_pool, 16);
+    }
     l.off_cand = off;
     off = rv_align(off + 8 * RV_WAVE, 16);
     l.off_hash = off;
@@ -331,21 +339,25 @@ __device__ __forceinline__ float rv_bf16(uint16_t v) {
 
 // Features of the search state after `step` expansions (identical code
 // path for trace recording and inference).  All inputs are wave-uniform.
+// None depends on the beam width L, so a predictor trained on long (large-L)
+// traces applies to searches of any L >= 2K.
 struct RvStopState {
     float d_entry, top1_0, topk_0;
-    int last_change, admitted_last;
+    int last_change, last_top1, admitted_last;
 };
 
 __device__ __forceinline__ void rv_features(const RvStopState &st,
-                                            const uint64_t *keys, int n, int L,
+                                            const uint64_t *keys, int n,
                                             int label_k, int step, int next_pos,
                                             int degree, float *f) {
     const float eps = 1e-30f;
-    float top1 = rv_key_dist(keys[0]);
-    int kk = (n < label_k ? n : label_k) - 1;
-    float topk = rv_key_dist(keys[kk]);
-    float worst = rv_key_dist(keys[n - 1]);
-    float fmin = next_pos < n ? rv_key_dist(keys[next_pos]) : worst;
+    const int K = n < label_k ? n : label_k;
+    const int k2 = n < 2 * label_k ? n : 2 * label_k;
+    const float top1 = rv_key_dist(keys[0]);
+    const float topk = rv_key_dist(keys[K - 1]);
+    const float d2k = rv_key_dist(keys[k2 - 1]);
+    const float fmin = next_pos < n ? rv_key_dist(keys[next_pos]) : d2k;
+    const int np = next_pos < 4 * label_k ? next_pos : 4 * label_k;
     f[0] = __logf(st.d_entry + 1.0f);
     f[1] = __logf(top1 + 1.0f);
     f[2] = __logf(topk + 1.0f);
@@ -358,15 +370,17 @@ __device__ __forceinline__ void rv_features(const RvStopState &st,
     f[9] = fmin / (topk + eps);
     f[10] = __log2f((float)(step - st.last_change) + 1.0f);
     f[11] = (float)st.admitted_last / (float)degree;
-    f[12] = (float)n / (float)L;
-    f[13] = (float)next_pos / (float)L;
-    f[14] = worst / (topk + eps);
-    f[15] = fmin / (worst + eps);
+    f[12] = __log2f((float)(step - st.last_top1) + 1.0f);
+    f[13] = (float)np / (float)label_k;
+    f[14] = d2k / (topk + eps);
+    f[15] = fmin / (d2k + eps);
 }
 
 // 16 -> 64 -> 64 -> 1 MLP, lane j owns hidden unit j.  FP32 accumulation
-// over BF16 weights; returns the logit (wave-uniform).
-__device__ float rv_mlp_logit(const RvMlp &mlp, const float *f_raw) {
+// over BF16 weights stored input-major (one contiguous 128-byte row per
+// input); the first hidden layer is broadcast through hbuf (64 floats of
+// LDS, 16-byte aligned).  Returns the logit (wave-uniform).
+__device__ float rv_mlp_logit(const RvMlp &mlp, const float *f_raw, float *hbuf) {
     const int j = rv_lane();
     float x[RV_MLP_IN];
 #pragma unroll
@@ -374,14 +388,22 @@ __device__ float rv_mlp_logit(const RvMlp &mlp, const float *f_raw) {
         x[i] = (f_raw[i] - mlp.feat_mean[i]) * mlp.feat_inv_std[i];
     float h1 = mlp.b1[j];
 #pragma unroll
-    for (int i = 0; i < RV_MLP_IN; i++) h1 += rv_bf16(mlp.w1[j * RV_MLP_IN + i]) * x[i];
+    for (int i = 0; i < RV_MLP_IN; i++) h1 += rv_bf16(mlp.w1[i * RV_MLP_HIDDEN + j]) * x[i];
     h1 = h1 > 0.0f ? h1 : 0.0f;
+    hbuf[j] = h1;
+    __syncthreads();
     float h2 = mlp.b2[j];
-#pragma unroll 8
-    for (int i = 0; i < RV_MLP_HIDDEN; i++) {
-        float hi = __shfl(h1, i);
-        h2 += rv_bf16(mlp.w2[j * RV_MLP_HIDDEN + i]) * hi;
+    const float4 *h4 = reinterpret_cast<const float4 *>(hbuf);
+#pragma unroll 4
+    for (int i4 = 0; i4 < RV_MLP_HIDDEN / 4; i4++) {
+        const float4 hv = h4[i4];
+        const uint16_t *w = mlp.w2 + 4 * i4 * RV_MLP_HIDDEN + j;
+        h2 += rv_bf16(w[0]) * hv.x;
+        h2 += rv_bf16(w[RV_MLP_HIDDEN]) * hv.y;
+        h2 += rv_bf16(w[2 * RV_MLP_HIDDEN]) * hv.z;
+        h2 += rv_bf16(w[3 * RV_MLP_HIDDEN]) * hv.w;
     }
+    __syncthreads();  // hbuf is reused by the next merge
     h2 = h2 > 0.0f ? h2 : 0.0f;
     float o = rv_bf16(mlp.w3[j]) * h2;
 #pragma unroll
@@ -391,7 +413,10 @@ __device__ float rv_mlp_logit(const RvMlp &mlp, const float *f_raw) {
 
 // ------------------------------------------------------------- the kernel
 
-template <int SCORER>
+// STOP is the learned-stop mode (b.stop_mode): 0 plain, 1 predict and stop,
+// 2 record traces.  Separate instantiations keep the plain search free of
+// the predictor's code and registers.
+template <int SCORER, int STOP>
 __global__ __launch_bounds__(RV_WAVE) void rv_search_kernel(RvIndex ix, RvBatch b,
                                                             RvLayout lay) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -401,6 +426,11 @@ __global__ __launch_bounds__(RV_WAVE) void rv_search_kernel(RvIndex ix, RvBatch 
     uint64_t *keys_buf[2] = {reinterpret_cast<uint64_t *>(smem + lay.off_keys0),
                              reinterpret_cast<uint64_t *>(smem + lay.off_keys1)};
     uint8_t *flags_buf[2] = {smem + lay.off_flags0, smem + lay.off_flags1};
+    // trace mode: the step at which each pool entry was scored
+    constexpr bool tracing = STOP == 2;
+    uint16_t *born_buf[2] = {
+        tracing ? reinterpret_cast<uint16_t *>(smem + lay.off_born0) : nullptr,
+        tracing ? reinterpret_cast<uint16_t *>(smem + lay.off_born1) : nullptr};
     uint64_t *cand = reinterpret_cast<uint64_t *>(smem + lay.off_cand);
     uint32_t *hash = reinterpret_cast<uint32_t *>(smem + lay.off_hash);
     const int lane = rv_lane();
@@ -420,13 +450,9 @@ __global__ __launch_bounds__(RV_WAVE) void rv_search_kernel(RvIndex ix, RvBatch 
         float *odist = b.out_dists + (size_t)q * b.out_stride;
         const int refine = b.refine_candidates;
         if (L <= 0 || L > b.max_pool || K <= 0 || K > L || K > b.out_stride) {
-            if (lane == 0) {
-                b.out_counts[q] = RV_QUERY_BAD_PARAM;
-                if (b.stop_mode == 2) {
-                    b.trace_meta[2 * (size_t)q] = 0;
-                    b.trace_meta[2 * (size_t)q + 1] = 0;
-                }
-            }
+            if (lane == 0) b.out_counts[q] = RV_QUERY_BAD_PARAM;
+            if (tracing && lane < RV_TRACE_META)
+                b.trace_meta[(size_t)q * RV_TRACE_META + lane] = 0;
             continue;
         }
 
@@ -468,7 +494,9 @@ __global__ __launch_bounds__(RV_WAVE) void rv_search_kernel(RvIndex ix, RvBatch 
         ss.top1_0 = 0.0f;
         ss.topk_0 = 0.0f;
         ss.last_change = 0;
+        ss.last_top1 = 0;
         ss.admitted_last = 0;
+        int born_step = 0;  // scored-at step given to new pool entries
         int trace_n = 0;
 
         // Merge one candidate per lane (key, or RV_KEY_MAX for none) into
@@ -481,6 +509,8 @@ __global__ __launch_bounds__(RV_WAVE) void rv_search_kernel(RvIndex ix, RvBatch 
             uint64_t *dst = keys_buf[cur ^ 1];
             uint8_t *fsrc = flags_buf[cur];
             uint8_t *fdst = flags_buf[cur ^ 1];
+            uint16_t *bsrc = born_buf[cur];
+            uint16_t *bdst = born_buf[cur ^ 1];
             if (n == L && key != RV_KEY_MAX && key >= src[L - 1]) key = RV_KEY_MAX;
             *first_new = 0x7fffffff;
             *admitted = 0;
@@ -517,6 +547,7 @@ __global__ __launch_bounds__(RV_WAVE) void rv_search_kernel(RvIndex ix, RvBatch 
                 if (np < L) {
                     dst[np] = pk;
                     fdst[np] = fsrc[j];
+                    if (tracing) bdst[np] = bsrc[j];
                 }
             }
             int pos = 0x7fffffff;
@@ -525,6 +556,7 @@ __global__ __launch_bounds__(RV_WAVE) void rv_search_kernel(RvIndex ix, RvBatch 
                 if (pos < L) {
                     dst[pos] = key;
                     fdst[pos] = 0;
+                    if (tracing) bdst[pos] = (uint16_t)born_step;
                 } else {
                     pos = 0x7fffffff;
                 }
@@ -627,10 +659,10 @@ __global__ __launch_bounds__(RV_WAVE) void rv_search_kernel(RvIndex ix, RvBatch 
             if (next < 0) break;
 
             // learned stop: features of the current state
-            if (b.stop_mode && step > 0 && step % b.stop_every == 0) {
+            if (STOP != 0 && step > 0 && step % b.stop_every == 0) {
                 float f[RV_MLP_IN];
-                rv_features(ss, keys, n, L, b.stop_label_k, step, next, expand, f);
-                if (b.stop_mode == 2) {
+                rv_features(ss, keys, n, b.stop_label_k, step, next, expand, f);
+                if (STOP == 2) {
                     if (trace_n < b.trace_cap) {
                         if (lane < RV_MLP_IN) {
                             float v = f[0];
@@ -643,7 +675,8 @@ __global__ __launch_bounds__(RV_WAVE) void rv_search_kernel(RvIndex ix, RvBatch 
                     }
                 } else if (step >= b.stop_min_steps) {
                     // the reduction leaves the same logit in every lane
-                    const float logit = rv_mlp_logit(b.mlp, f);
+                    const float logit =
+                        rv_mlp_logit(b.mlp, f, reinterpret_cast<float *>(cand));
                     const float p = 1.0f / (1.0f + __expf(-logit));
                     if (rv_uniform(p >= b.stop_threshold ? 1 : 0)) {
                         stopped = true;
@@ -698,18 +731,42 @@ __global__ __launch_bounds__(RV_WAVE) void rv_search_kernel(RvIndex ix, RvBatch 
             step++;
 
             int first_new, admitted, src_pos;
+            born_step = step < 65535 ? step : 65535;
             merge(key, next, &first_new, &admitted, &src_pos);
             ss.admitted_last = admitted;
             if (first_new < b.stop_label_k) ss.last_change = step;
+            if (first_new == 0) ss.last_top1 = step;
             // everything left of the expanded source is expanded, except the
             // new candidates; resume the scan at the first of either
             scan = first_new < src_pos + 1 ? first_new : src_pos + 1;
         }
         (void)stopped;
 
-        if (b.stop_mode == 2 && lane == 0) {
-            b.trace_meta[2 * (size_t)q] = trace_n;
-            b.trace_meta[2 * (size_t)q + 1] = ss.last_change;
+        if (tracing) {
+            // records, last top-K change, and the scored steps of the final
+            // top-K ascending (padded with the largest); a failed query
+            // keeps no records
+            int32_t *mq = b.trace_meta + (size_t)q * RV_TRACE_META;
+            const int kk = status == 0 ? (n < b.stop_label_k ? n : b.stop_label_k) : 0;
+            const uint16_t *born = born_buf[cur];
+            const int v = lane < kk ? (int)born[lane] : -1;
+            int rank = 0;
+            for (int j = 0; j < kk; j++) {
+                const int u = born[j];
+                rank += (u < v) || (u == v && j < lane);
+            }
+            int mx = v;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const int o = __shfl_xor(mx, off);
+                mx = o > mx ? o : mx;
+            }
+            if (lane < kk) mq[2 + rank] = v;
+            else if (lane < RV_TRACE_TOPK) mq[2 + lane] = mx < 0 ? 0 : mx;
+            if (lane == 0) {
+                mq[0] = status == 0 ? trace_n : 0;
+                mq[1] = ss.last_change;
+            }
         }
 
         // ---- output
@@ -959,43 +1016,70 @@ extern "C" int rv_fill_zero(void *p, size_t bytes, void *stream) {
 }
 
 static RvLayout rv_layout_for(const RvIndex *ix, const RvBatch *b) {
-    return rv_layout(ix->dim, b->scorer, ix->m, ix->ks, b->max_pool, b->hash_log2);
+    return rv_layout(ix->dim, b->scorer, ix->m, ix->ks, b->max_pool, b->hash_log2,
+                     b->stop_mode == 2);
 }
 
 extern "C" size_t rv_search_lds_bytes(const RvIndex *ix, const RvBatch *b) {
     return (size_t)rv_layout_for(ix, b).total;
 }
 
-template <int S>
-static int rv_occupancy(int lds) {
+// One instantiation per (scorer, learned-stop mode).
+template <int S, int T>
+static int rv_occupancy_t(int lds) {
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rv_search_kernel<S>,
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rv_search_kernel<S, T>,
                                                      RV_WAVE, lds) != hipSuccess)
         return 0;
     return blocks;
 }
 
-extern "C" int rv_search_slots_per_cu(const RvIndex *ix, const RvBatch *b) {
-    const int lds = rv_layout_for(ix, b).total;
-    switch (b->scorer) {
-    case RV_SCORE_ADC: return rv_occupancy<RV_SCORE_ADC>(lds);
-    case RV_SCORE_ADC_ONLINE: return rv_occupancy<RV_SCORE_ADC_ONLINE>(lds);
-    case RV_SCORE_SQ: return rv_occupancy<RV_SCORE_SQ>(lds);
+template <int S>
+static int rv_occupancy(int stop, int lds) {
+    switch (stop) {
+    case 0: return rv_occupancy_t<S, 0>(lds);
+    case 1: return rv_occupancy_t<S, 1>(lds);
+    case 2: return rv_occupancy_t<S, 2>(lds);
     }
     return 0;
 }
 
-template <int S>
-static int rv_launch_one(const RvIndex *ix, const RvBatch *b, int grid,
-                         RvLayout lay, hipStream_t s) {
+extern "C" int rv_search_slots_per_cu(const RvIndex *ix, const RvBatch *b) {
+    const int lds = rv_layout_for(ix, b).total;
+    switch (b->scorer) {
+    case RV_SCORE_ADC: return rv_occupancy<RV_SCORE_ADC>(b->stop_mode, lds);
+    case RV_SCORE_ADC_ONLINE: return rv_occupancy<RV_SCORE_ADC_ONLINE>(b->stop_mode, lds);
+    case RV_SCORE_SQ: return rv_occupancy<RV_SCORE_SQ>(b->stop_mode, lds);
+    }
+    return 0;
+}
+
+template <int S, int T>
+static int rv_launch_t(const RvIndex *ix, const RvBatch *b, int grid, RvLayout lay,
+                       hipStream_t s) {
     static int configured = 0;
     if (!configured) {
-        hipFuncSetAttribute((const void *)rv_search_kernel<S>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (rv_check(hipFuncSetAttribute((const void *)rv_search_kernel<S, T>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         160 * 1024),
+                     "hipFuncSetAttribute"))
+            return -1;
         configured = 1;
     }
-    rv_search_kernel<S><<<grid, RV_WAVE, lay.total, s>>>(*ix, *b, lay);
+    rv_search_kernel<S, T><<<grid, RV_WAVE, lay.total, s>>>(*ix, *b, lay);
     return rv_check(hipGetLastError(), "rv_search_kernel launch");
+}
+
+template <int S>
+static int rv_launch_one(const RvIndex *ix, const RvBatch *b, int grid, RvLayout lay,
+                         hipStream_t s) {
+    switch (b->stop_mode) {
+    case 0: return rv_launch_t<S, 0>(ix, b, grid, lay, s);
+    case 1: return rv_launch_t<S, 1>(ix, b, grid, lay, s);
+    case 2: return rv_launch_t<S, 2>(ix, b, grid, lay, s);
+    }
+    snprintf(g_rv_error, sizeof(g_rv_error), "rv_search_launch: unknown stop mode");
+    return -1;
 }
 
 extern "C" int rv_search_launch(const RvIndex *ix, const RvBatch *b,
@@ -1004,8 +1088,10 @@ extern "C" int rv_search_launch(const RvIndex *ix, const RvBatch *b,
     if (!ix || !b || !b->queue || b->nq < 0 || b->hash_log2 < 4 ||
         b->hash_log2 > 16 || b->max_pool <= 0 || b->probe_limit <= 0 ||
         b->n_entries <= 0 || b->expand_count <= 0 || ix->degree > RV_WAVE ||
-        ix->dim <= 0 || ix->dim > 4096 ||
-        (b->stop_mode && (b->stop_every <= 0 || b->stop_label_k <= 0))) {
+        ix->dim <= 0 || ix->dim > 4096 || b->stop_mode < 0 || b->stop_mode > 2 ||
+        (b->stop_mode && (b->stop_every <= 0 || b->stop_label_k <= 0)) ||
+        (b->stop_mode == 2 && (b->stop_label_k > RV_TRACE_TOPK || !b->trace_meta ||
+                               !b->trace_feats || !b->trace_steps || b->trace_cap <= 0))) {
         snprintf(g_rv_error, sizeof(g_rv_error), "rv_search_launch: bad parameters");
         return -1;
     }
@@ -1019,8 +1105,10 @@ extern "C" int rv_search_launch(const RvIndex *ix, const RvBatch *b,
     }
     if (grid_waves <= 0) {
         int dev = 0, cus = 0;
-        hipGetDevice(&dev);
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (rv_check(hipGetDevice(&dev), "hipGetDevice") ||
+            rv_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
+                     "hipDeviceGetAttribute"))
+            return -1;
         int per = rv_search_slots_per_cu(ix, b);
         if (per < 1) per = 1;
         grid_waves = cus * per;

@@ -502,6 +502,31 @@ def kmeans_representatives(x, count, device=None, iters=10, sample=None):
             np.ascontiguousarray(reps, dtype=np.int32))
 
 
+def route_on_host(queries, centroids, rep_ids, S, chunk=8192):
+    """Caller-side entry selection: for every query the representative ids
+    of its S nearest centroids (nearest first), as an int32 [nq, S] array of
+    per-query entry rows for a search request's entry_ids.  Distances are
+    ||q||^2 - 2 q.c + ||c||^2 in float32 BLAS (near ties may order
+    differently from the device router's sequential sums; both paths take
+    whatever rows the caller passes)."""
+    q = np.ascontiguousarray(queries, dtype=np.float32)
+    c = np.ascontiguousarray(centroids, dtype=np.float32)
+    S = int(S)
+    assert 0 < S <= c.shape[0]
+    cn = (c * c).sum(1)
+    out = np.empty((q.shape[0], S), dtype=np.int32)
+    cols = np.arange(S)
+    for s in range(0, q.shape[0], chunk):
+        d = cn[None, :] - 2.0 * (q[s:s + chunk] @ c.T)
+        part = np.argpartition(d, S - 1, axis=1)[:, :S] if S < c.shape[0] else \
+            np.tile(np.arange(c.shape[0]), (d.shape[0], 1))
+        pd = np.take_along_axis(d, part, axis=1)
+        order = np.lexsort((part, pd), axis=1)
+        best = np.take_along_axis(part, order, axis=1)
+        out[s:s + chunk] = rep_ids[best[:, cols]]
+    return out
+
+
 class PqIndexData:
     """Arrays of a pure-PQ index (all host numpy)."""
 

@@ -73,9 +73,10 @@ class RoverSearchBatch(C.Structure):
 class RoverLearnedStopTrain(C.Structure):
     _fields_ = [("queries", C.c_void_p), ("num_queries", C.c_int),
                 ("pool_size", C.c_int), ("correction", C.c_int),
-                ("label_k", C.c_int),
+                ("label_k", C.c_int), ("label_slack", C.c_int),
                 ("entry_points", C.c_int), ("entry_ids", C.c_void_p),
-                ("entry_count", C.c_int), ("route_entries", C.c_int),
+                ("entry_count", C.c_int), ("entry_stride", C.c_int),
+                ("route_entries", C.c_int),
                 ("expand_count", C.c_int),
                 ("stop_every", C.c_int), ("min_steps", C.c_int),
                 ("trace_cap", C.c_int), ("steps", C.c_int), ("batch", C.c_int),

@@ -84,13 +84,14 @@ class Calculator:
         check(native.lib().rover_pure_pq_original_ids(
             self.h, ptr(m), 0 if m is None else len(m)), "rover_pure_pq_original_ids")
 
-    def load_pq_index(self, data, chunk_vectors=None):
-        """Loads a rover_amd.index.PqIndexData (codes, graph, residuals)."""
+    def load_pq_index(self, data, chunk_vectors=None, with_router=True):
+        """Loads a rover_amd.index.PqIndexData (codes, graph, residuals and,
+        with with_router, its entry router)."""
         self.retrieval_pure_pq_load(data.codes, data.codebook, chunk_vectors)
         self.pure_pq_graph_load(data.graph, data.entry)
         if data.res_codes is not None:
             self.pure_pq_residual_load(data.res_codes, data.res_qparams, data.res_bits)
-        if getattr(data, "router_centroids", None) is not None:
+        if with_router and getattr(data, "router_centroids", None) is not None:
             self.pure_pq_router_load(data.router_centroids, data.router_ids)
 
     def pure_pq_router_load(self, centroids, rep_ids):
