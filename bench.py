@@ -18,7 +18,11 @@ residual correction; ADC ranking + final refine) the beam width L is the
 smallest of the sweep reaching recall@10 >= 0.95 against exact float32
 ground truth; the plain search uses the faster mode.  The learned stop
 predictor is trained on traces of that mode's plain search of 200,000
-training queries at L=512 and then swept the same way.
+training queries at L=512, once per label slack (label 1 once all but
+`slack` of the final top-10 are found), each swept the same way; the slack
+reaching the target fastest is reported.  `host_buffers_qps` is the plain
+search through the host-buffer entry point (query upload and result
+download over PCIe included).
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -89,13 +93,22 @@ def main():
     ap.add_argument("--train-queries", type=int, default=200_000)
     ap.add_argument("--train-L", type=int, default=512)
     ap.add_argument("--label-k", type=int, default=10)
-    ap.add_argument("--stop-every", type=int, default=2)
-    ap.add_argument("--trace-cap", type=int, default=512)
+    ap.add_argument("--label-slacks", default="0,1",
+                    help="label slacks to train (label 1 once all but `slack` of the "
+                         "final top-K are found); the best at the target is kept")
+    ap.add_argument("--stop-every", type=int, default=4)
+    ap.add_argument("--trace-cap", type=int, default=24,
+                    help="trace records per training query (steps up to "
+                         "trace-cap * stop-every)")
     ap.add_argument("--train-steps", type=int, default=2000)
     ap.add_argument("--save-index", default="",
                     help="write the built index, queries and ground truth (.npz)")
     ap.add_argument("--load-index", default="",
                     help="reuse an index written by --save-index (no rebuild)")
+    ap.add_argument("--save-model", default="",
+                    help="write the chosen learned-stop model (.npz)")
+    ap.add_argument("--load-model", default="",
+                    help="use a model written by --save-model instead of training")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="also run one plain copy kernel of this many MiB (a "
                          "known-traffic reference for profiler byte counters)")
@@ -241,46 +254,89 @@ def main():
     # ---- learned stop on the same mode
     learned = None
     if not args.no_learned:
-        train_q = data.gaussian_mixture(args.train_queries, args.dim, args.components,
-                                        8.0, 44, centers_seed=42)
-        # the mixture's centres depend only on centers_seed, as for the base
-        train_ents = index.route_on_host(train_q, ix.router_centroids, ix.router_ids, S)
-        log("training: %d queries at L=%d, entry rows chosen" % (len(train_q), args.train_L))
-        t = native.RoverLearnedStopTrain()
-        t.queries = train_q.ctypes.data
-        t.num_queries = train_q.shape[0]
-        t.pool_size = args.train_L
-        t.correction = 1 if mode == "online" else 0
-        t.label_k = args.label_k
-        t.entry_points = 1
-        t.entry_ids = train_ents.ctypes.data
-        t.entry_count = S
-        t.entry_stride = S
-        t.expand_count = args.R
-        t.stop_every = args.stop_every
-        t.min_steps = args.stop_every
-        t.trace_cap = args.trace_cap
-        t.steps = args.train_steps
-        t.batch = 8192
-        t.lr = 3e-3
-        t.seed = 44
-        tt = time.time()
-        native.check(native.lib().rover_learned_stop_train(calc.h, C.byref(t)),
-                     "rover_learned_stop_train")
-        training = dict(train_queries=args.train_queries, train_L=args.train_L,
-                        mode=mode, label_k=args.label_k, stop_every=args.stop_every,
-                        samples=int(t.samples), positive_frac=round(t.positive_frac, 4),
-                        final_loss=round(t.final_loss, 5),
-                        seconds=round(time.time() - tt, 2), threshold=0.9)
-        log("learned stop trained:", training)
-        rows, chosen = sweep(mode, True)
-        if chosen is None:
-            last = rows[-1]
+        if not args.load_model:
+            train_q = data.gaussian_mixture(args.train_queries, args.dim, args.components,
+                                            8.0, 44, centers_seed=42)
+            # the mixture's centres depend only on centers_seed, as for the base
+            train_ents = index.route_on_host(train_q, ix.router_centroids, ix.router_ids, S)
+            log("training: %d queries at L=%d, entry rows chosen"
+                % (len(train_q), args.train_L))
+        P = native.lib().rover_mlp_param_count()
+
+        def set_model(m):
+            native.check(native.lib().rover_learned_stop_set(
+                calc.h, m[0].ctypes.data, P, m[1].ctypes.data, m[2].ctypes.data,
+                args.stop_every, args.label_k, args.stop_every), "rover_learned_stop_set")
+
+        tried = []
+        if args.load_model:
+            z = np.load(args.load_model, allow_pickle=False)
+            model = [z["params"], z["mean"], z["inv_std"]]
+            training = json.loads(str(z["training"]))
+            set_model(model)
+            log("learned stop model loaded from", args.load_model)
+            rows, chosen = sweep(mode, True)
+            tried.append(dict(sweep=rows, chosen=chosen, training=training, model=model))
+        for slack in ([] if args.load_model else
+                      [int(v) for v in args.label_slacks.split(",")]):
+            t = native.RoverLearnedStopTrain()
+            t.queries = train_q.ctypes.data
+            t.num_queries = train_q.shape[0]
+            t.pool_size = args.train_L
+            t.correction = 1 if mode == "online" else 0
+            t.label_k = args.label_k
+            t.label_slack = slack
+            t.entry_points = 1
+            t.entry_ids = train_ents.ctypes.data
+            t.entry_count = S
+            t.entry_stride = S
+            t.expand_count = args.R
+            t.stop_every = args.stop_every
+            t.min_steps = args.stop_every
+            t.trace_cap = args.trace_cap
+            t.steps = args.train_steps
+            t.batch = 8192
+            t.lr = 3e-3
+            t.seed = 44
+            tt = time.time()
+            native.check(native.lib().rover_learned_stop_train(calc.h, C.byref(t)),
+                         "rover_learned_stop_train")
+            training = dict(train_queries=args.train_queries, train_L=args.train_L,
+                            mode=mode, label_k=args.label_k, label_slack=slack,
+                            stop_every=args.stop_every, trace_cap=args.trace_cap,
+                            samples=int(t.samples), positive_frac=round(t.positive_frac, 4),
+                            final_loss=round(t.final_loss, 5),
+                            seconds=round(time.time() - tt, 2), threshold=0.9)
+            log("learned stop trained:", training)
+            model = [np.zeros(P, np.float32), np.zeros(16, np.float32),
+                     np.zeros(16, np.float32)]
+            native.check(native.lib().rover_learned_stop_get(
+                calc.h, model[0].ctypes.data, P, model[1].ctypes.data,
+                model[2].ctypes.data), "rover_learned_stop_get")
+            rows, chosen = sweep(mode, True)
+            tried.append(dict(sweep=rows, chosen=chosen, training=training, model=model))
+        # the slack whose sweep reaches the target fastest (else the best recall)
+        hit = [v for v in tried if v["chosen"]]
+        if hit:
+            learned = max(hit, key=lambda v: v["chosen"]["qps"])
+        else:
+            learned = max(tried, key=lambda v: v["sweep"][-1]["recall10"])
+        set_model(learned["model"])
+        if args.save_model:
+            m = learned["model"]
+            np.savez(args.save_model, params=m[0], mean=m[1], inv_std=m[2],
+                     training=json.dumps(learned["training"]))
+        if learned["chosen"] is None:
+            last = learned["sweep"][-1]
             run(mode, last["L"], stats=True, learned=True)
             native.synchronize()
-            chosen = dict(L=last["L"], recall=last["recall10"], qps=last["qps"],
-                          stats=d_st.download(np.uint64, nq * 4).reshape(nq, 4))
-        learned = dict(sweep=rows, chosen=chosen, training=training)
+            learned["chosen"] = dict(L=last["L"], recall=last["recall10"], qps=last["qps"],
+                                     stats=d_st.download(np.uint64, nq * 4).reshape(nq, 4))
+        log("learned stop: label slack %d chosen" % learned["training"]["label_slack"])
+        others = [dict(label_slack=v["training"]["label_slack"], sweep=v["sweep"])
+                  for v in tried if v is not learned]
+        learned = dict(sweep=learned["sweep"], chosen=learned["chosen"],
+                       training=learned["training"], other_slacks=others)
 
     # ---- timed region: K steps of the full batch, queries resident in HBM
     log("timing plain %s at L=%d: %d warmup + %d steps" % (mode, L_plain, args.warmup,
@@ -299,12 +355,26 @@ def main():
     else:
         ms_per_step, qps, stats, L, label = ms_plain, qps_plain, plain[mode]["chosen"]["stats"], L_plain, "plain"
 
+    # plain search through the host-buffer entry point: queries and entry rows
+    # uploaded, results downloaded, each call synchronous
+    calc.search_batch(queries, L_plain, k, correction=1 if mode == "online" else 0,
+                      refine_candidates=refine_of(mode, L_plain), expand_count=args.R,
+                      entry_ids=ents)
+    t1 = time.time()
+    for _ in range(args.steps):
+        calc.search_batch(queries, L_plain, k, correction=1 if mode == "online" else 0,
+                          refine_candidates=refine_of(mode, L_plain), expand_count=args.R,
+                          entry_ids=ents)
+    host_qps = nq * args.steps / (time.time() - t1)
+    log("plain search from host buffers (PCIe included): %.0f QPS" % host_qps)
+
     code_bytes = index.code_bytes(args.m, 256)
     res_bytes = index.residual_code_bytes(args.dim, args.res_bits)
     algo = algorithmic_bytes(stats, args.R, code_bytes, res_bytes, args.dim, k,
                              refine_of(mode, L), ix.codebook.nbytes, mode == "online")
     achieved = algo / (ms_per_step / 1e3) / 1e9
-    kernel = "rv_search_kernel<%d>" % (1 if mode == "online" else 0)
+    kernel = "rv_search_kernel<%d, %d>" % (1 if mode == "online" else 0,
+                                           1 if label == "learned-stop" else 0)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                 "traffic": None, "kernel": kernel,
@@ -319,31 +389,35 @@ def main():
         threads = min(16, os.cpu_count() or 1)
         # the same per-query entry rows the GPU search takes
         log("cpu baseline: the oracle on %d host threads" % threads)
-        probe = min(nq, 16 * threads)
+        # whole passes over the query set until about cpu_seconds of work
+        passes, searched = 0, 0
         t1 = time.time()
-        pyoracle.search_batch(oix, scorer, queries[:probe], ents[:probe], args.R, L_plain,
-                              k, refine=refine_of(mode, L_plain), threads=threads)
-        rate = probe / max(time.time() - t1, 1e-6)
-        sample = int(min(nq, max(probe, rate * args.cpu_seconds)))
-        t1 = time.time()
-        oids, od, ocnt, _ = pyoracle.search_batch(oix, scorer, queries[:sample],
-                                                  ents[:sample], args.R, L_plain, k,
-                                                  refine=refine_of(mode, L_plain),
-                                                  threads=threads)
-        el = time.time() - t1
+        while True:
+            oids, od, ocnt, _ = pyoracle.search_batch(oix, scorer, queries, ents, args.R,
+                                                      L_plain, k,
+                                                      refine=refine_of(mode, L_plain),
+                                                      threads=threads)
+            passes += 1
+            searched += nq
+            el = time.time() - t1
+            if el >= args.cpu_seconds or passes >= 200:
+                break
+            if passes % 10 == 0:
+                log("cpu baseline: pass %d, %.1f s" % (passes, el))
         run(mode, L_plain)
         native.synchronize()
-        gids = d_ids.download(np.int32, nq * k).reshape(nq, k)[:sample]
-        gd = d_d.download(np.float32, nq * k).reshape(nq, k)[:sample]
+        gids = d_ids.download(np.int32, nq * k).reshape(nq, k)
+        gd = d_d.download(np.float32, nq * k).reshape(nq, k)
         same = float(((gids == oids).all(axis=1) &
                       (gd.view(np.uint32) == od.view(np.uint32)).all(axis=1)).mean())
-        cpu = {"value": round(sample / el, 1), "unit": "queries/s", "cores": threads,
+        cpu = {"value": round(searched / el, 1), "unit": "queries/s", "cores": threads,
                "kind": "port",
-               "sample": "%d of the queries, plain %s search at L=%d, one query per host "
-                         "thread (%.1f s)" % (sample, mode, L_plain, el),
+               "sample": "%d passes over the %d queries (%d searches), plain %s search "
+                         "at L=%d, one query per host thread (%.1f s)"
+                         % (passes, nq, searched, mode, L_plain, el),
                "gpu_results_identical_frac": same}
         log("cpu baseline: %.0f QPS on %d threads; GPU ids+distances identical on %.4f"
-            % (sample / el, threads, same))
+            % (searched / el, threads, same))
 
     def brief(v):
         if not v:
@@ -378,8 +452,10 @@ def main():
             "plain_ms_per_step": round(ms_plain, 4),
             "learned_speedup_vs_plain": round(qps / qps_plain, 3) if learned else None,
             "plain_modes": {m: brief(v) for m, v in plain.items()},
-            "learned": None if not learned else dict(brief(learned),
-                                                      training=learned["training"]),
+            "learned": None if not learned else dict(
+                brief(learned), training=learned["training"],
+                other_slacks=learned["other_slacks"]),
+            "host_buffers_qps": round(host_qps, 1),
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -342,14 +342,15 @@ __device__ __forceinline__ float rv_bf16(uint16_t v) {
 // None depends on the beam width L, so a predictor trained on long (large-L)
 // traces applies to searches of any L >= 2K.
 struct RvStopState {
-    float d_entry, top1_0, topk_0;
-    int last_change, last_top1, admitted_last;
+    float d_entry, d_entry2, top1_0, topk_0;
+    float improve_ema;  // decaying rate of steps that changed the top-K
+    int last_change, last_top1;
 };
 
 __device__ __forceinline__ void rv_features(const RvStopState &st,
                                             const uint64_t *keys, int n,
                                             int label_k, int step, int next_pos,
-                                            int degree, float *f) {
+                                            float *f) {
     const float eps = 1e-30f;
     const int K = n < label_k ? n : label_k;
     const int k2 = n < 2 * label_k ? n : 2 * label_k;
@@ -358,7 +359,7 @@ __device__ __forceinline__ void rv_features(const RvStopState &st,
     const float d2k = rv_key_dist(keys[k2 - 1]);
     const float fmin = next_pos < n ? rv_key_dist(keys[next_pos]) : d2k;
     const int np = next_pos < 4 * label_k ? next_pos : 4 * label_k;
-    f[0] = __logf(st.d_entry + 1.0f);
+    f[0] = st.d_entry2 > 0.0f ? st.d_entry / st.d_entry2 : 1.0f;
     f[1] = __logf(top1 + 1.0f);
     f[2] = __logf(topk + 1.0f);
     f[3] = top1 / (st.d_entry + eps);
@@ -369,7 +370,7 @@ __device__ __forceinline__ void rv_features(const RvStopState &st,
     f[8] = __logf(fmin + 1.0f);
     f[9] = fmin / (topk + eps);
     f[10] = __log2f((float)(step - st.last_change) + 1.0f);
-    f[11] = (float)st.admitted_last / (float)degree;
+    f[11] = st.improve_ema;
     f[12] = __log2f((float)(step - st.last_top1) + 1.0f);
     f[13] = (float)np / (float)label_k;
     f[14] = d2k / (topk + eps);
@@ -491,11 +492,12 @@ __global__ __launch_bounds__(RV_WAVE) void rv_search_kernel(RvIndex ix, RvBatch 
         uint64_t st_scored = 0, st_expanded = 0, st_dedup = 0;
         RvStopState ss;
         ss.d_entry = 0.0f;
+        ss.d_entry2 = 0.0f;
         ss.top1_0 = 0.0f;
         ss.topk_0 = 0.0f;
+        ss.improve_ema = 1.0f;
         ss.last_change = 0;
         ss.last_top1 = 0;
-        ss.admitted_last = 0;
         int born_step = 0;  // scored-at step given to new pool entries
         int trace_n = 0;
 
@@ -613,10 +615,12 @@ __global__ __launch_bounds__(RV_WAVE) void rv_search_kernel(RvIndex ix, RvBatch 
             st_scored += __popcll(__ballot(key != RV_KEY_MAX));
             st_dedup += __popcll(__ballot(dup));
             if (e0 == 0 && lane == 0 && key != RV_KEY_MAX) ss.d_entry = rv_key_dist(key);
+            if (e0 == 0 && lane == 1 && key != RV_KEY_MAX) ss.d_entry2 = rv_key_dist(key);
             int fn, adm, tr;
             merge(key, -1, &fn, &adm, &tr);
         }
         ss.d_entry = __shfl(ss.d_entry, 0);
+        ss.d_entry2 = __shfl(ss.d_entry2, 1);
         if (status == 0 && n == 0) status = RV_QUERY_BAD_ENTRY;
         // A long seed list would fill the visited hash with far-away seeds.
         // Keep only the pool members then: a seed outside the pool that the
@@ -661,7 +665,7 @@ __global__ __launch_bounds__(RV_WAVE) void rv_search_kernel(RvIndex ix, RvBatch 
             // learned stop: features of the current state
             if (STOP != 0 && step > 0 && step % b.stop_every == 0) {
                 float f[RV_MLP_IN];
-                rv_features(ss, keys, n, b.stop_label_k, step, next, expand, f);
+                rv_features(ss, keys, n, b.stop_label_k, step, next, f);
                 if (STOP == 2) {
                     if (trace_n < b.trace_cap) {
                         if (lane < RV_MLP_IN) {
@@ -733,7 +737,8 @@ __global__ __launch_bounds__(RV_WAVE) void rv_search_kernel(RvIndex ix, RvBatch 
             int first_new, admitted, src_pos;
             born_step = step < 65535 ? step : 65535;
             merge(key, next, &first_new, &admitted, &src_pos);
-            ss.admitted_last = admitted;
+            (void)admitted;
+            ss.improve_ema = 0.7f * ss.improve_ema + (first_new < b.stop_label_k ? 0.3f : 0.0f);
             if (first_new < b.stop_label_k) ss.last_change = step;
             if (first_new == 0) ss.last_top1 = step;
             // everything left of the expanded source is expanded, except the
