@@ -373,8 +373,11 @@ def main():
     algo = algorithmic_bytes(stats, args.R, code_bytes, res_bytes, args.dim, k,
                              refine_of(mode, L), ix.codebook.nbytes, mode == "online")
     achieved = algo / (ms_per_step / 1e3) / 1e9
-    kernel = "rv_search_kernel<%d, %d>" % (1 if mode == "online" else 0,
-                                           1 if label == "learned-stop" else 0)
+    # online scoring on 32 x 4-dim PQ with <= 32 neighbors runs the two-lane
+    # instantiation (scorer 3); ADC ranking is scorer 0
+    scorer = (3 if args.m * 4 == args.dim and args.m == 32 and args.R <= 32 else 1) \
+        if mode == "online" else 0
+    kernel = "rv_search_kernel<%d, %d>" % (scorer, 1 if label == "learned-stop" else 0)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                 "traffic": None, "kernel": kernel,

@@ -172,27 +172,39 @@ __device__ __forceinline__ float rv_decode_value(float c, int code, float s,
     return sum + z;
 }
 
-// Online correction for the common shape: M subspaces of SUB dims with
-// 8-bit PQ codes (ks = 256).  All code bytes, residual bytes and centroid
-// vectors of the candidate are loaded before the sequential accumulation,
-// so a candidate costs one round of independent loads instead of M
-// dependent ones.  Arithmetic order is the reference's.
-template <int M, int SUB>
+// Online correction for the common shapes: M subspaces of SUB dims with
+// 8-bit PQ codes (ks = 256) and RB-bit residuals.  The candidate's code row,
+// residual row and qparams are read with 16-byte (8-byte) vector loads and
+// all centroid rows are loaded before the sequential accumulation, so a
+// candidate costs two rounds of independent loads (row data, then codebook
+// rows) instead of M dependent ones.  Arithmetic order is the reference's.
+template <int M, int SUB, int RB>
 __device__ __forceinline__ bool rv_online_fixed(const RvIndex &ix, const float *q,
                                                 int id, float *out) {
     constexpr int D = M * SUB;
-    const float s = ix.res_qparams[2 * (size_t)id];
-    const float z = ix.res_qparams[2 * (size_t)id + 1];
+    constexpr int RBYTES = RB == 4 ? (D + 1) / 2 : D;
+    static_assert(M % 16 == 0 && RBYTES % 16 == 0, "16-byte rows");
+    const float2 qp = *reinterpret_cast<const float2 *>(ix.res_qparams + 2 * (size_t)id);
     uint32_t pw[M / 4];
-    const uint32_t *prow = reinterpret_cast<const uint32_t *>(ix.codes + (size_t)id * M);
+    const uint4 *prow = reinterpret_cast<const uint4 *>(ix.codes + (size_t)id * M);
 #pragma unroll
-    for (int w = 0; w < M / 4; w++) pw[w] = prow[w];
-    constexpr int RW = D / 4;  // 8-bit residual words; 4-bit uses half
-    uint32_t rw[RW];
-    const uint32_t *rrow = reinterpret_cast<const uint32_t *>(ix.res_codes + (size_t)id * ix.res_code_bytes);
-    const int rwords = ix.res_bits == 4 ? D / 8 : D / 4;
+    for (int w = 0; w < M / 16; w++) {
+        const uint4 v = prow[w];
+        pw[4 * w + 0] = v.x;
+        pw[4 * w + 1] = v.y;
+        pw[4 * w + 2] = v.z;
+        pw[4 * w + 3] = v.w;
+    }
+    uint32_t rw[RBYTES / 4];
+    const uint4 *rrow = reinterpret_cast<const uint4 *>(ix.res_codes + (size_t)id * RBYTES);
 #pragma unroll
-    for (int w = 0; w < RW; w++) rw[w] = w < rwords ? rrow[w] : 0u;
+    for (int w = 0; w < RBYTES / 16; w++) {
+        const uint4 v = rrow[w];
+        rw[4 * w + 0] = v.x;
+        rw[4 * w + 1] = v.y;
+        rw[4 * w + 2] = v.z;
+        rw[4 * w + 3] = v.w;
+    }
     float c[D];
 #pragma unroll
     for (int m = 0; m < M; m++) {
@@ -213,12 +225,13 @@ __device__ __forceinline__ bool rv_online_fixed(const RvIndex &ix, const float *
             for (int i = 0; i < SUB; i++) c[m * SUB + i] = cent[i];
         }
     }
+    const float s = qp.x, z = qp.y;
     if (!isfinite(s) || !isfinite(z) || !(s > 0.0f)) return false;
     float dist = 0.0f;
 #pragma unroll
     for (int d = 0; d < D; d++) {
         int code;
-        if (ix.res_bits == 4) code = (rw[d >> 3] >> (4 * (d & 7))) & 15;
+        if (RB == 4) code = (rw[d >> 3] >> (4 * (d & 7))) & 15;
         else code = (rw[d >> 2] >> (8 * (d & 3))) & 255;
         const float rec = rv_decode_value(c[d], code, s, z);
         const float diff = q[d] - rec;
@@ -231,9 +244,13 @@ __device__ __forceinline__ bool rv_online_fixed(const RvIndex &ix, const float *
 // Online correction distance; returns false for invalid qparams.
 __device__ bool rv_online(const RvIndex &ix, const float *q, int id,
                           float *out) {
-    if (ix.ks == 256 && (ix.res_bits == 8 || ix.res_bits == 4)) {
-        if (ix.m == 32 && ix.subdim == 4) return rv_online_fixed<32, 4>(ix, q, id, out);
-        if (ix.m == 48 && ix.subdim == 2) return rv_online_fixed<48, 2>(ix, q, id, out);
+    if (ix.ks == 256 && ix.m == 32 && ix.subdim == 4) {
+        if (ix.res_bits == 8) return rv_online_fixed<32, 4, 8>(ix, q, id, out);
+        if (ix.res_bits == 4) return rv_online_fixed<32, 4, 4>(ix, q, id, out);
+    }
+    if (ix.ks == 256 && ix.m == 48 && ix.subdim == 2) {
+        if (ix.res_bits == 8) return rv_online_fixed<48, 2, 8>(ix, q, id, out);
+        if (ix.res_bits == 4) return rv_online_fixed<48, 2, 4>(ix, q, id, out);
     }
     const uint8_t *pq = ix.codes + (size_t)id * ix.code_bytes;
     const uint8_t *res = ix.res_codes + (size_t)id * ix.res_code_bytes;
@@ -253,6 +270,81 @@ __device__ bool rv_online(const RvIndex &ix, const float *q, int id,
     }
     *out = dist;
     return true;
+}
+
+// Online correction with two lanes per candidate (M = 32 subspaces of 4
+// dims): lane c (< 32) and lane c + 32 score the same candidate.  Each lane
+// decodes its half of the dimensions into squared terms; the low lane sums
+// terms 0..63 sequentially, hands the partial sum to the high lane, which
+// continues with terms 64..127.  The additions happen in exactly the
+// reference's order, so the distance is bit-identical to the one-lane sum.
+// Must be called by the whole wavefront; `active` marks lanes whose
+// candidate is scored.  The result is valid in the high lane.
+template <int RB>
+__device__ __forceinline__ bool rv_online_pair32(const RvIndex &ix, const float *q,
+                                                 int id, bool active, bool high,
+                                                 float *out) {
+    constexpr int MH = 16, DH = 64;
+    constexpr int RH = RB == 4 ? 32 : 64;  // residual bytes per half
+    float t[DH];
+    float s = 1.0f, z = 0.0f;
+    if (active) {
+        const float2 qp = *reinterpret_cast<const float2 *>(ix.res_qparams + 2 * (size_t)id);
+        s = qp.x;
+        z = qp.y;
+        const uint4 pv = *reinterpret_cast<const uint4 *>(ix.codes + (size_t)id * 32 +
+                                                          (high ? 16 : 0));
+        const uint32_t pw[4] = {pv.x, pv.y, pv.z, pv.w};
+        uint32_t rw[RH / 4];
+        const uint4 *rrow = reinterpret_cast<const uint4 *>(
+            ix.res_codes + (size_t)id * (2 * RH) + (high ? RH : 0));
+#pragma unroll
+        for (int w = 0; w < RH / 16; w++) {
+            const uint4 v = rrow[w];
+            rw[4 * w + 0] = v.x;
+            rw[4 * w + 1] = v.y;
+            rw[4 * w + 2] = v.z;
+            rw[4 * w + 3] = v.w;
+        }
+        const int m0 = high ? MH : 0;
+        float4 cv[MH];
+#pragma unroll
+        for (int m = 0; m < MH; m++) {
+            const int code = (pw[m >> 2] >> (8 * (m & 3))) & 255;
+            cv[m] = *reinterpret_cast<const float4 *>(ix.codebook +
+                                                      ((size_t)(m0 + m) * 256 + code) * 4);
+        }
+        const float *qh = q + (high ? DH : 0);
+#pragma unroll
+        for (int d = 0; d < DH; d++) {
+            int code;
+            if (RB == 4) code = (rw[d >> 3] >> (4 * (d & 7))) & 15;
+            else code = (rw[d >> 2] >> (8 * (d & 3))) & 255;
+            const float4 c4 = cv[d >> 2];
+            const float c = (d & 3) == 0 ? c4.x : (d & 3) == 1 ? c4.y : (d & 3) == 2 ? c4.z : c4.w;
+            const float rec = rv_decode_value(c, code, s, z);
+            const float diff = qh[d] - rec;
+            t[d] = diff * diff;
+        }
+    } else {
+#pragma unroll
+        for (int d = 0; d < DH; d++) t[d] = 0.0f;
+    }
+    float acc = 0.0f;
+#pragma unroll
+    for (int d = 0; d < DH; d++) acc += t[d];
+    float dist = __shfl_xor(acc, 32);  // the high lane receives terms 0..63
+#pragma unroll
+    for (int d = 0; d < DH; d++) dist += t[d];
+    *out = dist;
+    return isfinite(s) && isfinite(z) && s > 0.0f;
+}
+
+// Two-lane online scoring is used for 32-subspace x 4-dim PQ with at most
+// 32 neighbors per expansion.
+__host__ __device__ inline bool rv_online_pairable(const RvIndex &ix, int expand) {
+    return expand <= 32 && ix.ks == 256 && ix.m == 32 && ix.subdim == 4 &&
+           (ix.res_bits == 8 || ix.res_bits == 4);
 }
 
 __device__ __forceinline__ float rv_sq(const uint8_t *qsq, const uint8_t *code,
@@ -437,6 +529,7 @@ __global__ __launch_bounds__(RV_WAVE) void rv_search_kernel(RvIndex ix, RvBatch 
     const int lane = rv_lane();
     const int hash_n = 1 << b.hash_log2;
     const int expand = b.expand_count < ix.degree ? b.expand_count : ix.degree;
+    constexpr bool pairable = SCORER == RV_SCORE_ONLINE_PAIR32;
     const float sq_scale_sq = ix.sq_scale * ix.sq_scale;
 
     for (;;) {
@@ -581,11 +674,37 @@ __global__ __launch_bounds__(RV_WAVE) void rv_search_kernel(RvIndex ix, RvBatch 
         // one shared list, or a row of entry_stride ids per query
         const int32_t *ents =
             b.entries + (b.entry_stride > 0 ? (size_t)q * b.entry_stride : 0);
-        for (int e0 = 0; e0 < b.n_entries && status == 0; e0 += RV_WAVE) {
-            const int e = e0 + lane;
+        // with two-lane online scoring, 32 entries per round: lane slot owns
+        // the entry (range and visited tests), lane slot + 32 its key
+        const int per_round = pairable ? 32 : RV_WAVE;
+        const int slot = pairable ? (lane & 31) : lane;
+        const bool owner = !pairable || lane < 32;
+        const int key_lane0 = pairable ? 32 : 0;
+        for (int e0 = 0; e0 < b.n_entries && status == 0; e0 += per_round) {
+            const int e = e0 + slot;
             uint64_t key = RV_KEY_MAX;
             bool bad = false, nonfinite = false, dup = false;
-            if (e < b.n_entries) {
+            if (pairable) {
+                int id = -1, fresh = 0;
+                if (owner && e < b.n_entries) {
+                    id = ents[e];
+                    if (id < 0 || id >= ix.n) bad = true;
+                    else if (!rv_visit(hash, b.hash_log2, b.probe_limit, (uint32_t)id)) dup = true;
+                    else fresh = 1;
+                }
+                id = __shfl(id, slot);
+                fresh = __shfl(fresh, slot);
+                const bool active = fresh != 0;
+                float d = 0.0f;
+                const bool ok =
+                    ix.res_bits == 8
+                        ? rv_online_pair32<8>(ix, qv, id, active, lane >= 32, &d)
+                        : rv_online_pair32<4>(ix, qv, id, active, lane >= 32, &d);
+                if (lane >= 32 && active) {
+                    if (!ok || !isfinite(d) || d == FLT_MAX) nonfinite = true;
+                    else key = rv_key(d + 0.0f, id);
+                }
+            } else if (e < b.n_entries) {
                 const int id = ents[e];
                 if (id < 0 || id >= ix.n) {
                     bad = true;
@@ -614,13 +733,14 @@ __global__ __launch_bounds__(RV_WAVE) void rv_search_kernel(RvIndex ix, RvBatch 
             if (status) break;
             st_scored += __popcll(__ballot(key != RV_KEY_MAX));
             st_dedup += __popcll(__ballot(dup));
-            if (e0 == 0 && lane == 0 && key != RV_KEY_MAX) ss.d_entry = rv_key_dist(key);
-            if (e0 == 0 && lane == 1 && key != RV_KEY_MAX) ss.d_entry2 = rv_key_dist(key);
+            if (e0 == 0 && lane == key_lane0 && key != RV_KEY_MAX) ss.d_entry = rv_key_dist(key);
+            if (e0 == 0 && lane == key_lane0 + 1 && key != RV_KEY_MAX)
+                ss.d_entry2 = rv_key_dist(key);
             int fn, adm, tr;
             merge(key, -1, &fn, &adm, &tr);
         }
-        ss.d_entry = __shfl(ss.d_entry, 0);
-        ss.d_entry2 = __shfl(ss.d_entry2, 1);
+        ss.d_entry = __shfl(ss.d_entry, key_lane0);
+        ss.d_entry2 = __shfl(ss.d_entry2, key_lane0 + 1);
         if (status == 0 && n == 0) status = RV_QUERY_BAD_ENTRY;
         // A long seed list would fill the visited hash with far-away seeds.
         // Keep only the pool members then: a seed outside the pool that the
@@ -706,7 +826,28 @@ __global__ __launch_bounds__(RV_WAVE) void rv_search_kernel(RvIndex ix, RvBatch 
             }
             uint64_t key = RV_KEY_MAX;
             bool dup = false, nonfinite = false;
-            if (lane < valid_n) {
+            if (pairable) {
+                // neighbor `slot` is scored by lanes slot and slot + 32; the
+                // low lane owns the visited test, the high lane the key
+                const int slot = lane & 31;
+                const int nbs = __shfl(nb, slot);
+                int fresh = 0;
+                if (lane < 32 && slot < valid_n) {
+                    fresh = rv_visit(hash, b.hash_log2, b.probe_limit, (uint32_t)nbs) ? 1 : 0;
+                    dup = !fresh;
+                }
+                fresh = __shfl(fresh, slot);
+                const bool active = slot < valid_n && fresh;
+                float d = 0.0f;
+                const bool ok =
+                    ix.res_bits == 8
+                        ? rv_online_pair32<8>(ix, qv, nbs, active, lane >= 32, &d)
+                        : rv_online_pair32<4>(ix, qv, nbs, active, lane >= 32, &d);
+                if (lane >= 32 && active) {
+                    if (!ok || !isfinite(d) || d == FLT_MAX) nonfinite = true;
+                    else key = rv_key(d + 0.0f, nbs);
+                }
+            } else if (lane < valid_n) {
                 if (!rv_visit(hash, b.hash_log2, b.probe_limit, (uint32_t)nb)) {
                     dup = true;
                 } else {
@@ -1055,6 +1196,8 @@ extern "C" int rv_search_slots_per_cu(const RvIndex *ix, const RvBatch *b) {
     case RV_SCORE_ADC: return rv_occupancy<RV_SCORE_ADC>(b->stop_mode, lds);
     case RV_SCORE_ADC_ONLINE: return rv_occupancy<RV_SCORE_ADC_ONLINE>(b->stop_mode, lds);
     case RV_SCORE_SQ: return rv_occupancy<RV_SCORE_SQ>(b->stop_mode, lds);
+    case RV_SCORE_ONLINE_PAIR32:
+        return rv_occupancy<RV_SCORE_ONLINE_PAIR32>(b->stop_mode, lds);
     }
     return 0;
 }
@@ -1087,10 +1230,17 @@ static int rv_launch_one(const RvIndex *ix, const RvBatch *b, int grid, RvLayout
     return -1;
 }
 
-extern "C" int rv_search_launch(const RvIndex *ix, const RvBatch *b,
+extern "C" int rv_search_launch(const RvIndex *ix, const RvBatch *b_in,
                                 int grid_waves, void *stream) {
     hipStream_t s = (hipStream_t)stream;
-    if (!ix || !b || !b->queue || b->nq < 0 || b->hash_log2 < 4 ||
+    if (!b_in) return -1;
+    RvBatch bb = *b_in;
+    const RvBatch *b = &bb;
+    // online correction on the 32 x 4-dim shape: two lanes per candidate
+    if (ix && bb.scorer == RV_SCORE_ADC_ONLINE &&
+        rv_online_pairable(*ix, bb.expand_count < ix->degree ? bb.expand_count : ix->degree))
+        bb.scorer = RV_SCORE_ONLINE_PAIR32;
+    if (!ix || !b->queue || b->nq < 0 || b->hash_log2 < 4 ||
         b->hash_log2 > 16 || b->max_pool <= 0 || b->probe_limit <= 0 ||
         b->n_entries <= 0 || b->expand_count <= 0 || ix->degree > RV_WAVE ||
         ix->dim <= 0 || ix->dim > 4096 || b->stop_mode < 0 || b->stop_mode > 2 ||
@@ -1125,6 +1275,8 @@ extern "C" int rv_search_launch(const RvIndex *ix, const RvBatch *b,
     case RV_SCORE_ADC_ONLINE:
         return rv_launch_one<RV_SCORE_ADC_ONLINE>(ix, b, grid_waves, lay, s);
     case RV_SCORE_SQ: return rv_launch_one<RV_SCORE_SQ>(ix, b, grid_waves, lay, s);
+    case RV_SCORE_ONLINE_PAIR32:
+        return rv_launch_one<RV_SCORE_ONLINE_PAIR32>(ix, b, grid_waves, lay, s);
     }
     snprintf(g_rv_error, sizeof(g_rv_error), "rv_search_launch: unknown scorer");
     return -1;

@@ -203,6 +203,37 @@ def test_sq_quantize_distance_search(golden):
                     assert tuple(st) == tuple(int(x) for x in s[key + "_stats"][qi])
 
 
+@pytest.mark.parametrize("bits", [8, 4])
+def test_sq_encoder_closed_form(golden, bits):
+    """The global-range SQ encoder (retrieval_rpc_demo_support.c:573-592
+    range -> scale, :664-700 per-value code) is pinned by what it must
+    satisfy rather than by a fixture of its own output: the global minimum
+    codes to 0 and the maximum to the top level, every value decodes to
+    within half a step, codes are monotone in the value, and 4-bit codes put
+    even dimensions in the low nibble (the layout the reference distance,
+    pinned by the golden SQ distances, reads)."""
+    x = golden("pq_d128_m32")["base"][:400]
+    levels = (1 << bits) - 1
+    scale, zero = pyoracle.sq_range_params(x, bits)
+    assert zero == np.float32(x.min())
+    assert abs(float(scale) * levels - float(x.max() - x.min())) <= 1e-5 * float(x.max())
+    codes = pyoracle.sq_encode(x, bits, scale, zero)
+    if bits == 4:
+        q = np.empty(x.shape, dtype=np.int64)
+        q[:, 0::2] = codes[:, : (x.shape[1] + 1) // 2] & 15
+        q[:, 1::2] = codes[:, : x.shape[1] // 2] >> 4
+    else:
+        q = codes.astype(np.int64)
+    assert q.min() == 0 and q.max() == levels
+    assert q.flat[np.argmin(x)] == 0 and q.flat[np.argmax(x)] == levels
+    deq = q * np.float64(scale) + np.float64(zero)
+    assert np.abs(deq - x).max() <= 0.5 * float(scale) * (1 + 1e-4)
+    order = np.argsort(x, axis=None, kind="stable")
+    assert np.all(np.diff(q.ravel()[order]) >= 0)
+    # the product encoder agrees with the oracle's
+    assert np.array_equal(rindex.encode_sq(x, bits, scale, zero), codes)
+
+
 def test_seed_lcg(golden):
     """The pure-PQ seed LCG against the reference's SQ seed generator, which
     shares the recurrence: for n = 65536 both map the state s to s >> 16.
