@@ -17,9 +17,10 @@ enum {
     RV_SCORE_ADC = 0,        /* PQ ADC ranking (correction off)           */
     RV_SCORE_ADC_ONLINE = 1, /* residual-reconstructed L2 (online)        */
     RV_SCORE_SQ = 2,         /* global-scale SQ8/SQ4 integer code distance */
-    RV_SCORE_ONLINE_PAIR32 = 3 /* online, two lanes per candidate; chosen by
-                                  rv_search_launch for M=32 x 4-dim, ks=256,
-                                  4/8-bit residuals, <= 32 neighbors */
+    RV_SCORE_ONLINE_PAIR32 = 3,   /* online, two lanes per candidate; chosen
+                                     by rv_search_launch for M=32 x 4-dim,
+                                     ks=256, 8-bit residuals, <= 32 neighbors */
+    RV_SCORE_ONLINE_PAIR32_R4 = 4 /* the same with 4-bit residuals */
 };
 
 /* Status written per query to out_counts when negative. */

@@ -39,6 +39,7 @@
 #include <string.h>
 
 #include "rover_hip.h"
+#include "rv_wave.h"
 
 #define RV_WAVE 64
 #define RV_EMPTY_SLOT 0xffffffffu
@@ -78,24 +79,6 @@ __device__ __forceinline__ uint64_t rv_uniform64(uint64_t v) {
 }
 __device__ __forceinline__ int rv_uniform(int v) {
     return __builtin_amdgcn_readfirstlane(v);
-}
-
-// Ascending bitonic sort of one 64-bit key per lane across the wavefront.
-__device__ __forceinline__ uint64_t rv_wave_sort(uint64_t key) {
-    const int lane = rv_lane();
-#pragma unroll
-    for (int k = 2; k <= RV_WAVE; k <<= 1) {
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            uint64_t other = __shfl_xor(key, j);
-            bool up = (lane & k) == 0;
-            bool lower = (lane & j) == 0;
-            uint64_t lo = key < other ? key : other;
-            uint64_t hi = key < other ? other : key;
-            key = (lower == up) ? lo : hi;
-        }
-    }
-    return key;
 }
 
 // Number of keys in sorted arr[0..n) strictly smaller than key.
@@ -279,16 +262,19 @@ __device__ bool rv_online(const RvIndex &ix, const float *q, int id,
 // continues with terms 64..127.  The additions happen in exactly the
 // reference's order, so the distance is bit-identical to the one-lane sum.
 // Must be called by the whole wavefront; `active` marks lanes whose
-// candidate is scored.  The result is valid in the high lane.
+// candidate is scored (the others score row 0, so that every lane runs the
+// same instructions, and ignore the result).  The result is valid in the
+// high lane.
 template <int RB>
 __device__ __forceinline__ bool rv_online_pair32(const RvIndex &ix, const float *q,
-                                                 int id, bool active, bool high,
+                                                 int id_in, bool active, bool high,
                                                  float *out) {
     constexpr int MH = 16, DH = 64;
     constexpr int RH = RB == 4 ? 32 : 64;  // residual bytes per half
+    const int id = active ? id_in : 0;
     float t[DH];
-    float s = 1.0f, z = 0.0f;
-    if (active) {
+    float s, z;
+    {
         const float2 qp = *reinterpret_cast<const float2 *>(ix.res_qparams + 2 * (size_t)id);
         s = qp.x;
         z = qp.y;
@@ -326,9 +312,6 @@ __device__ __forceinline__ bool rv_online_pair32(const RvIndex &ix, const float 
             const float diff = qh[d] - rec;
             t[d] = diff * diff;
         }
-    } else {
-#pragma unroll
-        for (int d = 0; d < DH; d++) t[d] = 0.0f;
     }
     float acc = 0.0f;
 #pragma unroll
@@ -510,8 +493,11 @@ __device__ float rv_mlp_logit(const RvMlp &mlp, const float *f_raw, float *hbuf)
 // 2 record traces.  Separate instantiations keep the plain search free of
 // the predictor's code and registers.
 template <int SCORER, int STOP>
-__global__ __launch_bounds__(RV_WAVE) void rv_search_kernel(RvIndex ix, RvBatch b,
-                                                            RvLayout lay) {
+// The plain two-lane online instantiation is held to 128 VGPRs (4 waves per
+// SIMD), which it fits without spilling.
+__global__ __launch_bounds__(RV_WAVE)
+__attribute__((amdgpu_waves_per_eu(SCORER == RV_SCORE_ONLINE_PAIR32 && STOP == 0 ? 4 : 1)))
+void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
     extern __shared__ __align__(16) unsigned char smem[];
     float *qv = reinterpret_cast<float *>(smem + lay.off_q);
     float *lut = reinterpret_cast<float *>(smem + lay.off_tab);
@@ -529,7 +515,9 @@ __global__ __launch_bounds__(RV_WAVE) void rv_search_kernel(RvIndex ix, RvBatch 
     const int lane = rv_lane();
     const int hash_n = 1 << b.hash_log2;
     const int expand = b.expand_count < ix.degree ? b.expand_count : ix.degree;
-    constexpr bool pairable = SCORER == RV_SCORE_ONLINE_PAIR32;
+    constexpr bool pairable =
+        SCORER == RV_SCORE_ONLINE_PAIR32 || SCORER == RV_SCORE_ONLINE_PAIR32_R4;
+    constexpr int pair_rb = SCORER == RV_SCORE_ONLINE_PAIR32_R4 ? 4 : 8;
     const float sq_scale_sq = ix.sq_scale * ix.sq_scale;
 
     for (;;) {
@@ -598,7 +586,9 @@ __global__ __launch_bounds__(RV_WAVE) void rv_search_kernel(RvIndex ix, RvBatch 
         // the pool.  Returns the lowest position a new candidate landed at
         // (or INT_MAX) via *first_new, the count admitted, and the new
         // position of the pool entry at old position `track` via *tracked.
-        auto merge = [&](uint64_t key, int track, int *first_new, int *admitted,
+        // span: where the candidate keys are (0: lanes 0..31, 1: lanes
+        // 32..63, 2: anywhere); the other lanes hold RV_KEY_MAX
+        auto merge = [&](uint64_t key, int span, int track, int *first_new, int *admitted,
                          int *tracked) {
             uint64_t *src = keys_buf[cur];
             uint64_t *dst = keys_buf[cur ^ 1];
@@ -611,8 +601,17 @@ __global__ __launch_bounds__(RV_WAVE) void rv_search_kernel(RvIndex ix, RvBatch 
             *admitted = 0;
             *tracked = track;
             if (__ballot(key != RV_KEY_MAX) == 0ull) return;
-            key = rv_wave_sort(key);
-            uint64_t prev = __shfl_up(key, 1);
+            if (span == 2) {
+                key = rv_sort_wave(key);
+            } else {
+                key = rv_sort_halves(key);
+                if (span == 1) {
+                    // the sorted upper half moves to lanes 0..31
+                    const uint64_t up = rv_xor_lane64<32>(key);
+                    key = lane < 32 ? up : RV_KEY_MAX;
+                }
+            }
+            const uint64_t prev = rv_prev_lane64(key);
             bool valid = key != RV_KEY_MAX && !(lane > 0 && prev == key);
             int lb = 0;
             if (valid) {
@@ -656,14 +655,9 @@ __global__ __launch_bounds__(RV_WAVE) void rv_search_kernel(RvIndex ix, RvBatch 
                     pos = 0x7fffffff;
                 }
             }
-            // wave min of pos
-            int mn = pos;
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                int o = __shfl_xor(mn, off);
-                mn = o < mn ? o : mn;
-            }
-            *first_new = rv_uniform(mn);
+            // keys are sorted and the valid ones ranked in lane order, so the
+            // lowest new position is the first valid lane's
+            *first_new = __builtin_amdgcn_readlane(pos, __ffsll((unsigned long long)vmask) - 1);
             *admitted = rv_uniform(__popcll(__ballot(pos != 0x7fffffff)));
             n = (n + cv) < L ? (n + cv) : L;
             cur ^= 1;
@@ -696,10 +690,7 @@ __global__ __launch_bounds__(RV_WAVE) void rv_search_kernel(RvIndex ix, RvBatch 
                 fresh = __shfl(fresh, slot);
                 const bool active = fresh != 0;
                 float d = 0.0f;
-                const bool ok =
-                    ix.res_bits == 8
-                        ? rv_online_pair32<8>(ix, qv, id, active, lane >= 32, &d)
-                        : rv_online_pair32<4>(ix, qv, id, active, lane >= 32, &d);
+                const bool ok = rv_online_pair32<pair_rb>(ix, qv, id, active, lane >= 32, &d);
                 if (lane >= 32 && active) {
                     if (!ok || !isfinite(d) || d == FLT_MAX) nonfinite = true;
                     else key = rv_key(d + 0.0f, id);
@@ -737,7 +728,7 @@ __global__ __launch_bounds__(RV_WAVE) void rv_search_kernel(RvIndex ix, RvBatch 
             if (e0 == 0 && lane == key_lane0 + 1 && key != RV_KEY_MAX)
                 ss.d_entry2 = rv_key_dist(key);
             int fn, adm, tr;
-            merge(key, -1, &fn, &adm, &tr);
+            merge(key, pairable ? 1 : 2, -1, &fn, &adm, &tr);
         }
         ss.d_entry = __shfl(ss.d_entry, key_lane0);
         ss.d_entry2 = __shfl(ss.d_entry2, key_lane0 + 1);
@@ -839,10 +830,7 @@ __global__ __launch_bounds__(RV_WAVE) void rv_search_kernel(RvIndex ix, RvBatch 
                 fresh = __shfl(fresh, slot);
                 const bool active = slot < valid_n && fresh;
                 float d = 0.0f;
-                const bool ok =
-                    ix.res_bits == 8
-                        ? rv_online_pair32<8>(ix, qv, nbs, active, lane >= 32, &d)
-                        : rv_online_pair32<4>(ix, qv, nbs, active, lane >= 32, &d);
+                const bool ok = rv_online_pair32<pair_rb>(ix, qv, nbs, active, lane >= 32, &d);
                 if (lane >= 32 && active) {
                     if (!ok || !isfinite(d) || d == FLT_MAX) nonfinite = true;
                     else key = rv_key(d + 0.0f, nbs);
@@ -877,7 +865,8 @@ __global__ __launch_bounds__(RV_WAVE) void rv_search_kernel(RvIndex ix, RvBatch 
 
             int first_new, admitted, src_pos;
             born_step = step < 65535 ? step : 65535;
-            merge(key, next, &first_new, &admitted, &src_pos);
+            merge(key, pairable ? 1 : (expand <= 32 ? 0 : 2), next, &first_new, &admitted,
+                  &src_pos);
             (void)admitted;
             ss.improve_ema = 0.7f * ss.improve_ema + (first_new < b.stop_label_k ? 0.3f : 0.0f);
             if (first_new < b.stop_label_k) ss.last_change = step;
@@ -1198,6 +1187,8 @@ extern "C" int rv_search_slots_per_cu(const RvIndex *ix, const RvBatch *b) {
     case RV_SCORE_SQ: return rv_occupancy<RV_SCORE_SQ>(b->stop_mode, lds);
     case RV_SCORE_ONLINE_PAIR32:
         return rv_occupancy<RV_SCORE_ONLINE_PAIR32>(b->stop_mode, lds);
+    case RV_SCORE_ONLINE_PAIR32_R4:
+        return rv_occupancy<RV_SCORE_ONLINE_PAIR32_R4>(b->stop_mode, lds);
     }
     return 0;
 }
@@ -1239,7 +1230,7 @@ extern "C" int rv_search_launch(const RvIndex *ix, const RvBatch *b_in,
     // online correction on the 32 x 4-dim shape: two lanes per candidate
     if (ix && bb.scorer == RV_SCORE_ADC_ONLINE &&
         rv_online_pairable(*ix, bb.expand_count < ix->degree ? bb.expand_count : ix->degree))
-        bb.scorer = RV_SCORE_ONLINE_PAIR32;
+        bb.scorer = ix->res_bits == 4 ? RV_SCORE_ONLINE_PAIR32_R4 : RV_SCORE_ONLINE_PAIR32;
     if (!ix || !b->queue || b->nq < 0 || b->hash_log2 < 4 ||
         b->hash_log2 > 16 || b->max_pool <= 0 || b->probe_limit <= 0 ||
         b->n_entries <= 0 || b->expand_count <= 0 || ix->degree > RV_WAVE ||
@@ -1277,6 +1268,8 @@ extern "C" int rv_search_launch(const RvIndex *ix, const RvBatch *b_in,
     case RV_SCORE_SQ: return rv_launch_one<RV_SCORE_SQ>(ix, b, grid_waves, lay, s);
     case RV_SCORE_ONLINE_PAIR32:
         return rv_launch_one<RV_SCORE_ONLINE_PAIR32>(ix, b, grid_waves, lay, s);
+    case RV_SCORE_ONLINE_PAIR32_R4:
+        return rv_launch_one<RV_SCORE_ONLINE_PAIR32_R4>(ix, b, grid_waves, lay, s);
     }
     snprintf(g_rv_error, sizeof(g_rv_error), "rv_search_launch: unknown scorer");
     return -1;
