@@ -366,6 +366,7 @@ __device__ __forceinline__ float rv_sq(const uint8_t *qsq, const uint8_t *code,
 struct RvLayout {
     int off_q, off_tab, off_keys0, off_keys1, off_flags0, off_flags1;
     int off_born0, off_born1;  // trace mode: step each pool entry was scored
+    int off_stop;              // learned-stop state of the current query
     int off_cand, off_hash, total;
 };
 
@@ -398,6 +399,8 @@ __host__ __device__ static inline RvLayout rv_layout(int dim, int scorer, int m,
         l.off_born1 = off;
         off = rv_align(off + 2 * max_pool, 16);
     }
+    l.off_stop = off;
+    off = rv_align(off + 64, 16);
     l.off_cand = off;
     off = rv_align(off + 8 * RV_WAVE, 16);
     l.off_hash = off;
@@ -456,8 +459,14 @@ __device__ __forceinline__ void rv_features(const RvStopState &st,
 // over BF16 weights stored input-major (one contiguous 128-byte row per
 // input); the first hidden layer is broadcast through hbuf (64 floats of
 // LDS, 16-byte aligned).  Returns the logit (wave-uniform).
-__device__ float rv_mlp_logit(const RvMlp &mlp, const float *f_raw, float *hbuf) {
+__device__ float rv_mlp_logit(const RvMlp &mlp_in, const float *f_raw, float *hbuf) {
     const int j = rv_lane();
+    // The weights are re-read at every evaluation: the empty asm hides the
+    // pointers' loop invariance, so the loads are not hoisted out of the
+    // search loop into registers the scoring needs.
+    RvMlp mlp = mlp_in;
+    asm volatile("" : "+s"(mlp.w1), "+s"(mlp.w2), "+s"(mlp.w3), "+s"(mlp.b1));
+    asm volatile("" : "+s"(mlp.b2), "+s"(mlp.b3), "+s"(mlp.feat_mean), "+s"(mlp.feat_inv_std));
     float x[RV_MLP_IN];
 #pragma unroll
     for (int i = 0; i < RV_MLP_IN; i++)
@@ -571,14 +580,18 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
         int n = 0;          // pool size
         int status = 0;
         uint64_t st_scored = 0, st_expanded = 0, st_dedup = 0;
-        RvStopState ss;
-        ss.d_entry = 0.0f;
-        ss.d_entry2 = 0.0f;
-        ss.top1_0 = 0.0f;
-        ss.topk_0 = 0.0f;
-        ss.improve_ema = 1.0f;
-        ss.last_change = 0;
-        ss.last_top1 = 0;
+        // learned-stop state lives in LDS: it is touched once per expansion
+        // and kept out of the registers the scoring needs
+        RvStopState &ss = *reinterpret_cast<RvStopState *>(smem + lay.off_stop);
+        if (STOP != 0) {
+            ss.d_entry = 0.0f;
+            ss.d_entry2 = 0.0f;
+            ss.top1_0 = 0.0f;
+            ss.topk_0 = 0.0f;
+            ss.improve_ema = 1.0f;
+            ss.last_change = 0;
+            ss.last_top1 = 0;
+        }
         int born_step = 0;  // scored-at step given to new pool entries
         int trace_n = 0;
 
@@ -724,14 +737,13 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
             if (status) break;
             st_scored += __popcll(__ballot(key != RV_KEY_MAX));
             st_dedup += __popcll(__ballot(dup));
-            if (e0 == 0 && lane == key_lane0 && key != RV_KEY_MAX) ss.d_entry = rv_key_dist(key);
-            if (e0 == 0 && lane == key_lane0 + 1 && key != RV_KEY_MAX)
+            if (STOP != 0 && e0 == 0 && lane == key_lane0 && key != RV_KEY_MAX)
+                ss.d_entry = rv_key_dist(key);
+            if (STOP != 0 && e0 == 0 && lane == key_lane0 + 1 && key != RV_KEY_MAX)
                 ss.d_entry2 = rv_key_dist(key);
             int fn, adm, tr;
             merge(key, pairable ? 1 : 2, -1, &fn, &adm, &tr);
         }
-        ss.d_entry = __shfl(ss.d_entry, key_lane0);
-        ss.d_entry2 = __shfl(ss.d_entry2, key_lane0 + 1);
         if (status == 0 && n == 0) status = RV_QUERY_BAD_ENTRY;
         // A long seed list would fill the visited hash with far-away seeds.
         // Keep only the pool members then: a seed outside the pool that the
@@ -746,7 +758,7 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
                 rv_visit(hash, b.hash_log2, b.probe_limit, (uint32_t)rv_key_id(k0[i]));
             __syncthreads();
         }
-        if (status == 0) {
+        if (STOP != 0 && status == 0) {
             const uint64_t *k0 = keys_buf[cur];
             ss.top1_0 = rv_key_dist(k0[0]);
             ss.topk_0 = rv_key_dist(k0[(n < b.stop_label_k ? n : b.stop_label_k) - 1]);
@@ -868,9 +880,12 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
             merge(key, pairable ? 1 : (expand <= 32 ? 0 : 2), next, &first_new, &admitted,
                   &src_pos);
             (void)admitted;
-            ss.improve_ema = 0.7f * ss.improve_ema + (first_new < b.stop_label_k ? 0.3f : 0.0f);
-            if (first_new < b.stop_label_k) ss.last_change = step;
-            if (first_new == 0) ss.last_top1 = step;
+            if (STOP != 0) {
+                ss.improve_ema =
+                    0.7f * ss.improve_ema + (first_new < b.stop_label_k ? 0.3f : 0.0f);
+                if (first_new < b.stop_label_k) ss.last_change = step;
+                if (first_new == 0) ss.last_top1 = step;
+            }
             // everything left of the expanded source is expanded, except the
             // new candidates; resume the scan at the first of either
             scan = first_new < src_pos + 1 ? first_new : src_pos + 1;
