@@ -258,6 +258,41 @@ AEEResult rover_search_batch(remote_handle64 h, const RoverSearchBatch *b);
 AEEResult rover_search_batch_device(remote_handle64 h,
                                     const RoverSearchBatch *b, void *stream);
 
+/* Batch search captured in a HIP graph (no reference counterpart): the
+ * work-queue reset and the search kernel of one device batch, captured on
+ * `stream` (a stream from rover_stream_create, not the default stream) and
+ * instantiated.  The graph keeps every pointer of `b` and reads the buffers
+ * at each replay, so callers refill queries / per-query L and k / entry rows
+ * in place and replay.  b->num_queries is the capacity; with
+ * d_num_queries != NULL the kernel reads the live count (<= capacity) from
+ * that device int32 at each replay, so one graph serves any batch size.
+ * The graph has its own work counter; it stays valid while the session's
+ * index is loaded.  route_entries is not supported (pass entry rows). */
+AEEResult rover_search_graph_create(remote_handle64 h, const RoverSearchBatch *b,
+                                    const int32_t *d_num_queries, void *stream,
+                                    void **graph);
+AEEResult rover_search_graph_launch(void *graph, void *stream);
+AEEResult rover_search_graph_destroy(void *graph);
+
+/* Autotuner (no reference counterpart): times the device batch `b` under
+ * launch configurations -- visited-hash size (2^10, 2^12, 2^14 slots and the
+ * default), probe limit (4, 8, 16) and grid fill (all query slots of every
+ * CU, 3/4, 1/2) -- `repeats` launches each after one warm-up, keeps only
+ * configurations whose ids, distances and counts equal the default
+ * configuration's, and returns the fastest.  Results are left in b's output
+ * buffers. */
+typedef struct RoverTuning {
+    int hash_log2;
+    int probe_limit;
+    int grid_waves;
+    float ms;          /* mean launch time of the chosen configuration */
+    float default_ms;  /* the same for the default configuration       */
+    int tried;         /* configurations timed                         */
+    int rejected;      /* configurations whose results differed        */
+} RoverTuning;
+AEEResult rover_autotune(remote_handle64 h, const RoverSearchBatch *b, void *stream,
+                         int repeats, RoverTuning *out);
+
 /* ------------------------------------------- learned early termination */
 
 /* Trains the stop predictor (16 -> 64 -> 64 -> 1 MLP, BF16 weights, FP32

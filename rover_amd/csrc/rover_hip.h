@@ -90,6 +90,9 @@ typedef struct RvBatch {
     uint64_t *out_stats;         /* [nq][4] scored, corrected, expanded,
                                     dedup; may be NULL                      */
     int32_t *queue;              /* device work counter, zeroed per launch  */
+    const int32_t *nq_dev;       /* optional: live query count in device
+                                    memory (<= nq), read at kernel start, so a
+                                    captured graph serves any batch size    */
     /* learned stop (stop_mode 0 = off) */
     int stop_mode;               /* 1: infer and stop; 2: record traces     */
     RvMlp mlp;
@@ -177,6 +180,16 @@ int rv_copy(void *d_dst, const void *d_src, size_t bytes, void *stream);
 /* Entry router: for each query the S (<= 16) nearest of nc centroids by
  * exact squared L2, written as their representative node ids
  * (out[nq][S], nearest first).  scratch holds nq*nc floats. */
+/* HIP graph capture of a launch sequence on `stream` (not the default
+ * stream): begin, launch as usual, end -> instantiated executable. */
+int rv_graph_begin(void *stream);
+int rv_graph_end(void *stream, void **exec);
+int rv_graph_launch(void *exec, void *stream);
+int rv_graph_destroy(void *exec);
+/* Sets the search kernel's attributes for this batch ahead of a capture and
+ * returns the occupancy-derived grid (waves) it would use. */
+int rv_search_prepare(const RvIndex *index, const RvBatch *batch);
+
 int rv_route_entries(const float *d_queries, int nq, int dim,
                      const float *d_centroids, int nc, const int32_t *d_rep_ids,
                      int S, float *d_scratch, int32_t *d_out, void *stream);

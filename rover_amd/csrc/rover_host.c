@@ -607,6 +607,233 @@ AEEResult rover_search_batch_device(remote_handle64 h, const RoverSearchBatch *b
     return rc;
 }
 
+/* ------------------------------------------------------------ graphs */
+
+#define ROVER_GRAPH_MAGIC 0x52564750u /* "RVGP" */
+
+typedef struct RoverGraph {
+    uint32_t magic;
+    void *exec;         /* instantiated HIP graph                    */
+    int32_t *d_queue;   /* the graph's own work counter              */
+    int32_t *d_entries; /* copy of the LCG seed list, when used      */
+} RoverGraph;
+
+static void graph_free(RoverGraph *g) {
+    if (!g) return;
+    if (g->exec) rv_graph_destroy(g->exec);
+    dfree(g->d_queue);
+    dfree(g->d_entries);
+    g->magic = 0;
+    free(g);
+}
+
+AEEResult rover_search_graph_create(remote_handle64 h, const RoverSearchBatch *b,
+                                    const int32_t *d_num_queries, void *stream,
+                                    void **graph) {
+    RoverCtx *c = ctx_of(h);
+    RvIndex ix;
+    RvBatch rb;
+    RoverGraph *g;
+    AEEResult rc;
+    int grid;
+    if (!c || !b || !stream || !graph || b->route_entries > 0 || b->num_queries <= 0)
+        return ROVER_EBADPARM;
+    *graph = NULL;
+    g = calloc(1, sizeof(*g));
+    if (!g) return ROVER_ENOMEMORY;
+    pthread_mutex_lock(&c->lock);
+    rc = rover_prepare_launch(c, b, b->pool_size, 0, &ix, &rb);
+    if (rc != ROVER_SUCCESS) goto fail;
+    rc = ROVER_ENOMEMORY;
+    if (rv_malloc((void **)&g->d_queue, 64)) {
+        set_err("graph work counter");
+        goto fail;
+    }
+    if (!b->entry_ids) {
+        /* the session's seed list may be replaced later: keep a copy */
+        if (rv_malloc((void **)&g->d_entries, (size_t)rb.n_entries * sizeof(int32_t)) ||
+            rv_memcpy_d2d(g->d_entries, rb.entries, (size_t)rb.n_entries * sizeof(int32_t))) {
+            set_err("graph seed list");
+            goto fail;
+        }
+        rb.entries = g->d_entries;
+    }
+    rb.queue = g->d_queue;
+    rb.nq_dev = d_num_queries;
+    rc = ROVER_EFAILED;
+    /* attributes and grid are settled before the capture begins */
+    grid = rv_search_prepare(&ix, &rb);
+    if (grid <= 0) {
+        set_err("graph prepare");
+        goto fail;
+    }
+    if (b->grid_waves > 0) grid = b->grid_waves;
+    if (rv_graph_begin(stream)) {
+        set_err("graph capture");
+        goto fail;
+    }
+    {
+        const int launch_rc = rv_search_launch(&ix, &rb, grid, stream);
+        const int end_rc = rv_graph_end(stream, &g->exec);
+        if (launch_rc || end_rc) {
+            set_err("graph capture");
+            goto fail;
+        }
+    }
+    g->magic = ROVER_GRAPH_MAGIC;
+    pthread_mutex_unlock(&c->lock);
+    *graph = g;
+    return ROVER_SUCCESS;
+fail:
+    pthread_mutex_unlock(&c->lock);
+    graph_free(g);
+    return rc;
+}
+
+AEEResult rover_search_graph_launch(void *graph, void *stream) {
+    RoverGraph *g = (RoverGraph *)graph;
+    if (!g || g->magic != ROVER_GRAPH_MAGIC || !stream) return ROVER_EBADPARM;
+    if (rv_graph_launch(g->exec, stream)) {
+        set_err("graph launch");
+        return ROVER_EFAILED;
+    }
+    return ROVER_SUCCESS;
+}
+
+AEEResult rover_search_graph_destroy(void *graph) {
+    RoverGraph *g = (RoverGraph *)graph;
+    if (!g || g->magic != ROVER_GRAPH_MAGIC) return ROVER_EBADPARM;
+    graph_free(g);
+    return ROVER_SUCCESS;
+}
+
+/* ------------------------------------------------------------ autotuner */
+
+/* Mean launch time of `repeats` launches after one warm-up, or -1. */
+static float time_launches(const RvIndex *ix, const RvBatch *rb, int grid, void *stream,
+                           int repeats, void *e0, void *e1) {
+    float ms = 0.0f;
+    if (rv_search_launch(ix, rb, grid, stream) || rv_event_record(e0, stream)) return -1.0f;
+    for (int r = 0; r < repeats; r++)
+        if (rv_search_launch(ix, rb, grid, stream)) return -1.0f;
+    if (rv_event_record(e1, stream) || rv_event_elapsed(e0, e1, &ms)) return -1.0f;
+    return ms / (float)repeats;
+}
+
+AEEResult rover_autotune(remote_handle64 h, const RoverSearchBatch *b, void *stream,
+                         int repeats, RoverTuning *out) {
+    static const int hashes[4] = {0, 10, 12, 14};
+    static const int probes[3] = {4, 8, 16};
+    static const int fill_num[3] = {4, 3, 2}; /* grid fill in quarters */
+    RoverCtx *c = ctx_of(h);
+    RvIndex ix;
+    RvBatch rb, cand;
+    AEEResult rc;
+    void *e0 = NULL, *e1 = NULL;
+    int32_t *ref_ids = NULL, *got_ids = NULL, *ref_cnt = NULL, *got_cnt = NULL;
+    float *ref_d = NULL, *got_d = NULL;
+    size_t nout, bytes_out;
+    if (!c || !b || !out || b->num_queries <= 0 || b->route_entries > 0)
+        return ROVER_EBADPARM;
+    if (repeats <= 0) repeats = 3;
+    memset(out, 0, sizeof(*out));
+    nout = (size_t)b->num_queries * (size_t)b->out_stride;
+    bytes_out = nout * sizeof(int32_t);
+    pthread_mutex_lock(&c->lock);
+    rc = rover_prepare_launch(c, b, b->pool_size, 0, &ix, &rb);
+    if (rc != ROVER_SUCCESS) goto out;
+    rc = ROVER_ENOMEMORY;
+    ref_ids = malloc(bytes_out);
+    got_ids = malloc(bytes_out);
+    ref_d = malloc(bytes_out);
+    got_d = malloc(bytes_out);
+    ref_cnt = malloc((size_t)b->num_queries * sizeof(int32_t));
+    got_cnt = malloc((size_t)b->num_queries * sizeof(int32_t));
+    if (!ref_ids || !got_ids || !ref_d || !got_d || !ref_cnt || !got_cnt) goto out;
+    rc = ROVER_EFAILED;
+    if (rv_event_create(&e0) || rv_event_create(&e1)) {
+        set_err("autotune events");
+        goto out;
+    }
+    /* the default configuration sets the reference results and time */
+    out->hash_log2 = rb.hash_log2;
+    out->probe_limit = rb.probe_limit;
+    out->grid_waves = rv_search_prepare(&ix, &rb);
+    if (out->grid_waves <= 0) {
+        set_err("autotune prepare");
+        goto out;
+    }
+    out->default_ms = time_launches(&ix, &rb, out->grid_waves, stream, repeats, e0, e1);
+    if (out->default_ms < 0.0f || rv_stream_synchronize(stream) ||
+        rv_memcpy_d2h(ref_ids, rb.out_ids, bytes_out) ||
+        rv_memcpy_d2h(ref_d, rb.out_dists, bytes_out) ||
+        rv_memcpy_d2h(ref_cnt, rb.out_counts, (size_t)b->num_queries * sizeof(int32_t))) {
+        set_err("autotune reference run");
+        goto out;
+    }
+    out->ms = out->default_ms;
+    out->tried = 1;
+    for (int hi = 0; hi < 4; hi++) {
+        for (int pi = 0; pi < 3; pi++) {
+            for (int fi = 0; fi < 3; fi++) {
+                int full, grid;
+                float ms;
+                cand = rb;
+                if (hashes[hi]) cand.hash_log2 = hashes[hi];
+                cand.probe_limit = probes[pi];
+                if (cand.hash_log2 == rb.hash_log2 && cand.probe_limit == rb.probe_limit &&
+                    fi == 0)
+                    continue; /* the default, already timed */
+                full = rv_search_prepare(&ix, &cand);
+                if (full <= 0) continue; /* does not fit in LDS */
+                grid = full * fill_num[fi] / 4;
+                if (grid < 1) grid = 1;
+                ms = time_launches(&ix, &cand, grid, stream, repeats, e0, e1);
+                if (ms < 0.0f || rv_stream_synchronize(stream) ||
+                    rv_memcpy_d2h(got_ids, cand.out_ids, bytes_out) ||
+                    rv_memcpy_d2h(got_d, cand.out_dists, bytes_out) ||
+                    rv_memcpy_d2h(got_cnt, cand.out_counts,
+                                  (size_t)b->num_queries * sizeof(int32_t))) {
+                    set_err("autotune run");
+                    goto out;
+                }
+                out->tried++;
+                if (memcmp(got_ids, ref_ids, bytes_out) || memcmp(got_d, ref_d, bytes_out) ||
+                    memcmp(got_cnt, ref_cnt, (size_t)b->num_queries * sizeof(int32_t))) {
+                    out->rejected++;
+                    continue;
+                }
+                if (ms < out->ms) {
+                    out->ms = ms;
+                    out->hash_log2 = cand.hash_log2;
+                    out->probe_limit = cand.probe_limit;
+                    out->grid_waves = grid;
+                }
+            }
+        }
+    }
+    /* leave the chosen configuration's results in the output buffers */
+    cand = rb;
+    cand.hash_log2 = out->hash_log2;
+    cand.probe_limit = out->probe_limit;
+    if (rv_search_launch(&ix, &cand, out->grid_waves, stream) || rv_stream_synchronize(stream)) {
+        set_err("autotune final run");
+        goto out;
+    }
+    rc = ROVER_SUCCESS;
+out:
+    pthread_mutex_unlock(&c->lock);
+    if (e0) rv_event_destroy(e0);
+    if (e1) rv_event_destroy(e1);
+    free(ref_ids);
+    free(got_ids);
+    free(ref_d);
+    free(got_d);
+    free(ref_cnt);
+    free(got_cnt);
+    return rc;
+}
+
 static int ensure_stage(RoverCtx *c, size_t bytes) {
     if (bytes <= c->stage_cap) return 0;
     dfree(c->d_stage);

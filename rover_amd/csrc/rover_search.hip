@@ -529,11 +529,17 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
     constexpr int pair_rb = SCORER == RV_SCORE_ONLINE_PAIR32_R4 ? 4 : 8;
     const float sq_scale_sq = ix.sq_scale * ix.sq_scale;
 
+    int nq = b.nq;
+    if (b.nq_dev) {
+        const int live = *b.nq_dev;
+        nq = live < nq ? (live > 0 ? live : 0) : nq;
+    }
+    nq = rv_uniform(nq);
     for (;;) {
         int q = 0;
         if (lane == 0) q = atomicAdd(b.queue, 1);
         q = rv_uniform(__shfl(q, 0));
-        if (q >= b.nq) break;
+        if (q >= nq) break;
 
         const int L = b.pool_sizes ? b.pool_sizes[q] : b.pool_size;
         const int K = b.num_outs ? b.num_outs[q] : b.num_out;
@@ -1170,9 +1176,6 @@ static RvLayout rv_layout_for(const RvIndex *ix, const RvBatch *b) {
                      b->stop_mode == 2);
 }
 
-extern "C" size_t rv_search_lds_bytes(const RvIndex *ix, const RvBatch *b) {
-    return (size_t)rv_layout_for(ix, b).total;
-}
 
 // One instantiation per (scorer, learned-stop mode).
 template <int S, int T>
@@ -1184,33 +1187,8 @@ static int rv_occupancy_t(int lds) {
     return blocks;
 }
 
-template <int S>
-static int rv_occupancy(int stop, int lds) {
-    switch (stop) {
-    case 0: return rv_occupancy_t<S, 0>(lds);
-    case 1: return rv_occupancy_t<S, 1>(lds);
-    case 2: return rv_occupancy_t<S, 2>(lds);
-    }
-    return 0;
-}
-
-extern "C" int rv_search_slots_per_cu(const RvIndex *ix, const RvBatch *b) {
-    const int lds = rv_layout_for(ix, b).total;
-    switch (b->scorer) {
-    case RV_SCORE_ADC: return rv_occupancy<RV_SCORE_ADC>(b->stop_mode, lds);
-    case RV_SCORE_ADC_ONLINE: return rv_occupancy<RV_SCORE_ADC_ONLINE>(b->stop_mode, lds);
-    case RV_SCORE_SQ: return rv_occupancy<RV_SCORE_SQ>(b->stop_mode, lds);
-    case RV_SCORE_ONLINE_PAIR32:
-        return rv_occupancy<RV_SCORE_ONLINE_PAIR32>(b->stop_mode, lds);
-    case RV_SCORE_ONLINE_PAIR32_R4:
-        return rv_occupancy<RV_SCORE_ONLINE_PAIR32_R4>(b->stop_mode, lds);
-    }
-    return 0;
-}
-
 template <int S, int T>
-static int rv_launch_t(const RvIndex *ix, const RvBatch *b, int grid, RvLayout lay,
-                       hipStream_t s) {
+static int rv_configure_t() {
     static int configured = 0;
     if (!configured) {
         if (rv_check(hipFuncSetAttribute((const void *)rv_search_kernel<S, T>,
@@ -1220,74 +1198,159 @@ static int rv_launch_t(const RvIndex *ix, const RvBatch *b, int grid, RvLayout l
             return -1;
         configured = 1;
     }
+    return 0;
+}
+
+template <int S, int T>
+static int rv_launch_t(const RvIndex *ix, const RvBatch *b, int grid, RvLayout lay,
+                       hipStream_t s) {
     rv_search_kernel<S, T><<<grid, RV_WAVE, lay.total, s>>>(*ix, *b, lay);
     return rv_check(hipGetLastError(), "rv_search_kernel launch");
 }
 
+// op 0: configure attributes, 1: occupancy (blocks per CU), 2: launch
 template <int S>
-static int rv_launch_one(const RvIndex *ix, const RvBatch *b, int grid, RvLayout lay,
-                         hipStream_t s) {
+static int rv_dispatch_stop(int op, const RvIndex *ix, const RvBatch *b, int grid,
+                            RvLayout lay, hipStream_t s) {
     switch (b->stop_mode) {
-    case 0: return rv_launch_t<S, 0>(ix, b, grid, lay, s);
-    case 1: return rv_launch_t<S, 1>(ix, b, grid, lay, s);
-    case 2: return rv_launch_t<S, 2>(ix, b, grid, lay, s);
+    case 0:
+        return op == 0 ? rv_configure_t<S, 0>()
+                       : op == 1 ? rv_occupancy_t<S, 0>(lay.total)
+                                 : rv_launch_t<S, 0>(ix, b, grid, lay, s);
+    case 1:
+        return op == 0 ? rv_configure_t<S, 1>()
+                       : op == 1 ? rv_occupancy_t<S, 1>(lay.total)
+                                 : rv_launch_t<S, 1>(ix, b, grid, lay, s);
+    case 2:
+        return op == 0 ? rv_configure_t<S, 2>()
+                       : op == 1 ? rv_occupancy_t<S, 2>(lay.total)
+                                 : rv_launch_t<S, 2>(ix, b, grid, lay, s);
     }
-    snprintf(g_rv_error, sizeof(g_rv_error), "rv_search_launch: unknown stop mode");
+    snprintf(g_rv_error, sizeof(g_rv_error), "rv_search: unknown stop mode");
     return -1;
 }
 
-extern "C" int rv_search_launch(const RvIndex *ix, const RvBatch *b_in,
-                                int grid_waves, void *stream) {
-    hipStream_t s = (hipStream_t)stream;
-    if (!b_in) return -1;
-    RvBatch bb = *b_in;
-    const RvBatch *b = &bb;
-    // online correction on the 32 x 4-dim shape: two lanes per candidate
-    if (ix && bb.scorer == RV_SCORE_ADC_ONLINE &&
-        rv_online_pairable(*ix, bb.expand_count < ix->degree ? bb.expand_count : ix->degree))
-        bb.scorer = ix->res_bits == 4 ? RV_SCORE_ONLINE_PAIR32_R4 : RV_SCORE_ONLINE_PAIR32;
-    if (!ix || !b->queue || b->nq < 0 || b->hash_log2 < 4 ||
-        b->hash_log2 > 16 || b->max_pool <= 0 || b->probe_limit <= 0 ||
-        b->n_entries <= 0 || b->expand_count <= 0 || ix->degree > RV_WAVE ||
-        ix->dim <= 0 || ix->dim > 4096 || b->stop_mode < 0 || b->stop_mode > 2 ||
-        (b->stop_mode && (b->stop_every <= 0 || b->stop_label_k <= 0)) ||
-        (b->stop_mode == 2 && (b->stop_label_k > RV_TRACE_TOPK || !b->trace_meta ||
-                               !b->trace_feats || !b->trace_steps || b->trace_cap <= 0))) {
+static int rv_dispatch(int op, const RvIndex *ix, const RvBatch *b, int grid, RvLayout lay,
+                       hipStream_t s) {
+    switch (b->scorer) {
+    case RV_SCORE_ADC: return rv_dispatch_stop<RV_SCORE_ADC>(op, ix, b, grid, lay, s);
+    case RV_SCORE_ADC_ONLINE:
+        return rv_dispatch_stop<RV_SCORE_ADC_ONLINE>(op, ix, b, grid, lay, s);
+    case RV_SCORE_SQ: return rv_dispatch_stop<RV_SCORE_SQ>(op, ix, b, grid, lay, s);
+    case RV_SCORE_ONLINE_PAIR32:
+        return rv_dispatch_stop<RV_SCORE_ONLINE_PAIR32>(op, ix, b, grid, lay, s);
+    case RV_SCORE_ONLINE_PAIR32_R4:
+        return rv_dispatch_stop<RV_SCORE_ONLINE_PAIR32_R4>(op, ix, b, grid, lay, s);
+    }
+    snprintf(g_rv_error, sizeof(g_rv_error), "rv_search: unknown scorer");
+    return -1;
+}
+
+// Validates a batch, picks the kernel instantiation and its LDS layout.
+static int rv_resolve(const RvIndex *ix, const RvBatch *b_in, RvBatch *bb, RvLayout *lay) {
+    if (!ix || !b_in || !b_in->queue || b_in->nq < 0 || b_in->hash_log2 < 4 ||
+        b_in->hash_log2 > 16 || b_in->max_pool <= 0 || b_in->probe_limit <= 0 ||
+        b_in->n_entries <= 0 || b_in->expand_count <= 0 || ix->degree > RV_WAVE ||
+        ix->dim <= 0 || ix->dim > 4096 || b_in->stop_mode < 0 || b_in->stop_mode > 2 ||
+        (b_in->stop_mode && (b_in->stop_every <= 0 || b_in->stop_label_k <= 0)) ||
+        (b_in->stop_mode == 2 &&
+         (b_in->stop_label_k > RV_TRACE_TOPK || !b_in->trace_meta || !b_in->trace_feats ||
+          !b_in->trace_steps || b_in->trace_cap <= 0))) {
         snprintf(g_rv_error, sizeof(g_rv_error), "rv_search_launch: bad parameters");
         return -1;
     }
-    if (b->nq == 0) return 0;
-    const RvLayout lay = rv_layout_for(ix, b);
-    if (lay.total > 160 * 1024) {
+    *bb = *b_in;
+    // online correction on the 32 x 4-dim shape: two lanes per candidate
+    if (bb->scorer == RV_SCORE_ADC_ONLINE &&
+        rv_online_pairable(*ix, bb->expand_count < ix->degree ? bb->expand_count : ix->degree))
+        bb->scorer = ix->res_bits == 4 ? RV_SCORE_ONLINE_PAIR32_R4 : RV_SCORE_ONLINE_PAIR32;
+    *lay = rv_layout_for(ix, bb);
+    if (lay->total > 160 * 1024) {
         snprintf(g_rv_error, sizeof(g_rv_error),
-                 "rv_search_launch: %d bytes of LDS per query exceed 160 KiB",
-                 lay.total);
+                 "rv_search_launch: %d bytes of LDS per query exceed 160 KiB", lay->total);
         return -1;
     }
-    if (grid_waves <= 0) {
-        int dev = 0, cus = 0;
-        if (rv_check(hipGetDevice(&dev), "hipGetDevice") ||
-            rv_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
-                     "hipDeviceGetAttribute"))
-            return -1;
-        int per = rv_search_slots_per_cu(ix, b);
-        if (per < 1) per = 1;
-        grid_waves = cus * per;
+    return 0;
+}
+
+extern "C" size_t rv_search_lds_bytes(const RvIndex *ix, const RvBatch *b) {
+    RvBatch bb;
+    RvLayout lay;
+    if (rv_resolve(ix, b, &bb, &lay)) return 0;
+    return (size_t)lay.total;
+}
+
+extern "C" int rv_search_slots_per_cu(const RvIndex *ix, const RvBatch *b) {
+    RvBatch bb;
+    RvLayout lay;
+    if (rv_resolve(ix, b, &bb, &lay)) return 0;
+    return rv_dispatch(1, ix, &bb, 0, lay, nullptr);
+}
+
+// Occupancy-derived grid: every CU filled with as many query slots as its
+// registers and LDS allow, capped at the batch size.
+static int rv_default_grid(const RvIndex *ix, const RvBatch *bb, RvLayout lay) {
+    int dev = 0, cus = 0;
+    if (rv_check(hipGetDevice(&dev), "hipGetDevice") ||
+        rv_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
+                 "hipDeviceGetAttribute"))
+        return -1;
+    int per = rv_dispatch(1, ix, bb, 0, lay, nullptr);
+    if (per < 1) per = 1;
+    int grid = cus * per;
+    if (grid > bb->nq) grid = bb->nq;
+    return grid < 1 ? 1 : grid;
+}
+
+extern "C" int rv_search_prepare(const RvIndex *ix, const RvBatch *b) {
+    RvBatch bb;
+    RvLayout lay;
+    if (rv_resolve(ix, b, &bb, &lay) || rv_dispatch(0, ix, &bb, 0, lay, nullptr)) return -1;
+    return rv_default_grid(ix, &bb, lay);
+}
+
+extern "C" int rv_search_launch(const RvIndex *ix, const RvBatch *b_in, int grid_waves,
+                                void *stream) {
+    RvBatch bb;
+    RvLayout lay;
+    if (rv_resolve(ix, b_in, &bb, &lay)) return -1;
+    if (bb.nq == 0) return 0;
+    if (rv_dispatch(0, ix, &bb, 0, lay, nullptr)) return -1;
+    if (grid_waves <= 0 && (grid_waves = rv_default_grid(ix, &bb, lay)) < 0) return -1;
+    if (grid_waves > bb.nq) grid_waves = bb.nq;
+    if (rv_fill_zero(bb.queue, sizeof(int32_t), stream)) return -1;
+    return rv_dispatch(2, ix, &bb, grid_waves, lay, (hipStream_t)stream);
+}
+
+extern "C" int rv_graph_begin(void *stream) {
+    if (!stream) {
+        snprintf(g_rv_error, sizeof(g_rv_error), "graph capture needs a created stream");
+        return -1;
     }
-    if (grid_waves > b->nq) grid_waves = b->nq;
-    if (rv_fill_zero(b->queue, sizeof(int32_t), stream)) return -1;
-    switch (b->scorer) {
-    case RV_SCORE_ADC: return rv_launch_one<RV_SCORE_ADC>(ix, b, grid_waves, lay, s);
-    case RV_SCORE_ADC_ONLINE:
-        return rv_launch_one<RV_SCORE_ADC_ONLINE>(ix, b, grid_waves, lay, s);
-    case RV_SCORE_SQ: return rv_launch_one<RV_SCORE_SQ>(ix, b, grid_waves, lay, s);
-    case RV_SCORE_ONLINE_PAIR32:
-        return rv_launch_one<RV_SCORE_ONLINE_PAIR32>(ix, b, grid_waves, lay, s);
-    case RV_SCORE_ONLINE_PAIR32_R4:
-        return rv_launch_one<RV_SCORE_ONLINE_PAIR32_R4>(ix, b, grid_waves, lay, s);
-    }
-    snprintf(g_rv_error, sizeof(g_rv_error), "rv_search_launch: unknown scorer");
-    return -1;
+    return rv_check(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal),
+                    "hipStreamBeginCapture");
+}
+
+extern "C" int rv_graph_end(void *stream, void **exec) {
+    hipGraph_t g = nullptr;
+    *exec = nullptr;
+    if (rv_check(hipStreamEndCapture((hipStream_t)stream, &g), "hipStreamEndCapture"))
+        return -1;
+    hipGraphExec_t e = nullptr;
+    const int rc = rv_check(hipGraphInstantiate(&e, g, nullptr, nullptr, 0),
+                            "hipGraphInstantiate");
+    (void)hipGraphDestroy(g);
+    if (rc) return -1;
+    *exec = (void *)e;
+    return 0;
+}
+
+extern "C" int rv_graph_launch(void *exec, void *stream) {
+    return rv_check(hipGraphLaunch((hipGraphExec_t)exec, (hipStream_t)stream), "hipGraphLaunch");
+}
+
+extern "C" int rv_graph_destroy(void *exec) {
+    return rv_check(hipGraphExecDestroy((hipGraphExec_t)exec), "hipGraphExecDestroy");
 }
 
 extern "C" int rv_pq_encode(const float *x, int n, int dim, int m, int ks,

@@ -85,6 +85,12 @@ class RoverLearnedStopTrain(C.Structure):
                 ("samples", C.c_long), ("positive_frac", C.c_float)]
 
 
+class RoverTuning(C.Structure):
+    _fields_ = [("hash_log2", C.c_int), ("probe_limit", C.c_int),
+                ("grid_waves", C.c_int), ("ms", C.c_float), ("default_ms", C.c_float),
+                ("tried", C.c_int), ("rejected", C.c_int)]
+
+
 # (name, restype, argtypes); every symbol include/rover_gpu.h declares.
 _H = C.c_uint64
 _P = C.c_void_p
@@ -118,6 +124,10 @@ SIGNATURES = [
     ("rover_pure_pq_view_release", None, []),
     ("rover_search_batch", _I, [_H, _P]),
     ("rover_search_batch_device", _I, [_H, _P, _P]),
+    ("rover_search_graph_create", _I, [_H, _P, _P, _P, C.POINTER(_P)]),
+    ("rover_search_graph_launch", _I, [_P, _P]),
+    ("rover_search_graph_destroy", _I, [_P]),
+    ("rover_autotune", _I, [_H, _P, _P, _I, _P]),
     ("rover_make_seed_ids", _I, [_I, _I, _I, C.c_uint32, _P]),
     ("rover_learned_stop_train", _I, [_H, _P]),
     ("rover_learned_stop_set", _I, [_H, _P, _I, _P, _P, _I, _I, _I]),

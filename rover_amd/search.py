@@ -228,6 +228,53 @@ class Calculator:
               "rover_search_batch_device")
 
 
+    def search_graph_create(self, d_queries, capacity, pool_size, num_out, d_ids,
+                            d_dists, d_counts, stride, stream, d_num_queries=None,
+                            method=native.METHOD_PQ, correction=0, refine_candidates=0,
+                            entry_points=1, expand_count=32, d_stats=None, hash_log2=0,
+                            probe_limit=0, grid_waves=0, learned_stop=0,
+                            stop_threshold=0.0, d_pool_sizes=None, d_num_outs=None,
+                            seed=native.ENTRY_POINT_SEED, d_entry_ids=None,
+                            entry_count=0, entry_stride=0):
+        """Captures a device batch in a HIP graph (rover_search_graph_create);
+        returns the graph handle for search_graph_launch."""
+        b = self._batch_struct(d_queries, capacity, pool_size, num_out, method,
+                               correction, refine_candidates, entry_points,
+                               expand_count, d_ids, d_dists, d_counts, stride,
+                               d_stats, hash_log2, probe_limit, grid_waves,
+                               learned_stop, stop_threshold, d_pool_sizes,
+                               d_num_outs, seed, d_entry_ids, entry_count,
+                               entry_stride, 0)
+        g = C.c_void_p()
+        check(native.lib().rover_search_graph_create(self.h, C.byref(b), d_num_queries,
+                                                     stream, C.byref(g)),
+              "rover_search_graph_create")
+        return g.value
+
+    @staticmethod
+    def search_graph_launch(graph, stream):
+        check(native.lib().rover_search_graph_launch(graph, stream), "rover_search_graph_launch")
+
+    @staticmethod
+    def search_graph_destroy(graph):
+        check(native.lib().rover_search_graph_destroy(graph), "rover_search_graph_destroy")
+
+    def autotune(self, d_queries, nq, pool_size, num_out, d_ids, d_dists, d_counts,
+                 stride, stream=None, repeats=3, method=native.METHOD_PQ, correction=0,
+                 refine_candidates=0, entry_points=1, expand_count=32,
+                 d_entry_ids=None, entry_count=0, entry_stride=0,
+                 seed=native.ENTRY_POINT_SEED):
+        """rover_autotune on a device batch; returns native.RoverTuning."""
+        b = self._batch_struct(d_queries, nq, pool_size, num_out, method, correction,
+                               refine_candidates, entry_points, expand_count, d_ids,
+                               d_dists, d_counts, stride, None, 0, 0, 0, 0, 0.0, None,
+                               None, seed, d_entry_ids, entry_count, entry_stride, 0)
+        t = native.RoverTuning()
+        check(native.lib().rover_autotune(self.h, C.byref(b), stream, repeats, C.byref(t)),
+              "rover_autotune")
+        return t
+
+
 def pure_pq_search_scalar(graph, codes, codebook, query, entry_ids,
                           expand_count, pool_capacity, num_out,
                           chunk_vectors=None, residual=None):
