@@ -64,6 +64,84 @@ def algorithmic_bytes(stats, R, code_bytes, res_bytes, dim, k, refine_F,
     return float(b.sum() + codebook_bytes)
 
 
+def batching_leg(calc, queries, ents, S, mode, refine_of, args, cap=4096, kmax=100,
+                 lmax=512):
+    """Uneven batches through one captured HIP graph: batch size uniform in
+    1..4096, per-query k from {1, 10, 100} and beam width uniform in
+    max(16, k)..512 (k <= L is required), queries drawn from the query set.
+    Each batch uploads its queries, entry rows, L and k and the live count,
+    replays the graph and downloads the results; wall latency includes those
+    transfers, device latency is the replay alone (HIP events)."""
+    rng = np.random.default_rng(2024)
+    dim = queries.shape[1]
+    dq = native.DeviceBuffer(cap * dim * 4)
+    de = native.DeviceBuffer(cap * S * 4)
+    dL = native.DeviceBuffer(cap * 4)
+    dK = native.DeviceBuffer(cap * 4)
+    ids = native.DeviceBuffer(cap * kmax * 4)
+    dd = native.DeviceBuffer(cap * kmax * 4)
+    cnt = native.DeviceBuffer(cap * 4)
+    live = native.DeviceBuffer(4)
+    stream = native.Stream()
+    live.upload(np.array([cap], dtype=np.int32))
+    dL.upload(np.full(cap, lmax, dtype=np.int32))
+    dK.upload(np.full(cap, 10, dtype=np.int32))
+    dq.upload(np.ascontiguousarray(queries[:cap], dtype=np.float32))
+    de.upload(np.ascontiguousarray(ents[:cap], dtype=np.int32))
+    g = calc.search_graph_create(dq.ptr, cap, lmax, kmax, ids.ptr, dd.ptr, cnt.ptr, kmax,
+                                 stream.ptr, d_num_queries=live.ptr,
+                                 correction=1 if mode == "online" else 0,
+                                 refine_candidates=refine_of(mode, lmax),
+                                 expand_count=args.R, d_pool_sizes=dL.ptr, d_num_outs=dK.ptr,
+                                 d_entry_ids=de.ptr, entry_count=S, entry_stride=S)
+    e0, e1 = native.Event(), native.Event()
+    wall, dev, total = [], [], 0
+    try:
+        calc.search_graph_launch(g, stream.ptr)  # warm-up
+        stream.synchronize()
+        log("batching: graph captured (capacity %d, L <= %d, k <= %d); %d batches"
+            % (cap, lmax, kmax, args.batches))
+        for _ in range(args.batches):
+            n = int(rng.integers(1, cap + 1))
+            sel = rng.integers(0, len(queries), n)
+            kq = rng.choice(np.array([1, 10, 100], dtype=np.int32), n)
+            lq = (np.maximum(16, kq) + rng.integers(0, lmax + 1 - np.maximum(16, kq))
+                  ).astype(np.int32)
+            q = np.ascontiguousarray(queries[sel], dtype=np.float32)
+            r = np.ascontiguousarray(ents[sel], dtype=np.int32)
+            t0 = time.perf_counter()
+            dq.upload(q)
+            de.upload(r)
+            dL.upload(lq)
+            dK.upload(kq)
+            live.upload(np.array([n], dtype=np.int32))
+            e0.record(stream.ptr)
+            calc.search_graph_launch(g, stream.ptr)
+            e1.record(stream.ptr)
+            stream.synchronize()
+            c = cnt.download(np.int32, n)
+            ids.download(np.int32, n * kmax)
+            dd.download(np.float32, n * kmax)
+            wall.append((time.perf_counter() - t0) * 1e3)
+            dev.append(e0.elapsed_ms(e1))
+            total += n
+            assert np.array_equal(c, kq), "batch search failed: %s" % np.unique(c)
+    finally:
+        calc.search_graph_destroy(g)
+    wall, dev = np.array(wall), np.array(dev)
+    out = dict(batches=args.batches, queries=total,
+               batch_size="uniform 1..%d" % cap, L="uniform max(16,k)..%d" % lmax,
+               k="uniform {1,10,100}", capture="one HIP graph, live count in HBM",
+               wall_p50_ms=round(float(np.percentile(wall, 50)), 3),
+               wall_p99_ms=round(float(np.percentile(wall, 99)), 3),
+               device_p50_ms=round(float(np.percentile(dev, 50)), 3),
+               device_p99_ms=round(float(np.percentile(dev, 99)), 3),
+               wall_qps=round(float(total / (wall.sum() / 1e3)), 1),
+               device_qps=round(float(total / (dev.sum() / 1e3)), 1))
+    log("batching:", out)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +168,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="time the default launch configuration instead of the tuned one")
+    ap.add_argument("--batches", type=int, default=48,
+                    help="uneven batches replayed through one captured HIP graph "
+                         "(0: skip)")
     ap.add_argument("--train-queries", type=int, default=200_000)
     ap.add_argument("--train-L", type=int, default=512)
     ap.add_argument("--label-k", type=int, default=10)
@@ -183,7 +266,10 @@ def main():
     def refine_of(mode, L):
         return 0 if mode == "online" else min(args.refine, L)
 
+    tuned = {}  # (mode, L, learned) -> (hash_log2, probe_limit, grid_waves)
+
     def run(mode, L, stats=False, learned=False):
+        h2, pl, gw = tuned.get((mode, L, learned), (0, 0, 0))
         calc.search_batch_device(dq.ptr, nq, L, k, d_ids.ptr, d_d.ptr, d_cnt.ptr, k,
                                  correction=1 if mode == "online" else 0,
                                  refine_candidates=refine_of(mode, L),
@@ -191,7 +277,21 @@ def main():
                                  d_stats=d_st.ptr if stats else None,
                                  learned_stop=1 if learned else 0,
                                  stop_threshold=0.9, d_entry_ids=d_ent.ptr,
-                                 entry_count=S, entry_stride=S)
+                                 entry_count=S, entry_stride=S, hash_log2=h2,
+                                 probe_limit=pl, grid_waves=gw)
+
+    def autotune(mode, L, learned):
+        t = calc.autotune(dq.ptr, nq, L, k, d_ids.ptr, d_d.ptr, d_cnt.ptr, k, repeats=3,
+                          correction=1 if mode == "online" else 0,
+                          refine_candidates=refine_of(mode, L), expand_count=args.R,
+                          d_entry_ids=d_ent.ptr, entry_count=S, entry_stride=S,
+                          learned_stop=1 if learned else 0)
+        tuned[(mode, L, learned)] = (t.hash_log2, t.probe_limit, t.grid_waves)
+        info = dict(hash_log2=t.hash_log2, probe_limit=t.probe_limit,
+                    grid_waves=t.grid_waves, ms=round(t.ms, 4),
+                    default_ms=round(t.default_ms, 4), tried=t.tried, rejected=t.rejected)
+        log("autotuned %s%s L=%d: %s" % (mode, "+learned" if learned else "", L, info))
+        return info
 
     def sweep(mode, learned):
         rows, chosen = [], None
@@ -341,12 +441,14 @@ def main():
     # ---- timed region: K steps of the full batch, queries resident in HBM
     log("timing plain %s at L=%d: %d warmup + %d steps" % (mode, L_plain, args.warmup,
                                                             args.steps))
+    tuning = {"plain": autotune(mode, L_plain, False) if not args.no_autotune else None}
     ms_plain = timed(mode, L_plain, False)
     qps_plain = nq / (ms_plain / 1e3)
     log("timed plain %s: L=%d %.3f ms/step %.0f QPS" % (mode, L_plain, ms_plain, qps_plain))
     if learned:
         L_l = learned["chosen"]["L"]
         log("timing learned %s at L=%d" % (mode, L_l))
+        tuning["learned"] = autotune(mode, L_l, True) if not args.no_autotune else None
         ms_l = timed(mode, L_l, True)
         qps_l = nq / (ms_l / 1e3)
         log("timed learned %s: L=%d %.3f ms/step %.0f QPS (%.2fx plain)" %
@@ -367,6 +469,14 @@ def main():
                           entry_ids=ents)
     host_qps = nq * args.steps / (time.time() - t1)
     log("plain search from host buffers (PCIe included): %.0f QPS" % host_qps)
+
+    batching = None
+    if args.batches > 0:
+        batching = batching_leg(calc, queries, ents, S, mode, refine_of, args)
+
+    # restore the main batch's results for the checks below
+    run(mode, L_plain)
+    native.synchronize()
 
     code_bytes = index.code_bytes(args.m, 256)
     res_bytes = index.residual_code_bytes(args.dim, args.res_bits)
@@ -459,6 +569,8 @@ def main():
                 brief(learned), training=learned["training"],
                 other_slacks=learned["other_slacks"]),
             "host_buffers_qps": round(host_qps, 1),
+            "launch_tuning": tuning,
+            "uneven_batches": batching,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

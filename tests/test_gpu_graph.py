@@ -121,6 +121,75 @@ def test_graph_replays_match_eager(setup, correction, use_rows):
         calc.search_graph_destroy(g)
 
 
+def test_graph_uneven_batch_matches_eager(setup):
+    """Per-query beam width and k from device arrays, captured once and
+    replayed with a new batch (sizes, L, k, queries) in the same buffers."""
+    calc, ix, queries, rows = setup
+    say("case graph uneven")
+    rng = np.random.default_rng(5)
+    KMAX, LMAX = 100, 256
+    stream = native.Stream()
+
+    def draw(n):
+        k = rng.choice(np.array([1, 10, 100], dtype=np.int32), n)
+        lo = np.maximum(16, k)
+        L_ = (lo + rng.integers(0, LMAX + 1 - lo)).astype(np.int32)
+        sel = rng.integers(0, len(queries), n)
+        return (np.ascontiguousarray(queries[sel], dtype=np.float32),
+                np.ascontiguousarray(rows[sel], dtype=np.int32), L_, k.astype(np.int32))
+
+    dq = native.DeviceBuffer(NQ * 128 * 4)
+    de = native.DeviceBuffer(NQ * 4 * 4)
+    dL = native.DeviceBuffer(NQ * 4)
+    dK = native.DeviceBuffer(NQ * 4)
+    ids = native.DeviceBuffer(NQ * KMAX * 4)
+    dd = native.DeviceBuffer(NQ * KMAX * 4)
+    cnt = native.DeviceBuffer(NQ * 4)
+    live = native.DeviceBuffer(4)
+
+    def load(q, r, L_, k):
+        n = len(q)
+        dq.upload(q)
+        de.upload(r)
+        dL.upload(L_)
+        dK.upload(k)
+        live.upload(np.array([n], dtype=np.int32))
+        ids.upload(np.full(NQ * KMAX, -1, dtype=np.int32))
+        cnt.upload(np.full(NQ, -9, dtype=np.int32))
+
+    def result(n):
+        return (ids.download(np.int32, NQ * KMAX).reshape(NQ, KMAX)[:n],
+                dd.download(np.float32, NQ * KMAX).reshape(NQ, KMAX)[:n],
+                cnt.download(np.int32, NQ)[:n])
+
+    g = calc.search_graph_create(dq.ptr, NQ, LMAX, KMAX, ids.ptr, dd.ptr, cnt.ptr, KMAX,
+                                 stream.ptr, d_num_queries=live.ptr, correction=1,
+                                 d_pool_sizes=dL.ptr, d_num_outs=dK.ptr, d_entry_ids=de.ptr,
+                                 entry_count=4, entry_stride=4)
+    try:
+        for n in (NQ, 1, 173):
+            batch = draw(n)
+            load(*batch)
+            calc.search_graph_launch(g, stream.ptr)
+            stream.synchronize()
+            got = result(n)
+            load(*batch)
+            calc.search_batch_device(dq.ptr, n, LMAX, KMAX, ids.ptr, dd.ptr, cnt.ptr, KMAX,
+                                     correction=1, d_pool_sizes=dL.ptr, d_num_outs=dK.ptr,
+                                     d_entry_ids=de.ptr, entry_count=4, entry_stride=4)
+            native.synchronize()
+            ref = result(n)
+            assert np.array_equal(ref[2], batch[3])  # every query returns its k
+            assert np.array_equal(got[2], ref[2])
+            for i in range(n):
+                kq = int(ref[2][i])
+                assert np.array_equal(got[0][i, :kq], ref[0][i, :kq])
+                assert np.array_equal(got[1][i, :kq].view(np.uint32),
+                                      ref[1][i, :kq].view(np.uint32))
+    finally:
+        calc.search_graph_destroy(g)
+
+
 def test_graph_rejects_default_stream(setup):
     calc, ix, queries, rows = setup
     b = Bufs(queries[:NQ], rows[:NQ])
