@@ -492,6 +492,16 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                 "traffic": None, "kernel": kernel,
                 "algorithmic_bytes_per_launch": int(algo)}
+    # HBM bytes per launch come from the profiler's FETCH_SIZE / WRITE_SIZE
+    # passes over this same workload (calibrated against a copy kernel),
+    # recorded next to the profile summary; used when kernel and L match
+    rec_path = os.path.join(ROOT, "profiles", "rv_search_kernel_online_1M_L32.json")
+    if os.path.exists(rec_path) and args.n == 1_000_000 and args.nq == 10_000:
+        rec = json.load(open(rec_path))
+        per = rec.get("traffic_bytes_per_launch", {}).get(kernel)
+        if per and rec.get("L") == L:
+            roofline["traffic"] = int(per)
+            roofline["traffic_source"] = os.path.relpath(rec_path, ROOT)
 
     cpu = None
     if not args.no_cpu:
