@@ -15,6 +15,22 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
 
 
+def pytest_collection_modifyitems(session, config, items):
+    """torch ships its own HIP runtime next to the system one that
+    librover.so links; in one process torch's must initialize first (some
+    GPU tests build graphs with torch after the library has run).  Touch
+    torch's device once before any GPU test when GPU tests are selected."""
+    if "not gpu" in (config.getoption("markexpr") or ""):
+        return
+    if any(item.get_closest_marker("gpu") for item in items):
+        try:
+            import torch
+            if torch.cuda.is_available():
+                torch.zeros(1, device="cuda:0")
+        except Exception:
+            pass
+
+
 def load_golden(name):
     z = np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)
     out = {k: z[k] for k in z.files}
