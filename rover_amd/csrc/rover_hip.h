@@ -57,9 +57,10 @@ typedef struct RvIndex {
 #define RV_TRACE_META (2 + RV_TRACE_TOPK)
 #define RV_MLP_HIDDEN 64
 typedef struct RvMlp {
-    const uint16_t *w1;    /* [16][64]  bf16, input-major (w1t[i][j]) */
+    const uint16_t *w1;    /* bf16, grouped [4][64][4]: (input i, unit j)
+                              at ((i / 4) * 64 + j) * 4 + i % 4          */
     const float *b1;       /* [64]           */
-    const uint16_t *w2;    /* [64][64]  bf16, input-major (w2t[i][j]) */
+    const uint16_t *w2;    /* bf16, grouped [16][64][4], same rule       */
     const float *b2;       /* [64]           */
     const uint16_t *w3;    /* [64]      bf16 */
     const float *b3;       /* [1]            */

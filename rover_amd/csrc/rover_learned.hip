@@ -221,19 +221,21 @@ __global__ void rv_mlp_adam_kernel(float *p, const float *g, float *m1, float *m
 }
 
 __global__ void rv_mlp_bf16_kernel(const float *p, uint16_t *w) {
-    // Inference copy as bf16, input-major so that the lanes of a wavefront
-    // (one hidden unit each) read one contiguous row per input:
-    // w1t[16][64] | w2t[64][64] | w3[64].
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    // Inference copy as bf16, grouped so that hidden unit j (one lane) reads
+    // four consecutive inputs' weights with one 8-byte load:
+    // w1g[4][64][4] | w2g[16][64][4] | w3[64], element (input i, unit j) of
+    // w1/w2 at ((i / 4) * 64 + j) * 4 + i % 4.
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int n1 = MLP_H * MLP_IN, n2 = MLP_H * MLP_H, n3 = MLP_H;
-    if (i < n1) {
-        const int in = i / MLP_H, j = i % MLP_H;
-        w[i] = rv_f32_to_bf16(p[OFF_W1 + j * MLP_IN + in]);
-    } else if (i < n1 + n2) {
-        const int in = (i - n1) / MLP_H, j = (i - n1) % MLP_H;
-        w[i] = rv_f32_to_bf16(p[OFF_W2 + j * MLP_H + in]);
-    } else if (i < n1 + n2 + n3) {
-        w[i] = rv_f32_to_bf16(p[OFF_W3 + (i - n1 - n2)]);
+    if (t < n1 + n2) {
+        const bool first = t < n1;
+        const int e = first ? t : t - n1;
+        const int g = e / (MLP_H * 4), r = e % (MLP_H * 4);
+        const int j = r / 4, in = g * 4 + r % 4;
+        w[t] = first ? rv_f32_to_bf16(p[OFF_W1 + j * MLP_IN + in])
+                     : rv_f32_to_bf16(p[OFF_W2 + j * MLP_H + in]);
+    } else if (t < n1 + n2 + n3) {
+        w[t] = rv_f32_to_bf16(p[OFF_W3 + (t - n1 - n2)]);
     }
 }
 

@@ -456,9 +456,13 @@ __device__ __forceinline__ void rv_features(const RvStopState &st,
 }
 
 // 16 -> 64 -> 64 -> 1 MLP, lane j owns hidden unit j.  FP32 accumulation
-// over BF16 weights stored input-major (one contiguous 128-byte row per
-// input); the first hidden layer is broadcast through hbuf (64 floats of
-// LDS, 16-byte aligned).  Returns the logit (wave-uniform).
+// over BF16 weights (inputs in ascending order); the weights of four
+// consecutive inputs for unit j come in one 8-byte load (grouped layout,
+// see rv_mlp_bf16_kernel); the first hidden layer is broadcast through hbuf
+// (64 floats of LDS, 16-byte aligned).  Returns the logit (wave-uniform).
+__device__ __forceinline__ float rv_bf16_lo(uint32_t v) { return __uint_as_float(v << 16); }
+__device__ __forceinline__ float rv_bf16_hi(uint32_t v) { return __uint_as_float(v & 0xffff0000u); }
+
 __device__ float rv_mlp_logit(const RvMlp &mlp_in, const float *f_raw, float *hbuf) {
     const int j = rv_lane();
     // The weights are re-read at every evaluation: the empty asm hides the
@@ -471,22 +475,30 @@ __device__ float rv_mlp_logit(const RvMlp &mlp_in, const float *f_raw, float *hb
 #pragma unroll
     for (int i = 0; i < RV_MLP_IN; i++)
         x[i] = (f_raw[i] - mlp.feat_mean[i]) * mlp.feat_inv_std[i];
+    const uint2 *w1 = reinterpret_cast<const uint2 *>(mlp.w1);
     float h1 = mlp.b1[j];
 #pragma unroll
-    for (int i = 0; i < RV_MLP_IN; i++) h1 += rv_bf16(mlp.w1[i * RV_MLP_HIDDEN + j]) * x[i];
+    for (int g = 0; g < RV_MLP_IN / 4; g++) {
+        const uint2 v = w1[g * RV_MLP_HIDDEN + j];
+        h1 += rv_bf16_lo(v.x) * x[4 * g + 0];
+        h1 += rv_bf16_hi(v.x) * x[4 * g + 1];
+        h1 += rv_bf16_lo(v.y) * x[4 * g + 2];
+        h1 += rv_bf16_hi(v.y) * x[4 * g + 3];
+    }
     h1 = h1 > 0.0f ? h1 : 0.0f;
     hbuf[j] = h1;
     __syncthreads();
     float h2 = mlp.b2[j];
     const float4 *h4 = reinterpret_cast<const float4 *>(hbuf);
-#pragma unroll 4
-    for (int i4 = 0; i4 < RV_MLP_HIDDEN / 4; i4++) {
-        const float4 hv = h4[i4];
-        const uint16_t *w = mlp.w2 + 4 * i4 * RV_MLP_HIDDEN + j;
-        h2 += rv_bf16(w[0]) * hv.x;
-        h2 += rv_bf16(w[RV_MLP_HIDDEN]) * hv.y;
-        h2 += rv_bf16(w[2 * RV_MLP_HIDDEN]) * hv.z;
-        h2 += rv_bf16(w[3 * RV_MLP_HIDDEN]) * hv.w;
+    const uint2 *w2 = reinterpret_cast<const uint2 *>(mlp.w2);
+#pragma unroll 2
+    for (int g = 0; g < RV_MLP_HIDDEN / 4; g++) {
+        const float4 hv = h4[g];
+        const uint2 v = w2[g * RV_MLP_HIDDEN + j];
+        h2 += rv_bf16_lo(v.x) * hv.x;
+        h2 += rv_bf16_hi(v.x) * hv.y;
+        h2 += rv_bf16_lo(v.y) * hv.z;
+        h2 += rv_bf16_hi(v.y) * hv.w;
     }
     __syncthreads();  // hbuf is reused by the next merge
     h2 = h2 > 0.0f ? h2 : 0.0f;
