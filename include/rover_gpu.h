@@ -126,6 +126,29 @@ AEEResult rover_route_entries(remote_handle64 h, const float *d_queries,
 AEEResult rover_pure_pq_original_ids(remote_handle64 h, const int32_t *map,
                                      int mapLen);
 
+/* Third-level refine artifact (ERROR4 mode of retrieval_final_refine_open_pq,
+ * retrieval_final_refine.c:98-190; written by retrieval_build_error4,
+ * retrieval_build_artifacts.c:25-66, and pq_refine_artifacts.py:47-65),
+ * given as the whole PQERR4V1 file image: 256-byte header, (scale, zero)
+ * [N][2] at 256, then 4096-byte blocks of 4096 / ((dim+1)/2) rows of
+ * (dim+1)/2 bytes of 4-bit codes (low nibble = even dim).  Checked as the
+ * reference's open checks it: magic, version, N and dim against the loaded
+ * PQ index, bit widths, block size, section sizes, the payload digest at
+ * +56, every (scale, zero) finite with scale >= 0, and, when expected_sha256
+ * is not NULL, the whole-file SHA-256 (RetrievalFinalRefineConfig.
+ * artifact_sha256).  The digests of the caller's base files (+88) and
+ * codebook (+184) bind the artifact to files and are the caller's to check;
+ * the id-map digest (+216) is checked against rover_pure_pq_original_ids at
+ * each ERROR4 launch.  Returns ROVER_EBADPARM for an artifact the reference
+ * would reject. */
+AEEResult rover_pure_pq_error4_load(remote_handle64 h, const uint8_t *artifact,
+                                    int64_t artifactLen,
+                                    const uint8_t *expected_sha256);
+AEEResult rover_pure_pq_error4_unload(remote_handle64 h);
+
+/* SHA-256 (FIPS 180-4) of a buffer, as the artifact digests use. */
+void rover_sha256(const void *data, size_t bytes, uint8_t out[32]);
+
 /* ---------------------------------------------------- 2. library search */
 
 typedef struct PurePqSearchIndexView {
@@ -248,7 +271,14 @@ typedef struct RoverSearchBatch {
     /* learned early termination (needs rover_learned_stop_set) */
     int learned_stop;
     float stop_threshold;        /* 0 -> 0.9 */
+    /* refine level (RETRIEVAL_REFINE_CURRENT / _ERROR4,
+       retrieval_final_refine.h:7-8): 0 or ROVER_REFINE_CURRENT decodes PQ +
+       residual; ROVER_REFINE_ERROR4 adds the error4 level
+       (needs rover_pure_pq_error4_load)                                    */
+    int refine_mode;
 } RoverSearchBatch;
+
+enum { ROVER_REFINE_CURRENT = 1, ROVER_REFINE_ERROR4 = 2 };
 
 /* Host buffers in and out; returns when results are in place. */
 AEEResult rover_search_batch(remote_handle64 h, const RoverSearchBatch *b);
@@ -369,6 +399,20 @@ AEEResult rover_build_residual_encode(const float *d_x, const uint8_t *d_codes,
                                       int n, int dim, int m, int ks,
                                       const float *d_codebook, int bits,
                                       uint8_t *d_res, float *d_qparams);
+
+/* error4 codes on the device, bit-identical to retrieval_build_error4
+ * (retrieval_build_artifacts.c:25-66): the reconstruction is the residual
+ * decode (retrieval_pq_residual.c:954-981), e = x - recon,
+ * scale = max(max e - min e, 1e-12) / 15, zero = min e,
+ * code = clamp(floor((e - zero) * (1 / scale) + 0.5), 0, 15).  Flat outputs
+ * [n][(dim+1)/2] and [n][2]; a row whose error is not finite gets a NaN
+ * scale (the loader rejects it).  Device pointers; synchronous. */
+AEEResult rover_build_error4_encode(const float *d_x, const uint8_t *d_codes,
+                                    const uint8_t *d_res,
+                                    const float *d_res_qparams, int n, int dim,
+                                    int m, int ks, const float *d_codebook,
+                                    int res_bits, uint8_t *d_e4_codes,
+                                    float *d_e4_qparams);
 
 /* Device memory helpers, so callers need no HIP headers. */
 AEEResult rover_device_alloc(void **d_ptr, size_t bytes);

@@ -25,7 +25,8 @@ class OrcIndex(C.Structure):
                 ("res_codes", C.c_void_p), ("res_qparams", C.c_void_p),
                 ("res_bits", C.c_int), ("sq_codes", C.c_void_p),
                 ("sq_bits", C.c_int), ("sq_scale", C.c_float),
-                ("sq_zero", C.c_float)]
+                ("sq_zero", C.c_float), ("e4_codes", C.c_void_p),
+                ("e4_qparams", C.c_void_p)]
 
 
 class OrcStats(C.Structure):
@@ -56,6 +57,7 @@ def oracle():
         L.orc_residual_encode.argtypes = [P, P, P, I, I, I, I, P, P]
         L.orc_residual_evaluate.argtypes = [P, P, I, P]
         L.orc_residual_decode.argtypes = [P, I, P]
+        L.orc_error4_encode.argtypes = [P, P, I, P, P]
         L.orc_sq_range_params.argtypes = [P, C.c_long, I, P, P]
         L.orc_sq_encode.argtypes = [P, I, I, F, F, P]
         L.orc_sq_quantize_query.argtypes = [P, I, I, F, F, P]
@@ -99,6 +101,10 @@ def reference():
         L.ref_search.argtypes = [P, I, I, P, I, P, I, I, I, P, P, I, I, I, I, I,
                                  P, P, I, F, F, P, P, I, P]
         L.ref_refine.argtypes = [P, I, I, P, P, P, I, I, P]
+        L.ref_error4_build.argtypes = [P, P, P, I, P, I, I, I, P, P, P]
+        L.ref_error4_file.argtypes = [P, P, C.c_long]
+        L.ref_error4_file.restype = C.c_long
+        L.ref_refine_mode.argtypes = [P, I, I, I, P, P, P, I, I, P]
         _ref = L
     return _ref
 
@@ -108,13 +114,16 @@ class Index:
 
     def __init__(self, graph, codes=None, codebook=None, res_codes=None,
                  res_qparams=None, res_bits=4, sq_codes=None, sq_bits=8,
-                 sq_scale=1.0, sq_zero=0.0, dim=None):
+                 sq_scale=1.0, sq_zero=0.0, dim=None, e4_codes=None,
+                 e4_qparams=None):
         self.graph = np.ascontiguousarray(graph, dtype=np.int32)
         self.codes = None if codes is None else np.ascontiguousarray(codes, dtype=np.uint8)
         self.codebook = None if codebook is None else np.ascontiguousarray(codebook, dtype=np.float32)
         self.res_codes = None if res_codes is None else np.ascontiguousarray(res_codes, dtype=np.uint8)
         self.res_qparams = None if res_qparams is None else np.ascontiguousarray(res_qparams, dtype=np.float32)
         self.sq_codes = None if sq_codes is None else np.ascontiguousarray(sq_codes, dtype=np.uint8)
+        self.e4_codes = None if e4_codes is None else np.ascontiguousarray(e4_codes, dtype=np.uint8)
+        self.e4_qparams = None if e4_qparams is None else np.ascontiguousarray(e4_qparams, dtype=np.float32)
         s = OrcIndex()
         s.graph = _p(self.graph)
         s.degree = self.graph.shape[1]
@@ -133,6 +142,8 @@ class Index:
         s.sq_bits = sq_bits
         s.sq_scale = sq_scale
         s.sq_zero = sq_zero
+        s.e4_codes = _p(self.e4_codes)
+        s.e4_qparams = _p(self.e4_qparams)
         self.struct = s
         self.dim = s.dim
 
@@ -239,6 +250,20 @@ def residual_encode(vec, pq_code, codebook, bits):
                                       _p(out), _p(qp))
     assert rc == 0
     return out, qp
+
+
+def error4_encode(x, recon):
+    """Per-row error4 codes [n][(dim+1)/2] and (scale, zero) [n][2] of the
+    raw vectors x against their reconstructions recon."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    r = np.ascontiguousarray(recon, dtype=np.float32)
+    n, dim = x.shape
+    codes = np.zeros((n, (dim + 1) // 2), dtype=np.uint8)
+    qp = np.zeros((n, 2), dtype=np.float32)
+    for i in range(n):
+        rc = oracle().orc_error4_encode(_p(x[i]), _p(r[i]), dim, _p(codes[i]), _p(qp[i]))
+        assert rc == 0
+    return codes, qp
 
 
 def sq_quantize_query(query, bits, scale, zero):

@@ -18,7 +18,10 @@
 #include <string.h>
 #include <unistd.h>
 
+#include "dsp_demo/retrieval_build_artifacts.h"
 #include "dsp_demo/retrieval_final_refine.h"
+#include "dsp_demo/retrieval_pq_artifact.h"
+#include "dsp_demo/retrieval_raw_reader.h"
 #include "dsp_demo/retrieval_l1_int8_core.h"
 #include "dsp_demo/retrieval_pq_core.h"
 #include "dsp_demo/retrieval_pq_residual.h"
@@ -93,6 +96,11 @@ typedef struct RefResidual {
     RetrievalPqResidualQuery *query;
     const uint8_t *chunks[1];
     int counts[1];
+    /* error4 artifact and the files it is bound to (ref_error4_build) */
+    int res_bits;
+    int e4_ready;
+    char e4_path[300], e4_raw[300], e4_codes[300], e4_book[300];
+    char e4_base[3][300];
 } RefResidual;
 
 /* Writes PQRES01 artifacts for (x, codes, codebook) with the reference
@@ -110,6 +118,7 @@ void *ref_residual_open(const float *x, const uint8_t *codes,
     char path[300];
     FILE *f;
     if (!r) return NULL;
+    r->res_bits = bits;
     snprintf(r->dir, sizeof(r->dir), "/tmp/rover_ref_XXXXXX");
     if (!mkdtemp(r->dir)) goto fail;
     snprintf(r->prefix, sizeof(r->prefix), "%s/index", r->dir);
@@ -171,6 +180,12 @@ void ref_residual_close(void *handle) {
     RefResidual *r = handle;
     char path[300];
     if (!r) return;
+    if (r->e4_ready) {
+        unlink(r->e4_path);
+        unlink(r->e4_raw);
+        unlink(r->e4_codes);
+        unlink(r->e4_book);
+    }
     retrieval_pq_residual_query_destroy(r->query);
     retrieval_pq_residual_index_close(r->index);
     snprintf(path, sizeof(path), "%s.pqres4", r->prefix); unlink(path);
@@ -317,6 +332,130 @@ int ref_refine(void *residual, int n, int dim, const float *query,
     c.provider = residual;
     c.graph_to_original = graph_to_original;
     if (retrieval_final_refine_open_pq(&c, "unused", &ctx)) return -100;
+    rc = retrieval_final_refine(ctx, query, ids, dists, count, output_k, NULL);
+    retrieval_final_refine_close(ctx);
+    return rc;
+}
+
+/* ---- third-level refine (error4), written and read by the reference ---- */
+
+static int ref_write_file(const char *path, const void *a, size_t an,
+                          const void *b, size_t bn) {
+    FILE *f = fopen(path, "wb");
+    int ok;
+    if (!f) return -1;
+    ok = fwrite(a, 1, an, f) == an && (!bn || fwrite(b, 1, bn, f) == bn);
+    return fclose(f) == 0 && ok ? 0 : -1;
+}
+
+/* Writes the base vectors as a raw fvecs file, the PQ codes and codebook as
+ * files (only hashed), runs retrieval_build_error4 with the residual decoder
+ * as the reconstruction, and copies the artifact's header [256], qparams
+ * [n][2] and codes (flattened out of the 4096-byte blocks, [n][(dim+1)/2])
+ * out.  The artifact stays on disk for ref_refine_mode(..., ERROR4). */
+int ref_error4_build(void *handle, const float *x, const uint8_t *codes,
+                     int code_bytes, const float *codebook, int book_floats,
+                     int n, int dim, uint8_t *out_header, float *out_qparams,
+                     uint8_t *out_codes) {
+    RefResidual *r = handle;
+    RetrievalRawReader *raw = NULL;
+    const char *paths[3];
+    int cb = (dim + 1) / 2, vpb = 4096 / cb, rc;
+    FILE *f, *raw_f;
+    if (!r || r->e4_ready) return -1;
+    snprintf(r->e4_raw, sizeof(r->e4_raw), "%s/base.fvecs", r->dir);
+    snprintf(r->e4_codes, sizeof(r->e4_codes), "%s/index.pqcodes", r->dir);
+    snprintf(r->e4_book, sizeof(r->e4_book), "%s/index.pqbook", r->dir);
+    snprintf(r->e4_path, sizeof(r->e4_path), "%s/error4.refine", r->dir);
+    snprintf(r->e4_base[0], sizeof(r->e4_base[0]), "%s", r->e4_codes);
+    snprintf(r->e4_base[1], sizeof(r->e4_base[1]), "%s.pqres%d", r->prefix,
+             r->res_bits);
+    snprintf(r->e4_base[2], sizeof(r->e4_base[2]), "%s.pqres%d.qparams",
+             r->prefix, r->res_bits);
+    raw_f = fopen(r->e4_raw, "wb");
+    if (!raw_f) return -2;
+    for (int i = 0; i < n; i++) {
+        int32_t d32 = dim;
+        if (fwrite(&d32, 4, 1, raw_f) != 1 ||
+            fwrite(x + (size_t)i * dim, 4, (size_t)dim, raw_f) != (size_t)dim) {
+            fclose(raw_f);
+            return -2;
+        }
+    }
+    if (fclose(raw_f) ||
+        ref_write_file(r->e4_codes, codes, (size_t)n * code_bytes, NULL, 0) ||
+        ref_write_file(r->e4_book, codebook, (size_t)book_floats * 4, NULL, 0))
+        return -2;
+    if (retrieval_raw_open(r->e4_raw, n, dim, (size_t)1 << 20, &raw)) return -3;
+    for (int i = 0; i < 3; i++) paths[i] = r->e4_base[i];
+    rc = retrieval_build_error4(r->e4_path, raw, n, dim, ref_refine_decode, r,
+                                paths, r->e4_book, 1);
+    retrieval_raw_close(raw);
+    if (rc) return -4;
+    r->e4_ready = 1;
+    f = fopen(r->e4_path, "rb");
+    if (!f) return -5;
+    rc = fread(out_header, 1, 256, f) == 256 &&
+         fread(out_qparams, 8, (size_t)n, f) == (size_t)n ? 0 : -5;
+    for (int i = 0; !rc && i < n; i++) {
+        long off = 256L + 8L * n + (long)(i / vpb) * 4096L + (long)(i % vpb) * cb;
+        if (fseek(f, off, SEEK_SET) ||
+            fread(out_codes + (size_t)i * cb, 1, cb, f) != (size_t)cb)
+            rc = -5;
+    }
+    fclose(f);
+    return rc;
+}
+
+/* Bytes of the artifact file ref_error4_build wrote (to copy it whole). */
+long ref_error4_file(void *handle, uint8_t *out, long cap) {
+    RefResidual *r = handle;
+    FILE *f;
+    long size;
+    if (!r || !r->e4_ready || !(f = fopen(r->e4_path, "rb"))) return -1;
+    fseek(f, 0, SEEK_END);
+    size = ftell(f);
+    if (out && size <= cap) {
+        fseek(f, 0, SEEK_SET);
+        if (fread(out, 1, (size_t)size, f) != (size_t)size) size = -1;
+    }
+    fclose(f);
+    return size;
+}
+
+/* retrieval_final_refine with the PQ residual decoder in `mode`
+ * (RETRIEVAL_REFINE_CURRENT or RETRIEVAL_REFINE_ERROR4).  ERROR4 opens the
+ * artifact written by ref_error4_build through retrieval_final_refine_open_pq,
+ * the path rpc_demo_pure_pq.c:518-533 takes: whole-file digest, payload
+ * digest, the three base files and the codebook are all checked.  Returns the
+ * output count, or the open error - 100. */
+int ref_refine_mode(void *residual, int mode, int n, int dim,
+                    const float *query, int32_t *ids, float *dists, int count,
+                    int output_k, const int32_t *graph_to_original) {
+    RefResidual *r = residual;
+    RetrievalFinalRefineConfig c;
+    RetrievalFinalRefineContext *ctx = NULL;
+    unsigned char digest[32];
+    const char *book = "unused";
+    int rc;
+    memset(&c, 0, sizeof(c));
+    c.mode = mode;
+    c.num_vectors = n;
+    c.dim = dim;
+    c.capacity = count > 0 ? count : 1;
+    c.cache_slots = 4;
+    c.decode = ref_refine_decode;
+    c.provider = residual;
+    c.graph_to_original = graph_to_original;
+    if (mode == RETRIEVAL_REFINE_ERROR4) {
+        if (!r->e4_ready || retrieval_build_hash(r->e4_path, digest)) return -200;
+        c.artifact_path = r->e4_path;
+        c.artifact_sha256 = digest;
+        for (int i = 0; i < 3; i++) c.base_paths[i] = r->e4_base[i];
+        book = r->e4_book;
+    }
+    rc = retrieval_final_refine_open_pq(&c, book, &ctx);
+    if (rc) return rc - 100;
     rc = retrieval_final_refine(ctx, query, ids, dists, count, output_k, NULL);
     retrieval_final_refine_close(ctx);
     return rc;

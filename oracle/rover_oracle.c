@@ -203,6 +203,49 @@ int orc_residual_decode(const OrcIndex *ix, int vector_id, float *out) {
     return 0;
 }
 
+/* retrieval_build_artifacts.c:25-66 (retrieval_build_error4), one row; the
+ * published generator pq_refine_artifacts.py:47-65 with
+ * generate_final_refine.py:36-44 (quantize_error) performs the same float32
+ * steps: e = x - recon; lo = min e, hi = max e (first strict extreme wins);
+ * scale = max(hi - lo, 1e-12) / 15; inv = 1 / scale;
+ * code = clamp(floor((e - lo) * inv + 0.5), 0, 15), low nibble = even d. */
+int orc_error4_encode(const float *x, const float *recon, int dim,
+                      uint8_t *out_code, float *out_qparams) {
+    float *e = malloc((size_t)dim * sizeof(float));
+    float lo = INFINITY, hi = -INFINITY, scale, inv, span;
+    if (!e) return -1;
+    for (int d = 0; d < dim; d++) {
+        volatile float diff = x[d] - recon[d];
+        e[d] = diff;
+        if (!isfinite(e[d])) {
+            free(e);
+            return -1;
+        }
+        if (e[d] < lo) lo = e[d];
+        if (e[d] > hi) hi = e[d];
+    }
+    {
+        volatile float sp = hi - lo;
+        span = sp;
+    }
+    scale = fmaxf(span, 1e-12f) / 15.0f;
+    inv = 1.0f / scale;
+    memset(out_code, 0, (size_t)(dim + 1) / 2);
+    for (int d = 0; d < dim; d++) {
+        volatile float shifted = e[d] - lo;
+        volatile float scaled = shifted * inv;
+        volatile float v = scaled + 0.5f;
+        int code = (int)floorf(v);
+        if (code < 0) code = 0;
+        if (code > 15) code = 15;
+        out_code[d / 2] |= (uint8_t)(code << ((d % 2) * 4));
+    }
+    out_qparams[0] = scale;
+    out_qparams[1] = lo;
+    free(e);
+    return 0;
+}
+
 /* retrieval_rpc_demo_support.c:573-627 (demo_set_qparams_from_range and
  * demo_compute_range): one global scale/zero over all values. */
 int orc_sq_range_params(const float *data, long count, int bits,
@@ -587,7 +630,8 @@ static int orc_refine_cmp(const void *a, const void *b) {
 }
 
 /* retrieval_final_refine.c:256-292 (refine_impl) with the PQ residual
- * decoder (retrieval_pq_residual.c:954-981) as the vector source:
+ * decoder (retrieval_pq_residual.c:954-981) as the vector source, plus the
+ * error4 level when the index carries one (ERROR4 mode):
  * dist = sum_d ((double)q - (double)v)^2 sequentially in f64, sorted by
  * (dist, original id), top output_k returned as (float)dist. */
 int orc_refine(const OrcIndex *ix, const float *query, const int32_t *ids,
@@ -622,6 +666,24 @@ int orc_refine(const OrcIndex *ix, const float *query, const int32_t *ids,
         if (orc_residual_decode(ix, id, vec) != 0) {
             rc = -1;
             goto out;
+        }
+        if (ix->e4_codes) {
+            /* retrieval_final_refine.c:221-246 (decode_vector, ERROR4):
+             * v = v + ((code * scale) + zero), three rounded f32 steps; the
+             * row is the graph id (PQ has no storage map) */
+            const uint8_t *c4 = ix->e4_codes + (size_t)id * ((ix->dim + 1) / 2);
+            const float s4 = ix->e4_qparams[2 * (size_t)id];
+            const float z4 = ix->e4_qparams[2 * (size_t)id + 1];
+            for (int d = 0; d < ix->dim; d++) {
+                volatile float product = (float)((c4[d >> 1] >> (4 * (d & 1))) & 15) * s4;
+                volatile float level = product + z4;
+                volatile float value = vec[d] + level;
+                vec[d] = value;
+                if (!isfinite(vec[d])) {
+                    rc = -4;
+                    goto out;
+                }
+            }
         }
         double acc = 0.0;
         for (int d = 0; d < ix->dim; d++) {

@@ -42,6 +42,11 @@ typedef struct OrcIndex {
     const uint8_t *sq_codes;   /* [num_vectors][sq_code_bytes] */
     int sq_bits;               /* 4 or 8 */
     float sq_scale, sq_zero;
+    /* third-level refine level (error4): 4-bit codes
+     * [num_vectors][(dim+1)/2], low nibble = even d, and (scale, zero)
+     * [num_vectors][2]; NULL refines without it */
+    const uint8_t *e4_codes;
+    const float *e4_qparams;
 } OrcIndex;
 
 typedef struct OrcStats {
@@ -65,6 +70,12 @@ int orc_residual_evaluate(const OrcIndex *index, const float *query,
                           int vector_id, float *out_distance);
 int orc_residual_decode(const OrcIndex *index, int vector_id,
                         float *out_vector);
+
+/* One vector's error4 code from the raw vector x and its reconstruction
+ * recon (the residual decode): out_code [(dim+1)/2], out_qparams (scale,
+ * zero).  Returns 0, or -1 on a non-finite error. */
+int orc_error4_encode(const float *x, const float *recon, int dim,
+                      uint8_t *out_code, float *out_qparams);
 
 int orc_sq_range_params(const float *data, long count, int bits,
                         float *out_scale, float *out_zero);

@@ -28,7 +28,7 @@ C_FLAGS = ["-std=gnu11", "-O2", "-fPIC", "-ffp-contract=off", "-Wall",
            "-Wno-unused-function"]
 
 HIP_SOURCES = ["rover_search.hip", "rover_learned.hip"]
-C_SOURCES = ["rover_host.c", "rover_learned_host.c"]
+C_SOURCES = ["rover_host.c", "rover_learned_host.c", "rover_sha256.c"]
 
 
 def _run(cmd, verbose):

@@ -37,6 +37,13 @@ typedef struct RoverCtx {
     float *d_qparams;
     int res_bits;
     int32_t *d_orig;
+    uint8_t orig_maps_sha[32]; /* digest of the id maps an error4 artifact
+                                  must be bound to (empty when no map)    */
+    /* third-level refine artifact (error4): code blocks and qparams */
+    uint8_t *d_e4_blocks;
+    float *d_e4_qparams;
+    int e4_loaded, e4_vpb, e4_code_bytes;
+    uint8_t e4_maps_sha[32];   /* the artifact's header digest at +216   */
     /* SQ index (retrieval_load) */
     int sq_loaded;
     int32_t *d_sq_graph;
@@ -64,6 +71,7 @@ typedef struct RoverCtx {
 } RoverCtx;
 
 RoverCtx *rover_ctx_of(remote_handle64 h);
+void rover_sha256(const void *data, size_t bytes, uint8_t out[32]);
 void rover_set_err(const char *what);
 void rover_set_errf(const char *msg);
 void rover_dfree(void *p);

@@ -31,6 +31,9 @@ enum {
     RV_QUERY_BAD_PARAM = -5    /* L/k outside the launch's limits           */
 };
 
+/* error4 artifact code blocks are this many bytes */
+#define RV_E4_BLOCK 4096
+
 /* Device-resident index.  All arrays are row-major and dense. */
 typedef struct RvIndex {
     const int32_t *graph;       /* [n][degree], -1 suffix padding         */
@@ -46,6 +49,12 @@ typedef struct RvIndex {
     int sq_bits, sq_code_bytes;
     float sq_scale, sq_zero;
     const int32_t *graph_to_original; /* optional permutation for refine  */
+    /* third-level refine level (error4): the artifact's code blocks as
+       stored (4096-byte blocks of e4_vpb rows of e4_code_bytes 4-bit codes,
+       low nibble = even dim) and (scale, zero) per row                    */
+    const uint8_t *e4_blocks;
+    const float *e4_qparams;
+    int e4_vpb, e4_code_bytes;
 } RvIndex;
 
 /* Learned early-termination predictor: 16 -> 64 -> 64 -> 1 MLP,
@@ -82,6 +91,7 @@ typedef struct RvBatch {
     int expand_count;
     int scorer;                  /* RV_SCORE_*                              */
     int refine_candidates;       /* 0: off; F: refine the top F, output k   */
+    int refine_error4;           /* refine adds the error4 level            */
     int hash_log2;               /* visited hash has 1<<hash_log2 slots     */
     int probe_limit;
     int32_t *out_ids;            /* [nq][out_stride]                        */
@@ -145,6 +155,11 @@ int rv_residual_encode(const float *d_x, const uint8_t *d_codes, int n,
                        int dim, int m, int ks, const float *d_codebook,
                        int bits, uint8_t *d_res, float *d_qparams,
                        void *stream);
+int rv_error4_encode(const float *d_x, const uint8_t *d_codes,
+                     const uint8_t *d_res, const float *d_res_qparams, int n,
+                     int dim, int m, int ks, const float *d_codebook,
+                     int res_bits, uint8_t *d_e4, float *d_e4_qparams,
+                     void *stream);
 
 /* MLP training kernels (FP32 master weights, BF16-rounded forward).
  * Flat parameter layout: w1[64][16] | b1[64] | w2[64][64] | b2[64] | w3[64] |

@@ -68,6 +68,7 @@ int calculator_open(const char *uri, remote_handle64 *handle) {
     c->magic = ROVER_CTX_MAGIC;
     c->device = 0;
     c->l1_bits = 8;
+    rover_sha256(NULL, 0, c->orig_maps_sha);
     pthread_mutex_init(&c->lock, NULL);
     if (rv_set_device(0) || rv_malloc((void **)&c->d_queue, 64)) {
         set_err("calculator_open");
@@ -94,6 +95,8 @@ AEEResult calculator_retrieval_pure_pq_unload_dma(remote_handle64 h) {
     dfree(c->d_res);
     dfree(c->d_qparams);
     dfree(c->d_orig);
+    rover_pure_pq_error4_unload(h);
+    rover_sha256(NULL, 0, c->orig_maps_sha);
     dfree(c->d_router_cent);
     dfree(c->d_router_ids);
     c->d_router_cent = NULL;
@@ -255,12 +258,34 @@ AEEResult rover_pure_pq_residual_load(remote_handle64 h, const uint8_t *codes,
     return ROVER_SUCCESS;
 }
 
+/* The id-map digest an error4 artifact is bound to
+ * (retrieval_final_refine.c:167-180): SHA-256 of graph_to_original then
+ * graph_to_storage as little-endian int32 (the PQ path has no storage map,
+ * so the identity), or of nothing when there is no map. */
+static int maps_digest(const int32_t *map, int n, uint8_t out[32]) {
+    int32_t *maps;
+    if (!map) {
+        rover_sha256(NULL, 0, out);
+        return 0;
+    }
+    maps = malloc((size_t)n * 2 * sizeof(int32_t));
+    if (!maps) return -1;
+    for (int i = 0; i < n; i++) {
+        maps[i] = map[i];
+        maps[(size_t)n + i] = i;
+    }
+    rover_sha256(maps, (size_t)n * 2 * sizeof(int32_t), out);
+    free(maps);
+    return 0;
+}
+
 AEEResult rover_pure_pq_original_ids(remote_handle64 h, const int32_t *map,
                                      int mapLen) {
     RoverCtx *c = ctx_of(h);
     if (!c) return ROVER_EBADPARM;
     dfree(c->d_orig);
     c->d_orig = NULL;
+    maps_digest(NULL, 0, c->orig_maps_sha);
     if (!map) return ROVER_SUCCESS;
     if (!c->pq_loaded || mapLen != c->n) return ROVER_EBADPARM;
     {
@@ -276,9 +301,103 @@ AEEResult rover_pure_pq_original_ids(remote_handle64 h, const int32_t *map,
         }
         free(seen);
     }
-    if (dalloc_copy((void **)&c->d_orig, map, (size_t)mapLen * sizeof(int32_t)))
+    if (maps_digest(map, c->n, c->orig_maps_sha)) return ROVER_ENOMEMORY;
+    if (dalloc_copy((void **)&c->d_orig, map, (size_t)mapLen * sizeof(int32_t))) {
+        maps_digest(NULL, 0, c->orig_maps_sha);
         return ROVER_ENOMEMORY;
+    }
     return ROVER_SUCCESS;
+}
+
+static uint32_t le32(const uint8_t *p) {
+    return (uint32_t)p[0] | (uint32_t)p[1] << 8 | (uint32_t)p[2] << 16 |
+           (uint32_t)p[3] << 24;
+}
+
+static uint64_t le64(const uint8_t *p) { return le32(p) | (uint64_t)le32(p + 4) << 32; }
+
+AEEResult rover_pure_pq_error4_unload(remote_handle64 h) {
+    RoverCtx *c = ctx_of(h);
+    if (!c) return ROVER_EBADPARM;
+    dfree(c->d_e4_blocks);
+    dfree(c->d_e4_qparams);
+    c->d_e4_blocks = NULL;
+    c->d_e4_qparams = NULL;
+    c->e4_loaded = 0;
+    return ROVER_SUCCESS;
+}
+
+/* The checks of refine_open in ERROR4 mode (retrieval_final_refine.c:
+ * 128-160 and 181-186) on an in-memory image. */
+AEEResult rover_pure_pq_error4_load(remote_handle64 h, const uint8_t *a,
+                                    int64_t len, const uint8_t *expected_sha256) {
+    RoverCtx *c = ctx_of(h);
+    uint8_t digest[32];
+    uint64_t n, qb, cbytes;
+    int cb, vpb;
+    if (!c || !a) return ROVER_EBADPARM;
+    if (!c->pq_loaded) {
+        snprintf(g_err, sizeof(g_err), "error4_load: no PQ index loaded");
+        return ROVER_EBADPARM;
+    }
+    n = (uint64_t)c->n;
+    cb = (c->dim + 1) / 2;
+    vpb = RV_E4_BLOCK / cb;
+    if (vpb <= 0 || len < 256) goto bad;
+    qb = n * 8;
+    cbytes = (n + (uint64_t)vpb - 1) / (uint64_t)vpb * RV_E4_BLOCK;
+    if (memcmp(a, "PQERR4V1", 8)) {
+        snprintf(g_err, sizeof(g_err), "error4_load: %s",
+                 memcmp(a, "SQERR4V1", 8) ? "not an error4 artifact"
+                                          : "SQ artifact; the PQ refine needs PQERR4V1");
+        return ROVER_EBADPARM;
+    }
+    if (le32(a + 8) != 1 || le32(a + 12) != (uint32_t)n ||
+        le32(a + 16) != (uint32_t)c->dim ||
+        (le32(a + 20) != 4 && le32(a + 20) != 8) ||
+        (le32(a + 24) != 4 && le32(a + 24) != 8) || le32(a + 28) != RV_E4_BLOCK ||
+        le32(a + 32) != 1 || le32(a + 36) != 1 || le64(a + 40) != qb ||
+        le64(a + 48) != cbytes || (uint64_t)len != 256 + qb + cbytes)
+        goto bad;
+    if (expected_sha256) {
+        rover_sha256(a, (size_t)len, digest);
+        if (memcmp(digest, expected_sha256, 32)) {
+            snprintf(g_err, sizeof(g_err), "error4_load: file digest mismatch");
+            return ROVER_EBADPARM;
+        }
+    }
+    rover_sha256(a + 256, (size_t)(qb + cbytes), digest);
+    if (memcmp(digest, a + 56, 32)) {
+        snprintf(g_err, sizeof(g_err), "error4_load: payload digest mismatch");
+        return ROVER_EBADPARM;
+    }
+    for (uint64_t i = 0; i < n; i++) {
+        float sz[2];
+        memcpy(sz, a + 256 + 8 * i, 8);
+        if (!isfinite(sz[0]) || sz[0] < 0.0f || !isfinite(sz[1])) {
+            snprintf(g_err, sizeof(g_err), "error4_load: bad qparams at row %llu",
+                     (unsigned long long)i);
+            return ROVER_EBADPARM;
+        }
+    }
+    pthread_mutex_lock(&c->lock);
+    rover_pure_pq_error4_unload(h);
+    if (dalloc_copy((void **)&c->d_e4_qparams, a + 256, (size_t)qb) ||
+        dalloc_copy((void **)&c->d_e4_blocks, a + 256 + qb, (size_t)cbytes)) {
+        rover_pure_pq_error4_unload(h);
+        pthread_mutex_unlock(&c->lock);
+        set_err("error4_load");
+        return ROVER_ENOMEMORY;
+    }
+    memcpy(c->e4_maps_sha, a + 216, 32);
+    c->e4_vpb = vpb;
+    c->e4_code_bytes = cb;
+    c->e4_loaded = 1;
+    pthread_mutex_unlock(&c->lock);
+    return ROVER_SUCCESS;
+bad:
+    snprintf(g_err, sizeof(g_err), "error4_load: header or size mismatch");
+    return ROVER_EBADPARM;
 }
 
 AEEResult calculator_retrieval_load(remote_handle64 h, const int32_t *graph,
@@ -417,6 +536,12 @@ static int fill_index(const RoverCtx *c, int method, int correction,
     ix->res_bits = c->res_bits;
     ix->res_code_bytes = c->res_bits == 4 ? c->dim / 2 + c->dim % 2 : c->dim;
     ix->graph_to_original = c->d_orig;
+    if (c->e4_loaded) {
+        ix->e4_blocks = c->d_e4_blocks;
+        ix->e4_qparams = c->d_e4_qparams;
+        ix->e4_vpb = c->e4_vpb;
+        ix->e4_code_bytes = c->e4_code_bytes;
+    }
     return 0;
 }
 
@@ -432,8 +557,23 @@ AEEResult rover_prepare_launch(RoverCtx *c, const RoverSearchBatch *b,
         !b->out_ids || !b->out_dists || !b->out_counts ||
         (b->num_queries > 0 && !b->queries) || b->refine_candidates < 0 ||
         (b->method != ROVER_METHOD_PQ && b->method != ROVER_METHOD_SQ) ||
-        (b->method == ROVER_METHOD_SQ && (b->correction || b->refine_candidates)))
+        (b->method == ROVER_METHOD_SQ && (b->correction || b->refine_candidates)) ||
+        (b->refine_mode != 0 && b->refine_mode != ROVER_REFINE_CURRENT &&
+         b->refine_mode != ROVER_REFINE_ERROR4) ||
+        (b->refine_mode == ROVER_REFINE_ERROR4 && b->refine_candidates <= 0))
         return ROVER_EBADPARM;
+    if (b->refine_mode == ROVER_REFINE_ERROR4) {
+        if (!c->e4_loaded) {
+            snprintf(g_err, sizeof(g_err), "search: no error4 artifact loaded");
+            return ROVER_EFAILED;
+        }
+        /* the reference refuses to open an artifact bound to other id maps */
+        if (memcmp(c->e4_maps_sha, c->orig_maps_sha, 32)) {
+            snprintf(g_err, sizeof(g_err),
+                     "search: error4 artifact is bound to another id map");
+            return ROVER_EBADPARM;
+        }
+    }
     if (fill_index(c, b->method, b->correction, b->refine_candidates, ix)) {
         snprintf(g_err, sizeof(g_err), "search: index not loaded");
         return ROVER_EFAILED;
@@ -458,6 +598,7 @@ AEEResult rover_prepare_launch(RoverCtx *c, const RoverSearchBatch *b,
                      ? RV_SCORE_SQ
                      : (b->correction ? RV_SCORE_ADC_ONLINE : RV_SCORE_ADC);
     rb->refine_candidates = b->refine_candidates;
+    rb->refine_error4 = b->refine_mode == ROVER_REFINE_ERROR4;
     rb->hash_log2 = b->hash_log2 > 0 ? b->hash_log2 : default_hash_log2(max_pool);
     rb->probe_limit = b->probe_limit > 0 ? b->probe_limit : 8;
     rb->out_ids = b->out_ids;
@@ -1176,6 +1317,26 @@ AEEResult rover_build_pq_encode(const float *d_x, int n, int dim, int m, int ks,
         return ROVER_EBADPARM;
     if (rv_pq_encode(d_x, n, dim, m, ks, d_codebook, d_codes, NULL)) {
         set_err("rover_build_pq_encode");
+        return ROVER_EFAILED;
+    }
+    return ROVER_SUCCESS;
+}
+
+AEEResult rover_build_error4_encode(const float *d_x, const uint8_t *d_codes,
+                                    const uint8_t *d_res,
+                                    const float *d_res_qparams, int n, int dim,
+                                    int m, int ks, const float *d_codebook,
+                                    int res_bits, uint8_t *d_e4_codes,
+                                    float *d_e4_qparams) {
+    if (!d_x || !d_codes || !d_res || !d_res_qparams || !d_codebook ||
+        !d_e4_codes || !d_e4_qparams || n <= 0 || dim <= 0 || dim > 8192 ||
+        m <= 0 || dim % m || (ks != 16 && ks != 256) ||
+        (res_bits != 4 && res_bits != 8))
+        return ROVER_EBADPARM;
+    if (rv_error4_encode(d_x, d_codes, d_res, d_res_qparams, n, dim, m, ks,
+                         d_codebook, res_bits, d_e4_codes, d_e4_qparams, NULL) ||
+        rv_synchronize()) {
+        set_err("rover_build_error4_encode");
         return ROVER_EFAILED;
     }
     return ROVER_SUCCESS;

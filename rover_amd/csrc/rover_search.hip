@@ -950,8 +950,10 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
             if (lane == 0) b.out_counts[q] = cnt;
         } else {
             // final refine of the first F pool members
-            // (retrieval_final_refine.c:256-292 with the residual decoder)
+            // (retrieval_final_refine.c:256-292 with the residual decoder,
+            // plus the error4 level of decode_vector, :221-246, when asked)
             const int F = refine < n ? refine : n;
+            const bool e4 = b.refine_error4 != 0;
             uint64_t *dbits = keys_buf[cur ^ 1];
             bool bad = false;
             for (int i = lane; i < F; i += RV_WAVE) {
@@ -960,6 +962,14 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
                 const uint8_t *res = ix.res_codes + (size_t)id * ix.res_code_bytes;
                 const float s = ix.res_qparams[2 * (size_t)id];
                 const float z = ix.res_qparams[2 * (size_t)id + 1];
+                const uint8_t *c4 = nullptr;
+                float s4 = 0.0f, z4 = 0.0f;
+                if (e4) {
+                    c4 = ix.e4_blocks + (size_t)(id / ix.e4_vpb) * RV_E4_BLOCK +
+                         (size_t)(id % ix.e4_vpb) * ix.e4_code_bytes;
+                    s4 = ix.e4_qparams[2 * (size_t)id];
+                    z4 = ix.e4_qparams[2 * (size_t)id + 1];
+                }
                 double acc = 0.0;
                 if (!isfinite(s) || !isfinite(z) || !(s > 0.0f)) bad = true;
                 int d = 0;
@@ -969,6 +979,13 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
                     for (int t = 0; t < ix.subdim; t++, d++) {
                         float v = rv_decode_value(cent[t], rv_res_code(res, ix.res_bits, d), s, z);
                         if (!isfinite(v)) bad = true;
+                        if (e4) {
+                            // add_f32(v, (code * scale) + zero), each rounded
+                            const float level =
+                                (float)((c4[d >> 1] >> (4 * (d & 1))) & 15) * s4 + z4;
+                            v = v + level;
+                            if (!isfinite(v)) bad = true;
+                        }
                         double delta = (double)qv[d] - (double)v;
                         acc += delta * delta;
                     }
@@ -1136,6 +1153,74 @@ __global__ __launch_bounds__(RV_WAVE) void rv_residual_encode_kernel(
     if (lane == 0) {
         qparams[2 * (size_t)v] = bad ? NAN : scale;
         qparams[2 * (size_t)v + 1] = lo;
+    }
+}
+
+// retrieval_build_artifacts.c:25-66 (retrieval_build_error4), one vector
+// per wavefront.  recon = the residual decode (retrieval_pq_residual.c:
+// 954-981), e = x - recon; zero = the first minimum of e and hi the first
+// maximum (the reference's strict scans keep the lowest dimension among
+// equal values; the reduction carries the dimension to do the same),
+// scale = max(hi - zero, 1e-12) / 15, code = clamp(floor((e - zero) *
+// (1 / scale) + 0.5), 0, 15), two codes per byte, low nibble = even dim.
+__device__ __forceinline__ float rv_e4_error(const float *xv, const uint8_t *code,
+                                             const uint8_t *res, const float *codebook,
+                                             int d, int sub, int ks, int res_bits,
+                                             float s, float z) {
+    const int sb = d / sub;
+    const float c = codebook[((size_t)sb * ks + rv_code_get(code, sb, ks)) * sub + (d % sub)];
+    const float rec = rv_decode_value(c, rv_res_code(res, res_bits, d), s, z);
+    return xv[d] - rec;
+}
+
+__global__ __launch_bounds__(RV_WAVE) void rv_error4_encode_kernel(
+    const float *x, const uint8_t *codes, const uint8_t *res, const float *res_qp,
+    int n, int dim, int m, int ks, const float *codebook, int code_bytes,
+    int res_bits, int res_bytes, uint8_t *e4, int e4_bytes, float *e4_qp) {
+    const int v = blockIdx.x;
+    const int lane = rv_lane();
+    if (v >= n) return;
+    const int sub = dim / m;
+    const uint8_t *code = codes + (size_t)v * code_bytes;
+    const uint8_t *rv = res + (size_t)v * res_bytes;
+    const float s = res_qp[2 * (size_t)v], z = res_qp[2 * (size_t)v + 1];
+    const float *xv = x + (size_t)v * dim;
+    float lo = INFINITY, hi = -INFINITY;
+    int lo_d = 0x7fffffff, hi_d = 0x7fffffff;
+    bool bad = false;
+    for (int d = lane; d < dim; d += RV_WAVE) {
+        const float e = rv_e4_error(xv, code, rv, codebook, d, sub, ks, res_bits, s, z);
+        if (!isfinite(e)) bad = true;
+        if (e < lo) lo = e, lo_d = d;
+        if (e > hi) hi = e, hi_d = d;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const float a = __shfl_xor(lo, off), b2 = __shfl_xor(hi, off);
+        const int ad = __shfl_xor(lo_d, off), bd = __shfl_xor(hi_d, off);
+        if (a < lo || (a == lo && ad < lo_d)) lo = a, lo_d = ad;
+        if (b2 > hi || (b2 == hi && bd < hi_d)) hi = b2, hi_d = bd;
+    }
+    bad = __ballot(bad) != 0ull;
+    const float scale = fmaxf(hi - lo, 1.0e-12f) / 15.0f;
+    const float inv = 1.0f / scale;
+    uint8_t *out = e4 + (size_t)v * e4_bytes;
+    for (int byte = lane; byte < e4_bytes; byte += RV_WAVE) {
+        int packed = 0;
+        for (int h = 0; h < 2; h++) {
+            const int d = 2 * byte + h;
+            if (d >= dim) break;
+            const float e = rv_e4_error(xv, code, rv, codebook, d, sub, ks, res_bits, s, z);
+            const float t = (e - lo) * inv + 0.5f;
+            int c = bad ? 0 : (int)floorf(t);
+            c = c < 0 ? 0 : (c > 15 ? 15 : c);
+            packed |= c << (4 * h);
+        }
+        out[byte] = (uint8_t)packed;
+    }
+    if (lane == 0) {
+        e4_qp[2 * (size_t)v] = bad ? NAN : scale;
+        e4_qp[2 * (size_t)v + 1] = lo;
     }
 }
 
@@ -1386,6 +1471,22 @@ extern "C" int rv_residual_encode(const float *x, const uint8_t *codes, int n,
     rv_residual_encode_kernel<<<n, RV_WAVE, 0, (hipStream_t)stream>>>(
         x, codes, n, dim, m, ks, codebook, code_bytes, bits, res, res_bytes, qparams);
     return rv_check(hipGetLastError(), "rv_residual_encode");
+}
+
+extern "C" int rv_error4_encode(const float *x, const uint8_t *codes,
+                                const uint8_t *res, const float *res_qp, int n,
+                                int dim, int m, int ks, const float *codebook,
+                                int res_bits, uint8_t *e4, float *e4_qp,
+                                void *stream) {
+    if (n <= 0 || m <= 0 || dim <= 0 || dim % m || (res_bits != 4 && res_bits != 8))
+        return -1;
+    const int code_bytes = ks <= 16 ? m / 2 + m % 2 : m;
+    const int res_bytes = res_bits == 4 ? dim / 2 + dim % 2 : dim;
+    const int e4_bytes = (dim + 1) / 2;
+    rv_error4_encode_kernel<<<n, RV_WAVE, 0, (hipStream_t)stream>>>(
+        x, codes, res, res_qp, n, dim, m, ks, codebook, code_bytes, res_bits,
+        res_bytes, e4, e4_bytes, e4_qp);
+    return rv_check(hipGetLastError(), "rv_error4_encode");
 }
 
 // ----------------------------------------------------------- entry router

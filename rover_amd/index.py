@@ -611,3 +611,33 @@ def encode_on_device(x, codebook, res_bits):
     res = dres.download(np.uint8, n * rb).reshape(n, rb)
     qp = dqp.download(np.float32, n * 2).reshape(n, 2)
     return codes, res, qp
+
+
+def encode_error4_on_device(x, codes, codebook, res_codes, res_qparams, res_bits):
+    """Third-level refine codes with the library's HIP encoder
+    (rover_build_error4_encode): returns ([n][(dim+1)/2] uint8 codes,
+    [n][2] float32 (scale, zero)).  Raises if any row's error is not finite."""
+    from . import native
+    L = native.lib()
+    m, ks, sub = codebook.shape
+    n, dim = x.shape
+    e4b = (dim + 1) // 2
+    if (m * sub != dim or codes.shape != (n, code_bytes(m, ks))
+            or res_codes.shape != (n, residual_code_bytes(dim, res_bits))
+            or res_qparams.shape != (n, 2)):
+        raise ValueError("error4 encode: inconsistent index shapes")
+    dx = native.DeviceBuffer.from_array(np.ascontiguousarray(x, dtype=np.float32))
+    dcodes = native.DeviceBuffer.from_array(np.ascontiguousarray(codes, dtype=np.uint8))
+    dcb = native.DeviceBuffer.from_array(np.ascontiguousarray(codebook, dtype=np.float32))
+    dres = native.DeviceBuffer.from_array(np.ascontiguousarray(res_codes, dtype=np.uint8))
+    dqp = native.DeviceBuffer.from_array(np.ascontiguousarray(res_qparams, dtype=np.float32))
+    de4 = native.DeviceBuffer(n * e4b)
+    de4qp = native.DeviceBuffer(n * 8)
+    native.check(L.rover_build_error4_encode(dx.ptr, dcodes.ptr, dres.ptr, dqp.ptr, n, dim,
+                                             m, ks, dcb.ptr, res_bits, de4.ptr, de4qp.ptr),
+                 "rover_build_error4_encode")
+    e4 = de4.download(np.uint8, n * e4b).reshape(n, e4b)
+    e4qp = de4qp.download(np.float32, n * 2).reshape(n, 2)
+    if not np.isfinite(e4qp).all():
+        raise ValueError("non-finite reconstruction error")
+    return e4, e4qp

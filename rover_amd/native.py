@@ -12,6 +12,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "librover.so")
 
 ROVER_SUCCESS = 0
+ROVER_EFAILED = 1
+ROVER_ENOMEMORY = 2
+ROVER_EBADPARM = 14
 METHOD_PQ = 0
 METHOD_SQ = 1
 ENTRY_POINT_SEED = 42
@@ -19,7 +22,11 @@ ABI_VERSION = 1
 
 
 class RoverError(RuntimeError):
-    pass
+    """A failed op; .code is its AEEResult (ROVER_EBADPARM, ...)."""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
 
 
 class PurePqSearchIndexView(C.Structure):
@@ -67,7 +74,11 @@ class RoverSearchBatch(C.Structure):
                 ("out_counts", C.c_void_p), ("out_stride", C.c_int),
                 ("out_stats", C.c_void_p), ("hash_log2", C.c_int),
                 ("probe_limit", C.c_int), ("grid_waves", C.c_int),
-                ("learned_stop", C.c_int), ("stop_threshold", C.c_float)]
+                ("learned_stop", C.c_int), ("stop_threshold", C.c_float),
+                ("refine_mode", C.c_int)]
+
+
+REFINE_CURRENT, REFINE_ERROR4 = 1, 2
 
 
 class RoverLearnedStopTrain(C.Structure):
@@ -115,6 +126,9 @@ SIGNATURES = [
     ("rover_pure_pq_residual_load", _I,
      [_H, _P, C.c_int64, _P, C.c_int64, C.c_int32]),
     ("rover_pure_pq_original_ids", _I, [_H, _P, _I]),
+    ("rover_pure_pq_error4_load", _I, [_H, _P, C.c_int64, _P]),
+    ("rover_pure_pq_error4_unload", _I, [_H]),
+    ("rover_sha256", None, [_P, C.c_size_t, _P]),
     ("rover_pure_pq_router_load", _I, [_H, _P, _I, _P, _I]),
     ("rover_route_entries", _I, [_H, _P, _I, _I, _P, _P]),
     ("rover_pq_residual_evaluator", _I, [_P, _I, _F, _P]),
@@ -137,6 +151,8 @@ SIGNATURES = [
     ("rover_mlp_adam_step", _I, [_P, _P, _P, _P, _I, _F, _F]),
     ("rover_build_pq_encode", _I, [_P, _I, _I, _I, _I, _P, _P]),
     ("rover_build_residual_encode", _I, [_P, _P, _I, _I, _I, _I, _P, _I, _P, _P]),
+    ("rover_build_error4_encode", _I,
+     [_P, _P, _P, _P, _I, _I, _I, _I, _P, _I, _P, _P]),
     ("rover_device_alloc", _I, [C.POINTER(_P), C.c_size_t]),
     ("rover_device_free", _I, [_P]),
     ("rover_device_upload", _I, [_P, _P, C.c_size_t]),
@@ -183,7 +199,7 @@ def last_error():
 
 def check(rc, what):
     if rc != ROVER_SUCCESS:
-        raise RoverError("%s failed (rc=%d): %s" % (what, rc, last_error()))
+        raise RoverError("%s failed (rc=%d): %s" % (what, rc, last_error()), rc)
 
 
 def ptr(a):

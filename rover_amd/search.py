@@ -84,6 +84,21 @@ class Calculator:
         check(native.lib().rover_pure_pq_original_ids(
             self.h, ptr(m), 0 if m is None else len(m)), "rover_pure_pq_original_ids")
 
+    def pure_pq_error4_load(self, artifact, expected_sha256=None):
+        """Loads a PQERR4V1 third-level refine artifact (the file image as
+        bytes or a uint8 array); expected_sha256 (32 bytes) checks the whole
+        file as RetrievalFinalRefineConfig.artifact_sha256 does."""
+        a = np.frombuffer(artifact, dtype=np.uint8) if isinstance(artifact, (bytes, bytearray)) \
+            else np.ascontiguousarray(artifact, dtype=np.uint8)
+        e = None if expected_sha256 is None else np.frombuffer(bytes(expected_sha256), dtype=np.uint8)
+        if e is not None and e.size != 32:
+            raise ValueError("expected_sha256 must be 32 bytes")
+        check(native.lib().rover_pure_pq_error4_load(self.h, ptr(a), a.size, ptr(e)),
+              "rover_pure_pq_error4_load")
+
+    def pure_pq_error4_unload(self):
+        check(native.lib().rover_pure_pq_error4_unload(self.h), "rover_pure_pq_error4_unload")
+
     def load_pq_index(self, data, chunk_vectors=None, with_router=True):
         """Loads a rover_amd.index.PqIndexData (codes, graph, residuals and,
         with with_router, its entry router)."""
@@ -143,8 +158,9 @@ class Calculator:
                       out_counts, stride, out_stats, hash_log2, probe_limit,
                       grid_waves, learned_stop, stop_threshold, pool_sizes,
                       num_outs, seed, entry_ids=None, entry_count=0,
-                      entry_stride=0, route_entries=0):
+                      entry_stride=0, route_entries=0, refine_mode=0):
         b = native.RoverSearchBatch()
+        b.refine_mode = int(refine_mode)
         b.entry_ids = entry_ids
         b.entry_count = int(entry_count)
         b.entry_stride = int(entry_stride)
@@ -178,7 +194,7 @@ class Calculator:
                      expand_count=32, stats=False, hash_log2=0, probe_limit=0,
                      grid_waves=0, learned_stop=0, stop_threshold=0.0,
                      seed=native.ENTRY_POINT_SEED, entry_ids=None,
-                     route_entries=0):
+                     route_entries=0, refine_mode=0):
         """Host arrays in and out.  entry_ids: explicit seeds, a list shared
         by all queries or a [nq, S] array of per-query rows (otherwise graph
         entry + LCG seeds); route_entries=S: per-query seeds from the router.  pool_size / num_out: int or per-query
@@ -202,7 +218,7 @@ class Calculator:
                                hash_log2, probe_limit, grid_waves, learned_stop,
                                stop_threshold, ptr(Ls), ptr(Ks), seed, ptr(e),
                                0 if e is None else (stride or len(e)), stride,
-                               route_entries)
+                               route_entries, refine_mode)
         check(native.lib().rover_search_batch(self.h, C.byref(b)), "rover_search_batch")
         return (ids, d, cnt, st) if stats else (ids, d, cnt)
 
@@ -214,7 +230,8 @@ class Calculator:
                             stop_threshold=0.0, d_pool_sizes=None,
                             d_num_outs=None, stream=None,
                             seed=native.ENTRY_POINT_SEED, d_entry_ids=None,
-                            entry_count=0, entry_stride=0, route_entries=0):
+                            entry_count=0, entry_stride=0, route_entries=0,
+                            refine_mode=0):
         """Device pointers in and out; asynchronous.  With per-query device
         arrays, pool_size must be their maximum."""
         b = self._batch_struct(d_queries, nq, pool_size, num_out, method,
@@ -223,7 +240,7 @@ class Calculator:
                                d_stats, hash_log2, probe_limit, grid_waves,
                                learned_stop, stop_threshold, d_pool_sizes,
                                d_num_outs, seed, d_entry_ids, entry_count,
-                               entry_stride, route_entries)
+                               entry_stride, route_entries, refine_mode)
         check(native.lib().rover_search_batch_device(self.h, C.byref(b), stream),
               "rover_search_batch_device")
 
@@ -235,7 +252,7 @@ class Calculator:
                             probe_limit=0, grid_waves=0, learned_stop=0,
                             stop_threshold=0.0, d_pool_sizes=None, d_num_outs=None,
                             seed=native.ENTRY_POINT_SEED, d_entry_ids=None,
-                            entry_count=0, entry_stride=0):
+                            entry_count=0, entry_stride=0, refine_mode=0):
         """Captures a device batch in a HIP graph (rover_search_graph_create);
         returns the graph handle for search_graph_launch."""
         b = self._batch_struct(d_queries, capacity, pool_size, num_out, method,
@@ -244,7 +261,7 @@ class Calculator:
                                d_stats, hash_log2, probe_limit, grid_waves,
                                learned_stop, stop_threshold, d_pool_sizes,
                                d_num_outs, seed, d_entry_ids, entry_count,
-                               entry_stride, 0)
+                               entry_stride, 0, refine_mode)
         g = C.c_void_p()
         check(native.lib().rover_search_graph_create(self.h, C.byref(b), d_num_queries,
                                                      stream, C.byref(g)),

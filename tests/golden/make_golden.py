@@ -3,13 +3,14 @@
 Run in a container that has the Rover source tree (oracle/Makefile builds
 oracle/_ref/librover_ref.so from it):
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py           # every fixture
+    python tests/golden/make_golden.py error4    # only the *_error4 files
 
 Every expected output in the fixtures comes from the reference's own scalar
 C: retrieval_pure_pq_search_scalar_with_policy, retrieval_pq_build_lut,
 retrieval_pq_adc_ids_batch_scalar, retrieval_pq_train/encode_batch, the
 PQRES01 residual writer/reader and its evaluator/decoder,
-retrieval_final_refine, the SQ quantize/distance kernels and
+retrieval_final_refine (CURRENT and ERROR4), retrieval_build_error4, the SQ quantize/distance kernels and
 retrieval_sqmd_make_seed_ids.  Inputs are small synthetic indexes; base
 vectors are multiples of 1/16 so they are stored exactly as uint16.
 """
@@ -230,11 +231,75 @@ def seeds_case(name):
     print("wrote", path, os.path.getsize(path), "bytes")
 
 
+def error4_case(name):
+    """Third-level refine for an existing PQ fixture: the reference's
+    retrieval_build_error4 writes the PQERR4V1 artifact (its residual decoder
+    as the reconstruction), and retrieval_final_refine in ERROR4 and CURRENT
+    modes refines the fixture's refine candidates.  Writes <name>_error4.npz."""
+    ref = pyoracle.reference()
+    g = dict(np.load(os.path.join(HERE, name + ".npz"), allow_pickle=False))
+    base = g["base_x16"].astype(np.float32) / np.float32(16.0)
+    queries = g["queries_x16"].astype(np.float32) / np.float32(16.0)
+    cb, codes = g["codebook"], g["codes"]
+    m, ks, sub = cb.shape
+    n, dim = base.shape
+    bits = int(g["res_bits"])
+    res = np.zeros_like(g["res_codes"])
+    qp = np.zeros_like(g["res_qparams"])
+    handle = ref.ref_residual_open(_p(base), _p(codes), _p(cb), n, dim, m, ks,
+                                   bits, _p(res), _p(qp))
+    assert handle
+    assert np.array_equal(res, g["res_codes"]) and np.array_equal(qp, g["res_qparams"])
+    header = np.zeros(256, dtype=np.uint8)
+    e4_qp = np.zeros((n, 2), dtype=np.float32)
+    e4_codes = np.zeros((n, (dim + 1) // 2), dtype=np.uint8)
+    rc = ref.ref_error4_build(handle, _p(base), _p(codes), codes.shape[1], _p(cb),
+                              cb.size, n, dim, _p(header), _p(e4_qp), _p(e4_codes))
+    assert rc == 0, rc
+    size = ref.ref_error4_file(handle, None, 0)
+    artifact = np.zeros(size, dtype=np.uint8)
+    assert ref.ref_error4_file(handle, _p(artifact), size) == size
+    F, K = int(g["refine_F"]), int(g["refine_K"])
+    mode, ep, expand, L = (int(v) for v in g["refine_source"])
+    key = "m%d_e%d_x%d_L%d" % (mode, ep, expand, L)
+    out_ids = {1: [], 2: []}
+    out_d = {1: [], 2: []}
+    for qi, q in enumerate(queries):
+        for rmode in (1, 2):
+            ids = np.ascontiguousarray(g[key + "_ids"][qi][:F]).copy()
+            d = np.zeros(F, dtype=np.float32)
+            rc = ref.ref_refine_mode(handle, rmode, n, dim, _p(q), _p(ids), _p(d),
+                                     F, K, None)
+            assert rc == K, (rmode, rc)
+            out_ids[rmode].append(ids[:K].copy())
+            out_d[rmode].append(d[:K].copy())
+    # an ERROR4 open with a graph->original map the artifact is not bound to
+    perm = np.ascontiguousarray(np.arange(n, dtype=np.int32)[::-1])
+    ids = np.ascontiguousarray(g[key + "_ids"][0][:F]).copy()
+    d = np.zeros(F, dtype=np.float32)
+    rc_map = ref.ref_refine_mode(handle, 2, n, dim, _p(queries[0]), _p(ids), _p(d),
+                                 F, K, _p(perm))
+    ref.ref_residual_close(handle)
+    assert np.array_equal(out_ids[1], g["refine_ids"])
+    out = dict(artifact=artifact, header=header, e4_qparams=e4_qp, e4_codes=e4_codes,
+               refine_current_ids=np.stack(out_ids[1]),
+               refine_current_dists=np.stack(out_d[1]),
+               refine_e4_ids=np.stack(out_ids[2]), refine_e4_dists=np.stack(out_d[2]),
+               rc_e4_unbound_map=np.int32(rc_map))
+    path = os.path.join(HERE, name + "_error4.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
 def main():
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "all"],
                    check=True)
     if not pyoracle.ref_available():
         sys.exit("reference build missing: oracle/_ref/librover_ref.so")
+    if sys.argv[1:] == ["error4"]:
+        for name in ("pq_d128_m32", "pq_d96_m48", "pq_d64_ks16"):
+            error4_case(name)
+        return
     base, queries, graph, entry = pq_case(
         "pq_d128_m32", n=1500, dim=128, m=32, ks=256, R=32, res_bits=4,
         comps=48, nq=12, Ls=(16, 64, 160), expands=(32, 12), seed=42)
@@ -244,6 +309,8 @@ def main():
     pq_case("pq_d64_ks16", n=800, dim=64, m=32, ks=16, R=16, res_bits=8,
             comps=24, nq=8, Ls=(16, 64), expands=(16,), seed=62)
     seeds_case("seeds")
+    for name in ("pq_d128_m32", "pq_d96_m48", "pq_d64_ks16"):
+        error4_case(name)
 
 
 if __name__ == "__main__":
