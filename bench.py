@@ -38,7 +38,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from rover_amd import data, index, native  # noqa: E402
+from rover_amd import artifacts, data, index, native  # noqa: E402
 from rover_amd.search import Calculator  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
@@ -164,6 +164,9 @@ def main():
     ap.add_argument("--refine", type=int, default=64,
                     help="refine candidates F of the ADC mode")
     ap.add_argument("--modes", default="online,adc")
+    ap.add_argument("--refine-factor", type=int, default=2,
+                    help="refine leg: refine the top factor*k of the pool, with and "
+                         "without the error4 level (0 skips the leg)")
     ap.add_argument("--target", type=float, default=0.95)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -213,6 +216,7 @@ def main():
                                router_centroids=z["router_centroids"],
                                router_ids=z["router_ids"])
         queries, gt = z["queries"], z["gt"]
+        e4 = (z["e4_codes"], z["e4_qparams"]) if "e4_codes" in z.files else None
         assert queries.shape[0] == args.nq and ix.res_bits == args.res_bits
         log("index loaded from", args.load_index)
     else:
@@ -228,6 +232,11 @@ def main():
             ix.router_centroids.shape)
         gt = data.ground_truth(base, queries, k=100, device=dev)
         log("ground truth done")
+        e4 = None
+        if args.refine_factor > 0:
+            e4 = index.encode_error4_on_device(base, ix.codes, ix.codebook, ix.res_codes,
+                                               ix.res_qparams, ix.res_bits)
+            log("error4 codes encoded on the device")
         torch.cuda.synchronize()
         torch.cuda.empty_cache()
         if args.save_index:
@@ -235,7 +244,8 @@ def main():
                      codes=ix.codes, codebook=ix.codebook, res_codes=ix.res_codes,
                      res_qparams=ix.res_qparams, res_bits=np.int32(ix.res_bits),
                      router_centroids=ix.router_centroids, router_ids=ix.router_ids,
-                     queries=queries, gt=gt)
+                     queries=queries, gt=gt,
+                     **({} if e4 is None else dict(e4_codes=e4[0], e4_qparams=e4[1])))
             log("index saved to", args.save_index)
         del base
 
@@ -268,11 +278,13 @@ def main():
 
     tuned = {}  # (mode, L, learned) -> (hash_log2, probe_limit, grid_waves)
 
-    def run(mode, L, stats=False, learned=False):
+    def run(mode, L, stats=False, learned=False, refine=None, refine_mode=0):
         h2, pl, gw = tuned.get((mode, L, learned), (0, 0, 0))
         calc.search_batch_device(dq.ptr, nq, L, k, d_ids.ptr, d_d.ptr, d_cnt.ptr, k,
                                  correction=1 if mode == "online" else 0,
-                                 refine_candidates=refine_of(mode, L),
+                                 refine_candidates=refine_of(mode, L) if refine is None
+                                 else min(refine, L),
+                                 refine_mode=refine_mode,
                                  expand_count=args.R,
                                  d_stats=d_st.ptr if stats else None,
                                  learned_stop=1 if learned else 0,
@@ -470,6 +482,42 @@ def main():
     host_qps = nq * args.steps / (time.time() - t1)
     log("plain search from host buffers (PCIe included): %.0f QPS" % host_qps)
 
+    # ---- refine leg: the top refine_factor*k pool members re-ranked in FP64
+    # from PQ + residual (CURRENT) and with the error4 level (ERROR4)
+    refine_leg = None
+    if args.refine_factor > 0 and e4 is not None:
+        calc.pure_pq_error4_load(artifacts.error4_image(e4[0], e4[1], args.dim, 8,
+                                                        args.res_bits))
+        F = args.refine_factor * k
+        refine_leg = {"refine_candidates": F, "mode": mode}
+        for name, rmode in (("current", native.REFINE_CURRENT),
+                            ("error4", native.REFINE_ERROR4)):
+            rows, hit = [], None
+            for Lr in sweep_Ls:
+                if Lr < F:
+                    continue
+                run(mode, Lr, refine=F, refine_mode=rmode)
+                native.synchronize()
+                ids = d_ids.download(np.int32, nq * k).reshape(nq, k)
+                cnt = d_cnt.download(np.int32, nq)
+                assert (cnt == k).all(), "refine search failed: %s" % np.unique(cnt)
+                rec = data.recall_at_k(ids, cnt, gt, k)
+                for _ in range(args.warmup):
+                    run(mode, Lr, refine=F, refine_mode=rmode)
+                e0, e1 = native.Event(), native.Event()
+                e0.record()
+                for _ in range(args.steps):
+                    run(mode, Lr, refine=F, refine_mode=rmode)
+                e1.record()
+                ms = e0.elapsed_ms(e1) / args.steps
+                rows.append(dict(L=Lr, recall10=round(rec, 4), qps=round(nq / (ms / 1e3), 1)))
+                log("refine %s F=%d L=%d recall@10=%.4f qps=%.0f" % (name, F, Lr, rec,
+                                                                    nq / (ms / 1e3)))
+                if rec >= args.target:
+                    hit = rows[-1]
+                    break
+            refine_leg[name] = dict(sweep=rows, at_target=hit)
+
     batching = None
     if args.batches > 0:
         batching = batching_leg(calc, queries, ents, S, mode, refine_of, args)
@@ -579,6 +627,7 @@ def main():
                 brief(learned), training=learned["training"],
                 other_slacks=learned["other_slacks"]),
             "host_buffers_qps": round(host_qps, 1),
+            "refine": refine_leg,
             "launch_tuning": tuning,
             "uneven_batches": batching,
         },
