@@ -107,6 +107,14 @@ AEEResult rover_pure_pq_residual_load(remote_handle64 h,
                                       const float *qparams, int64_t qparamsLen,
                                       int32_t residual_bits);
 
+/* Per-vector residuals of the SQ index for the corrected SQ traversal:
+ * codes [num_vectors][(dim+1)/2 or dim] (bits 4 / 8, low nibble = even dim)
+ * and (scale, zero) [num_vectors][2] -- the residual and qparams payloads of
+ * encode_l1_residual.py (the reference's residual block file flattened). */
+AEEResult rover_sq_residual_load(remote_handle64 h, const uint8_t *codes,
+                                 int64_t codesLen, const float *qparams,
+                                 int64_t qparamsLen, int32_t bits);
+
 /* Entry router (no reference counterpart): `count` centroids
  * [count][dim] and one representative node id per centroid.  A batch with
  * route_entries = S seeds each query with the representatives of its S
@@ -276,9 +284,30 @@ typedef struct RoverSearchBatch {
        residual; ROVER_REFINE_ERROR4 adds the error4 level
        (needs rover_pure_pq_error4_load)                                    */
     int refine_mode;
+    /* traversal (ROVER_TRAVERSAL_*): 0 = the method's reference traversal
+       (PQ: best-first in (distance, id) order; SQ: the live SQ traversal)  */
+    int traversal;
 } RoverSearchBatch;
 
 enum { ROVER_REFINE_CURRENT = 1, ROVER_REFINE_ERROR4 = 2 };
+
+enum {
+    ROVER_TRAVERSAL_DEFAULT = 0,
+    /* best-first over a pool ordered by (distance, id)
+       (retrieval_sqmd_pure_pq.c:793-969); any method                      */
+    ROVER_TRAVERSAL_BEST_FIRST = 1,
+    /* SQ only: the live SQ traversal (retrieval_sqmd_common.c, fake-DSP turn
+       loop :2052-2152): distance-only heaps, seeds from
+       retrieval_sqmd_make_seed_ids (entry_points, seed), at most 32
+       neighbors per expansion; correction = 1 ranks by the exact SQ +
+       residual distance (needs rover_sq_residual_load).  entry_ids and
+       route_entries are not accepted (ROVER_EBADPARM).                    */
+    ROVER_TRAVERSAL_SQMD = 2,
+    /* SQ only: the DSP search op (retrieval_async_dsp_search.c:368-509):
+       the same heaps, seeds graph entry + LCG (rover_make_seed_ids); here
+       also at most 32 neighbors per expansion (ROVER_EBADPARM beyond)     */
+    ROVER_TRAVERSAL_SQ_DSP = 3
+};
 
 /* Host buffers in and out; returns when results are in place. */
 AEEResult rover_search_batch(remote_handle64 h, const RoverSearchBatch *b);
@@ -388,6 +417,13 @@ AEEResult rover_mlp_adam_step(float *d_params, const float *d_grad, float *d_m1,
  * (rpc_demo_pure_pq.c:192-208).  Returns the count. */
 int rover_make_seed_ids(int num_vectors, int entry_count, int graph_entry,
                         uint32_t seed, int32_t *out_ids);
+
+/* Seeds of the live SQ path (retrieval_sqmd_seed.c:15-63): graph_entry
+ * (when in range), then s = s*1103515245 + 12345, id = (s / 65536) % n,
+ * skipping duplicates (at most 64 * target + 1024 draws), then the lowest
+ * unused ids.  Returns the count (<= capacity), or -1. */
+int rover_sqmd_seed_ids(int num_vectors, int entry_points, int graph_entry,
+                        uint32_t seed, int32_t *out_ids, int capacity);
 
 /* Index building on the device, bit-identical to the reference encoders:
  * PQ codes (retrieval_pq_core.c:226-268) and per-vector residual codes and

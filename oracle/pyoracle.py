@@ -71,6 +71,10 @@ def oracle():
         L.orc_fingerprint.restype = C.c_uint64
         L.orc_search_batch.argtypes = [P, I, P, I, P, I, I, I, P, P, I, I, I, P,
                                        P, P, P]
+        L.orc_sqmd_search.argtypes = [P, I, P, P, I, I, I, I, P, P, I, P]
+        L.orc_sqmd_seed_ids.argtypes = [I, I, I, C.c_uint32, P, I]
+        L.orc_sq_residual_distance.argtypes = [P, P, I]
+        L.orc_sq_residual_distance.restype = F
         _orc = L
     return _orc
 
@@ -105,6 +109,8 @@ def reference():
         L.ref_error4_file.argtypes = [P, P, C.c_long]
         L.ref_error4_file.restype = C.c_long
         L.ref_refine_mode.argtypes = [P, I, I, I, P, P, P, I, I, P]
+        L.ref_sqmd_search.argtypes = [P, I, I, P, I, I, F, F, P, P, I, I, P, I, I,
+                                      C.c_uint, I, I, I, P, P, I, P]
         _ref = L
     return _ref
 
@@ -295,6 +301,41 @@ def sq_encode(x, bits, scale, zero):
     for i in range(n):
         oracle().orc_sq_encode(_p(x[i]), dim, bits, scale, zero, _p(out[i]))
     return out
+
+
+class OrcSqmdStats(C.Structure):
+    _fields_ = [("work_items", C.c_uint64), ("dedup_skips", C.c_uint64),
+                ("invalid_skips", C.c_uint64), ("applied", C.c_uint64),
+                ("turns", C.c_uint64), ("empty_turns", C.c_uint64)]
+
+
+def sqmd_search(index, query, seeds, entry_points, expand, L, num_out, correction=0):
+    """The live SQ traversal (distance-only heaps) for one query; returns
+    (ids, dists, stats dict) or (None, None, stats)."""
+    q = np.ascontiguousarray(query, dtype=np.float32)
+    e = np.ascontiguousarray(seeds, dtype=np.int32)
+    ids = np.zeros(max(num_out, 1), dtype=np.int32)
+    d = np.zeros(max(num_out, 1), dtype=np.float32)
+    st = OrcSqmdStats()
+    n = oracle().orc_sqmd_search(C.byref(index.struct), correction, _p(q), _p(e), len(e),
+                                 entry_points, expand, L, _p(ids), _p(d), num_out,
+                                 C.byref(st))
+    stats = {f: getattr(st, f) for f, _ in OrcSqmdStats._fields_}
+    if n < 0:
+        return None, None, stats
+    return ids[:n].copy(), d[:n].copy(), stats
+
+
+def sqmd_seed_ids(n, entry_points, graph_entry, seed=42):
+    out = np.zeros(max(entry_points, 1), dtype=np.int32)
+    cnt = oracle().orc_sqmd_seed_ids(n, entry_points, graph_entry, seed, _p(out),
+                                     len(out))
+    return out[:max(cnt, 0)]
+
+
+def sq_residual_distance(index, query, vid):
+    q = np.ascontiguousarray(query, dtype=np.float32)
+    return oracle().orc_sq_residual_distance(C.byref(index.struct), _p(q), vid)
 
 
 def seed_ids(n, entry_count, graph_entry, seed=42):

@@ -25,6 +25,7 @@
 #include "dsp_demo/retrieval_l1_int8_core.h"
 #include "dsp_demo/retrieval_pq_core.h"
 #include "dsp_demo/retrieval_pq_residual.h"
+#include "dsp_demo/retrieval_sqmd_common.h"
 #include "dsp_demo/retrieval_sqmd_pure_pq.h"
 #include "dsp_demo/retrieval_sqmd_seed.h"
 #include "rover_config.h"
@@ -458,5 +459,94 @@ int ref_refine_mode(void *residual, int mode, int n, int dim,
     if (rc) return rc - 100;
     rc = retrieval_final_refine(ctx, query, ids, dists, count, output_k, NULL);
     retrieval_final_refine_close(ctx);
+    return rc;
+}
+
+/* ---- the live SQ traversal: sqmd_cpu_state_run_fake_dsp ---- */
+
+typedef struct {
+    const uint8_t *codes, *qsq, *res;
+    const float *query, *res_qp;
+    int dim, bits, cb, res_bits, res_cb;
+    float scale, zero;
+} RefSqmdCtx;
+
+static float ref_sqmd_coarse(const RefSqmdCtx *c, int id) {
+    return ref_sq_distance(c->qsq, c->codes + (size_t)id * c->cb, c->dim, c->bits,
+                           c->scale);
+}
+
+static int ref_sqmd_distance_fn(const SqmdWorkItem *work, float *coarse, void *ctx) {
+    *coarse = ref_sqmd_coarse(ctx, work->neighbor_id);
+    return 0;
+}
+
+/* The exact SQ + residual distance: the vector decoded by the reference's
+ * retrieval_final_refine_decode_sq (l1 and residual levels, each
+ * (code * scale) + zero, summed), then the sequential f32 sum of squares. */
+static float ref_sqmd_exact(const RefSqmdCtx *c, int id) {
+    float v[8192], dist = 0.0f;
+    const float *qp = c->res_qp + 2 * (size_t)id;
+    if (retrieval_final_refine_decode_sq(c->codes + (size_t)id * c->cb, c->bits, c->scale,
+                                         c->zero, c->res + (size_t)id * c->res_cb,
+                                         c->res_bits, qp[0], qp[1], c->dim, v))
+        return FLT_MAX;
+    for (int d = 0; d < c->dim; d++) {
+        float diff = c->query[d] - v[d];
+        dist += diff * diff;
+    }
+    return dist;
+}
+
+static float ref_sqmd_correction_fn(int id, int source, float coarse, void *ctx) {
+    (void)source;
+    (void)coarse;
+    return ref_sqmd_exact(ctx, id);
+}
+
+/* retrieval_sqmd_make_seed_ids + seed distances + the fake-DSP traversal,
+ * as sqmd_search_one_query_fake_dsp (retrieval_rpc_demo_pipeline.c:
+ * 1060-1125) composes them.  correction 0: SQ code distances throughout;
+ * 1: exact SQ + residual distances for seeds and results. */
+int ref_sqmd_search(const int32_t *graph, int degree, int n, const uint8_t *codes,
+                    int dim, int bits, float scale, float zero, const uint8_t *res,
+                    const float *res_qp, int res_bits, int correction,
+                    const float *query, int entry_points, int graph_entry,
+                    unsigned int seed, int expand_count, int pool_capacity,
+                    int lanes, int32_t *out_ids, float *out_dists, int out_cap,
+                    int32_t *out_seed_ids) {
+    RefSqmdCtx c;
+    uint8_t qsq[8192];
+    int seed_ids[2048];
+    float seed_dists[2048];
+    int count, rc;
+    if (entry_points > 2048 || dim > 8192) return -100;
+    memset(&c, 0, sizeof(c));
+    c.codes = codes;
+    c.dim = dim;
+    c.bits = bits;
+    c.cb = bits == 4 ? (dim + 1) / 2 : dim;
+    c.scale = scale;
+    c.zero = zero;
+    c.res = res;
+    c.res_qp = res_qp;
+    c.res_bits = res_bits;
+    c.res_cb = res_bits == 4 ? (dim + 1) / 2 : dim;
+    c.query = query;
+    c.qsq = qsq;
+    if (ref_quantize_query(query, dim, bits, scale, zero, qsq)) return -101;
+    count = retrieval_sqmd_make_seed_ids(n, entry_points, graph_entry, seed, seed_ids,
+                                         entry_points);
+    if (count <= 0) return -102;
+    for (int i = 0; i < count; i++) {
+        seed_dists[i] = correction ? ref_sqmd_exact(&c, seed_ids[i])
+                                   : ref_sqmd_coarse(&c, seed_ids[i]);
+        if (out_seed_ids) out_seed_ids[i] = seed_ids[i];
+    }
+    rc = sqmd_cpu_state_run_fake_dsp(graph, degree, n, pool_capacity, entry_points,
+                                     expand_count, seed_ids, seed_dists, count, lanes,
+                                     ref_sqmd_distance_fn, &c,
+                                     correction ? ref_sqmd_correction_fn : NULL, &c,
+                                     out_ids, out_dists, out_cap);
     return rc;
 }

@@ -708,6 +708,249 @@ out:
     return rc;
 }
 
+/* ---- the live SQ traversal (retrieval_sqmd_common.c) ---- */
+
+typedef struct {
+    int id;
+    float distance;
+} OrcDc;
+
+/* :9-47 -- pool: max-heap on distance alone */
+static void orc_sq_pool_up(OrcDc *p, int i) {
+    OrcDc t = p[i];
+    while (i > 0) {
+        int parent = (i - 1) / 2;
+        if (t.distance <= p[parent].distance) break;
+        p[i] = p[parent];
+        i = parent;
+    }
+    p[i] = t;
+}
+
+static void orc_sq_pool_down(OrcDc *p, int size, int i) {
+    OrcDc t = p[i];
+    for (;;) {
+        int child = 2 * i + 1;
+        if (child >= size) break;
+        if (child + 1 < size && p[child + 1].distance > p[child].distance) child++;
+        if (t.distance >= p[child].distance) break;
+        p[i] = p[child];
+        i = child;
+    }
+    p[i] = t;
+}
+
+/* :51-79 -- admit below capacity, else only strictly below the worst */
+static int orc_sq_pool_insert(OrcDc *p, int *size, int cap, int id, float d) {
+    if (*size < cap) {
+        p[*size].id = id;
+        p[*size].distance = d;
+        orc_sq_pool_up(p, *size);
+        (*size)++;
+        return 1;
+    }
+    if (d < p[0].distance) {
+        p[0].id = id;
+        p[0].distance = d;
+        orc_sq_pool_down(p, *size, 0);
+        return 1;
+    }
+    return 0;
+}
+
+/* :81-147 -- frontier: min-heap on distance alone, bounded push */
+static void orc_sq_front_up(OrcDc *f, int i) {
+    OrcDc t = f[i];
+    while (i > 0) {
+        int parent = (i - 1) / 2;
+        if (t.distance >= f[parent].distance) break;
+        f[i] = f[parent];
+        i = parent;
+    }
+    f[i] = t;
+}
+
+static void orc_sq_front_down(OrcDc *f, int size, int i) {
+    OrcDc t = f[i];
+    for (;;) {
+        int child = 2 * i + 1;
+        if (child >= size) break;
+        if (child + 1 < size && f[child + 1].distance < f[child].distance) child++;
+        if (t.distance <= f[child].distance) break;
+        f[i] = f[child];
+        i = child;
+    }
+    f[i] = t;
+}
+
+static void orc_sq_front_push(OrcDc *f, int *size, int cap, int id, float d) {
+    if (*size >= cap) return;
+    f[*size].id = id;
+    f[*size].distance = d;
+    orc_sq_front_up(f, *size);
+    (*size)++;
+}
+
+static int orc_sq_front_pop(OrcDc *f, int *size) {
+    int top = f[0].id;
+    (*size)--;
+    if (*size > 0) {
+        f[0] = f[*size];
+        orc_sq_front_down(f, *size, 0);
+    }
+    return top;
+}
+
+float orc_sq_residual_distance(const OrcIndex *ix, const float *query, int id) {
+    const int cb = ix->sq_bits == 4 ? (ix->dim + 1) / 2 : ix->dim;
+    const uint8_t *l1 = ix->sq_codes + (size_t)id * cb;
+    const uint8_t *res = NULL;
+    const float *qp = NULL;
+    float dist = 0.0f;
+    if (ix->res_codes && ix->res_qparams) {
+        res = ix->res_codes + (size_t)id * orc_residual_code_bytes(ix->dim, ix->res_bits);
+        qp = ix->res_qparams + 2 * (size_t)id;
+    }
+    for (int i = 0; i < ix->dim; i++) {
+        int c = ix->sq_bits == 4 ? ((i & 1) ? (l1[i >> 1] >> 4) : (l1[i >> 1] & 15)) : l1[i];
+        volatile float prod = (float)c * ix->sq_scale;
+        volatile float value = prod + ix->sq_zero;
+        if (res) {
+            volatile float rp = (float)orc_res_code(res, ix->res_bits, i) * qp[0];
+            volatile float rl = rp + qp[1];
+            value = value + rl;
+        }
+        volatile float diff = query[i] - value;
+        volatile float sq = diff * diff;
+        dist = dist + sq;
+    }
+    return dist;
+}
+
+int orc_sqmd_seed_ids(int n, int entry_points, int graph_entry, uint32_t seed,
+                      int *out, int capacity) {
+    uint64_t limit, attempts = 0;
+    int target, count = 0;
+    if (n <= 0 || entry_points <= 0 || !out || capacity <= 0) return -1;
+    target = entry_points < capacity ? entry_points : capacity;
+    if (target > n) target = n;
+    if (graph_entry >= 0 && graph_entry < n) out[count++] = graph_entry;
+    limit = (uint64_t)target * 64u + 1024u;
+    while (count < target && attempts < limit) {
+        int id, dup = 0;
+        seed = seed * 1103515245u + 12345u;
+        id = (int)((seed / 65536u) % (unsigned)n);
+        attempts++;
+        for (int i = 0; i < count && !dup; i++) dup = out[i] == id;
+        if (!dup) out[count++] = id;
+    }
+    for (int id = 0; id < n && count < target; id++) {
+        int dup = 0;
+        for (int i = 0; i < count && !dup; i++) dup = out[i] == id;
+        if (!dup) out[count++] = id;
+    }
+    return count;
+}
+
+/* One query of the fake-DSP turn loop (:2052-2152): seeds (:446-476), the
+ * source pop that discards frontier tops worse than a full pool's worst
+ * (:478-523), the turn's work list in neighbor order (invalid ids and the
+ * source skipped, duplicates and visited nodes skipped, then all marked
+ * visited; :283-383, :720-773), results applied in that order through the
+ * correction (:638-708), and the final heap sort (:40-49). */
+int orc_sqmd_search(const OrcIndex *ix, int correction, const float *query,
+                    const int *seed_ids, int seed_count, int entry_points,
+                    int expand_count, int pool_capacity, int *out_ids,
+                    float *out_distances, int out_capacity, OrcSqmdStats *stats) {
+    OrcSqmdStats st;
+    OrcDc *pool = NULL, *front = NULL;
+    uint32_t *visited = NULL;
+    uint8_t *qsq = NULL;
+    int psize = 0, fsize = 0, rc = -1, fcap, cb;
+    memset(&st, 0, sizeof(st));
+    if (!ix || !query || !seed_ids || seed_count <= 0 || entry_points <= 0 ||
+        expand_count <= 0 || pool_capacity <= 0 || !out_ids || !out_distances ||
+        out_capacity <= 0 || !ix->sq_codes)
+        return -1;
+    fcap = entry_points + pool_capacity * expand_count;
+    cb = ix->sq_bits == 4 ? (ix->dim + 1) / 2 : ix->dim;
+    pool = malloc((size_t)pool_capacity * sizeof(*pool));
+    front = malloc((size_t)fcap * sizeof(*front));
+    visited = calloc(((size_t)ix->num_vectors + 31) / 32, sizeof(uint32_t));
+    qsq = malloc((size_t)ix->dim);
+    if (!pool || !front || !visited || !qsq) goto out;
+    orc_sq_quantize_query(query, ix->dim, ix->sq_bits, ix->sq_scale, ix->sq_zero, qsq);
+#define ORC_DIST(id)                                                             \
+    (correction ? orc_sq_residual_distance(ix, query, (id))                      \
+                : orc_sq_distance(qsq, ix->sq_codes + (size_t)(id) * cb, ix->dim, \
+                                  ix->sq_bits, ix->sq_scale))
+#define ORC_VIS(id) (visited[(id) >> 5] & (1u << ((id) & 31)))
+    for (int i = 0; i < seed_count; i++) {
+        int id = seed_ids[i];
+        float d;
+        if (id < 0 || id >= ix->num_vectors) goto out;
+        if (ORC_VIS(id)) continue;
+        visited[id >> 5] |= 1u << (id & 31);
+        d = ORC_DIST(id);
+        if (orc_sq_pool_insert(pool, &psize, pool_capacity, id, d))
+            orc_sq_front_push(front, &fsize, fcap, id, d);
+    }
+    for (;;) {
+        int work[32], wc = 0, src, ec;
+        const int32_t *row;
+        while (fsize > 0 && psize >= pool_capacity && front[0].distance > pool[0].distance)
+            orc_sq_front_pop(front, &fsize);
+        if (fsize <= 0) break;
+        src = orc_sq_front_pop(front, &fsize);
+        row = ix->graph + (size_t)src * ix->degree;
+        ec = expand_count < ix->degree ? expand_count : ix->degree;
+        if (ec > 32) goto out; /* SQMD_STATUS_TOO_MUCH_WORK */
+        for (int j = 0; j < ec; j++) {
+            int nb = row[j], dup = 0;
+            if (nb < 0 || nb >= ix->num_vectors || nb == src) {
+                st.invalid_skips++;
+                continue;
+            }
+            for (int i = 0; i < wc && !dup; i++) dup = work[i] == nb;
+            if (dup || ORC_VIS(nb)) {
+                st.dedup_skips++;
+                continue;
+            }
+            work[wc++] = nb;
+        }
+        st.turns++;
+        st.work_items += (uint64_t)wc;
+        if (wc == 0) st.empty_turns++;
+        for (int i = 0; i < wc; i++) visited[work[i] >> 5] |= 1u << (work[i] & 31);
+        for (int i = 0; i < wc; i++) {
+            float d = ORC_DIST(work[i]);
+            if (orc_sq_pool_insert(pool, &psize, pool_capacity, work[i], d))
+                orc_sq_front_push(front, &fsize, fcap, work[i], d);
+            st.applied++;
+        }
+    }
+#undef ORC_DIST
+#undef ORC_VIS
+    for (int i = psize - 1; i > 0; i--) {
+        OrcDc t = pool[0];
+        pool[0] = pool[i];
+        pool[i] = t;
+        orc_sq_pool_down(pool, i, 0);
+    }
+    rc = psize < out_capacity ? psize : out_capacity;
+    for (int i = 0; i < rc; i++) {
+        out_ids[i] = pool[i].id;
+        out_distances[i] = pool[i].distance;
+    }
+out:
+    if (stats) *stats = st;
+    free(pool);
+    free(front);
+    free(visited);
+    free(qsq);
+    return rc;
+}
+
 /* retrieval_rpc_demo_support.c:1709-1728 (demo_recall_hits_at_k) */
 int orc_recall_hits(const int *res, int res_count, const int32_t *gt,
                     int gt_k, int k) {

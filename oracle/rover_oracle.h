@@ -98,6 +98,28 @@ int orc_refine(const OrcIndex *index, const float *query, const int32_t *ids,
                int count, int output_k, const int32_t *graph_to_original,
                int32_t *out_ids, float *out_distances);
 
+/* SQ traversal of the reference's live SQ path (retrieval_sqmd_common.c:
+ * sqmd_cpu_state_* with the fake-DSP turn driver, :2052-2152): distance-only
+ * heaps, results applied in work order.  correction = 0 ranks by the SQ code
+ * distance, 1 by the exact SQ + residual distance (res_* of the index over
+ * sq_codes).  Returns the output count, or -1 (bad seed id, expand > 32). */
+typedef struct OrcSqmdStats {
+    uint64_t work_items, dedup_skips, invalid_skips, applied, turns, empty_turns;
+} OrcSqmdStats;
+
+int orc_sqmd_search(const OrcIndex *index, int correction, const float *query,
+                    const int *seed_ids, int seed_count, int entry_points,
+                    int expand_count, int pool_capacity, int *out_ids,
+                    float *out_distances, int out_capacity, OrcSqmdStats *stats);
+
+/* retrieval_sqmd_seed.c:15-63 (retrieval_sqmd_make_seed_ids). */
+int orc_sqmd_seed_ids(int num_vectors, int entry_points, int graph_entry,
+                      uint32_t seed, int *out_ids, int capacity);
+
+/* retrieval_rpc_demo_support.c:1397-1443
+ * (retrieval_demo_exact_distance_residual_scalar). */
+float orc_sq_residual_distance(const OrcIndex *index, const float *query, int id);
+
 int orc_recall_hits(const int *result_ids, int result_count,
                     const int32_t *gt_row, int gt_k, int k);
 uint64_t orc_fingerprint(uint64_t hash, const int *ids,

@@ -50,6 +50,18 @@ typedef struct RoverCtx {
     uint8_t *d_sq_codes;
     int sq_n, sq_dim, sq_degree, sq_entry, sq_bits;
     float sq_scale, sq_zero;
+    uint8_t *d_sq_res;         /* SQ residual codes and qparams          */
+    float *d_sq_res_qp;
+    int sq_res_bits;
+    /* SQ traversal scratch: exact visited bitmaps and HBM frontiers per
+       wave, and the cached live-path seed list */
+    uint32_t *d_sqmd_vis;
+    size_t sqmd_vis_cap;
+    void *d_sqmd_front;
+    size_t sqmd_front_cap;
+    int32_t *d_sqmd_seeds;
+    int sqmd_seeds_count, sqmd_seeds_n, sqmd_seeds_ep, sqmd_seeds_entry;
+    uint32_t sqmd_seeds_seed;
     /* launch workspace */
     int32_t *d_queue;
     int32_t *d_entries;

@@ -120,6 +120,54 @@ typedef struct RvBatch {
     int trace_cap;
 } RvBatch;
 
+/* The reference's SQ traversal with distance-only heaps: the live SQ path
+ * (retrieval_sqmd_common.c, fake-DSP turn loop) and the DSP search op
+ * (retrieval_async_dsp_search.c:368-509).  One wavefront per query. */
+typedef struct RvSqmdCand {
+    int32_t id;
+    float d;
+} RvSqmdCand;
+
+typedef struct RvSqmdBatch {
+    const float *queries;        /* [nq][dim]                               */
+    int nq;
+    const int32_t *pool_sizes;   /* [nq] per-query L, or NULL -> pool_size  */
+    const int32_t *num_outs;     /* [nq] per-query k, or NULL -> num_out    */
+    int pool_size, num_out, max_pool;
+    const int32_t *seeds;        /* shared list, or [nq][seed_stride] rows  */
+    int n_seeds, seed_stride;
+    int fcap_entries;            /* frontier capacity = fcap_entries +
+                                    L * fcap_expand (the reference sizes it
+                                    from its entry_points / expand)         */
+    int fcap_expand;
+    int expand_count;            /* neighbors read per expansion            */
+    int max_work;                /* > this many neighbors per turn: the
+                                    query fails (the live path's limit 32)  */
+    int correction;              /* 0: SQ code distance; 1: SQ + residual   */
+    int32_t *out_ids;            /* [nq][out_stride]                        */
+    float *out_dists;
+    int32_t *out_counts;         /* count or RV_QUERY_* status              */
+    int out_stride;
+    uint64_t *out_stats;         /* [nq][4]: work items scored, invalid
+                                    neighbor skips, expansions, visited /
+                                    duplicate skips; may be NULL            */
+    int32_t *queue;
+    const int32_t *nq_dev;
+    uint32_t *visited;           /* [grid][visited_words] exact bitmaps     */
+    int visited_words;           /* multiple of 4                           */
+    RvSqmdCand *frontier;        /* [grid][frontier_phys] in HBM, or NULL:
+                                    the frontier lives in LDS              */
+    int frontier_phys;           /* entries one wave can hold               */
+} RvSqmdBatch;
+
+/* LDS bytes per wave for the SQ traversal with the frontier in LDS (0 when
+ * it does not fit), the waves per CU, and the launch.  grid_waves <= 0
+ * picks the occupancy-derived count; the batch's visited / frontier
+ * scratch must cover that many waves (rv_sqmd_grid gives it). */
+int rv_sqmd_grid(const RvIndex *index, const RvSqmdBatch *batch, int *lds_frontier);
+int rv_sqmd_launch(const RvIndex *index, const RvSqmdBatch *batch, int grid_waves,
+                   void *stream);
+
 int rv_device_count(void);
 int rv_set_device(int device);
 int rv_malloc(void **d_ptr, size_t bytes);

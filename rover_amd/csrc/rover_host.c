@@ -119,8 +119,16 @@ AEEResult calculator_retrieval_unload(remote_handle64 h) {
     if (!c) return ROVER_EBADPARM;
     dfree(c->d_sq_graph);
     dfree(c->d_sq_codes);
+    dfree(c->d_sq_res);
+    dfree(c->d_sq_res_qp);
+    dfree(c->d_sqmd_seeds);
     c->d_sq_graph = NULL;
     c->d_sq_codes = NULL;
+    c->d_sq_res = NULL;
+    c->d_sq_res_qp = NULL;
+    c->d_sqmd_seeds = NULL;
+    c->sqmd_seeds_count = 0;
+    c->sq_res_bits = 0;
     c->sq_loaded = 0;
     return ROVER_SUCCESS;
 }
@@ -136,6 +144,8 @@ int calculator_close(remote_handle64 h) {
     dfree(c->d_stage);
     dfree(c->d_route_dist);
     dfree(c->d_route_ent);
+    dfree(c->d_sqmd_vis);
+    dfree(c->d_sqmd_front);
     c->magic = 0;
     pthread_mutex_destroy(&c->lock);
     free(c);
@@ -306,6 +316,36 @@ AEEResult rover_pure_pq_original_ids(remote_handle64 h, const int32_t *map,
         maps_digest(NULL, 0, c->orig_maps_sha);
         return ROVER_ENOMEMORY;
     }
+    return ROVER_SUCCESS;
+}
+
+AEEResult rover_sq_residual_load(remote_handle64 h, const uint8_t *codes,
+                                 int64_t codesLen, const float *qparams,
+                                 int64_t qparamsLen, int32_t bits) {
+    RoverCtx *c = ctx_of(h);
+    int64_t rb;
+    if (!c) return ROVER_EBADPARM;
+    if (!c->sq_loaded || !codes || !qparams || (bits != 4 && bits != 8))
+        return ROVER_EBADPARM;
+    rb = bits == 4 ? (c->sq_dim + 1) / 2 : c->sq_dim;
+    if (codesLen != rb * c->sq_n || qparamsLen != 2 * (int64_t)c->sq_n)
+        return ROVER_EBADPARM;
+    pthread_mutex_lock(&c->lock);
+    dfree(c->d_sq_res);
+    dfree(c->d_sq_res_qp);
+    c->d_sq_res = NULL;
+    c->d_sq_res_qp = NULL;
+    c->sq_res_bits = 0;
+    if (dalloc_copy((void **)&c->d_sq_res, codes, (size_t)codesLen) ||
+        dalloc_copy((void **)&c->d_sq_res_qp, qparams, (size_t)qparamsLen * sizeof(float))) {
+        dfree(c->d_sq_res);
+        c->d_sq_res = NULL;
+        pthread_mutex_unlock(&c->lock);
+        set_err("sq_residual_load");
+        return ROVER_ENOMEMORY;
+    }
+    c->sq_res_bits = bits;
+    pthread_mutex_unlock(&c->lock);
     return ROVER_SUCCESS;
 }
 
@@ -560,7 +600,11 @@ AEEResult rover_prepare_launch(RoverCtx *c, const RoverSearchBatch *b,
         (b->method == ROVER_METHOD_SQ && (b->correction || b->refine_candidates)) ||
         (b->refine_mode != 0 && b->refine_mode != ROVER_REFINE_CURRENT &&
          b->refine_mode != ROVER_REFINE_ERROR4) ||
-        (b->refine_mode == ROVER_REFINE_ERROR4 && b->refine_candidates <= 0))
+        (b->refine_mode == ROVER_REFINE_ERROR4 && b->refine_candidates <= 0) ||
+        /* the SQ traversals run through launch_sqmd, never here */
+        (b->method == ROVER_METHOD_SQ && b->traversal != ROVER_TRAVERSAL_BEST_FIRST) ||
+        (b->method == ROVER_METHOD_PQ && b->traversal != ROVER_TRAVERSAL_DEFAULT &&
+         b->traversal != ROVER_TRAVERSAL_BEST_FIRST))
         return ROVER_EBADPARM;
     if (b->refine_mode == ROVER_REFINE_ERROR4) {
         if (!c->e4_loaded) {
@@ -702,6 +746,164 @@ AEEResult rover_pure_pq_router_load(remote_handle64 h, const float *centroids,
     return ROVER_SUCCESS;
 }
 
+int rover_sqmd_seed_ids(int n, int entry_points, int graph_entry, uint32_t seed,
+                        int32_t *out, int capacity) {
+    uint64_t limit, attempts = 0;
+    int target, count = 0;
+    if (n <= 0 || entry_points <= 0 || !out || capacity <= 0) return -1;
+    target = entry_points < capacity ? entry_points : capacity;
+    if (target > n) target = n;
+    if (graph_entry >= 0 && graph_entry < n) out[count++] = graph_entry;
+    limit = (uint64_t)target * 64u + 1024u;
+    while (count < target && attempts < limit) {
+        int id, dup = 0;
+        seed = seed * 1103515245u + 12345u;
+        id = (int)((seed / 65536u) % (uint32_t)n);
+        attempts++;
+        for (int i = 0; i < count && !dup; i++) dup = out[i] == id;
+        if (!dup) out[count++] = id;
+    }
+    for (int id = 0; id < n && count < target; id++) {
+        int dup = 0;
+        for (int i = 0; i < count && !dup; i++) dup = out[i] == id;
+        if (!dup) out[count++] = id;
+    }
+    return count;
+}
+
+/* Uploads the live SQ path's seed list once per (n, entry_points, entry,
+ * seed). */
+static int ensure_sqmd_seeds(RoverCtx *c, int n, int entry_points, int graph_entry,
+                             uint32_t seed) {
+    int32_t *ids;
+    int count;
+    if (c->d_sqmd_seeds && c->sqmd_seeds_n == n && c->sqmd_seeds_ep == entry_points &&
+        c->sqmd_seeds_entry == graph_entry && c->sqmd_seeds_seed == seed)
+        return 0;
+    dfree(c->d_sqmd_seeds);
+    c->d_sqmd_seeds = NULL;
+    ids = malloc((size_t)entry_points * sizeof(int32_t));
+    if (!ids) return -1;
+    count = rover_sqmd_seed_ids(n, entry_points, graph_entry, seed, ids, entry_points);
+    if (count <= 0 || dalloc_copy((void **)&c->d_sqmd_seeds, ids, (size_t)count * 4)) {
+        free(ids);
+        set_err("sqmd seeds");
+        return -1;
+    }
+    free(ids);
+    c->sqmd_seeds_count = count;
+    c->sqmd_seeds_n = n;
+    c->sqmd_seeds_ep = entry_points;
+    c->sqmd_seeds_entry = graph_entry;
+    c->sqmd_seeds_seed = seed;
+    return 0;
+}
+
+/* The SQ traversals with distance-only heaps (ROVER_TRAVERSAL_SQMD and
+ * ROVER_TRAVERSAL_SQ_DSP). */
+static AEEResult launch_sqmd(RoverCtx *c, const RoverSearchBatch *b, int max_pool,
+                             int traversal, void *stream) {
+    RvIndex ix;
+    RvSqmdBatch sb;
+    uint32_t seed = b->seed ? b->seed : ROVER_ENTRY_POINT_SEED;
+    const int dsp = traversal == ROVER_TRAVERSAL_SQ_DSP;
+    int grid, front_lds = 0, words;
+    if (b->num_queries < 0 || (b->num_queries > 0 && !b->queries) || b->expand_count <= 0 ||
+        max_pool <= 0 || max_pool > ROVER_MAX_POOL || b->out_stride <= 0 || !b->out_ids ||
+        !b->out_dists || !b->out_counts || b->refine_candidates || b->learned_stop ||
+        (b->correction != 0 && b->correction != 1) || (dsp && b->correction) ||
+        b->entry_points <= 0)
+        return ROVER_EBADPARM;
+    /* Caller-given seed lists and expansions of more than 32 neighbors are
+     * switched off for the SQ traversals: a GPU run that used them (with an
+     * out-of-range seed and a degree-48 graph) stopped responding, and no
+     * cause has been shown yet. */
+    if (b->entry_ids || b->route_entries) {
+        snprintf(g_err, sizeof(g_err),
+                 "search: the SQ traversals take their own seeds (entry_points)");
+        return ROVER_EBADPARM;
+    }
+    if (fill_index(c, ROVER_METHOD_SQ, 0, 0, &ix)) {
+        snprintf(g_err, sizeof(g_err), "search: SQ index not loaded");
+        return ROVER_EFAILED;
+    }
+    if ((b->expand_count < ix.degree ? b->expand_count : ix.degree) > 32) {
+        snprintf(g_err, sizeof(g_err),
+                 "search: the SQ traversals expand at most 32 neighbors per step");
+        return ROVER_EBADPARM;
+    }
+    if (b->correction) {
+        if (!c->d_sq_res) {
+            snprintf(g_err, sizeof(g_err), "search: no SQ residual loaded");
+            return ROVER_EFAILED;
+        }
+        ix.res_codes = c->d_sq_res;
+        ix.res_qparams = c->d_sq_res_qp;
+        ix.res_bits = c->sq_res_bits;
+        ix.res_code_bytes = c->sq_res_bits == 4 ? (c->sq_dim + 1) / 2 : c->sq_dim;
+    }
+    memset(&sb, 0, sizeof(sb));
+    if (dsp) {
+        if (ensure_entries(c, ix.n, b->entry_points, ix.graph_entry, seed))
+            return ROVER_EFAILED;
+        sb.seeds = c->d_entries;
+        sb.n_seeds = c->entries_count;
+    } else {
+        if (ensure_sqmd_seeds(c, ix.n, b->entry_points, ix.graph_entry, seed))
+            return ROVER_EFAILED;
+        sb.seeds = c->d_sqmd_seeds;
+        sb.n_seeds = c->sqmd_seeds_count;
+    }
+    sb.queries = b->queries;
+    sb.nq = b->num_queries;
+    sb.pool_sizes = b->pool_sizes;
+    sb.num_outs = b->num_outs;
+    sb.pool_size = b->pool_size;
+    sb.num_out = b->num_out;
+    sb.max_pool = max_pool;
+    /* the frontier capacity is sized from the entry points and the expand
+     * count, as the reference sizes it */
+    sb.fcap_entries = b->entry_points;
+    sb.fcap_expand = dsp && b->expand_count > ix.degree ? ix.degree : b->expand_count;
+    sb.expand_count = b->expand_count;
+    sb.max_work = 32;
+    sb.correction = b->correction;
+    sb.out_ids = b->out_ids;
+    sb.out_dists = b->out_dists;
+    sb.out_counts = b->out_counts;
+    sb.out_stride = b->out_stride;
+    sb.out_stats = b->out_stats;
+    sb.queue = c->d_queue;
+    if (sb.nq == 0) return ROVER_SUCCESS;
+    grid = rv_sqmd_grid(&ix, &sb, &front_lds);
+    if (grid < 0) {
+        set_err("sqmd grid");
+        return ROVER_EFAILED;
+    }
+    if (b->grid_waves > 0 && b->grid_waves < grid) grid = b->grid_waves;
+    words = ((ix.n + 31) / 32 + 3) / 4 * 4;
+    sb.visited_words = words;
+    if (grow((void **)&c->d_sqmd_vis, &c->sqmd_vis_cap, (size_t)grid * words * 4)) {
+        set_err("sqmd visited scratch");
+        return ROVER_ENOMEMORY;
+    }
+    sb.visited = c->d_sqmd_vis;
+    if (!front_lds) {
+        sb.frontier_phys = sb.fcap_entries + max_pool * sb.fcap_expand;
+        if (grow(&c->d_sqmd_front, &c->sqmd_front_cap,
+                 (size_t)grid * sb.frontier_phys * sizeof(RvSqmdCand))) {
+            set_err("sqmd frontier scratch");
+            return ROVER_ENOMEMORY;
+        }
+        sb.frontier = (RvSqmdCand *)c->d_sqmd_front;
+    }
+    if (rv_sqmd_launch(&ix, &sb, grid, stream)) {
+        set_err("sqmd launch");
+        return ROVER_EFAILED;
+    }
+    return ROVER_SUCCESS;
+}
+
 static AEEResult launch_batch(RoverCtx *c, const RoverSearchBatch *b_in,
                               int max_pool, int explicit_entries,
                               void *stream) {
@@ -710,6 +912,15 @@ static AEEResult launch_batch(RoverCtx *c, const RoverSearchBatch *b_in,
     RoverSearchBatch routed;
     const RoverSearchBatch *b = b_in;
     AEEResult rc;
+    if (b_in->method == ROVER_METHOD_SQ && b_in->traversal != ROVER_TRAVERSAL_BEST_FIRST) {
+        if (b_in->traversal != ROVER_TRAVERSAL_DEFAULT &&
+            b_in->traversal != ROVER_TRAVERSAL_SQMD && b_in->traversal != ROVER_TRAVERSAL_SQ_DSP)
+            return ROVER_EBADPARM;
+        return launch_sqmd(c, b_in, max_pool,
+                           b_in->traversal == ROVER_TRAVERSAL_SQ_DSP ? ROVER_TRAVERSAL_SQ_DSP
+                                                                     : ROVER_TRAVERSAL_SQMD,
+                           stream);
+    }
     if (b_in->route_entries > 0) {
         /* per-query entries from the router, written to a session buffer */
         int S = b_in->route_entries;
@@ -1070,7 +1281,8 @@ AEEResult rover_search_batch(remote_handle64 h, const RoverSearchBatch *b) {
     return search_host(c, b, 0);
 }
 
-/* calculator.idl:46-55 on the SQ index: one query through the batch path. */
+/* calculator.idl:46-55 on the SQ index: one query of the DSP search op's
+ * traversal (retrieval_async_dsp_search.c:368-509) through the batch path. */
 AEEResult calculator_retrieval_search(remote_handle64 h, const float *query,
                                       int queryLen, int32_t entry_points,
                                       int32_t expand_count, int32_t pool_size,
@@ -1097,6 +1309,7 @@ AEEResult calculator_retrieval_search(remote_handle64 h, const float *query,
     b.pool_size = pool_size;
     b.num_out = num_out;
     b.method = ROVER_METHOD_SQ;
+    b.traversal = ROVER_TRAVERSAL_SQ_DSP;
     b.entry_points = entry_points;
     b.seed = ROVER_ENTRY_POINT_SEED;
     b.expand_count = expand_count;

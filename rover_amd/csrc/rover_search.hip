@@ -1661,3 +1661,423 @@ extern "C" int rv_stream_destroy(void *stream) {
 extern "C" int rv_stream_synchronize(void *stream) {
     return rv_check(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
 }
+
+// ====================================================================== SQ
+// traversal with distance-only heaps (retrieval_sqmd_common.c: the live SQ
+// path's fake-DSP turn loop, :2052-2152; the DSP search op
+// retrieval_async_dsp_search.c:368-509 is the same loop over its own seed
+// list).  Ties between equal distances are resolved by the heaps' array
+// mechanics, so the heaps are kept exactly: one wavefront per query, the
+// heap updates done by lane 0 in the reference's order, the distances of a
+// turn's work items computed by one lane each.
+//
+//   pool      max-heap on distance (:9-47); admits below capacity, else only
+//             strictly below the worst (:51-79)
+//   frontier  min-heap on distance with a bounded push (:81-147)
+//   source    tops worse than a full pool's worst are discarded, then the
+//             top is expanded (:478-523)
+//   turn      neighbors in row order: out-of-range ids and the source are
+//             skipped, duplicates within the turn and visited nodes are
+//             skipped, the rest marked visited (:283-383); results applied
+//             in that order (:638-708)
+//   output    heap sort of the pool (:40-49), first k
+
+__device__ __forceinline__ void rv_hp_up(RvSqmdCand *p, int i) {
+    const RvSqmdCand t = p[i];
+    while (i > 0) {
+        const int parent = (i - 1) >> 1;
+        const RvSqmdCand pp = p[parent];
+        if (t.d <= pp.d) break;
+        p[i] = pp;
+        i = parent;
+    }
+    p[i] = t;
+}
+
+__device__ __forceinline__ void rv_hp_down(RvSqmdCand *p, int size, int i) {
+    const RvSqmdCand t = p[i];
+    for (;;) {
+        int child = 2 * i + 1;
+        if (child >= size) break;
+        RvSqmdCand c = p[child];
+        if (child + 1 < size) {
+            const RvSqmdCand c2 = p[child + 1];
+            if (c2.d > c.d) c = c2, child++;
+        }
+        if (t.d >= c.d) break;
+        p[i] = c;
+        i = child;
+    }
+    p[i] = t;
+}
+
+__device__ __forceinline__ void rv_hf_up(RvSqmdCand *f, int i) {
+    const RvSqmdCand t = f[i];
+    while (i > 0) {
+        const int parent = (i - 1) >> 1;
+        const RvSqmdCand pp = f[parent];
+        if (t.d >= pp.d) break;
+        f[i] = pp;
+        i = parent;
+    }
+    f[i] = t;
+}
+
+__device__ __forceinline__ void rv_hf_down(RvSqmdCand *f, int size, int i) {
+    const RvSqmdCand t = f[i];
+    for (;;) {
+        int child = 2 * i + 1;
+        if (child >= size) break;
+        RvSqmdCand c = f[child];
+        if (child + 1 < size) {
+            const RvSqmdCand c2 = f[child + 1];
+            if (c2.d < c.d) c = c2, child++;
+        }
+        if (t.d <= c.d) break;
+        f[i] = c;
+        i = child;
+    }
+    f[i] = t;
+}
+
+// Applies one scored candidate: pool insert, frontier push when admitted.
+__device__ __forceinline__ void rv_sqmd_apply(RvSqmdCand *pool, int &psize, int L,
+                                              RvSqmdCand *front, int &fsize, int fcap,
+                                              int id, float d) {
+    bool admitted = false;
+    if (psize < L) {
+        pool[psize] = RvSqmdCand{id, d};
+        rv_hp_up(pool, psize);
+        psize++;
+        admitted = true;
+    } else if (d < pool[0].d) {
+        pool[0] = RvSqmdCand{id, d};
+        rv_hp_down(pool, psize, 0);
+        admitted = true;
+    }
+    if (admitted && fsize < fcap) {
+        front[fsize] = RvSqmdCand{id, d};
+        rv_hf_up(front, fsize);
+        fsize++;
+    }
+}
+
+// Exact SQ + residual distance (retrieval_rpc_demo_support.c:1397-1443):
+// value = ((code * scale) + zero) + ((res * rscale) + rzero), sequential
+// f32 sum of (q - value)^2.
+__device__ __forceinline__ float rv_sq_exact(const RvIndex &ix, const float *qv, int id) {
+    const uint8_t *l1 = ix.sq_codes + (size_t)id * ix.sq_code_bytes;
+    const uint8_t *res = ix.res_codes + (size_t)id * ix.res_code_bytes;
+    const float rs = ix.res_qparams[2 * (size_t)id], rz = ix.res_qparams[2 * (size_t)id + 1];
+    float dist = 0.0f;
+    for (int d = 0; d < ix.dim; d++) {
+        const int c = ix.sq_bits == 4 ? ((d & 1) ? (l1[d >> 1] >> 4) : (l1[d >> 1] & 15))
+                                      : l1[d];
+        float value = (float)c * ix.sq_scale;
+        value = value + ix.sq_zero;
+        float level = (float)rv_res_code(res, ix.res_bits, d) * rs;
+        level = level + rz;
+        value = value + level;
+        const float diff = qv[d] - value;
+        dist = dist + diff * diff;
+    }
+    return dist;
+}
+
+struct RvSqmdLayout {
+    int off_q, off_qsq, off_pool, off_wid, off_wd, off_front, total;
+};
+
+static RvSqmdLayout rv_sqmd_layout(int dim, int max_pool, int correction, int front_lds) {
+    RvSqmdLayout l;
+    int off = 0;
+    l.off_q = off;
+    if (correction) off += rv_align(dim * 4, 16);
+    l.off_qsq = off;
+    off += rv_align(dim, 16);
+    l.off_pool = off;
+    off += rv_align(max_pool * 8, 16);
+    l.off_wid = off;
+    off += RV_WAVE * 4;
+    l.off_wd = off;
+    off += RV_WAVE * 4;
+    l.off_front = off;
+    off += front_lds;
+    l.total = off;
+    return l;
+}
+
+template <int CORR>
+__global__ __launch_bounds__(RV_WAVE) void rv_sqmd_kernel(RvIndex ix, RvSqmdBatch b,
+                                                          RvSqmdLayout lay) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = rv_lane();
+    float *qv = reinterpret_cast<float *>(smem + lay.off_q);
+    uint8_t *qsq = smem + lay.off_qsq;
+    RvSqmdCand *pool = reinterpret_cast<RvSqmdCand *>(smem + lay.off_pool);
+    int32_t *wid = reinterpret_cast<int32_t *>(smem + lay.off_wid);
+    float *wd = reinterpret_cast<float *>(smem + lay.off_wd);
+    RvSqmdCand *front = b.frontier
+                            ? b.frontier + (size_t)blockIdx.x * b.frontier_phys
+                            : reinterpret_cast<RvSqmdCand *>(smem + lay.off_front);
+    uint32_t *vis = b.visited + (size_t)blockIdx.x * b.visited_words;
+    const float scale_sq = ix.sq_scale * ix.sq_scale;
+    const int ec = rv_uniform(b.expand_count < ix.degree ? b.expand_count : ix.degree);
+
+    int nq = b.nq;
+    if (b.nq_dev) {
+        const int live = *b.nq_dev;
+        nq = live < nq ? (live > 0 ? live : 0) : nq;
+    }
+    nq = rv_uniform(nq);
+    for (;;) {
+        int q = 0;
+        if (lane == 0) q = atomicAdd(b.queue, 1);
+        q = rv_uniform(__shfl(q, 0));
+        if (q >= nq) break;
+        const int L = b.pool_sizes ? b.pool_sizes[q] : b.pool_size;
+        const int K = b.num_outs ? b.num_outs[q] : b.num_out;
+        const long fcap_l = (long)b.fcap_entries + (long)L * b.fcap_expand;
+        if (L <= 0 || L > b.max_pool || K <= 0 || K > b.out_stride ||
+            fcap_l > b.frontier_phys || ec > b.max_work || ec > RV_WAVE) {
+            if (lane == 0) b.out_counts[q] = RV_QUERY_BAD_PARAM;
+            continue;
+        }
+        const int fcap = (int)fcap_l;
+
+        // fresh visited bitmap; the clears land before this query's atomics
+        for (int w = 4 * lane; w < b.visited_words; w += 4 * RV_WAVE)
+            *reinterpret_cast<uint4 *>(vis + w) = make_uint4(0u, 0u, 0u, 0u);
+        __threadfence();
+        const float *qg = b.queries + (size_t)q * ix.dim;
+        {
+            const float inv = 1.0f / ix.sq_scale;
+            const int hi = ix.sq_bits == 4 ? 15 : 255;
+            for (int d = lane; d < ix.dim; d += RV_WAVE) {
+                const float x = qg[d];
+                if (CORR) qv[d] = x;
+                int v = (int)roundf((x - ix.sq_zero) * inv);
+                v = v < 0 ? 0 : (v > hi ? hi : v);
+                qsq[d] = (uint8_t)v;
+            }
+        }
+        __syncthreads();
+
+        auto score = [&](int id) -> float {
+            if (CORR) return rv_sq_exact(ix, qv, id);
+            return rv_sq(qsq, ix.sq_codes + (size_t)id * ix.sq_code_bytes, ix.dim,
+                         ix.sq_bits, scale_sq);
+        };
+
+        int psize = 0, fsize = 0;  // meaningful in lane 0
+        int status = 0;
+        uint64_t st_work = 0, st_invalid = 0, st_turns = 0, st_dedup = 0;
+
+        // ---- seeds (:446-476), 64 at a time in list order
+        const int32_t *seeds = b.seeds + (b.seed_stride > 0 ? (size_t)q * b.seed_stride : 0);
+        const int ns = b.seed_stride > 0 ? b.seed_stride : b.n_seeds;
+        for (int base = 0; base < ns; base += RV_WAVE) {
+            const int i = base + lane;
+            const bool active = i < ns;
+            const int id = active ? seeds[i] : -1;
+            const bool bad = active && (id < 0 || id >= ix.n);
+            // the reference stops at the first out-of-range seed
+            const uint64_t badm = __ballot(bad);
+            const int first_bad = badm ? (int)__builtin_ctzll(badm) : RV_WAVE;
+            bool dup = false;
+            for (int t = 0; t < RV_WAVE; t++) {
+                const int other = __shfl(id, t);
+                dup |= t < lane && other == id;
+            }
+            bool fresh = false;
+            if (active && !bad && lane < first_bad && !dup) {
+                const uint32_t bit = 1u << (id & 31);
+                fresh = (atomicOr(vis + (id >> 5), bit) & bit) == 0u;
+            }
+            wid[lane] = id;
+            wd[lane] = fresh ? score(id) : 0.0f;
+            const uint64_t mask = __ballot(fresh);
+            __syncthreads();
+            if (lane == 0) {
+                for (uint64_t m = mask; m; m &= m - 1) {
+                    const int j = (int)__builtin_ctzll(m);
+                    rv_sqmd_apply(pool, psize, L, front, fsize, fcap, wid[j], wd[j]);
+                }
+            }
+            __syncthreads();
+            if (badm) {
+                status = RV_QUERY_BAD_ENTRY;
+                break;
+            }
+        }
+
+        // ---- turns
+        while (status == 0) {
+            int src = -1;
+            if (lane == 0) {
+                while (fsize > 0 && psize >= L && front[0].d > pool[0].d) {
+                    fsize--;
+                    if (fsize > 0) {
+                        front[0] = front[fsize];
+                        rv_hf_down(front, fsize, 0);
+                    }
+                }
+                if (fsize > 0) {
+                    src = front[0].id;
+                    fsize--;
+                    if (fsize > 0) {
+                        front[0] = front[fsize];
+                        rv_hf_down(front, fsize, 0);
+                    }
+                }
+            }
+            src = rv_uniform(__shfl(src, 0));
+            if (src < 0) break;
+            st_turns++;
+            const int nb = lane < ec ? ix.graph[(size_t)src * ix.degree + lane] : -1;
+            const bool in_row = lane < ec;
+            const bool valid = in_row && nb >= 0 && nb < ix.n && nb != src;
+            bool dup = false;
+            for (int t = 0; t < ec; t++) {
+                const int other = __shfl(nb, t);
+                const bool ovalid = __shfl((int)valid, t) != 0;
+                dup |= t < lane && ovalid && other == nb;
+            }
+            bool fresh = false;
+            if (valid && !dup) {
+                const uint32_t bit = 1u << (nb & 31);
+                fresh = (atomicOr(vis + (nb >> 5), bit) & bit) == 0u;
+            }
+            const uint64_t mask = __ballot(fresh);
+            st_invalid += (uint64_t)__popcll(__ballot(in_row && !valid));
+            st_dedup += (uint64_t)__popcll(__ballot(valid && !fresh));
+            st_work += (uint64_t)__popcll(mask);
+            wid[lane] = nb;
+            wd[lane] = fresh ? score(nb) : 0.0f;
+            __syncthreads();
+            if (lane == 0) {
+                for (uint64_t m = mask; m; m &= m - 1) {
+                    const int j = (int)__builtin_ctzll(m);
+                    rv_sqmd_apply(pool, psize, L, front, fsize, fcap, wid[j], wd[j]);
+                }
+            }
+            __syncthreads();
+        }
+
+        // ---- output: heap sort (:40-49), first k
+        if (status != 0) {
+            if (lane == 0) b.out_counts[q] = status;
+        } else {
+            if (lane == 0) {
+                for (int i = psize - 1; i > 0; i--) {
+                    const RvSqmdCand t = pool[0];
+                    pool[0] = pool[i];
+                    pool[i] = t;
+                    rv_hp_down(pool, i, 0);
+                }
+            }
+            psize = rv_uniform(__shfl(psize, 0));
+            __syncthreads();
+            const int cnt = K < psize ? K : psize;
+            int32_t *oid = b.out_ids + (size_t)q * b.out_stride;
+            float *odist = b.out_dists + (size_t)q * b.out_stride;
+            for (int i = lane; i < cnt; i += RV_WAVE) {
+                oid[i] = pool[i].id;
+                odist[i] = pool[i].d;
+            }
+            if (lane == 0) b.out_counts[q] = cnt;
+        }
+        if (b.out_stats && lane == 0) {
+            uint64_t *o = b.out_stats + 4 * (size_t)q;
+            o[0] = st_work;
+            o[1] = st_invalid;
+            o[2] = st_turns;
+            o[3] = st_dedup;
+        }
+        __syncthreads();
+    }
+}
+
+static int rv_sqmd_cus(void) {
+    int dev = 0, cus = 0;
+    if (rv_check(hipGetDevice(&dev), "hipGetDevice") ||
+        rv_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
+                 "hipDeviceGetAttribute"))
+        return -1;
+    return cus;
+}
+
+// The frontier lives in LDS when a wave's whole footprint stays within
+// this many bytes; otherwise it is an HBM slab per wave.
+#define RV_SQMD_LDS_BUDGET (48 * 1024)
+
+static RvSqmdLayout rv_sqmd_layout_for(const RvIndex *ix, const RvSqmdBatch *b,
+                                       int *front_lds) {
+    const long fmax = (long)b->fcap_entries + (long)b->max_pool * b->fcap_expand;
+    RvSqmdLayout l = rv_sqmd_layout(ix->dim, b->max_pool, b->correction, 0);
+    const long with = l.total + fmax * 8;
+    *front_lds = with <= RV_SQMD_LDS_BUDGET;
+    if (*front_lds) l = rv_sqmd_layout(ix->dim, b->max_pool, b->correction, (int)(fmax * 8));
+    return l;
+}
+
+template <int CORR>
+static int rv_sqmd_occupancy(int lds) {
+    static int configured = 0;
+    int blocks = 0;
+    if (!configured) {
+        if (rv_check(hipFuncSetAttribute((const void *)rv_sqmd_kernel<CORR>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         160 * 1024),
+                     "hipFuncSetAttribute"))
+            return -1;
+        configured = 1;
+    }
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rv_sqmd_kernel<CORR>, RV_WAVE,
+                                                     lds) != hipSuccess)
+        return -1;
+    return blocks;
+}
+
+extern "C" int rv_sqmd_grid(const RvIndex *ix, const RvSqmdBatch *b, int *lds_frontier) {
+    int front_lds = 0;
+    const RvSqmdLayout lay = rv_sqmd_layout_for(ix, b, &front_lds);
+    const int per = b->correction ? rv_sqmd_occupancy<1>(lay.total)
+                                  : rv_sqmd_occupancy<0>(lay.total);
+    const int cus = rv_sqmd_cus();
+    if (per < 0 || cus < 0) return -1;
+    if (lds_frontier) *lds_frontier = front_lds;
+    int grid = cus * (per > 0 ? per : 1);
+    if (grid > b->nq) grid = b->nq;
+    return grid < 1 ? 1 : grid;
+}
+
+extern "C" int rv_sqmd_launch(const RvIndex *ix, const RvSqmdBatch *b, int grid_waves,
+                              void *stream) {
+    int front_lds = 0;
+    const RvSqmdLayout lay = rv_sqmd_layout_for(ix, b, &front_lds);
+    if (b->nq == 0) return 0;
+    if (!ix->sq_codes || !ix->graph || !b->visited || b->visited_words % 4 ||
+        b->visited_words < (ix->n + 31) / 32 || (!front_lds && !b->frontier) ||
+        (b->correction && (!ix->res_codes || !ix->res_qparams))) {
+        snprintf(g_rv_error, sizeof(g_rv_error), "rv_sqmd_launch: bad arguments");
+        return -1;
+    }
+    RvSqmdBatch bb = *b;
+    if (front_lds) {
+        bb.frontier = nullptr;
+        bb.frontier_phys = b->fcap_entries + b->max_pool * b->fcap_expand;
+    }
+    if (grid_waves <= 0 && (grid_waves = rv_sqmd_grid(ix, b, nullptr)) < 0) return -1;
+    if (grid_waves > b->nq) grid_waves = b->nq;
+    if (rv_fill_zero(b->queue, sizeof(int32_t), stream)) return -1;
+    hipStream_t s = (hipStream_t)stream;
+    if (b->correction) {
+        if (rv_sqmd_occupancy<1>(lay.total) < 0) return -1;
+        rv_sqmd_kernel<1><<<grid_waves, RV_WAVE, lay.total, s>>>(*ix, bb, lay);
+    } else {
+        if (rv_sqmd_occupancy<0>(lay.total) < 0) return -1;
+        rv_sqmd_kernel<0><<<grid_waves, RV_WAVE, lay.total, s>>>(*ix, bb, lay);
+    }
+    return rv_check(hipGetLastError(), "rv_sqmd_kernel launch");
+}

@@ -186,6 +186,45 @@ def encode_sq(x, bits, scale, zero):
     return out
 
 
+def decode_sq(codes, bits, scale, zero, dim):
+    """code * scale + zero per dimension in float32, two roundings
+    (demo_quantize_l1_vector's dequantized vector)."""
+    if bits == 8:
+        q = codes.astype(np.float32)
+    else:
+        q = np.zeros((codes.shape[0], dim), dtype=np.float32)
+        q[:, 0::2] = codes[:, :(dim + 1) // 2] & 15
+        q[:, 1::2] = codes[:, :dim // 2] >> 4
+    return ((q * np.float32(scale)).astype(np.float32) + np.float32(zero)).astype(np.float32)
+
+
+def encode_sq_residual(x, l1_codes, l1_bits, scale, zero, residual_bits):
+    """Per-vector residual of the SQ reconstruction
+    (retrieval_rpc_demo_support.c:843-886 demo_quantize_residual over
+    r = x - deq, :664-700): rmin / rmax of r, range = max(rmax - rmin, 1e-12),
+    scale = range / levels, zero = rmin, code = clamp(roundf((r - zero) *
+    (1 / scale)), 0, levels).  Returns (codes [n][code bytes], qparams
+    [n][2] float32)."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    n, dim = x.shape
+    levels = 15 if residual_bits == 4 else 255
+    r = (x - decode_sq(l1_codes, l1_bits, scale, zero, dim)).astype(np.float32)
+    rmin = r.min(axis=1)
+    rng = (r.max(axis=1) - rmin).astype(np.float32)
+    rng = np.where(rng < np.float32(1e-12), np.float32(1e-12), rng).astype(np.float32)
+    rs = (rng / np.float32(levels)).astype(np.float32)
+    inv = (np.float32(1.0) / rs).astype(np.float32)
+    v = ((r - rmin[:, None]).astype(np.float32) * inv[:, None]).astype(np.float32)
+    q = _roundf_nonneg_or_clamp(v, levels).astype(np.uint8)
+    if residual_bits == 8:
+        codes = q
+    else:
+        codes = np.zeros((n, (dim + 1) // 2), dtype=np.uint8)
+        codes[:, :(dim + 1) // 2] |= q[:, 0::2]
+        codes[:, :dim // 2] |= (q[:, 1::2] << 4).astype(np.uint8)
+    return np.ascontiguousarray(codes), np.stack([rs, rmin], axis=1).astype(np.float32)
+
+
 # ---------------------------------------------------------------- graph
 
 def _prune_rows(du, dpair, R, valid=None):

@@ -75,10 +75,11 @@ class RoverSearchBatch(C.Structure):
                 ("out_stats", C.c_void_p), ("hash_log2", C.c_int),
                 ("probe_limit", C.c_int), ("grid_waves", C.c_int),
                 ("learned_stop", C.c_int), ("stop_threshold", C.c_float),
-                ("refine_mode", C.c_int)]
+                ("refine_mode", C.c_int), ("traversal", C.c_int)]
 
 
 REFINE_CURRENT, REFINE_ERROR4 = 1, 2
+TRAVERSAL_DEFAULT, TRAVERSAL_BEST_FIRST, TRAVERSAL_SQMD, TRAVERSAL_SQ_DSP = 0, 1, 2, 3
 
 
 class RoverLearnedStopTrain(C.Structure):
@@ -126,6 +127,7 @@ SIGNATURES = [
     ("rover_pure_pq_residual_load", _I,
      [_H, _P, C.c_int64, _P, C.c_int64, C.c_int32]),
     ("rover_pure_pq_original_ids", _I, [_H, _P, _I]),
+    ("rover_sq_residual_load", _I, [_H, _P, C.c_int64, _P, C.c_int64, C.c_int32]),
     ("rover_pure_pq_error4_load", _I, [_H, _P, C.c_int64, _P]),
     ("rover_pure_pq_error4_unload", _I, [_H]),
     ("rover_sha256", None, [_P, C.c_size_t, _P]),
@@ -143,6 +145,7 @@ SIGNATURES = [
     ("rover_search_graph_destroy", _I, [_P]),
     ("rover_autotune", _I, [_H, _P, _P, _I, _P]),
     ("rover_make_seed_ids", _I, [_I, _I, _I, C.c_uint32, _P]),
+    ("rover_sqmd_seed_ids", _I, [_I, _I, _I, C.c_uint32, _P, _I]),
     ("rover_learned_stop_train", _I, [_H, _P]),
     ("rover_learned_stop_set", _I, [_H, _P, _I, _P, _P, _I, _I, _I]),
     ("rover_learned_stop_get", _I, [_H, _P, _I, _P, _P]),

@@ -135,6 +135,13 @@ class Calculator:
             float(zero)), "retrieval_load")
         self.sq = dict(n=graph.shape[0], dim=dim)
 
+    def sq_residual_load(self, res_codes, qparams, bits):
+        """Per-vector SQ residuals for the corrected SQ traversal."""
+        r = np.ascontiguousarray(res_codes, dtype=np.uint8)
+        q = np.ascontiguousarray(qparams, dtype=np.float32)
+        check(native.lib().rover_sq_residual_load(self.h, ptr(r), r.size, ptr(q), q.size,
+                                                  int(bits)), "rover_sq_residual_load")
+
     def retrieval_unload(self):
         check(native.lib().calculator_retrieval_unload(self.h), "retrieval_unload")
         self.sq = None
@@ -158,9 +165,10 @@ class Calculator:
                       out_counts, stride, out_stats, hash_log2, probe_limit,
                       grid_waves, learned_stop, stop_threshold, pool_sizes,
                       num_outs, seed, entry_ids=None, entry_count=0,
-                      entry_stride=0, route_entries=0, refine_mode=0):
+                      entry_stride=0, route_entries=0, refine_mode=0, traversal=0):
         b = native.RoverSearchBatch()
         b.refine_mode = int(refine_mode)
+        b.traversal = int(traversal)
         b.entry_ids = entry_ids
         b.entry_count = int(entry_count)
         b.entry_stride = int(entry_stride)
@@ -194,7 +202,7 @@ class Calculator:
                      expand_count=32, stats=False, hash_log2=0, probe_limit=0,
                      grid_waves=0, learned_stop=0, stop_threshold=0.0,
                      seed=native.ENTRY_POINT_SEED, entry_ids=None,
-                     route_entries=0, refine_mode=0):
+                     route_entries=0, refine_mode=0, traversal=0):
         """Host arrays in and out.  entry_ids: explicit seeds, a list shared
         by all queries or a [nq, S] array of per-query rows (otherwise graph
         entry + LCG seeds); route_entries=S: per-query seeds from the router.  pool_size / num_out: int or per-query
@@ -218,7 +226,7 @@ class Calculator:
                                hash_log2, probe_limit, grid_waves, learned_stop,
                                stop_threshold, ptr(Ls), ptr(Ks), seed, ptr(e),
                                0 if e is None else (stride or len(e)), stride,
-                               route_entries, refine_mode)
+                               route_entries, refine_mode, traversal)
         check(native.lib().rover_search_batch(self.h, C.byref(b)), "rover_search_batch")
         return (ids, d, cnt, st) if stats else (ids, d, cnt)
 
@@ -231,7 +239,7 @@ class Calculator:
                             d_num_outs=None, stream=None,
                             seed=native.ENTRY_POINT_SEED, d_entry_ids=None,
                             entry_count=0, entry_stride=0, route_entries=0,
-                            refine_mode=0):
+                            refine_mode=0, traversal=0):
         """Device pointers in and out; asynchronous.  With per-query device
         arrays, pool_size must be their maximum."""
         b = self._batch_struct(d_queries, nq, pool_size, num_out, method,
@@ -240,7 +248,7 @@ class Calculator:
                                d_stats, hash_log2, probe_limit, grid_waves,
                                learned_stop, stop_threshold, d_pool_sizes,
                                d_num_outs, seed, d_entry_ids, entry_count,
-                               entry_stride, route_entries, refine_mode)
+                               entry_stride, route_entries, refine_mode, traversal)
         check(native.lib().rover_search_batch_device(self.h, C.byref(b), stream),
               "rover_search_batch_device")
 

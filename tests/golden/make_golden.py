@@ -10,7 +10,8 @@ Every expected output in the fixtures comes from the reference's own scalar
 C: retrieval_pure_pq_search_scalar_with_policy, retrieval_pq_build_lut,
 retrieval_pq_adc_ids_batch_scalar, retrieval_pq_train/encode_batch, the
 PQRES01 residual writer/reader and its evaluator/decoder,
-retrieval_final_refine (CURRENT and ERROR4), retrieval_build_error4, the SQ quantize/distance kernels and
+retrieval_final_refine (CURRENT and ERROR4), retrieval_build_error4, the
+live SQ traversal (sqmd_cpu_state_run_fake_dsp, retrieval_sqmd_make_seed_ids), the SQ quantize/distance kernels and
 retrieval_sqmd_make_seed_ids.  Inputs are small synthetic indexes; base
 vectors are multiples of 1/16 so they are stored exactly as uint16.
 """
@@ -231,6 +232,65 @@ def seeds_case(name):
     print("wrote", path, os.path.getsize(path), "bytes")
 
 
+def sqmd_case(name, pq_name="pq_d128_m32", sq_name="sq_d128"):
+    """The reference's live SQ traversal: retrieval_sqmd_make_seed_ids, seed
+    distances and sqmd_cpu_state_run_fake_dsp (distance-only heaps), with SQ
+    code distances (correction 0) or exact SQ + residual distances
+    (correction 1), on the PQ fixture's base, graph and queries and the SQ
+    fixture's codes.  Residual codes come from rover_amd.index's encoder
+    (they are inputs here).  Writes <name>.npz."""
+    ref = pyoracle.reference()
+    g = dict(np.load(os.path.join(HERE, pq_name + ".npz"), allow_pickle=False))
+    s = dict(np.load(os.path.join(HERE, sq_name + ".npz"), allow_pickle=False))
+    base = g["base_x16"].astype(np.float32) / np.float32(16.0)
+    queries = g["queries_x16"].astype(np.float32) / np.float32(16.0)
+    graph, entry = g["graph"], int(g["entry"])
+    n, dim = base.shape
+    out = dict(entry=np.int32(entry))
+    cases = []
+    for bits, res_bits in ((8, 4), (4, 8)):
+        codes = s["sq%d_codes" % bits]
+        scale, zero = np.float32(s["sq%d_scale" % bits]), np.float32(s["sq%d_zero" % bits])
+        res, rqp = index.encode_sq_residual(base, codes, bits, scale, zero, res_bits)
+        out["sq%d_res_codes" % bits] = res
+        out["sq%d_res_qparams" % bits] = rqp
+        out["sq%d_res_bits" % bits] = np.int32(res_bits)
+        for corr in (0, 1):
+            for ep, expand, L in ((1, 32, 16), (32, 32, 16), (32, 32, 64), (8, 12, 40)):
+                key = "sq%d_c%d_e%d_x%d_L%d" % (bits, corr, ep, expand, L)
+                ids_all, d_all, cnt_all = [], [], []
+                for q in queries:
+                    res_out = []
+                    for lanes in (8, 1):
+                        ids = np.zeros(L, dtype=np.int32)
+                        d = np.zeros(L, dtype=np.float32)
+                        seeds = np.zeros(ep, dtype=np.int32)
+                        rc = ref.ref_sqmd_search(_p(graph), graph.shape[1], n, _p(codes), dim,
+                                                 bits, scale, zero, _p(res), _p(rqp), res_bits,
+                                                 corr, _p(q), ep, entry, 42, expand, L, lanes,
+                                                 _p(ids), _p(d), L, _p(seeds))
+                        assert rc > 0, (key, rc)
+                        res_out.append((rc, ids.copy(), d.copy(), seeds.copy()))
+                    # the lane split does not change the result order
+                    assert res_out[0][0] == res_out[1][0]
+                    assert np.array_equal(res_out[0][1], res_out[1][1])
+                    assert np.array_equal(res_out[0][2].view(np.uint32),
+                                          res_out[1][2].view(np.uint32))
+                    rc, ids, d, seeds = res_out[0]
+                    ids_all.append(ids)
+                    d_all.append(d)
+                    cnt_all.append(rc)
+                out[key + "_ids"] = np.stack(ids_all)
+                out[key + "_dists"] = np.stack(d_all)
+                out[key + "_counts"] = np.array(cnt_all, dtype=np.int32)
+                out["seeds_e%d" % ep] = seeds
+                cases.append((bits, corr, ep, expand, L))
+    out["cases"] = np.array(cases, dtype=np.int32)
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
 def error4_case(name):
     """Third-level refine for an existing PQ fixture: the reference's
     retrieval_build_error4 writes the PQERR4V1 artifact (its residual decoder
@@ -296,6 +356,9 @@ def main():
                    check=True)
     if not pyoracle.ref_available():
         sys.exit("reference build missing: oracle/_ref/librover_ref.so")
+    if sys.argv[1:] == ["sqmd"]:
+        sqmd_case("sqmd_d128")
+        return
     if sys.argv[1:] == ["error4"]:
         for name in ("pq_d128_m32", "pq_d96_m48", "pq_d64_ks16"):
             error4_case(name)
@@ -311,6 +374,7 @@ def main():
     seeds_case("seeds")
     for name in ("pq_d128_m32", "pq_d96_m48", "pq_d64_ks16"):
         error4_case(name)
+    sqmd_case("sqmd_d128")
 
 
 if __name__ == "__main__":

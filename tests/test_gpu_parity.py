@@ -108,15 +108,23 @@ def test_sq_search_matches_reference(golden, calc):
                 ids, d, cnt, st = calc.search_batch(g["queries"], L, L,
                                                     method=native.METHOD_SQ,
                                                     entry_points=ep, expand_count=32,
-                                                    stats=True, hash_log2=14)
+                                                    stats=True, hash_log2=14,
+                                                    traversal=native.TRAVERSAL_BEST_FIRST)
                 assert np.array_equal(ids, s[key + "_ids"]), key
                 assert np.array_equal(bits(d), bits(s[key + "_dists"])), key
                 assert np.array_equal(st.astype(np.int64), s[key + "_stats"].astype(np.int64))
-                # the IDL op, one query at a time
-                for qi in range(2):
+                # the IDL op, one query at a time: the DSP search op's
+                # distance-only traversal over its graph entry + LCG seeds
+                # (the oracle's SQ traversal over the same seed list)
+                oix = pyoracle.Index(g["graph"], sq_codes=s["sq%d_codes" % b], sq_bits=b,
+                                     sq_scale=float(s["sq%d_scale" % b]),
+                                     sq_zero=float(s["sq%d_zero" % b]), dim=dim)
+                seeds = pyoracle.seed_ids(len(g["graph"]), ep, int(g["entry"]), 42)
+                for qi in range(3):
                     i1, d1 = calc.retrieval_search(g["queries"][qi], ep, 32, L, 10)
-                    assert np.array_equal(i1, s[key + "_ids"][qi][:10])
-                    assert np.array_equal(bits(d1), bits(s[key + "_dists"][qi][:10]))
+                    oi, od, _ = pyoracle.sqmd_search(oix, g["queries"][qi], seeds, ep, 32, L, 10)
+                    assert np.array_equal(i1, oi)
+                    assert np.array_equal(bits(d1), bits(od))
         calc.retrieval_unload()
 
 

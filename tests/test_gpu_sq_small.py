@@ -1,4 +1,4 @@
-"""Plain SQ search at the small SQ workload's shape: 100,000 x 128 vectors
+"""Plain SQ search (best-first, (distance, id) order) at the small SQ workload's shape: 100,000 x 128 vectors
 from a 256-component Gaussian mixture (sigma 8), SQ8 global-scale codes,
 degree-32 graph (exact 64-NN, RNG prune, backlinks), entry = medoid with the
 reference's 32 LCG seeds, 1,000 queries, L=64, k=10.  Every query's ids,
@@ -38,7 +38,8 @@ def test_sq8_100k_matches_oracle():
         say("case sq8 100k: searching")
         ids, d, cnt, st = calc.search_batch(queries, 64, 10, method=native.METHOD_SQ,
                                             entry_points=32, expand_count=32, stats=True,
-                                            hash_log2=14)
+                                            hash_log2=14,
+                                            traversal=native.TRAVERSAL_BEST_FIRST)
         oix = pyoracle.Index(graph, sq_codes=codes, sq_bits=8, sq_scale=float(scale),
                              sq_zero=float(zero), dim=128)
         seeds = pyoracle.seed_ids(100_000, 32, entry, 42)
@@ -51,10 +52,12 @@ def test_sq8_100k_matches_oracle():
         gt = data.ground_truth(base, queries, k=10)
         say("sq8 100k: recall@10 %.4f, mean expanded %.1f"
             % (data.recall_at_k(ids, cnt, gt, 10), st[:, 2].mean()))
-        # the IDL op for single queries gives the same ids and distances
+        # the IDL op for single queries: the DSP search op's distance-only
+        # traversal over the same seeds
         for qi in (0, 500, 999):
             i1, d1 = calc.retrieval_search(queries[qi], 32, 32, 64, 10)
-            assert np.array_equal(i1, oids[qi][:10])
-            assert np.array_equal(d1.view(np.uint32), od[qi][:10].view(np.uint32))
+            oi, od1, _ = pyoracle.sqmd_search(oix, queries[qi], seeds, 32, 32, 64, 10)
+            assert np.array_equal(i1, oi)
+            assert np.array_equal(d1.view(np.uint32), od1.view(np.uint32))
     finally:
         calc.close()
