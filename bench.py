@@ -142,6 +142,78 @@ def batching_leg(calc, queries, ents, S, mode, refine_of, args, cap=4096, kmax=1
     return out
 
 
+def sq_leg_run(args, n=100_000, nq=1000, L=64, k=10, ep=32):
+    """The small SQ workload: 100k x 128 mixture (256 components, sigma 8),
+    SQ8 codes with 4-bit per-vector residuals, degree-32 graph, entry =
+    medoid, 1k queries, L=64, k=10.  Times the reference's SQ traversal
+    (distance-only heaps; plain and with the exact residual correction) and
+    the best-first SQ search over the same index, HIP events around single
+    launches with queries in HBM; the oracle's SQ traversal on the host for
+    comparison."""
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    base, queries = data.workload(n, 128, 256, 8.0, nq, base_seed=7, query_seed=8)
+    graph = index.build_graph(base, 32, device=torch.device("cuda:0"))
+    torch.cuda.synchronize()
+    entry = index.medoid(base)
+    scale, zero = index.sq_params(base, 8)
+    codes = index.encode_sq(base, 8, scale, zero)
+    res, rqp = index.encode_sq_residual(base, codes, 8, scale, zero, 4)
+    gt = data.ground_truth(base, queries, k=k)
+    calc = Calculator()
+    calc.rover_configure(l1_bits=8)
+    calc.retrieval_load(graph, codes, 128, entry, scale, zero)
+    calc.sq_residual_load(res, rqp, 4)
+    dq = native.DeviceBuffer.from_array(queries)
+    d_ids, d_d = native.DeviceBuffer(nq * k * 4), native.DeviceBuffer(nq * k * 4)
+    d_cnt, d_st = native.DeviceBuffer(nq * 4), native.DeviceBuffer(nq * 32)
+    out = {"workload": "100k x 128 mixture (256 comps, sigma 8), SQ8 + 4-bit residuals, "
+                       "R=32, entry medoid, %d seeds, %d queries, L=%d, k=%d" % (ep, nq, L, k)}
+    oix = pyoracle.Index(graph, res_codes=res, res_qparams=rqp, res_bits=4, sq_codes=codes,
+                         sq_bits=8, sq_scale=float(scale), sq_zero=float(zero), dim=128)
+    seeds = pyoracle.sqmd_seed_ids(n, ep, entry, 42)
+    for name, trav, corr in (("sqmd", native.TRAVERSAL_SQMD, 0),
+                             ("sqmd_corrected", native.TRAVERSAL_SQMD, 1),
+                             ("best_first", native.TRAVERSAL_BEST_FIRST, 0)):
+        def launch():
+            calc.search_batch_device(dq.ptr, nq, L, k, d_ids.ptr, d_d.ptr, d_cnt.ptr, k,
+                                     method=native.METHOD_SQ, correction=corr,
+                                     entry_points=ep, expand_count=32, d_stats=d_st.ptr,
+                                     traversal=trav)
+        launch()
+        native.synchronize()
+        ids = d_ids.download(np.int32, nq * k).reshape(nq, k)
+        cnt = d_cnt.download(np.int32, nq)
+        st = d_st.download(np.uint64, nq * 4).reshape(nq, 4)
+        assert (cnt == k).all(), (name, np.unique(cnt))
+        for _ in range(args.warmup):
+            launch()
+        e0, e1 = native.Event(), native.Event()
+        e0.record()
+        for _ in range(args.steps):
+            launch()
+        e1.record()
+        ms = e0.elapsed_ms(e1) / args.steps
+        row = dict(recall10=round(data.recall_at_k(ids, cnt, gt, k), 4),
+                   qps=round(nq / (ms / 1e3), 1), ms_per_batch=round(ms, 4),
+                   mean_expanded=round(float(st[:, 2].mean()), 2))
+        if trav == native.TRAVERSAL_SQMD:
+            # the oracle's SQ traversal on one host thread, and the identity check
+            t1 = time.time()
+            same = 0
+            for qi in range(nq):
+                oi, od, _ = pyoracle.sqmd_search(oix, queries[qi], seeds, ep, 32, L, k, corr)
+                same += int(np.array_equal(oi, ids[qi]))
+            el = time.time() - t1
+            row.update(cpu_oracle_qps_1thread=round(nq / el, 1),
+                       gpu_ids_identical_frac=same / nq)
+        log("sq leg %s: %s" % (name, row))
+        out[name] = row
+    calc.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,6 +236,9 @@ def main():
     ap.add_argument("--refine", type=int, default=64,
                     help="refine candidates F of the ADC mode")
     ap.add_argument("--modes", default="online,adc")
+    ap.add_argument("--no-sq", action="store_true",
+                    help="skip the SQ leg (100k x 128 SQ8, L=64: the reference's SQ "
+                         "traversal, plain and corrected, and the best-first SQ search)")
     ap.add_argument("--refine-factor", type=int, default=2,
                     help="refine leg: refine the top factor*k of the pool, with and "
                          "without the error4 level (0 skips the leg)")
@@ -518,6 +593,10 @@ def main():
                     break
             refine_leg[name] = dict(sweep=rows, at_target=hit)
 
+    sq_leg = None
+    if not args.no_sq:
+        sq_leg = sq_leg_run(args)
+
     batching = None
     if args.batches > 0:
         batching = batching_leg(calc, queries, ents, S, mode, refine_of, args)
@@ -628,6 +707,7 @@ def main():
                 other_slacks=learned["other_slacks"]),
             "host_buffers_qps": round(host_qps, 1),
             "refine": refine_leg,
+            "sq": sq_leg,
             "launch_tuning": tuning,
             "uneven_batches": batching,
         },
