@@ -109,6 +109,16 @@ def reference():
         L.ref_error4_file.argtypes = [P, P, C.c_long]
         L.ref_error4_file.restype = C.c_long
         L.ref_refine_mode.argtypes = [P, I, I, I, P, P, P, I, I, P]
+        L.ref_pq_artifacts_write.argtypes = [C.c_char_p, P, I, I, I, I, I, I, I, I, C.c_uint,
+                                             P, P, P, P]
+        L.ref_pq_artifacts_load.argtypes = [C.c_char_p, P, P, P, P, P, C.c_size_t, P,
+                                            C.c_size_t]
+        L.ref_residual_prefix.argtypes = [P]
+        L.ref_residual_prefix.restype = C.c_char_p
+        L.ref_pq_residual_read.argtypes = [C.c_char_p, P, I, I, I, I, I, I, P, P, P, P, I,
+                                           P, I, P]
+        L.ref_results_write.argtypes = [C.c_char_p, I, C.c_char_p, I, P, P, P, I]
+        L.ref_nsg_load.argtypes = [C.c_char_p, I, I, P, P]
         L.ref_sqmd_search.argtypes = [P, I, I, P, I, I, F, F, P, P, I, I, P, I, I,
                                       C.c_uint, I, I, I, P, P, I, P]
         _ref = L

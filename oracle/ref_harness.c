@@ -20,6 +20,7 @@
 
 #include "dsp_demo/retrieval_build_artifacts.h"
 #include "dsp_demo/retrieval_final_refine.h"
+#include "dsp_demo/retrieval_nsg_loader.h"
 #include "dsp_demo/retrieval_pq_artifact.h"
 #include "dsp_demo/retrieval_raw_reader.h"
 #include "dsp_demo/retrieval_l1_int8_core.h"
@@ -548,5 +549,138 @@ int ref_sqmd_search(const int32_t *graph, int degree, int n, const uint8_t *code
                                      ref_sqmd_distance_fn, &c,
                                      correction ? ref_sqmd_correction_fn : NULL, &c,
                                      out_ids, out_dists, out_cap);
+    return rc;
+}
+
+/* ---- on-disk formats, written and read by the reference ---- */
+
+static void ref_fill_meta(RetrievalPqArtifactMetadata *m, const uint8_t *uuid, int n,
+                          int dim, int pq_m, int ks, int trainer, int sample,
+                          int train_count, int iters, unsigned int seed,
+                          const uint8_t *dataset_fp, const uint8_t *layout_fp) {
+    memset(m, 0, sizeof(*m));
+    memcpy(m->index_uuid, uuid, 16);
+    m->num_vectors = (uint64_t)n;
+    m->dim = (uint32_t)dim;
+    m->m = (uint32_t)pq_m;
+    m->ks = (uint32_t)ks;
+    m->subdim = (uint32_t)(dim / pq_m);
+    m->code_bytes = (uint32_t)retrieval_pq_code_bytes(pq_m, ks);
+    m->trainer_kind = (uint32_t)trainer;
+    m->sample_strategy = (uint32_t)sample;
+    m->train_count = (uint64_t)train_count;
+    m->kmeans_iters = (uint32_t)iters;
+    m->seed = seed;
+    memcpy(m->dataset_fingerprint, dataset_fp, 32);
+    memcpy(m->layout_fingerprint, layout_fp, 32);
+}
+
+/* PUREPQ01 .pqbook/.pqcodes through retrieval_pq_artifacts_write. */
+int ref_pq_artifacts_write(const char *prefix, const uint8_t *uuid, int n, int dim,
+                           int pq_m, int ks, int trainer, int sample, int train_count,
+                           int iters, unsigned int seed, const uint8_t *dataset_fp,
+                           const uint8_t *layout_fp, const float *codebook,
+                           const uint8_t *codes) {
+    RetrievalPqArtifactMetadata m;
+    ref_fill_meta(&m, uuid, n, dim, pq_m, ks, trainer, sample, train_count, iters, seed,
+                  dataset_fp, layout_fp);
+    return retrieval_pq_artifacts_write(prefix, &m, codebook, codes);
+}
+
+/* retrieval_pq_artifacts_load: returns 0 and copies the payloads out, or the
+ * loader's error. */
+int ref_pq_artifacts_load(const char *prefix, int *out_n, int *out_dim, int *out_m,
+                          int *out_ks, float *codebook, size_t codebook_cap,
+                          uint8_t *codes, size_t codes_cap) {
+    RetrievalPqArtifacts a;
+    int rc;
+    memset(&a, 0, sizeof(a));
+    rc = retrieval_pq_artifacts_load(prefix, &a);
+    if (rc) return rc;
+    *out_n = (int)a.metadata.num_vectors;
+    *out_dim = (int)a.metadata.dim;
+    *out_m = (int)a.metadata.m;
+    *out_ks = (int)a.metadata.ks;
+    if (a.codebook_values <= codebook_cap && a.codes_bytes <= codes_cap) {
+        memcpy(codebook, a.codebook, a.codebook_values * sizeof(float));
+        memcpy(codes, a.codes, a.codes_bytes);
+    } else {
+        rc = -50;
+    }
+    retrieval_pq_artifacts_release(&a);
+    return rc;
+}
+
+/* The prefix under which ref_residual_open's PQRES01 files lie. */
+const char *ref_residual_prefix(void *handle) {
+    RefResidual *r = handle;
+    return r ? r->prefix : NULL;
+}
+
+/* Opens PQRES01 files at `prefix` (written elsewhere) with the reference's
+ * reader over the given PQ metadata / codebook / codes, and decodes `count`
+ * vectors (ids) into out [count][dim].  Returns 0 or the open / decode
+ * error. */
+int ref_pq_residual_read(const char *prefix, const uint8_t *uuid, int n, int dim,
+                         int pq_m, int ks, int train_count, int iters,
+                         const uint8_t *dataset_fp, const uint8_t *layout_fp,
+                         const float *codebook, const uint8_t *codes, int bits,
+                         const int32_t *ids, int count, float *out) {
+    RetrievalPqArtifactMetadata m;
+    RetrievalPqResidualOpenRequest req;
+    RetrievalPqResidualIndex *index = NULL;
+    RetrievalPqResidualQuery *query = NULL;
+    const uint8_t *chunks[1] = {codes};
+    int counts[1] = {n};
+    int rc;
+    ref_fill_meta(&m, uuid, n, dim, pq_m, ks, 0, 0, train_count, iters, 0, dataset_fp,
+                  layout_fp);
+    memset(&req, 0, sizeof(req));
+    req.artifact_prefix = prefix;
+    req.residual_bits = bits;
+    req.cache_slots = 8;
+    req.pq_metadata = &m;
+    req.codebook = codebook;
+    req.code_chunks = chunks;
+    req.chunk_vector_counts = counts;
+    req.chunk_count = 1;
+    req.chunk_vectors = n;
+    rc = retrieval_pq_residual_index_open(&req, &index);
+    if (rc) return -1000 + rc;
+    rc = retrieval_pq_residual_query_create(index, &query);
+    for (int i = 0; !rc && i < count; i++)
+        rc = retrieval_pq_residual_query_decode(query, ids[i], out + (size_t)i * dim);
+    retrieval_pq_residual_query_destroy(query);
+    retrieval_pq_residual_index_close(index);
+    return rc;
+}
+
+/* results.jsonl through rover_results_open / rover_write_result /
+ * rover_results_close (config.c:129-155); order_map may be NULL. */
+int ref_results_write(const char *path, int n, const char *order_map, int nq,
+                      const int32_t *ids, const float *dists, const int32_t *counts,
+                      int stride) {
+    int rc = 0;
+    rover_config.output = path;
+    rover_config.n = n;
+    rover_config.order_map = order_map ? order_map : "";
+    if (rover_results_open()) return -1;
+    for (int q = 0; q < nq && !rc; q++)
+        rc = rover_write_result(q, ids + (size_t)q * stride, dists + (size_t)q * stride,
+                                counts[q]);
+    if (rover_results_close()) rc = -2;
+    return rc;
+}
+
+/* retrieval_nsg_load_fixed: graph [n][width], stats {file width, entry,
+ * truncated nodes, truncated edges}. */
+int ref_nsg_load(const char *path, int n, int width, int32_t *graph, int64_t *stats) {
+    RetrievalNsgLoadStats st;
+    char err[256];
+    int rc = retrieval_nsg_load_fixed(path, n, width, graph, &st, err, sizeof(err));
+    stats[0] = st.file_width;
+    stats[1] = st.entry_point;
+    stats[2] = (int64_t)st.truncated_nodes;
+    stats[3] = (int64_t)st.truncated_edges;
     return rc;
 }

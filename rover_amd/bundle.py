@@ -1,0 +1,657 @@
+"""Rover index bundles and the on-disk formats of their members.
+
+A bundle is a directory with an `index.json` (schema `rover.index.v1`,
+index_bundle.py:17, 187-230) listing its member files with byte counts and
+SHA-256 digests, and the members themselves:
+
+  graph.rover          `<4i>` (0x35314252, n, dim, degree) + int32 [n][degree],
+                       -1 padded (index_bundle.py:43-52, 95-103)
+  PQ   index.pqbook    PUREPQ01, kind 1: f32 [m][ks][subdim]
+       index.pqcodes   PUREPQ01, kind 2: u8 [n][code_bytes]
+                       (256-byte header, retrieval_pq_artifact.c:341-407)
+       index.pqres{4,8}           PQRES01 kind 1; 4096-byte blocks from 4096
+       index.pqres{4,8}.qparams   PQRES01 kind 2; (scale, zero) [n] from 256
+                       (retrieval_pq_residual.c:216-283, 627-690)
+  SQ   l1.bin          32-byte header <IIiiiff4s> + codes
+       qparams.bin     32-byte header <IIiiff8s> + (scale, zero) [n]
+       residual.bin    4096-byte blocks (encode_l1_residual.py:10-18, 111-251)
+  error4.refine        PQERR4V1 / SQERR4V1 (rover_amd.artifacts)
+  order.txt            optional graph row -> original id map
+
+Other graph formats can be imported (`read_graph`: dense, nndescent, nsg) or
+loaded the way the runtime loads NSG files (`load_nsg`,
+retrieval_nsg_loader.c:25-189).  `write_results` writes results.jsonl as
+config.c:129-155 does.  `Bundle.load` puts a bundle into a `Calculator`.
+"""
+import hashlib
+import json
+import os
+import struct
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+SCHEMA = "rover.index.v1"
+ROVER_GRAPH_MAGIC = 0x35314252
+BLOCK = 4096
+
+
+def _sha(path):
+    h = hashlib.sha256()
+    with open(path, "rb") as f:
+        for chunk in iter(lambda: f.read(8 << 20), b""):
+            h.update(chunk)
+    return h.hexdigest()
+
+
+def _record(path, root):
+    path, root = Path(path), Path(root)
+    return dict(path=str(path.relative_to(root)), bytes=path.stat().st_size, sha256=_sha(path))
+
+
+# ---------------------------------------------------------------- graphs
+
+def _check_rows(graph, n):
+    for start in range(0, n, 8192):
+        rows = np.asarray(graph[start:start + 8192])
+        if np.any(rows < -1) or np.any(rows >= n):
+            raise ValueError("Out-of-range graph neighbor")
+        if np.any(rows == np.arange(start, start + len(rows))[:, None]):
+            raise ValueError("Graph contains a self edge")
+        ordered = np.sort(rows, axis=1)
+        if np.any((ordered[:, 1:] >= 0) & (ordered[:, 1:] == ordered[:, :-1])):
+            raise ValueError("Graph contains duplicate edges")
+
+
+def read_graph(path, kind, n, degree, dim):
+    """int32 [n][degree] from a graph file of format `kind` ('rover',
+    'dense', 'nndescent', 'nsg'), with index_bundle.py:43-92's checks:
+    sizes and headers, neighbors in [-1, n), no self edges, no duplicates;
+    an NSG row longer than `degree` is an error here (the runtime loader,
+    `load_nsg`, truncates instead)."""
+    path = Path(path)
+    size = path.stat().st_size
+    if kind == "rover":
+        if size != 16 + n * degree * 4:
+            raise ValueError("Rover graph size mismatch")
+        with path.open("rb") as f:
+            if f.read(16) != struct.pack("<4i", ROVER_GRAPH_MAGIC, n, dim, degree):
+                raise ValueError("Rover graph header mismatch")
+        graph = np.fromfile(path, dtype="<i4", offset=16).reshape(n, degree)
+    elif kind == "dense":
+        if size != n * degree * 4:
+            raise ValueError("Dense graph size mismatch")
+        graph = np.fromfile(path, dtype="<i4").reshape(n, degree)
+    elif kind == "nndescent":
+        with path.open("rb") as f:
+            header = struct.unpack("<3i", f.read(12))
+        if header != (n, degree, 0) or size != 12 + n * degree * 8:
+            raise ValueError("NNDescent graph shape mismatch")
+        rows = np.fromfile(path, dtype="u1", offset=12).reshape(n, degree * 8)
+        graph = np.ascontiguousarray(rows[:, :degree * 4]).view("<i4").reshape(n, degree)
+    elif kind == "nsg":
+        graph = np.full((n, degree), -1, dtype="<i4")
+        data = np.fromfile(path, dtype="<u4")
+        if len(data) < 2:
+            raise ValueError("NSG header mismatch")
+        width, entry = int(data[0]), int(data[1])
+        if not width or entry >= n:
+            raise ValueError("NSG header mismatch")
+        pos = 2
+        for i in range(n):
+            if pos >= len(data):
+                raise ValueError("Truncated NSG graph")
+            k = int(data[pos])
+            pos += 1
+            if k > width:
+                raise ValueError("NSG row exceeds declared width")
+            if k > degree:
+                raise ValueError("NSG row exceeds graph-degree; provide a graph within the "
+                                 "requested width")
+            if pos + k > len(data):
+                raise ValueError("Truncated NSG graph")
+            graph[i, :k] = data[pos:pos + k].view("<i4")
+            pos += k
+        if pos != len(data) or size % 4:
+            raise ValueError("Trailing NSG graph rows")
+    else:
+        raise ValueError("Unknown graph format: " + str(kind))
+    graph = np.ascontiguousarray(graph, dtype=np.int32)
+    _check_rows(graph, n)
+    return graph
+
+
+def nsg_entry(path):
+    """The entry point stored in an NSG file's header."""
+    with open(path, "rb") as f:
+        return struct.unpack("<2I", f.read(8))[1]
+
+
+def write_graph(path, graph, dim):
+    """graph.rover (index_bundle.py:95-103)."""
+    graph = np.ascontiguousarray(graph, dtype="<i4")
+    n, degree = graph.shape
+    with Path(path).open("xb") as f:
+        f.write(struct.pack("<4i", ROVER_GRAPH_MAGIC, n, dim, degree))
+        f.write(graph.tobytes())
+        f.flush()
+        os.fsync(f.fileno())
+
+
+@dataclass
+class NsgStats:
+    file_width: int
+    entry_point: int
+    truncated_nodes: int
+    truncated_edges: int
+
+
+def load_nsg(path, n, width):
+    """The runtime NSG loader's reading (retrieval_nsg_loader.c:25-189):
+    header (width, entry) with 0 < width <= n and entry < n; each row's kept
+    prefix (at most `width` ids) must be in range, not the node itself and
+    without duplicates; ids past `width` are dropped (counted) after range
+    and self checks; nothing may follow the last row.  Returns
+    (int32 [n][width] -1 padded, NsgStats); raises ValueError."""
+    data = np.fromfile(path, dtype="<u4")
+    if len(data) < 2:
+        raise ValueError("truncated NSG header")
+    fw, entry = int(data[0]), int(data[1])
+    if fw == 0 or fw > n:
+        raise ValueError("invalid NSG width %d for N=%d" % (fw, n))
+    if entry >= n:
+        raise ValueError("invalid NSG entry point %d for N=%d" % (entry, n))
+    graph = np.full((n, width), -1, dtype=np.int32)
+    tn = te = 0
+    pos = 2
+    for node in range(n):
+        if pos >= len(data):
+            raise ValueError("truncated degree at node %d" % node)
+        deg = int(data[pos])
+        pos += 1
+        if deg > fw:
+            raise ValueError("node %d degree %d exceeds header width %d" % (node, deg, fw))
+        keep = min(deg, width)
+        if pos + keep > len(data):
+            raise ValueError("truncated neighbor payload at node %d" % node)
+        row = data[pos:pos + keep].view("<i4")
+        pos += keep
+        if np.any(row < 0) or np.any(row >= n):
+            raise ValueError("node %d has an invalid neighbor" % node)
+        if np.any(row == node):
+            raise ValueError("node %d has a self edge" % node)
+        if len(np.unique(row)) != len(row):
+            raise ValueError("node %d has a duplicate neighbor" % node)
+        graph[node, :keep] = row
+        if deg > keep:
+            extra = data[pos:pos + deg - keep]
+            if len(extra) != deg - keep:
+                raise ValueError("truncated excess payload at node %d" % node)
+            if np.any(extra >= n):
+                raise ValueError("node %d has an invalid excess neighbor" % node)
+            if np.any(extra == node):
+                raise ValueError("node %d has an excess self edge" % node)
+            pos += deg - keep
+            tn += 1
+            te += deg - keep
+    if pos != len(data) or Path(path).stat().st_size % 4:
+        raise ValueError("unexpected data after %d NSG rows" % n)
+    return graph, NsgStats(fw, entry, tn, te)
+
+
+# ---------------------------------------------------------- PUREPQ01
+
+PQ_MAGIC = b"PUREPQ01"
+PQ_KIND_CODEBOOK, PQ_KIND_CODES = 1, 2
+PQ_ENDIAN = 0x01020304
+
+
+@dataclass
+class PqMeta:
+    """PUREPQ01 header metadata (retrieval_pq_artifact.h RetrievalPqArtifactMetadata)."""
+    index_uuid: bytes
+    num_vectors: int
+    dim: int
+    m: int
+    ks: int
+    subdim: int
+    code_bytes: int
+    trainer_kind: int = 0
+    sample_strategy: int = 0
+    train_count: int = 1
+    kmeans_iters: int = 1
+    seed: int = 0
+    dataset_fingerprint: bytes = b"\0" * 32
+    layout_fingerprint: bytes = b"\0" * 32
+
+    def validate(self):
+        """retrieval_pq_artifact.c:295-316 (validate_metadata)."""
+        if self.ks not in (16, 256):
+            raise ValueError("PQ ks must be 16 or 256")
+        expect = (self.m + 1) // 2 if self.ks == 16 else self.m
+        if (self.index_uuid == b"\0" * 16 or self.num_vectors == 0 or self.dim == 0 or
+                self.m == 0 or self.subdim == 0 or self.m * self.subdim != self.dim or
+                self.code_bytes != expect or self.train_count == 0 or
+                self.train_count > self.num_vectors or self.kmeans_iters == 0):
+            raise ValueError("invalid PQ artifact metadata")
+
+
+def encode_pq_header(kind, meta, payload_bytes, payload_sha):
+    h = bytearray(256)
+    h[0:8] = PQ_MAGIC
+    struct.pack_into("<HHIII", h, 8, 1, 1, 256, kind, PQ_ENDIAN)
+    h[32:48] = meta.index_uuid
+    struct.pack_into("<QIIIII", h, 48, meta.num_vectors, meta.dim, meta.m, meta.ks,
+                     meta.subdim, meta.code_bytes)
+    struct.pack_into("<IIIIIQIIQ", h, 76, 1, 1, 1, meta.trainer_kind, meta.sample_strategy,
+                     meta.train_count, meta.kmeans_iters, meta.seed, payload_bytes)
+    h[120:152] = payload_sha
+    h[152:184] = meta.dataset_fingerprint
+    h[184:216] = meta.layout_fingerprint
+    return bytes(h)
+
+
+def decode_pq_header(h, expected_kind):
+    """retrieval_pq_artifact.c:373-406 (pq_decode_header)."""
+    if len(h) < 256 or h[0:8] != PQ_MAGIC:
+        raise ValueError("invalid Pure-PQ magic/header length")
+    major, minor, hbytes, kind, endian = struct.unpack_from("<HHIII", h, 8)
+    metric, cdt, bdt = struct.unpack_from("<III", h, 76)
+    if major != 1 or minor > 1 or hbytes != 256 or kind != expected_kind or \
+            endian != PQ_ENDIAN or (metric, cdt, bdt) != (1, 1, 1):
+        raise ValueError("invalid Pure-PQ header")
+    n, dim, m, ks, sub, cb = struct.unpack_from("<QIIIII", h, 48)
+    trainer, sample, train_count, iters, seed, payload = struct.unpack_from("<IIQIIQ", h, 88)
+    meta = PqMeta(bytes(h[32:48]), n, dim, m, ks, sub, cb, trainer, sample, train_count,
+                  iters, seed, bytes(h[152:184]), bytes(h[184:216]))
+    meta.validate()
+    return meta, payload, bytes(h[120:152])
+
+
+def read_pq_artifacts(prefix):
+    """(PqMeta, codebook f32 [m][ks][subdim], codes u8 [n][code_bytes]) from
+    prefix.pqbook / prefix.pqcodes: headers, payload sizes and digests, and
+    equal metadata in both files."""
+    out = []
+    for kind, ext in ((PQ_KIND_CODEBOOK, ".pqbook"), (PQ_KIND_CODES, ".pqcodes")):
+        raw = Path(str(prefix) + ext).read_bytes()
+        meta, payload, digest = decode_pq_header(raw[:256], kind)
+        body = raw[256:]
+        if len(body) != payload or hashlib.sha256(body).digest() != digest:
+            raise ValueError("Pure-PQ payload size or digest mismatch: " + ext)
+        out.append((meta, body))
+    (mb, book), (mc, codes) = out
+    if mb != mc:
+        raise ValueError("Pure-PQ codebook / codes metadata differ")
+    if len(book) != mb.m * mb.ks * mb.subdim * 4 or len(codes) != mb.num_vectors * mb.code_bytes:
+        raise ValueError("Pure-PQ payload shape mismatch")
+    return (mb, np.frombuffer(book, dtype="<f4").reshape(mb.m, mb.ks, mb.subdim).copy(),
+            np.frombuffer(codes, dtype=np.uint8).reshape(mb.num_vectors, mb.code_bytes).copy())
+
+
+def write_pq_artifacts(prefix, meta, codebook, codes):
+    meta.validate()
+    for kind, ext, arr in ((PQ_KIND_CODEBOOK, ".pqbook", np.ascontiguousarray(codebook, "<f4")),
+                           (PQ_KIND_CODES, ".pqcodes", np.ascontiguousarray(codes, np.uint8))):
+        body = arr.tobytes()
+        with open(str(prefix) + ext, "xb") as f:
+            f.write(encode_pq_header(kind, meta, len(body), hashlib.sha256(body).digest()))
+            f.write(body)
+
+
+# ----------------------------------------------------------- PQRES01
+
+RES_MAGIC = b"PQRES01\0"
+RES_KIND_BLOCKS, RES_KIND_QPARAMS = 1, 2
+
+
+def _res_header(kind, storage, meta, bits, payload, digest):
+    cb = (meta.dim + 1) // 2 if bits == 4 else meta.dim
+    vpb = BLOCK // cb
+    blocks = (meta.num_vectors + vpb - 1) // vpb
+    h = bytearray(256)
+    h[0:8] = RES_MAGIC
+    struct.pack_into("<HHIIIII", h, 8, 1, 0, storage, kind, PQ_ENDIAN, bits, cb)
+    h[32:48] = meta.index_uuid
+    struct.pack_into("<QIIIIQ", h, 48, meta.num_vectors, meta.dim, BLOCK, vpb, blocks, payload)
+    h[80:112] = digest
+    h[112:144] = meta.dataset_fingerprint
+    h[144:176] = meta.layout_fingerprint
+    struct.pack_into("<I", h, 176, 8)
+    return bytes(h)
+
+
+def write_pq_residual(prefix, meta, bits, codes, qparams):
+    """prefix.pqres{bits} and .qparams as retrieval_pq_residual.c writes them
+    (blocks file: header, zero padding to 4096, then blocks; qparams file:
+    header, then (scale, zero) pairs)."""
+    codes = np.ascontiguousarray(codes, np.uint8)
+    qparams = np.ascontiguousarray(qparams, "<f4")
+    cb = (meta.dim + 1) // 2 if bits == 4 else meta.dim
+    vpb = BLOCK // cb
+    n = meta.num_vectors
+    blocks = (n + vpb - 1) // vpb
+    body = np.zeros(blocks * BLOCK, np.uint8)
+    rows = np.arange(n)
+    body[((rows // vpb) * BLOCK + (rows % vpb) * cb)[:, None] + np.arange(cb)] = codes
+    body = body.tobytes()
+    qp = qparams.tobytes()
+    base = "%s.pqres%d" % (prefix, bits)
+    with open(base, "xb") as f:
+        f.write(_res_header(RES_KIND_BLOCKS, BLOCK, meta, bits, len(body),
+                            hashlib.sha256(body).digest()))
+        f.write(b"\0" * (BLOCK - 256))
+        f.write(body)
+    with open(base + ".qparams", "xb") as f:
+        f.write(_res_header(RES_KIND_QPARAMS, 256, meta, bits, len(qp),
+                            hashlib.sha256(qp).digest()))
+        f.write(qp)
+
+
+def read_pq_residual(prefix, meta, bits, verify_blocks=True):
+    """(codes u8 [n][code bytes], qparams f32 [n][2]) with
+    retrieval_pq_residual_index_open's checks (retrieval_pq_residual.c:
+    716-801): headers, metadata equal between the files and to the PQ
+    artifacts', sizes, the qparams digest (and the blocks digest)."""
+    base = "%s.pqres%d" % (prefix, bits)
+    heads = []
+    for path, kind, storage in ((base, RES_KIND_BLOCKS, BLOCK),
+                                (base + ".qparams", RES_KIND_QPARAMS, 256)):
+        raw = Path(path).read_bytes()
+        h = raw[:256]
+        if len(h) < 256 or h[0:8] != RES_MAGIC:
+            raise ValueError("invalid PQRES header")
+        major, minor, st, k, endian, rbits, rcb = struct.unpack_from("<HHIIIII", h, 8)
+        n, dim, bb, vpb, blocks, payload = struct.unpack_from("<QIIIIQ", h, 48)
+        if (major, st, k, endian) != (1, storage, kind, PQ_ENDIAN) or minor > 0 or \
+                struct.unpack_from("<I", h, 176)[0] != 8:
+            raise ValueError("invalid PQRES header")
+        ecb = (dim + 1) // 2 if rbits == 4 else dim
+        if rbits not in (4, 8) or n == 0 or dim == 0 or bb != BLOCK or rcb != ecb or \
+                vpb != BLOCK // ecb or blocks != (n + vpb - 1) // vpb:
+            raise ValueError("invalid PQRES metadata")
+        key = (rbits, rcb, bytes(h[32:48]), n, dim, vpb, blocks, bytes(h[112:176]))
+        heads.append((key, payload, bytes(h[80:112]), raw[storage:]))
+    (kb, pb, db, body), (kq, pq, dq, qbody) = heads
+    if kb != kq or kb[2] != meta.index_uuid or kb[3] != meta.num_vectors or \
+            kb[4] != meta.dim or kb[7] != meta.dataset_fingerprint + meta.layout_fingerprint or \
+            kb[0] != bits:
+        raise ValueError("PQRES metadata does not match the PQ artifacts")
+    n, cb, vpb = kb[3], kb[1], kb[5]
+    if pb != kb[6] * BLOCK or len(body) != pb or pq != n * 8 or len(qbody) != pq:
+        raise ValueError("PQRES payload size mismatch")
+    if hashlib.sha256(qbody).digest() != dq or \
+            (verify_blocks and hashlib.sha256(body).digest() != db):
+        raise ValueError("PQRES payload digest mismatch")
+    blk = np.frombuffer(body, np.uint8)
+    rows = np.arange(n)
+    codes = blk[((rows // vpb) * BLOCK + (rows % vpb) * cb)[:, None] + np.arange(cb)]
+    return codes, np.frombuffer(qbody, "<f4").reshape(n, 2).copy()
+
+
+# ----------------------------------------------------------- SQ files
+
+L1_MAGIC, QP_MAGIC = 0x314C4D53, 0x51504D53
+
+
+def write_sq(directory, codes, bits, scale, zero, res_codes, res_qparams, res_bits, dim):
+    """l1.bin, residual.bin, qparams.bin as encode_l1_residual.py writes
+    them."""
+    d = Path(directory)
+    n = len(codes)
+    with (d / "l1.bin").open("xb") as f:
+        f.write(struct.pack("<IIiiiff4s", L1_MAGIC, 1, n, dim, bits, float(scale),
+                            float(zero), b"\0" * 4))
+        f.write(np.ascontiguousarray(codes, np.uint8).tobytes())
+    with (d / "qparams.bin").open("xb") as f:
+        f.write(struct.pack("<IIiiff8s", QP_MAGIC, 1, n, res_bits, float(scale), float(zero),
+                            b"\0" * 8))
+        f.write(np.ascontiguousarray(res_qparams, "<f4").tobytes())
+    cb = (dim + 1) // 2 if res_bits == 4 else dim
+    vpb = BLOCK // cb
+    blocks = (n + vpb - 1) // vpb
+    body = np.zeros(blocks * BLOCK, np.uint8)
+    rows = np.arange(n)
+    body[((rows // vpb) * BLOCK + (rows % vpb) * cb)[:, None] + np.arange(cb)] = res_codes
+    (d / "residual.bin").write_bytes(body.tobytes())
+
+
+def read_sq(directory, n, dim):
+    """(codes, bits, scale, zero, res_codes, res_qparams, res_bits) with
+    generate_final_refine.py:64-82's header and size checks."""
+    d = Path(directory)
+    raw = (d / "l1.bin").read_bytes()
+    magic, ver, hn, hd, bits, scale, zero, _ = struct.unpack_from("<IIiiiff4s", raw, 0)
+    qraw = (d / "qparams.bin").read_bytes()
+    qm, qv, qn, rbits, qs, qz, _ = struct.unpack_from("<IIiiff8s", qraw, 0)
+    if (magic, ver, hn, hd) != (L1_MAGIC, 1, n, dim) or (qm, qv, qn) != (QP_MAGIC, 1, n) or \
+            bits not in (4, 8) or rbits not in (4, 8) or (scale, zero) != (qs, qz) or \
+            not np.isfinite([scale, zero]).all() or scale <= 0:
+        raise ValueError("invalid SQ headers")
+    lc = (dim + 1) // 2 if bits == 4 else dim
+    rc = (dim + 1) // 2 if rbits == 4 else dim
+    vpb = BLOCK // rc
+    res = (d / "residual.bin").read_bytes()
+    if len(raw) != 32 + n * lc or len(qraw) != 32 + 8 * n or \
+            len(res) != ((n + vpb - 1) // vpb) * BLOCK:
+        raise ValueError("SQ payload size mismatch")
+    codes = np.frombuffer(raw, np.uint8, offset=32).reshape(n, lc).copy()
+    qp = np.frombuffer(qraw, "<f4", offset=32).reshape(n, 2).copy()
+    blk = np.frombuffer(res, np.uint8)
+    rows = np.arange(n)
+    rcodes = blk[((rows // vpb) * BLOCK + (rows % vpb) * rc)[:, None] + np.arange(rc)]
+    return codes, bits, np.float32(scale), np.float32(zero), rcodes, qp, rbits
+
+
+# -------------------------------------------------------------- index.json
+
+def validate(directory, third=True, require_graph=True):
+    """index.json and its members as index_bundle.py:187-230 checks them:
+    schema, method, dimensions, bit widths, required members, each member's
+    size and SHA-256, the PQ file names.  Returns the parsed index.json."""
+    directory = Path(directory).resolve()
+    data = json.loads((directory / "index.json").read_text())
+    if data.get("schema") != SCHEMA:
+        raise ValueError("Unsupported Rover index schema")
+    if data.get("method") not in ("sq", "pq") or not 0 < data["dim"] <= 1024 or \
+            not 0 < data["graph_degree"] < data["n"] <= 2147483647:
+        raise ValueError("Invalid index method or dimensions")
+    if data["residual_bits"] not in (4, 8) or \
+            data.get("sq_bits" if data["method"] == "sq" else "pq_bits") not in (4, 8):
+        raise ValueError("Index bits must be 4 or 8")
+    required = (["l1", "residual", "qparams"] if data["method"] == "sq" else
+                ["pq_book", "pq_codes", "pq_residual", "pq_qparams", "pq_manifest",
+                 "pq_residual_manifest"])
+    if require_graph:
+        required.append("graph")
+    if third:
+        required.append("error4")
+    if data.get("reordered"):
+        required.append("order_map")
+    if data.get("residual_layout", "a") != "a":
+        required.append("residual_map")
+    for key in required:
+        if key not in data["files"]:
+            raise ValueError("Index missing " + key)
+    names = set()
+    for key, item in data["files"].items():
+        if key == "error4" and not third:
+            continue
+        path = (directory / item["path"]).resolve()
+        if Path(item["path"]).is_absolute() or directory not in path.parents or \
+                not path.is_file():
+            raise ValueError("Invalid index member " + key)
+        if path.name in names or path.name == "insert.checkpoint":
+            raise ValueError("Colliding index member names")
+        names.add(path.name)
+        if path.stat().st_size != item["bytes"] or _sha(path) != item["sha256"]:
+            raise ValueError("Index member changed: " + key)
+    if data["method"] == "pq":
+        sfx = ".pqres%d" % data["residual_bits"]
+        want = dict(pq_book="index.pqbook", pq_codes="index.pqcodes",
+                    pq_manifest="index.pq.json", pq_residual="index" + sfx,
+                    pq_qparams="index" + sfx + ".qparams",
+                    pq_residual_manifest="index" + sfx + ".json")
+        if any(Path(data["files"][k]["path"]).name != v for k, v in want.items()):
+            raise ValueError("PQ bundle filenames must use the index prefix")
+    return data
+
+
+@dataclass
+class Bundle:
+    """A bundle's members in memory, ready for the library."""
+    directory: Path
+    meta: dict
+    graph: np.ndarray
+    graph_entry: int
+    method: str
+    dim: int
+    n: int
+    pq_meta: PqMeta = None
+    codebook: np.ndarray = None
+    codes: np.ndarray = None
+    res_codes: np.ndarray = None
+    res_qparams: np.ndarray = None
+    res_bits: int = 0
+    sq_codes: np.ndarray = None
+    sq_bits: int = 0
+    sq_scale: float = 0.0
+    sq_zero: float = 0.0
+    error4: bytes = None
+    error4_sha256: bytes = None
+    original_ids: np.ndarray = None
+    extra: dict = field(default_factory=dict)
+
+    def load(self, calc, third=None):
+        """Loads the bundle into a rover_amd.search.Calculator: PQ codes,
+        graph, residuals, the id map and (third, default: when the bundle has
+        one) the error4 artifact; or the SQ codes, graph and residuals."""
+        if self.method == "pq":
+            calc.retrieval_pure_pq_load(self.codes, self.codebook, uuid=np.frombuffer(
+                self.pq_meta.index_uuid, np.uint8))
+            calc.pure_pq_graph_load(self.graph, self.graph_entry)
+            calc.pure_pq_residual_load(self.res_codes, self.res_qparams, self.res_bits)
+            if self.original_ids is not None:
+                calc.pure_pq_original_ids(self.original_ids)
+            if (third if third is not None else self.error4 is not None):
+                if self.error4 is None:
+                    raise ValueError("bundle has no error4 artifact")
+                calc.pure_pq_error4_load(self.error4, self.error4_sha256)
+        else:
+            calc.rover_configure(l1_bits=self.sq_bits)
+            calc.retrieval_load(self.graph, self.sq_codes, self.dim, self.graph_entry,
+                                self.sq_scale, self.sq_zero)
+            calc.sq_residual_load(self.res_codes, self.res_qparams, self.res_bits)
+
+
+def open_bundle(directory, third=None, verify=True):
+    """Reads a bundle directory (validated as `validate` does when verify)."""
+    directory = Path(directory).resolve()
+    data = json.loads((directory / "index.json").read_text())
+    has_e4 = "error4" in data.get("files", {})
+    third = has_e4 if third is None else third
+    if verify:
+        data = validate(directory, third=third)
+    files = data["files"]
+    n, dim, degree = data["n"], data["dim"], data["graph_degree"]
+    graph = read_graph(directory / files["graph"]["path"], "rover", n, degree, dim)
+    b = Bundle(directory, data, graph, int(data.get("graph_entry", 0)), data["method"], dim, n)
+    if data.get("reordered") and "order_map" in files:
+        b.original_ids = np.loadtxt(directory / files["order_map"]["path"], dtype=np.int32,
+                                    ndmin=1)
+    if b.method == "pq":
+        prefix = directory / "index"
+        b.pq_meta, b.codebook, b.codes = read_pq_artifacts(prefix)
+        b.res_bits = data["residual_bits"]
+        b.res_codes, b.res_qparams = read_pq_residual(prefix, b.pq_meta, b.res_bits)
+    else:
+        (b.sq_codes, b.sq_bits, b.sq_scale, b.sq_zero, b.res_codes, b.res_qparams,
+         b.res_bits) = read_sq(directory, n, dim)
+    if third and has_e4:
+        b.error4 = (directory / files["error4"]["path"]).read_bytes()
+        b.error4_sha256 = bytes.fromhex(files["error4"]["sha256"])
+    return b
+
+
+def write_pq_bundle(directory, graph, graph_entry, codebook, codes, res_codes, res_qparams,
+                    res_bits, meta=None, error4=None):
+    """A PQ bundle from arrays (graph.rover, the PQ and PQRES artifacts,
+    optional error4.refine, index.json).  The manifests index.pq.json and
+    index.pqres*.json record the artifacts' sizes and digests."""
+    d = Path(directory)
+    d.mkdir(parents=True, exist_ok=True)
+    n, degree = graph.shape
+    m, ks, sub = codebook.shape
+    dim = m * sub
+    meta = meta or PqMeta(hashlib.sha256(b"rover-amd").digest()[:16], n, dim, m, ks, sub,
+                          (m + 1) // 2 if ks == 16 else m, 0, 1, n, 1, 0)
+    write_graph(d / "graph.rover", graph, dim)
+    prefix = d / "index"
+    write_pq_artifacts(prefix, meta, codebook, codes)
+    write_pq_residual(prefix, meta, res_bits, res_codes, res_qparams)
+    sfx = ".pqres%d" % res_bits
+    (d / "index.pq.json").write_text(json.dumps(
+        {k: _record(str(prefix) + e, d) for k, e in (("codebook", ".pqbook"),
+                                                       ("codes", ".pqcodes"))}, indent=2))
+    (d / ("index" + sfx + ".json")).write_text(json.dumps(
+        {k: _record(str(prefix) + sfx + e, d) for k, e in (("blocks", ""),
+                                                             ("qparams", ".qparams"))}, indent=2))
+    files = dict(graph=_record(d / "graph.rover", d))
+    for key, ext in (("pq_book", ".pqbook"), ("pq_codes", ".pqcodes"),
+                     ("pq_manifest", ".pq.json"), ("pq_residual", sfx),
+                     ("pq_qparams", sfx + ".qparams"), ("pq_residual_manifest", sfx + ".json")):
+        files[key] = _record(str(prefix) + ext, d)
+    if error4 is not None:
+        (d / "error4.refine").write_bytes(error4)
+        files["error4"] = _record(d / "error4.refine", d)
+    data = dict(schema=SCHEMA, n=n, dim=dim, graph_degree=degree, method="pq",
+                residual_bits=res_bits, pq_bits=8 if ks == 256 else 4, pq_m=m,
+                third_residual=error4 is not None, reordered=False, layout="original",
+                graph_format="rover", graph_entry=int(graph_entry), files=files)
+    (d / "index.json").write_text(json.dumps(data, indent=2) + "\n")
+    return data
+
+
+def write_sq_bundle(directory, graph, graph_entry, codes, bits, scale, zero, res_codes,
+                    res_qparams, res_bits, dim):
+    """An SQ bundle from arrays (graph.rover, l1.bin, residual.bin,
+    qparams.bin, index.json)."""
+    d = Path(directory)
+    d.mkdir(parents=True, exist_ok=True)
+    n, degree = graph.shape
+    write_graph(d / "graph.rover", graph, dim)
+    write_sq(d, codes, bits, scale, zero, res_codes, res_qparams, res_bits, dim)
+    files = {k: _record(d / f, d) for k, f in (("graph", "graph.rover"), ("l1", "l1.bin"),
+                                                ("residual", "residual.bin"),
+                                                ("qparams", "qparams.bin"))}
+    data = dict(schema=SCHEMA, n=n, dim=dim, graph_degree=degree, method="sq",
+                residual_bits=res_bits, sq_bits=bits, third_residual=False, reordered=False,
+                layout="original", graph_format="rover", graph_entry=int(graph_entry),
+                files=files)
+    (d / "index.json").write_text(json.dumps(data, indent=2) + "\n")
+    return data
+
+
+# ------------------------------------------------------------ results
+
+def write_results(path, ids, dists, counts, original_ids=None, first_query=0):
+    """results.jsonl as rover_write_result writes it (config.c:140-152): one
+    line per query, `{"query":q,"ids":[...],"distances":[...]}`, ids mapped
+    through original_ids when given, distances printed with %.9g."""
+    lines = []
+    for q in range(len(counts)):
+        c = int(counts[q])
+        row = ids[q][:c]
+        if c < 0 or np.any(row < 0) or (original_ids is not None and
+                                         np.any(row >= len(original_ids))):
+            raise ValueError("invalid result row %d" % q)
+        d = np.asarray(dists[q][:c], dtype=np.float32)
+        if not np.isfinite(d).all():
+            raise ValueError("non-finite distance in result row %d" % q)
+        out = original_ids[row] if original_ids is not None else row
+        lines.append('{"query":%d,"ids":[%s],"distances":[%s]}\n' % (
+            first_query + q, ",".join("%d" % v for v in out),
+            ",".join("%.9g" % float(v) for v in d)))
+    with open(path, "x") as f:
+        f.write("".join(lines))

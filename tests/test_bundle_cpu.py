@@ -1,0 +1,268 @@
+"""Index bundles and on-disk formats against the reference's own writers and
+readers (compiled from the Rover sources into oracle/_ref, so these tests
+run where that library was built): PUREPQ01 both ways and byte for byte,
+PQRES01 both ways and byte for byte, results.jsonl byte for byte, the
+runtime NSG loader; plus the bundle tool's graph formats and index.json
+checks."""
+import ctypes as C
+import hashlib
+import json
+import os
+import struct
+
+import numpy as np
+import pytest
+
+import pyoracle
+from rover_amd import bundle
+
+pytestmark = pytest.mark.skipif(not pyoracle.ref_available(),
+                                reason="compiled reference (oracle/_ref) not built")
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+@pytest.fixture(scope="module")
+def pq(golden):
+    g = golden("pq_d96_m48")
+    return g
+
+
+def meta_of(g, uuid=bytes(range(1, 17)), train_count=None, iters=1, fp=(b"\0" * 32, b"\0" * 32)):
+    m, ks, sub = g["codebook"].shape
+    n = g["codes"].shape[0]
+    return bundle.PqMeta(uuid, n, m * sub, m, ks, sub, g["codes"].shape[1], 0, 0,
+                         train_count or n, iters, 0, fp[0], fp[1])
+
+
+def test_pq_artifacts_both_ways(pq, tmp_path):
+    ref = pyoracle.reference()
+    cb, codes = pq["codebook"], pq["codes"]
+    m, ks, sub = cb.shape
+    n = len(codes)
+    uuid = np.frombuffer(hashlib.sha256(b"u").digest()[:16], np.uint8).copy()
+    dfp = np.frombuffer(hashlib.sha256(b"d").digest(), np.uint8).copy()
+    lfp = np.frombuffer(hashlib.sha256(b"l").digest(), np.uint8).copy()
+    refp = str(tmp_path / "ref")
+    assert ref.ref_pq_artifacts_write(refp.encode(), _p(uuid), n, m * sub, m, ks, 3, 1, n // 2,
+                                      7, 99, _p(dfp), _p(lfp), _p(cb), _p(codes)) == 0
+    meta, rcb, rcodes = bundle.read_pq_artifacts(refp)
+    assert np.array_equal(rcb, cb) and np.array_equal(rcodes, codes)
+    assert (meta.index_uuid, meta.trainer_kind, meta.sample_strategy, meta.train_count,
+            meta.kmeans_iters, meta.seed) == (uuid.tobytes(), 3, 1, n // 2, 7, 99)
+    ours = str(tmp_path / "ours")
+    bundle.write_pq_artifacts(ours, meta, cb, codes)
+    for ext in (".pqbook", ".pqcodes"):
+        assert open(refp + ext, "rb").read() == open(ours + ext, "rb").read()
+    out_cb = np.zeros_like(cb)
+    out_codes = np.zeros_like(codes)
+    dims = [C.c_int() for _ in range(4)]
+    assert ref.ref_pq_artifacts_load(ours.encode(), *[C.byref(d) for d in dims], _p(out_cb),
+                                     out_cb.size, _p(out_codes), out_codes.size) == 0
+    assert [d.value for d in dims] == [n, m * sub, m, ks]
+    assert np.array_equal(out_cb, cb) and np.array_equal(out_codes, codes)
+    # a flipped payload byte fails both readers
+    raw = bytearray(open(ours + ".pqcodes", "rb").read())
+    raw[300] ^= 1
+    open(ours + ".pqcodes", "wb").write(bytes(raw))
+    with pytest.raises(ValueError):
+        bundle.read_pq_artifacts(ours)
+    assert ref.ref_pq_artifacts_load(ours.encode(), *[C.byref(d) for d in dims], _p(out_cb),
+                                     out_cb.size, _p(out_codes), out_codes.size) != 0
+
+
+@pytest.mark.parametrize("name", ["pq_d96_m48", "pq_d64_ks16"])
+def test_pq_residual_both_ways(golden, name, tmp_path):
+    g = golden(name)
+    ref = pyoracle.reference()
+    base, codes, cb = g["base"], g["codes"], g["codebook"]
+    m, ks, sub = cb.shape
+    n, dim = base.shape
+    bits = int(g["res_bits"])
+    res = np.zeros_like(g["res_codes"])
+    qp = np.zeros_like(g["res_qparams"])
+    h = ref.ref_residual_open(_p(base), _p(codes), _p(cb), n, dim, m, ks, bits, _p(res), _p(qp))
+    assert h
+    try:
+        prefix = ref.ref_residual_prefix(h).decode()
+        meta = meta_of(g)
+        rcodes, rqp = bundle.read_pq_residual(prefix, meta, bits)
+        assert np.array_equal(rcodes, g["res_codes"]) and np.array_equal(rqp, g["res_qparams"])
+        ours = str(tmp_path / "index")
+        bundle.write_pq_residual(ours, meta, bits, rcodes, rqp)
+        for ext in ("", ".qparams"):
+            a = open("%s.pqres%d%s" % (prefix, bits, ext), "rb").read()
+            b = open("%s.pqres%d%s" % (ours, bits, ext), "rb").read()
+            assert a == b, ext
+    finally:
+        ref.ref_residual_close(h)
+    # the reference reader opens our files and decodes what the oracle decodes
+    ids = np.arange(0, n, 37, dtype=np.int32)
+    out = np.zeros((len(ids), dim), dtype=np.float32)
+    uuid = np.frombuffer(bytes(range(1, 17)), np.uint8).copy()
+    z = np.zeros(32, np.uint8)
+    assert ref.ref_pq_residual_read(ours.encode(), _p(uuid), n, dim, m, ks, n, 1, _p(z), _p(z),
+                                    _p(cb), _p(codes), bits, _p(ids), len(ids), _p(out)) == 0
+    oix = pyoracle.Index(g["graph"], codes, cb, g["res_codes"], g["res_qparams"], bits)
+    for i, vid in enumerate(ids):
+        assert np.array_equal(out[i].view(np.uint32),
+                              pyoracle.residual_decode(oix, int(vid)).view(np.uint32))
+    # a residual bound to another index (uuid) is refused by both
+    other = meta_of(g, uuid=bytes(range(2, 18)))
+    with pytest.raises(ValueError):
+        bundle.read_pq_residual(ours, other, bits)
+    uuid2 = np.frombuffer(bytes(range(2, 18)), np.uint8).copy()
+    assert ref.ref_pq_residual_read(ours.encode(), _p(uuid2), n, dim, m, ks, n, 1, _p(z), _p(z),
+                                    _p(cb), _p(codes), bits, _p(ids), len(ids), _p(out)) != 0
+
+
+def test_results_jsonl_byte_for_byte(tmp_path):
+    ref = pyoracle.reference()
+    rng = np.random.default_rng(1)
+    n, nq, k = 500, 7, 10
+    ids = rng.integers(0, n, (nq, k), dtype=np.int32)
+    d = (rng.random((nq, k)) * 1e4).astype(np.float32)
+    d[0, 0] = np.float32(1.0) / np.float32(3.0)
+    d[1, 1] = np.float32(1e-30)
+    counts = np.array([10, 3, 0, 10, 10, 1, 10], dtype=np.int32)
+    for order in (None, rng.permutation(n).astype(np.int32)):
+        rp = str(tmp_path / ("ref%d.jsonl" % (order is not None)))
+        op = str(tmp_path / ("ours%d.jsonl" % (order is not None)))
+        mp = None
+        if order is not None:
+            mp = str(tmp_path / "order.txt")
+            np.savetxt(mp, order, fmt="%d")
+        assert ref.ref_results_write(rp.encode(), n, mp.encode() if mp else None, nq,
+                                     _p(ids), _p(d), _p(counts), k) == 0
+        bundle.write_results(op, ids, d, counts, original_ids=order)
+        assert open(rp, "rb").read() == open(op, "rb").read()
+        for line in open(op):
+            json.loads(line)
+
+
+def _nsg_file(path, width, entry, rows):
+    with open(path, "wb") as f:
+        f.write(struct.pack("<2I", width, entry))
+        for r in rows:
+            f.write(struct.pack("<I", len(r)))
+            f.write(np.asarray(r, dtype="<u4").tobytes())
+
+
+def test_runtime_nsg_loader_matches_reference(tmp_path):
+    ref = pyoracle.reference()
+    rng = np.random.default_rng(2)
+    n = 300
+    rows = []
+    for i in range(n):
+        k = int(rng.integers(0, 40))
+        cand = rng.permutation(n)
+        rows.append([int(v) for v in cand[cand != i][:k]])
+    path = str(tmp_path / "g.nsg")
+    _nsg_file(path, 40, 17, rows)
+    for width in (16, 32, 48):
+        g, st = bundle.load_nsg(path, n, width)
+        rg = np.zeros((n, width), dtype=np.int32)
+        rst = np.zeros(4, dtype=np.int64)
+        assert ref.ref_nsg_load(path.encode(), n, width, _p(rg), _p(rst)) == 0
+        assert np.array_equal(g, rg)
+        assert (st.file_width, st.entry_point, st.truncated_nodes, st.truncated_edges) == \
+            tuple(int(v) for v in rst)
+    bad_cases = {
+        "width0": (0, 0, rows),
+        "entry": (40, n, rows),
+        "self": (40, 0, [[0, 1]] + rows[1:]),
+        "dup": (40, 0, [[1, 1]] + rows[1:]),
+        "range": (40, 0, [[n]] + rows[1:]),
+        "over": (4, 0, [[1, 2, 3, 4, 5]] + [[]] * (n - 1)),
+    }
+    for name, (w, e, rr) in bad_cases.items():
+        p = str(tmp_path / (name + ".nsg"))
+        _nsg_file(p, w, e, rr)
+        rg = np.zeros((n, 32), dtype=np.int32)
+        rst = np.zeros(4, dtype=np.int64)
+        assert ref.ref_nsg_load(p.encode(), n, 32, _p(rg), _p(rst)) != 0, name
+        with pytest.raises(ValueError):
+            bundle.load_nsg(p, n, 32)
+    with open(path, "ab") as f:
+        f.write(b"\0\0\0\0")
+    assert ref.ref_nsg_load(path.encode(), n, 32, _p(rg), _p(rst)) != 0
+    with pytest.raises(ValueError):
+        bundle.load_nsg(path, n, 32)
+
+
+def test_graph_formats_and_checks(tmp_path, pq):
+    graph = pq["graph"]
+    n, degree = graph.shape
+    dim = pq["codebook"].shape[0] * pq["codebook"].shape[2]
+    bundle.write_graph(tmp_path / "g.rover", graph, dim)
+    assert np.array_equal(bundle.read_graph(tmp_path / "g.rover", "rover", n, degree, dim), graph)
+    graph.astype("<i4").tofile(tmp_path / "g.dense")
+    assert np.array_equal(bundle.read_graph(tmp_path / "g.dense", "dense", n, degree, dim), graph)
+    with open(tmp_path / "g.nnd", "wb") as f:
+        f.write(struct.pack("<3i", n, degree, 0))
+        rows = np.concatenate([graph, np.zeros_like(graph)], axis=1).astype("<i4")
+        f.write(rows.tobytes())
+    assert np.array_equal(bundle.read_graph(tmp_path / "g.nnd", "nndescent", n, degree, dim),
+                          graph)
+    _nsg_file(tmp_path / "g.nsg", degree, 5, [r[r >= 0] for r in graph])
+    assert np.array_equal(bundle.read_graph(tmp_path / "g.nsg", "nsg", n, degree, dim), graph)
+    assert bundle.nsg_entry(tmp_path / "g.nsg") == 5
+    bad = graph.copy()
+    bad[3, 0] = 3
+    bad.astype("<i4").tofile(tmp_path / "self.dense")
+    with pytest.raises(ValueError):
+        bundle.read_graph(tmp_path / "self.dense", "dense", n, degree, dim)
+    bad = graph.copy()
+    bad[4, 1] = bad[4, 0]
+    bad.astype("<i4").tofile(tmp_path / "dup.dense")
+    with pytest.raises(ValueError):
+        bundle.read_graph(tmp_path / "dup.dense", "dense", n, degree, dim)
+    with pytest.raises(ValueError):
+        bundle.read_graph(tmp_path / "g.rover", "rover", n, degree, dim + 1)
+
+
+def test_pq_bundle_round_trip_and_tamper(pq, tmp_path):
+    from rover_amd import artifacts
+    g = pq
+    d = tmp_path / "bundle"
+    image = open(os.path.join(os.path.dirname(__file__), "golden", "pq_d96_m48_error4.npz"),
+                 "rb")
+    e4 = np.load(image, allow_pickle=False)["artifact"].tobytes()
+    bundle.write_pq_bundle(d, g["graph"], int(g["entry"]), g["codebook"], g["codes"],
+                           g["res_codes"], g["res_qparams"], int(g["res_bits"]), error4=e4)
+    b = bundle.open_bundle(d)
+    assert b.method == "pq" and b.graph_entry == int(g["entry"])
+    for a, want in ((b.graph, g["graph"]), (b.codes, g["codes"]), (b.codebook, g["codebook"]),
+                    (b.res_codes, g["res_codes"]), (b.res_qparams, g["res_qparams"])):
+        assert np.array_equal(a, want)
+    assert b.error4 == e4 and b.error4_sha256 == hashlib.sha256(e4).digest()
+    artifacts.parse_error4(b.error4)
+    # a changed member is refused
+    raw = bytearray((d / "graph.rover").read_bytes())
+    raw[-1] ^= 1
+    (d / "graph.rover").write_bytes(bytes(raw))
+    with pytest.raises(ValueError):
+        bundle.validate(d)
+
+
+def test_sq_files_round_trip(golden, tmp_path):
+    from rover_amd import index as rindex
+    g, s = golden("pq_d128_m32"), golden("sq_d128")
+    base = g["base"]
+    n, dim = base.shape
+    for bits, rb in ((8, 4), (4, 8)):
+        d = tmp_path / ("sq%d" % bits)
+        d.mkdir()
+        codes = s["sq%d_codes" % bits]
+        res, qp = rindex.encode_sq_residual(base, codes, bits, s["sq%d_scale" % bits],
+                                            s["sq%d_zero" % bits], rb)
+        bundle.write_sq(d, codes, bits, s["sq%d_scale" % bits], s["sq%d_zero" % bits], res, qp,
+                        rb, dim)
+        c2, b2, sc, zr, r2, q2, rb2 = bundle.read_sq(d, n, dim)
+        assert (b2, rb2) == (bits, rb)
+        assert sc == s["sq%d_scale" % bits] and zr == s["sq%d_zero" % bits]
+        assert np.array_equal(c2, codes) and np.array_equal(r2, res) and np.array_equal(q2, qp)
+        with pytest.raises(ValueError):
+            bundle.read_sq(d, n + 1, dim)
