@@ -450,6 +450,20 @@ AEEResult rover_build_error4_encode(const float *d_x, const uint8_t *d_codes,
                                     int res_bits, uint8_t *d_e4_codes,
                                     float *d_e4_qparams);
 
+/* Graph construction on the device, bit-identical to the reference's
+ * search-build with the exact candidate provider and one iteration
+ * (retrieval_search_build_graph_ex, retrieval_search_build_core.c:1286-1455
+ * with retrieval_search_build_exact_provider, :314-361): per node the
+ * `candidates` closest others by exact squared L2 (sequential f32 sum,
+ * ties by id) merged with the node's random initial row (LCG from `seed`),
+ * then the RNG occlusion rule (prune_mode 1) or the closest r (0), fill
+ * from the remaining candidates, random fill, the row by (distance, id).
+ * d_x [n][dim] and d_graph [n][r] are device pointers; r and candidates
+ * <= 64, r < n; synchronous. */
+AEEResult rover_build_graph_exact(const float *d_x, int n, int dim, int r,
+                                  int candidates, uint32_t seed, int prune_mode,
+                                  int32_t *d_graph);
+
 /* Device memory helpers, so callers need no HIP headers. */
 AEEResult rover_device_alloc(void **d_ptr, size_t bytes);
 AEEResult rover_device_free(void *d_ptr);

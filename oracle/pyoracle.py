@@ -71,6 +71,9 @@ def oracle():
         L.orc_fingerprint.restype = C.c_uint64
         L.orc_search_batch.argtypes = [P, I, P, I, P, I, I, I, P, P, I, I, I, P,
                                        P, P, P]
+        L.orc_build_exact.argtypes = [P, I, I, I, I, C.c_uint32, I, P]
+        L.orc_l2.argtypes = [P, I, I, I]
+        L.orc_l2.restype = F
         L.orc_sqmd_search.argtypes = [P, I, P, P, I, I, I, I, P, P, I, P]
         L.orc_sqmd_seed_ids.argtypes = [I, I, I, C.c_uint32, P, I]
         L.orc_sq_residual_distance.argtypes = [P, P, I]
@@ -119,6 +122,7 @@ def reference():
                                            P, I, P]
         L.ref_results_write.argtypes = [C.c_char_p, I, C.c_char_p, I, P, P, P, I]
         L.ref_nsg_load.argtypes = [C.c_char_p, I, I, P, P]
+        L.ref_build_exact.argtypes = [P, I, I, I, I, C.c_uint, I, I, P]
         L.ref_sqmd_search.argtypes = [P, I, I, P, I, I, F, F, P, P, I, I, P, I, I,
                                       C.c_uint, I, I, I, P, P, I, P]
         _ref = L
@@ -346,6 +350,16 @@ def sqmd_seed_ids(n, entry_points, graph_entry, seed=42):
 def sq_residual_distance(index, query, vid):
     q = np.ascontiguousarray(query, dtype=np.float32)
     return oracle().orc_sq_residual_distance(C.byref(index.struct), _p(q), vid)
+
+
+def build_exact(base, r, candidates, seed=42, prune_mode=1):
+    """The exact-candidate graph build; int32 [n][r]."""
+    x = np.ascontiguousarray(base, dtype=np.float32)
+    n, dim = x.shape
+    out = np.zeros((n, r), dtype=np.int32)
+    rc = oracle().orc_build_exact(_p(x), n, dim, r, candidates, seed, prune_mode, _p(out))
+    assert rc == 0
+    return out
 
 
 def seed_ids(n, entry_count, graph_entry, seed=42):

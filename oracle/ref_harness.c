@@ -21,6 +21,7 @@
 #include "dsp_demo/retrieval_build_artifacts.h"
 #include "dsp_demo/retrieval_final_refine.h"
 #include "dsp_demo/retrieval_nsg_loader.h"
+#include "dsp_demo/retrieval_search_build_core.h"
 #include "dsp_demo/retrieval_pq_artifact.h"
 #include "dsp_demo/retrieval_raw_reader.h"
 #include "dsp_demo/retrieval_l1_int8_core.h"
@@ -682,5 +683,34 @@ int ref_nsg_load(const char *path, int n, int width, int32_t *graph, int64_t *st
     stats[1] = st.entry_point;
     stats[2] = (int64_t)st.truncated_nodes;
     stats[3] = (int64_t)st.truncated_edges;
+    return rc;
+}
+
+/* ---- graph construction: retrieval_search_build_graph_ex with the exact
+ * provider (brute-force candidates), one iteration ---- */
+int ref_build_exact(const float *base, int n, int dim, int r, int candidates,
+                    unsigned int seed, int prune_mode, int threads, int32_t *out) {
+    RetrievalSearchBuildConfig cfg;
+    RetrievalSearchBuildRuntime rt;
+    RetrievalRnnGraph g;
+    int rc;
+    memset(&cfg, 0, sizeof(cfg));
+    memset(&rt, 0, sizeof(rt));
+    memset(&g, 0, sizeof(g));
+    cfg.r = r;
+    cfg.iterations = 1;
+    cfg.candidate_count = candidates;
+    cfg.entry_points = 1;
+    cfg.expand_count = 1;
+    cfg.pool_size = candidates;
+    cfg.seed = seed;
+    cfg.prune_mode = prune_mode;
+    cfg.batch_align = 1;
+    rt.threads = threads;
+    rt.search = retrieval_search_build_exact_provider;
+    rc = retrieval_search_build_graph_ex(base, n, dim, &cfg, &rt, &g, NULL);
+    if (rc) return rc;
+    rc = retrieval_rnn_export_topk(&g, r, out);
+    retrieval_rnn_graph_free(&g);
     return rc;
 }

@@ -951,6 +951,131 @@ out:
     return rc;
 }
 
+/* ---- graph construction (retrieval_search_build_core.c) ---- */
+
+float orc_l2(const float *base, int dim, int a, int b) {
+    const float *va = base + (size_t)a * dim, *vb = base + (size_t)b * dim;
+    float acc = 0.0f;
+    for (int d = 0; d < dim; d++) {
+        float diff = va[d] - vb[d];
+        acc += diff * diff;
+    }
+    return acc;
+}
+
+static uint32_t orc_build_lcg(uint32_t x) { return x * 1664525u + 1013904223u; }
+
+/* :108-113 (sb_seed_for_node) */
+static uint32_t orc_node_seed(uint32_t seed, int node) {
+    uint32_t x = seed ^ (uint32_t)(node + 1) * 0x9e3779b9u;
+    x = orc_build_lcg(x);
+    x ^= x >> 16;
+    return orc_build_lcg(x);
+}
+
+typedef struct {
+    int id;
+    float d;
+} OrcEdge;
+
+/* (distance, id) ascending (:115-131 and rnn_edge_is_better) */
+static int orc_edge_cmp(const void *a, const void *b) {
+    const OrcEdge *x = a, *y = b;
+    if (x->d < y->d) return -1;
+    if (x->d > y->d) return 1;
+    return (x->id > y->id) - (x->id < y->id);
+}
+
+static int orc_edges_contain(const OrcEdge *e, int count, int id) {
+    for (int i = 0; i < count; i++)
+        if (e[i].id == id) return 1;
+    return 0;
+}
+
+/* Draws from the node's LCG until `row` holds r distinct ids other than u
+ * (at most n*n draws), then fills with the lowest unused ids (:682-776 and
+ * :781-831 share this shape). */
+static void orc_random_fill(const float *base, int n, int dim, int r, uint32_t state,
+                            int u, OrcEdge *row, int *count) {
+    uint64_t attempts = 0, limit = (uint64_t)(uint32_t)n * (uint64_t)(uint32_t)n;
+    while (*count < r && attempts < limit) {
+        int v;
+        state = orc_build_lcg(state);
+        v = (int)(state % (uint32_t)n);
+        if (v != u && !orc_edges_contain(row, *count, v)) {
+            row[*count].id = v;
+            row[*count].d = orc_l2(base, dim, u, v);
+            (*count)++;
+        }
+        attempts++;
+    }
+    for (int v = 0; v < n && *count < r; v++) {
+        if (v == u || orc_edges_contain(row, *count, v)) continue;
+        row[*count].id = v;
+        row[*count].d = orc_l2(base, dim, u, v);
+        (*count)++;
+    }
+}
+
+int orc_build_exact(const float *base, int n, int dim, int r, int candidates,
+                    uint32_t seed, int prune_mode, int32_t *out) {
+    OrcEdge *all, *init, *items, *row;
+    int rc = -1;
+    if (!base || n <= 1 || dim <= 0 || r <= 0 || r >= n || candidates < r ||
+        (prune_mode != 0 && prune_mode != 1) || !out)
+        return -1;
+    all = malloc((size_t)n * sizeof(*all));
+    init = malloc((size_t)r * sizeof(*init));
+    items = malloc((size_t)(candidates + r) * sizeof(*items));
+    row = malloc((size_t)r * sizeof(*row));
+    if (!all || !init || !items || !row) goto done;
+    for (int u = 0; u < n; u++) {
+        int ni = 0, na = 0, count = 0, deg = 0, cap = candidates + r;
+        /* random initial row, sorted */
+        orc_random_fill(base, n, dim, r, orc_node_seed(seed, u), u, init, &ni);
+        qsort(init, (size_t)ni, sizeof(*init), orc_edge_cmp);
+        /* exact candidates: every other node by (distance, id), first C */
+        for (int v = 0; v < n; v++) {
+            if (v == u) continue;
+            all[na].id = v;
+            all[na].d = orc_l2(base, dim, u, v);
+            na++;
+        }
+        qsort(all, (size_t)na, sizeof(*all), orc_edge_cmp);
+        if (na > candidates) na = candidates;
+        /* candidates then the previous row, first occurrence kept */
+        for (int i = 0; i < na && count < cap; i++)
+            if (!orc_edges_contain(items, count, all[i].id)) items[count++] = all[i];
+        for (int i = 0; i < ni && count < cap; i++)
+            if (!orc_edges_contain(items, count, init[i].id)) items[count++] = init[i];
+        qsort(items, (size_t)count, sizeof(*items), orc_edge_cmp);
+        if (prune_mode == 0) {
+            for (int i = 0; i < count && deg < r; i++) row[deg++] = items[i];
+        } else {
+            for (int i = 0; i < count && deg < r; i++) {
+                int occluded = 0;
+                for (int j = 0; j < deg && !occluded; j++)
+                    occluded = orc_l2(base, dim, row[j].id, items[i].id) < items[i].d;
+                if (!occluded) row[deg++] = items[i];
+            }
+            for (int i = 0; i < count && deg < r; i++)
+                if (!orc_edges_contain(row, deg, items[i].id)) row[deg++] = items[i];
+        }
+        if (deg < r)
+            orc_random_fill(base, n, dim, r, orc_node_seed(seed ^ 0xa5a55a5au, u), u, row,
+                            &deg);
+        qsort(row, (size_t)deg, sizeof(*row), orc_edge_cmp);
+        for (int i = 0; i < r; i++) out[(size_t)u * r + i] = i < deg ? row[i].id : -1;
+    }
+    rc = 0;
+done:
+    free(all);
+    free(init);
+    free(items);
+    free(row);
+    return rc;
+}
+
 /* retrieval_rpc_demo_support.c:1709-1728 (demo_recall_hits_at_k) */
 int orc_recall_hits(const int *res, int res_count, const int32_t *gt,
                     int gt_k, int k) {

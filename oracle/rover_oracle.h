@@ -120,6 +120,19 @@ int orc_sqmd_seed_ids(int num_vectors, int entry_points, int graph_entry,
  * (retrieval_demo_exact_distance_residual_scalar). */
 float orc_sq_residual_distance(const OrcIndex *index, const float *query, int id);
 
+/* Graph construction as retrieval_search_build_graph_ex builds it with the
+ * exact provider and one iteration (retrieval_search_build_core.c:682-776
+ * random initial rows, :314-361 exact candidates, :833-968 merge / prune /
+ * fill, :781-831 random fill, retrieval_rnn_descent_core.c:438-470 row
+ * order).  prune_mode 0 keeps the closest r candidates, 1 applies the RNG
+ * occlusion rule.  out [n][r].  Returns 0 or -1. */
+int orc_build_exact(const float *base, int n, int dim, int r, int candidates,
+                    uint32_t seed, int prune_mode, int32_t *out);
+
+/* retrieval_rnn_descent_core.c:192-212: sequential f32 sum of squared
+ * differences. */
+float orc_l2(const float *base, int dim, int a, int b);
+
 int orc_recall_hits(const int *result_ids, int result_count,
                     const int32_t *gt_row, int gt_k, int k);
 uint64_t orc_fingerprint(uint64_t hash, const int *ids,

@@ -1,0 +1,349 @@
+// rover_build.hip -- CDNA4 (gfx950) graph construction: the reference's
+// search-build with the exact candidate provider and one iteration
+// (retrieval_search_build_core.c:1286-1455), bit for bit.
+//
+//   rv_knn_exact_kernel   exact squared L2 from every node to every other
+//                         (retrieval_rnn_descent_core.c:192-212: sequential
+//                         f32 sum of (a - b)^2 over the dimensions, no
+//                         contraction) and the 64 closest per node by
+//                         (distance, id) (:314-361 sorts all of them).  A
+//                         workgroup holds 8 nodes (2 per wavefront) and
+//                         streams 64-node column tiles through LDS; each lane
+//                         owns one column and computes its distance to the
+//                         wave's 2 nodes with 16-byte LDS reads; the wave
+//                         keeps each node's best 64 as one 64-bit key per
+//                         lane, merging a tile by a bitonic sort and a
+//                         6-step bitonic merge (skipped when no key of the
+//                         tile beats the current 64th).
+//   rv_build_rows_kernel  per node, one wavefront: the random initial row
+//                         (:682-776), the candidate list (exact candidates
+//                         then the initial row, first occurrence kept), the
+//                         RNG occlusion rule or the closest r (:904-949),
+//                         fill from the remaining candidates, random fill
+//                         (:781-831), the row in (distance, id) order.
+//
+// Bound: VALU (the exact distances are sequential per pair and cannot use
+// the matrix cores without changing their rounding).
+#include <hip/hip_runtime.h>
+
+#include <float.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include "rover_hip.h"
+#include "rv_wave.h"
+
+#define RB_WAVE 64
+#define RB_WAVES 4         // wavefronts per workgroup in the kNN kernel
+#define RB_ROWS 2          // nodes per wavefront in the kNN kernel
+#define RB_TILE 64         // column nodes per tile
+#define RB_KEY_MAX 0xffffffffffffffffull
+
+static int rb_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) {
+        char msg[256];
+        snprintf(msg, sizeof(msg), "%s: %s", what, hipGetErrorString(e));
+        rv_set_error(msg);
+        return -1;
+    }
+    return 0;
+}
+
+__device__ __forceinline__ uint64_t rb_key(float d, int id) {
+    return ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)id;
+}
+
+// Merges 64 new keys into the ascending 64-key list `cur` (one key per
+// lane), keeping the 64 smallest, ascending.
+__device__ __forceinline__ uint64_t rb_merge64(uint64_t cur, uint64_t fresh) {
+    fresh = rv_sort_wave(fresh);
+    const uint64_t rev = ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(fresh >> 32), 63 - __lane_id())
+                          << 32) |
+                         (uint32_t)__shfl((int)(uint32_t)fresh, 63 - __lane_id());
+    uint64_t m = cur < rev ? cur : rev;  // bitonic: the 64 smallest
+    m = rv_bitonic_step<64, 32>(m);
+    m = rv_bitonic_step<64, 16>(m);
+    m = rv_bitonic_step<64, 8>(m);
+    m = rv_bitonic_step<64, 4>(m);
+    m = rv_bitonic_step<64, 2>(m);
+    m = rv_bitonic_step<64, 1>(m);
+    return m;
+}
+
+// Exact squared L2 of two rows in global memory, the reference's order.
+__device__ __forceinline__ float rb_l2(const float *x, int dim, int a, int b) {
+    const float *va = x + (size_t)a * dim, *vb = x + (size_t)b * dim;
+    float acc = 0.0f;
+    for (int d = 0; d < dim; d++) {
+        const float diff = va[d] - vb[d];
+        acc = acc + diff * diff;
+    }
+    return acc;
+}
+
+// ---------------------------------------------------------------- kNN
+
+// VEC: dimensions read 4 at a time (dim % 4 == 0); the tile row stride is
+// dim + 4 floats so the 16-byte reads of 16 consecutive lanes fall in
+// distinct banks.
+template <bool VEC>
+__global__ __launch_bounds__(RB_WAVE *RB_WAVES) void rv_knn_exact_kernel(const float *x, int n,
+                                                                         int dim,
+                                                                         uint64_t *keys) {
+    extern __shared__ __align__(16) float lds[];
+    const int stride = VEC ? dim + 4 : dim + 1;
+    float *tile = lds;                                   // [64][stride]
+    float *rows = lds + RB_TILE * stride;                // [8][dim]
+    const int lane = (int)__lane_id();
+    const int wave = (int)threadIdx.x / RB_WAVE;
+    const int u0 = blockIdx.x * (RB_WAVES * RB_ROWS);
+    for (int i = threadIdx.x; i < RB_WAVES * RB_ROWS * dim; i += blockDim.x) {
+        const int r = i / dim, u = u0 + r;
+        rows[i] = u < n ? x[(size_t)u * dim + (i % dim)] : 0.0f;
+    }
+    const int ua = u0 + wave * RB_ROWS, ub = ua + 1;
+    const float *ra = rows + (wave * RB_ROWS) * dim;
+    const float *rbv = ra + dim;
+    uint64_t best_a = RB_KEY_MAX, best_b = RB_KEY_MAX;
+    for (int c0 = 0; c0 < n; c0 += RB_TILE) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < RB_TILE * dim; i += blockDim.x) {
+            const int c = i / dim, v = c0 + c;
+            tile[c * stride + (i % dim)] = v < n ? x[(size_t)v * dim + (i % dim)] : 0.0f;
+        }
+        __syncthreads();
+        const int v = c0 + lane;
+        const float *col = tile + lane * stride;
+        float da = 0.0f, db = 0.0f;
+        if (VEC) {
+            for (int d = 0; d < dim; d += 4) {
+                const float4 c = *reinterpret_cast<const float4 *>(col + d);
+                const float4 a = *reinterpret_cast<const float4 *>(ra + d);
+                const float4 b = *reinterpret_cast<const float4 *>(rbv + d);
+                float t;
+                t = a.x - c.x; da = da + t * t;
+                t = a.y - c.y; da = da + t * t;
+                t = a.z - c.z; da = da + t * t;
+                t = a.w - c.w; da = da + t * t;
+                t = b.x - c.x; db = db + t * t;
+                t = b.y - c.y; db = db + t * t;
+                t = b.z - c.z; db = db + t * t;
+                t = b.w - c.w; db = db + t * t;
+            }
+        } else {
+            for (int d = 0; d < dim; d++) {
+                const float c = col[d];
+                float t = ra[d] - c;
+                da = da + t * t;
+                t = rbv[d] - c;
+                db = db + t * t;
+            }
+        }
+        const bool in = v < n;
+        const uint64_t ka = in && v != ua ? rb_key(da, v) : RB_KEY_MAX;
+        const uint64_t kb = in && v != ub ? rb_key(db, v) : RB_KEY_MAX;
+        // the current 64th key is lane 63's; merge only when the tile can
+        // improve the list
+        const uint64_t lim_a = ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(best_a >> 32), 63) << 32) |
+                               (uint32_t)__shfl((int)(uint32_t)best_a, 63);
+        const uint64_t lim_b = ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(best_b >> 32), 63) << 32) |
+                               (uint32_t)__shfl((int)(uint32_t)best_b, 63);
+        if (__ballot(ka < lim_a)) best_a = rb_merge64(best_a, ka);
+        if (__ballot(kb < lim_b)) best_b = rb_merge64(best_b, kb);
+    }
+    if (ua < n) keys[(size_t)ua * RB_WAVE + lane] = best_a;
+    if (ub < n) keys[(size_t)ub * RB_WAVE + lane] = best_b;
+}
+
+// -------------------------------------------------------------- rows
+
+__device__ __forceinline__ uint32_t rb_lcg(uint32_t x) { return x * 1664525u + 1013904223u; }
+
+// retrieval_search_build_core.c:108-113 (sb_seed_for_node)
+__device__ __forceinline__ uint32_t rb_node_seed(uint32_t seed, int node) {
+    uint32_t x = seed ^ (uint32_t)(node + 1) * 0x9e3779b9u;
+    x = rb_lcg(x);
+    x ^= x >> 16;
+    return rb_lcg(x);
+}
+
+// Lane 0 only: draws from the node's LCG until ids[0..*count) holds r
+// distinct ids other than u (at most n*n draws), then the lowest unused
+// ids (:682-776 and :781-831 share this shape).  Distances are filled in
+// by the caller.
+__device__ static void rb_draw(int n, int r, uint32_t state, int u, int32_t *ids, int *count) {
+    const uint64_t limit = (uint64_t)(uint32_t)n * (uint64_t)(uint32_t)n;
+    uint64_t attempts = 0;
+    while (*count < r && attempts < limit) {
+        state = rb_lcg(state);
+        const int v = (int)(state % (uint32_t)n);
+        bool dup = v == u;
+        for (int i = 0; i < *count && !dup; i++) dup = ids[i] == v;
+        if (!dup) ids[(*count)++] = v;
+        attempts++;
+    }
+    for (int v = 0; v < n && *count < r; v++) {
+        bool dup = v == u;
+        for (int i = 0; i < *count && !dup; i++) dup = ids[i] == v;
+        if (!dup) ids[(*count)++] = v;
+    }
+}
+
+__global__ __launch_bounds__(RB_WAVE) void rv_build_rows_kernel(const float *x, int n, int dim,
+                                                                int r, int cand, uint32_t seed,
+                                                                int prune_mode,
+                                                                const uint64_t *knn,
+                                                                int32_t *graph) {
+    __shared__ int32_t s_ids[RB_WAVE];        // drawn ids (initial row / random fill)
+    __shared__ uint64_t s_lists[2 * RB_WAVE];  // exact candidates | initial row
+    __shared__ uint64_t s_items[2 * RB_WAVE];  // candidate keys, ascending
+    __shared__ uint64_t s_row[RB_WAVE];        // accepted keys
+    __shared__ int s_count;
+    const int lane = (int)__lane_id();
+    for (int u = blockIdx.x; u < n; u += gridDim.x) {
+        // ---- the random initial row, sorted
+        if (lane == 0) {
+            int cnt = 0;
+            rb_draw(n, r, rb_node_seed(seed, u), u, s_ids, &cnt);
+            s_count = cnt;
+        }
+        __syncthreads();
+        const int ni = s_count;
+        uint64_t init = lane < ni ? rb_key(rb_l2(x, dim, u, s_ids[lane]), s_ids[lane])
+                                  : RB_KEY_MAX;
+        init = rv_sort_wave(init);
+        const uint64_t prov = lane < cand ? knn[(size_t)u * RB_WAVE + lane] : RB_KEY_MAX;
+        __syncthreads();
+        s_lists[lane] = prov;
+        s_lists[RB_WAVE + lane] = init;
+        __syncthreads();
+        // ---- candidates: merge the two ascending lists, dropping the
+        // initial row's ids that are exact candidates (same key)
+        if (lane == 0) {
+            int i = 0, j = 0, cnt = 0;
+            while (i < cand || j < ni) {
+                const uint64_t a = i < cand ? s_lists[i] : RB_KEY_MAX;
+                const uint64_t b = j < ni ? s_lists[RB_WAVE + j] : RB_KEY_MAX;
+                if (a == RB_KEY_MAX && b == RB_KEY_MAX) break;
+                if (a < b) {
+                    s_items[cnt++] = a, i++;
+                } else if (b < a) {
+                    s_items[cnt++] = b, j++;
+                } else {
+                    s_items[cnt++] = a, i++, j++;
+                }
+            }
+            s_count = cnt;
+        }
+        __syncthreads();
+        const int count = s_count;
+        int deg = 0;
+        if (prune_mode == 0) {
+            deg = count < r ? count : r;
+            if (lane < deg) s_row[lane] = s_items[lane];
+        } else {
+            // RNG occlusion: a candidate is kept unless an accepted neighbor
+            // is strictly closer to it than u is
+            for (int i = 0; i < count && deg < r; i++) {
+                const uint64_t key = s_items[i];
+                const int c = (int)(uint32_t)key;
+                const float dc = __uint_as_float((uint32_t)(key >> 32));
+                bool occ = false;
+                if (lane < deg) occ = rb_l2(x, dim, (int)(uint32_t)s_row[lane], c) < dc;
+                if (__ballot(occ) == 0ull) {
+                    if (lane == 0) s_row[deg] = key;
+                    deg++;
+                }
+                __syncthreads();
+            }
+            // fill from the remaining candidates in order
+            for (int i = 0; i < count && deg < r; i++) {
+                const uint64_t key = s_items[i];
+                const bool has = lane < deg && s_row[lane] == key;
+                if (__ballot(has) == 0ull) {
+                    if (lane == 0) s_row[deg] = key;
+                    deg++;
+                }
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+        // ---- random fill when fewer than r
+        if (deg < r) {
+            if (lane == 0) {
+                int cnt = deg;
+                for (int k = 0; k < deg; k++) s_ids[k] = (int)(uint32_t)s_row[k];
+                rb_draw(n, r, rb_node_seed(seed ^ 0xa5a55a5au, u), u, s_ids, &cnt);
+                s_count = cnt;
+            }
+            __syncthreads();
+            const int total = s_count;
+            if (lane >= deg && lane < total) s_row[lane] = rb_key(rb_l2(x, dim, u, s_ids[lane]),
+                                                                  s_ids[lane]);
+            deg = total;
+            __syncthreads();
+        }
+        // ---- the row in (distance, id) order
+        uint64_t rowk = lane < deg ? s_row[lane] : RB_KEY_MAX;
+        rowk = rv_sort_wave(rowk);
+        if (lane < r) graph[(size_t)u * r + lane] = lane < deg ? (int32_t)(uint32_t)rowk : -1;
+        __syncthreads();
+    }
+}
+
+extern "C" int rv_build_exact(const float *x, int n, int dim, int r, int cand, uint32_t seed,
+                              int prune_mode, int32_t *graph, void *stream) {
+    if (!x || !graph || n <= 1 || dim <= 0 || dim > 4096 || r <= 0 || r >= n ||
+        r > RB_WAVE || cand < r || cand > RB_WAVE || (prune_mode != 0 && prune_mode != 1)) {
+        rv_set_error("rv_build_exact: bad arguments");
+        return -1;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    uint64_t *keys = nullptr;
+    if (rb_check(hipMalloc((void **)&keys, (size_t)n * RB_WAVE * sizeof(uint64_t)),
+                 "hipMalloc knn"))
+        return -1;
+    const bool vec = dim % 4 == 0;
+    const int stride = vec ? dim + 4 : dim + 1;
+    const size_t lds = ((size_t)RB_TILE * stride + (size_t)RB_WAVES * RB_ROWS * dim) *
+                       sizeof(float);
+    const int blocks = (n + RB_WAVES * RB_ROWS - 1) / (RB_WAVES * RB_ROWS);
+    int rc = 0;
+    if (lds > 160 * 1024) {
+        rv_set_error("rv_build_exact: dimension too large for the LDS tile");
+        rc = -1;
+    } else if (vec) {
+        rc = rb_check(hipFuncSetAttribute((const void *)rv_knn_exact_kernel<true>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          160 * 1024), "hipFuncSetAttribute");
+        if (!rc) {
+            rv_knn_exact_kernel<true><<<blocks, RB_WAVE * RB_WAVES, lds, s>>>(x, n, dim, keys);
+            rc = rb_check(hipGetLastError(), "rv_knn_exact_kernel");
+        }
+    } else {
+        rc = rb_check(hipFuncSetAttribute((const void *)rv_knn_exact_kernel<false>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          160 * 1024), "hipFuncSetAttribute");
+        if (!rc) {
+            rv_knn_exact_kernel<false><<<blocks, RB_WAVE * RB_WAVES, lds, s>>>(x, n, dim, keys);
+            rc = rb_check(hipGetLastError(), "rv_knn_exact_kernel");
+        }
+    }
+    if (!rc) {
+        int dev = 0, cus = 0;
+        rc = rb_check(hipGetDevice(&dev), "hipGetDevice") ||
+             rb_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
+                      "hipDeviceGetAttribute");
+        if (!rc) {
+            int grid = cus * 16;
+            if (grid > n) grid = n;
+            rv_build_rows_kernel<<<grid, RB_WAVE, 0, s>>>(x, n, dim, r, cand, seed, prune_mode,
+                                                          keys, graph);
+            rc = rb_check(hipGetLastError(), "rv_build_rows_kernel");
+        }
+    }
+    // the scratch is released once the kernels are done with it
+    const int sync_rc = rb_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    (void)hipFree(keys);
+    return rc ? rc : sync_rc;
+}

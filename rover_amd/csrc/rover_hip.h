@@ -168,6 +168,17 @@ int rv_sqmd_grid(const RvIndex *index, const RvSqmdBatch *batch, int *lds_fronti
 int rv_sqmd_launch(const RvIndex *index, const RvSqmdBatch *batch, int grid_waves,
                    void *stream);
 
+/* Graph construction on the device, bit-identical to
+ * retrieval_search_build_graph_ex with the exact provider and one iteration:
+ * exact candidates (the `candidates` closest by (distance, id)), the
+ * node's random initial row, RNG occlusion (prune_mode 1) or the closest r
+ * (0), fill, random fill, rows by (distance, id).  r, candidates <= 64.
+ * d_graph [n][r].  Asynchronous on `stream`; allocates a scratch of
+ * n * 512 bytes (freed before returning, after a synchronize). */
+int rv_build_exact(const float *d_x, int n, int dim, int r, int candidates,
+                   uint32_t seed, int prune_mode, int32_t *d_graph, void *stream);
+void rv_set_error(const char *msg);
+
 int rv_device_count(void);
 int rv_set_device(int device);
 int rv_malloc(void **d_ptr, size_t bytes);

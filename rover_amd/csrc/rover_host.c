@@ -1555,6 +1555,19 @@ AEEResult rover_build_error4_encode(const float *d_x, const uint8_t *d_codes,
     return ROVER_SUCCESS;
 }
 
+AEEResult rover_build_graph_exact(const float *d_x, int n, int dim, int r,
+                                  int candidates, uint32_t seed, int prune_mode,
+                                  int32_t *d_graph) {
+    if (!d_x || !d_graph || n <= 1 || dim <= 0 || dim > 4096 || r <= 0 || r >= n ||
+        r > 64 || candidates < r || candidates > 64 || (prune_mode != 0 && prune_mode != 1))
+        return ROVER_EBADPARM;
+    if (rv_build_exact(d_x, n, dim, r, candidates, seed, prune_mode, d_graph, NULL)) {
+        set_err("rover_build_graph_exact");
+        return ROVER_EFAILED;
+    }
+    return ROVER_SUCCESS;
+}
+
 AEEResult rover_build_residual_encode(const float *d_x, const uint8_t *d_codes,
                                       int n, int dim, int m, int ks,
                                       const float *d_codebook, int bits,

@@ -58,6 +58,10 @@ static int rv_check(hipError_t e, const char *what) {
 
 extern "C" const char *rv_last_error(void) { return g_rv_error; }
 
+extern "C" void rv_set_error(const char *msg) {
+    snprintf(g_rv_error, sizeof(g_rv_error), "%s", msg);
+}
+
 // ---------------------------------------------------------------- wave utils
 
 __device__ __forceinline__ int rv_lane() { return (int)__lane_id(); }

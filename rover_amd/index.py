@@ -680,3 +680,17 @@ def encode_error4_on_device(x, codes, codebook, res_codes, res_qparams, res_bits
     if not np.isfinite(e4qp).all():
         raise ValueError("non-finite reconstruction error")
     return e4, e4qp
+
+
+def build_graph_exact(x, R, candidates=64, seed=42, prune_mode=1):
+    """The reference's exact-candidate graph build on the device
+    (rover_build_graph_exact): int32 [n, R], rows by (distance, id)."""
+    from . import native
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    n, dim = x.shape
+    dx = native.DeviceBuffer.from_array(x)
+    dg = native.DeviceBuffer(n * R * 4)
+    native.check(native.lib().rover_build_graph_exact(dx.ptr, n, dim, R, candidates, seed,
+                                                      prune_mode, dg.ptr),
+                 "rover_build_graph_exact")
+    return dg.download(np.int32, n * R).reshape(n, R)

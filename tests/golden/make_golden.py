@@ -11,7 +11,8 @@ C: retrieval_pure_pq_search_scalar_with_policy, retrieval_pq_build_lut,
 retrieval_pq_adc_ids_batch_scalar, retrieval_pq_train/encode_batch, the
 PQRES01 residual writer/reader and its evaluator/decoder,
 retrieval_final_refine (CURRENT and ERROR4), retrieval_build_error4, the
-live SQ traversal (sqmd_cpu_state_run_fake_dsp, retrieval_sqmd_make_seed_ids), the SQ quantize/distance kernels and
+live SQ traversal (sqmd_cpu_state_run_fake_dsp, retrieval_sqmd_make_seed_ids),
+the exact-provider graph build (retrieval_search_build_graph_ex), the SQ quantize/distance kernels and
 retrieval_sqmd_make_seed_ids.  Inputs are small synthetic indexes; base
 vectors are multiples of 1/16 so they are stored exactly as uint16.
 """
@@ -291,6 +292,29 @@ def sqmd_case(name, pq_name="pq_d128_m32", sq_name="sq_d128"):
     print("wrote", path, os.path.getsize(path), "bytes")
 
 
+def graph_case(name):
+    """The reference's search-build with the exact provider
+    (retrieval_search_build_graph_ex, one iteration) on small inputs:
+    integer-coordinate data (distance ties) and a mixture, prune modes 1 and
+    0.  Writes <name>.npz."""
+    ref = pyoracle.reference()
+    out, cases = {}, []
+    specs = ((600, 16, 16, 32, 1, 4, True), (600, 30, 12, 24, 0, 4, True),
+             (1000, 32, 24, 64, 1, 16, False), (400, 8, 8, 8, 1, 2, True))
+    for ci, (n, dim, r, cand, prune, comps, integer) in enumerate(specs):
+        base, _ = data.workload(n, dim, comps, 8.0, 1, base_seed=70 + ci, query_seed=80 + ci)
+        base = np.ascontiguousarray(np.round(base) if integer else base, dtype=np.float32)
+        g = np.zeros((n, r), dtype=np.int32)
+        assert ref.ref_build_exact(_p(base), n, dim, r, cand, 42, prune, 4, _p(g)) == 0
+        out["base%d" % ci] = base
+        out["graph%d" % ci] = g
+        cases.append((n, dim, r, cand, prune, 42))
+    out["cases"] = np.array(cases, dtype=np.int64)
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
 def error4_case(name):
     """Third-level refine for an existing PQ fixture: the reference's
     retrieval_build_error4 writes the PQERR4V1 artifact (its residual decoder
@@ -356,6 +380,9 @@ def main():
                    check=True)
     if not pyoracle.ref_available():
         sys.exit("reference build missing: oracle/_ref/librover_ref.so")
+    if sys.argv[1:] == ["graph"]:
+        graph_case("graph_build")
+        return
     if sys.argv[1:] == ["sqmd"]:
         sqmd_case("sqmd_d128")
         return
@@ -375,6 +402,7 @@ def main():
     for name in ("pq_d128_m32", "pq_d96_m48", "pq_d64_ks16"):
         error4_case(name)
     sqmd_case("sqmd_d128")
+    graph_case("graph_build")
 
 
 if __name__ == "__main__":
