@@ -149,13 +149,14 @@ def sq_leg_run(args, n=100_000, nq=1000, L=64, k=10, ep=32):
     (distance-only heaps; plain and with the exact residual correction) and
     the best-first SQ search over the same index, HIP events around single
     launches with queries in HBM; the oracle's SQ traversal on the host for
-    comparison."""
-    import torch
+    comparison.  The graph is built on the device by the reference's
+    exact-candidate search-build (exact 64-NN, RNG prune to 32)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     base, queries = data.workload(n, 128, 256, 8.0, nq, base_seed=7, query_seed=8)
-    graph = index.build_graph(base, 32, device=torch.device("cuda:0"))
-    torch.cuda.synchronize()
+    t_build = time.time()
+    graph = index.build_graph_exact(base, 32, 64, 42, 1)  # exact 64-NN + RNG prune
+    t_build = time.time() - t_build
     entry = index.medoid(base)
     scale, zero = index.sq_params(base, 8)
     codes = index.encode_sq(base, 8, scale, zero)
@@ -169,7 +170,9 @@ def sq_leg_run(args, n=100_000, nq=1000, L=64, k=10, ep=32):
     d_ids, d_d = native.DeviceBuffer(nq * k * 4), native.DeviceBuffer(nq * k * 4)
     d_cnt, d_st = native.DeviceBuffer(nq * 4), native.DeviceBuffer(nq * 32)
     out = {"workload": "100k x 128 mixture (256 comps, sigma 8), SQ8 + 4-bit residuals, "
-                       "R=32, entry medoid, %d seeds, %d queries, L=%d, k=%d" % (ep, nq, L, k)}
+                       "R=32 (exact 64-NN + RNG prune, built on the device), entry medoid, "
+                       "%d seeds, %d queries, L=%d, k=%d" % (ep, nq, L, k),
+           "graph_build_s": round(t_build, 3)}
     oix = pyoracle.Index(graph, res_codes=res, res_qparams=rqp, res_bits=4, sq_codes=codes,
                          sq_bits=8, sq_scale=float(scale), sq_zero=float(zero), dim=128)
     seeds = pyoracle.sqmd_seed_ids(n, ep, entry, 42)

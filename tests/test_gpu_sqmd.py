@@ -202,11 +202,9 @@ def test_sqmd_small_sq_workload_vs_oracle(calc):
     SQ8 codes with 4-bit residuals, degree-32 graph, 32 live-path seeds,
     L = 64, k = 10, 1,000 queries; plain and corrected traversal against the
     oracle, recall reported."""
-    import torch
     say("case sqmd 100k: building")
     base, queries = data.workload(100_000, 128, 256, 8.0, 1000)
-    graph = rindex.build_graph(base, 32, device=torch.device("cuda:0"))
-    torch.cuda.synchronize()
+    graph = rindex.build_graph_exact(base, 32, 64, 42, 1)
     entry = rindex.medoid(base)
     scale, zero = rindex.sq_params(base, 8)
     codes = rindex.encode_sq(base, 8, scale, zero)
