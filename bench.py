@@ -239,6 +239,9 @@ def main():
     ap.add_argument("--refine", type=int, default=64,
                     help="refine candidates F of the ADC mode")
     ap.add_argument("--modes", default="online,adc")
+    ap.add_argument("--only-sq", action="store_true",
+                    help="run only the SQ leg (graph build and SQ searches) and print its "
+                         "JSON; a profiling aid, not the benchmark line")
     ap.add_argument("--no-sq", action="store_true",
                     help="skip the SQ leg (100k x 128 SQ8, L=64: the reference's SQ "
                          "traversal, plain and corrected, and the best-first SQ search)")
@@ -283,6 +286,9 @@ def main():
               "(queries are independent; multi-GPU is data parallel over queries)",
               file=sys.stderr)
         sys.exit(2)
+    if args.only_sq:
+        print(json.dumps({"sq": sq_leg_run(args)}), flush=True)
+        return
     sweep_Ls = [int(v) for v in args.Ls.split(",")]
     if min(sweep_Ls) < args.k:
         ap.error("every beam width in --Ls must be >= k (%d)" % args.k)
