@@ -54,4 +54,9 @@ def test_device_build_100k():
         d = [pyoracle.oracle().orc_l2(x.ctypes.data, 128, u, int(v)) for v in g[u]]
         keys = list(zip(d, g[u].tolist()))
         assert keys == sorted(keys)
+        # nothing occludes the nearest neighbor, so it heads the row
+        near = np.argsort(((x.astype(np.float64) - x[u]) ** 2).sum(axis=1))[:6]
+        best = min((pyoracle.oracle().orc_l2(x.ctypes.data, 128, u, int(v)), int(v))
+                   for v in near if v != u)
+        assert best[1] == g[u][0]
     say("graph 100k x 128, R=32, 64 exact candidates: %.2f s" % el)
