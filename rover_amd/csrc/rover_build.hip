@@ -7,14 +7,15 @@
 //                         f32 sum of (a - b)^2 over the dimensions, no
 //                         contraction) and the 64 closest per node by
 //                         (distance, id) (:314-361 sorts all of them).  A
-//                         workgroup holds 8 nodes (2 per wavefront) and
-//                         streams 64-node column tiles through LDS; each lane
-//                         owns one column and computes its distance to the
-//                         wave's 2 nodes with 16-byte LDS reads; the wave
-//                         keeps each node's best 64 as one 64-bit key per
-//                         lane, merging a tile by a bitonic sort and a
-//                         6-step bitonic merge (skipped when no key of the
-//                         tile beats the current 64th).
+//                         workgroup holds 16 nodes (4 per wavefront) and
+//                         streams 64-node column tiles through LDS, 128
+//                         dimensions at a time; each lane owns one column
+//                         and computes its distance to the wave's 4 nodes
+//                         with 16-byte LDS reads; the wave keeps each node's
+//                         best 64 as one 64-bit key per lane, merging a tile
+//                         by a bitonic sort and a 6-step bitonic merge
+//                         (skipped when no key of the tile beats the current
+//                         64th).
 //   rv_build_rows_kernel  per node, one wavefront: the random initial row
 //                         (:682-776), the candidate list (exact candidates
 //                         then the initial row, first occurrence kept), the
@@ -35,7 +36,8 @@
 
 #define RB_WAVE 64
 #define RB_WAVES 4         // wavefronts per workgroup in the kNN kernel
-#define RB_ROWS 2          // nodes per wavefront in the kNN kernel
+#define RB_ROWS 4          // nodes per wavefront in the kNN kernel
+#define RB_DC 128          // dimensions per LDS chunk in the kNN kernel
 #define RB_TILE 64         // column nodes per tile
 #define RB_KEY_MAX 0xffffffffffffffffull
 
@@ -83,76 +85,111 @@ __device__ __forceinline__ float rb_l2(const float *x, int dim, int a, int b) {
 
 // ---------------------------------------------------------------- kNN
 
+// Copies rows [r0, r0 + count) x dimensions [d0, d0 + dc) of x into LDS
+// rows of `stride` floats (zeros past n).  Thread t handles element
+// t + k * blockDim.x of the (row, dimension-group) grid; the row and group
+// indexes advance by fixed steps, so the loop divides nothing.  VEC moves
+// 16 bytes per element (dc % 4 == 0).
+template <bool VEC>
+__device__ __forceinline__ void rb_stage(const float *x, int n, int dim, int r0, int count,
+                                         int d0, int dc, float *dst, int stride) {
+    const int nv = VEC ? dc >> 2 : dc;
+    const int total = count * nv;
+    const int step_r = (int)blockDim.x / nv, step_e = (int)blockDim.x - step_r * nv;
+    int r = (int)threadIdx.x / nv, e = (int)threadIdx.x - r * nv;
+    for (int i = threadIdx.x; i < total; i += blockDim.x) {
+        const int v = r0 + r;
+        if (VEC) {
+            float4 val = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (v < n) val = *reinterpret_cast<const float4 *>(x + (size_t)v * dim + d0 + 4 * e);
+            *reinterpret_cast<float4 *>(dst + r * stride + 4 * e) = val;
+        } else {
+            dst[r * stride + e] = v < n ? x[(size_t)v * dim + d0 + e] : 0.0f;
+        }
+        r += step_r;
+        e += step_e;
+        if (e >= nv) {
+            e -= nv;
+            r++;
+        }
+    }
+}
+
 // VEC: dimensions read 4 at a time (dim % 4 == 0); the tile row stride is
-// dim + 4 floats so the 16-byte reads of 16 consecutive lanes fall in
-// distinct banks.
+// RB_DC + 4 floats so the 16-byte reads of 16 consecutive lanes fall in
+// distinct banks (RB_DC + 1 for single floats).  The dimensions pass through
+// LDS in chunks of RB_DC in ascending order, and each lane carries its
+// RB_ROWS partial sums from chunk to chunk, so every distance is still one
+// sequential sum over the dimensions.
 template <bool VEC>
 __global__ __launch_bounds__(RB_WAVE *RB_WAVES) void rv_knn_exact_kernel(const float *x, int n,
                                                                          int dim,
                                                                          uint64_t *keys) {
-    extern __shared__ __align__(16) float lds[];
-    const int stride = VEC ? dim + 4 : dim + 1;
-    float *tile = lds;                                   // [64][stride]
-    float *rows = lds + RB_TILE * stride;                // [8][dim]
+    constexpr int NODES = RB_WAVES * RB_ROWS;
+    constexpr int STRIDE = VEC ? RB_DC + 4 : RB_DC + 1;
+    __shared__ __align__(16) float tile[RB_TILE * STRIDE];  // [64][STRIDE] columns
+    __shared__ __align__(16) float rows[NODES * RB_DC];     // [NODES][RB_DC] the nodes
     const int lane = (int)__lane_id();
     const int wave = (int)threadIdx.x / RB_WAVE;
-    const int u0 = blockIdx.x * (RB_WAVES * RB_ROWS);
-    for (int i = threadIdx.x; i < RB_WAVES * RB_ROWS * dim; i += blockDim.x) {
-        const int r = i / dim, u = u0 + r;
-        rows[i] = u < n ? x[(size_t)u * dim + (i % dim)] : 0.0f;
-    }
-    const int ua = u0 + wave * RB_ROWS, ub = ua + 1;
-    const float *ra = rows + (wave * RB_ROWS) * dim;
-    const float *rbv = ra + dim;
-    uint64_t best_a = RB_KEY_MAX, best_b = RB_KEY_MAX;
+    const int u0 = blockIdx.x * NODES;
+    const int nchunk = (dim + RB_DC - 1) / RB_DC;
+    const float *col = tile + lane * STRIDE;
+    const float *mine = rows + wave * RB_ROWS * RB_DC;
+    uint64_t best[RB_ROWS];
+#pragma unroll
+    for (int r = 0; r < RB_ROWS; r++) best[r] = RB_KEY_MAX;
     for (int c0 = 0; c0 < n; c0 += RB_TILE) {
-        __syncthreads();
-        for (int i = threadIdx.x; i < RB_TILE * dim; i += blockDim.x) {
-            const int c = i / dim, v = c0 + c;
-            tile[c * stride + (i % dim)] = v < n ? x[(size_t)v * dim + (i % dim)] : 0.0f;
+        float acc[RB_ROWS];
+#pragma unroll
+        for (int r = 0; r < RB_ROWS; r++) acc[r] = 0.0f;
+        for (int k = 0; k < nchunk; k++) {
+            const int d0 = k * RB_DC, dc = min(RB_DC, dim - d0);
+            __syncthreads();
+            rb_stage<VEC>(x, n, dim, c0, RB_TILE, d0, dc, tile, STRIDE);
+            if (nchunk > 1 || c0 == 0) rb_stage<VEC>(x, n, dim, u0, NODES, d0, dc, rows, RB_DC);
+            __syncthreads();
+            if (VEC) {
+                for (int d = 0; d < dc; d += 4) {
+                    const float4 c = *reinterpret_cast<const float4 *>(col + d);
+#pragma unroll
+                    for (int r = 0; r < RB_ROWS; r++) {
+                        const float4 a = *reinterpret_cast<const float4 *>(mine + r * RB_DC + d);
+                        float t;
+                        t = a.x - c.x; acc[r] = acc[r] + t * t;
+                        t = a.y - c.y; acc[r] = acc[r] + t * t;
+                        t = a.z - c.z; acc[r] = acc[r] + t * t;
+                        t = a.w - c.w; acc[r] = acc[r] + t * t;
+                    }
+                }
+            } else {
+                for (int d = 0; d < dc; d++) {
+                    const float c = col[d];
+#pragma unroll
+                    for (int r = 0; r < RB_ROWS; r++) {
+                        const float t = mine[r * RB_DC + d] - c;
+                        acc[r] = acc[r] + t * t;
+                    }
+                }
+            }
         }
-        __syncthreads();
         const int v = c0 + lane;
-        const float *col = tile + lane * stride;
-        float da = 0.0f, db = 0.0f;
-        if (VEC) {
-            for (int d = 0; d < dim; d += 4) {
-                const float4 c = *reinterpret_cast<const float4 *>(col + d);
-                const float4 a = *reinterpret_cast<const float4 *>(ra + d);
-                const float4 b = *reinterpret_cast<const float4 *>(rbv + d);
-                float t;
-                t = a.x - c.x; da = da + t * t;
-                t = a.y - c.y; da = da + t * t;
-                t = a.z - c.z; da = da + t * t;
-                t = a.w - c.w; da = da + t * t;
-                t = b.x - c.x; db = db + t * t;
-                t = b.y - c.y; db = db + t * t;
-                t = b.z - c.z; db = db + t * t;
-                t = b.w - c.w; db = db + t * t;
-            }
-        } else {
-            for (int d = 0; d < dim; d++) {
-                const float c = col[d];
-                float t = ra[d] - c;
-                da = da + t * t;
-                t = rbv[d] - c;
-                db = db + t * t;
-            }
+#pragma unroll
+        for (int r = 0; r < RB_ROWS; r++) {
+            const int u = u0 + wave * RB_ROWS + r;
+            const uint64_t key = v < n && v != u ? rb_key(acc[r], v) : RB_KEY_MAX;
+            // the current 64th key is lane 63's; merge only when the tile
+            // can improve the list
+            const uint64_t lim =
+                ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(best[r] >> 32), 63) << 32) |
+                (uint32_t)__shfl((int)(uint32_t)best[r], 63);
+            if (__ballot(key < lim)) best[r] = rb_merge64(best[r], key);
         }
-        const bool in = v < n;
-        const uint64_t ka = in && v != ua ? rb_key(da, v) : RB_KEY_MAX;
-        const uint64_t kb = in && v != ub ? rb_key(db, v) : RB_KEY_MAX;
-        // the current 64th key is lane 63's; merge only when the tile can
-        // improve the list
-        const uint64_t lim_a = ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(best_a >> 32), 63) << 32) |
-                               (uint32_t)__shfl((int)(uint32_t)best_a, 63);
-        const uint64_t lim_b = ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(best_b >> 32), 63) << 32) |
-                               (uint32_t)__shfl((int)(uint32_t)best_b, 63);
-        if (__ballot(ka < lim_a)) best_a = rb_merge64(best_a, ka);
-        if (__ballot(kb < lim_b)) best_b = rb_merge64(best_b, kb);
     }
-    if (ua < n) keys[(size_t)ua * RB_WAVE + lane] = best_a;
-    if (ub < n) keys[(size_t)ub * RB_WAVE + lane] = best_b;
+#pragma unroll
+    for (int r = 0; r < RB_ROWS; r++) {
+        const int u = u0 + wave * RB_ROWS + r;
+        if (u < n) keys[(size_t)u * RB_WAVE + lane] = best[r];
+    }
 }
 
 // -------------------------------------------------------------- rows
@@ -303,31 +340,14 @@ extern "C" int rv_build_exact(const float *x, int n, int dim, int r, int cand, u
     if (rb_check(hipMalloc((void **)&keys, (size_t)n * RB_WAVE * sizeof(uint64_t)),
                  "hipMalloc knn"))
         return -1;
-    const bool vec = dim % 4 == 0;
-    const int stride = vec ? dim + 4 : dim + 1;
-    const size_t lds = ((size_t)RB_TILE * stride + (size_t)RB_WAVES * RB_ROWS * dim) *
-                       sizeof(float);
     const int blocks = (n + RB_WAVES * RB_ROWS - 1) / (RB_WAVES * RB_ROWS);
     int rc = 0;
-    if (lds > 160 * 1024) {
-        rv_set_error("rv_build_exact: dimension too large for the LDS tile");
-        rc = -1;
-    } else if (vec) {
-        rc = rb_check(hipFuncSetAttribute((const void *)rv_knn_exact_kernel<true>,
-                                          hipFuncAttributeMaxDynamicSharedMemorySize,
-                                          160 * 1024), "hipFuncSetAttribute");
-        if (!rc) {
-            rv_knn_exact_kernel<true><<<blocks, RB_WAVE * RB_WAVES, lds, s>>>(x, n, dim, keys);
-            rc = rb_check(hipGetLastError(), "rv_knn_exact_kernel");
-        }
+    if (dim % 4 == 0) {
+        rv_knn_exact_kernel<true><<<blocks, RB_WAVE * RB_WAVES, 0, s>>>(x, n, dim, keys);
+        rc = rb_check(hipGetLastError(), "rv_knn_exact_kernel");
     } else {
-        rc = rb_check(hipFuncSetAttribute((const void *)rv_knn_exact_kernel<false>,
-                                          hipFuncAttributeMaxDynamicSharedMemorySize,
-                                          160 * 1024), "hipFuncSetAttribute");
-        if (!rc) {
-            rv_knn_exact_kernel<false><<<blocks, RB_WAVE * RB_WAVES, lds, s>>>(x, n, dim, keys);
-            rc = rb_check(hipGetLastError(), "rv_knn_exact_kernel");
-        }
+        rv_knn_exact_kernel<false><<<blocks, RB_WAVE * RB_WAVES, 0, s>>>(x, n, dim, keys);
+        rc = rb_check(hipGetLastError(), "rv_knn_exact_kernel");
     }
     if (!rc) {
         int dev = 0, cus = 0;
