@@ -3,15 +3,17 @@ learned early termination against the plain fixed-L search.
 
 Workload: N=1,000,000 base vectors, d=128, from a 1024-component Gaussian
 mixture (centres U[0,100]^d, sigma 8, clipped to [0,255]); PQ M=32 x 256
-(8-bit codes) plus per-vector residual codes; degree-32 graph from exact
-64-NN, RNG pruning and backlinks; 10,000 queries from the same mixture.
-The index is built on the GPU at start-up and the queries are resident in
-HBM.
+(8-bit codes) plus per-vector residual codes; degree-32 graph from the
+reference's exact search-build (exact 64-NN, random initial rows, RNG
+pruning with fill), built on the device; 10,000 queries from the same
+mixture.  The index is built on the GPU at start-up and the queries are
+resident in HBM.
 
-Entries: the caller passes each query its own entry row (the request's
-entry_ids, one row of S ids per query): the representative nodes of the
-query's S nearest k-means centroids (4096 centroids, S=4), chosen on the
-host before the timed region, like any caller-supplied seed list.
+Entries: the reference's seeds, the medoid plus 31 ids from its LCG
+(entry_points 32), the same list for every query and passed as entry rows.
+The routed leg repeats the plain / learned comparison with per-query rows:
+the representative nodes of the query's 4 nearest of 4096 k-means
+centroids, chosen on the host before the timed region.
 
 One "step" is one batched search of all queries.  For each mode (online
 residual correction; ADC ranking + final refine) the beam width L is the
@@ -233,7 +235,17 @@ def main():
                     help="entry-router centroids (k-means over the base)")
     ap.add_argument("--route", type=int, default=4,
                     help="entries per query: representatives of its nearest centroids")
-    ap.add_argument("--Ls", default="10,12,16,20,24,32,48,64,96,128",
+    ap.add_argument("--graph", choices=["backlinks", "reference"], default="reference",
+                    help="backlinks: exact 64-NN + RNG prune + reverse edges (torch); "
+                         "reference: the reference's exact search-build on the device "
+                         "(random initial rows, RNG prune with fill)")
+    ap.add_argument("--entry", choices=["routed", "medoid"], default="medoid",
+                    help="routed: per-query rows from the k-means router; medoid: "
+                         "the reference's seeds (medoid + LCG ids, --entry-points)")
+    ap.add_argument("--entry-points", type=int, default=32)
+    ap.add_argument("--no-routed-leg", action="store_true",
+                    help="skip the plain / learned comparison with routed entry rows")
+    ap.add_argument("--Ls", default="10,12,16,20,24,32,48,64,96,128,192,256,384,512,768,1024",
                     help="beam widths swept per mode (each >= k)")
     ap.add_argument("--k", type=int, default=10)
     ap.add_argument("--refine", type=int, default=64,
@@ -309,9 +321,14 @@ def main():
         log("device", torch.cuda.get_device_name(0))
         base, queries = data.workload(args.n, args.dim, args.components, 8.0, args.nq)
         log("data", base.shape, queries.shape)
+        graph = None
+        if args.graph == "reference":
+            tg = time.time()
+            graph = index.build_graph_exact(base, args.R, 2 * args.R, 42, 1)
+            log("reference search-build graph on the device: %.2f s" % (time.time() - tg))
         ix = index.build_pq_index(base, m=args.m, R=args.R, res_bits=args.res_bits,
                                   iters=20, train_max=100000, device=dev,
-                                  n_router=args.router)
+                                  n_router=args.router, graph=graph)
         log("index built: graph", ix.graph.shape, "medoid", ix.entry, "router",
             ix.router_centroids.shape)
         gt = data.ground_truth(base, queries, k=100, device=dev)
@@ -343,8 +360,18 @@ def main():
         a.close()
         b2.close()
 
-    S = args.route
-    ents = index.route_on_host(queries, ix.router_centroids, ix.router_ids, S)
+    if args.entry == "medoid":
+        seeds = native.make_seed_ids(len(ix.graph), args.entry_points, ix.entry)
+        S = len(seeds)
+
+        def entry_rows(q):
+            return np.ascontiguousarray(np.tile(seeds, (len(q), 1)), dtype=np.int32)
+    else:
+        S = args.route
+
+        def entry_rows(q):
+            return index.route_on_host(q, ix.router_centroids, ix.router_ids, S)
+    ents = entry_rows(queries)
     log("entry rows chosen on the host: %d queries x %d ids" % ents.shape)
     calc = Calculator()
     calc.load_pq_index(ix, with_router=False)
@@ -352,6 +379,8 @@ def main():
     nq, k = args.nq, args.k
     dq = native.DeviceBuffer.from_array(queries)
     d_ent = native.DeviceBuffer.from_array(ents)
+    # the entry rows the search launches take (the routed leg swaps them)
+    es = dict(ptr=d_ent.ptr, S=S)
     d_ids = native.DeviceBuffer(nq * k * 4)
     d_d = native.DeviceBuffer(nq * k * 4)
     d_cnt = native.DeviceBuffer(nq * 4)
@@ -372,15 +401,15 @@ def main():
                                  expand_count=args.R,
                                  d_stats=d_st.ptr if stats else None,
                                  learned_stop=1 if learned else 0,
-                                 stop_threshold=0.9, d_entry_ids=d_ent.ptr,
-                                 entry_count=S, entry_stride=S, hash_log2=h2,
+                                 stop_threshold=0.9, d_entry_ids=es["ptr"],
+                                 entry_count=es["S"], entry_stride=es["S"], hash_log2=h2,
                                  probe_limit=pl, grid_waves=gw)
 
     def autotune(mode, L, learned):
         t = calc.autotune(dq.ptr, nq, L, k, d_ids.ptr, d_d.ptr, d_cnt.ptr, k, repeats=3,
                           correction=1 if mode == "online" else 0,
                           refine_candidates=refine_of(mode, L), expand_count=args.R,
-                          d_entry_ids=d_ent.ptr, entry_count=S, entry_stride=S,
+                          d_entry_ids=es["ptr"], entry_count=es["S"], entry_stride=es["S"],
                           learned_stop=1 if learned else 0)
         tuned[(mode, L, learned)] = (t.hash_log2, t.probe_limit, t.grid_waves)
         info = dict(hash_log2=t.hash_log2, probe_limit=t.probe_limit,
@@ -454,7 +483,7 @@ def main():
             train_q = data.gaussian_mixture(args.train_queries, args.dim, args.components,
                                             8.0, 44, centers_seed=42)
             # the mixture's centres depend only on centers_seed, as for the base
-            train_ents = index.route_on_host(train_q, ix.router_centroids, ix.router_ids, S)
+            train_ents = entry_rows(train_q)
             log("training: %d queries at L=%d, entry rows chosen"
                 % (len(train_q), args.train_L))
         P = native.lib().rover_mlp_param_count()
@@ -463,6 +492,43 @@ def main():
             native.check(native.lib().rover_learned_stop_set(
                 calc.h, m[0].ctypes.data, P, m[1].ctypes.data, m[2].ctypes.data,
                 args.stop_every, args.label_k, args.stop_every), "rover_learned_stop_set")
+
+        def train_model(slack, tq, tents, tS):
+            t = native.RoverLearnedStopTrain()
+            t.queries = tq.ctypes.data
+            t.num_queries = tq.shape[0]
+            t.pool_size = args.train_L
+            t.correction = 1 if mode == "online" else 0
+            t.label_k = args.label_k
+            t.label_slack = slack
+            t.entry_points = 1
+            t.entry_ids = tents.ctypes.data
+            t.entry_count = tS
+            t.entry_stride = tS
+            t.expand_count = args.R
+            t.stop_every = args.stop_every
+            t.min_steps = args.stop_every
+            t.trace_cap = args.trace_cap
+            t.steps = args.train_steps
+            t.batch = 8192
+            t.lr = 3e-3
+            t.seed = 44
+            tt = time.time()
+            native.check(native.lib().rover_learned_stop_train(calc.h, C.byref(t)),
+                         "rover_learned_stop_train")
+            training = dict(train_queries=len(tq), train_L=args.train_L,
+                            mode=mode, label_k=args.label_k, label_slack=slack,
+                            stop_every=args.stop_every, trace_cap=args.trace_cap,
+                            samples=int(t.samples), positive_frac=round(t.positive_frac, 4),
+                            final_loss=round(t.final_loss, 5),
+                            seconds=round(time.time() - tt, 2), threshold=0.9)
+            log("learned stop trained:", training)
+            model = [np.zeros(P, np.float32), np.zeros(16, np.float32),
+                     np.zeros(16, np.float32)]
+            native.check(native.lib().rover_learned_stop_get(
+                calc.h, model[0].ctypes.data, P, model[1].ctypes.data,
+                model[2].ctypes.data), "rover_learned_stop_get")
+            return model, training
 
         tried = []
         if args.load_model:
@@ -475,40 +541,7 @@ def main():
             tried.append(dict(sweep=rows, chosen=chosen, training=training, model=model))
         for slack in ([] if args.load_model else
                       [int(v) for v in args.label_slacks.split(",")]):
-            t = native.RoverLearnedStopTrain()
-            t.queries = train_q.ctypes.data
-            t.num_queries = train_q.shape[0]
-            t.pool_size = args.train_L
-            t.correction = 1 if mode == "online" else 0
-            t.label_k = args.label_k
-            t.label_slack = slack
-            t.entry_points = 1
-            t.entry_ids = train_ents.ctypes.data
-            t.entry_count = S
-            t.entry_stride = S
-            t.expand_count = args.R
-            t.stop_every = args.stop_every
-            t.min_steps = args.stop_every
-            t.trace_cap = args.trace_cap
-            t.steps = args.train_steps
-            t.batch = 8192
-            t.lr = 3e-3
-            t.seed = 44
-            tt = time.time()
-            native.check(native.lib().rover_learned_stop_train(calc.h, C.byref(t)),
-                         "rover_learned_stop_train")
-            training = dict(train_queries=args.train_queries, train_L=args.train_L,
-                            mode=mode, label_k=args.label_k, label_slack=slack,
-                            stop_every=args.stop_every, trace_cap=args.trace_cap,
-                            samples=int(t.samples), positive_frac=round(t.positive_frac, 4),
-                            final_loss=round(t.final_loss, 5),
-                            seconds=round(time.time() - tt, 2), threshold=0.9)
-            log("learned stop trained:", training)
-            model = [np.zeros(P, np.float32), np.zeros(16, np.float32),
-                     np.zeros(16, np.float32)]
-            native.check(native.lib().rover_learned_stop_get(
-                calc.h, model[0].ctypes.data, P, model[1].ctypes.data,
-                model[2].ctypes.data), "rover_learned_stop_get")
+            model, training = train_model(slack, train_q, train_ents, S)
             rows, chosen = sweep(mode, True)
             tried.append(dict(sweep=rows, chosen=chosen, training=training, model=model))
         # the slack whose sweep reaches the target fastest (else the best recall)
@@ -517,7 +550,8 @@ def main():
             learned = max(hit, key=lambda v: v["chosen"]["qps"])
         else:
             learned = max(tried, key=lambda v: v["sweep"][-1]["recall10"])
-        set_model(learned["model"])
+        main_model = learned["model"]
+        set_model(main_model)
         if args.save_model:
             m = learned["model"]
             np.savez(args.save_model, params=m[0], mean=m[1], inv_std=m[2],
@@ -565,6 +599,49 @@ def main():
                           entry_ids=ents)
     host_qps = nq * args.steps / (time.time() - t1)
     log("plain search from host buffers (PCIe included): %.0f QPS" % host_qps)
+
+    # ---- routed leg: the same plain / learned comparison with per-query entry
+    # rows from the k-means router (representatives of the centroids nearest
+    # the query) in place of the shared seed list
+    routed = None
+    if not args.no_routed_leg:
+        rS = args.route
+        rents = index.route_on_host(queries, ix.router_centroids, ix.router_ids, rS)
+        d_rent = native.DeviceBuffer.from_array(rents)
+        saved_tuned = dict(tuned)
+        tuned.clear()
+        es.update(ptr=d_rent.ptr, S=rS)
+        routed = {"entries": "per query: representatives of its %d nearest of %d k-means "
+                             "centroids" % (rS, len(ix.router_ids))}
+        log("routed leg: %s" % routed["entries"])
+        _, pch = sweep(mode, False)
+        if pch:
+            if not args.no_autotune:
+                autotune(mode, pch["L"], False)
+            ms_rp = timed(mode, pch["L"], False)
+            routed["plain"] = dict(L=pch["L"], recall10=round(pch["recall"], 4),
+                                   qps=round(nq / (ms_rp / 1e3), 1),
+                                   ms_per_step=round(ms_rp, 4))
+            log("routed leg, timed plain: %s" % routed["plain"])
+            if learned and not args.load_model:
+                tr_ents = index.route_on_host(train_q, ix.router_centroids, ix.router_ids, rS)
+                rmodel, rtraining = train_model(0, train_q, tr_ents, rS)
+                set_model(rmodel)
+                _, lch = sweep(mode, True)
+                if lch:
+                    if not args.no_autotune:
+                        autotune(mode, lch["L"], True)
+                    ms_rl = timed(mode, lch["L"], True)
+                    routed["learned"] = dict(
+                        L=lch["L"], recall10=round(lch["recall"], 4),
+                        qps=round(nq / (ms_rl / 1e3), 1), ms_per_step=round(ms_rl, 4),
+                        speedup_vs_plain=round(ms_rp / ms_rl, 3), label_slack=0)
+                    log("routed leg, timed learned: %s" % routed["learned"])
+                set_model(main_model)
+        es.update(ptr=d_ent.ptr, S=S)
+        tuned.clear()
+        tuned.update(saved_tuned)
+        d_rent.close()
 
     # ---- refine leg: the top refine_factor*k pool members re-ranked in FP64
     # from PQ + residual (CURRENT) and with the error4 level (ERROR4)
@@ -631,7 +708,7 @@ def main():
     # HBM bytes per launch come from the profiler's FETCH_SIZE / WRITE_SIZE
     # passes over this same workload (calibrated against a copy kernel),
     # recorded next to the profile summary; used when kernel and L match
-    rec_path = os.path.join(ROOT, "profiles", "rv_search_kernel_online_1M_L32.json")
+    rec_path = os.path.join(ROOT, "profiles", "rv_search_kernel_online_1M_L%d.json" % L)
     if os.path.exists(rec_path) and args.n == 1_000_000 and args.nq == 10_000:
         rec = json.load(open(rec_path))
         per = rec.get("traffic_bytes_per_launch", {}).get(kernel)
@@ -701,12 +778,17 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "1M x 128 Gaussian mixture (1024 comps, sigma 8); PQ 32x256 + "
-                        "%d-bit residuals; R=32 graph (exact 64-NN, RNG prune, backlinks); "
-                        "%d queries, k=%d" % (args.res_bits, nq, k),
+                        "%d-bit residuals; R=32 graph (%s); %d queries, k=%d"
+                        % (args.res_bits,
+                           "exact 64-NN, RNG prune, backlinks" if args.graph == "backlinks"
+                           else "the reference's exact search-build: 64-NN, random initial "
+                                "rows, RNG prune with fill", nq, k),
             "search": label, "mode": mode, "L": L,
             "refine_candidates": refine_of(mode, L),
-            "entries": "per query: representatives of its %d nearest of %d k-means "
-                       "centroids, passed as entry_ids rows" % (S, len(ix.router_ids)),
+            "entries": ("per query: representatives of its %d nearest of %d k-means "
+                        "centroids, passed as entry_ids rows" % (S, len(ix.router_ids))
+                        if args.entry == "routed" else
+                        "the reference's seeds: medoid + %d LCG ids" % (S - 1)),
             "plain_qps": round(qps_plain, 1), "plain_L": L_plain,
             "plain_ms_per_step": round(ms_plain, 4),
             "learned_speedup_vs_plain": round(qps / qps_plain, 3) if learned else None,
@@ -717,6 +799,7 @@ def main():
             "host_buffers_qps": round(host_qps, 1),
             "refine": refine_leg,
             "sq": sq_leg,
+            "routed": routed,
             "launch_tuning": tuning,
             "uneven_batches": batching,
         },

@@ -27,9 +27,9 @@
 //   refine retrieval_final_refine.c:256-292 (f64 sequential, (dist, orig))
 //
 // LDS holds, per query: the query vector, the ADC table (m*ks f32) or the
-// quantized query, the sorted pool as two ping-pong arrays of 64-bit keys
-// (float bits << 32 | id; distances are >= +0 so unsigned order equals
-// (distance, id) order), expanded flags, and the visited hash.
+// quantized query, the sorted pool as one array of 64-bit keys (float
+// bits << 32 | id; distances are >= +0 so unsigned order equals (distance,
+// id) order) merged in place, expanded flags, and the visited hash.
 #include <hip/hip_runtime.h>
 
 #include <float.h>
@@ -368,8 +368,9 @@ __device__ __forceinline__ float rv_sq(const uint8_t *qsq, const uint8_t *code,
 // ----------------------------------------------------------- LDS layout
 
 struct RvLayout {
-    int off_q, off_tab, off_keys0, off_keys1, off_flags0, off_flags1;
-    int off_born0, off_born1;  // trace mode: step each pool entry was scored
+    int off_q, off_tab, off_keys, off_flags;
+    int off_born;    // trace mode: step each pool entry was scored
+    int off_refine;  // final refine: f64 distance bits of the refined members
     int off_stop;              // learned-stop state of the current query
     int off_cand, off_hash, total;
 };
@@ -380,7 +381,8 @@ __host__ __device__ static inline int rv_align(int v, int a) {
 
 __host__ __device__ static inline RvLayout rv_layout(int dim, int scorer, int m,
                                                     int ks, int max_pool,
-                                                    int hash_log2, bool trace) {
+                                                    int hash_log2, bool trace,
+                                                    int max_refine) {
     RvLayout l;
     int off = 0;
     l.off_q = off;
@@ -388,20 +390,19 @@ __host__ __device__ static inline RvLayout rv_layout(int dim, int scorer, int m,
     l.off_tab = off;
     if (scorer == RV_SCORE_ADC) off = rv_align(off + 4 * m * ks, 16);
     else if (scorer == RV_SCORE_SQ) off = rv_align(off + dim, 16);
-    l.off_keys0 = off;
+    l.off_keys = off;
     off = rv_align(off + 8 * max_pool, 16);
-    l.off_keys1 = off;
-    off = rv_align(off + 8 * max_pool, 16);
-    l.off_flags0 = off;
+    l.off_flags = off;
     off = rv_align(off + max_pool, 16);
-    l.off_flags1 = off;
-    off = rv_align(off + max_pool, 16);
-    l.off_born0 = l.off_born1 = -1;
+    l.off_born = -1;
     if (trace) {
-        l.off_born0 = off;
+        l.off_born = off;
         off = rv_align(off + 2 * max_pool, 16);
-        l.off_born1 = off;
-        off = rv_align(off + 2 * max_pool, 16);
+    }
+    l.off_refine = -1;
+    if (max_refine > 0) {
+        l.off_refine = off;
+        off = rv_align(off + 8 * (max_refine < max_pool ? max_refine : max_pool), 16);
     }
     l.off_stop = off;
     off = rv_align(off + 64, 16);
@@ -527,14 +528,12 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
     float *qv = reinterpret_cast<float *>(smem + lay.off_q);
     float *lut = reinterpret_cast<float *>(smem + lay.off_tab);
     uint8_t *qsq = reinterpret_cast<uint8_t *>(smem + lay.off_tab);
-    uint64_t *keys_buf[2] = {reinterpret_cast<uint64_t *>(smem + lay.off_keys0),
-                             reinterpret_cast<uint64_t *>(smem + lay.off_keys1)};
-    uint8_t *flags_buf[2] = {smem + lay.off_flags0, smem + lay.off_flags1};
-    // trace mode: the step at which each pool entry was scored
+    // the pool: keys ascending, expanded flags, and (trace mode) the step
+    // at which each entry was scored
+    uint64_t *pool_keys = reinterpret_cast<uint64_t *>(smem + lay.off_keys);
+    uint8_t *pool_flags = smem + lay.off_flags;
     constexpr bool tracing = STOP == 2;
-    uint16_t *born_buf[2] = {
-        tracing ? reinterpret_cast<uint16_t *>(smem + lay.off_born0) : nullptr,
-        tracing ? reinterpret_cast<uint16_t *>(smem + lay.off_born1) : nullptr};
+    uint16_t *pool_born = tracing ? reinterpret_cast<uint16_t *>(smem + lay.off_born) : nullptr;
     uint64_t *cand = reinterpret_cast<uint64_t *>(smem + lay.off_cand);
     uint32_t *hash = reinterpret_cast<uint32_t *>(smem + lay.off_hash);
     const int lane = rv_lane();
@@ -598,7 +597,6 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
         }
         __syncthreads();
 
-        int cur = 0;        // which ping-pong buffer holds the pool
         int n = 0;          // pool size
         int status = 0;
         uint64_t st_scored = 0, st_expanded = 0, st_dedup = 0;
@@ -625,13 +623,7 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
         // 32..63, 2: anywhere); the other lanes hold RV_KEY_MAX
         auto merge = [&](uint64_t key, int span, int track, int *first_new, int *admitted,
                          int *tracked) {
-            uint64_t *src = keys_buf[cur];
-            uint64_t *dst = keys_buf[cur ^ 1];
-            uint8_t *fsrc = flags_buf[cur];
-            uint8_t *fdst = flags_buf[cur ^ 1];
-            uint16_t *bsrc = born_buf[cur];
-            uint16_t *bdst = born_buf[cur ^ 1];
-            if (n == L && key != RV_KEY_MAX && key >= src[L - 1]) key = RV_KEY_MAX;
+            if (n == L && key != RV_KEY_MAX && key >= pool_keys[L - 1]) key = RV_KEY_MAX;
             *first_new = 0x7fffffff;
             *admitted = 0;
             *tracked = track;
@@ -650,42 +642,61 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
             bool valid = key != RV_KEY_MAX && !(lane > 0 && prev == key);
             int lb = 0;
             if (valid) {
-                lb = rv_lower_bound(src, n, key);
-                if (lb < n && src[lb] == key) valid = false;  // already pooled
+                lb = rv_lower_bound(pool_keys, n, key);
+                if (lb < n && pool_keys[lb] == key) valid = false;  // already pooled
             }
             const uint64_t vmask = __ballot(valid);
             const int cv = __popcll(vmask);
             if (cv == 0) return;
             const int rank = __popcll(vmask & ((1ull << lane) - 1ull));
-            // sorted valid candidates, compacted, for the pool-side search
-            if (valid) cand[rank] = key;
             if (track >= 0 && track < n) {
-                const uint64_t tk = src[track];
+                const uint64_t tk = pool_keys[track];
                 *tracked = track + rv_uniform(__popcll(__ballot(valid && key < tk)));
             }
-            __syncthreads();
-            for (int j = lane; j < n; j += RV_WAVE) {
-                const uint64_t pk = src[j];
-                int lo = 0, hi = cv;
-                while (lo < hi) {
-                    int mid = (lo + hi) >> 1;
-                    if (cand[mid] < pk) lo = mid + 1;
-                    else hi = mid;
+            // Pool entry j moves right by #{valid c : lb_c <= j}: no
+            // candidate equals a pool key, so candidate c sorts below entry
+            // j exactly when lb_c <= j.  New positions increase with j, so
+            // moving the pool in place, last 64-entry chunk first and each
+            // chunk read before it is written, never overwrites an entry
+            // that has not moved yet; entries left of the smallest lb stay.
+            // The valid lanes hold ascending keys and lbs; walking them from
+            // the top, `above` counts those right of the current chunk.
+            const int lb_min =
+                __builtin_amdgcn_readlane(lb, __ffsll((unsigned long long)vmask) - 1);
+            uint64_t rem = vmask;
+            int above = 0;
+            for (int j0 = ((n - 1) >> 6) << 6; j0 >= 0 && j0 + RV_WAVE > lb_min;
+                 j0 -= RV_WAVE) {
+                const int j = j0 + lane;
+                int right = above;
+                while (rem) {
+                    const int top = 63 - __clzll((long long)rem);
+                    const int lbc = __builtin_amdgcn_readlane(lb, top);
+                    if (lbc < j0) break;
+                    right += lbc > j ? 1 : 0;
+                    above++;
+                    rem &= ~(1ull << top);
                 }
-                const int np = j + lo;
-                if (np < L) {
-                    dst[np] = pk;
-                    fdst[np] = fsrc[j];
-                    if (tracing) bdst[np] = bsrc[j];
+                const int shift = cv - right;
+                if (j < n && shift > 0) {
+                    const uint64_t pk = pool_keys[j];
+                    const uint8_t pf = pool_flags[j];
+                    const uint16_t pb = tracing ? pool_born[j] : 0;
+                    const int np = j + shift;
+                    if (np < L) {
+                        pool_keys[np] = pk;
+                        pool_flags[np] = pf;
+                        if (tracing) pool_born[np] = pb;
+                    }
                 }
             }
             int pos = 0x7fffffff;
             if (valid) {
                 pos = lb + rank;
                 if (pos < L) {
-                    dst[pos] = key;
-                    fdst[pos] = 0;
-                    if (tracing) bdst[pos] = (uint16_t)born_step;
+                    pool_keys[pos] = key;
+                    pool_flags[pos] = 0;
+                    if (tracing) pool_born[pos] = (uint16_t)born_step;
                 } else {
                     pos = 0x7fffffff;
                 }
@@ -695,7 +706,6 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
             *first_new = __builtin_amdgcn_readlane(pos, __ffsll((unsigned long long)vmask) - 1);
             *admitted = rv_uniform(__popcll(__ballot(pos != 0x7fffffff)));
             n = (n + cv) < L ? (n + cv) : L;
-            cur ^= 1;
             __syncthreads();
         };
 
@@ -775,13 +785,13 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
         if (status == 0 && b.n_entries * 4 > hash_n) {
             for (int i = lane; i < hash_n; i += RV_WAVE) hash[i] = RV_EMPTY_SLOT;
             __syncthreads();
-            const uint64_t *k0 = keys_buf[cur];
+            const uint64_t *k0 = pool_keys;
             for (int i = lane; i < n; i += RV_WAVE)
                 rv_visit(hash, b.hash_log2, b.probe_limit, (uint32_t)rv_key_id(k0[i]));
             __syncthreads();
         }
         if (STOP != 0 && status == 0) {
-            const uint64_t *k0 = keys_buf[cur];
+            const uint64_t *k0 = pool_keys;
             ss.top1_0 = rv_key_dist(k0[0]);
             ss.topk_0 = rv_key_dist(k0[(n < b.stop_label_k ? n : b.stop_label_k) - 1]);
         }
@@ -791,8 +801,8 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
         int step = 0;
         bool stopped = false;
         while (status == 0) {
-            const uint64_t *keys = keys_buf[cur];
-            uint8_t *flags = flags_buf[cur];
+            const uint64_t *keys = pool_keys;
+            uint8_t *flags = pool_flags;
             int next = -1;
             while (scan < n) {
                 const int p = scan + lane;
@@ -920,7 +930,7 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
             // keeps no records
             int32_t *mq = b.trace_meta + (size_t)q * RV_TRACE_META;
             const int kk = status == 0 ? (n < b.stop_label_k ? n : b.stop_label_k) : 0;
-            const uint16_t *born = born_buf[cur];
+            const uint16_t *born = pool_born;
             const int v = lane < kk ? (int)born[lane] : -1;
             int rank = 0;
             for (int j = 0; j < kk; j++) {
@@ -942,7 +952,7 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
         }
 
         // ---- output
-        const uint64_t *keys = keys_buf[cur];
+        const uint64_t *keys = pool_keys;
         if (status != 0) {
             if (lane == 0) b.out_counts[q] = status;
         } else if (refine == 0) {
@@ -958,7 +968,7 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
             // plus the error4 level of decode_vector, :221-246, when asked)
             const int F = refine < n ? refine : n;
             const bool e4 = b.refine_error4 != 0;
-            uint64_t *dbits = keys_buf[cur ^ 1];
+            uint64_t *dbits = reinterpret_cast<uint64_t *>(smem + lay.off_refine);
             bool bad = false;
             for (int i = lane; i < F; i += RV_WAVE) {
                 const int id = rv_key_id(keys[i]);
@@ -1274,7 +1284,7 @@ extern "C" int rv_fill_zero(void *p, size_t bytes, void *stream) {
 
 static RvLayout rv_layout_for(const RvIndex *ix, const RvBatch *b) {
     return rv_layout(ix->dim, b->scorer, ix->m, ix->ks, b->max_pool, b->hash_log2,
-                     b->stop_mode == 2);
+                     b->stop_mode == 2, b->refine_candidates);
 }
 
 
