@@ -327,6 +327,86 @@ __device__ __forceinline__ bool rv_online_pair32(const RvIndex &ix, const float 
     return isfinite(s) && isfinite(z) && s > 0.0f;
 }
 
+// G-lane online scoring of one candidate (G = 4, 8, 16; same 32-subspace
+// x 4-dim shapes as above): the lane with sub-index j of the group decodes
+// dimensions [j*128/G, (j+1)*128/G) and forms their squared differences,
+// then the sequential sum runs through the group in dimension order, each
+// lane adding its terms to the running sum handed up from the lane below.
+// The distance is left in the group's last lane (sub == G - 1).  Used when
+// an expansion has few unvisited neighbors, so that the per-dimension work
+// is spread over the lanes the idle candidates would have held.
+template <int G, int RB>
+__device__ __forceinline__ bool rv_online_group(const RvIndex &ix, const float *q, int id_in,
+                                                bool active, int sub, float *out) {
+    constexpr int DG = 128 / G;       // dimensions per lane
+    constexpr int MG = DG / 4;        // subspaces per lane
+    constexpr int RBY = DG * RB / 8;  // residual bytes per lane
+    const int id = active ? id_in : 0;
+    const float2 qp = *reinterpret_cast<const float2 *>(ix.res_qparams + 2 * (size_t)id);
+    const float s = qp.x, z = qp.y;
+    const uint8_t *crow = ix.codes + (size_t)id * 32 + sub * MG;
+    uint32_t cw[MG >= 4 ? MG / 4 : 1];
+    if (MG == 2) {
+        cw[0] = *reinterpret_cast<const uint16_t *>(crow);
+    } else if (MG == 4) {
+        cw[0] = *reinterpret_cast<const uint32_t *>(crow);
+    } else {
+        const uint2 v = *reinterpret_cast<const uint2 *>(crow);
+        cw[0] = v.x;
+        cw[MG / 4 - 1] = v.y;
+    }
+    const uint8_t *rrow = ix.res_codes + (size_t)id * (16 * RB) + sub * RBY;
+    uint32_t rw[RBY / 4];
+    if (RBY == 4) {
+        rw[0] = *reinterpret_cast<const uint32_t *>(rrow);
+    } else if (RBY == 8) {
+        const uint2 v = *reinterpret_cast<const uint2 *>(rrow);
+        rw[0] = v.x;
+        rw[RBY / 4 - 1] = v.y;
+    } else {
+#pragma unroll
+        for (int w = 0; w < RBY / 16; w++) {
+            const uint4 v = reinterpret_cast<const uint4 *>(rrow)[w];
+            rw[4 * w + 0] = v.x;
+            rw[4 * w + 1] = v.y;
+            rw[4 * w + 2] = v.z;
+            rw[4 * w + 3] = v.w;
+        }
+    }
+    const float *qs = q + sub * DG;
+    float t[DG];
+#pragma unroll
+    for (int m = 0; m < MG; m++) {
+        const int code = (cw[m >> 2] >> (8 * (m & 3))) & 255;
+        const float4 c4 = *reinterpret_cast<const float4 *>(
+            ix.codebook + ((size_t)(sub * MG + m) * 256 + code) * 4);
+        const float cs[4] = {c4.x, c4.y, c4.z, c4.w};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int d = 4 * m + k;
+            int rc;
+            if (RB == 4) rc = (rw[d >> 3] >> (4 * (d & 7))) & 15;
+            else rc = (rw[d >> 2] >> (8 * (d & 3))) & 255;
+            const float rec = rv_decode_value(cs[k], rc, s, z);
+            const float diff = qs[d] - rec;
+            t[d] = diff * diff;
+        }
+    }
+    float acc = 0.0f;
+#pragma unroll
+    for (int j = 0; j < G; j++) {
+        const float below = __shfl_up(acc, 1);
+        if (sub == j) {
+            float a = j == 0 ? 0.0f : below;
+#pragma unroll
+            for (int d = 0; d < DG; d++) a += t[d];
+            acc = a;
+        }
+    }
+    *out = acc;
+    return isfinite(s) && isfinite(z) && s > 0.0f;
+}
+
 // Two-lane online scoring is used for 32-subspace x 4-dim PQ with at most
 // 32 neighbors per expansion.
 __host__ __device__ inline bool rv_online_pairable(const RvIndex &ix, int expand) {
@@ -861,23 +941,55 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
             }
             uint64_t key = RV_KEY_MAX;
             bool dup = false, nonfinite = false;
+            int span = expand <= 32 ? 0 : 2;  // where the keys are, for the merge
             if (pairable) {
-                // neighbor `slot` is scored by lanes slot and slot + 32; the
-                // low lane owns the visited test, the high lane the key
-                const int slot = lane & 31;
-                const int nbs = __shfl(nb, slot);
+                // lane i < valid_n owns neighbor i's visited test
                 int fresh = 0;
-                if (lane < 32 && slot < valid_n) {
-                    fresh = rv_visit(hash, b.hash_log2, b.probe_limit, (uint32_t)nbs) ? 1 : 0;
+                if (lane < valid_n) {
+                    fresh = rv_visit(hash, b.hash_log2, b.probe_limit, (uint32_t)nb) ? 1 : 0;
                     dup = !fresh;
                 }
-                fresh = __shfl(fresh, slot);
-                const bool active = slot < valid_n && fresh;
-                float d = 0.0f;
-                const bool ok = rv_online_pair32<pair_rb>(ix, qv, nbs, active, lane >= 32, &d);
-                if (lane >= 32 && active) {
-                    if (!ok || !isfinite(d) || d == FLT_MAX) nonfinite = true;
-                    else key = rv_key(d + 0.0f, nbs);
+                const uint64_t fmask = __ballot(fresh != 0);
+                const int nf = __popcll(fmask);
+                if (nf > 8) {
+                    // neighbor `slot` is scored by lanes slot and slot + 32;
+                    // the high lane holds the key
+                    const int slot = lane & 31;
+                    const int nbs = __shfl(nb, slot);
+                    const bool active = __shfl(fresh, slot) != 0;
+                    float d = 0.0f;
+                    const bool ok =
+                        rv_online_pair32<pair_rb>(ix, qv, nbs, active, lane >= 32, &d);
+                    if (lane >= 32 && active) {
+                        if (!ok || !isfinite(d) || d == FLT_MAX) nonfinite = true;
+                        else key = rv_key(d + 0.0f, nbs);
+                    }
+                    span = 1;
+                } else if (nf > 0) {
+                    // the unvisited neighbors, compacted: candidate r is
+                    // scored by the G lanes r*G .. r*G + G - 1 and its key
+                    // lands in lane r
+                    int *cids = reinterpret_cast<int *>(cand);
+                    if (fresh) cids[__popcll(fmask & ((1ull << lane) - 1ull))] = nb;
+                    __syncthreads();
+                    const int G = nf <= 4 ? 16 : 8;
+                    const int r = lane / G;
+                    const bool active = r < nf;
+                    const int id = active ? cids[r] : 0;
+                    float d = 0.0f;
+                    bool ok;
+                    if (G == 16) ok = rv_online_group<16, pair_rb>(ix, qv, id, active, lane & 15, &d);
+                    else ok = rv_online_group<8, pair_rb>(ix, qv, id, active, lane & 7, &d);
+                    const int from = lane < nf ? lane * G + G - 1 : 63;
+                    const float dr = __shfl(d, from);
+                    const int okr = __shfl(ok ? 1 : 0, from);
+                    const int idr = __shfl(id, from);
+                    if (lane < nf) {
+                        if (!okr || !isfinite(dr) || dr == FLT_MAX) nonfinite = true;
+                        else key = rv_key(dr + 0.0f, idr);
+                    }
+                    span = 0;
+                    __syncthreads();  // cids is reused by the next expansion
                 }
             } else if (lane < valid_n) {
                 if (!rv_visit(hash, b.hash_log2, b.probe_limit, (uint32_t)nb)) {
@@ -909,8 +1021,7 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
 
             int first_new, admitted, src_pos;
             born_step = step < 65535 ? step : 65535;
-            merge(key, pairable ? 1 : (expand <= 32 ? 0 : 2), next, &first_new, &admitted,
-                  &src_pos);
+            merge(key, span, next, &first_new, &admitted, &src_pos);
             (void)admitted;
             if (STOP != 0) {
                 ss.improve_ema =
