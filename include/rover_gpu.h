@@ -115,6 +115,20 @@ AEEResult rover_sq_residual_load(remote_handle64 h, const uint8_t *codes,
                                  int64_t codesLen, const float *qparams,
                                  int64_t qparamsLen, int32_t bits);
 
+/* Split-distance bias for the SQ traversal (replaces the reference's
+ * sqmd_load_correction_bias, retrieval_rpc_demo_support.c:237-299, which
+ * reads SQMD_CORRECTION_BIAS_PATH): the image of an SQMDBIAS file written by
+ * generate_sqmd_correction_bias.py -- 64-byte header, then f32 bias
+ * [num_vectors].  Needs the SQ index with 4-bit codes and its residual
+ * loaded; the header must match them (vectors, dim, bits), the payload's
+ * FNV-1a64 and, when source_fingerprint is not NULL, its 16 bytes (the
+ * first 16 of SHA-256 over the SHA-256 digests of the l1, residual and
+ * qparams files).  Searches with correction = 2 then rank by the split
+ * distance.  Reloading the residual drops the bias. */
+AEEResult rover_sq_split_bias_load(remote_handle64 h, const uint8_t *image,
+                                   int64_t imageLen,
+                                   const uint8_t *source_fingerprint);
+
 /* Entry router (no reference counterpart): `count` centroids
  * [count][dim] and one representative node id per centroid.  A batch with
  * route_entries = S seeds each query with the representatives of its S
@@ -300,7 +314,10 @@ enum {
        loop :2052-2152): distance-only heaps, seeds from
        retrieval_sqmd_make_seed_ids (entry_points, seed), at most 32
        neighbors per expansion; correction = 1 ranks by the exact SQ +
-       residual distance (needs rover_sq_residual_load).  entry_ids and
+       residual distance (needs rover_sq_residual_load); correction = 2 by
+       the split distance (retrieval_rpc_demo_pipeline.c:936-1000: the u8
+       query on the SQ grid refined 17x, coarse u8 x u4 distance plus bias -
+       2 <query, residual>; needs rover_sq_split_bias_load).  entry_ids and
        route_entries are not accepted (ROVER_EBADPARM).                    */
     ROVER_TRAVERSAL_SQMD = 2,
     /* SQ only: the DSP search op (retrieval_async_dsp_search.c:368-509):

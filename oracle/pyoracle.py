@@ -26,7 +26,7 @@ class OrcIndex(C.Structure):
                 ("res_bits", C.c_int), ("sq_codes", C.c_void_p),
                 ("sq_bits", C.c_int), ("sq_scale", C.c_float),
                 ("sq_zero", C.c_float), ("e4_codes", C.c_void_p),
-                ("e4_qparams", C.c_void_p)]
+                ("e4_qparams", C.c_void_p), ("split_bias", C.c_void_p)]
 
 
 class OrcStats(C.Structure):
@@ -78,6 +78,9 @@ def oracle():
         L.orc_sqmd_seed_ids.argtypes = [I, I, I, C.c_uint32, P, I]
         L.orc_sq_residual_distance.argtypes = [P, P, I]
         L.orc_sq_residual_distance.restype = F
+        L.orc_sq_query_aligned_u8.argtypes = [P, I, F, F, P, P]
+        L.orc_sq_split_distance.argtypes = [P, P, C.c_uint64, I]
+        L.orc_sq_split_distance.restype = F
         _orc = L
     return _orc
 
@@ -125,6 +128,8 @@ def reference():
         L.ref_build_exact.argtypes = [P, I, I, I, I, C.c_uint, I, I, P]
         L.ref_sqmd_search.argtypes = [P, I, I, P, I, I, F, F, P, P, I, I, P, I, I,
                                       C.c_uint, I, I, I, P, P, I, P]
+        L.ref_sqmd_split_search.argtypes = [P, I, I, P, I, F, F, P, P, I, P, P, I, I,
+                                            C.c_uint, I, I, I, P, P, I]
         _ref = L
     return _ref
 
@@ -135,7 +140,7 @@ class Index:
     def __init__(self, graph, codes=None, codebook=None, res_codes=None,
                  res_qparams=None, res_bits=4, sq_codes=None, sq_bits=8,
                  sq_scale=1.0, sq_zero=0.0, dim=None, e4_codes=None,
-                 e4_qparams=None):
+                 e4_qparams=None, split_bias=None):
         self.graph = np.ascontiguousarray(graph, dtype=np.int32)
         self.codes = None if codes is None else np.ascontiguousarray(codes, dtype=np.uint8)
         self.codebook = None if codebook is None else np.ascontiguousarray(codebook, dtype=np.float32)
@@ -144,6 +149,7 @@ class Index:
         self.sq_codes = None if sq_codes is None else np.ascontiguousarray(sq_codes, dtype=np.uint8)
         self.e4_codes = None if e4_codes is None else np.ascontiguousarray(e4_codes, dtype=np.uint8)
         self.e4_qparams = None if e4_qparams is None else np.ascontiguousarray(e4_qparams, dtype=np.float32)
+        self.split_bias = None if split_bias is None else np.ascontiguousarray(split_bias, dtype=np.float32)
         s = OrcIndex()
         s.graph = _p(self.graph)
         s.degree = self.graph.shape[1]
@@ -164,6 +170,7 @@ class Index:
         s.sq_zero = sq_zero
         s.e4_codes = _p(self.e4_codes)
         s.e4_qparams = _p(self.e4_qparams)
+        s.split_bias = _p(self.split_bias)
         self.struct = s
         self.dim = s.dim
 
@@ -345,6 +352,21 @@ def sqmd_seed_ids(n, entry_points, graph_entry, seed=42):
     cnt = oracle().orc_sqmd_seed_ids(n, entry_points, graph_entry, seed, _p(out),
                                      len(out))
     return out[:max(cnt, 0)]
+
+
+def sq_query_aligned_u8(query, scale, zero):
+    """The query on the 17x-refined SQ grid and its sum, or None."""
+    q = np.ascontiguousarray(query, dtype=np.float32)
+    out = np.zeros(q.shape[0], dtype=np.uint8)
+    sq = C.c_uint64()
+    if oracle().orc_sq_query_aligned_u8(_p(q), q.shape[0], scale, zero, _p(out), C.byref(sq)):
+        return None
+    return out, sq.value
+
+
+def sq_split_distance(index, query, vid):
+    qu8, sq = sq_query_aligned_u8(query, index.struct.sq_scale, index.struct.sq_zero)
+    return oracle().orc_sq_split_distance(C.byref(index.struct), _p(qu8), sq, vid)
 
 
 def sq_residual_distance(index, query, vid):

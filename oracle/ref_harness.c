@@ -553,6 +553,100 @@ int ref_sqmd_search(const int32_t *graph, int degree, int n, const uint8_t *code
     return rc;
 }
 
+/* ---- the split distance (retrieval_rpc_demo_pipeline.c:906-1000) ----
+ * The query on the 17x-refined SQ grid through the reference's
+ * retrieval_quantize_query_l1_aligned_u8; coarse distances through
+ * retrieval_compute_u8_u4_l1_distance_scalar (the DSP's split contract,
+ * retrieval_async_dsp_search.c:1004-1009); the delta through
+ * retrieval_compute_u8_u{4,8}_residual_dot_scalar and the bias, composed as
+ * sqmd_split_residual_delta_fn / sqmd_split_distance_for_id compose them
+ * (those two are static in the pipeline file); the traversal is the
+ * fake-DSP turn loop with coarse + delta as the applied distance. */
+typedef struct {
+    RefSqmdCtx base;
+    const float *bias;
+    uint8_t qu8[8192];
+    uint64_t sum_q;
+} RefSplitCtx;
+
+static float ref_split_delta(const RefSplitCtx *c, int id) {
+    const float *qp = c->base.res_qp + 2 * (size_t)id;
+    const uint8_t *res = c->base.res + (size_t)id * c->base.res_cb;
+    double dot = c->base.res_bits == 8
+        ? retrieval_compute_u8_u8_residual_dot_scalar(c->qu8, c->sum_q, res, c->base.dim,
+                                                      c->base.scale, c->base.zero, qp[0], qp[1])
+        : retrieval_compute_u8_u4_residual_dot_scalar(c->qu8, c->sum_q, res, c->base.dim,
+                                                      c->base.scale, c->base.zero, qp[0], qp[1]);
+    double delta = (double)c->bias[id] - 2.0 * dot;
+    return isfinite(delta) ? (float)delta : NAN;
+}
+
+static float ref_split_coarse(const RefSplitCtx *c, int id) {
+    return retrieval_compute_u8_u4_l1_distance_scalar(
+        c->qu8, c->base.codes + (size_t)id * c->base.cb, c->base.dim, c->base.scale);
+}
+
+static int ref_split_distance_fn(const SqmdWorkItem *work, float *coarse, void *ctx) {
+    *coarse = ref_split_coarse(ctx, work->neighbor_id);
+    return 0;
+}
+
+static float ref_split_apply(float coarse, float delta) {
+    float d = coarse + delta;
+    if (d < 0.0f && d > -1.0e-4f) d = 0.0f;
+    return d;
+}
+
+static float ref_split_correction_fn(int id, int source, float coarse, void *ctx) {
+    (void)source;
+    return ref_split_apply(coarse, ref_split_delta(ctx, id));
+}
+
+/* Returns the result count, or < 0 (-101: the query does not quantize,
+ * -103: a seed's split distance fails, as sqmd_make_seed rejects it). */
+int ref_sqmd_split_search(const int32_t *graph, int degree, int n, const uint8_t *codes,
+                          int dim, float scale, float zero, const uint8_t *res,
+                          const float *res_qp, int res_bits, const float *bias,
+                          const float *query, int entry_points, int graph_entry,
+                          unsigned int seed, int expand_count, int pool_capacity, int lanes,
+                          int32_t *out_ids, float *out_dists, int out_cap) {
+    RefSplitCtx c;
+    RetrievalQueryU8Stats qst;
+    int seed_ids[2048];
+    float seed_dists[2048];
+    int count;
+    if (entry_points > 2048 || dim > 8192) return -100;
+    memset(&c, 0, sizeof(c));
+    c.base.codes = codes;
+    c.base.dim = dim;
+    c.base.bits = 4;
+    c.base.cb = (dim + 1) / 2;
+    c.base.scale = scale;
+    c.base.zero = zero;
+    c.base.res = res;
+    c.base.res_qp = res_qp;
+    c.base.res_bits = res_bits;
+    c.base.res_cb = res_bits == 4 ? (dim + 1) / 2 : dim;
+    c.base.query = query;
+    c.bias = bias;
+    if (retrieval_quantize_query_l1_aligned_u8(query, dim, scale, zero, c.qu8, &qst)) return -101;
+    c.sum_q = qst.sum_q;
+    count = retrieval_sqmd_make_seed_ids(n, entry_points, graph_entry, seed, seed_ids,
+                                         entry_points);
+    if (count <= 0) return -102;
+    for (int i = 0; i < count; i++) {
+        float coarse = ref_split_coarse(&c, seed_ids[i]);
+        float delta = ref_split_delta(&c, seed_ids[i]);
+        float d = (!isfinite(coarse) || !isfinite(delta)) ? FLT_MAX : ref_split_apply(coarse, delta);
+        if (!isfinite(d) || d < 0.0f) return -103;
+        seed_dists[i] = d;
+    }
+    return sqmd_cpu_state_run_fake_dsp(graph, degree, n, pool_capacity, entry_points,
+                                       expand_count, seed_ids, seed_dists, count, lanes,
+                                       ref_split_distance_fn, &c, ref_split_correction_fn, &c,
+                                       out_ids, out_dists, out_cap);
+}
+
 /* ---- on-disk formats, written and read by the reference ---- */
 
 static void ref_fill_meta(RetrievalPqArtifactMetadata *m, const uint8_t *uuid, int n,

@@ -827,6 +827,69 @@ float orc_sq_residual_distance(const OrcIndex *ix, const float *query, int id) {
     return dist;
 }
 
+/* retrieval_l1_int8_core.c:65-110 (retrieval_quantize_query_l1_aligned_u8):
+ * step scale / 17 in double, round half away from zero, clip to [0, 255]. */
+int orc_sq_query_aligned_u8(const float *query, int dim, float scale, float zero,
+                            uint8_t *out, uint64_t *sum_q) {
+    const double qs = (double)scale / 17.0;
+    uint64_t sum = 0;
+    if (!query || !out || dim <= 0 || !isfinite(scale) || !isfinite(zero) || scale <= 0.0f ||
+        !isfinite(qs) || qs <= 0.0)
+        return -1;
+    for (int i = 0; i < dim; i++) {
+        const double raw = ((double)query[i] - (double)zero) / qs;
+        long code;
+        if (!isfinite(query[i]) || !isfinite(raw)) return -1;
+        code = lround(raw);
+        if (code < 0) code = 0;
+        else if (code > 255) code = 255;
+        out[i] = (uint8_t)code;
+        sum += (uint64_t)code;
+    }
+    if (sum_q) *sum_q = sum;
+    return 0;
+}
+
+/* The split distance of retrieval_rpc_demo_pipeline.c:936-1000
+ * (sqmd_split_residual_delta_fn, sqmd_split_distance_for_id) with the
+ * scalar kernels it calls: retrieval_l1_int8_core.c:171-191 (coarse: the
+ * int64 sum of (q - 17 c)^2 times (scale / 17)^2 in double, to float) and
+ * :193-260 (the residual dot from integer sums sum q v and sum v, combined
+ * in double); delta = (double)bias - 2 dot, to float; distance = coarse +
+ * delta in float, with (-1e-4, 0) clamped to 0.  Returns NAN where the
+ * reference would fail the candidate (non-finite or negative). */
+float orc_sq_split_distance(const OrcIndex *ix, const uint8_t *qu8, uint64_t sum_q, int id) {
+    const int cb = (ix->dim + 1) / 2;
+    const uint8_t *l1 = ix->sq_codes + (size_t)id * cb;
+    const uint8_t *res = ix->res_codes + (size_t)id * orc_residual_code_bytes(ix->dim, ix->res_bits);
+    const double qs = (double)ix->sq_scale / 17.0;
+    const float rs = ix->res_qparams[2 * (size_t)id], rz = ix->res_qparams[2 * (size_t)id + 1];
+    int64_t acc = 0;
+    uint64_t sum_qv = 0, sum_v = 0;
+    double dot, delta;
+    float coarse, dl, dist;
+    for (int i = 0; i < ix->dim; i++) {
+        const int c = (i & 1) ? (l1[i >> 1] >> 4) : (l1[i >> 1] & 15);
+        const int32_t diff = (int32_t)qu8[i] - 17 * c;
+        const uint32_t v = (uint32_t)orc_res_code(res, ix->res_bits, i);
+        acc += (int64_t)diff * (int64_t)diff;
+        sum_qv += (uint64_t)qu8[i] * (uint64_t)v;
+        sum_v += (uint64_t)v;
+    }
+    coarse = (float)((double)acc * qs * qs);
+    if (!isfinite(rs) || !isfinite(rz) || rs < 0.0f) return NAN;
+    dot = qs * (double)rs * (double)sum_qv + qs * (double)rz * (double)sum_q +
+          (double)ix->sq_zero * (double)rs * (double)sum_v +
+          (double)ix->dim * (double)ix->sq_zero * (double)rz;
+    delta = (double)ix->split_bias[id] - 2.0 * dot;
+    if (!isfinite(delta)) return NAN;
+    dl = (float)delta;
+    dist = coarse + dl;
+    if (dist < 0.0f && dist > -1.0e-4f) dist = 0.0f;
+    if (!isfinite(dist) || dist < 0.0f) return NAN;
+    return dist;
+}
+
 int orc_sqmd_seed_ids(int n, int entry_points, int graph_entry, uint32_t seed,
                       int *out, int capacity) {
     uint64_t limit, attempts = 0;
@@ -867,10 +930,14 @@ int orc_sqmd_search(const OrcIndex *ix, int correction, const float *query,
     uint32_t *visited = NULL;
     uint8_t *qsq = NULL;
     int psize = 0, fsize = 0, rc = -1, fcap, cb;
+    uint64_t sum_q = 0;
     memset(&st, 0, sizeof(st));
     if (!ix || !query || !seed_ids || seed_count <= 0 || entry_points <= 0 ||
         expand_count <= 0 || pool_capacity <= 0 || !out_ids || !out_distances ||
         out_capacity <= 0 || !ix->sq_codes)
+        return -1;
+    if (correction == 2 && (ix->sq_bits != 4 || !ix->split_bias || !ix->res_codes ||
+                            !ix->res_qparams))
         return -1;
     fcap = entry_points + pool_capacity * expand_count;
     cb = ix->sq_bits == 4 ? (ix->dim + 1) / 2 : ix->dim;
@@ -879,9 +946,15 @@ int orc_sqmd_search(const OrcIndex *ix, int correction, const float *query,
     visited = calloc(((size_t)ix->num_vectors + 31) / 32, sizeof(uint32_t));
     qsq = malloc((size_t)ix->dim);
     if (!pool || !front || !visited || !qsq) goto out;
-    orc_sq_quantize_query(query, ix->dim, ix->sq_bits, ix->sq_scale, ix->sq_zero, qsq);
+    if (correction == 2) {
+        if (orc_sq_query_aligned_u8(query, ix->dim, ix->sq_scale, ix->sq_zero, qsq, &sum_q))
+            goto out;
+    } else {
+        orc_sq_quantize_query(query, ix->dim, ix->sq_bits, ix->sq_scale, ix->sq_zero, qsq);
+    }
 #define ORC_DIST(id)                                                             \
-    (correction ? orc_sq_residual_distance(ix, query, (id))                      \
+    (correction == 2 ? orc_sq_split_distance(ix, qsq, sum_q, (id))               \
+     : correction ? orc_sq_residual_distance(ix, query, (id))                    \
                 : orc_sq_distance(qsq, ix->sq_codes + (size_t)(id) * cb, ix->dim, \
                                   ix->sq_bits, ix->sq_scale))
 #define ORC_VIS(id) (visited[(id) >> 5] & (1u << ((id) & 31)))
@@ -892,6 +965,7 @@ int orc_sqmd_search(const OrcIndex *ix, int correction, const float *query,
         if (ORC_VIS(id)) continue;
         visited[id >> 5] |= 1u << (id & 31);
         d = ORC_DIST(id);
+        if (isnan(d)) goto out;
         if (orc_sq_pool_insert(pool, &psize, pool_capacity, id, d))
             orc_sq_front_push(front, &fsize, fcap, id, d);
     }
@@ -924,6 +998,7 @@ int orc_sqmd_search(const OrcIndex *ix, int correction, const float *query,
         for (int i = 0; i < wc; i++) visited[work[i] >> 5] |= 1u << (work[i] & 31);
         for (int i = 0; i < wc; i++) {
             float d = ORC_DIST(work[i]);
+            if (isnan(d)) goto out;
             if (orc_sq_pool_insert(pool, &psize, pool_capacity, work[i], d))
                 orc_sq_front_push(front, &fsize, fcap, work[i], d);
             st.applied++;

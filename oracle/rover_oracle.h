@@ -47,6 +47,9 @@ typedef struct OrcIndex {
      * [num_vectors][2]; NULL refines without it */
     const uint8_t *e4_codes;
     const float *e4_qparams;
+    /* SQ split distance: the per-vector bias sum_d (2 a_d r_d + r_d^2) of an
+     * SQMDBIAS artifact, f32 [num_vectors]; NULL without one */
+    const float *split_bias;
 } OrcIndex;
 
 typedef struct OrcStats {
@@ -119,6 +122,15 @@ int orc_sqmd_seed_ids(int num_vectors, int entry_points, int graph_entry,
 /* retrieval_rpc_demo_support.c:1397-1443
  * (retrieval_demo_exact_distance_residual_scalar). */
 float orc_sq_residual_distance(const OrcIndex *index, const float *query, int id);
+
+/* SQ split distance (SQ4 codes, 4- or 8-bit residuals, split_bias set):
+ * the query quantized on the SQ grid refined 17x (u8), and for candidate id
+ * the coarse u8 x u4 distance plus (bias - 2 <query_u8, residual>).
+ * orc_sq_query_aligned_u8 returns 0 or -1 (bad query or scale). */
+int orc_sq_query_aligned_u8(const float *query, int dim, float scale, float zero,
+                            uint8_t *out, uint64_t *sum_q);
+float orc_sq_split_distance(const OrcIndex *index, const uint8_t *query_u8,
+                            uint64_t sum_q, int id);
 
 /* Graph construction as retrieval_search_build_graph_ex builds it with the
  * exact provider and one iteration (retrieval_search_build_core.c:682-776

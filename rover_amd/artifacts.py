@@ -21,6 +21,15 @@ generate_final_refine.py:134-145):
     256 + 8 n   4096-byte blocks of 4096 // ((dim+1)/2) rows, zero-padded
 
 Codes are low nibble first (even dimension in the low nibble).
+
+SQMDBIAS -- the split-distance bias of the SQ traversal (written by
+generate_sqmd_correction_bias.py:86-190, read by
+retrieval_rpc_demo_support.c:209-299): a 64-byte header
+<8sIIIIIIQQ16s> (magic, version 1, header bytes 64, n, dim, l1 bits 4,
+residual bits 4 or 8, payload bytes 4 n, FNV-1a64 of the payload, the
+first 16 bytes of SHA-256 over the SHA-256 digests of the l1, residual and
+qparams files) and f32 bias [n], bias_v = sum_d (2 a_d r_d + r_d^2) with
+a = SQ level (code * scale + zero) and r = residual level, in float64.
 """
 import hashlib
 import struct
@@ -146,3 +155,93 @@ def check_error4_identity(art, base_paths, identity_path):
             raise ValueError("error4 artifact is bound to another base file: %s" % p)
     if file_sha256(identity_path) != art.identity_sha256:
         raise ValueError("error4 artifact is bound to another identity file: %s" % identity_path)
+
+
+BIAS_HEADER = struct.Struct("<8sIIIIIIQQ16s")
+BIAS_MAGIC = b"SQMDBIAS"
+assert BIAS_HEADER.size == 64
+
+
+def fnv1a64(data):
+    """FNV-1a 64 over bytes (the bias payload checksum)."""
+    v = 14695981039346656037
+    for b in bytes(data):
+        v = ((v ^ b) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    return v
+
+
+def _fnv1a64_np(payload):
+    # the same recurrence over a NumPy byte array, 64-bit wrap-around
+    v = np.uint64(14695981039346656037)
+    prime = np.uint64(1099511628211)
+    with np.errstate(over="ignore"):
+        for b in np.frombuffer(payload, dtype=np.uint8):
+            v = (v ^ np.uint64(b)) * prime
+    return int(v)
+
+
+def _unpack_u4(packed, dim):
+    out = np.empty((packed.shape[0], dim), dtype=np.uint8)
+    out[:, 0::2] = packed[:, : (dim + 1) // 2] & np.uint8(0x0F)
+    if dim > 1:
+        out[:, 1::2] = packed[:, : dim // 2] >> np.uint8(4)
+    return out
+
+
+def split_bias(sq_codes, sq_scale, sq_zero, res_codes, res_qparams, res_bits, dim,
+               chunk=8192):
+    """Per-vector split-distance bias from SQ4 codes [n][(dim+1)/2] and
+    residual codes [n][(dim+1)/2 or dim] with (scale, zero) [n][2]: the
+    float64 arithmetic of generate_sqmd_correction_bias.py:159-176 (NumPy
+    row sums), cast to f32."""
+    n = sq_codes.shape[0]
+    if res_bits not in (4, 8):
+        raise ValueError("split bias needs 4- or 8-bit residuals")
+    qp = np.asarray(res_qparams, dtype=np.float32)
+    if not np.isfinite(qp).all() or (qp[:, 0] < 0).any():
+        raise ValueError("invalid residual quantization parameters")
+    out = np.empty(n, dtype="<f4")
+    for s0 in range(0, n, chunk):
+        s1 = min(n, s0 + chunk)
+        a = _unpack_u4(np.asarray(sq_codes[s0:s1]), dim).astype(np.float64) * float(sq_scale) \
+            + float(sq_zero)
+        rc = (_unpack_u4(np.asarray(res_codes[s0:s1]), dim) if res_bits == 4
+              else np.asarray(res_codes[s0:s1])).astype(np.float64)
+        scales = qp[s0:s1, 0].astype(np.float64)[:, None]
+        zeros = qp[s0:s1, 1].astype(np.float64)[:, None]
+        r = rc * scales + zeros
+        out[s0:s1] = np.sum(2.0 * a * r + r * r, axis=1, dtype=np.float64).astype("<f4")
+    if not np.isfinite(out).all():
+        raise ValueError("non-finite split bias")
+    return out
+
+
+def source_fingerprint(l1_sha256, residual_sha256, qparams_sha256):
+    """The bias's binding to its three source files (digests as bytes)."""
+    return hashlib.sha256(bytes(l1_sha256) + bytes(residual_sha256) +
+                          bytes(qparams_sha256)).digest()[:16]
+
+
+def split_bias_image(bias, dim, res_bits, fingerprint=b"\0" * 16):
+    """The SQMDBIAS file image of a bias array."""
+    bias = np.ascontiguousarray(bias, dtype="<f4")
+    payload = bias.tobytes()
+    return BIAS_HEADER.pack(BIAS_MAGIC, 1, 64, bias.shape[0], dim, 4, res_bits, len(payload),
+                            _fnv1a64_np(payload), bytes(fingerprint)) + payload
+
+
+def parse_split_bias(image):
+    """Header fields and the bias of an SQMDBIAS image, checked as the
+    reference's loader checks them (fingerprint left to the caller)."""
+    buf = bytes(image)
+    if len(buf) < 64:
+        raise ValueError("split bias image too short")
+    magic, ver, hb, n, dim, lb, rb, pb, fnv, fp = BIAS_HEADER.unpack_from(buf, 0)
+    if magic != BIAS_MAGIC or ver != 1 or hb != 64 or pb != 4 * n or len(buf) != 64 + pb:
+        raise ValueError("bad split bias header")
+    if _fnv1a64_np(buf[64:]) != fnv:
+        raise ValueError("split bias payload checksum mismatch")
+    bias = np.frombuffer(buf, dtype="<f4", count=n, offset=64)
+    if not np.isfinite(bias).all():
+        raise ValueError("non-finite split bias")
+    return dict(n=n, dim=dim, l1_bits=lb, residual_bits=rb, fingerprint=fp), bias.copy()

@@ -53,6 +53,7 @@ typedef struct RoverCtx {
     uint8_t *d_sq_res;         /* SQ residual codes and qparams          */
     float *d_sq_res_qp;
     int sq_res_bits;
+    float *d_split_bias;       /* SQMDBIAS payload for the split distance */
     /* SQ traversal scratch: exact visited bitmaps and HBM frontiers per
        wave, and the cached live-path seed list */
     uint32_t *d_sqmd_vis;

@@ -55,6 +55,8 @@ typedef struct RvIndex {
     const uint8_t *e4_blocks;
     const float *e4_qparams;
     int e4_vpb, e4_code_bytes;
+    /* SQ split distance: per-vector bias of an SQMDBIAS artifact [n]     */
+    const float *split_bias;
 } RvIndex;
 
 /* Learned early-termination predictor: 16 -> 64 -> 64 -> 1 MLP,

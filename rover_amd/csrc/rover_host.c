@@ -122,6 +122,8 @@ AEEResult calculator_retrieval_unload(remote_handle64 h) {
     dfree(c->d_sq_res);
     dfree(c->d_sq_res_qp);
     dfree(c->d_sqmd_seeds);
+    dfree(c->d_split_bias);
+    c->d_split_bias = NULL;
     c->d_sq_graph = NULL;
     c->d_sq_codes = NULL;
     c->d_sq_res = NULL;
@@ -333,8 +335,10 @@ AEEResult rover_sq_residual_load(remote_handle64 h, const uint8_t *codes,
     pthread_mutex_lock(&c->lock);
     dfree(c->d_sq_res);
     dfree(c->d_sq_res_qp);
+    dfree(c->d_split_bias);  /* a bias is built from one residual */
     c->d_sq_res = NULL;
     c->d_sq_res_qp = NULL;
+    c->d_split_bias = NULL;
     c->sq_res_bits = 0;
     if (dalloc_copy((void **)&c->d_sq_res, codes, (size_t)codesLen) ||
         dalloc_copy((void **)&c->d_sq_res_qp, qparams, (size_t)qparamsLen * sizeof(float))) {
@@ -355,6 +359,64 @@ static uint32_t le32(const uint8_t *p) {
 }
 
 static uint64_t le64(const uint8_t *p) { return le32(p) | (uint64_t)le32(p + 4) << 32; }
+
+/* retrieval_rpc_demo_support.c:227-235 (sqmd_fnv1a64) */
+static uint64_t fnv1a64(const uint8_t *p, size_t n) {
+    uint64_t v = 14695981039346656037ull;
+    for (size_t i = 0; i < n; i++) {
+        v ^= p[i];
+        v *= 1099511628211ull;
+    }
+    return v;
+}
+
+/* retrieval_rpc_demo_support.c:237-299 (sqmd_load_correction_bias): the
+ * 64-byte SQMDBIAS header (magic, version 1, header bytes 64, vectors,
+ * dim, l1 bits, residual bits, payload bytes, payload FNV-1a64, source
+ * fingerprint) checked against the loaded SQ index and residual, then the
+ * f32 payload, every value finite. */
+AEEResult rover_sq_split_bias_load(remote_handle64 h, const uint8_t *image, int64_t imageLen,
+                                   const uint8_t *source_fingerprint) {
+    RoverCtx *c = ctx_of(h);
+    static const uint8_t magic[8] = {'S', 'Q', 'M', 'D', 'B', 'I', 'A', 'S'};
+    const float *bias;
+    size_t payload;
+    if (!c || !image) return ROVER_EBADPARM;
+    if (!c->sq_loaded || !c->d_sq_res) {
+        snprintf(g_err, sizeof(g_err), "split bias: load the SQ index and its residual first");
+        return ROVER_EBADPARM;
+    }
+    payload = (size_t)c->sq_n * sizeof(float);
+    if (imageLen != (int64_t)(64 + payload) || memcmp(image, magic, 8) != 0 ||
+        le32(image + 8) != 1u || le32(image + 12) != 64u || le32(image + 16) != (uint32_t)c->sq_n ||
+        le32(image + 20) != (uint32_t)c->sq_dim || le32(image + 24) != (uint32_t)c->sq_bits ||
+        c->sq_bits != 4 || le32(image + 28) != (uint32_t)c->sq_res_bits ||
+        le64(image + 32) != (uint64_t)payload ||
+        (source_fingerprint && memcmp(image + 48, source_fingerprint, 16) != 0) ||
+        fnv1a64(image + 64, payload) != le64(image + 40)) {
+        snprintf(g_err, sizeof(g_err), "split bias: header, fingerprint or payload check failed");
+        return ROVER_EBADPARM;
+    }
+    bias = (const float *)(image + 64);
+    for (int i = 0; i < c->sq_n; i++) {
+        float v;
+        memcpy(&v, bias + i, sizeof(v));
+        if (!isfinite(v)) {
+            snprintf(g_err, sizeof(g_err), "split bias: non-finite value");
+            return ROVER_EBADPARM;
+        }
+    }
+    pthread_mutex_lock(&c->lock);
+    dfree(c->d_split_bias);
+    c->d_split_bias = NULL;
+    if (dalloc_copy((void **)&c->d_split_bias, image + 64, payload)) {
+        pthread_mutex_unlock(&c->lock);
+        set_err("split bias upload");
+        return ROVER_ENOMEMORY;
+    }
+    pthread_mutex_unlock(&c->lock);
+    return ROVER_SUCCESS;
+}
 
 AEEResult rover_pure_pq_error4_unload(remote_handle64 h) {
     RoverCtx *c = ctx_of(h);
@@ -811,7 +873,7 @@ static AEEResult launch_sqmd(RoverCtx *c, const RoverSearchBatch *b, int max_poo
     if (b->num_queries < 0 || (b->num_queries > 0 && !b->queries) || b->expand_count <= 0 ||
         max_pool <= 0 || max_pool > ROVER_MAX_POOL || b->out_stride <= 0 || !b->out_ids ||
         !b->out_dists || !b->out_counts || b->refine_candidates || b->learned_stop ||
-        (b->correction != 0 && b->correction != 1) || (dsp && b->correction) ||
+        (b->correction < 0 || b->correction > 2) || (dsp && b->correction) ||
         b->entry_points <= 0)
         return ROVER_EBADPARM;
     /* Caller-given seed lists and expansions of more than 32 neighbors are
@@ -841,6 +903,15 @@ static AEEResult launch_sqmd(RoverCtx *c, const RoverSearchBatch *b, int max_poo
         ix.res_qparams = c->d_sq_res_qp;
         ix.res_bits = c->sq_res_bits;
         ix.res_code_bytes = c->sq_res_bits == 4 ? (c->sq_dim + 1) / 2 : c->sq_dim;
+    }
+    if (b->correction == 2) {
+        /* the split distance needs SQ4 codes and a loaded bias */
+        if (!c->d_split_bias || c->sq_bits != 4) {
+            snprintf(g_err, sizeof(g_err), "search: the split distance needs SQ4 codes and "
+                                           "rover_sq_split_bias_load");
+            return ROVER_EBADPARM;
+        }
+        ix.split_bias = c->d_split_bias;
     }
     memset(&sb, 0, sizeof(sb));
     if (dsp) {

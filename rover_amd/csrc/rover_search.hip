@@ -1909,6 +1909,42 @@ __device__ __forceinline__ float rv_sq_exact(const RvIndex &ix, const float *qv,
     return dist;
 }
 
+// The split distance (retrieval_rpc_demo_pipeline.c:936-1000 with the
+// scalar kernels of retrieval_l1_int8_core.c:171-260): qu8 is the query on
+// the SQ grid refined 17x (u8) and sum_q its sum.  coarse = the integer sum
+// of (q - 17 c)^2 times (scale / 17)^2 in double, to float; the residual dot
+// from the integer sums sum q v and sum v, combined in double; delta =
+// (double)bias - 2 dot, to float; distance = coarse + delta in float, with
+// (-1e-4, 0) clamped to 0.  NAN where the reference fails the candidate.
+__device__ __forceinline__ float rv_sq_split(const RvIndex &ix, const uint8_t *qu8,
+                                             uint64_t sum_q, int id) {
+    const uint8_t *l1 = ix.sq_codes + (size_t)id * ix.sq_code_bytes;
+    const uint8_t *res = ix.res_codes + (size_t)id * ix.res_code_bytes;
+    const float rs = ix.res_qparams[2 * (size_t)id], rz = ix.res_qparams[2 * (size_t)id + 1];
+    const double qs = (double)ix.sq_scale / 17.0;
+    int64_t acc = 0;
+    uint64_t sum_qv = 0, sum_v = 0;
+    for (int d = 0; d < ix.dim; d++) {
+        const int c = (d & 1) ? (l1[d >> 1] >> 4) : (l1[d >> 1] & 15);
+        const int diff = (int)qu8[d] - 17 * c;
+        const uint32_t v = (uint32_t)rv_res_code(res, ix.res_bits, d);
+        acc += (int64_t)(diff * diff);
+        sum_qv += (uint64_t)((uint32_t)qu8[d] * v);
+        sum_v += v;
+    }
+    const float coarse = (float)((double)acc * qs * qs);
+    if (!isfinite(rs) || !isfinite(rz) || rs < 0.0f) return __int_as_float(0x7fc00000);
+    const double dot = qs * (double)rs * (double)sum_qv + qs * (double)rz * (double)sum_q +
+                       (double)ix.sq_zero * (double)rs * (double)sum_v +
+                       (double)ix.dim * (double)ix.sq_zero * (double)rz;
+    const double delta = (double)ix.split_bias[id] - 2.0 * dot;
+    if (!isfinite(delta)) return __int_as_float(0x7fc00000);
+    float dist = coarse + (float)delta;
+    if (dist < 0.0f && dist > -1.0e-4f) dist = 0.0f;
+    if (!isfinite(dist) || dist < 0.0f) return __int_as_float(0x7fc00000);
+    return dist;
+}
+
 struct RvSqmdLayout {
     int off_q, off_qsq, off_pool, off_wid, off_wd, off_front, total;
 };
@@ -1917,7 +1953,7 @@ static RvSqmdLayout rv_sqmd_layout(int dim, int max_pool, int correction, int fr
     RvSqmdLayout l;
     int off = 0;
     l.off_q = off;
-    if (correction) off += rv_align(dim * 4, 16);
+    if (correction == 1) off += rv_align(dim * 4, 16);
     l.off_qsq = off;
     off += rv_align(dim, 16);
     l.off_pool = off;
@@ -1975,12 +2011,33 @@ __global__ __launch_bounds__(RV_WAVE) void rv_sqmd_kernel(RvIndex ix, RvSqmdBatc
             *reinterpret_cast<uint4 *>(vis + w) = make_uint4(0u, 0u, 0u, 0u);
         __threadfence();
         const float *qg = b.queries + (size_t)q * ix.dim;
-        {
+        uint64_t sum_q = 0;
+        bool query_bad = false;
+        if (CORR == 2) {
+            // retrieval_l1_int8_core.c:65-110: step scale / 17 in double,
+            // round half away from zero, clip to [0, 255]
+            const double qs = (double)ix.sq_scale / 17.0;
+            uint32_t part = 0;
+            bool bad = false;
+            for (int d = lane; d < ix.dim; d += RV_WAVE) {
+                const float x = qg[d];
+                const double raw = ((double)x - (double)ix.sq_zero) / qs;
+                if (!isfinite(x) || !isfinite(raw)) bad = true;
+                long v = bad ? 0 : lround(raw);
+                v = v < 0 ? 0 : (v > 255 ? 255 : v);
+                qsq[d] = (uint8_t)v;
+                part += (uint32_t)v;
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
+            sum_q = (uint64_t)rv_uniform((int)part);
+            query_bad = __ballot(bad) != 0ull;
+        } else {
             const float inv = 1.0f / ix.sq_scale;
             const int hi = ix.sq_bits == 4 ? 15 : 255;
             for (int d = lane; d < ix.dim; d += RV_WAVE) {
                 const float x = qg[d];
-                if (CORR) qv[d] = x;
+                if (CORR == 1) qv[d] = x;
                 int v = (int)roundf((x - ix.sq_zero) * inv);
                 v = v < 0 ? 0 : (v > hi ? hi : v);
                 qsq[d] = (uint8_t)v;
@@ -1989,19 +2046,20 @@ __global__ __launch_bounds__(RV_WAVE) void rv_sqmd_kernel(RvIndex ix, RvSqmdBatc
         __syncthreads();
 
         auto score = [&](int id) -> float {
-            if (CORR) return rv_sq_exact(ix, qv, id);
+            if (CORR == 2) return rv_sq_split(ix, qsq, sum_q, id);
+            if (CORR == 1) return rv_sq_exact(ix, qv, id);
             return rv_sq(qsq, ix.sq_codes + (size_t)id * ix.sq_code_bytes, ix.dim,
                          ix.sq_bits, scale_sq);
         };
 
         int psize = 0, fsize = 0;  // meaningful in lane 0
-        int status = 0;
+        int status = query_bad ? RV_QUERY_NONFINITE : 0;
         uint64_t st_work = 0, st_invalid = 0, st_turns = 0, st_dedup = 0;
 
         // ---- seeds (:446-476), 64 at a time in list order
         const int32_t *seeds = b.seeds + (b.seed_stride > 0 ? (size_t)q * b.seed_stride : 0);
         const int ns = b.seed_stride > 0 ? b.seed_stride : b.n_seeds;
-        for (int base = 0; base < ns; base += RV_WAVE) {
+        for (int base = 0; base < ns && status == 0; base += RV_WAVE) {
             const int i = base + lane;
             const bool active = i < ns;
             const int id = active ? seeds[i] : -1;
@@ -2020,8 +2078,13 @@ __global__ __launch_bounds__(RV_WAVE) void rv_sqmd_kernel(RvIndex ix, RvSqmdBatc
                 fresh = (atomicOr(vis + (id >> 5), bit) & bit) == 0u;
             }
             wid[lane] = id;
-            wd[lane] = fresh ? score(id) : 0.0f;
+            const float sd = fresh ? score(id) : 0.0f;
+            wd[lane] = sd;
             const uint64_t mask = __ballot(fresh);
+            if (CORR == 2 && __ballot(fresh && isnan(sd)) != 0ull) {
+                status = RV_QUERY_NONFINITE;
+                break;
+            }
             __syncthreads();
             if (lane == 0) {
                 for (uint64_t m = mask; m; m &= m - 1) {
@@ -2078,7 +2141,12 @@ __global__ __launch_bounds__(RV_WAVE) void rv_sqmd_kernel(RvIndex ix, RvSqmdBatc
             st_dedup += (uint64_t)__popcll(__ballot(valid && !fresh));
             st_work += (uint64_t)__popcll(mask);
             wid[lane] = nb;
-            wd[lane] = fresh ? score(nb) : 0.0f;
+            const float nd = fresh ? score(nb) : 0.0f;
+            wd[lane] = nd;
+            if (CORR == 2 && __ballot(fresh && isnan(nd)) != 0ull) {
+                status = RV_QUERY_NONFINITE;
+                break;
+            }
             __syncthreads();
             if (lane == 0) {
                 for (uint64_t m = mask; m; m &= m - 1) {
@@ -2167,8 +2235,9 @@ static int rv_sqmd_occupancy(int lds) {
 extern "C" int rv_sqmd_grid(const RvIndex *ix, const RvSqmdBatch *b, int *lds_frontier) {
     int front_lds = 0;
     const RvSqmdLayout lay = rv_sqmd_layout_for(ix, b, &front_lds);
-    const int per = b->correction ? rv_sqmd_occupancy<1>(lay.total)
-                                  : rv_sqmd_occupancy<0>(lay.total);
+    const int per = b->correction == 2   ? rv_sqmd_occupancy<2>(lay.total)
+                    : b->correction == 1 ? rv_sqmd_occupancy<1>(lay.total)
+                                         : rv_sqmd_occupancy<0>(lay.total);
     const int cus = rv_sqmd_cus();
     if (per < 0 || cus < 0) return -1;
     if (lds_frontier) *lds_frontier = front_lds;
@@ -2184,7 +2253,8 @@ extern "C" int rv_sqmd_launch(const RvIndex *ix, const RvSqmdBatch *b, int grid_
     if (b->nq == 0) return 0;
     if (!ix->sq_codes || !ix->graph || !b->visited || b->visited_words % 4 ||
         b->visited_words < (ix->n + 31) / 32 || (!front_lds && !b->frontier) ||
-        (b->correction && (!ix->res_codes || !ix->res_qparams))) {
+        (b->correction && (!ix->res_codes || !ix->res_qparams)) ||
+        (b->correction == 2 && (!ix->split_bias || ix->sq_bits != 4))) {
         snprintf(g_rv_error, sizeof(g_rv_error), "rv_sqmd_launch: bad arguments");
         return -1;
     }
@@ -2197,7 +2267,10 @@ extern "C" int rv_sqmd_launch(const RvIndex *ix, const RvSqmdBatch *b, int grid_
     if (grid_waves > b->nq) grid_waves = b->nq;
     if (rv_fill_zero(b->queue, sizeof(int32_t), stream)) return -1;
     hipStream_t s = (hipStream_t)stream;
-    if (b->correction) {
+    if (b->correction == 2) {
+        if (rv_sqmd_occupancy<2>(lay.total) < 0) return -1;
+        rv_sqmd_kernel<2><<<grid_waves, RV_WAVE, lay.total, s>>>(*ix, bb, lay);
+    } else if (b->correction) {
         if (rv_sqmd_occupancy<1>(lay.total) < 0) return -1;
         rv_sqmd_kernel<1><<<grid_waves, RV_WAVE, lay.total, s>>>(*ix, bb, lay);
     } else {

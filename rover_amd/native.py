@@ -128,6 +128,7 @@ SIGNATURES = [
      [_H, _P, C.c_int64, _P, C.c_int64, C.c_int32]),
     ("rover_pure_pq_original_ids", _I, [_H, _P, _I]),
     ("rover_sq_residual_load", _I, [_H, _P, C.c_int64, _P, C.c_int64, C.c_int32]),
+    ("rover_sq_split_bias_load", _I, [_H, _P, C.c_int64, _P]),
     ("rover_pure_pq_error4_load", _I, [_H, _P, C.c_int64, _P]),
     ("rover_pure_pq_error4_unload", _I, [_H]),
     ("rover_sha256", None, [_P, C.c_size_t, _P]),

@@ -142,6 +142,18 @@ class Calculator:
         check(native.lib().rover_sq_residual_load(self.h, ptr(r), r.size, ptr(q), q.size,
                                                   int(bits)), "rover_sq_residual_load")
 
+    def sq_split_bias_load(self, image, fingerprint=None):
+        """The SQMDBIAS artifact (file image) for searches with correction=2;
+        fingerprint: its 16-byte binding to the l1 / residual / qparams
+        files, checked when given."""
+        buf = np.frombuffer(bytes(image), dtype=np.uint8)
+        fp = None if fingerprint is None else np.frombuffer(bytes(fingerprint), dtype=np.uint8)
+        if fp is not None and fp.size != 16:
+            raise ValueError("fingerprint must be 16 bytes")
+        check(native.lib().rover_sq_split_bias_load(self.h, ptr(buf), buf.size,
+                                                    None if fp is None else ptr(fp)),
+              "rover_sq_split_bias_load")
+
     def retrieval_unload(self):
         check(native.lib().calculator_retrieval_unload(self.h), "retrieval_unload")
         self.sq = None

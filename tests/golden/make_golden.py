@@ -30,7 +30,7 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 import ctypes as C  # noqa: E402
 
 import pyoracle  # noqa: E402
-from rover_amd import data, index  # noqa: E402
+from rover_amd import artifacts, data, index  # noqa: E402
 
 
 def _p(a):
@@ -292,6 +292,55 @@ def sqmd_case(name, pq_name="pq_d128_m32", sq_name="sq_d128"):
     print("wrote", path, os.path.getsize(path), "bytes")
 
 
+def split_case(name, pq_name="pq_d128_m32", sq_name="sq_d128"):
+    """The SQ traversal ranked by the split distance, through the
+    reference's own arithmetic (retrieval_quantize_query_l1_aligned_u8,
+    retrieval_compute_u8_u4_l1_distance_scalar, the residual dot kernels)
+    composed as the pipeline composes them, and its fake-DSP turn loop;
+    SQ4 codes of the SQ fixture, 4- and 8-bit residuals from
+    rover_amd.index's encoder and biases from rover_amd.artifacts.split_bias
+    (inputs here).  Writes <name>.npz."""
+    ref = pyoracle.reference()
+    g = dict(np.load(os.path.join(HERE, pq_name + ".npz"), allow_pickle=False))
+    s = dict(np.load(os.path.join(HERE, sq_name + ".npz"), allow_pickle=False))
+    base = g["base_x16"].astype(np.float32) / np.float32(16.0)
+    queries = g["queries_x16"].astype(np.float32) / np.float32(16.0)
+    graph, entry = g["graph"], int(g["entry"])
+    n, dim = base.shape
+    codes = s["sq4_codes"]
+    scale, zero = np.float32(s["sq4_scale"]), np.float32(s["sq4_zero"])
+    out = dict(entry=np.int32(entry))
+    cases = []
+    for res_bits in (4, 8):
+        res, rqp = index.encode_sq_residual(base, codes, 4, scale, zero, res_bits)
+        bias = artifacts.split_bias(codes, scale, zero, res, rqp, res_bits, dim)
+        out["r%d_res_codes" % res_bits] = res
+        out["r%d_res_qparams" % res_bits] = rqp
+        out["r%d_bias" % res_bits] = bias
+        for ep, expand, L in ((1, 32, 16), (32, 32, 16), (32, 32, 64), (8, 12, 40)):
+            key = "r%d_e%d_x%d_L%d" % (res_bits, ep, expand, L)
+            ids_all, d_all, cnt_all = [], [], []
+            for q in queries:
+                ids = np.zeros(L, dtype=np.int32)
+                d = np.zeros(L, dtype=np.float32)
+                rc = ref.ref_sqmd_split_search(_p(graph), graph.shape[1], n, _p(codes), dim,
+                                               scale, zero, _p(res), _p(rqp), res_bits,
+                                               _p(bias), _p(q), ep, entry, 42, expand, L, 8,
+                                               _p(ids), _p(d), L)
+                assert rc > 0, (key, rc)
+                ids_all.append(ids)
+                d_all.append(d)
+                cnt_all.append(rc)
+            out[key + "_ids"] = np.stack(ids_all)
+            out[key + "_dists"] = np.stack(d_all)
+            out[key + "_counts"] = np.array(cnt_all, dtype=np.int32)
+            cases.append((res_bits, ep, expand, L))
+    out["cases"] = np.array(cases, dtype=np.int32)
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
 def graph_case(name):
     """The reference's search-build with the exact provider
     (retrieval_search_build_graph_ex, one iteration) on small inputs:
@@ -386,6 +435,9 @@ def main():
     if sys.argv[1:] == ["sqmd"]:
         sqmd_case("sqmd_d128")
         return
+    if sys.argv[1:] == ["split"]:
+        split_case("sqmd_split")
+        return
     if sys.argv[1:] == ["error4"]:
         for name in ("pq_d128_m32", "pq_d96_m48", "pq_d64_ks16"):
             error4_case(name)
@@ -402,6 +454,7 @@ def main():
     for name in ("pq_d128_m32", "pq_d96_m48", "pq_d64_ks16"):
         error4_case(name)
     sqmd_case("sqmd_d128")
+    split_case("sqmd_split")
     graph_case("graph_build")
 
 
