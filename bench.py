@@ -152,7 +152,9 @@ def sq_leg_run(args, n=100_000, nq=1000, L=64, k=10, ep=32):
     the best-first SQ search over the same index, HIP events around single
     launches with queries in HBM; the oracle's SQ traversal on the host for
     comparison.  The graph is built on the device by the reference's
-    exact-candidate search-build (exact 64-NN, RNG prune to 32)."""
+    exact-candidate search-build (exact 64-NN, RNG prune to 32).  The same
+    base with SQ4 codes and 4-bit residuals runs the traversal plain,
+    corrected and with the split distance (SQMDBIAS bias)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     base, queries = data.workload(n, 128, 256, 8.0, nq, base_seed=7, query_seed=8)
@@ -213,6 +215,49 @@ def sq_leg_run(args, n=100_000, nq=1000, L=64, k=10, ep=32):
             el = time.time() - t1
             row.update(cpu_oracle_qps_1thread=round(nq / el, 1),
                        gpu_ids_identical_frac=same / nq)
+        log("sq leg %s: %s" % (name, row))
+        out[name] = row
+    # the split distance needs SQ4 codes: the same base and graph with SQ4
+    # codes, 4-bit residuals of the SQ4 reconstruction and their SQMDBIAS
+    scale4, zero4 = index.sq_params(base, 4)
+    codes4 = index.encode_sq(base, 4, scale4, zero4)
+    res4, rqp4 = index.encode_sq_residual(base, codes4, 4, scale4, zero4, 4)
+    bias4 = artifacts.split_bias(codes4, scale4, zero4, res4, rqp4, 4, 128)
+    calc.rover_configure(l1_bits=4)
+    calc.retrieval_load(graph, codes4, 128, entry, scale4, zero4)
+    calc.sq_residual_load(res4, rqp4, 4)
+    calc.sq_split_bias_load(artifacts.split_bias_image(bias4, 128, 4))
+    oix4 = pyoracle.Index(graph, res_codes=res4, res_qparams=rqp4, res_bits=4,
+                          sq_codes=codes4, sq_bits=4, sq_scale=float(scale4),
+                          sq_zero=float(zero4), dim=128, split_bias=bias4)
+    for name, corr in (("sq4_sqmd", 0), ("sq4_sqmd_corrected", 1), ("sq4_sqmd_split", 2)):
+        def launch():
+            calc.search_batch_device(dq.ptr, nq, L, k, d_ids.ptr, d_d.ptr, d_cnt.ptr, k,
+                                     method=native.METHOD_SQ, correction=corr,
+                                     entry_points=ep, expand_count=32, d_stats=d_st.ptr,
+                                     traversal=native.TRAVERSAL_SQMD)
+        launch()
+        native.synchronize()
+        ids = d_ids.download(np.int32, nq * k).reshape(nq, k)
+        cnt = d_cnt.download(np.int32, nq)
+        st = d_st.download(np.uint64, nq * 4).reshape(nq, 4)
+        assert (cnt == k).all(), (name, np.unique(cnt))
+        for _ in range(args.warmup):
+            launch()
+        e0, e1 = native.Event(), native.Event()
+        e0.record()
+        for _ in range(args.steps):
+            launch()
+        e1.record()
+        ms = e0.elapsed_ms(e1) / args.steps
+        same = 0
+        for qi in range(nq):
+            oi, od, _ = pyoracle.sqmd_search(oix4, queries[qi], seeds, ep, 32, L, k, corr)
+            same += int(np.array_equal(oi, ids[qi]))
+        row = dict(recall10=round(data.recall_at_k(ids, cnt, gt, k), 4),
+                   qps=round(nq / (ms / 1e3), 1), ms_per_batch=round(ms, 4),
+                   mean_expanded=round(float(st[:, 2].mean()), 2),
+                   gpu_ids_identical_frac=same / nq)
         log("sq leg %s: %s" % (name, row))
         out[name] = row
     calc.close()
