@@ -481,6 +481,19 @@ AEEResult rover_build_graph_exact(const float *d_x, int n, int dim, int r,
                                   int candidates, uint32_t seed, int prune_mode,
                                   int32_t *d_graph);
 
+/* RNN-Descent graph construction on the device, bit-identical to the
+ * reference's retrieval_rnn_build_graph_ex
+ * (retrieval_rnn_descent_core.c:1432-1548) followed by
+ * retrieval_rnn_export_topk (:475-570): s random initial neighbors per node
+ * (LCG from `seed`), t1 rounds of t2 update passes (occlusion by earlier
+ * row members, pending insertions applied per node in (src, dst) order),
+ * reverse edges between rounds, rows of at most r; d_graph [n][top_k]
+ * by (distance, id), -1 padded.  d_x [n][dim] and d_graph are device
+ * pointers; s <= r <= 64, top_k <= r; synchronous. */
+AEEResult rover_build_graph_rnn(const float *d_x, int n, int dim, int s, int r,
+                                int t1, int t2, uint32_t seed, int top_k,
+                                int32_t *d_graph);
+
 /* Device memory helpers, so callers need no HIP headers. */
 AEEResult rover_device_alloc(void **d_ptr, size_t bytes);
 AEEResult rover_device_free(void *d_ptr);

@@ -72,6 +72,7 @@ def oracle():
         L.orc_search_batch.argtypes = [P, I, P, I, P, I, I, I, P, P, I, I, I, P,
                                        P, P, P]
         L.orc_build_exact.argtypes = [P, I, I, I, I, C.c_uint32, I, P]
+        L.orc_rnn_build.argtypes = [P, I, I, I, I, I, I, C.c_uint32, I, P]
         L.orc_l2.argtypes = [P, I, I, I]
         L.orc_l2.restype = F
         L.orc_sqmd_search.argtypes = [P, I, P, P, I, I, I, I, P, P, I, P]
@@ -126,6 +127,7 @@ def reference():
         L.ref_results_write.argtypes = [C.c_char_p, I, C.c_char_p, I, P, P, P, I]
         L.ref_nsg_load.argtypes = [C.c_char_p, I, I, P, P]
         L.ref_build_exact.argtypes = [P, I, I, I, I, C.c_uint, I, I, P]
+        L.ref_rnn_build.argtypes = [P, I, I, I, I, I, I, C.c_uint, I, I, P]
         L.ref_sqmd_search.argtypes = [P, I, I, P, I, I, F, F, P, P, I, I, P, I, I,
                                       C.c_uint, I, I, I, P, P, I, P]
         L.ref_sqmd_split_search.argtypes = [P, I, I, P, I, F, F, P, P, I, P, P, I, I,
@@ -380,6 +382,17 @@ def build_exact(base, r, candidates, seed=42, prune_mode=1):
     n, dim = x.shape
     out = np.zeros((n, r), dtype=np.int32)
     rc = oracle().orc_build_exact(_p(x), n, dim, r, candidates, seed, prune_mode, _p(out))
+    assert rc == 0
+    return out
+
+
+def rnn_build(base, s, r, t1, t2, seed=42, top_k=None):
+    """RNN-Descent; int32 [n][top_k], -1 padded."""
+    x = np.ascontiguousarray(base, dtype=np.float32)
+    n, dim = x.shape
+    top_k = r if top_k is None else top_k
+    out = np.zeros((n, top_k), dtype=np.int32)
+    rc = oracle().orc_rnn_build(_p(x), n, dim, s, r, t1, t2, seed, top_k, _p(out))
     assert rc == 0
     return out
 

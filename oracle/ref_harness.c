@@ -808,3 +808,30 @@ int ref_build_exact(const float *base, int n, int dim, int r, int candidates,
     retrieval_rnn_graph_free(&g);
     return rc;
 }
+
+/* RNN-Descent: retrieval_rnn_build_graph_ex with `threads` workers and one
+ * distance per batch entry (the CPU provider), then
+ * retrieval_rnn_export_topk; out [n][top_k]. */
+int ref_rnn_build(const float *base, int n, int dim, int s, int r, int t1, int t2,
+                  unsigned int seed, int threads, int top_k, int32_t *out) {
+    RetrievalRnnBuildConfig cfg;
+    RetrievalRnnBuildRuntime rt;
+    RetrievalRnnGraph g;
+    int rc;
+    memset(&cfg, 0, sizeof(cfg));
+    memset(&rt, 0, sizeof(rt));
+    memset(&g, 0, sizeof(g));
+    cfg.s = s;
+    cfg.r = r;
+    cfg.t1 = t1;
+    cfg.t2 = t2;
+    cfg.seed = seed;
+    rt.threads = threads;
+    rt.batch_size = 1;
+    rt.min_batch = 1;
+    rc = retrieval_rnn_build_graph_ex(base, n, dim, &cfg, &rt, &g, NULL);
+    if (rc) return rc;
+    rc = retrieval_rnn_export_topk(&g, top_k, out);
+    retrieval_rnn_graph_free(&g);
+    return rc;
+}

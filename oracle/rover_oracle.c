@@ -1263,3 +1263,203 @@ int orc_search_batch(const OrcIndex *ix, int scorer, const float *queries,
     for (int t = 0; t < threads; t++) pthread_join(tid[t], NULL);
     return b.failed ? -1 : 0;
 }
+
+/* ---- RNN-Descent (retrieval_rnn_descent_core.c:1432-1548) ----
+ * Rows hold up to r (id, distance, new-flag) entries.  Initial rows: s
+ * distinct random others per node from the node's LCG (:571-738); then t1
+ * rounds of t2 update passes (:901-1161), with reverse edges added between
+ * rounds (:1163-1305); rows sorted by (distance, id) and exported with -1
+ * padding (:438-470, :475-570). */
+
+typedef struct {
+    int n, r;
+    int *deg;
+    int *ids;
+    float *dist;
+    uint8_t *fresh;
+} OrcRnn;
+
+typedef struct {
+    int src, dst;
+    float dist;
+} OrcIns;
+
+static int orc_rnn_better(float nd, int ni, float od, int oi) {
+    return nd < od || (nd == od && ni < oi);
+}
+
+static int orc_rnn_has(const OrcRnn *g, int src, int dst) {
+    for (int i = 0; i < g->deg[src]; i++)
+        if (g->ids[(size_t)src * g->r + i] == dst) return 1;
+    return 0;
+}
+
+/* :304-375 */
+static int orc_rnn_insert(OrcRnn *g, int src, int dst, float d) {
+    size_t row = (size_t)src * g->r;
+    int far = -1;
+    if (src == dst || orc_rnn_has(g, src, dst)) return 0;
+    if (g->deg[src] < g->r) {
+        g->ids[row + g->deg[src]] = dst;
+        g->dist[row + g->deg[src]] = d;
+        g->fresh[row + g->deg[src]] = 1;
+        g->deg[src]++;
+        return 1;
+    }
+    for (int i = 0; i < g->deg[src]; i++)
+        if (far < 0 || orc_rnn_better(g->dist[row + far], g->ids[row + far], g->dist[row + i],
+                                      g->ids[row + i]))
+            far = i;
+    if (far >= 0 && orc_rnn_better(d, dst, g->dist[row + far], g->ids[row + far])) {
+        g->ids[row + far] = dst;
+        g->dist[row + far] = d;
+        g->fresh[row + far] = 1;
+        return 1;
+    }
+    return 0;
+}
+
+/* :438-470, insertion sort by (distance, id) */
+static void orc_rnn_sort_row(OrcRnn *g, int u) {
+    size_t row = (size_t)u * g->r;
+    for (int i = 1; i < g->deg[u]; i++) {
+        int id = g->ids[row + i], j = i - 1;
+        float d = g->dist[row + i];
+        uint8_t f = g->fresh[row + i];
+        while (j >= 0 && orc_rnn_better(d, id, g->dist[row + j], g->ids[row + j])) {
+            g->ids[row + j + 1] = g->ids[row + j];
+            g->dist[row + j + 1] = g->dist[row + j];
+            g->fresh[row + j + 1] = g->fresh[row + j];
+            j--;
+        }
+        g->ids[row + j + 1] = id;
+        g->dist[row + j + 1] = d;
+        g->fresh[row + j + 1] = f;
+    }
+}
+
+static int orc_ins_cmp(const void *a, const void *b) {
+    const OrcIns *x = a, *y = b;
+    if (x->src != y->src) return x->src < y->src ? -1 : 1;
+    if (x->dst != y->dst) return x->dst < y->dst ? -1 : 1;
+    if (x->dist < y->dist) return -1;
+    if (x->dist > y->dist) return 1;
+    return 0;
+}
+
+static void orc_rnn_apply(OrcRnn *g, OrcIns *items, size_t count) {
+    if (count > 1) qsort(items, count, sizeof(*items), orc_ins_cmp);
+    for (size_t i = 0; i < count; i++) orc_rnn_insert(g, items[i].src, items[i].dst, items[i].dist);
+}
+
+int orc_rnn_build(const float *base, int n, int dim, int s, int r, int t1, int t2,
+                  uint32_t seed, int top_k, int32_t *out) {
+    OrcRnn g;
+    OrcIns *pend = NULL;
+    int *cand = NULL, rc = -1;
+    size_t np = 0;
+    if (!base || !out || n <= 1 || dim <= 0 || s <= 0 || r <= 0 || t1 <= 0 || t2 <= 0 ||
+        s > r || top_k <= 0)
+        return -1;
+    g.n = n;
+    g.r = r;
+    g.deg = calloc((size_t)n, sizeof(int));
+    g.ids = malloc((size_t)n * r * sizeof(int));
+    g.dist = malloc((size_t)n * r * sizeof(float));
+    g.fresh = calloc((size_t)n * r, 1);
+    pend = malloc((size_t)n * r * sizeof(OrcIns));
+    cand = malloc((size_t)r * sizeof(int));
+    if (!g.deg || !g.ids || !g.dist || !g.fresh || !pend || !cand) goto out;
+    if (s > n - 1) s = n - 1;
+    for (int u = 0; u < n; u++) {
+        uint32_t st = seed ^ (uint32_t)(u + 1) * 0x9e3779b9u;
+        uint64_t attempts = 0, limit = (uint64_t)(uint32_t)n * (uint64_t)(uint32_t)n;
+        int cnt = 0;
+        st = st * 1664525u + 1013904223u;
+        st ^= st >> 16;
+        st = st * 1664525u + 1013904223u;
+        while (cnt < s && attempts < limit) {
+            int v, dup = 0;
+            st = st * 1664525u + 1013904223u;
+            v = (int)(st % (uint32_t)n);
+            if (u != v) {
+                for (int i = 0; i < cnt && !dup; i++) dup = cand[i] == v;
+                if (!dup) cand[cnt++] = v;
+            }
+            attempts++;
+        }
+        for (int v = 0; v < n && cnt < s; v++) {
+            int dup = v == u;
+            for (int i = 0; i < cnt && !dup; i++) dup = cand[i] == v;
+            if (!dup) cand[cnt++] = v;
+        }
+        for (int i = 0; i < cnt; i++) orc_rnn_insert(&g, u, cand[i], orc_l2(base, dim, u, cand[i]));
+    }
+    for (int a = 0; a < t1; a++) {
+        for (int b = 0; b < t2; b++) {
+            np = 0;
+            for (int u = 0; u < n; u++) {
+                size_t row = (size_t)u * r;
+                int i = 0;
+                orc_rnn_sort_row(&g, u);
+                while (i < g.deg[u]) {
+                    const int v = g.ids[row + i];
+                    const float duv = g.dist[row + i];
+                    const uint8_t fv = g.fresh[row + i];
+                    int removed = 0;
+                    for (int w = 0; w < i; w++) {
+                        const int wid = g.ids[row + w];
+                        float dvw;
+                        if (!fv && !g.fresh[row + w]) continue;
+                        dvw = orc_l2(base, dim, v, wid);
+                        if (duv >= dvw) {
+                            pend[np].src = wid;
+                            pend[np].dst = v;
+                            pend[np].dist = dvw;
+                            np++;
+                            removed = 1;
+                            break;
+                        }
+                    }
+                    if (removed) {
+                        for (int k = i + 1; k < g.deg[u]; k++) {
+                            g.ids[row + k - 1] = g.ids[row + k];
+                            g.dist[row + k - 1] = g.dist[row + k];
+                            g.fresh[row + k - 1] = g.fresh[row + k];
+                        }
+                        g.deg[u]--;
+                    } else {
+                        i++;
+                    }
+                }
+                for (int k = 0; k < g.deg[u]; k++) g.fresh[row + k] = 0;
+            }
+            orc_rnn_apply(&g, pend, np);
+        }
+        if (a + 1 < t1) {
+            np = 0;
+            for (int u = 0; u < n; u++)
+                for (int k = 0; k < g.deg[u]; k++) {
+                    pend[np].src = g.ids[(size_t)u * r + k];
+                    pend[np].dst = u;
+                    pend[np].dist = g.dist[(size_t)u * r + k];
+                    np++;
+                }
+            orc_rnn_apply(&g, pend, np);
+        }
+    }
+    for (int u = 0; u < n; u++) {
+        orc_rnn_sort_row(&g, u);
+        for (int j = 0; j < top_k; j++)
+            out[(size_t)u * top_k + j] = j < g.deg[u] ? g.ids[(size_t)u * r + j] : -1;
+    }
+    rc = 0;
+out:
+    free(g.deg);
+    free(g.ids);
+    free(g.dist);
+    free(g.fresh);
+    free(pend);
+    free(cand);
+    return rc;
+}

@@ -141,6 +141,14 @@ float orc_sq_split_distance(const OrcIndex *index, const uint8_t *query_u8,
 int orc_build_exact(const float *base, int n, int dim, int r, int candidates,
                     uint32_t seed, int prune_mode, int32_t *out);
 
+/* RNN-Descent graph construction (retrieval_rnn_build_graph_ex,
+ * retrieval_rnn_descent_core.c:1432-1548, then retrieval_rnn_export_topk):
+ * s random initial neighbors per node, t1 rounds of t2 update passes with
+ * reverse edges between rounds, rows of at most r; out [n][top_k] by
+ * (distance, id), -1 padded.  Returns 0 or -1. */
+int orc_rnn_build(const float *base, int n, int dim, int s, int r, int t1, int t2,
+                  uint32_t seed, int top_k, int32_t *out);
+
 /* retrieval_rnn_descent_core.c:192-212: sequential f32 sum of squared
  * differences. */
 float orc_l2(const float *base, int dim, int a, int b);

@@ -22,10 +22,21 @@
 //                         RNG occlusion rule or the closest r (:904-949),
 //                         fill from the remaining candidates, random fill
 //                         (:781-831), the row in (distance, id) order.
+//   rv_rnn_*_kernel       RNN-Descent (retrieval_rnn_descent_core.c:1432-1548)
+//                         on rows of at most r (id, distance, new-flag)
+//                         entries in HBM, one wavefront per node: random
+//                         initial rows (:571-738), update passes (:901-1014:
+//                         the row sorted, the pair distances it needs
+//                         computed across the lanes into LDS, then the
+//                         sequential occlusion walk), reverse edges
+//                         (:1163-1305), and the insertions each pass
+//                         produces applied per source node in (src, dst)
+//                         order after a device radix sort (:854-899).
 //
 // Bound: VALU (the exact distances are sequential per pair and cannot use
 // the matrix cores without changing their rounding).
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <float.h>
 #include <stdint.h>
@@ -365,5 +376,323 @@ extern "C" int rv_build_exact(const float *x, int n, int dim, int r, int cand, u
     // the scratch is released once the kernels are done with it
     const int sync_rc = rb_check(hipStreamSynchronize(s), "hipStreamSynchronize");
     (void)hipFree(keys);
+    return rc ? rc : sync_rc;
+}
+
+// ------------------------------------------------------- RNN-Descent
+
+__device__ __forceinline__ uint64_t rb_shfl_xor64(uint64_t v, int m) {
+    const int lo = __shfl_xor((int)(uint32_t)v, m);
+    const int hi = __shfl_xor((int)(uint32_t)(v >> 32), m);
+    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
+// :571-738: s distinct random others per node from the node's LCG (then
+// the lowest unused ids), each inserted with its distance and marked new.
+__global__ __launch_bounds__(RB_WAVE) void rv_rnn_init_kernel(const float *x, int n, int dim,
+                                                              int s, int r, uint32_t seed,
+                                                              int32_t *ids, float *dists,
+                                                              uint8_t *fresh, int32_t *deg) {
+    __shared__ int32_t s_ids[RB_WAVE];
+    __shared__ int s_count;
+    const int lane = (int)__lane_id();
+    for (int u = blockIdx.x; u < n; u += gridDim.x) {
+        if (lane == 0) {
+            int cnt = 0;
+            rb_draw(n, s, rb_node_seed(seed, u), u, s_ids, &cnt);
+            s_count = cnt;
+        }
+        __syncthreads();
+        const int cnt = s_count;
+        if (lane < r) {
+            const bool have = lane < cnt;
+            const size_t slot = (size_t)u * r + lane;
+            ids[slot] = have ? s_ids[lane] : -1;
+            dists[slot] = have ? rb_l2(x, dim, u, s_ids[lane]) : FLT_MAX;
+            fresh[slot] = have ? 1 : 0;
+        }
+        if (lane == 0) deg[u] = cnt;
+        __syncthreads();
+    }
+}
+
+// :901-1014, one pass for one node u: the row sorted by (distance, id);
+// entry i (in the current row) is dropped when some earlier entry w, with
+// v or w new, has dist(u, v) >= dist(v, w) -- the first such w in row
+// order -- and the pending insertion (w, v, dist(v, w)) recorded; all flags
+// then cleared.  Pending slots [u][r]: key src << 32 | dst, or all ones.
+__global__ __launch_bounds__(RB_WAVE) void rv_rnn_update_kernel(const float *x, int n, int dim,
+                                                                int r, int32_t *ids,
+                                                                float *dists, uint8_t *fresh,
+                                                                int32_t *deg, uint64_t *pkeys,
+                                                                float *pvals) {
+    __shared__ uint64_t s_k0[RB_WAVE];
+    __shared__ uint8_t s_f0[RB_WAVE];
+    __shared__ int32_t s_id[RB_WAVE];
+    __shared__ float s_d[RB_WAVE];
+    __shared__ uint8_t s_f[RB_WAVE];
+    __shared__ int32_t s_ord[RB_WAVE];
+    __shared__ float s_pair[RB_WAVE * RB_WAVE];  // [i][w], w < i, sorted positions
+    const int lane = (int)__lane_id();
+    for (int u = blockIdx.x; u < n; u += gridDim.x) {
+        const size_t row = (size_t)u * r;
+        const int dg = deg[u];
+        uint64_t key = RB_KEY_MAX;
+        uint8_t fl = 0;
+        if (lane < dg) {
+            key = rb_key(dists[row + lane], ids[row + lane]);
+            fl = fresh[row + lane];
+        }
+        s_k0[lane] = key;
+        s_f0[lane] = fl;
+        key = rv_sort_wave(key);
+        __syncthreads();
+        // ids are distinct, so each sorted key names one original entry
+        uint8_t f = 0;
+        if (lane < dg)
+            for (int t = 0; t < dg; t++)
+                if (s_k0[t] == key) f = s_f0[t];
+        s_id[lane] = (int32_t)(uint32_t)key;
+        s_d[lane] = __uint_as_float((uint32_t)(key >> 32));
+        s_f[lane] = f;
+        s_ord[lane] = lane;
+        __syncthreads();
+        const int pairs = dg * (dg - 1) / 2;
+        for (int p = lane; p < pairs; p += RB_WAVE) {
+            int i = (int)((1.0f + sqrtf(1.0f + 8.0f * (float)p)) * 0.5f);
+            while (i > 1 && i * (i - 1) / 2 > p) i--;
+            while ((i + 1) * i / 2 <= p) i++;
+            const int w = p - i * (i - 1) / 2;
+            if (s_f[i] || s_f[w]) s_pair[i * RB_WAVE + w] = rb_l2(x, dim, s_id[i], s_id[w]);
+        }
+        __syncthreads();
+        int cur = dg, i = 0, np = 0;
+        while (i < cur) {
+            const int oi = s_ord[i];
+            const float duv = s_d[oi];
+            const bool fv = s_f[oi] != 0;
+            bool q = false;
+            if (lane < i) {
+                const int ow = s_ord[lane];
+                q = (fv || s_f[ow]) && duv >= s_pair[oi * RB_WAVE + ow];
+            }
+            const uint64_t m = __ballot(q);
+            if (m) {
+                const int ow = s_ord[__ffsll((unsigned long long)m) - 1];
+                if (lane == 0) {
+                    pkeys[row + np] = ((uint64_t)(uint32_t)s_id[ow] << 32) | (uint32_t)s_id[oi];
+                    pvals[row + np] = s_pair[oi * RB_WAVE + ow];
+                }
+                np++;
+                const int nxt = lane >= i && lane + 1 < cur ? s_ord[lane + 1] : 0;
+                __syncthreads();
+                if (lane >= i && lane + 1 < cur) s_ord[lane] = nxt;
+                __syncthreads();
+                cur--;
+            } else {
+                i++;
+            }
+        }
+        if (lane < r) {
+            if (lane >= np) {
+                pkeys[row + lane] = RB_KEY_MAX;
+                pvals[row + lane] = 0.0f;
+            }
+            const bool have = lane < cur;
+            const int o = have ? s_ord[lane] : 0;
+            ids[row + lane] = have ? s_id[o] : -1;
+            dists[row + lane] = have ? s_d[o] : FLT_MAX;
+            fresh[row + lane] = 0;
+        }
+        if (lane == 0) deg[u] = cur;
+        __syncthreads();
+    }
+}
+
+// :1163-1205: every edge u -> v (distance d) as the pending insertion
+// (v, u, d).
+__global__ void rv_rnn_reverse_kernel(int n, int r, const int32_t *ids, const float *dists,
+                                      const int32_t *deg, uint64_t *pkeys, float *pvals) {
+    const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (t >= (size_t)n * r) return;
+    const int u = (int)(t / r), j = (int)(t % r);
+    if (j < deg[u]) {
+        pkeys[t] = ((uint64_t)(uint32_t)ids[t] << 32) | (uint32_t)u;
+        pvals[t] = dists[t];
+    } else {
+        pkeys[t] = RB_KEY_MAX;
+        pvals[t] = 0.0f;
+    }
+}
+
+__device__ __forceinline__ size_t rb_lower_bound(const uint64_t *a, size_t count, uint64_t k) {
+    size_t lo = 0, hi = count;
+    while (lo < hi) {
+        const size_t mid = (lo + hi) >> 1;
+        if (a[mid] < k) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// :854-899 with :304-375: the sorted insertions of node u applied in
+// order: self edges and present edges skipped, appended while the row has
+// room, else replacing the row's largest (distance, id) when the new edge
+// is smaller; inserted entries are marked new.
+__global__ __launch_bounds__(RB_WAVE) void rv_rnn_apply_kernel(int n, int r, int32_t *ids,
+                                                               float *dists, uint8_t *fresh,
+                                                               int32_t *deg,
+                                                               const uint64_t *skeys,
+                                                               const float *svals,
+                                                               size_t count) {
+    const int lane = (int)__lane_id();
+    for (int u = blockIdx.x; u < n; u += gridDim.x) {
+        const size_t lo = rb_lower_bound(skeys, count, (uint64_t)(uint32_t)u << 32);
+        const size_t hi = rb_lower_bound(skeys, count, (uint64_t)(uint32_t)(u + 1) << 32);
+        if (lo == hi) continue;
+        const size_t row = (size_t)u * r;
+        int dg = deg[u];
+        int id = -1;
+        float d = FLT_MAX;
+        uint8_t f = 0;
+        if (lane < r) {
+            id = ids[row + lane];
+            d = dists[row + lane];
+            f = fresh[row + lane];
+        }
+        for (size_t t = lo; t < hi; t++) {
+            const int dst = (int)(uint32_t)skeys[t];
+            const float nd = svals[t];
+            if (dst == u) continue;
+            if (__ballot(lane < dg && id == dst)) continue;
+            if (dg < r) {
+                if (lane == dg) {
+                    id = dst;
+                    d = nd;
+                    f = 1;
+                }
+                dg++;
+            } else {
+                uint64_t mx = lane < dg ? rb_key(d, id) : 0ull;
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) {
+                    const uint64_t o = rb_shfl_xor64(mx, off);
+                    mx = o > mx ? o : mx;
+                }
+                const float fd = __uint_as_float((uint32_t)(mx >> 32));
+                const int fid = (int)(uint32_t)mx;
+                if ((nd < fd || (nd == fd && dst < fid)) && lane < dg && id == fid) {
+                    id = dst;
+                    d = nd;
+                    f = 1;
+                }
+            }
+        }
+        if (lane < r) {
+            ids[row + lane] = id;
+            dists[row + lane] = d;
+            fresh[row + lane] = f;
+        }
+        if (lane == 0) deg[u] = dg;
+    }
+}
+
+// :438-470 and :475-494: rows by (distance, id), the first top_k ids, -1
+// padded.
+__global__ __launch_bounds__(RB_WAVE) void rv_rnn_export_kernel(int n, int r, int top_k,
+                                                                const int32_t *ids,
+                                                                const float *dists,
+                                                                const int32_t *deg,
+                                                                int32_t *graph) {
+    const int lane = (int)__lane_id();
+    for (int u = blockIdx.x; u < n; u += gridDim.x) {
+        const size_t row = (size_t)u * r;
+        const int dg = deg[u];
+        uint64_t key = lane < dg ? rb_key(dists[row + lane], ids[row + lane]) : RB_KEY_MAX;
+        key = rv_sort_wave(key);
+        if (lane < top_k) graph[(size_t)u * top_k + lane] = lane < dg ? (int32_t)(uint32_t)key : -1;
+    }
+}
+
+extern "C" int rv_build_rnn(const float *x, int n, int dim, int s, int r, int t1, int t2,
+                            uint32_t seed, int top_k, int32_t *graph, void *stream) {
+    if (!x || !graph || n <= 1 || dim <= 0 || s <= 0 || r <= 0 || r > RB_WAVE || s > r ||
+        t1 <= 0 || t2 <= 0 || top_k <= 0 || top_k > r || (size_t)n * r > 0x7fffffffu) {
+        rv_set_error("rv_build_rnn: bad arguments");
+        return -1;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    const size_t slots = (size_t)n * r;
+    int32_t *ids = nullptr, *deg = nullptr;
+    float *dists = nullptr, *pv = nullptr, *pv2 = nullptr;
+    uint8_t *fresh = nullptr;
+    uint64_t *pk = nullptr, *pk2 = nullptr;
+    void *tmp = nullptr;
+    size_t tmp_bytes = 0;
+    int rc = rb_check(hipMalloc((void **)&ids, slots * 4), "hipMalloc rnn ids") ||
+             rb_check(hipMalloc((void **)&dists, slots * 4), "hipMalloc rnn dists") ||
+             rb_check(hipMalloc((void **)&fresh, slots), "hipMalloc rnn flags") ||
+             rb_check(hipMalloc((void **)&deg, (size_t)n * 4), "hipMalloc rnn degrees") ||
+             rb_check(hipMalloc((void **)&pk, slots * 8), "hipMalloc rnn keys") ||
+             rb_check(hipMalloc((void **)&pk2, slots * 8), "hipMalloc rnn keys") ||
+             rb_check(hipMalloc((void **)&pv, slots * 4), "hipMalloc rnn values") ||
+             rb_check(hipMalloc((void **)&pv2, slots * 4), "hipMalloc rnn values");
+    if (!rc)
+        rc = rb_check(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, pk, pk2, pv, pv2,
+                                                         (int)slots, 0, 64, st),
+                      "radix sort size");
+    if (!rc) rc = rb_check(hipMalloc(&tmp, tmp_bytes > 0 ? tmp_bytes : 16), "hipMalloc sort");
+    int grid = 0;
+    if (!rc) {
+        int dev = 0, cus = 0;
+        rc = rb_check(hipGetDevice(&dev), "hipGetDevice") ||
+             rb_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
+                      "hipDeviceGetAttribute");
+        grid = cus * 8 < n ? cus * 8 : n;
+    }
+    // sorts the pending insertions and applies them per source node
+    auto apply = [&]() -> int {
+        size_t b = tmp_bytes;
+        if (rb_check(hipcub::DeviceRadixSort::SortPairs(tmp, b, pk, pk2, pv, pv2, (int)slots, 0,
+                                                        64, st),
+                     "radix sort"))
+            return -1;
+        rv_rnn_apply_kernel<<<grid, RB_WAVE, 0, st>>>(n, r, ids, dists, fresh, deg, pk2, pv2,
+                                                      slots);
+        return rb_check(hipGetLastError(), "rv_rnn_apply_kernel");
+    };
+    if (!rc) {
+        rv_rnn_init_kernel<<<grid, RB_WAVE, 0, st>>>(x, n, dim, s, r, seed, ids, dists, fresh,
+                                                     deg);
+        rc = rb_check(hipGetLastError(), "rv_rnn_init_kernel");
+    }
+    for (int a = 0; a < t1 && !rc; a++) {
+        for (int b = 0; b < t2 && !rc; b++) {
+            rv_rnn_update_kernel<<<grid, RB_WAVE, 0, st>>>(x, n, dim, r, ids, dists, fresh, deg,
+                                                           pk, pv);
+            rc = rb_check(hipGetLastError(), "rv_rnn_update_kernel") || apply();
+        }
+        if (!rc && a + 1 < t1) {
+            const int threads = 256;
+            rv_rnn_reverse_kernel<<<(int)((slots + threads - 1) / threads), threads, 0, st>>>(
+                n, r, ids, dists, deg, pk, pv);
+            rc = rb_check(hipGetLastError(), "rv_rnn_reverse_kernel") || apply();
+        }
+    }
+    if (!rc) {
+        rv_rnn_export_kernel<<<grid, RB_WAVE, 0, st>>>(n, r, top_k, ids, dists, deg, graph);
+        rc = rb_check(hipGetLastError(), "rv_rnn_export_kernel");
+    }
+    // the scratch is released once the kernels are done with it
+    const int sync_rc = rb_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    (void)hipFree(ids);
+    (void)hipFree(dists);
+    (void)hipFree(fresh);
+    (void)hipFree(deg);
+    (void)hipFree(pk);
+    (void)hipFree(pk2);
+    (void)hipFree(pv);
+    (void)hipFree(pv2);
+    (void)hipFree(tmp);
     return rc ? rc : sync_rc;
 }

@@ -179,6 +179,9 @@ int rv_sqmd_launch(const RvIndex *index, const RvSqmdBatch *batch, int grid_wave
  * n * 512 bytes (freed before returning, after a synchronize). */
 int rv_build_exact(const float *d_x, int n, int dim, int r, int candidates,
                    uint32_t seed, int prune_mode, int32_t *d_graph, void *stream);
+/* RNN-Descent on the device (synchronous): out d_graph [n][top_k]. */
+int rv_build_rnn(const float *d_x, int n, int dim, int s, int r, int t1, int t2,
+                 uint32_t seed, int top_k, int32_t *d_graph, void *stream);
 void rv_set_error(const char *msg);
 
 int rv_device_count(void);

@@ -1639,6 +1639,18 @@ AEEResult rover_build_graph_exact(const float *d_x, int n, int dim, int r,
     return ROVER_SUCCESS;
 }
 
+AEEResult rover_build_graph_rnn(const float *d_x, int n, int dim, int s, int r, int t1,
+                                int t2, uint32_t seed, int top_k, int32_t *d_graph) {
+    if (!d_x || !d_graph || n <= 1 || dim <= 0 || s <= 0 || r <= 0 || r > 64 || s > r ||
+        t1 <= 0 || t2 <= 0 || top_k <= 0 || top_k > r || (int64_t)n * r > 0x7fffffff)
+        return ROVER_EBADPARM;
+    if (rv_build_rnn(d_x, n, dim, s, r, t1, t2, seed, top_k, d_graph, NULL)) {
+        set_err("rover_build_graph_rnn");
+        return ROVER_EFAILED;
+    }
+    return ROVER_SUCCESS;
+}
+
 AEEResult rover_build_residual_encode(const float *d_x, const uint8_t *d_codes,
                                       int n, int dim, int m, int ks,
                                       const float *d_codebook, int bits,

@@ -682,6 +682,20 @@ def encode_error4_on_device(x, codes, codebook, res_codes, res_qparams, res_bits
     return e4, e4qp
 
 
+def build_graph_rnn(x, s=16, R=32, t1=4, t2=8, seed=42, top_k=None):
+    """RNN-Descent on the device (rover_build_graph_rnn): int32 [n, top_k]
+    (top_k defaults to R), rows by (distance, id), -1 padded."""
+    from . import native
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    n, dim = x.shape
+    top_k = R if top_k is None else top_k
+    dx = native.DeviceBuffer.from_array(x)
+    dg = native.DeviceBuffer(n * top_k * 4)
+    native.check(native.lib().rover_build_graph_rnn(dx.ptr, n, dim, s, R, t1, t2, seed, top_k,
+                                                    dg.ptr), "rover_build_graph_rnn")
+    return dg.download(np.int32, n * top_k).reshape(n, top_k)
+
+
 def build_graph_exact(x, R, candidates=64, seed=42, prune_mode=1):
     """The reference's exact-candidate graph build on the device
     (rover_build_graph_exact): int32 [n, R], rows by (distance, id)."""

@@ -364,6 +364,29 @@ def graph_case(name):
     print("wrote", path, os.path.getsize(path), "bytes")
 
 
+def rnn_case(name):
+    """The reference's RNN-Descent (retrieval_rnn_build_graph_ex with four
+    worker threads, then retrieval_rnn_export_topk) on small inputs:
+    integer-coordinate data (distance ties) and mixtures, several (s, r, t1,
+    t2).  Writes <name>.npz."""
+    ref = pyoracle.reference()
+    out, cases = {}, []
+    specs = ((600, 16, 8, 16, 2, 3, 4, True), (900, 32, 16, 32, 3, 2, 8, False),
+             (500, 24, 12, 24, 4, 4, 4, True), (300, 8, 16, 16, 2, 2, 2, False))
+    for ci, (n, dim, s_, r, t1, t2, comps, integer) in enumerate(specs):
+        base, _ = data.workload(n, dim, comps, 8.0, 1, base_seed=90 + ci, query_seed=95 + ci)
+        base = np.ascontiguousarray(np.round(base) if integer else base, dtype=np.float32)
+        g = np.zeros((n, r), dtype=np.int32)
+        assert ref.ref_rnn_build(_p(base), n, dim, s_, r, t1, t2, 42, 4, r, _p(g)) == 0
+        out["base%d" % ci] = base
+        out["graph%d" % ci] = g
+        cases.append((n, dim, s_, r, t1, t2, 42))
+    out["cases"] = np.array(cases, dtype=np.int64)
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
 def error4_case(name):
     """Third-level refine for an existing PQ fixture: the reference's
     retrieval_build_error4 writes the PQERR4V1 artifact (its residual decoder
@@ -438,6 +461,9 @@ def main():
     if sys.argv[1:] == ["split"]:
         split_case("sqmd_split")
         return
+    if sys.argv[1:] == ["rnn"]:
+        rnn_case("rnn_build")
+        return
     if sys.argv[1:] == ["error4"]:
         for name in ("pq_d128_m32", "pq_d96_m48", "pq_d64_ks16"):
             error4_case(name)
@@ -456,6 +482,7 @@ def main():
     sqmd_case("sqmd_d128")
     split_case("sqmd_split")
     graph_case("graph_build")
+    rnn_case("rnn_build")
 
 
 if __name__ == "__main__":
