@@ -217,6 +217,35 @@ def sq_leg_run(args, n=100_000, nq=1000, L=64, k=10, ep=32):
                        gpu_ids_identical_frac=same / nq)
         log("sq leg %s: %s" % (name, row))
         out[name] = row
+    # graph construction: the device builders on the whole base, the
+    # compiled reference's builders (16 host threads) on a 20k-point sample
+    builds = {"device_exact_s": round(t_build, 3)}
+    t1 = time.time()
+    rnn = index.build_graph_rnn(base, 16, 32, 4, 4, 42)
+    builds["device_rnn_s"] = round(time.time() - t1, 3)
+    builds["rnn_mean_degree"] = round(float((rnn >= 0).sum(1).mean()), 2)
+    if pyoracle.ref_available():
+        sample = np.ascontiguousarray(base[:20_000])
+        ns = sample.shape[0]
+        ref = pyoracle.reference()
+        g = np.zeros((ns, 32), dtype=np.int32)
+        t1 = time.time()
+        rc = ref.ref_build_exact(sample.ctypes.data, ns, 128, 32, 64, 42, 1, 16, g.ctypes.data)
+        builds["reference_exact_20k_s_16threads"] = round(time.time() - t1, 3) if rc == 0 else None
+        t1 = time.time()
+        rc = ref.ref_rnn_build(sample.ctypes.data, ns, 128, 16, 32, 4, 4, 42, 16, 32,
+                               g.ctypes.data)
+        builds["reference_rnn_20k_s_16threads"] = round(time.time() - t1, 3) if rc == 0 else None
+        t1 = time.time()
+        gd = index.build_graph_exact(sample, 32, 64, 42, 1)
+        builds["device_exact_20k_s"] = round(time.time() - t1, 3)
+        t1 = time.time()
+        gr = index.build_graph_rnn(sample, 16, 32, 4, 4, 42)
+        builds["device_rnn_20k_s"] = round(time.time() - t1, 3)
+        builds["rnn_20k_identical_to_reference"] = bool(np.array_equal(gr, g))
+        del gd
+    log("graph builds: %s" % builds)
+    out["graph_builds"] = builds
     # the split distance needs SQ4 codes: the same base and graph with SQ4
     # codes, 4-bit residuals of the SQ4 reconstruction and their SQMDBIAS
     scale4, zero4 = index.sq_params(base, 4)
