@@ -351,7 +351,7 @@ AEEResult rover_search_graph_launch(void *graph, void *stream);
 AEEResult rover_search_graph_destroy(void *graph);
 
 /* Autotuner (no reference counterpart): times the device batch `b` under
- * launch configurations -- visited-hash size (2^10, 2^12, 2^14 slots and the
+ * launch configurations -- visited-hash size (2^9, 2^10, 2^12, 2^14 slots and the
  * default), probe limit (4, 8, 16) and grid fill (all query slots of every
  * CU, 3/4, 1/2) -- `repeats` launches each after one warm-up, keeps only
  * configurations whose ids, distances and counts equal the default

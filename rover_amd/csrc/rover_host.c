@@ -1145,7 +1145,7 @@ static float time_launches(const RvIndex *ix, const RvBatch *rb, int grid, void 
 
 AEEResult rover_autotune(remote_handle64 h, const RoverSearchBatch *b, void *stream,
                          int repeats, RoverTuning *out) {
-    static const int hashes[4] = {0, 10, 12, 14};
+    static const int hashes[5] = {0, 9, 10, 12, 14};
     static const int probes[3] = {4, 8, 16};
     static const int fill_num[3] = {4, 3, 2}; /* grid fill in quarters */
     RoverCtx *c = ctx_of(h);
@@ -1196,7 +1196,7 @@ AEEResult rover_autotune(remote_handle64 h, const RoverSearchBatch *b, void *str
     }
     out->ms = out->default_ms;
     out->tried = 1;
-    for (int hi = 0; hi < 4; hi++) {
+    for (int hi = 0; hi < 5; hi++) {
         for (int pi = 0; pi < 3; pi++) {
             for (int fi = 0; fi < 3; fi++) {
                 int full, grid;

@@ -600,7 +600,9 @@ __device__ float rv_mlp_logit(const RvMlp &mlp_in, const float *f_raw, float *hb
 // the predictor's code and registers.
 template <int SCORER, int STOP>
 // The plain two-lane online instantiation is held to 128 VGPRs (4 waves per
-// SIMD), which it fits without spilling.
+// SIMD); the few values that spill live outside the scoring loop.  Held to
+// the same budget, the learned-stop instantiation spills into its feature
+// and predictor code and runs slower than at 3 waves per SIMD.
 __global__ __launch_bounds__(RV_WAVE)
 __attribute__((amdgpu_waves_per_eu(SCORER == RV_SCORE_ONLINE_PAIR32 && STOP == 0 ? 4 : 1)))
 void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
