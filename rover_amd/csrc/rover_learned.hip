@@ -10,7 +10,9 @@
 // One workgroup of 256 threads processes tiles of 64 samples: the tile's
 // activations and the weights sit in LDS, every thread owns a fixed slice of
 // the weight gradients and accumulates it over its tiles in registers, then
-// adds it to the global gradient once.
+// writes it to the workgroup's row of a partial-sum slab; a second kernel
+// adds the rows in workgroup order, so a training run is reproducible bit
+// for bit (no floating-point atomics).
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -49,13 +51,13 @@ __device__ static inline float rv_round_bf16(float f) {
 
 // x: [n][16] normalized features, y: [n] labels in {0,1}, wt: [n] sample
 // weights (0 drops a sample).  idx (optional): sample ids to gather.
-// Adds the gradient of sum_i wt_i * BCE_i to grad and sum wt_i*BCE_i to
-// *loss; writes the logits to out when out != NULL.
+// Writes this workgroup's share of the gradient of sum_i wt_i * BCE_i and of
+// sum_i wt_i * BCE_i to part[blockIdx.x][0..MLP_PARAMS] (the loss last);
+// writes the logits to out when out != NULL.
 __global__ __launch_bounds__(THREADS) void rv_mlp_fwd_bwd_kernel(
     const float *__restrict__ params, const float *__restrict__ x,
     const float *__restrict__ y, const float *__restrict__ wt,
-    const int32_t *__restrict__ idx, int n, float *grad, float *loss,
-    float *out, int want_grad) {
+    const int32_t *__restrict__ idx, int n, float *part, float *out, int want_grad) {
     __shared__ float W1[MLP_H * MLP_IN];
     __shared__ float W2[MLP_H * MLP_H];
     __shared__ float W3[MLP_H];
@@ -193,18 +195,32 @@ __global__ __launch_bounds__(THREADS) void rv_mlp_fwd_bwd_kernel(
         }
         __syncthreads();
     }
-    if (t == 0 && loss) atomicAdd(loss, lsum);
+    float *row = part + (size_t)blockIdx.x * (MLP_PARAMS + 1);
+    if (t == 0) row[MLP_PARAMS] = lsum;
     if (!want_grad) return;
 #pragma unroll
-    for (int e = 0; e < 16; e++) atomicAdd(&grad[OFF_W2 + t * 16 + e], gw2[e]);
+    for (int e = 0; e < 16; e++) row[OFF_W2 + t * 16 + e] = gw2[e];
 #pragma unroll
-    for (int e = 0; e < 4; e++) atomicAdd(&grad[OFF_W1 + t * 4 + e], gw1[e]);
+    for (int e = 0; e < 4; e++) row[OFF_W1 + t * 4 + e] = gw1[e];
     if (t < MLP_H) {
-        atomicAdd(&grad[OFF_W3 + t], gw3);
-        atomicAdd(&grad[OFF_B2 + t], gb2);
-        atomicAdd(&grad[OFF_B1 + t], gb1);
+        row[OFF_W3 + t] = gw3;
+        row[OFF_B2 + t] = gb2;
+        row[OFF_B1 + t] = gb1;
     }
-    if (t == 0) atomicAdd(&grad[OFF_B3], gb3);
+    if (t == 0) row[OFF_B3] = gb3;
+}
+
+// grad[i] += sum over the workgroups' rows, in workgroup order; the last
+// column is the loss.
+__global__ void rv_mlp_reduce_kernel(const float *part, int rows, float *grad, float *loss) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > MLP_PARAMS) return;
+    if (i < MLP_PARAMS && !grad) return;
+    if (i == MLP_PARAMS && !loss) return;
+    float s = 0.f;
+    for (int b = 0; b < rows; b++) s += part[(size_t)b * (MLP_PARAMS + 1) + i];
+    if (i < MLP_PARAMS) grad[i] += s;
+    else *loss += s;
 }
 
 __global__ void rv_mlp_adam_kernel(float *p, const float *g, float *m1, float *m2,
@@ -306,12 +322,21 @@ extern "C" int rv_mlp_fwd_bwd_sampled(const float *params, const float *x,
                                       const float *y, const float *wt,
                                       const int32_t *idx, int n, float *grad,
                                       float *loss, float *out, void *stream) {
+    // the partial-sum slab (1024 rows at most), kept for the process
+    static float *slab = nullptr;
     if (n <= 0) return 0;
     int tiles = (n + TILE - 1) / TILE;
     int grid = tiles < 1024 ? tiles : 1024;
+    if (!slab && chk(hipMalloc((void **)&slab, (size_t)1024 * (MLP_PARAMS + 1) * sizeof(float)),
+                     "rv_mlp_fwd_bwd slab"))
+        return -1;
     rv_mlp_fwd_bwd_kernel<<<grid, THREADS, 0, (hipStream_t)stream>>>(
-        params, x, y, wt, idx, n, grad, loss, out, grad != NULL);
-    return chk(hipGetLastError(), "rv_mlp_fwd_bwd");
+        params, x, y, wt, idx, n, slab, out, grad != NULL);
+    if (chk(hipGetLastError(), "rv_mlp_fwd_bwd")) return -1;
+    if (!grad && !loss) return 0;
+    rv_mlp_reduce_kernel<<<(MLP_PARAMS + 1 + 255) / 256, 256, 0, (hipStream_t)stream>>>(
+        slab, grid, grad, loss);
+    return chk(hipGetLastError(), "rv_mlp_reduce");
 }
 
 extern "C" int rv_mlp_adam(float *params, const float *grad, float *m1,

@@ -147,3 +147,26 @@ def test_learned_stop_gain(trained):
         % (rp, ep, rl, el))
     assert rl >= rp - 0.02
     assert el <= 0.85 * ep
+
+
+def test_training_is_reproducible(trained):
+    """The same traces, seed and schedule give the same weights bit for bit
+    (the gradient is summed over workgroups in a fixed order)."""
+    calc, ix, queries, gt, t = trained
+    P = native.lib().rover_mlp_param_count()
+
+    def weights():
+        m = [np.zeros(P, np.float32), np.zeros(16, np.float32), np.zeros(16, np.float32)]
+        native.check(native.lib().rover_learned_stop_get(calc.h, m[0].ctypes.data, P,
+                                                         m[1].ctypes.data, m[2].ctypes.data),
+                     "rover_learned_stop_get")
+        return m
+
+    first = weights()
+    t2 = native.RoverLearnedStopTrain()
+    C.memmove(C.addressof(t2), C.addressof(t), C.sizeof(t))
+    native.check(native.lib().rover_learned_stop_train(calc.h, C.byref(t2)), "train again")
+    second = weights()
+    for a, b in zip(first, second):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert t2.final_loss == t.final_loss
