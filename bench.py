@@ -379,6 +379,7 @@ def main():
     if min(sweep_Ls) < args.k:
         ap.error("every beam width in --Ls must be >= k (%d)" % args.k)
 
+    base = None  # kept for the reference's residual encoder (reference_cpu)
     if args.load_index:
         z = np.load(args.load_index, allow_pickle=False)
         ix = index.PqIndexData(z["graph"], int(z["entry"]), z["codes"], z["codebook"],
@@ -422,7 +423,6 @@ def main():
                      queries=queries, gt=gt,
                      **({} if e4 is None else dict(e4_codes=e4[0], e4_qparams=e4[1])))
             log("index saved to", args.save_index)
-        del base
 
     if args.calibrate_copy_mb:
         nb = args.calibrate_copy_mb << 20
@@ -791,6 +791,7 @@ def main():
             roofline["traffic_source"] = os.path.relpath(rec_path, ROOT)
 
     cpu = None
+    reference_cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle
@@ -828,6 +829,57 @@ def main():
                "gpu_results_identical_frac": same}
         log("cpu baseline: %.0f QPS on %d threads; GPU ids+distances identical on %.4f"
             % (searched / el, threads, same))
+
+        # the reference's own C search (retrieval_pure_pq_search_scalar_with_
+        # policy with its residual evaluator, compiled from its sources) on
+        # one host thread over a bounded sample of the same queries
+        if pyoracle.ref_available() and base is not None and mode == "online":
+            ref = pyoracle.reference()
+            n_base = ix.graph.shape[0]
+            res_r = np.zeros_like(ix.res_codes)
+            qp_r = np.zeros_like(ix.res_qparams)
+            xb = np.ascontiguousarray(base, dtype=np.float32)
+            h = ref.ref_residual_open(xb.ctypes.data, ix.codes.ctypes.data,
+                                      ix.codebook.ctypes.data, n_base, args.dim, args.m, 256,
+                                      ix.res_bits, res_r.ctypes.data, qp_r.ctypes.data)
+            if h:
+                try:
+                    enc_same = bool(np.array_equal(res_r, ix.res_codes) and
+                                    np.array_equal(qp_r.view(np.uint32),
+                                                   ix.res_qparams.view(np.uint32)))
+                    graph = np.ascontiguousarray(ix.graph, dtype=np.int32)
+                    codes = np.ascontiguousarray(ix.codes)
+                    cb = np.ascontiguousarray(ix.codebook, dtype=np.float32)
+                    ids_r = np.zeros(k, dtype=np.int32)
+                    d_r = np.zeros(k, dtype=np.float32)
+                    st_r = np.zeros(4, dtype=np.uint64)
+                    done, same_r = 0, 0
+                    t1 = time.time()
+                    while done < nq and time.time() - t1 < args.cpu_seconds:
+                        q = np.ascontiguousarray(queries[done], dtype=np.float32)
+                        e = np.ascontiguousarray(ents[done], dtype=np.int32)
+                        rc = ref.ref_search(graph.ctypes.data, graph.shape[1], n_base,
+                                            codes.ctypes.data, 16384, cb.ctypes.data,
+                                            args.dim, args.m, 256, q.ctypes.data,
+                                            e.ctypes.data, len(e), args.R, L_plain, k, 1, h,
+                                            None, 8, C.c_float(1.0), C.c_float(0.0),
+                                            ids_r.ctypes.data, d_r.ctypes.data, k,
+                                            st_r.ctypes.data)
+                        same_r += int(rc == k and np.array_equal(ids_r, gids[done]) and
+                                      np.array_equal(d_r.view(np.uint32),
+                                                     gd[done].view(np.uint32)))
+                        done += 1
+                    el_r = time.time() - t1
+                    reference_cpu = {
+                        "value": round(done / el_r, 1), "unit": "queries/s", "cores": 1,
+                        "kind": "reference",
+                        "sample": "the first %d queries, plain %s search at L=%d, one host "
+                                  "thread (%.1f s)" % (done, mode, L_plain, el_r),
+                        "gpu_results_identical_frac": same_r / max(done, 1),
+                        "residual_encoder_identical": enc_same}
+                    log("reference cpu: %s" % reference_cpu)
+                finally:
+                    ref.ref_residual_close(h)
 
     def brief(v):
         if not v:
@@ -873,6 +925,7 @@ def main():
             "host_buffers_qps": round(host_qps, 1),
             "refine": refine_leg,
             "sq": sq_leg,
+            "reference_cpu": reference_cpu,
             "routed": routed,
             "launch_tuning": tuning,
             "uneven_batches": batching,
