@@ -180,6 +180,27 @@ def sq_leg_run(args, n=100_000, nq=1000, L=64, k=10, ep=32):
     oix = pyoracle.Index(graph, res_codes=res, res_qparams=rqp, res_bits=4, sq_codes=codes,
                          sq_bits=8, sq_scale=float(scale), sq_zero=float(zero), dim=128)
     seeds = pyoracle.sqmd_seed_ids(n, ep, entry, 42)
+    graph = np.ascontiguousarray(graph, dtype=np.int32)
+
+    def reference_sq(search, gpu_ids, budget_s=5.0):
+        """The reference's own SQ traversal (its fake-DSP turn loop, compiled)
+        on one host thread over the first queries for about budget_s, and the
+        share whose top-k ids equal the GPU's."""
+        if not pyoracle.ref_available():
+            return {}
+        oid = np.zeros(L, dtype=np.int32)
+        od = np.zeros(L, dtype=np.float32)
+        done = same_r = 0
+        t1 = time.time()
+        while done < nq and time.time() - t1 < budget_s:
+            q = np.ascontiguousarray(queries[done], dtype=np.float32)
+            rc = search(q, oid, od)
+            same_r += int(rc >= k and np.array_equal(oid[:k], gpu_ids[done]))
+            done += 1
+        el_r = time.time() - t1
+        return dict(reference_qps_1thread=round(done / el_r, 1), reference_sample=done,
+                    reference_ids_identical_frac=same_r / max(done, 1))
+
     for name, trav, corr in (("sqmd", native.TRAVERSAL_SQMD, 0),
                              ("sqmd_corrected", native.TRAVERSAL_SQMD, 1),
                              ("best_first", native.TRAVERSAL_BEST_FIRST, 0)):
@@ -215,6 +236,11 @@ def sq_leg_run(args, n=100_000, nq=1000, L=64, k=10, ep=32):
             el = time.time() - t1
             row.update(cpu_oracle_qps_1thread=round(nq / el, 1),
                        gpu_ids_identical_frac=same / nq)
+            row.update(reference_sq(lambda q, oid, od: pyoracle.reference().ref_sqmd_search(
+                graph.ctypes.data, graph.shape[1], n, codes.ctypes.data, 128, 8,
+                C.c_float(scale), C.c_float(zero), res.ctypes.data, rqp.ctypes.data, 4, corr,
+                q.ctypes.data, ep, int(entry), 42, 32, L, 8, oid.ctypes.data, od.ctypes.data, L,
+                None), ids))
         log("sq leg %s: %s" % (name, row))
         out[name] = row
     # graph construction: the device builders on the whole base, the
@@ -287,6 +313,18 @@ def sq_leg_run(args, n=100_000, nq=1000, L=64, k=10, ep=32):
                    qps=round(nq / (ms / 1e3), 1), ms_per_batch=round(ms, 4),
                    mean_expanded=round(float(st[:, 2].mean()), 2),
                    gpu_ids_identical_frac=same / nq)
+        if corr == 2:
+            row.update(reference_sq(lambda q, oid, od: pyoracle.reference().ref_sqmd_split_search(
+                graph.ctypes.data, graph.shape[1], n, codes4.ctypes.data, 128,
+                C.c_float(scale4), C.c_float(zero4), res4.ctypes.data, rqp4.ctypes.data, 4,
+                bias4.ctypes.data, q.ctypes.data, ep, int(entry), 42, 32, L, 8, oid.ctypes.data,
+                od.ctypes.data, L), ids))
+        else:
+            row.update(reference_sq(lambda q, oid, od: pyoracle.reference().ref_sqmd_search(
+                graph.ctypes.data, graph.shape[1], n, codes4.ctypes.data, 128, 4,
+                C.c_float(scale4), C.c_float(zero4), res4.ctypes.data, rqp4.ctypes.data, 4,
+                corr, q.ctypes.data, ep, int(entry), 42, 32, L, 8, oid.ctypes.data,
+                od.ctypes.data, L, None), ids))
         log("sq leg %s: %s" % (name, row))
         out[name] = row
     calc.close()
