@@ -43,6 +43,21 @@ def uniform_stride_sample(n, count):
     return (np.arange(count, dtype=np.int64) * (n - 1)) // max(count - 1, 1)
 
 
+def _segment_means(torch, pts, assign, k, cent):
+    """Lloyd's centroid update on the device without floating-point atomics
+    (so a build is reproducible run to run): points stably sorted by their
+    cluster, float64 prefix sums differenced at the cluster boundaries;
+    empty clusters keep their centroid."""
+    order = torch.argsort(assign, stable=True)
+    cs = torch.cumsum(pts[order].double(), 0)
+    cs = torch.cat([torch.zeros_like(cs[:1]), cs])
+    counts = torch.bincount(assign, minlength=k)
+    ends = torch.cumsum(counts, 0)
+    sums = cs[ends] - cs[ends - counts]
+    nz = counts > 0
+    cent[nz] = (sums[nz] / counts[nz, None].double()).to(cent.dtype)
+
+
 def train_pq(x, m, ks=256, iters=20, train_max=100000, device=None):
     """k-means codebook float32 [m, ks, dim/m]."""
     n, dim = x.shape
@@ -61,10 +76,7 @@ def train_pq(x, m, ks=256, iters=20, train_max=100000, device=None):
                 d = (xs * xs).sum(1, keepdim=True) - 2.0 * xs @ cent.T + \
                     (cent * cent).sum(1)[None, :]
                 a = torch.argmin(d, dim=1)
-                sums = torch.zeros_like(cent).index_add_(0, a, xs)
-                counts = torch.bincount(a, minlength=ks).to(xs.dtype)
-                nz = counts > 0
-                cent[nz] = sums[nz] / counts[nz, None]
+                _segment_means(torch, xs, a, ks, cent)
             out[s] = cent.cpu().numpy()
         return out
     out = np.empty((m, ks, sub), dtype=np.float32)
@@ -507,10 +519,7 @@ def kmeans_representatives(x, count, device=None, iters=10, sample=None):
         for _ in range(iters):
             d = (t * t).sum(1, keepdim=True) - 2.0 * t @ cent.T + (cent * cent).sum(1)[None]
             a = torch.argmin(d, dim=1)
-            sums = torch.zeros_like(cent).index_add_(0, a, t)
-            cnt = torch.bincount(a, minlength=count).to(t.dtype)
-            nz = cnt > 0
-            cent[nz] = sums[nz] / cnt[nz, None]
+            _segment_means(torch, t, a, count, cent)
         xt = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).to(device)
         xn = (xt * xt).sum(1)
         best = torch.full((count,), float("inf"), device=device)
