@@ -510,10 +510,11 @@ struct RvStopState {
     int last_change, last_top1;
 };
 
-__device__ __forceinline__ void rv_features(const RvStopState &st,
-                                            const uint64_t *keys, int n,
-                                            int label_k, int step, int next_pos,
-                                            float *f) {
+// Feature i (0..15) of the search state; lane i computes feature i, so the
+// vector takes one register per lane (lanes 16..63 return 0).
+__device__ __forceinline__ float rv_feature(const RvStopState &st, const uint64_t *keys,
+                                            int n, int label_k, int step, int next_pos,
+                                            int i) {
     const float eps = 1e-30f;
     const int K = n < label_k ? n : label_k;
     const int k2 = n < 2 * label_k ? n : 2 * label_k;
@@ -522,22 +523,27 @@ __device__ __forceinline__ void rv_features(const RvStopState &st,
     const float d2k = rv_key_dist(keys[k2 - 1]);
     const float fmin = next_pos < n ? rv_key_dist(keys[next_pos]) : d2k;
     const int np = next_pos < 4 * label_k ? next_pos : 4 * label_k;
-    f[0] = st.d_entry2 > 0.0f ? st.d_entry / st.d_entry2 : 1.0f;
-    f[1] = __logf(top1 + 1.0f);
-    f[2] = __logf(topk + 1.0f);
-    f[3] = top1 / (st.d_entry + eps);
-    f[4] = topk / (st.d_entry + eps);
-    f[5] = top1 / (st.top1_0 + eps);
-    f[6] = topk / (st.topk_0 + eps);
-    f[7] = __log2f((float)step + 1.0f);
-    f[8] = __logf(fmin + 1.0f);
-    f[9] = fmin / (topk + eps);
-    f[10] = __log2f((float)(step - st.last_change) + 1.0f);
-    f[11] = st.improve_ema;
-    f[12] = __log2f((float)(step - st.last_top1) + 1.0f);
-    f[13] = (float)np / (float)label_k;
-    f[14] = d2k / (topk + eps);
-    f[15] = fmin / (d2k + eps);
+    float v = 0.0f;
+    switch (i) {
+    case 0: v = st.d_entry2 > 0.0f ? st.d_entry / st.d_entry2 : 1.0f; break;
+    case 1: v = __logf(top1 + 1.0f); break;
+    case 2: v = __logf(topk + 1.0f); break;
+    case 3: v = top1 / (st.d_entry + eps); break;
+    case 4: v = topk / (st.d_entry + eps); break;
+    case 5: v = top1 / (st.top1_0 + eps); break;
+    case 6: v = topk / (st.topk_0 + eps); break;
+    case 7: v = __log2f((float)step + 1.0f); break;
+    case 8: v = __logf(fmin + 1.0f); break;
+    case 9: v = fmin / (topk + eps); break;
+    case 10: v = __log2f((float)(step - st.last_change) + 1.0f); break;
+    case 11: v = st.improve_ema; break;
+    case 12: v = __log2f((float)(step - st.last_top1) + 1.0f); break;
+    case 13: v = (float)np / (float)label_k; break;
+    case 14: v = d2k / (topk + eps); break;
+    case 15: v = fmin / (d2k + eps); break;
+    default: break;
+    }
+    return v;
 }
 
 // 16 -> 64 -> 64 -> 1 MLP, lane j owns hidden unit j.  FP32 accumulation
@@ -548,7 +554,11 @@ __device__ __forceinline__ void rv_features(const RvStopState &st,
 __device__ __forceinline__ float rv_bf16_lo(uint32_t v) { return __uint_as_float(v << 16); }
 __device__ __forceinline__ float rv_bf16_hi(uint32_t v) { return __uint_as_float(v & 0xffff0000u); }
 
-__device__ float rv_mlp_logit(const RvMlp &mlp_in, const float *f_raw, float *hbuf) {
+__device__ __forceinline__ float rv_lane_value(float v, int src) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+
+__device__ float rv_mlp_logit(const RvMlp &mlp_in, float f_lane, float *hbuf) {
     const int j = rv_lane();
     // The weights are re-read at every evaluation: the empty asm hides the
     // pointers' loop invariance, so the loads are not hoisted out of the
@@ -556,19 +566,19 @@ __device__ float rv_mlp_logit(const RvMlp &mlp_in, const float *f_raw, float *hb
     RvMlp mlp = mlp_in;
     asm volatile("" : "+s"(mlp.w1), "+s"(mlp.w2), "+s"(mlp.w3), "+s"(mlp.b1));
     asm volatile("" : "+s"(mlp.b2), "+s"(mlp.b3), "+s"(mlp.feat_mean), "+s"(mlp.feat_inv_std));
-    float x[RV_MLP_IN];
-#pragma unroll
-    for (int i = 0; i < RV_MLP_IN; i++)
-        x[i] = (f_raw[i] - mlp.feat_mean[i]) * mlp.feat_inv_std[i];
+    // lane i < 16 holds input i, normalized; each is read as a wave-uniform
+    // scalar when the layer needs it
+    const int fi = j < RV_MLP_IN ? j : 0;
+    const float x = (f_lane - mlp.feat_mean[fi]) * mlp.feat_inv_std[fi];
     const uint2 *w1 = reinterpret_cast<const uint2 *>(mlp.w1);
     float h1 = mlp.b1[j];
 #pragma unroll
     for (int g = 0; g < RV_MLP_IN / 4; g++) {
         const uint2 v = w1[g * RV_MLP_HIDDEN + j];
-        h1 += rv_bf16_lo(v.x) * x[4 * g + 0];
-        h1 += rv_bf16_hi(v.x) * x[4 * g + 1];
-        h1 += rv_bf16_lo(v.y) * x[4 * g + 2];
-        h1 += rv_bf16_hi(v.y) * x[4 * g + 3];
+        h1 += rv_bf16_lo(v.x) * rv_lane_value(x, 4 * g + 0);
+        h1 += rv_bf16_hi(v.x) * rv_lane_value(x, 4 * g + 1);
+        h1 += rv_bf16_lo(v.y) * rv_lane_value(x, 4 * g + 2);
+        h1 += rv_bf16_hi(v.y) * rv_lane_value(x, 4 * g + 3);
     }
     h1 = h1 > 0.0f ? h1 : 0.0f;
     hbuf[j] = h1;
@@ -873,6 +883,10 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
             __syncthreads();
         }
         if (STOP != 0 && status == 0) {
+            // the first two entries' distances were taken in the lanes that
+            // held their keys; every lane computes features from them
+            ss.d_entry = rv_lane_value(ss.d_entry, key_lane0);
+            ss.d_entry2 = rv_lane_value(ss.d_entry2, key_lane0 + 1);
             const uint64_t *k0 = pool_keys;
             ss.top1_0 = rv_key_dist(k0[0]);
             ss.topk_0 = rv_key_dist(k0[(n < b.stop_label_k ? n : b.stop_label_k) - 1]);
@@ -901,16 +915,11 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
 
             // learned stop: features of the current state
             if (STOP != 0 && step > 0 && step % b.stop_every == 0) {
-                float f[RV_MLP_IN];
-                rv_features(ss, keys, n, b.stop_label_k, step, next, f);
+                const float f = rv_feature(ss, keys, n, b.stop_label_k, step, next, lane);
                 if (STOP == 2) {
                     if (trace_n < b.trace_cap) {
-                        if (lane < RV_MLP_IN) {
-                            float v = f[0];
-#pragma unroll
-                            for (int i = 1; i < RV_MLP_IN; i++) v = lane == i ? f[i] : v;
-                            b.trace_feats[((size_t)q * b.trace_cap + trace_n) * RV_MLP_IN + lane] = v;
-                        }
+                        if (lane < RV_MLP_IN)
+                            b.trace_feats[((size_t)q * b.trace_cap + trace_n) * RV_MLP_IN + lane] = f;
                         if (lane == 0) b.trace_steps[(size_t)q * b.trace_cap + trace_n] = step;
                         trace_n++;
                     }
