@@ -3,6 +3,7 @@
 librover.so = HIP kernels for gfx950 (hipcc) + the C host library (gcc),
 linked in-tree under rover_amd/lib/ so it travels with the source tree.
 """
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -46,22 +47,22 @@ def _run(cmd, verbose):
 
 def build_native(verbose=False):
     os.makedirs(LIBDIR, exist_ok=True)
-    objs = []
+    objs, cmds = [], []
     inc = ["-I", os.path.join(ROOT, "include"), "-I", CSRC]
-    for src in HIP_SOURCES:
+    for src in HIP_SOURCES + C_SOURCES:
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue
         obj = os.path.join(LIBDIR, src + ".o")
-        _run([HIPCC] + HIP_FLAGS + inc + ["-c", path, "-o", obj], verbose)
+        if src.endswith(".hip"):
+            cmds.append([HIPCC] + HIP_FLAGS + inc + ["-c", path, "-o", obj])
+        else:
+            cmds.append(["gcc"] + C_FLAGS + inc + ["-c", path, "-o", obj])
         objs.append(obj)
-    for src in C_SOURCES:
-        path = os.path.join(CSRC, src)
-        if not os.path.exists(path):
-            continue
-        obj = os.path.join(LIBDIR, src + ".o")
-        _run(["gcc"] + C_FLAGS + inc + ["-c", path, "-o", obj], verbose)
-        objs.append(obj)
+    # translation units are independent: compile them side by side
+    with concurrent.futures.ThreadPoolExecutor(max_workers=8) as ex:
+        for f in [ex.submit(_run, c, verbose) for c in cmds]:
+            f.result()
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp]
          + objs + ["-lpthread", "-lm"], verbose)

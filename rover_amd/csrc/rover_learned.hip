@@ -173,6 +173,7 @@ __global__ __launch_bounds__(THREADS) void rv_mlp_fwd_bwd_kernel(
             for (int e = 0; e < 16; e++) {
                 const int p = t * 16 + e, j = p / MLP_H, i = p % MLP_H;
                 float a = 0.f;
+#pragma unroll 4
                 for (int r = 0; r < TILE; r++) a += H2[r * MLP_H + j] * H1[r * MLP_H + i];
                 gw2[e] += a;
             }
@@ -180,6 +181,7 @@ __global__ __launch_bounds__(THREADS) void rv_mlp_fwd_bwd_kernel(
             for (int e = 0; e < 4; e++) {
                 const int p = t * 4 + e, j = p / MLP_IN, f = p % MLP_IN;
                 float a = 0.f;
+#pragma unroll 4
                 for (int r = 0; r < TILE; r++) a += D1[r * MLP_H + j] * X[r * MLP_IN + f];
                 gw1[e] += a;
             }
