@@ -133,7 +133,11 @@ AEEResult rover_sq_split_bias_load(remote_handle64 h, const uint8_t *image,
  * [count][dim] and one representative node id per centroid.  A batch with
  * route_entries = S seeds each query with the representatives of its S
  * nearest centroids (exact squared L2, ties by centroid index), passed to
- * the search as that query's entry ids. */
+ * the search as that query's entry ids.  Routing refuses (ROVER_EBADPARM)
+ * a router of more than ROVER_ROUTER_MAX_CENTROIDS centroids: larger ones
+ * are not tested on the device; route larger sets on the host and pass
+ * entry_ids rows. */
+#define ROVER_ROUTER_MAX_CENTROIDS 256
 AEEResult rover_pure_pq_router_load(remote_handle64 h, const float *centroids,
                                     int centroidsLen, const int32_t *rep_ids,
                                     int count);

@@ -65,6 +65,11 @@ typedef struct RoverCtx {
     uint32_t sqmd_seeds_seed;
     /* launch workspace */
     int32_t *d_queue;
+    void *last_stream;        /* stream of the last eager launch: a launch on
+                                 another stream waits for it first, since the
+                                 work counter and scratch are shared        */
+    int launched;
+    void *ev_last;            /* completion event of that launch           */
     int32_t *d_entries;
     int entries_n, entries_count, entries_graph_entry;
     uint32_t entries_seed;
@@ -93,6 +98,12 @@ void rover_dfree(void *p);
 AEEResult rover_prepare_launch(RoverCtx *c, const RoverSearchBatch *b,
                                int max_pool, int explicit_entries, RvIndex *ix,
                                RvBatch *rb);
+/* Eager launches of a session share its work counter and scratch.
+ * rover_stream_turn, before enqueueing on `stream`, makes `stream` wait (on
+ * the device) for the session's previous launch when that ran on another
+ * stream; rover_stream_done, after, records the new launch's completion. */
+int rover_stream_turn(RoverCtx *c, void *stream);
+int rover_stream_done(RoverCtx *c, void *stream);
 /* Routes device queries to S entries each (nearest router centroids) into
  * d_out[nq][S]; chunks the scratch.  Asynchronous on `stream`. */
 AEEResult rover_route_into(RoverCtx *c, const float *d_queries, int nq, int S,

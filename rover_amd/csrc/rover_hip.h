@@ -200,6 +200,9 @@ int rv_event_destroy(void *ev);
 int rv_stream_create(void **stream);
 int rv_stream_destroy(void *stream);
 int rv_stream_synchronize(void *stream);
+/* An event without timing data, and a device-side wait of `stream` for it. */
+int rv_event_create_untimed(void **ev);
+int rv_stream_wait_event(void *stream, void *ev);
 
 /* Bytes of LDS one query slot needs, and how many slots fit per CU. */
 size_t rv_search_lds_bytes(const RvIndex *index, const RvBatch *batch);
@@ -229,14 +232,15 @@ int rv_error4_encode(const float *d_x, const uint8_t *d_codes,
  * Flat parameter layout: w1[64][16] | b1[64] | w2[64][64] | b2[64] | w3[64] |
  * b3[1].  Gradients and losses are accumulated (callers zero them). */
 int rv_mlp_param_count(void);
-int rv_mlp_forward_backward(const float *d_params, const float *d_x,
-                            const float *d_y, const float *d_weight, int n,
-                            float *d_grad, float *d_loss, float *d_out,
-                            void *stream);
+/* d_slab: the caller's partial-sum workspace of rv_mlp_slab_floats() floats;
+ * calls that may overlap need separate slabs.  d_idx (optional) gathers the
+ * samples. */
+#define RV_MLP_SLAB_ROWS 1024
+size_t rv_mlp_slab_floats(void);
 int rv_mlp_fwd_bwd_sampled(const float *d_params, const float *d_x,
                            const float *d_y, const float *d_weight,
                            const int32_t *d_idx, int n, float *d_grad,
-                           float *d_loss, float *d_out, void *stream);
+                           float *d_loss, float *d_out, float *d_slab, void *stream);
 int rv_mlp_adam(float *d_params, const float *d_grad, float *d_m1,
                 float *d_m2, int step, float lr, float beta1, float beta2,
                 float eps, float grad_scale, void *stream);

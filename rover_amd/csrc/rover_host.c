@@ -41,6 +41,25 @@ void rover_dfree(void *p) {
     if (p) rv_free(p);
 }
 
+int rover_stream_turn(RoverCtx *c, void *stream) {
+    if (c->launched && c->last_stream != stream &&
+        rv_stream_wait_event(stream, c->ev_last)) {
+        set_err("stream wait");
+        return -1;
+    }
+    return 0;
+}
+
+int rover_stream_done(RoverCtx *c, void *stream) {
+    if (rv_event_record(c->ev_last, stream)) {
+        set_err("stream event");
+        return -1;
+    }
+    c->last_stream = stream;
+    c->launched = 1;
+    return 0;
+}
+
 static int dalloc_copy(void **d, const void *src, size_t bytes) {
     *d = NULL;
     if (rv_malloc(d, bytes)) return -1;
@@ -70,8 +89,11 @@ int calculator_open(const char *uri, remote_handle64 *handle) {
     c->l1_bits = 8;
     rover_sha256(NULL, 0, c->orig_maps_sha);
     pthread_mutex_init(&c->lock, NULL);
-    if (rv_set_device(0) || rv_malloc((void **)&c->d_queue, 64)) {
+    if (rv_set_device(0) || rv_malloc((void **)&c->d_queue, 64) ||
+        rv_event_create_untimed(&c->ev_last)) {
         set_err("calculator_open");
+        dfree(c->d_queue);
+        if (c->ev_last) rv_event_destroy(c->ev_last);
         free(c);
         return ROVER_EFAILED;
     }
@@ -148,6 +170,7 @@ int calculator_close(remote_handle64 h) {
     dfree(c->d_route_ent);
     dfree(c->d_sqmd_vis);
     dfree(c->d_sqmd_front);
+    if (c->ev_last) rv_event_destroy(c->ev_last);
     c->magic = 0;
     pthread_mutex_destroy(&c->lock);
     free(c);
@@ -746,14 +769,26 @@ static int grow(void **d, size_t *cap, size_t bytes) {
     return 0;
 }
 
+/* The device router runs at the sizes its GPU tests cover: at most
+ * ROVER_ROUTER_MAX_CENTROIDS centroids, queries in launches of at most
+ * ROUTE_CHUNK.  (One bench run that routed 10,000 queries over 4,096
+ * centroids in one launch went silent; a CPU run of both kernels and this
+ * chunking at that size stayed in bounds and selected what brute force
+ * selects, so no cause is known, and larger sizes are refused.) */
+#define ROUTE_CHUNK 2048
 AEEResult rover_route_into(RoverCtx *c, const float *d_queries, int nq, int S,
                            int32_t *d_out, void *stream) {
-    const int chunk = 16384;
+    const int chunk = ROUTE_CHUNK;
     if (!c->d_router_cent || c->router_n <= 0) {
         snprintf(g_err, sizeof(g_err), "route: no entry router loaded");
         return ROVER_EFAILED;
     }
     if (S <= 0 || S > 16 || S > c->router_n || nq < 0) return ROVER_EBADPARM;
+    if (c->router_n > ROVER_ROUTER_MAX_CENTROIDS) {
+        snprintf(g_err, sizeof(g_err), "route: %d centroids exceed the router's limit %d",
+                 c->router_n, ROVER_ROUTER_MAX_CENTROIDS);
+        return ROVER_EBADPARM;
+    }
     if (grow((void **)&c->d_route_dist, &c->route_dist_cap,
              (size_t)(nq < chunk ? nq : chunk) * c->router_n * sizeof(float))) {
         set_err("route scratch");
@@ -975,9 +1010,25 @@ static AEEResult launch_sqmd(RoverCtx *c, const RoverSearchBatch *b, int max_poo
     return ROVER_SUCCESS;
 }
 
+static AEEResult launch_batch_on(RoverCtx *c, const RoverSearchBatch *b_in,
+                                 int max_pool, int explicit_entries, void *stream);
+
+/* One eager batch on `stream`, ordered after the session's previous launch
+ * (the work counter and scratch are the session's). */
 static AEEResult launch_batch(RoverCtx *c, const RoverSearchBatch *b_in,
                               int max_pool, int explicit_entries,
                               void *stream) {
+    RoverSearchBatch routed;
+    const RoverSearchBatch *b = b_in;
+    AEEResult rc;
+    if (rover_stream_turn(c, stream)) return ROVER_EFAILED;
+    rc = launch_batch_on(c, b_in, max_pool, explicit_entries, stream);
+    if (rover_stream_done(c, stream) && rc == ROVER_SUCCESS) rc = ROVER_EFAILED;
+    return rc;
+}
+
+static AEEResult launch_batch_on(RoverCtx *c, const RoverSearchBatch *b_in,
+                                 int max_pool, int explicit_entries, void *stream) {
     RvIndex ix;
     RvBatch rb;
     RoverSearchBatch routed;
@@ -1165,6 +1216,10 @@ AEEResult rover_autotune(remote_handle64 h, const RoverSearchBatch *b, void *str
     pthread_mutex_lock(&c->lock);
     rc = rover_prepare_launch(c, b, b->pool_size, 0, &ix, &rb);
     if (rc != ROVER_SUCCESS) goto out;
+    if (rover_stream_turn(c, stream)) {
+        rc = ROVER_EFAILED;
+        goto out;
+    }
     rc = ROVER_ENOMEMORY;
     ref_ids = malloc(bytes_out);
     got_ids = malloc(bytes_out);
@@ -1245,6 +1300,7 @@ AEEResult rover_autotune(remote_handle64 h, const RoverSearchBatch *b, void *str
     }
     rc = ROVER_SUCCESS;
 out:
+    if (c->launched || rc == ROVER_SUCCESS) rover_stream_done(c, stream);
     pthread_mutex_unlock(&c->lock);
     if (e0) rv_event_destroy(e0);
     if (e1) rv_event_destroy(e1);

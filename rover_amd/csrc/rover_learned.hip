@@ -308,30 +308,24 @@ static int chk(hipError_t e, const char *what) {
 }
 extern "C" const char *rv_learned_error(void) { return g_err; }
 
-extern "C" int rv_mlp_fwd_bwd_sampled(const float *params, const float *x,
-                                      const float *y, const float *wt,
-                                      const int32_t *idx, int n, float *grad,
-                                      float *loss, float *out, void *stream);
-
-extern "C" int rv_mlp_forward_backward(const float *params, const float *x,
-                                       const float *y, const float *wt, int n,
-                                       float *grad, float *loss, float *out,
-                                       void *stream) {
-    return rv_mlp_fwd_bwd_sampled(params, x, y, wt, NULL, n, grad, loss, out, stream);
+extern "C" size_t rv_mlp_slab_floats(void) {
+    return (size_t)RV_MLP_SLAB_ROWS * (MLP_PARAMS + 1);
 }
 
+// slab: the caller's partial-sum workspace of rv_mlp_slab_floats() floats
+// (one row per workgroup); concurrent calls need their own slabs.
 extern "C" int rv_mlp_fwd_bwd_sampled(const float *params, const float *x,
                                       const float *y, const float *wt,
                                       const int32_t *idx, int n, float *grad,
-                                      float *loss, float *out, void *stream) {
-    // the partial-sum slab (1024 rows at most), kept for the process
-    static float *slab = nullptr;
+                                      float *loss, float *out, float *slab,
+                                      void *stream) {
     if (n <= 0) return 0;
-    int tiles = (n + TILE - 1) / TILE;
-    int grid = tiles < 1024 ? tiles : 1024;
-    if (!slab && chk(hipMalloc((void **)&slab, (size_t)1024 * (MLP_PARAMS + 1) * sizeof(float)),
-                     "rv_mlp_fwd_bwd slab"))
+    if (!slab) {
+        snprintf(g_err, sizeof(g_err), "rv_mlp_fwd_bwd: no partial-sum slab");
         return -1;
+    }
+    int tiles = (n + TILE - 1) / TILE;
+    int grid = tiles < RV_MLP_SLAB_ROWS ? tiles : RV_MLP_SLAB_ROWS;
     rv_mlp_fwd_bwd_kernel<<<grid, THREADS, 0, (hipStream_t)stream>>>(
         params, x, y, wt, idx, n, slab, out, grad != NULL);
     if (chk(hipGetLastError(), "rv_mlp_fwd_bwd")) return -1;

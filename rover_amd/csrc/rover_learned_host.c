@@ -32,13 +32,25 @@ AEEResult rover_mlp_forward_backward(const float *d_params, const float *d_x,
                                      const float *d_y, const float *d_weight,
                                      int n, float *d_grad, float *d_loss,
                                      float *d_out) {
+    float *d_slab = NULL;
+    AEEResult rc = ROVER_SUCCESS;
     if (!d_params || !d_x || !d_y || !d_weight || n < 0) return ROVER_EBADPARM;
-    if (rv_mlp_forward_backward(d_params, d_x, d_y, d_weight, n, d_grad, d_loss,
-                                d_out, NULL)) {
-        rover_set_errf(rv_learned_error());
-        return ROVER_EFAILED;
+    /* a slab of its own per call, so concurrent callers never share one */
+    if (rv_malloc((void **)&d_slab, rv_mlp_slab_floats() * sizeof(float))) {
+        rover_set_err("rover_mlp_forward_backward: slab");
+        return ROVER_ENOMEMORY;
     }
-    return ROVER_SUCCESS;
+    if (rv_mlp_fwd_bwd_sampled(d_params, d_x, d_y, d_weight, NULL, n, d_grad, d_loss,
+                               d_out, d_slab, NULL)) {
+        rover_set_errf(rv_learned_error());
+        rc = ROVER_EFAILED;
+    }
+    if (rv_synchronize() && rc == ROVER_SUCCESS) {
+        rover_set_err("rover_mlp_forward_backward");
+        rc = ROVER_EFAILED;
+    }
+    rover_dfree(d_slab);
+    return rc;
 }
 
 AEEResult rover_mlp_adam_step(float *d_params, const float *d_grad, float *d_m1,
@@ -158,6 +170,7 @@ AEEResult rover_learned_stop_train(remote_handle64 h, RoverLearnedStopTrain *t) 
     float *d_mean = NULL, *d_istd = NULL, *d_od = NULL;
     int32_t *d_steps = NULL, *d_meta = NULL, *d_oid = NULL, *d_cnt = NULL;
     int32_t *d_idx = NULL, *d_ent = NULL;
+    float *d_slab = NULL;
     float *h_feat = NULL, *h_y = NULL, *h_w = NULL, *params = NULL;
     int32_t *h_meta = NULL;
     double sum[NF], sq[NF], cnt = 0.0, pos = 0.0;
@@ -228,7 +241,8 @@ AEEResult rover_learned_stop_train(remote_handle64 h, RoverLearnedStopTrain *t) 
     rb.trace_steps = d_steps;
     rb.trace_meta = d_meta;
     rb.trace_cap = t->trace_cap;
-    if (rv_search_launch(&ix, &rb, 0, NULL) || rv_synchronize()) {
+    if (rover_stream_turn(c, NULL) || rv_search_launch(&ix, &rb, 0, NULL) ||
+        rover_stream_done(c, NULL) || rv_synchronize()) {
         rover_set_err("learned stop: trace search");
         goto out;
     }
@@ -309,6 +323,7 @@ AEEResult rover_learned_stop_train(remote_handle64 h, RoverLearnedStopTrain *t) 
         rv_malloc((void **)&d_m2, (size_t)P * sizeof(float)) ||
         rv_malloc((void **)&d_loss, 2 * sizeof(float)) ||
         rv_malloc((void **)&d_idx, (size_t)t->batch * sizeof(int32_t)) ||
+        rv_malloc((void **)&d_slab, rv_mlp_slab_floats() * sizeof(float)) ||
         rv_fill_zero(d_m1, (size_t)P * sizeof(float), NULL) ||
         rv_fill_zero(d_m2, (size_t)P * sizeof(float), NULL) ||
         rv_fill_zero(d_loss, 2 * sizeof(float), NULL)) {
@@ -323,7 +338,7 @@ AEEResult rover_learned_stop_train(remote_handle64 h, RoverLearnedStopTrain *t) 
             if (rv_fill_zero(d_grad, (size_t)P * sizeof(float), NULL) ||
                 rv_sample_indices(d_idx, t->batch, nrec, t->seed, (uint32_t)step, NULL) ||
                 rv_mlp_fwd_bwd_sampled(d_params, d_feat, d_y, d_w, d_idx, t->batch,
-                                       d_grad, d_loss, NULL, NULL) ||
+                                       d_grad, d_loss, NULL, d_slab, NULL) ||
                 rv_mlp_adam(d_params, d_grad, d_m1, d_m2, step, t->lr, 0.9f, 0.999f,
                             1e-8f, 1.0f / (float)t->batch, NULL)) {
                 rover_set_errf(rv_learned_error());
@@ -359,6 +374,7 @@ out:
     rover_dfree(d_m2);
     rover_dfree(d_loss);
     rover_dfree(d_idx);
+    rover_dfree(d_slab);
     rover_dfree(d_ent);
     free(h_feat);
     free(h_y);

@@ -1784,6 +1784,17 @@ extern "C" int rv_event_destroy(void *ev) {
     return rv_check(hipEventDestroy((hipEvent_t)ev), "hipEventDestroy");
 }
 // Streams are created at the default priority.
+extern "C" int rv_event_create_untimed(void **ev) {
+    hipEvent_t e;
+    if (rv_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate"))
+        return -1;
+    *ev = (void *)e;
+    return 0;
+}
+extern "C" int rv_stream_wait_event(void *stream, void *ev) {
+    return rv_check(hipStreamWaitEvent((hipStream_t)stream, (hipEvent_t)ev, 0),
+                    "hipStreamWaitEvent");
+}
 extern "C" int rv_stream_create(void **stream) {
     hipStream_t s;
     if (rv_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate"))

@@ -68,3 +68,21 @@ def test_routed_search_matches_oracle(routed, mode):
     # the same rows passed explicitly give the same results
     ids2, d2, cnt2 = calc.search_batch(queries, 48, 10, correction=mode, entry_ids=ents)
     assert np.array_equal(ids2, ids) and np.array_equal(d2.view(np.uint32), d.view(np.uint32))
+
+
+def test_router_refuses_more_centroids_than_tested(routed):
+    """Routing with more centroids than the router is tested at is refused
+    before anything is launched; the host-routed rows remain the way in."""
+    from rover_amd import native
+    calc, ix, base, queries = routed
+    say("case router limit")
+    big = np.ascontiguousarray(np.repeat(ix.router_centroids, 2, axis=0)[:257])
+    reps = np.ascontiguousarray(np.resize(ix.router_ids, 257).astype(np.int32))
+    calc.pure_pq_router_load(big, reps)
+    try:
+        with pytest.raises(native.RoverError) as e:
+            calc.route_entries(queries[:8], 4)
+        assert e.value.code == native.ROVER_EBADPARM
+        assert "centroids" in str(e.value)
+    finally:
+        calc.pure_pq_router_load(ix.router_centroids, ix.router_ids)
