@@ -338,6 +338,24 @@ AEEResult rover_search_batch(remote_handle64 h, const RoverSearchBatch *b);
 AEEResult rover_search_batch_device(remote_handle64 h,
                                     const RoverSearchBatch *b, void *stream);
 
+/* rover_search_batch_device that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Every pointer of `b` is a device address, as above; the library's own
+ * workspace (the work counter and the graph entry + LCG seed list, at most
+ * ROVER_NOALLOC_MAX_SEEDS entry points) is a fixed array on each device,
+ * ROVER_NOALLOC_SCRATCH_BYTES large, held until the process ends.  Calls on
+ * one device share that array, and it keeps the last call's contents:
+ * calls that overlap on one GPU (two streams, two threads, two handles)
+ * race and must be ordered by the caller.  Best-first traversal only: routed
+ * entries (route_entries > 0) and the SQ heap traversals return
+ * ROVER_EBADPARM, as do more entry points than the array holds. */
+#define ROVER_NOALLOC_MAX_SEEDS 16384
+#define ROVER_NOALLOC_SCRATCH_BYTES (256 + 4 * ROVER_NOALLOC_MAX_SEEDS)
+AEEResult rover_search_batch_device_noalloc(remote_handle64 h,
+                                            const RoverSearchBatch *b,
+                                            void *stream);
+
 /* Batch search captured in a HIP graph (no reference counterpart): the
  * work-queue reset and the search kernel of one device batch, captured on
  * `stream` (a stream from rover_stream_create, not the default stream) and

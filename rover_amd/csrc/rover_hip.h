@@ -204,6 +204,20 @@ int rv_stream_synchronize(void *stream);
 int rv_event_create_untimed(void **ev);
 int rv_stream_wait_event(void *stream, void *ev);
 
+/* Fixed per-device workspace for the no-allocation batch entry point: the
+ * work counter at offset 0, then at most RV_NOALLOC_MAX_SEEDS seed ids at
+ * RV_NOALLOC_SEED_OFFSET.  rv_noalloc_scratch makes `device` current once,
+ * resolves the workspace's device address with hipGetSymbolAddress, keeps
+ * it, and returns it (0) or -1. */
+#define RV_NOALLOC_SEED_OFFSET 256
+#define RV_NOALLOC_MAX_SEEDS 16384
+#define RV_NOALLOC_BYTES (RV_NOALLOC_SEED_OFFSET + 4 * RV_NOALLOC_MAX_SEEDS)
+int rv_noalloc_scratch(int device, void **d_scratch);
+/* The graph entry + LCG seed list (rpc_demo_pure_pq.c:192-208) of `count`
+ * (<= n) ids, written to d_out by a kernel on `stream`. */
+int rv_seed_list(int32_t *d_out, int n, int count, int graph_entry, uint32_t seed,
+                 void *stream);
+
 /* Bytes of LDS one query slot needs, and how many slots fit per CU. */
 size_t rv_search_lds_bytes(const RvIndex *index, const RvBatch *batch);
 int rv_search_slots_per_cu(const RvIndex *index, const RvBatch *batch);

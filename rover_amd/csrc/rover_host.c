@@ -15,6 +15,10 @@
 #include <stdlib.h>
 #include <string.h>
 
+_Static_assert(ROVER_NOALLOC_MAX_SEEDS == RV_NOALLOC_MAX_SEEDS &&
+                   ROVER_NOALLOC_SCRATCH_BYTES == RV_NOALLOC_BYTES,
+               "no-allocation workspace limits");
+
 static __thread char g_err[512];
 
 void rover_set_err(const char *what) {
@@ -90,7 +94,7 @@ int calculator_open(const char *uri, remote_handle64 *handle) {
     rover_sha256(NULL, 0, c->orig_maps_sha);
     pthread_mutex_init(&c->lock, NULL);
     if (rv_set_device(0) || rv_malloc((void **)&c->d_queue, 64) ||
-        rv_event_create_untimed(&c->ev_last)) {
+        rv_event_create_untimed(&c->ev_last) || rv_noalloc_scratch(0, &c->d_noalloc)) {
         set_err("calculator_open");
         dfree(c->d_queue);
         if (c->ev_last) rv_event_destroy(c->ev_last);
@@ -1077,6 +1081,70 @@ AEEResult rover_search_batch_device(remote_handle64 h, const RoverSearchBatch *b
     if (!c || !b) return ROVER_EBADPARM;
     pthread_mutex_lock(&c->lock);
     rc = launch_batch(c, b, b->pool_size, 0, stream);
+    pthread_mutex_unlock(&c->lock);
+    return rc;
+}
+
+/* rover_search_batch_device with every workspace in the device's fixed
+ * no-allocation array (rv_noalloc_scratch, resolved at calculator_open): the
+ * work counter and the graph entry + LCG seed list, which a kernel writes
+ * there.  Supports the best-first traversal (PQ, and SQ with
+ * ROVER_TRAVERSAL_BEST_FIRST); routed entries and the SQ heap traversals need
+ * scratch that grows with the batch or the index and are refused. */
+AEEResult rover_search_batch_device_noalloc(remote_handle64 h, const RoverSearchBatch *b,
+                                            void *stream) {
+    RoverCtx *c = ctx_of(h);
+    RvIndex ix;
+    RvBatch rb;
+    AEEResult rc;
+    uint32_t seed;
+    int32_t *work, *seeds;
+    if (!c || !b) return ROVER_EBADPARM;
+    if (!c->d_noalloc) {
+        snprintf(g_err, sizeof(g_err), "search_noalloc: no workspace on this device");
+        return ROVER_EFAILED;
+    }
+    if (b->route_entries > 0) {
+        snprintf(g_err, sizeof(g_err),
+                 "search_noalloc: routed entries need a routing scratch; pass entry_ids");
+        return ROVER_EBADPARM;
+    }
+    if (b->method == ROVER_METHOD_SQ && b->traversal != ROVER_TRAVERSAL_BEST_FIRST) {
+        snprintf(g_err, sizeof(g_err),
+                 "search_noalloc: SQ batches need traversal ROVER_TRAVERSAL_BEST_FIRST");
+        return ROVER_EBADPARM;
+    }
+    if (!b->entry_ids && b->entry_points > RV_NOALLOC_MAX_SEEDS) {
+        snprintf(g_err, sizeof(g_err),
+                 "search_noalloc: %d entry points exceed the workspace's %d seed ids",
+                 b->entry_points, RV_NOALLOC_MAX_SEEDS);
+        return ROVER_EBADPARM;
+    }
+    work = (int32_t *)c->d_noalloc;
+    seeds = (int32_t *)((char *)c->d_noalloc + RV_NOALLOC_SEED_OFFSET);
+    seed = b->seed ? b->seed : ROVER_ENTRY_POINT_SEED;
+    pthread_mutex_lock(&c->lock);
+    rc = rover_prepare_launch(c, b, b->pool_size, 1, &ix, &rb);
+    if (rc != ROVER_SUCCESS) goto out;
+    rc = ROVER_EFAILED;
+    rb.queue = work;
+    if (!b->entry_ids) {
+        int count = b->entry_points < ix.n ? b->entry_points : ix.n;
+        rb.entries = seeds;
+        rb.n_entries = count;
+        rb.entry_stride = 0;
+        if (b->num_queries > 0 &&
+            rv_seed_list(seeds, ix.n, count, ix.graph_entry, seed, stream)) {
+            set_err("search_noalloc seeds");
+            goto out;
+        }
+    }
+    if (rv_search_launch(&ix, &rb, b->grid_waves, stream)) {
+        set_err("search_noalloc launch");
+        goto out;
+    }
+    rc = ROVER_SUCCESS;
+out:
     pthread_mutex_unlock(&c->lock);
     return rc;
 }

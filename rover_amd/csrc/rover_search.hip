@@ -33,6 +33,7 @@
 #include <hip/hip_runtime.h>
 
 #include <float.h>
+#include <pthread.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -1402,6 +1403,76 @@ extern "C" int rv_fill_zero(void *p, size_t bytes, void *stream) {
         rv_zero_bytes_kernel<<<grid, 256, 0, s>>>((uint8_t *)p, bytes);
     }
     return rv_check(hipGetLastError(), "zero kernel");
+}
+
+// ------------------------------------------- fixed no-allocation workspace
+
+// Workspace of the entry point that allocates nothing: the work counter in
+// the first RV_NOALLOC_SEED_OFFSET bytes, then up to RV_NOALLOC_MAX_SEEDS
+// seed ids.  Each device holds its own copy for the life of the process; the
+// host reaches it only through the address hipGetSymbolAddress returns for
+// that device (the symbol's name is a host-side stand-in, not a device
+// address).
+static __device__ __align__(256) uint8_t scratch[RV_NOALLOC_BYTES];
+
+#define RV_MAX_DEVICES 64
+static void *g_scratch_ptr[RV_MAX_DEVICES];
+static pthread_mutex_t g_scratch_mu = PTHREAD_MUTEX_INITIALIZER;
+
+extern "C" int rv_noalloc_scratch(int device, void **out) {
+    int rc = 0;
+    *out = nullptr;
+    if (device < 0 || device >= RV_MAX_DEVICES) {
+        snprintf(g_rv_error, sizeof(g_rv_error), "rv_noalloc_scratch: device %d", device);
+        return -1;
+    }
+    pthread_mutex_lock(&g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int prev = 0;
+        void *p = nullptr;
+        rc = rv_check(hipGetDevice(&prev), "hipGetDevice");
+        if (!rc && prev != device) rc = rv_check(hipSetDevice(device), "hipSetDevice");
+        if (!rc) {
+            rc = rv_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)),
+                          "hipGetSymbolAddress(scratch)");
+            if (!rc && !p) {
+                snprintf(g_rv_error, sizeof(g_rv_error), "hipGetSymbolAddress: null");
+                rc = -1;
+            }
+            if (!rc) g_scratch_ptr[device] = p;
+        }
+        if (prev != device) hipSetDevice(prev);
+    }
+    *out = g_scratch_ptr[device];
+    pthread_mutex_unlock(&g_scratch_mu);
+    return rc;
+}
+
+// The reference's seed list (rpc_demo_pure_pq.c:192-208) written on the
+// device: graph_entry first (when in range), then the LCG draws mapped by a
+// multiply-shift.  One thread; count is at most a few thousand.
+__global__ void rv_lcg_seeds_kernel(int32_t *out, int n, int count, int graph_entry,
+                                    uint32_t seed) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int i = 0; i < count; i++) {
+        if (i == 0 && graph_entry >= 0 && graph_entry < n) {
+            out[i] = graph_entry;
+        } else {
+            seed = seed * 1103515245u + 12345u;
+            out[i] = (int32_t)(((uint64_t)seed * (uint64_t)(uint32_t)n) >> 32);
+        }
+    }
+}
+
+extern "C" int rv_seed_list(int32_t *d_out, int n, int count, int graph_entry,
+                            uint32_t seed, void *stream) {
+    if (!d_out || n <= 0 || count <= 0 || count > n) {
+        snprintf(g_rv_error, sizeof(g_rv_error), "rv_seed_list: bad parameters");
+        return -1;
+    }
+    rv_lcg_seeds_kernel<<<1, 64, 0, (hipStream_t)stream>>>(d_out, n, count, graph_entry,
+                                                          seed);
+    return rv_check(hipGetLastError(), "rv_lcg_seeds_kernel launch");
 }
 
 static RvLayout rv_layout_for(const RvIndex *ix, const RvBatch *b) {

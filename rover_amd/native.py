@@ -141,6 +141,7 @@ SIGNATURES = [
     ("rover_pure_pq_view_release", None, []),
     ("rover_search_batch", _I, [_H, _P]),
     ("rover_search_batch_device", _I, [_H, _P, _P]),
+    ("rover_search_batch_device_noalloc", _I, [_H, _P, _P]),
     ("rover_search_graph_create", _I, [_H, _P, _P, _P, C.POINTER(_P)]),
     ("rover_search_graph_launch", _I, [_P, _P]),
     ("rover_search_graph_destroy", _I, [_P]),

@@ -251,9 +251,10 @@ class Calculator:
                             d_num_outs=None, stream=None,
                             seed=native.ENTRY_POINT_SEED, d_entry_ids=None,
                             entry_count=0, entry_stride=0, route_entries=0,
-                            refine_mode=0, traversal=0):
+                            refine_mode=0, traversal=0, noalloc=False):
         """Device pointers in and out; asynchronous.  With per-query device
-        arrays, pool_size must be their maximum."""
+        arrays, pool_size must be their maximum.  noalloc=True calls
+        rover_search_batch_device_noalloc (fixed per-device workspace)."""
         b = self._batch_struct(d_queries, nq, pool_size, num_out, method,
                                correction, refine_candidates, entry_points,
                                expand_count, d_ids, d_dists, d_counts, stride,
@@ -261,8 +262,10 @@ class Calculator:
                                learned_stop, stop_threshold, d_pool_sizes,
                                d_num_outs, seed, d_entry_ids, entry_count,
                                entry_stride, route_entries, refine_mode, traversal)
-        check(native.lib().rover_search_batch_device(self.h, C.byref(b), stream),
-              "rover_search_batch_device")
+        fn = native.lib().rover_search_batch_device_noalloc if noalloc else \
+            native.lib().rover_search_batch_device
+        check(fn(self.h, C.byref(b), stream),
+              "rover_search_batch_device" + ("_noalloc" if noalloc else ""))
 
 
     def search_graph_create(self, d_queries, capacity, pool_size, num_out, d_ids,
