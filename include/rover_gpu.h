@@ -429,6 +429,21 @@ typedef struct RoverLearnedStopTrain {
 } RoverLearnedStopTrain;
 AEEResult rover_learned_stop_train(remote_handle64 h, RoverLearnedStopTrain *t);
 
+/* The plain search of a device batch (pointers in HBM, asynchronous on
+ * `stream`) with the stop features recorded, for training predictors
+ * outside the library and for checking one: every stop_every expansions the
+ * 16 raw features the learned stop evaluates go to
+ * d_feats [num_queries][trace_cap][16] and the expansion count to
+ * d_steps [num_queries][trace_cap]; per query d_meta [num_queries][18] holds
+ * the records kept, the last expansion that changed the top label_k
+ * (<= 16), then the expansions at which the final top label_k members were
+ * scored, ascending.  Results go to b's outputs as usual; learned_stop and
+ * route_entries must be 0.  No reference counterpart. */
+AEEResult rover_search_trace_device(remote_handle64 h, const RoverSearchBatch *b,
+                                    int stop_every, int label_k, int trace_cap,
+                                    float *d_feats, int32_t *d_steps,
+                                    int32_t *d_meta, void *stream);
+
 /* Installs / reads back the predictor: FP32 parameters in the flat order
  * w1[64][16] | b1[64] | w2[64][64] | b2[64] | w3[64] | b3[1] (rounded to
  * BF16 for inference) and the feature normalization (x - mean) * inv_std. */
@@ -456,6 +471,15 @@ AEEResult rover_mlp_adam_step(float *d_params, const float *d_grad, float *d_m1,
  * (rpc_demo_pure_pq.c:192-208).  Returns the count. */
 int rover_make_seed_ids(int num_vectors, int entry_count, int graph_entry,
                         uint32_t seed, int32_t *out_ids);
+
+/* The demo's result fingerprint (pure_pq_result_fingerprint,
+ * retrieval_rpc_demo_pure_pq.c:387-400): FNV-1a 64 over each result's id
+ * then distance bit pattern, 4 little-endian bytes each, chained from `hash`
+ * (start from ROVER_FINGERPRINT_BASIS; the demo chains every query's whole
+ * sorted pool in query order). */
+#define ROVER_FINGERPRINT_BASIS 14695981039346656037ull
+uint64_t rover_result_fingerprint(uint64_t hash, const int32_t *ids,
+                                  const float *dists, int count);
 
 /* Seeds of the live SQ path (retrieval_sqmd_seed.c:15-63): graph_entry
  * (when in range), then s = s*1103515245 + 12345, id = (s / 65536) % n,

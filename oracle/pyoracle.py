@@ -403,6 +403,13 @@ def seed_ids(n, entry_count, graph_entry, seed=42):
     return out[:cnt]
 
 
+def recall_hits(result_ids, result_count, gt_row, gt_k, k):
+    """orc_recall_hits: demo_recall_hits_at_k for one query."""
+    r = np.ascontiguousarray(result_ids, dtype=np.int32)
+    g = np.ascontiguousarray(gt_row, dtype=np.int32)
+    return oracle().orc_recall_hits(_p(r), int(result_count), _p(g), int(gt_k), int(k))
+
+
 def fingerprint(ids_rows, dist_rows, counts, h=14695981039346656037):
     for i in range(len(counts)):
         c = int(counts[i])

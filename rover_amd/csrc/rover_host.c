@@ -565,6 +565,23 @@ AEEResult calculator_retrieval_load(remote_handle64 h, const int32_t *graph,
     return ROVER_SUCCESS;
 }
 
+/* The reference demo's known-answer hash of search results
+ * (pure_pq_result_fingerprint, retrieval_rpc_demo_pure_pq.c:387-400):
+ * FNV-1a 64 over each result as 8 little-endian bytes, the id's 4 then the
+ * distance's bit pattern's 4, chained from `hash`. */
+uint64_t rover_result_fingerprint(uint64_t hash, const int32_t *ids, const float *dists,
+                                  int count) {
+    const uint64_t prime = 0x100000001b3ull;
+    for (int i = 0; i < count; i++) {
+        uint32_t dbits;
+        uint64_t rec;
+        memcpy(&dbits, &dists[i], 4);
+        rec = (uint64_t)(uint32_t)ids[i] | (uint64_t)dbits << 32;
+        for (int b = 0; b < 64; b += 8) hash = (hash ^ ((rec >> b) & 0xffu)) * prime;
+    }
+    return hash;
+}
+
 /* rpc_demo_pure_pq.c:192-208 (pure_pq_make_seed_ids); the DSP search uses
  * the same recurrence (retrieval_async_dsp_search.c:219-230). */
 int rover_make_seed_ids(int n, int entry_count, int graph_entry, uint32_t seed,

@@ -157,6 +157,44 @@ static void init_params(float *p, int P, uint32_t seed) {
     p[P - 1] = 0.0f;
 }
 
+/* The plain search of a device batch with its stop features recorded: every
+ * stop_every expansions the 16 features the learned stop would evaluate
+ * (d_feats [nq][trace_cap][16], raw) and the expansion count
+ * (d_steps [nq][trace_cap]); per query d_meta [nq][RV_TRACE_META]: records
+ * kept, last top-K change, then the expansion at which each final top-K
+ * member was scored, ascending. */
+AEEResult rover_search_trace_device(remote_handle64 h, const RoverSearchBatch *b,
+                                    int stop_every, int label_k, int trace_cap,
+                                    float *d_feats, int32_t *d_steps, int32_t *d_meta,
+                                    void *stream) {
+    RoverCtx *c = rover_ctx_of(h);
+    RvIndex ix;
+    RvBatch rb;
+    AEEResult rc;
+    if (!c || !b || !d_feats || !d_steps || !d_meta || stop_every <= 0 || trace_cap <= 0 ||
+        label_k <= 0 || label_k > RV_TRACE_TOPK || b->learned_stop || b->route_entries > 0 ||
+        (b->method == ROVER_METHOD_SQ && b->traversal != ROVER_TRAVERSAL_BEST_FIRST))
+        return ROVER_EBADPARM;
+    pthread_mutex_lock(&c->lock);
+    rc = rover_prepare_launch(c, b, b->pool_size, 0, &ix, &rb);
+    if (rc == ROVER_SUCCESS) {
+        rb.stop_mode = 2;
+        rb.stop_every = stop_every;
+        rb.stop_label_k = label_k;
+        rb.trace_feats = d_feats;
+        rb.trace_steps = d_steps;
+        rb.trace_meta = d_meta;
+        rb.trace_cap = trace_cap;
+        if (rover_stream_turn(c, stream) || rv_search_launch(&ix, &rb, b->grid_waves, stream) ||
+            rover_stream_done(c, stream)) {
+            rover_set_err("trace search");
+            rc = ROVER_EFAILED;
+        }
+    }
+    pthread_mutex_unlock(&c->lock);
+    return rc;
+}
+
 AEEResult rover_learned_stop_train(remote_handle64 h, RoverLearnedStopTrain *t) {
     RoverCtx *c = rover_ctx_of(h);
     RoverSearchBatch b;

@@ -147,8 +147,10 @@ SIGNATURES = [
     ("rover_search_graph_destroy", _I, [_P]),
     ("rover_autotune", _I, [_H, _P, _P, _I, _P]),
     ("rover_make_seed_ids", _I, [_I, _I, _I, C.c_uint32, _P]),
+    ("rover_result_fingerprint", C.c_uint64, [C.c_uint64, _P, _P, _I]),
     ("rover_sqmd_seed_ids", _I, [_I, _I, _I, C.c_uint32, _P, _I]),
     ("rover_learned_stop_train", _I, [_H, _P]),
+    ("rover_search_trace_device", _I, [_H, _P, _I, _I, _I, _P, _P, _P, _P]),
     ("rover_learned_stop_set", _I, [_H, _P, _I, _P, _P, _I, _I, _I]),
     ("rover_learned_stop_get", _I, [_H, _P, _I, _P, _P]),
     ("rover_mlp_param_count", _I, []),
@@ -273,6 +275,27 @@ def make_seed_ids(n, entry_points, graph_entry, seed=ENTRY_POINT_SEED):
     out = np.zeros(max(entry_points, 1), dtype=np.int32)
     cnt = lib().rover_make_seed_ids(n, entry_points, graph_entry, seed, ptr(out))
     return out[:cnt]
+
+
+FINGERPRINT_BASIS = 14695981039346656037
+
+
+def result_fingerprint(ids, dists, h=FINGERPRINT_BASIS):
+    """rover_result_fingerprint over one result list (ids, distances)."""
+    i = np.ascontiguousarray(ids, dtype=np.int32)
+    d = np.ascontiguousarray(dists, dtype=np.float32)
+    assert i.shape == d.shape
+    return lib().rover_result_fingerprint(h, ptr(i), ptr(d), int(i.size))
+
+
+def results_fingerprint(ids_rows, dist_rows, counts, h=FINGERPRINT_BASIS):
+    """The demo's fingerprint chained over queries: row q contributes its
+    first counts[q] results."""
+    for q in range(len(counts)):
+        c = int(counts[q])
+        if c > 0:
+            h = result_fingerprint(ids_rows[q, :c], dist_rows[q, :c], h)
+    return h
 
 
 class Event:

@@ -5,6 +5,7 @@ oracle/_ref/librover_ref.so from it):
 
     python tests/golden/make_golden.py           # every fixture
     python tests/golden/make_golden.py error4    # only the *_error4 files
+    python tests/golden/make_golden.py r8        # only pq_d128_m32_r8 (+ error4)
 
 Every expected output in the fixtures comes from the reference's own scalar
 C: retrieval_pure_pq_search_scalar_with_policy, retrieval_pq_build_lut,
@@ -447,6 +448,16 @@ def error4_case(name):
     print("wrote", path, os.path.getsize(path), "bytes")
 
 
+def pq_r8_case():
+    """The measured scorer's shape: M = 32 subspaces of 4 dims, 256
+    centroids, 8-bit residuals, 32 neighbors per expansion (the two-lane
+    online scorer) and 6 (at most 8 unvisited neighbors per expansion: the
+    grouped scorer)."""
+    pq_case("pq_d128_m32_r8", n=2000, dim=128, m=32, ks=256, R=32, res_bits=8,
+            comps=48, nq=12, Ls=(16, 64, 160), expands=(32, 6), seed=72)
+    error4_case("pq_d128_m32_r8")
+
+
 def main():
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "all"],
                    check=True)
@@ -464,6 +475,9 @@ def main():
     if sys.argv[1:] == ["rnn"]:
         rnn_case("rnn_build")
         return
+    if sys.argv[1:] == ["r8"]:
+        pq_r8_case()
+        return
     if sys.argv[1:] == ["error4"]:
         for name in ("pq_d128_m32", "pq_d96_m48", "pq_d64_ks16"):
             error4_case(name)
@@ -479,6 +493,7 @@ def main():
     seeds_case("seeds")
     for name in ("pq_d128_m32", "pq_d96_m48", "pq_d64_ks16"):
         error4_case(name)
+    pq_r8_case()
     sqmd_case("sqmd_d128")
     split_case("sqmd_split")
     graph_case("graph_build")

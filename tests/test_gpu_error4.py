@@ -15,7 +15,7 @@ from rover_amd import index as rindex
 from rover_amd.search import Calculator
 
 pytestmark = pytest.mark.gpu
-PQ_SETS = ["pq_d128_m32", "pq_d96_m48", "pq_d64_ks16"]
+PQ_SETS = ["pq_d128_m32", "pq_d96_m48", "pq_d64_ks16", "pq_d128_m32_r8"]
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 

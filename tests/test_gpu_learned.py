@@ -170,3 +170,70 @@ def test_training_is_reproducible(trained):
     for a, b in zip(first, second):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     assert t2.final_loss == t.final_loss
+
+
+def test_fused_predictor_matches_cpu_forward(trained):
+    """The predictor fused in the search kernel (grouped BF16 weights, FP32
+    accumulation) against oracle/mlp_ref.py's float64 forward pass: the
+    features the plain search records every stop_every expansions
+    (rover_search_trace_device), normalized in float32 as the kernel does,
+    give per record a probability; the first record at or after min_steps
+    whose probability reaches 0.9 is where the learned search must stop
+    (its expansion count), or it runs as the plain search does.  Queries
+    with a probability within 1e-4 of 0.9 before the stop (float32 against
+    float64 arithmetic differ by about 1e-6 there) are left out as
+    ambiguous; every other query must stop exactly there."""
+    calc, ix, queries, gt, t = trained
+    say("case fused predictor vs mlp_ref")
+    L, K, cap = 128, 10, 512
+    P = native.lib().rover_mlp_param_count()
+    params = np.zeros(P, np.float32)
+    mean = np.zeros(16, np.float32)
+    istd = np.zeros(16, np.float32)
+    native.check(native.lib().rover_learned_stop_get(calc.h, params.ctypes.data, P,
+                                                     mean.ctypes.data, istd.ctypes.data),
+                 "rover_learned_stop_get")
+    rows = calc.route_entries(queries, 4)
+    nq = queries.shape[0]
+    dq = native.DeviceBuffer.from_array(queries)
+    de = native.DeviceBuffer.from_array(rows)
+    dids, dd, dc = (native.DeviceBuffer(nq * K * 4), native.DeviceBuffer(nq * K * 4),
+                    native.DeviceBuffer(nq * 4))
+    dfeat = native.DeviceBuffer(nq * cap * 16 * 4)
+    dsteps = native.DeviceBuffer(nq * cap * 4)
+    dmeta = native.DeviceBuffer(nq * 18 * 4)
+    b = calc._batch_struct(dq.ptr, nq, L, K, native.METHOD_PQ, 1, 0, 1, 32, dids.ptr,
+                           dd.ptr, dc.ptr, K, None, 0, 0, 0, 0, 0.0, None, None,
+                           native.ENTRY_POINT_SEED, de.ptr, 4, 4)
+    native.check(native.lib().rover_search_trace_device(
+        calc.h, C.byref(b), int(t.stop_every), 10, cap, dfeat.ptr, dsteps.ptr, dmeta.ptr,
+        None), "rover_search_trace_device")
+    native.synchronize()
+    feats = dfeat.download(np.float32, nq * cap * 16).reshape(nq, cap, 16)
+    steps = dsteps.download(np.int32, nq * cap).reshape(nq, cap)
+    meta = dmeta.download(np.int32, nq * 18).reshape(nq, 18)
+    _, _, _, st_plain = calc.search_batch(queries, L, K, stats=True, correction=1,
+                                          entry_ids=rows)
+    _, _, _, st_learn = calc.search_batch(queries, L, K, stats=True, correction=1,
+                                          entry_ids=rows, learned_stop=1,
+                                          stop_threshold=0.9)
+    checked = early = 0
+    for q in range(nq):
+        recs = int(meta[q, 0])
+        assert 0 < recs < cap
+        assert np.all(steps[q, :recs] == t.stop_every * np.arange(1, recs + 1))
+        x = ((feats[q, :recs] - mean).astype(np.float32) * istd).astype(np.float32)
+        logit, _ = mlp_ref.forward(params.astype(np.float64), x)
+        p = 1.0 / (1.0 + np.exp(-logit))
+        live = steps[q, :recs] >= t.min_steps
+        hit = np.nonzero(live & (p >= 0.9))[0]
+        stop_at = int(steps[q, hit[0]]) if len(hit) else int(st_plain[q, 2])
+        upto = hit[0] + 1 if len(hit) else recs
+        if np.any(live[:upto] & (np.abs(p[:upto] - 0.9) < 1e-4)):
+            continue
+        checked += 1
+        early += stop_at < int(st_plain[q, 2])
+        assert int(st_learn[q, 2]) == stop_at, (q, int(st_learn[q, 2]), stop_at)
+    say("fused predictor: %d of %d queries checked, %d stopped early" % (checked, nq, early))
+    assert checked >= 0.9 * nq
+    assert early >= 0.2 * checked

@@ -13,7 +13,7 @@ from rover_amd import native
 from rover_amd.search import Calculator, pure_pq_search_scalar
 
 pytestmark = pytest.mark.gpu
-PQ_SETS = ["pq_d128_m32", "pq_d96_m48", "pq_d64_ks16"]
+PQ_SETS = ["pq_d128_m32", "pq_d96_m48", "pq_d64_ks16", "pq_d128_m32_r8"]
 
 
 def say(*a):

@@ -6,13 +6,17 @@ fixtures leave something open, the tests check what the mathematics fixes:
 closed forms in float64, brute force on tiny inputs, and invariants of the
 best-first search.
 """
+import os
+import struct
+
 import numpy as np
 import pytest
 
 import pyoracle
+from rover_amd import data
 from rover_amd import index as rindex
 
-PQ_SETS = ["pq_d128_m32", "pq_d96_m48", "pq_d64_ks16"]
+PQ_SETS = ["pq_d128_m32", "pq_d96_m48", "pq_d64_ks16", "pq_d128_m32_r8"]
 
 
 def oindex(g):
@@ -353,3 +357,100 @@ def test_oracle_matches_reference_live():
                 assert tuple(st) == tuple(int(v) for v in st_r)
     finally:
         ref.ref_residual_close(h)
+
+
+def _fnv1a64(data, h=0xcbf29ce484222325):
+    """FNV-1a 64 as published (offset basis 0xcbf29ce484222325, prime
+    0x100000001b3), byte by byte."""
+    for byte in data:
+        h ^= byte
+        h = (h * 0x100000001b3) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def test_fnv1a64_known_answers():
+    # the published FNV-1a 64 test vectors pin the restatement used below
+    assert _fnv1a64(b"") == 0xcbf29ce484222325
+    assert _fnv1a64(b"a") == 0xaf63dc4c8601ec8c
+    assert _fnv1a64(b"foobar") == 0x85944171f73967e8
+
+
+def test_result_fingerprint_bytes_and_order():
+    """pure_pq_result_fingerprint (rpc_demo_pure_pq.c:387-400): FNV-1a 64
+    over each result's id then distance bits, 4 bytes each, low byte first,
+    chained over results and queries from the offset basis."""
+    rng = np.random.default_rng(11)
+    ids = rng.integers(-5, 1 << 30, size=(3, 7)).astype(np.int32)
+    d = rng.random((3, 7)).astype(np.float32) * 1e4
+    counts = np.array([7, 0, 4], dtype=np.int32)
+    raw = b"".join(struct.pack("<iI", int(ids[q, i]), int(d[q, i].view(np.uint32)))
+                   for q in range(3) for i in range(counts[q]))
+    assert pyoracle.fingerprint(ids, d, counts) == _fnv1a64(raw)
+    # no results leaves the offset basis; swapped fields or big-endian bytes differ
+    assert pyoracle.fingerprint(ids, d, np.zeros(3, np.int32)) == 0xcbf29ce484222325
+    swapped = b"".join(struct.pack("<Ii", int(d[q, i].view(np.uint32)), int(ids[q, i]))
+                       for q in range(3) for i in range(counts[q]))
+    assert _fnv1a64(swapped) != _fnv1a64(raw)
+    big = b"".join(struct.pack(">iI", int(ids[q, i]), int(d[q, i].view(np.uint32)))
+                   for q in range(3) for i in range(counts[q]))
+    assert _fnv1a64(big) != _fnv1a64(raw)
+    # the product's fingerprint (librover) equals the oracle's
+    from rover_amd import native
+    if os.path.exists(native.LIB_PATH):
+        h = 0xcbf29ce484222325
+        for q in range(3):
+            h = native.result_fingerprint(ids[q, :counts[q]], d[q, :counts[q]], h)
+        assert h == _fnv1a64(raw)
+
+
+def test_recall_hits_brute_force():
+    """demo_recall_hits_at_k (retrieval_rpc_demo_support.c:1709-1728): the
+    number of i < min(k, result_count) whose id is among the first
+    min(gt_k, k) ground-truth ids (a repeated result id counts each time),
+    against a direct count on random cases, with short result lists and
+    ground-truth rows longer and shorter than k; the product's recall@k
+    (rover_amd.data.recall_at_k) is the same count over k."""
+    rng = np.random.default_rng(12)
+    for trial in range(400):
+        gt_k = int(rng.integers(1, 24))
+        k = int(rng.integers(1, 24))
+        rc = int(rng.integers(0, 30))
+        gt = rng.choice(50, size=gt_k, replace=False).astype(np.int32)
+        res = rng.integers(0, 50, size=max(rc, 1)).astype(np.int32)
+        if trial % 5 == 0 and gt_k > k:
+            # a result that is in gt only beyond position k must not count
+            res[0] = gt[gt_k - 1]
+        head = set(int(x) for x in gt[:min(gt_k, k)])
+        expect = sum(1 for i in range(min(k, rc)) if int(res[i]) in head)
+        assert pyoracle.recall_hits(res, rc, gt, gt_k, k) == expect, (trial, k, gt_k, rc)
+        row = np.full((1, max(rc, 1)), -1, dtype=np.int32)
+        row[0, :rc] = res[:rc]
+        assert data.recall_at_k(row, np.array([rc]), gt[None, :], k) == expect / float(k)
+    # k larger than the result count
+    assert pyoracle.recall_hits(np.array([3, 4]), 2, np.array([4, 3, 9]), 3, 10) == 2
+    assert pyoracle.recall_hits(np.array([9]), 1, np.array([4, 3, 9]), 3, 2) == 0
+
+
+def test_sq_encoder_rounds_exact_halves_away_from_zero():
+    """roundf at exact half steps (retrieval_rpc_demo_support.c:664-700):
+    with the range [0, 255] (8-bit) or [0, 15] (4-bit) the step is exactly
+    1, so x = j + 0.5 codes to j + 1; round-half-even would give even codes."""
+    for bits in (8, 4):
+        levels = (1 << bits) - 1
+        halves = np.arange(levels, dtype=np.float32) + np.float32(0.5)
+        row = np.concatenate([[0.0, float(levels)], halves]).astype(np.float32)
+        row = np.resize(row, ((len(row) + 1) // 2) * 2).astype(np.float32)
+        x = row[None, :]
+        scale, zero = pyoracle.sq_range_params(x, bits)
+        assert scale == np.float32(1.0) and zero == np.float32(0.0)
+        expect = np.floor(row.astype(np.float64) + 0.5).astype(np.int64)
+        for codes in (pyoracle.sq_encode(x, bits, scale, zero),
+                      rindex.encode_sq(x, bits, scale, zero)):
+            if bits == 4:
+                q = np.empty(row.shape, dtype=np.int64)
+                q[0::2] = codes[0, : (len(row) + 1) // 2] & 15
+                q[1::2] = codes[0, : len(row) // 2] >> 4
+            else:
+                q = codes[0].astype(np.int64)
+            assert np.array_equal(q, expect)
+            assert q[3] == 2 and q[4] == 3  # 1.5 -> 2, 2.5 -> 3 (not 2)
