@@ -25,6 +25,7 @@ config.c:129-155 does.  `Bundle.load` puts a bundle into a `Calculator`.
 """
 import hashlib
 import json
+import mmap
 import os
 import struct
 from dataclasses import dataclass, field
@@ -37,88 +38,149 @@ ROVER_GRAPH_MAGIC = 0x35314252
 BLOCK = 4096
 
 
-def _sha(path):
+class BundleError(ValueError):
+    """A bundle or one of its member files does not hold what its format or
+    index.json says it holds."""
+
+
+def file_digest(path):
+    """Hex SHA-256 of a file, hashed from a read-only memory map."""
     h = hashlib.sha256()
     with open(path, "rb") as f:
-        for chunk in iter(lambda: f.read(8 << 20), b""):
-            h.update(chunk)
+        if os.fstat(f.fileno()).st_size:
+            with mmap.mmap(f.fileno(), 0, access=mmap.ACCESS_READ) as mm:
+                h.update(mm)
     return h.hexdigest()
 
 
-def _record(path, root):
-    path, root = Path(path), Path(root)
-    return dict(path=str(path.relative_to(root)), bytes=path.stat().st_size, sha256=_sha(path))
+@dataclass(frozen=True)
+class MemberRecord:
+    """One entry of index.json's `files`: the member's path relative to the
+    bundle directory, its size and its SHA-256."""
+    path: str
+    bytes: int
+    sha256: str
+
+    @classmethod
+    def of(cls, path, root):
+        p = Path(path)
+        return cls(p.relative_to(Path(root)).as_posix(), p.stat().st_size, file_digest(p))
+
+    def as_json(self):
+        return {"path": self.path, "bytes": self.bytes, "sha256": self.sha256}
+
+
+def _member_json(path, root):
+    return MemberRecord.of(path, root).as_json()
 
 
 # ---------------------------------------------------------------- graphs
+#
+# Each decoder turns a file's bytes (a read-only uint8 memory map) into the
+# dense int32 [n][degree] rows the search reads, -1 padded; sizes and headers
+# are checked against the expected shape before anything is decoded.
 
-def _check_rows(graph, n):
-    for start in range(0, n, 8192):
-        rows = np.asarray(graph[start:start + 8192])
-        if np.any(rows < -1) or np.any(rows >= n):
-            raise ValueError("Out-of-range graph neighbor")
-        if np.any(rows == np.arange(start, start + len(rows))[:, None]):
-            raise ValueError("Graph contains a self edge")
-        ordered = np.sort(rows, axis=1)
-        if np.any((ordered[:, 1:] >= 0) & (ordered[:, 1:] == ordered[:, :-1])):
-            raise ValueError("Graph contains duplicate edges")
+def _expect_size(buf, want, what):
+    if buf.size != want:
+        raise BundleError("%s: %d bytes, expected %d" % (what, buf.size, want))
+
+
+def _decode_rover(buf, n, degree, dim):
+    _expect_size(buf, 16 + 4 * n * degree, "rover graph")
+    head = tuple(int(v) for v in buf[:16].view("<i4"))
+    if head != (ROVER_GRAPH_MAGIC, n, dim, degree):
+        raise BundleError("rover graph header %r does not describe n=%d dim=%d degree=%d"
+                          % (head, n, dim, degree))
+    return buf[16:].view("<i4").reshape(n, degree)
+
+
+def _decode_dense(buf, n, degree, dim):
+    _expect_size(buf, 4 * n * degree, "dense graph")
+    return buf.view("<i4").reshape(n, degree)
+
+
+def _decode_nndescent(buf, n, degree, dim):
+    # a 12-byte (n, degree, 0) header, then per node `degree` int32 ids
+    # followed by `degree` 4-byte distances, which are dropped
+    _expect_size(buf, 12 + 8 * n * degree, "nndescent graph")
+    head = tuple(int(v) for v in buf[:12].view("<i4"))
+    if head != (n, degree, 0):
+        raise BundleError("nndescent graph header %r, expected (%d, %d, 0)"
+                          % (head, n, degree))
+    rows = buf[12:].reshape(n, 8 * degree)[:, :4 * degree]
+    return np.ascontiguousarray(rows).view("<i4").reshape(n, degree)
+
+
+def _decode_nsg(buf, n, degree, dim):
+    # u32 width, u32 entry, then per node u32 k and k ids; a row may not be
+    # wider than the header's width nor than the bundle's degree
+    if buf.size < 8 or buf.size % 4:
+        raise BundleError("nsg graph: %d bytes is not a header plus 4-byte words"
+                          % buf.size)
+    words = buf.view("<u4")
+    width, entry = int(words[0]), int(words[1])
+    if width == 0 or entry >= n:
+        raise BundleError("nsg graph header: width %d, entry %d for %d nodes"
+                          % (width, entry, n))
+    out = np.full((n, degree), -1, dtype="<i4")
+    at = 2
+    for node in range(n):
+        if at >= words.size:
+            raise BundleError("nsg graph ends before node %d" % node)
+        k = int(words[at])
+        if k > min(width, degree):
+            raise BundleError("nsg node %d has %d neighbors; the header allows %d and "
+                              "the bundle's degree is %d" % (node, k, width, degree))
+        if at + 1 + k > words.size:
+            raise BundleError("nsg graph ends inside node %d" % node)
+        out[node, :k] = words[at + 1:at + 1 + k].view("<i4")
+        at += 1 + k
+    if at != words.size:
+        raise BundleError("nsg graph: %d words after the last node" % (words.size - at))
+    return out
+
+
+_GRAPH_DECODERS = {"rover": _decode_rover, "dense": _decode_dense,
+                   "nndescent": _decode_nndescent, "nsg": _decode_nsg}
+
+
+def check_adjacency(graph, n, slab_entries=1 << 22):
+    """Every id in [-1, n); no node lists itself; no node lists a neighbor
+    twice.  Duplicates are found as repeated edge keys row * n + id over
+    slabs of about slab_entries entries."""
+    if graph.size == 0:
+        return
+    lo, hi = int(graph.min()), int(graph.max())
+    if lo < -1 or hi >= n:
+        raise BundleError("graph neighbor id %d outside [-1, %d)" % (lo if lo < -1 else hi, n))
+    degree = graph.shape[1]
+    step = max(1, slab_entries // max(degree, 1))
+    for r0 in range(0, graph.shape[0], step):
+        slab = np.asarray(graph[r0:r0 + step], dtype=np.int64)
+        owner = np.arange(r0, r0 + slab.shape[0], dtype=np.int64)[:, None]
+        selfish = np.nonzero((slab == owner).any(axis=1))[0]
+        if selfish.size:
+            raise BundleError("graph node %d lists itself" % (r0 + int(selfish[0])))
+        keys = np.sort((owner * n + slab)[slab >= 0])
+        rep = np.nonzero(keys[1:] == keys[:-1])[0]
+        if rep.size:
+            key = int(keys[rep[0]])
+            raise BundleError("graph node %d lists neighbor %d twice" % (key // n, key % n))
 
 
 def read_graph(path, kind, n, degree, dim):
-    """int32 [n][degree] from a graph file of format `kind` ('rover',
-    'dense', 'nndescent', 'nsg'), with index_bundle.py:43-92's checks:
-    sizes and headers, neighbors in [-1, n), no self edges, no duplicates;
-    an NSG row longer than `degree` is an error here (the runtime loader,
-    `load_nsg`, truncates instead)."""
-    path = Path(path)
-    size = path.stat().st_size
-    if kind == "rover":
-        if size != 16 + n * degree * 4:
-            raise ValueError("Rover graph size mismatch")
-        with path.open("rb") as f:
-            if f.read(16) != struct.pack("<4i", ROVER_GRAPH_MAGIC, n, dim, degree):
-                raise ValueError("Rover graph header mismatch")
-        graph = np.fromfile(path, dtype="<i4", offset=16).reshape(n, degree)
-    elif kind == "dense":
-        if size != n * degree * 4:
-            raise ValueError("Dense graph size mismatch")
-        graph = np.fromfile(path, dtype="<i4").reshape(n, degree)
-    elif kind == "nndescent":
-        with path.open("rb") as f:
-            header = struct.unpack("<3i", f.read(12))
-        if header != (n, degree, 0) or size != 12 + n * degree * 8:
-            raise ValueError("NNDescent graph shape mismatch")
-        rows = np.fromfile(path, dtype="u1", offset=12).reshape(n, degree * 8)
-        graph = np.ascontiguousarray(rows[:, :degree * 4]).view("<i4").reshape(n, degree)
-    elif kind == "nsg":
-        graph = np.full((n, degree), -1, dtype="<i4")
-        data = np.fromfile(path, dtype="<u4")
-        if len(data) < 2:
-            raise ValueError("NSG header mismatch")
-        width, entry = int(data[0]), int(data[1])
-        if not width or entry >= n:
-            raise ValueError("NSG header mismatch")
-        pos = 2
-        for i in range(n):
-            if pos >= len(data):
-                raise ValueError("Truncated NSG graph")
-            k = int(data[pos])
-            pos += 1
-            if k > width:
-                raise ValueError("NSG row exceeds declared width")
-            if k > degree:
-                raise ValueError("NSG row exceeds graph-degree; provide a graph within the "
-                                 "requested width")
-            if pos + k > len(data):
-                raise ValueError("Truncated NSG graph")
-            graph[i, :k] = data[pos:pos + k].view("<i4")
-            pos += k
-        if pos != len(data) or size % 4:
-            raise ValueError("Trailing NSG graph rows")
-    else:
-        raise ValueError("Unknown graph format: " + str(kind))
-    graph = np.ascontiguousarray(graph, dtype=np.int32)
-    _check_rows(graph, n)
+    """int32 [n][degree] from a graph file in format `kind` ('rover',
+    'dense', 'nndescent', 'nsg'; the formats of index_bundle.py:43-92), its
+    size and header checked against (n, degree, dim) and its rows by
+    check_adjacency.  An NSG row wider than `degree` is refused here (the
+    runtime loader, `load_nsg`, truncates instead)."""
+    decode = _GRAPH_DECODERS.get(kind)
+    if decode is None:
+        raise BundleError("unknown graph format %r (known: %s)"
+                          % (kind, ", ".join(sorted(_GRAPH_DECODERS))))
+    buf = np.fromfile(path, dtype=np.uint8)
+    graph = np.array(decode(buf, n, degree, dim), dtype=np.int32, order="C")
+    check_adjacency(graph, n)
     return graph
 
 
@@ -446,56 +508,91 @@ def read_sq(directory, n, dim):
 
 # -------------------------------------------------------------- index.json
 
+# Members every bundle of a method lists in index.json, and member file names
+# the bundle format reserves for other tools.
+_METHOD_MEMBERS = {
+    "sq": frozenset({"l1", "residual", "qparams"}),
+    "pq": frozenset({"pq_book", "pq_codes", "pq_residual", "pq_qparams", "pq_manifest",
+                     "pq_residual_manifest"}),
+}
+_RESERVED_NAMES = frozenset({"insert.checkpoint"})
+
+
+def pq_member_names(residual_bits, prefix="index"):
+    """The file names a PQ bundle's artifacts carry: all share the prefix."""
+    res = "%s.pqres%d" % (prefix, residual_bits)
+    return {"pq_book": prefix + ".pqbook", "pq_codes": prefix + ".pqcodes",
+            "pq_manifest": prefix + ".pq.json", "pq_residual": res,
+            "pq_qparams": res + ".qparams", "pq_residual_manifest": res + ".json"}
+
+
+def _check_fields(meta):
+    if meta.get("schema") != SCHEMA:
+        raise BundleError("index.json schema %r is not %s" % (meta.get("schema"), SCHEMA))
+    method = meta.get("method")
+    if method not in _METHOD_MEMBERS:
+        raise BundleError("index.json method %r is neither sq nor pq" % (method,))
+    n, dim, degree = meta.get("n"), meta.get("dim"), meta.get("graph_degree")
+    if not all(isinstance(v, int) for v in (n, dim, degree)) or not 0 < dim <= 1024 or \
+            not 0 < degree < n <= 2**31 - 1:
+        raise BundleError("index.json shape n=%r dim=%r graph_degree=%r out of range"
+                          % (n, dim, degree))
+    code_bits = meta.get("sq_bits" if method == "sq" else "pq_bits")
+    if meta.get("residual_bits") not in (4, 8) or code_bits not in (4, 8):
+        raise BundleError("index.json code widths %r / residual %r: each must be 4 or 8"
+                          % (code_bits, meta.get("residual_bits")))
+
+
+def _member_path(root, key, rel):
+    p = Path(rel)
+    full = (root / p).resolve()
+    if p.is_absolute() or root not in full.parents or not full.is_file():
+        raise BundleError("index member %s (%s) is not a file inside the bundle" % (key, rel))
+    return full
+
+
 def validate(directory, third=True, require_graph=True):
-    """index.json and its members as index_bundle.py:187-230 checks them:
-    schema, method, dimensions, bit widths, required members, each member's
-    size and SHA-256, the PQ file names.  Returns the parsed index.json."""
-    directory = Path(directory).resolve()
-    data = json.loads((directory / "index.json").read_text())
-    if data.get("schema") != SCHEMA:
-        raise ValueError("Unsupported Rover index schema")
-    if data.get("method") not in ("sq", "pq") or not 0 < data["dim"] <= 1024 or \
-            not 0 < data["graph_degree"] < data["n"] <= 2147483647:
-        raise ValueError("Invalid index method or dimensions")
-    if data["residual_bits"] not in (4, 8) or \
-            data.get("sq_bits" if data["method"] == "sq" else "pq_bits") not in (4, 8):
-        raise ValueError("Index bits must be 4 or 8")
-    required = (["l1", "residual", "qparams"] if data["method"] == "sq" else
-                ["pq_book", "pq_codes", "pq_residual", "pq_qparams", "pq_manifest",
-                 "pq_residual_manifest"])
-    if require_graph:
-        required.append("graph")
-    if third:
-        required.append("error4")
-    if data.get("reordered"):
-        required.append("order_map")
-    if data.get("residual_layout", "a") != "a":
-        required.append("residual_map")
-    for key in required:
-        if key not in data["files"]:
-            raise ValueError("Index missing " + key)
-    names = set()
-    for key, item in data["files"].items():
-        if key == "error4" and not third:
-            continue
-        path = (directory / item["path"]).resolve()
-        if Path(item["path"]).is_absolute() or directory not in path.parents or \
-                not path.is_file():
-            raise ValueError("Invalid index member " + key)
-        if path.name in names or path.name == "insert.checkpoint":
-            raise ValueError("Colliding index member names")
-        names.add(path.name)
-        if path.stat().st_size != item["bytes"] or _sha(path) != item["sha256"]:
-            raise ValueError("Index member changed: " + key)
-    if data["method"] == "pq":
-        sfx = ".pqres%d" % data["residual_bits"]
-        want = dict(pq_book="index.pqbook", pq_codes="index.pqcodes",
-                    pq_manifest="index.pq.json", pq_residual="index" + sfx,
-                    pq_qparams="index" + sfx + ".qparams",
-                    pq_residual_manifest="index" + sfx + ".json")
-        if any(Path(data["files"][k]["path"]).name != v for k, v in want.items()):
-            raise ValueError("PQ bundle filenames must use the index prefix")
-    return data
+    """Checks a bundle directory against its index.json (the rover.index.v1
+    rules of index_bundle.py:187-230) and returns the parsed index.json:
+    the schema, method, shape and bit-width fields; the members the method
+    and flags need (graph unless require_graph is False, error4 when third,
+    order_map when reordered, residual_map for a non-default residual
+    layout); PQ artifacts named with the common `index` prefix; members as
+    files inside the directory with distinct names none of which the format
+    reserves; and each member's size and SHA-256.  error4 is not checked
+    when third is False.  Cheap checks come first, digests last."""
+    root = Path(directory).resolve()
+    meta = json.loads((root / "index.json").read_text())
+    _check_fields(meta)
+    files = meta.get("files") or {}
+    needed = set(_METHOD_MEMBERS[meta["method"]])
+    needed |= {key for key, on in (("graph", require_graph), ("error4", third),
+                                   ("order_map", bool(meta.get("reordered"))),
+                                   ("residual_map", meta.get("residual_layout", "a") != "a"))
+               if on}
+    missing = sorted(needed - set(files))
+    if missing:
+        raise BundleError("index.json lists no %s" % ", ".join(missing))
+    if meta["method"] == "pq":
+        wrong = {k: Path(files[k]["path"]).name
+                 for k, want in pq_member_names(meta["residual_bits"]).items()
+                 if Path(files[k]["path"]).name != want}
+        if wrong:
+            raise BundleError("PQ artifacts must share the 'index' prefix: %s" % wrong)
+    members = {k: v for k, v in files.items() if third or k != "error4"}
+    paths = {k: _member_path(root, k, v["path"]) for k, v in members.items()}
+    names = [p.name for p in paths.values()]
+    clash = sorted({x for x in names if names.count(x) > 1} | (set(names) & _RESERVED_NAMES))
+    if clash:
+        raise BundleError("index member names collide or are reserved: %s" % clash)
+    for key, p in paths.items():
+        if p.stat().st_size != members[key]["bytes"]:
+            raise BundleError("index member %s is %d bytes, index.json says %d"
+                              % (key, p.stat().st_size, members[key]["bytes"]))
+    for key, p in paths.items():
+        if file_digest(p) != members[key]["sha256"]:
+            raise BundleError("index member %s does not match its SHA-256" % key)
+    return meta
 
 
 @dataclass
@@ -592,19 +689,19 @@ def write_pq_bundle(directory, graph, graph_entry, codebook, codes, res_codes, r
     write_pq_residual(prefix, meta, res_bits, res_codes, res_qparams)
     sfx = ".pqres%d" % res_bits
     (d / "index.pq.json").write_text(json.dumps(
-        {k: _record(str(prefix) + e, d) for k, e in (("codebook", ".pqbook"),
+        {k: _member_json(str(prefix) + e, d) for k, e in (("codebook", ".pqbook"),
                                                        ("codes", ".pqcodes"))}, indent=2))
     (d / ("index" + sfx + ".json")).write_text(json.dumps(
-        {k: _record(str(prefix) + sfx + e, d) for k, e in (("blocks", ""),
+        {k: _member_json(str(prefix) + sfx + e, d) for k, e in (("blocks", ""),
                                                              ("qparams", ".qparams"))}, indent=2))
-    files = dict(graph=_record(d / "graph.rover", d))
+    files = dict(graph=_member_json(d / "graph.rover", d))
     for key, ext in (("pq_book", ".pqbook"), ("pq_codes", ".pqcodes"),
                      ("pq_manifest", ".pq.json"), ("pq_residual", sfx),
                      ("pq_qparams", sfx + ".qparams"), ("pq_residual_manifest", sfx + ".json")):
-        files[key] = _record(str(prefix) + ext, d)
+        files[key] = _member_json(str(prefix) + ext, d)
     if error4 is not None:
         (d / "error4.refine").write_bytes(error4)
-        files["error4"] = _record(d / "error4.refine", d)
+        files["error4"] = _member_json(d / "error4.refine", d)
     data = dict(schema=SCHEMA, n=n, dim=dim, graph_degree=degree, method="pq",
                 residual_bits=res_bits, pq_bits=8 if ks == 256 else 4, pq_m=m,
                 third_residual=error4 is not None, reordered=False, layout="original",
@@ -622,7 +719,7 @@ def write_sq_bundle(directory, graph, graph_entry, codes, bits, scale, zero, res
     n, degree = graph.shape
     write_graph(d / "graph.rover", graph, dim)
     write_sq(d, codes, bits, scale, zero, res_codes, res_qparams, res_bits, dim)
-    files = {k: _record(d / f, d) for k, f in (("graph", "graph.rover"), ("l1", "l1.bin"),
+    files = {k: _member_json(d / f, d) for k, f in (("graph", "graph.rover"), ("l1", "l1.bin"),
                                                 ("residual", "residual.bin"),
                                                 ("qparams", "qparams.bin"))}
     data = dict(schema=SCHEMA, n=n, dim=dim, graph_degree=degree, method="sq",

@@ -266,3 +266,70 @@ def test_sq_files_round_trip(golden, tmp_path):
         assert np.array_equal(c2, codes) and np.array_equal(r2, res) and np.array_equal(q2, qp)
         with pytest.raises(ValueError):
             bundle.read_sq(d, n + 1, dim)
+
+
+def test_validate_rejections(pq, tmp_path):
+    """What validate refuses, one planted fault at a time on a good PQ
+    bundle: schema, method, shape, bit widths, a missing member, an
+    artifact without the common prefix, a member outside the bundle, two
+    members with one file name, the reserved checkpoint name, a size and a
+    digest that disagree with the file."""
+    g = pq
+    d = tmp_path / "b"
+    bundle.write_pq_bundle(d, g["graph"], int(g["entry"]), g["codebook"], g["codes"],
+                           g["res_codes"], g["res_qparams"], int(g["res_bits"]))
+    good = json.loads((d / "index.json").read_text())
+    assert bundle.validate(d, third=False) == good
+
+    def refused(mutate, third=False, match=None):
+        meta = json.loads(json.dumps(good))
+        mutate(meta)
+        (d / "index.json").write_text(json.dumps(meta))
+        with pytest.raises(bundle.BundleError, match=match):
+            bundle.validate(d, third=third)
+        (d / "index.json").write_text(json.dumps(good))
+
+    refused(lambda m: m.update(schema="rover.index.v0"), match="schema")
+    refused(lambda m: m.update(method="ivf"), match="method")
+    refused(lambda m: m.update(dim=0), match="shape")
+    refused(lambda m: m.update(graph_degree=m["n"]), match="shape")
+    refused(lambda m: m.update(residual_bits=6), match="widths")
+    refused(lambda m: m["files"].pop("pq_qparams"), match="pq_qparams")
+    refused(lambda m: None, third=True, match="error4")
+    refused(lambda m: m.update(reordered=True), match="order_map")
+    (d / "other.pqbook").write_bytes((d / "index.pqbook").read_bytes())
+    refused(lambda m: m["files"]["pq_book"].update(path="other.pqbook"), match="prefix")
+    (tmp_path / "outside.rover").write_bytes((d / "graph.rover").read_bytes())
+    refused(lambda m: m["files"]["graph"].update(path="../outside.rover"), match="inside")
+    refused(lambda m: m["files"]["graph"].update(path=str(d / "graph.rover")),
+            match="inside")
+    sub = d / "sub"
+    sub.mkdir()
+    (sub / "graph.rover").write_bytes((d / "graph.rover").read_bytes())
+    refused(lambda m: m["files"].update(extra=dict(m["files"]["graph"], path="sub/graph.rover")),
+            match="collide")
+    (d / "insert.checkpoint").write_bytes(b"x")
+    refused(lambda m: m["files"].update(ckpt=dict(path="insert.checkpoint", bytes=1,
+                                                  sha256=hashlib.sha256(b"x").hexdigest())),
+            match="reserved")
+    refused(lambda m: m["files"]["graph"].update(bytes=m["files"]["graph"]["bytes"] + 4),
+            match="bytes")
+    refused(lambda m: m["files"]["graph"].update(sha256="0" * 64), match="SHA-256")
+    assert bundle.validate(d, third=False) == good
+
+
+def test_check_adjacency_reports_the_node(pq):
+    graph = np.array(pq["graph"], dtype=np.int32)
+    n = graph.shape[0]
+    bundle.check_adjacency(graph, n)
+    for r, c, v, what in ((7, 2, 7, "node 7 lists itself"),
+                          (9, 3, None, "node 9 lists neighbor"),
+                          (11, 0, n, "outside"), (11, 0, -2, "outside")):
+        bad = graph.copy()
+        bad[r, c] = bad[r, 0 if c else 1] if v is None else v
+        with pytest.raises(bundle.BundleError, match=what):
+            bundle.check_adjacency(bad, n, slab_entries=64)
+    # -1 padding may repeat
+    pad = graph.copy()
+    pad[5, -3:] = -1
+    bundle.check_adjacency(pad, n)
