@@ -77,6 +77,34 @@ AEEResult calculator_retrieval_search(remote_handle64 h, const float *query,
 /* calculator.idl:56 */
 AEEResult calculator_retrieval_unload(remote_handle64 h);
 
+/* calculator.idl:59-70 (retrieval_load_dma; retrieval_async_dsp_search.c:
+ * 558-666).  The IDL's file descriptors become host pointers, copied into
+ * HBM: `graph` int32[num_vectors*graph_k], or NULL where the reference
+ * passes RETRIEVAL_DMA_NO_GRAPH_FD (-1; the graph stays with the caller:
+ * give it with rover_sq_graph_load before searching), and l1_codesLen
+ * (1..ROVER_MAX_L1_CHUNKS) chunk base pointers, chunk c holding
+ * min(l1_chunk_vectors, num_vectors - c*l1_chunk_vectors) code rows.  Both
+ * SQ loads are dropped first; an entry point outside [0, num_vectors)
+ * means none (seeds from the LCG only). */
+#define ROVER_MAX_L1_CHUNKS 16 /* retrieval_rpc_demo.h:17-18 MAX_L1_CHUNKS */
+AEEResult calculator_retrieval_load_dma(remote_handle64 h, const int32_t *graph,
+                                        const uint8_t *const *l1_codes,
+                                        int l1_codesLen, int32_t l1_chunk_vectors,
+                                        int32_t num_vectors, int32_t dim,
+                                        int32_t graph_k, int32_t graph_entry_point,
+                                        float l1_scale, float l1_zero_point);
+
+/* calculator.idl:71 (retrieval_unload_dma): drops an index loaded by
+ * retrieval_load_dma (one from retrieval_load stays loaded). */
+AEEResult calculator_retrieval_unload_dma(remote_handle64 h);
+
+/* The SQ graph for an index loaded without one (no reference op: the
+ * reference keeps that graph on the host, retrieval_rpc_demo_pipeline.c:
+ * 3066-3077, and the GPU needs it in HBM). */
+AEEResult rover_sq_graph_load(remote_handle64 h, const int32_t *graph,
+                              int64_t graphLen, int32_t num_vectors,
+                              int32_t graph_k, int32_t graph_entry_point);
+
 /* calculator.idl:119-131 (retrieval_pure_pq_load_dma): the IDL's
  * `fd_pq_codes` sequence becomes the chunk base pointers. */
 AEEResult calculator_retrieval_pure_pq_load_dma(

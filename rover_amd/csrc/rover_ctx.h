@@ -49,6 +49,7 @@ typedef struct RoverCtx {
     int32_t *d_sq_graph;
     uint8_t *d_sq_codes;
     int sq_n, sq_dim, sq_degree, sq_entry, sq_bits;
+    int sq_dma;                /* loaded by retrieval_load_dma            */
     float sq_scale, sq_zero;
     uint8_t *d_sq_res;         /* SQ residual codes and qparams          */
     float *d_sq_res_qp;

@@ -158,6 +158,7 @@ AEEResult calculator_retrieval_unload(remote_handle64 h) {
     c->sqmd_seeds_count = 0;
     c->sq_res_bits = 0;
     c->sq_loaded = 0;
+    c->sq_dma = 0;
     return ROVER_SUCCESS;
 }
 
@@ -565,6 +566,99 @@ AEEResult calculator_retrieval_load(remote_handle64 h, const int32_t *graph,
     return ROVER_SUCCESS;
 }
 
+/* calculator.idl:59-70 (retrieval_load_dma; retrieval_async_dsp_search.c:
+ * 558-666): the SQ index from chunked l1 codes.  The IDL's file descriptors
+ * become pointers: `graph` (NULL where the reference passes
+ * RETRIEVAL_DMA_NO_GRAPH_FD: the graph stays with the caller, to be given by
+ * rover_sq_graph_load before a search) and the chunk base pointers, chunk c
+ * holding min(l1_chunk_vectors, num_vectors - c * l1_chunk_vectors) rows.
+ * The data is copied into HBM.  As in the reference, both SQ loads are
+ * dropped first and an entry point outside [0, num_vectors) means none. */
+AEEResult calculator_retrieval_load_dma(remote_handle64 h, const int32_t *graph,
+                                        const uint8_t *const *l1_codes, int l1_codesLen,
+                                        int32_t l1_chunk_vectors, int32_t num_vectors,
+                                        int32_t dim, int32_t graph_k,
+                                        int32_t graph_entry_point, float l1_scale,
+                                        float l1_zero_point) {
+    RoverCtx *c = ctx_of(h);
+    size_t cb;
+    if (!c) return ROVER_EBADPARM;
+    calculator_retrieval_unload(h);
+    if (num_vectors <= 0 || dim <= 0 || dim > 4096 || graph_k <= 0 || graph_k > 64 ||
+        !l1_codes || l1_codesLen < 1 || l1_codesLen > ROVER_MAX_L1_CHUNKS ||
+        l1_chunk_vectors <= 0 ||
+        (int64_t)l1_codesLen * l1_chunk_vectors < (int64_t)num_vectors ||
+        (int64_t)(l1_codesLen - 1) * l1_chunk_vectors >= (int64_t)num_vectors ||
+        !isfinite(l1_scale) || !(l1_scale > 0.0f) || !isfinite(l1_zero_point))
+        return ROVER_EBADPARM;
+    for (int i = 0; i < l1_codesLen; i++)
+        if (!l1_codes[i]) return ROVER_EBADPARM;
+    cb = (size_t)(c->l1_bits == 4 ? (dim + 1) / 2 : dim);
+    if (rv_malloc((void **)&c->d_sq_codes, (size_t)num_vectors * cb) ||
+        (graph && dalloc_copy((void **)&c->d_sq_graph, graph,
+                              (size_t)num_vectors * graph_k * 4))) {
+        set_err("retrieval_load_dma");
+        calculator_retrieval_unload(h);
+        return ROVER_ENOMEMORY;
+    }
+    for (int i = 0; i < l1_codesLen; i++) {
+        size_t first = (size_t)i * l1_chunk_vectors;
+        size_t rows = (size_t)num_vectors - first < (size_t)l1_chunk_vectors
+                          ? (size_t)num_vectors - first
+                          : (size_t)l1_chunk_vectors;
+        if (rv_memcpy_h2d(c->d_sq_codes + first * cb, l1_codes[i], rows * cb)) {
+            set_err("retrieval_load_dma upload");
+            calculator_retrieval_unload(h);
+            return ROVER_EFAILED;
+        }
+    }
+    c->sq_n = num_vectors;
+    c->sq_dim = dim;
+    c->sq_degree = graph_k;
+    c->sq_entry = graph_entry_point >= 0 && graph_entry_point < num_vectors
+                      ? graph_entry_point : -1;
+    c->sq_bits = c->l1_bits;
+    c->sq_scale = l1_scale;
+    c->sq_zero = l1_zero_point;
+    c->sq_loaded = 1;
+    c->sq_dma = 1;
+    c->entries_count = 0;
+    return ROVER_SUCCESS;
+}
+
+/* calculator.idl:71 (retrieval_unload_dma): drops an index that
+ * retrieval_load_dma loaded; an index from retrieval_load stays, as the
+ * reference leaves a copied index to retrieval_unload. */
+AEEResult calculator_retrieval_unload_dma(remote_handle64 h) {
+    RoverCtx *c = ctx_of(h);
+    if (!c) return ROVER_EBADPARM;
+    if (c->sq_dma) calculator_retrieval_unload(h);
+    return ROVER_SUCCESS;
+}
+
+/* The SQ graph in HBM for an index loaded without one (the reference keeps
+ * that graph on the host: retrieval_rpc_demo_pipeline.c:3066-3077). */
+AEEResult rover_sq_graph_load(remote_handle64 h, const int32_t *graph, int64_t graphLen,
+                              int32_t num_vectors, int32_t graph_k,
+                              int32_t graph_entry_point) {
+    RoverCtx *c = ctx_of(h);
+    if (!c) return ROVER_EBADPARM;
+    if (!c->sq_loaded) return ROVER_EFAILED;
+    if (!graph || num_vectors != c->sq_n || graph_k <= 0 || graph_k > 64 ||
+        graphLen != (int64_t)num_vectors * graph_k || graph_entry_point >= num_vectors)
+        return ROVER_EBADPARM;
+    dfree(c->d_sq_graph);
+    c->d_sq_graph = NULL;
+    if (dalloc_copy((void **)&c->d_sq_graph, graph, (size_t)graphLen * 4)) {
+        set_err("rover_sq_graph_load");
+        return ROVER_ENOMEMORY;
+    }
+    c->sq_degree = graph_k;
+    c->sq_entry = graph_entry_point;
+    c->entries_count = 0;
+    return ROVER_SUCCESS;
+}
+
 /* The reference demo's known-answer hash of search results
  * (pure_pq_result_fingerprint, retrieval_rpc_demo_pure_pq.c:387-400):
  * FNV-1a 64 over each result as 8 little-endian bytes, the id's 4 then the
@@ -651,7 +745,7 @@ static int fill_index(const RoverCtx *c, int method, int correction,
                       int refine, RvIndex *ix) {
     memset(ix, 0, sizeof(*ix));
     if (method == ROVER_METHOD_SQ) {
-        if (!c->sq_loaded) return -1;
+        if (!c->sq_loaded || !c->d_sq_graph) return -1;
         ix->graph = c->d_sq_graph;
         ix->degree = c->sq_degree;
         ix->n = c->sq_n;
@@ -725,7 +819,7 @@ AEEResult rover_prepare_launch(RoverCtx *c, const RoverSearchBatch *b,
         }
     }
     if (fill_index(c, b->method, b->correction, b->refine_candidates, ix)) {
-        snprintf(g_err, sizeof(g_err), "search: index not loaded");
+        snprintf(g_err, sizeof(g_err), "search: index or its graph not loaded");
         return ROVER_EFAILED;
     }
     if (!explicit_entries && !b->entry_ids &&
@@ -942,7 +1036,7 @@ static AEEResult launch_sqmd(RoverCtx *c, const RoverSearchBatch *b, int max_poo
         return ROVER_EBADPARM;
     }
     if (fill_index(c, ROVER_METHOD_SQ, 0, 0, &ix)) {
-        snprintf(g_err, sizeof(g_err), "search: SQ index not loaded");
+        snprintf(g_err, sizeof(g_err), "search: SQ index or its graph not loaded");
         return ROVER_EFAILED;
     }
     if ((b->expand_count < ix.degree ? b->expand_count : ix.degree) > 32) {

@@ -118,6 +118,10 @@ SIGNATURES = [
      [_H, _P, _I, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P, _I, _P, _I,
       _P]),
     ("calculator_retrieval_unload", _I, [_H]),
+    ("calculator_retrieval_load_dma", _I,
+     [_H, _P, _P, _I, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _F, _F]),
+    ("calculator_retrieval_unload_dma", _I, [_H]),
+    ("rover_sq_graph_load", _I, [_H, _P, C.c_int64, C.c_int32, C.c_int32, C.c_int32]),
     ("calculator_retrieval_pure_pq_load_dma", _I,
      [_H, _P, _I, _P, _I, C.c_int32, _P, _I, C.c_int32, C.c_int32, C.c_int32,
       C.c_int32, C.c_int32, C.c_int32, _P, _I]),

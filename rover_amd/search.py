@@ -135,6 +135,31 @@ class Calculator:
             float(zero)), "retrieval_load")
         self.sq = dict(n=graph.shape[0], dim=dim)
 
+    def retrieval_load_dma(self, graph, l1_codes, dim, entry, scale, zero,
+                           chunk_vectors=None):
+        """calculator.idl:59-70: the SQ index from chunked codes (the IDL's
+        fds become chunk pointers); graph None is RETRIEVAL_DMA_NO_GRAPH_FD
+        (give it later with sq_graph_load)."""
+        l1_codes = np.ascontiguousarray(l1_codes, dtype=np.uint8)
+        n = l1_codes.shape[0]
+        cv = chunk_vectors or n
+        chunks = [np.ascontiguousarray(l1_codes[s:s + cv]) for s in range(0, n, cv)]
+        ptrs = (C.c_void_p * len(chunks))(*[c.ctypes.data for c in chunks])
+        g = None if graph is None else np.ascontiguousarray(graph, dtype=np.int32)
+        degree = g.shape[1] if g is not None else 1
+        check(native.lib().calculator_retrieval_load_dma(
+            self.h, ptr(g), ptrs, len(chunks), cv, n, dim, degree, int(entry),
+            float(scale), float(zero)), "retrieval_load_dma")
+        self.sq = dict(n=n, dim=dim)
+
+    def retrieval_unload_dma(self):
+        check(native.lib().calculator_retrieval_unload_dma(self.h), "retrieval_unload_dma")
+
+    def sq_graph_load(self, graph, entry):
+        g = np.ascontiguousarray(graph, dtype=np.int32)
+        check(native.lib().rover_sq_graph_load(self.h, ptr(g), g.size, g.shape[0],
+                                               g.shape[1], int(entry)), "rover_sq_graph_load")
+
     def sq_residual_load(self, res_codes, qparams, bits):
         """Per-vector SQ residuals for the corrected SQ traversal."""
         r = np.ascontiguousarray(res_codes, dtype=np.uint8)

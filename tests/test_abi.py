@@ -35,3 +35,32 @@ def test_seed_ids_host_only():
     # host-side helper, no device involved
     ids = native.make_seed_ids(1000, 4, 5, 42)
     assert ids[0] == 5 and len(ids) == 4
+
+
+REFERENCE_TREE = os.environ.get("ROVER_REFERENCE_TREE", "/root/reference")
+REF_HEADER = os.path.join(REFERENCE_TREE, "src", "dsp_demo", "retrieval_sqmd_pure_pq.h")
+
+
+def test_struct_layout_matches_reference_header(tmp_path):
+    """The library-search structs of rover_gpu.h have the layout of the
+    reference's own retrieval_sqmd_pure_pq.h (sizes, every field offset, the
+    correction-mode values), checked by compiling tests/c/layout_probe.c
+    against each header and comparing.  Needs a Rover source tree."""
+    import pytest
+    import subprocess
+    if not os.path.exists(REF_HEADER):
+        pytest.skip("no Rover source tree at %s" % REFERENCE_TREE)
+    c = os.path.join(ROOT, "tests", "c")
+    objs = []
+    for name, flags in (("ref", ["-DLAYOUT_REFERENCE", "-I", os.path.join(REFERENCE_TREE, "src"),
+                                 "-I", os.path.join(REFERENCE_TREE, "config")]),
+                        ("ours", ["-I", os.path.join(ROOT, "include")])):
+        o = str(tmp_path / (name + ".o"))
+        subprocess.run(["gcc", "-std=gnu11", "-c", os.path.join(c, "layout_probe.c"), "-o", o]
+                       + flags, check=True)
+        objs.append(o)
+    exe = str(tmp_path / "layout")
+    subprocess.run(["gcc", os.path.join(c, "layout_main.c")] + objs + ["-o", exe], check=True)
+    r = subprocess.run([exe], stdout=subprocess.PIPE, text=True)
+    assert r.returncode == 0, r.stdout
+    assert "equal" in r.stdout

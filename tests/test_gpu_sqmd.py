@@ -3,6 +3,7 @@ every case of the reference's sqmd_cpu_state_run_fake_dsp fixture, the
 HBM-frontier path at large L, per-query seed rows, the corrected traversal
 at the small SQ workload's shape against the oracle, and the error paths.
 Ids, distances and counts are compared bit-for-bit."""
+import ctypes as C
 import os
 import sys
 
@@ -228,3 +229,64 @@ def test_sqmd_small_sq_workload_vs_oracle(calc):
             assert int(st[qi, 2]) == ost["turns"]
         say("sqmd 100k correction %d: recall@10 %.4f, mean expansions %.1f"
             % (corr, data.recall_at_k(ids, cnt, gt, 10), st[:, 2].mean()))
+
+
+def test_retrieval_load_dma_equals_copy_load(fx):
+    """calculator_retrieval_load_dma (chunked codes, as the reference's
+    live SQ callers load) gives the index retrieval_load gives: the DSP
+    search op and both batch traversals return the same ids and distance
+    bits.  Without a graph (RETRIEVAL_DMA_NO_GRAPH_FD) searches fail until
+    rover_sq_graph_load supplies it; unload_dma leaves a copied index alone;
+    chunk lists that do not cover the index are refused."""
+    g, s, z, queries = fx
+    dim = queries.shape[1]
+    n = len(g["graph"])
+    entry = int(g["entry"])
+    for b in (8, 4):
+        say("case retrieval_load_dma sq%d" % b)
+        codes = s["sq%d_codes" % b]
+        sc, zr = s["sq%d_scale" % b], s["sq%d_zero" % b]
+        with Calculator() as a, Calculator() as c:
+            for x in (a, c):
+                x.rover_configure(l1_bits=b)
+            a.retrieval_load(g["graph"], codes, dim, entry, sc, zr)
+            c.retrieval_load_dma(g["graph"], codes, dim, entry, sc, zr, chunk_vectors=300)
+
+            def same(search):
+                r1, r2 = search(a), search(c)
+                for u, v in zip(r1, r2):
+                    assert np.array_equal(np.asarray(u).view(np.uint32),
+                                          np.asarray(v).view(np.uint32))
+
+            for qi in range(4):
+                same(lambda x: x.retrieval_search(queries[qi], 8, 32, 48, 10))
+            for trav in (SQMD, native.TRAVERSAL_BEST_FIRST):
+                same(lambda x: x.search_batch(queries, 64, 10, method=native.METHOD_SQ,
+                                              entry_points=32, expand_count=32,
+                                              traversal=trav))
+            # no graph: refused until one is given
+            c.retrieval_load_dma(None, codes, dim, entry, sc, zr, chunk_vectors=512)
+            with pytest.raises(native.RoverError):
+                c.search_batch(queries[:2], 64, 10, method=native.METHOD_SQ,
+                               entry_points=32, expand_count=32, traversal=SQMD)
+            c.sq_graph_load(g["graph"], entry)
+            same(lambda x: x.search_batch(queries, 64, 10, method=native.METHOD_SQ,
+                                          entry_points=32, expand_count=32, traversal=SQMD))
+            # unload_dma drops a dma-loaded index, not a copied one
+            a.retrieval_unload_dma()
+            c.retrieval_unload_dma()
+            a.search_batch(queries[:2], 64, 10, method=native.METHOD_SQ, entry_points=8,
+                           expand_count=32, traversal=SQMD)
+            with pytest.raises(native.RoverError):
+                c.search_batch(queries[:2], 64, 10, method=native.METHOD_SQ, entry_points=8,
+                               expand_count=32, traversal=SQMD)
+            # chunk lists: more than 16 chunks, or too few rows, are refused
+            with pytest.raises(native.RoverError) as e:
+                c.retrieval_load_dma(g["graph"], codes, dim, entry, sc, zr,
+                                     chunk_vectors=max(1, n // 17))
+            assert e.value.code == native.ROVER_EBADPARM
+            ptrs = (C.c_void_p * 1)(codes.ctypes.data)
+            rc = native.lib().calculator_retrieval_load_dma(
+                c.h, g["graph"].ctypes.data, ptrs, 1, n - 1, n, dim, g["graph"].shape[1],
+                entry, float(sc), float(zr))
+            assert rc == native.ROVER_EBADPARM
