@@ -22,14 +22,18 @@ ground truth; the plain search uses the faster mode.  The learned stop
 predictor is trained on traces of that mode's plain search of 200,000
 training queries at L=512, once per label slack (label 1 once all but
 `slack` of the final top-10 are found), each swept the same way; the slack
-reaching the target fastest is reported.  `host_buffers_qps` is the plain
-search through the host-buffer entry point (query upload and result
-download over PCIe included).
+reaching the target fastest is reported.  The L grid is refined by
+bisection to the smallest L reaching the target.  `value` and
+`ms_per_step` are end to end: each step uploads the queries and entry rows
+from page-locked host memory, searches and downloads the results
+(rover_search_batch); `config.device_resident` has the same searches with
+everything already in HBM, whose kernel time the roofline uses.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
 import ctypes as C
+import gc
 import json
 import os
 import sys
@@ -40,7 +44,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from rover_amd import artifacts, data, index, native  # noqa: E402
+from rover_amd import artifacts, data, index, metrics, native  # noqa: E402
 from rover_amd.search import Calculator  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
@@ -64,6 +68,91 @@ def algorithmic_bytes(stats, R, code_bytes, res_bytes, dim, k, refine_F,
         b += (res_bytes + 8) * Cq
     b += (code_bytes + res_bytes + 8) * refine_F + 4 * dim + 8 * k
     return float(b.sum() + codebook_bytes)
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or the platform string."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or platform.machine()
+
+
+def reference_leg(pyoracle, ix, base, queries, ents, gids, gd, L, k, args, threads):
+    """The reference's own C search (retrieval_pure_pq_search_scalar_with_
+    policy with its residual evaluator, compiled from its sources) over a
+    bounded sample of the queries on `threads` host threads, each with its
+    own residual reader; the share of queries whose ids and distance bits
+    equal the GPU's, and whether the reference's residual encoder reproduces
+    the index's residual codes."""
+    import threading
+    ref = pyoracle.reference()
+    n_base = ix.graph.shape[0]
+    res_r = np.zeros_like(ix.res_codes)
+    qp_r = np.zeros_like(ix.res_qparams)
+    xb = np.ascontiguousarray(base, dtype=np.float32)
+    codes = np.ascontiguousarray(ix.codes)
+    cb = np.ascontiguousarray(ix.codebook, dtype=np.float32)
+    graph = np.ascontiguousarray(ix.graph, dtype=np.int32)
+    h = ref.ref_residual_open(xb.ctypes.data, codes.ctypes.data, cb.ctypes.data, n_base,
+                              args.dim, args.m, 256, ix.res_bits, res_r.ctypes.data,
+                              qp_r.ctypes.data)
+    if not h:
+        return None
+    forks = []
+    try:
+        enc_same = bool(np.array_equal(res_r, ix.res_codes) and
+                        np.array_equal(qp_r.view(np.uint32), ix.res_qparams.view(np.uint32)))
+        forks = [ref.ref_residual_fork(h) for _ in range(threads)]
+        if not all(forks):
+            return None
+        nq = queries.shape[0]
+        done = [0] * threads
+        same = [0] * threads
+        deadline = time.time() + args.cpu_seconds
+
+        def work(t):
+            ids_r = np.zeros(k, dtype=np.int32)
+            d_r = np.zeros(k, dtype=np.float32)
+            st_r = np.zeros(4, dtype=np.uint64)
+            for qi in range(t, nq, threads):
+                if time.time() >= deadline:
+                    break
+                q = np.ascontiguousarray(queries[qi], dtype=np.float32)
+                e = np.ascontiguousarray(ents[qi], dtype=np.int32)
+                rc = ref.ref_search(graph.ctypes.data, graph.shape[1], n_base,
+                                    codes.ctypes.data, 16384, cb.ctypes.data, args.dim, args.m,
+                                    256, q.ctypes.data, e.ctypes.data, len(e), args.R, L, k, 1,
+                                    forks[t], None, 8, C.c_float(1.0), C.c_float(0.0),
+                                    ids_r.ctypes.data, d_r.ctypes.data, k, st_r.ctypes.data)
+                same[t] += int(rc == k and np.array_equal(ids_r, gids[qi]) and
+                               np.array_equal(d_r.view(np.uint32), gd[qi].view(np.uint32)))
+                done[t] += 1
+
+        t1 = time.time()
+        th = [threading.Thread(target=work, args=(t,)) for t in range(threads)]
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+        el = time.time() - t1
+        total = sum(done)
+        return {"value": round(total / el, 1), "unit": "queries/s", "cores": threads,
+                "kind": "reference",
+                "sample": "%d of the queries (query i on thread i mod %d), plain online "
+                          "search at L=%d (%.1f s)" % (total, threads, L, el),
+                "gpu_results_identical_frac": sum(same) / max(total, 1),
+                "residual_encoder_identical": enc_same}
+    finally:
+        for f in forks:
+            if f:
+                ref.ref_residual_close(f)
+        ref.ref_residual_close(h)
 
 
 def batching_leg(calc, queries, ents, S, mode, refine_of, args, cap=4096, kmax=100,
@@ -373,6 +462,12 @@ def main():
                     help="refine leg: refine the top factor*k of the pool, with and "
                          "without the error4 level (0 skips the leg)")
     ap.add_argument("--target", type=float, default=0.95)
+    ap.add_argument("--no-bisect", dest="bisect", action="store_false",
+                    help="take the smallest L of the --Ls grid reaching the target, "
+                         "without the bisection between grid points")
+    ap.add_argument("--metrics-csv", default="",
+                    help="also write the reference demo's metrics.csv rows "
+                         "(retrieval_rpc_demo_pure_pq.c:453-462, 704-728) here")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
@@ -503,8 +598,13 @@ def main():
 
     tuned = {}  # (mode, L, learned) -> (hash_log2, probe_limit, grid_waves)
 
-    def run(mode, L, stats=False, learned=False, refine=None, refine_mode=0):
+    def run(mode, L, stats=False, learned=False, refine=None, refine_mode=0, exact_hash=False):
         h2, pl, gw = tuned.get((mode, L, learned), (0, 0, 0))
+        if exact_hash:
+            # a visited hash of 2^14 slots probed up to 64 times drops no
+            # entry at these sizes, so the counters are the distinct nodes
+            # scored and expanded (what the oracle's stats count)
+            h2, pl = 14, 64
         calc.search_batch_device(dq.ptr, nq, L, k, d_ids.ptr, d_d.ptr, d_cnt.ptr, k,
                                  correction=1 if mode == "online" else 0,
                                  refine_candidates=refine_of(mode, L) if refine is None
@@ -530,31 +630,50 @@ def main():
         log("autotuned %s%s L=%d: %s" % (mode, "+learned" if learned else "", L, info))
         return info
 
+    def probe(mode, L, learned, rows, bisect=False):
+        run(mode, L, stats=True, learned=learned)
+        native.synchronize()
+        ids = d_ids.download(np.int32, nq * k).reshape(nq, k)
+        cnt = d_cnt.download(np.int32, nq)
+        st = d_st.download(np.uint64, nq * 4).reshape(nq, 4)
+        assert (cnt == k).all(), "search failed: %s" % np.unique(cnt)
+        rec = data.recall_at_k(ids, cnt, gt, k)
+        e0, e1 = native.Event(), native.Event()
+        e0.record()
+        run(mode, L, learned=learned)
+        e1.record()
+        ms = e0.elapsed_ms(e1)
+        rows.append(dict(L=L, recall10=round(rec, 4), qps=round(nq / (ms / 1e3), 1),
+                         mean_expanded=round(float(st[:, 2].mean()), 2),
+                         mean_scored=round(float(st[:, 0].mean()), 2),
+                         **({"bisect": True} if bisect else {})))
+        log("%s%s L=%d recall@10=%.4f qps=%.0f expanded=%.1f scored=%.1f%s" %
+            (mode, "+learned" if learned else "", L, rec, nq / (ms / 1e3),
+             st[:, 2].mean(), st[:, 0].mean(), " (bisection)" if bisect else ""))
+        return dict(L=L, stats=st.copy(), qps=nq / (ms / 1e3), recall=rec)
+
     def sweep(mode, learned):
-        rows, chosen = [], None
+        """The grid of --Ls up to the first L reaching the target, then a
+        bisection over the integers between the last grid L below the
+        target and that one: the smallest L whose recall@10 reaches it
+        (recall grows with L)."""
+        rows, chosen, lo = [], None, max(k, 1) - 1
         log("sweep %s%s over L = %s" % (mode, "+learned" if learned else "", args.Ls))
         for L in sweep_Ls:
-            run(mode, L, stats=True, learned=learned)
-            native.synchronize()
-            ids = d_ids.download(np.int32, nq * k).reshape(nq, k)
-            cnt = d_cnt.download(np.int32, nq)
-            st = d_st.download(np.uint64, nq * 4).reshape(nq, 4)
-            assert (cnt == k).all(), "search failed: %s" % np.unique(cnt)
-            rec = data.recall_at_k(ids, cnt, gt, k)
-            e0, e1 = native.Event(), native.Event()
-            e0.record()
-            run(mode, L, learned=learned)
-            e1.record()
-            ms = e0.elapsed_ms(e1)
-            rows.append(dict(L=L, recall10=round(rec, 4), qps=round(nq / (ms / 1e3), 1),
-                             mean_expanded=round(float(st[:, 2].mean()), 2),
-                             mean_scored=round(float(st[:, 0].mean()), 2)))
-            log("%s%s L=%d recall@10=%.4f qps=%.0f expanded=%.1f scored=%.1f" %
-                (mode, "+learned" if learned else "", L, rec, nq / (ms / 1e3),
-                 st[:, 2].mean(), st[:, 0].mean()))
-            if chosen is None and rec >= args.target:
-                chosen = dict(L=L, stats=st.copy(), qps=nq / (ms / 1e3), recall=rec)
+            r = probe(mode, L, learned, rows)
+            if r["recall"] >= args.target:
+                chosen = r
                 break
+            lo = L
+        if chosen is not None and args.bisect:
+            hi = chosen["L"]
+            while hi - lo > 1:
+                mid = (lo + hi) // 2
+                r = probe(mode, mid, learned, rows, bisect=True)
+                if r["recall"] >= args.target:
+                    hi, chosen = mid, r
+                else:
+                    lo = mid
         return rows, chosen
 
     def timed(mode, L, learned):
@@ -568,6 +687,57 @@ def main():
             ev[i + 1].record()
         native.synchronize()
         return sum(ev[i].elapsed_ms(ev[i + 1]) for i in range(args.steps)) / args.steps
+
+    # end-to-end timing through the host-buffer entry point: page-locked
+    # query, entry-row and result buffers; each step uploads the queries and
+    # entry rows, searches and downloads the results (rover_search_batch
+    # returns when the results are in host memory)
+    pin_q = native.PinnedArray(queries.shape, np.float32)
+    pin_q.array[:] = queries
+    pin_e = native.PinnedArray(ents.shape, np.int32)
+    pin_e.array[:] = ents
+    pin_ids = native.PinnedArray((nq, k), np.int32)
+    pin_d = native.PinnedArray((nq, k), np.float32)
+    pin_cnt = native.PinnedArray((nq,), np.int32)
+
+    def timed_host(configs):
+        """End-to-end steps of each (mode, L, learned) configuration, the
+        configurations interleaved step by step so drift on the host or the
+        device affects them alike.  Returns per configuration the mean ms
+        per step and whether the results equal the device-resident run's."""
+        structs = []
+        for mode, L, learned in configs:
+            h2, pl, gw = tuned.get((mode, L, learned), (0, 0, 0))
+            structs.append(calc._batch_struct(
+                pin_q.ptr, nq, L, k, native.METHOD_PQ, 1 if mode == "online" else 0,
+                refine_of(mode, L), 1, args.R, pin_ids.ptr, pin_d.ptr, pin_cnt.ptr, k, None,
+                h2, pl, gw, 1 if learned else 0, 0.9, None, None, native.ENTRY_POINT_SEED,
+                pin_e.ptr, S, S))
+        times = [[] for _ in configs]
+        gc.collect()
+        gc.disable()  # no collector pauses inside the timed calls
+        for step in range(args.warmup + args.steps):
+            for i, b in enumerate(structs):
+                t0 = time.perf_counter()
+                native.check(native.lib().rover_search_batch(calc.h, C.byref(b)),
+                             "rover_search_batch")
+                if step >= args.warmup:
+                    times[i].append(time.perf_counter() - t0)
+        gc.enable()
+        out = []
+        for (mode, L, learned), b, t in zip(configs, structs, times):
+            native.check(native.lib().rover_search_batch(calc.h, C.byref(b)),
+                         "rover_search_batch")
+            host_ids, host_d = pin_ids.array.ravel().copy(), pin_d.array.ravel().copy()
+            run(mode, L, learned=learned)
+            native.synchronize()
+            same = bool(np.array_equal(host_ids, d_ids.download(np.int32, nq * k)) and
+                        np.array_equal(host_d.view(np.uint32),
+                                       d_d.download(np.uint32, nq * k)))
+            log("end to end %s%s L=%d: steps %s ms" % (mode, "+learned" if learned else "", L,
+                                                       " ".join("%.3f" % (1e3 * x) for x in t)))
+            out.append((1e3 * sum(t) / len(t), same, 1e3 * float(np.median(t))))
+        return out
 
     # ---- plain search: best mode at the target recall
     plain = {}
@@ -680,37 +850,50 @@ def main():
         learned = dict(sweep=learned["sweep"], chosen=learned["chosen"],
                        training=learned["training"], other_slacks=others)
 
-    # ---- timed region: K steps of the full batch, queries resident in HBM
+    # ---- timed region: K steps of the full batch.  The headline is end to
+    # end (queries and entry rows uploaded, results downloaded); the
+    # device-resident kernel time is measured too and sets the roofline.
     log("timing plain %s at L=%d: %d warmup + %d steps" % (mode, L_plain, args.warmup,
                                                             args.steps))
     tuning = {"plain": autotune(mode, L_plain, False) if not args.no_autotune else None}
-    ms_plain = timed(mode, L_plain, False)
-    qps_plain = nq / (ms_plain / 1e3)
-    log("timed plain %s: L=%d %.3f ms/step %.0f QPS" % (mode, L_plain, ms_plain, qps_plain))
+    ms_plain_dev = timed(mode, L_plain, False)
     if learned:
         L_l = learned["chosen"]["L"]
-        log("timing learned %s at L=%d" % (mode, L_l))
         tuning["learned"] = autotune(mode, L_l, True) if not args.no_autotune else None
-        ms_l = timed(mode, L_l, True)
-        qps_l = nq / (ms_l / 1e3)
-        log("timed learned %s: L=%d %.3f ms/step %.0f QPS (%.2fx plain)" %
-            (mode, L_l, ms_l, qps_l, qps_l / qps_plain))
-        ms_per_step, qps, stats, L, label = ms_l, qps_l, learned["chosen"]["stats"], L_l, "learned-stop"
+        ms_l_dev = timed(mode, L_l, True)
+        (ms_plain, same_plain, med_plain), (ms_l, same_l, med_l) = timed_host(
+            [(mode, L_plain, False), (mode, L_l, True)])
     else:
-        ms_per_step, qps, stats, L, label = ms_plain, qps_plain, plain[mode]["chosen"]["stats"], L_plain, "plain"
-
-    # plain search through the host-buffer entry point: queries and entry rows
-    # uploaded, results downloaded, each call synchronous
-    calc.search_batch(queries, L_plain, k, correction=1 if mode == "online" else 0,
-                      refine_candidates=refine_of(mode, L_plain), expand_count=args.R,
-                      entry_ids=ents)
-    t1 = time.time()
-    for _ in range(args.steps):
-        calc.search_batch(queries, L_plain, k, correction=1 if mode == "online" else 0,
-                          refine_candidates=refine_of(mode, L_plain), expand_count=args.R,
-                          entry_ids=ents)
-    host_qps = nq * args.steps / (time.time() - t1)
-    log("plain search from host buffers (PCIe included): %.0f QPS" % host_qps)
+        ((ms_plain, same_plain, med_plain),) = timed_host([(mode, L_plain, False)])
+    qps_plain = nq / (ms_plain / 1e3)
+    log("timed plain %s: L=%d end to end %.3f ms/step %.0f QPS; device-resident %.3f ms"
+        % (mode, L_plain, ms_plain, qps_plain, ms_plain_dev))
+    device_resident = {"plain": dict(L=L_plain, ms_per_step=round(ms_plain_dev, 4),
+                                     qps=round(nq / (ms_plain_dev / 1e3), 1))}
+    median_ms = {"plain": round(med_plain, 4)}
+    host_same = {"plain": same_plain}
+    if learned:
+        host_same["learned"] = same_l
+        median_ms["learned"] = round(med_l, 4)
+        qps_l = nq / (ms_l / 1e3)
+        device_resident["learned"] = dict(L=L_l, ms_per_step=round(ms_l_dev, 4),
+                                          qps=round(nq / (ms_l_dev / 1e3), 1),
+                                          speedup_vs_plain=round(ms_plain_dev / ms_l_dev, 3))
+        log("timed learned %s: L=%d end to end %.3f ms/step %.0f QPS (%.2fx plain); "
+            "device-resident %.3f ms" % (mode, L_l, ms_l, qps_l, qps_l / qps_plain, ms_l_dev))
+        ms_per_step, qps, L, label, kernel_ms = ms_l, qps_l, L_l, "learned-stop", ms_l_dev
+    else:
+        ms_per_step, qps, L, label, kernel_ms = ms_plain, qps_plain, L_plain, "plain", ms_plain_dev
+    host_qps = qps_plain
+    # counters for the algorithmic bytes: the timed launch's own counters
+    # include nodes re-scored after the tuned (small) visited hash dropped
+    # them; a near-exact hash gives the distinct counts (same ids)
+    run(mode, L, stats=True, learned=label == "learned-stop", exact_hash=True)
+    native.synchronize()
+    stats = d_st.download(np.uint64, nq * 4).reshape(nq, 4)
+    run(mode, L_plain, stats=True, exact_hash=True)
+    native.synchronize()
+    plain_exact_stats = d_st.download(np.uint64, nq * 4).reshape(nq, 4)
 
     # ---- routed leg: the same plain / learned comparison with per-query entry
     # rows from the k-means router (representatives of the centroids nearest
@@ -807,7 +990,7 @@ def main():
     res_bytes = index.residual_code_bytes(args.dim, args.res_bits)
     algo = algorithmic_bytes(stats, args.R, code_bytes, res_bytes, args.dim, k,
                              refine_of(mode, L), ix.codebook.nbytes, mode == "online")
-    achieved = algo / (ms_per_step / 1e3) / 1e9
+    achieved = algo / (kernel_ms / 1e3) / 1e9
     # online scoring on 32 x 4-dim PQ with <= 32 neighbors runs the two-lane
     # instantiation (scorer 3); ADC ranking is scorer 0
     scorer = (3 if args.m * 4 == args.dim and args.m == 32 and args.R <= 32 else 1) \
@@ -816,7 +999,10 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                 "traffic": None, "kernel": kernel,
-                "algorithmic_bytes_per_launch": int(algo)}
+                "algorithmic_bytes_per_launch": int(algo),
+                "kernel_ms": round(kernel_ms, 4),
+                "counters": "distinct scored / expanded nodes (near-exact visited hash); "
+                            "the plain search's equal the oracle's PurePqSearchStats"}
     # HBM bytes per launch come from the profiler's FETCH_SIZE / WRITE_SIZE
     # passes over this same workload (calibrated against a copy kernel),
     # recorded next to the profile summary; used when kernel and L match
@@ -830,22 +1016,28 @@ def main():
 
     cpu = None
     reference_cpu = None
+    fingerprint = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle
         oix = pyoracle.Index.from_pq(ix)
         scorer = pyoracle.SCORE_ONLINE if mode == "online" else pyoracle.SCORE_ADC
-        threads = min(16, os.cpu_count() or 1)
-        # the same per-query entry rows the GPU search takes
-        log("cpu baseline: the oracle on %d host threads" % threads)
+        # every host thread this process may run on, up to the 16 a
+        # one-GPU share of the host provides
+        avail = (len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
+                 else (os.cpu_count() or 1))
+        threads = max(1, min(16, avail))
+        host = dict(cpu_model=cpu_model(), threads_used=threads, threads_available=avail,
+                    cpus_visible=os.cpu_count())
+        log("cpu baseline: the oracle on %d host threads (%s)" % (threads, host["cpu_model"]))
         # whole passes over the query set until about cpu_seconds of work
         passes, searched = 0, 0
         t1 = time.time()
         while True:
-            oids, od, ocnt, _ = pyoracle.search_batch(oix, scorer, queries, ents, args.R,
-                                                      L_plain, k,
-                                                      refine=refine_of(mode, L_plain),
-                                                      threads=threads)
+            oids, od, ocnt, ost = pyoracle.search_batch(oix, scorer, queries, ents, args.R,
+                                                        L_plain, k,
+                                                        refine=refine_of(mode, L_plain),
+                                                        threads=threads)
             passes += 1
             searched += nq
             el = time.time() - t1
@@ -859,65 +1051,106 @@ def main():
         gd = d_d.download(np.float32, nq * k).reshape(nq, k)
         same = float(((gids == oids).all(axis=1) &
                       (gd.view(np.uint32) == od.view(np.uint32)).all(axis=1)).mean())
+        # the plain search's distinct scored / corrected / expanded / dedup
+        # counters equal the oracle's PurePqSearchStats
+        stats_same = float((plain_exact_stats.astype(np.int64) ==
+                            ost.astype(np.int64)).all(axis=1).mean())
         cpu = {"value": round(searched / el, 1), "unit": "queries/s", "cores": threads,
                "kind": "port",
                "sample": "%d passes over the %d queries (%d searches), plain %s search "
                          "at L=%d, one query per host thread (%.1f s)"
                          % (passes, nq, searched, mode, L_plain, el),
-               "gpu_results_identical_frac": same}
-        log("cpu baseline: %.0f QPS on %d threads; GPU ids+distances identical on %.4f"
-            % (searched / el, threads, same))
+               "host": host,
+               "gpu_results_identical_frac": same,
+               "gpu_stats_identical_frac": stats_same}
+        log("cpu baseline: %.0f QPS on %d threads; GPU ids+distances identical on %.4f, "
+            "counters on %.4f" % (searched / el, threads, same, stats_same))
+
+        # the reference demo's result fingerprint (rpc_demo_pure_pq.c:387-400,
+        # 556, 614): FNV-1a 64 over every query's whole sorted pool, plain
+        # search, on the GPU (the library's fingerprint) and in the oracle
+        fids = native.DeviceBuffer(nq * L_plain * 4)
+        fd = native.DeviceBuffer(nq * L_plain * 4)
+        fcnt = native.DeviceBuffer(nq * 4)
+        calc.search_batch_device(dq.ptr, nq, L_plain, L_plain, fids.ptr, fd.ptr, fcnt.ptr,
+                                 L_plain, correction=1 if mode == "online" else 0,
+                                 expand_count=args.R, d_entry_ids=d_ent.ptr, entry_count=S,
+                                 entry_stride=S)
+        native.synchronize()
+        g_fp = native.results_fingerprint(fids.download(np.int32, nq * L_plain).reshape(nq, -1),
+                                          fd.download(np.float32, nq * L_plain).reshape(nq, -1),
+                                          fcnt.download(np.int32, nq))
+        fi, fdd, fc, _ = pyoracle.search_batch(oix, scorer, queries, ents, args.R, L_plain,
+                                               L_plain, threads=threads)
+        o_fp = pyoracle.fingerprint(fi, fdd, fc)
+        fingerprint = {"search_pool_fingerprint": "%016x" % g_fp,
+                       "oracle": "%016x" % o_fp, "equal": g_fp == o_fp,
+                       "what": "plain %s search, L=%d, %d queries, whole sorted pools"
+                               % (mode, L_plain, nq)}
+        log("result fingerprint: GPU %016x oracle %016x" % (g_fp, o_fp))
+        fids.close()
+        fd.close()
+        fcnt.close()
 
         # the reference's own C search (retrieval_pure_pq_search_scalar_with_
         # policy with its residual evaluator, compiled from its sources) on
-        # one host thread over a bounded sample of the same queries
+        # the same host threads, one residual reader per thread, over a
+        # bounded sample of the same queries
         if pyoracle.ref_available() and base is not None and mode == "online":
-            ref = pyoracle.reference()
-            n_base = ix.graph.shape[0]
-            res_r = np.zeros_like(ix.res_codes)
-            qp_r = np.zeros_like(ix.res_qparams)
-            xb = np.ascontiguousarray(base, dtype=np.float32)
-            h = ref.ref_residual_open(xb.ctypes.data, ix.codes.ctypes.data,
-                                      ix.codebook.ctypes.data, n_base, args.dim, args.m, 256,
-                                      ix.res_bits, res_r.ctypes.data, qp_r.ctypes.data)
-            if h:
-                try:
-                    enc_same = bool(np.array_equal(res_r, ix.res_codes) and
-                                    np.array_equal(qp_r.view(np.uint32),
-                                                   ix.res_qparams.view(np.uint32)))
-                    graph = np.ascontiguousarray(ix.graph, dtype=np.int32)
-                    codes = np.ascontiguousarray(ix.codes)
-                    cb = np.ascontiguousarray(ix.codebook, dtype=np.float32)
-                    ids_r = np.zeros(k, dtype=np.int32)
-                    d_r = np.zeros(k, dtype=np.float32)
-                    st_r = np.zeros(4, dtype=np.uint64)
-                    done, same_r = 0, 0
-                    t1 = time.time()
-                    while done < nq and time.time() - t1 < args.cpu_seconds:
-                        q = np.ascontiguousarray(queries[done], dtype=np.float32)
-                        e = np.ascontiguousarray(ents[done], dtype=np.int32)
-                        rc = ref.ref_search(graph.ctypes.data, graph.shape[1], n_base,
-                                            codes.ctypes.data, 16384, cb.ctypes.data,
-                                            args.dim, args.m, 256, q.ctypes.data,
-                                            e.ctypes.data, len(e), args.R, L_plain, k, 1, h,
-                                            None, 8, C.c_float(1.0), C.c_float(0.0),
-                                            ids_r.ctypes.data, d_r.ctypes.data, k,
-                                            st_r.ctypes.data)
-                        same_r += int(rc == k and np.array_equal(ids_r, gids[done]) and
-                                      np.array_equal(d_r.view(np.uint32),
-                                                     gd[done].view(np.uint32)))
-                        done += 1
-                    el_r = time.time() - t1
-                    reference_cpu = {
-                        "value": round(done / el_r, 1), "unit": "queries/s", "cores": 1,
-                        "kind": "reference",
-                        "sample": "the first %d queries, plain %s search at L=%d, one host "
-                                  "thread (%.1f s)" % (done, mode, L_plain, el_r),
-                        "gpu_results_identical_frac": same_r / max(done, 1),
-                        "residual_encoder_identical": enc_same}
-                    log("reference cpu: %s" % reference_cpu)
-                finally:
-                    ref.ref_residual_close(h)
+            reference_cpu = reference_leg(pyoracle, ix, base, queries, ents, gids, gd,
+                                          L_plain, k, args, threads)
+            if reference_cpu:
+                reference_cpu["host"] = host
+                log("reference cpu: %s" % reference_cpu)
+
+    if args.metrics_csv:
+        # one row per timed setting, as the reference demo writes them
+        lines = [metrics.header()]
+        settings = [("plain", L_plain, False, ms_plain)]
+        if label == "learned-stop":
+            settings.append(("learned-stop", L, True, ms_per_step))
+        for name, Lx, lrn, ms in settings:
+            kk = min(100, Lx)
+            mids = native.DeviceBuffer(nq * Lx * 4)
+            md = native.DeviceBuffer(nq * Lx * 4)
+            mcnt = native.DeviceBuffer(nq * 4)
+            mst = native.DeviceBuffer(nq * 32)
+            calc.search_batch_device(dq.ptr, nq, Lx, Lx, mids.ptr, md.ptr, mcnt.ptr, Lx,
+                                     correction=1 if mode == "online" else 0,
+                                     expand_count=args.R, d_stats=mst.ptr,
+                                     learned_stop=1 if lrn else 0, stop_threshold=0.9,
+                                     d_entry_ids=d_ent.ptr, entry_count=S, entry_stride=S,
+                                     hash_log2=14, probe_limit=64)
+            native.synchronize()
+            mi = mids.download(np.int32, nq * Lx).reshape(nq, Lx)
+            mdd = md.download(np.float32, nq * Lx).reshape(nq, Lx)
+            mc = mcnt.download(np.int32, nq)
+            ms_ = mst.download(np.uint64, nq * 4).reshape(nq, 4)
+            total_us = int(round(ms * 1e3))
+            lines.append(metrics.row(
+                dataset="mixture-%dx%d" % (len(ix.graph), args.dim), layout="original",
+                code_bits=8, ks=256, m=args.m, code_bytes=index.code_bytes(args.m, 256),
+                compression_x=args.dim * 4 // index.code_bytes(args.m, 256),
+                entry_points=S, pool_size=Lx, queries=nq,
+                recall_at_1=metrics.recall_percent(mi, mc, gt, 1),
+                recall_at_10=metrics.recall_percent(mi, mc, gt, 10),
+                recall_at_100=metrics.recall_percent(mi, mc, gt, kk),
+                wall_us=total_us // nq, work_items=int(ms_[:, 0].sum()) // nq, dsp_adc_us=0,
+                dsp_wait_polls=0, correction_mode="online" if mode == "online" else "off",
+                residual_bits=args.res_bits if mode == "online" else 0,
+                corrected_candidates=int(ms_[:, 1].sum()), residual_cache_hits=0,
+                residual_cache_misses=0, residual_cache_errors=0,
+                artifact_prefix=name, window=1, source_jobs=int(ms_[:, 2].sum()),
+                source_retired=int(ms_[:, 2].sum()), source_high_water=1, top_k=k,
+                refine_mode=0, refine_factor=0, refine_budget=0, refine_candidates=0,
+                refine_us=0, refine_blocks=0, refine_bytes=0, refine_hits=0,
+                search_pool_fingerprint=native.results_fingerprint(mi, mdd, mc),
+                total_wall_us=total_us))
+            for b_ in (mids, md, mcnt, mst):
+                b_.close()
+        with open(args.metrics_csv, "w") as f:
+            f.write("".join(lines))
+        log("metrics.csv rows written to", args.metrics_csv)
 
     def brief(v):
         if not v:
@@ -953,14 +1186,21 @@ def main():
                         "centroids, passed as entry_ids rows" % (S, len(ix.router_ids))
                         if args.entry == "routed" else
                         "the reference's seeds: medoid + %d LCG ids" % (S - 1)),
+            "timing": "end to end per step: page-locked host queries and entry rows "
+                      "uploaded, search, results downloaded (rover_search_batch); "
+                      "device_resident: the same search with everything in HBM, "
+                      "HIP events on the launch stream (the roofline's time)",
             "plain_qps": round(qps_plain, 1), "plain_L": L_plain,
             "plain_ms_per_step": round(ms_plain, 4),
             "learned_speedup_vs_plain": round(qps / qps_plain, 3) if learned else None,
+            "device_resident": device_resident,
+            "host_results_identical": host_same,
+            "end_to_end_median_ms": median_ms,
+            "result_fingerprint": fingerprint,
             "plain_modes": {m: brief(v) for m, v in plain.items()},
             "learned": None if not learned else dict(
                 brief(learned), training=learned["training"],
                 other_slacks=learned["other_slacks"]),
-            "host_buffers_qps": round(host_qps, 1),
             "refine": refine_leg,
             "sq": sq_leg,
             "reference_cpu": reference_cpu,

@@ -571,6 +571,10 @@ AEEResult rover_build_graph_rnn(const float *d_x, int n, int dim, int s, int r,
 /* Device memory helpers, so callers need no HIP headers. */
 AEEResult rover_device_alloc(void **d_ptr, size_t bytes);
 AEEResult rover_device_free(void *d_ptr);
+/* Page-locked host memory: query and result buffers of rover_search_batch
+ * in it are copied by DMA, overlapping nothing but skipping HIP's staging. */
+AEEResult rover_host_alloc(void **ptr, size_t bytes);
+AEEResult rover_host_free(void *ptr);
 AEEResult rover_device_upload(void *d_dst, const void *src, size_t bytes);
 AEEResult rover_device_download(void *dst, const void *d_src, size_t bytes);
 AEEResult rover_device_synchronize(void);

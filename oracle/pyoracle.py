@@ -107,6 +107,8 @@ def reference():
         L.ref_residual_open.argtypes = [P, P, P, I, I, I, I, I, P, P]
         L.ref_residual_open.restype = P
         L.ref_residual_close.argtypes = [P]
+        L.ref_residual_fork.argtypes = [P]
+        L.ref_residual_fork.restype = P
         L.ref_residual_evaluate.argtypes = [P, P, I, P]
         L.ref_residual_decode.argtypes = [P, I, P]
         L.ref_search.argtypes = [P, I, I, P, I, P, I, I, I, P, P, I, I, I, I, I,

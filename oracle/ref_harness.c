@@ -99,6 +99,7 @@ typedef struct RefResidual {
     RetrievalPqResidualQuery *query;
     const uint8_t *chunks[1];
     int counts[1];
+    const float *codebook;
     /* error4 artifact and the files it is bound to (ref_error4_build) */
     int res_bits;
     int e4_ready;
@@ -160,6 +161,7 @@ void *ref_residual_open(const float *x, const uint8_t *codes,
     fclose(f);
     r->chunks[0] = codes;
     r->counts[0] = n;
+    r->codebook = codebook;
     memset(&req, 0, sizeof(req));
     req.artifact_prefix = r->prefix;
     req.residual_bits = bits;
@@ -179,10 +181,49 @@ fail:
     return NULL;
 }
 
+/* A second reader of the artifacts `handle` wrote, with its own residual
+ * index and query (the reference's reader keeps per-query state and a block
+ * cache, so concurrent searches need one each).  Close forks before the
+ * handle that owns the files. */
+void *ref_residual_fork(void *handle) {
+    RefResidual *r = handle, *f;
+    RetrievalPqResidualOpenRequest req;
+    if (!r || !(f = calloc(1, sizeof(*f)))) return NULL;
+    memcpy(f->prefix, r->prefix, sizeof(f->prefix));
+    f->dir[0] = 0; /* owns no files */
+    f->meta = r->meta;
+    f->res_bits = r->res_bits;
+    f->chunks[0] = r->chunks[0];
+    f->counts[0] = r->counts[0];
+    memset(&req, 0, sizeof(req));
+    req.artifact_prefix = f->prefix;
+    req.residual_bits = f->res_bits;
+    req.cache_slots = 64;
+    req.pq_metadata = &f->meta;
+    req.codebook = r->codebook;
+    f->codebook = r->codebook;
+    req.code_chunks = f->chunks;
+    req.chunk_vector_counts = f->counts;
+    req.chunk_count = 1;
+    req.chunk_vectors = f->counts[0];
+    if (retrieval_pq_residual_index_open(&req, &f->index) ||
+        retrieval_pq_residual_query_create(f->index, &f->query)) {
+        free(f);
+        return NULL;
+    }
+    return f;
+}
+
 void ref_residual_close(void *handle) {
     RefResidual *r = handle;
     char path[300];
     if (!r) return;
+    if (!r->dir[0]) {
+        retrieval_pq_residual_query_destroy(r->query);
+        retrieval_pq_residual_index_close(r->index);
+        free(r);
+        return;
+    }
     if (r->e4_ready) {
         unlink(r->e4_path);
         unlink(r->e4_raw);

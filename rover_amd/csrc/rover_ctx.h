@@ -71,6 +71,7 @@ typedef struct RoverCtx {
                                  work counter and scratch are shared        */
     int launched;
     void *ev_last;            /* completion event of that launch           */
+    void *stream;             /* the session's stream for host-buffer calls */
     void *d_noalloc;          /* this device's fixed no-allocation workspace */
     int32_t *d_entries;
     int entries_n, entries_count, entries_graph_entry;

@@ -200,6 +200,12 @@ int rv_event_destroy(void *ev);
 int rv_stream_create(void **stream);
 int rv_stream_destroy(void *stream);
 int rv_stream_synchronize(void *stream);
+/* Copies ordered on `stream` (asynchronous for page-locked host memory). */
+int rv_memcpy_h2d_async(void *d_dst, const void *src, size_t bytes, void *stream);
+int rv_memcpy_d2h_async(void *dst, const void *d_src, size_t bytes, void *stream);
+/* Page-locked host memory. */
+int rv_host_alloc(void **p, size_t bytes);
+int rv_host_free(void *p);
 /* An event without timing data, and a device-side wait of `stream` for it. */
 int rv_event_create_untimed(void **ev);
 int rv_stream_wait_event(void *stream, void *ev);

@@ -94,10 +94,12 @@ int calculator_open(const char *uri, remote_handle64 *handle) {
     rover_sha256(NULL, 0, c->orig_maps_sha);
     pthread_mutex_init(&c->lock, NULL);
     if (rv_set_device(0) || rv_malloc((void **)&c->d_queue, 64) ||
-        rv_event_create_untimed(&c->ev_last) || rv_noalloc_scratch(0, &c->d_noalloc)) {
+        rv_event_create_untimed(&c->ev_last) || rv_stream_create(&c->stream) ||
+        rv_noalloc_scratch(0, &c->d_noalloc)) {
         set_err("calculator_open");
         dfree(c->d_queue);
         if (c->ev_last) rv_event_destroy(c->ev_last);
+        if (c->stream) rv_stream_destroy(c->stream);
         free(c);
         return ROVER_EFAILED;
     }
@@ -175,6 +177,10 @@ int calculator_close(remote_handle64 h) {
     dfree(c->d_route_ent);
     dfree(c->d_sqmd_vis);
     dfree(c->d_sqmd_front);
+    if (c->stream) {
+        rv_stream_synchronize(c->stream);
+        rv_stream_destroy(c->stream);
+    }
     if (c->ev_last) rv_event_destroy(c->ev_last);
     c->magic = 0;
     pthread_mutex_destroy(&c->lock);
@@ -1549,29 +1555,38 @@ static AEEResult search_host(RoverCtx *c, const RoverSearchBatch *b,
         d.out_counts = (int32_t *)(base + oc);
         d.out_stats = b->out_stats ? (uint64_t *)(base + os) : NULL;
         d.entry_ids = b->entry_ids ? (const int32_t *)(base + oe) : NULL;
-        if (entries_bytes && rv_memcpy_h2d(base + oe, b->entry_ids, entries_bytes)) {
-            set_err("batch upload");
-            goto out;
-        }
-        if (rv_memcpy_h2d(base + oq, b->queries,
-                          (size_t)b->num_queries * dim * sizeof(float)) ||
+        /* uploads, the search and downloads are ordered on the session's
+           stream; from page-locked host memory (rover_host_alloc) the copies
+           are DMA transfers, from pageable memory HIP stages them */
+        void *s = c->stream;
+        if (rover_stream_turn(c, s)) goto out;
+        if ((entries_bytes && rv_memcpy_h2d_async(base + oe, b->entry_ids, entries_bytes, s)) ||
+            rv_memcpy_h2d_async(base + oq, b->queries,
+                                (size_t)b->num_queries * dim * sizeof(float), s) ||
             (b->pool_sizes &&
-             rv_memcpy_h2d(base + ol, b->pool_sizes, (size_t)b->num_queries * 4)) ||
+             rv_memcpy_h2d_async(base + ol, b->pool_sizes, (size_t)b->num_queries * 4, s)) ||
             (b->num_outs &&
-             rv_memcpy_h2d(base + ok, b->num_outs, (size_t)b->num_queries * 4))) {
+             rv_memcpy_h2d_async(base + ok, b->num_outs, (size_t)b->num_queries * 4, s))) {
             set_err("batch upload");
+            rv_stream_synchronize(s);
             goto out;
         }
-        rc = launch_batch(c, &d, max_pool, explicit_entries, NULL);
-        if (rc != ROVER_SUCCESS) goto out;
+        rc = launch_batch(c, &d, max_pool, explicit_entries, s);
+        if (rc != ROVER_SUCCESS) {
+            rv_stream_synchronize(s);
+            goto out;
+        }
         rc = ROVER_EFAILED;
-        if (rv_synchronize() ||
-            rv_memcpy_d2h(b->out_ids, base + oi, (size_t)b->num_queries * b->out_stride * 4) ||
-            rv_memcpy_d2h(b->out_dists, base + od, (size_t)b->num_queries * b->out_stride * 4) ||
-            rv_memcpy_d2h(b->out_counts, base + oc, (size_t)b->num_queries * 4) ||
+        if (rv_memcpy_d2h_async(b->out_ids, base + oi,
+                                (size_t)b->num_queries * b->out_stride * 4, s) ||
+            rv_memcpy_d2h_async(b->out_dists, base + od,
+                                (size_t)b->num_queries * b->out_stride * 4, s) ||
+            rv_memcpy_d2h_async(b->out_counts, base + oc, (size_t)b->num_queries * 4, s) ||
             (b->out_stats &&
-             rv_memcpy_d2h(b->out_stats, base + os, (size_t)b->num_queries * 32))) {
+             rv_memcpy_d2h_async(b->out_stats, base + os, (size_t)b->num_queries * 32, s)) ||
+            rv_stream_synchronize(s)) {
             set_err("batch download");
+            rv_stream_synchronize(s);
             goto out;
         }
         rc = ROVER_SUCCESS;
@@ -1805,6 +1820,24 @@ AEEResult rover_device_alloc(void **d_ptr, size_t bytes) {
     }
     return ROVER_SUCCESS;
 }
+AEEResult rover_host_alloc(void **ptr, size_t bytes) {
+    if (!ptr) return ROVER_EBADPARM;
+    *ptr = NULL;
+    if (rv_host_alloc(ptr, bytes)) {
+        set_err("rover_host_alloc");
+        return ROVER_ENOMEMORY;
+    }
+    return ROVER_SUCCESS;
+}
+
+AEEResult rover_host_free(void *ptr) {
+    if (ptr && rv_host_free(ptr)) {
+        set_err("rover_host_free");
+        return ROVER_EFAILED;
+    }
+    return ROVER_SUCCESS;
+}
+
 AEEResult rover_device_free(void *d_ptr) {
     return rv_free(d_ptr) ? ROVER_EFAILED : ROVER_SUCCESS;
 }

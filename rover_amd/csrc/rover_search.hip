@@ -1855,6 +1855,19 @@ extern "C" int rv_event_destroy(void *ev) {
     return rv_check(hipEventDestroy((hipEvent_t)ev), "hipEventDestroy");
 }
 // Streams are created at the default priority.
+extern "C" int rv_memcpy_h2d_async(void *dst, const void *src, size_t bytes, void *stream) {
+    return rv_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream),
+                    "hipMemcpyAsync H2D");
+}
+extern "C" int rv_memcpy_d2h_async(void *dst, const void *src, size_t bytes, void *stream) {
+    return rv_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream),
+                    "hipMemcpyAsync D2H");
+}
+extern "C" int rv_host_alloc(void **p, size_t bytes) {
+    return rv_check(hipHostMalloc(p, bytes ? bytes : 16, hipHostMallocDefault),
+                    "hipHostMalloc");
+}
+extern "C" int rv_host_free(void *p) { return rv_check(hipHostFree(p), "hipHostFree"); }
 extern "C" int rv_event_create_untimed(void **ev) {
     hipEvent_t e;
     if (rv_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate"))

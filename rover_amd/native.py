@@ -168,6 +168,8 @@ SIGNATURES = [
      [_P, _P, _P, _P, _I, _I, _I, _I, _P, _I, _P, _P]),
     ("rover_device_alloc", _I, [C.POINTER(_P), C.c_size_t]),
     ("rover_device_free", _I, [_P]),
+    ("rover_host_alloc", _I, [C.POINTER(_P), C.c_size_t]),
+    ("rover_host_free", _I, [_P]),
     ("rover_device_upload", _I, [_P, _P, C.c_size_t]),
     ("rover_device_download", _I, [_P, _P, C.c_size_t]),
     ("rover_device_synchronize", _I, []),
@@ -258,6 +260,31 @@ class DeviceBuffer:
     def close(self):
         if self.ptr is not None and _lib is not None:
             _lib.rover_device_free(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PinnedArray:
+    """A numpy array over page-locked host memory (rover_host_alloc)."""
+
+    def __init__(self, shape, dtype):
+        dt = np.dtype(dtype)
+        n = int(np.prod(shape)) * dt.itemsize
+        p = C.c_void_p()
+        check(lib().rover_host_alloc(C.byref(p), max(n, 16)), "rover_host_alloc")
+        self.ptr = p.value
+        buf = (C.c_char * max(n, 16)).from_address(self.ptr)
+        self.array = np.frombuffer(buf, dtype=dt, count=int(np.prod(shape))).reshape(shape)
+
+    def close(self):
+        if self.ptr is not None and _lib is not None:
+            self.array = None
+            _lib.rover_host_free(self.ptr)
         self.ptr = None
 
     def __del__(self):

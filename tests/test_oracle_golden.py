@@ -454,3 +454,60 @@ def test_sq_encoder_rounds_exact_halves_away_from_zero():
                 q = codes[0].astype(np.int64)
             assert np.array_equal(q, expect)
             assert q[3] == 2 and q[4] == 3  # 1.5 -> 2, 2.5 -> 3 (not 2)
+
+
+def test_reference_residual_forks_search_in_threads(golden):
+    """The CPU baseline runs the compiled reference search on several host
+    threads, one residual reader each (ref_residual_fork): every thread's
+    results equal the single-reader run on the same queries."""
+    import ctypes as C
+    import threading
+    if not pyoracle.ref_available():
+        pytest.skip("reference build not available")
+    ref = pyoracle.reference()
+    g = golden("pq_d128_m32_r8")
+    n, R = g["graph"].shape
+    m, ks, sub = g["codebook"].shape
+    dim = m * sub
+    base = np.ascontiguousarray(g["base"], dtype=np.float32)
+    codes, cb = np.ascontiguousarray(g["codes"]), np.ascontiguousarray(g["codebook"])
+    graph = np.ascontiguousarray(g["graph"], dtype=np.int32)
+    res = np.zeros_like(g["res_codes"])
+    qp = np.zeros_like(g["res_qparams"])
+    h = ref.ref_residual_open(base.ctypes.data, codes.ctypes.data, cb.ctypes.data, n, dim, m,
+                              ks, int(g["res_bits"]), res.ctypes.data, qp.ctypes.data)
+    assert h
+    ents = pyoracle.seed_ids(n, 32, int(g["entry"]), 42)
+    queries = np.ascontiguousarray(g["queries"], dtype=np.float32)
+
+    def search(handle, q):
+        ids = np.zeros(10, np.int32)
+        d = np.zeros(10, np.float32)
+        st = np.zeros(4, np.uint64)
+        rc = ref.ref_search(graph.ctypes.data, R, n, codes.ctypes.data, 300, cb.ctypes.data,
+                            dim, m, ks, q.ctypes.data, ents.ctypes.data, len(ents), 32, 64,
+                            10, 1, handle, None, 8, C.c_float(1.0), C.c_float(0.0),
+                            ids.ctypes.data, d.ctypes.data, 10, st.ctypes.data)
+        return rc, ids.copy(), d.view(np.uint32).copy()
+
+    try:
+        want = [search(h, q) for q in queries]
+        forks = [ref.ref_residual_fork(h) for _ in range(4)]
+        assert all(forks)
+        got = [None] * len(queries)
+
+        def work(t):
+            for qi in range(t, len(queries), 4):
+                got[qi] = search(forks[t], queries[qi])
+
+        th = [threading.Thread(target=work, args=(t,)) for t in range(4)]
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+        for (rc0, i0, d0), (rc1, i1, d1) in zip(want, got):
+            assert rc0 == rc1 == 10 and np.array_equal(i0, i1) and np.array_equal(d0, d1)
+        for f in forks:
+            ref.ref_residual_close(f)
+    finally:
+        ref.ref_residual_close(h)
