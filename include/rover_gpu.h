@@ -200,6 +200,52 @@ AEEResult rover_pure_pq_error4_load(remote_handle64 h, const uint8_t *artifact,
                                     const uint8_t *expected_sha256);
 AEEResult rover_pure_pq_error4_unload(remote_handle64 h);
 
+/* Third-level refine artifact of the SQ index (SQERR4V1; the SQ path opens
+ * it with retrieval_final_refine_open and no codebook,
+ * retrieval_rpc_demo_pipeline.c:3026-3045, retrieval_final_refine.c:98-190;
+ * written by retrieval_build_error4 with pq = 0,
+ * retrieval_build_artifacts.c:25-66): the whole file image, checked as for
+ * PQERR4V1 against the loaded SQ index; the SQ index has no id maps, so the
+ * artifact must be bound to none.  SQ traversal batches with
+ * refine_candidates = F > 0 re-rank their first F results in FP64 from the
+ * SQ + residual decode (retrieval_final_refine_decode_sq, needs
+ * rover_sq_residual_load), refine_mode = ROVER_REFINE_ERROR4 adding this
+ * artifact's level. */
+AEEResult rover_sq_error4_load(remote_handle64 h, const uint8_t *artifact,
+                               int64_t artifactLen, const uint8_t *expected_sha256);
+AEEResult rover_sq_error4_unload(remote_handle64 h);
+
+/* Reconstructed vectors out [count][dim] (host buffers) of the loaded
+ * index: the PQ + residual decode (retrieval_pq_residual_query_decode,
+ * retrieval_pq_residual.c:954-981) or the SQ + residual decode
+ * (retrieval_final_refine_decode_sq, retrieval_final_refine.c:55-67), each
+ * step a rounded f32 as in the reference.  Ids out of range give
+ * ROVER_EBADPARM. */
+AEEResult rover_pure_pq_decode(remote_handle64 h, const int32_t *ids, int count,
+                               float *out);
+AEEResult rover_sq_decode(remote_handle64 h, const int32_t *ids, int count,
+                          float *out);
+
+/* Refine traces of the reference demos.  ROVER_TRACE_TEXT is the pure-PQ
+ * demo's text trace (retrieval_rpc_demo_pure_pq.c:535-538, 615-645): per
+ * query "Q pool query count budget", a "C id dist" line per search result,
+ * a "V id v0 v1 ..." line with the decoded vector of each of the first
+ * decoded_count results (the demo decodes min(count, budget or top-k)), and
+ * an "O id dist" line per output, numbers as C's %a of a double.
+ * ROVER_TRACE_P15 is the SQ pipeline's binary trace (retrieval_rpc_demo_
+ * pipeline.c:3047-3050, 3276-3308): "P15TRC01", then per query u32 (entry
+ * points, pool, query, count), the ids and f32 distances of the search
+ * results, u32 n and the n refined ids and distances.  No reference
+ * counterpart as functions: the demos write them inline. */
+enum { ROVER_TRACE_TEXT = 1, ROVER_TRACE_P15 = 2 };
+AEEResult rover_trace_open(const char *path, int format, void **trace);
+AEEResult rover_trace_query(void *trace, int entry_points, int pool_size,
+                            int query_index, int refine_budget, const int32_t *ids,
+                            const float *dists, int count, const float *decoded,
+                            int decoded_count, int dim, const int32_t *out_ids,
+                            const float *out_dists, int out_count);
+AEEResult rover_trace_close(void *trace);
+
 /* SHA-256 (FIPS 180-4) of a buffer, as the artifact digests use. */
 void rover_sha256(const void *data, size_t bytes, uint8_t out[32]);
 

@@ -57,6 +57,7 @@ def oracle():
         L.orc_residual_encode.argtypes = [P, P, P, I, I, I, I, P, P]
         L.orc_residual_evaluate.argtypes = [P, P, I, P]
         L.orc_residual_decode.argtypes = [P, I, P]
+        L.orc_sq_decode.argtypes = [P, I, P]
         L.orc_error4_encode.argtypes = [P, P, I, P, P]
         L.orc_sq_range_params.argtypes = [P, C.c_long, I, P, P]
         L.orc_sq_encode.argtypes = [P, I, I, F, F, P]
@@ -109,6 +110,12 @@ def reference():
         L.ref_residual_close.argtypes = [P]
         L.ref_residual_fork.argtypes = [P]
         L.ref_residual_fork.restype = P
+        L.ref_sq_refine_open.argtypes = [P, P, I, F, F, P, I, P, I, I]
+        L.ref_sq_refine_open.restype = P
+        L.ref_sq_refine_artifact.argtypes = [P, P, C.c_long]
+        L.ref_sq_refine_artifact.restype = C.c_long
+        L.ref_sq_refine.argtypes = [P, I, P, P, P, I, I, P]
+        L.ref_sq_refine_close.argtypes = [P]
         L.ref_residual_evaluate.argtypes = [P, P, I, P]
         L.ref_residual_decode.argtypes = [P, I, P]
         L.ref_search.argtypes = [P, I, I, P, I, P, I, I, I, P, P, I, I, I, I, I,
@@ -265,6 +272,13 @@ def residual_evaluate(index, query, vid):
 def residual_decode(index, vid):
     out = np.zeros(index.dim, dtype=np.float32)
     rc = oracle().orc_residual_decode(C.byref(index.struct), vid, _p(out))
+    return None if rc else out
+
+
+def sq_decode(index, vid):
+    """orc_sq_decode: the SQ index's SQ + residual reconstruction."""
+    out = np.zeros(index.dim, dtype=np.float32)
+    rc = oracle().orc_sq_decode(C.byref(index.struct), vid, _p(out))
     return None if rc else out
 
 

@@ -505,6 +505,150 @@ int ref_refine_mode(void *residual, int mode, int n, int dim,
     return rc;
 }
 
+/* ---- SQ final refine (SQERR4V1), written and read by the reference ---- */
+
+typedef struct RefSqRefine {
+    char dir[128], raw[300], l1[300], res[300], qp[300], art[300];
+    const uint8_t *l1_codes, *res_codes;
+    const float *res_qp;
+    int n, dim, l1_bits, res_bits, l1_cb, res_cb, ready;
+    float l1_scale, l1_zero;
+} RefSqRefine;
+
+/* retrieval_final_refine_decode_sq over the SQ codes and residuals: the
+ * reconstruction the live SQ path refines from (pipeline.c:2725-2738). */
+static int ref_sq_refine_decode(void *provider, int id, float *out) {
+    const RefSqRefine *r = provider;
+    if (id < 0 || id >= r->n) return RETRIEVAL_REFINE_INVALID;
+    return retrieval_final_refine_decode_sq(
+        r->l1_codes + (size_t)id * r->l1_cb, r->l1_bits, r->l1_scale, r->l1_zero,
+        r->res_codes + (size_t)id * r->res_cb, r->res_bits, r->res_qp[2 * (size_t)id],
+        r->res_qp[2 * (size_t)id + 1], r->dim, out);
+}
+
+/* Writes the base vectors (raw fvecs) and the SQ code, residual and qparams
+ * arrays as files (only hashed), builds the SQERR4V1 artifact with
+ * retrieval_build_error4 (pq = 0) over the SQ decoder, and returns a handle
+ * for ref_sq_refine; the artifact's bytes via ref_sq_refine_artifact. */
+void *ref_sq_refine_open(const float *x, const uint8_t *l1, int l1_bits, float l1_scale,
+                         float l1_zero, const uint8_t *res, int res_bits,
+                         const float *res_qp, int n, int dim) {
+    RefSqRefine *r = calloc(1, sizeof(*r));
+    RetrievalRawReader *raw = NULL;
+    const char *paths[3];
+    FILE *f;
+    if (!r) return NULL;
+    r->l1_codes = l1;
+    r->res_codes = res;
+    r->res_qp = res_qp;
+    r->n = n;
+    r->dim = dim;
+    r->l1_bits = l1_bits;
+    r->res_bits = res_bits;
+    r->l1_cb = l1_bits == 4 ? (dim + 1) / 2 : dim;
+    r->res_cb = res_bits == 4 ? (dim + 1) / 2 : dim;
+    r->l1_scale = l1_scale;
+    r->l1_zero = l1_zero;
+    snprintf(r->dir, sizeof(r->dir), "/tmp/rover_sqref_XXXXXX");
+    if (!mkdtemp(r->dir)) goto fail;
+    snprintf(r->raw, sizeof(r->raw), "%s/base.fvecs", r->dir);
+    snprintf(r->l1, sizeof(r->l1), "%s/l1.bin", r->dir);
+    snprintf(r->res, sizeof(r->res), "%s/residual.bin", r->dir);
+    snprintf(r->qp, sizeof(r->qp), "%s/qparams.bin", r->dir);
+    snprintf(r->art, sizeof(r->art), "%s/error4.refine", r->dir);
+    f = fopen(r->raw, "wb");
+    if (!f) goto fail;
+    for (int i = 0; i < n; i++) {
+        int32_t d32 = dim;
+        if (fwrite(&d32, 4, 1, f) != 1 ||
+            fwrite(x + (size_t)i * dim, 4, (size_t)dim, f) != (size_t)dim) {
+            fclose(f);
+            goto fail;
+        }
+    }
+    if (fclose(f) || ref_write_file(r->l1, l1, (size_t)n * r->l1_cb, NULL, 0) ||
+        ref_write_file(r->res, res, (size_t)n * r->res_cb, NULL, 0) ||
+        ref_write_file(r->qp, res_qp, (size_t)n * 8, NULL, 0))
+        goto fail;
+    if (retrieval_raw_open(r->raw, n, dim, (size_t)1 << 20, &raw)) goto fail;
+    paths[0] = r->l1;
+    paths[1] = r->res;
+    paths[2] = r->qp;
+    if (retrieval_build_error4(r->art, raw, n, dim, ref_sq_refine_decode, r, paths,
+                               r->raw, 0)) {
+        retrieval_raw_close(raw);
+        goto fail;
+    }
+    retrieval_raw_close(raw);
+    r->ready = 1;
+    return r;
+fail:
+    free(r);
+    return NULL;
+}
+
+long ref_sq_refine_artifact(void *handle, uint8_t *out, long cap) {
+    RefSqRefine *r = handle;
+    FILE *f;
+    long size;
+    if (!r || !r->ready || !(f = fopen(r->art, "rb"))) return -1;
+    fseek(f, 0, SEEK_END);
+    size = ftell(f);
+    if (out && size <= cap) {
+        fseek(f, 0, SEEK_SET);
+        if (fread(out, 1, (size_t)size, f) != (size_t)size) size = -1;
+    }
+    fclose(f);
+    return size;
+}
+
+/* retrieval_final_refine with the SQ decoder in `mode` (CURRENT or ERROR4,
+ * the latter through retrieval_final_refine_open with no codebook: the
+ * SQERR4V1 path of pipeline.c:3026-3045).  Returns the output count, or the
+ * open error - 100. */
+int ref_sq_refine(void *handle, int mode, const float *query, int32_t *ids, float *dists,
+                  int count, int output_k, const int32_t *graph_to_original) {
+    RefSqRefine *r = handle;
+    RetrievalFinalRefineConfig c;
+    RetrievalFinalRefineContext *ctx = NULL;
+    unsigned char digest[32];
+    int rc;
+    memset(&c, 0, sizeof(c));
+    c.mode = mode;
+    c.num_vectors = r->n;
+    c.dim = r->dim;
+    c.capacity = count > 0 ? count : 1;
+    c.cache_slots = 4;
+    c.decode = ref_sq_refine_decode;
+    c.provider = r;
+    c.graph_to_original = graph_to_original;
+    if (mode == RETRIEVAL_REFINE_ERROR4) {
+        if (retrieval_build_hash(r->art, digest)) return -200;
+        c.artifact_path = r->art;
+        c.artifact_sha256 = digest;
+        c.base_paths[0] = r->l1;
+        c.base_paths[1] = r->res;
+        c.base_paths[2] = r->qp;
+    }
+    rc = retrieval_final_refine_open(&c, &ctx);
+    if (rc) return rc - 100;
+    rc = retrieval_final_refine(ctx, query, ids, dists, count, output_k, NULL);
+    retrieval_final_refine_close(ctx);
+    return rc;
+}
+
+void ref_sq_refine_close(void *handle) {
+    RefSqRefine *r = handle;
+    if (!r) return;
+    unlink(r->raw);
+    unlink(r->l1);
+    unlink(r->res);
+    unlink(r->qp);
+    unlink(r->art);
+    rmdir(r->dir);
+    free(r);
+}
+
 /* ---- the live SQ traversal: sqmd_cpu_state_run_fake_dsp ---- */
 
 typedef struct {

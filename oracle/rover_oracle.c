@@ -629,6 +629,40 @@ static int orc_refine_cmp(const void *a, const void *b) {
     return (x->original > y->original) - (x->original < y->original);
 }
 
+/* retrieval_final_refine.c:55-67 (retrieval_final_refine_decode_sq): per
+ * dimension add_f32(decode_value(l1 code, scale, zero), decode_value(residual
+ * code, rscale, rzero)), decode_value = (code * scale) + zero, every step a
+ * rounded f32 (:46-54).  -1 for parameters the reference refuses, -4 for a
+ * non-finite value. */
+int orc_sq_decode(const OrcIndex *ix, int id, float *out) {
+    const int lcb = ix->sq_bits == 4 ? (ix->dim + 1) / 2 : ix->dim;
+    const int rcb = ix->res_bits == 4 ? (ix->dim + 1) / 2 : ix->dim;
+    const uint8_t *l1, *res;
+    float rs, rz;
+    if (!ix->sq_codes || !ix->res_codes || id < 0 || id >= ix->num_vectors ||
+        (ix->sq_bits != 4 && ix->sq_bits != 8) || (ix->res_bits != 4 && ix->res_bits != 8))
+        return -1;
+    l1 = ix->sq_codes + (size_t)id * lcb;
+    res = ix->res_codes + (size_t)id * rcb;
+    rs = ix->res_qparams[2 * (size_t)id];
+    rz = ix->res_qparams[2 * (size_t)id + 1];
+    if (!isfinite(ix->sq_scale) || !(ix->sq_scale > 0.0f) || !isfinite(ix->sq_zero) ||
+        !isfinite(rs) || rs < 0.0f || !isfinite(rz))
+        return -1;
+    for (int d = 0; d < ix->dim; d++) {
+        const unsigned lc = ix->sq_bits == 4 ? (l1[d >> 1] >> (4 * (d & 1))) & 15u : l1[d];
+        const unsigned rc = ix->res_bits == 4 ? (res[d >> 1] >> (4 * (d & 1))) & 15u : res[d];
+        volatile float lp = (float)lc * ix->sq_scale;
+        volatile float lv = lp + ix->sq_zero;
+        volatile float rp = (float)rc * rs;
+        volatile float rv = rp + rz;
+        volatile float v = lv + rv;
+        out[d] = v;
+        if (!isfinite(out[d])) return -4;
+    }
+    return 0;
+}
+
 /* retrieval_final_refine.c:256-292 (refine_impl) with the PQ residual
  * decoder (retrieval_pq_residual.c:954-981) as the vector source, plus the
  * error4 level when the index carries one (ERROR4 mode):
@@ -663,8 +697,11 @@ int orc_refine(const OrcIndex *ix, const float *query, const int32_t *ids,
             }
         items[i].graph = id;
         items[i].original = graph_to_original ? graph_to_original[id] : id;
-        if (orc_residual_decode(ix, id, vec) != 0) {
-            rc = -1;
+        /* the SQ index (no PQ codebook) refines from the SQ decoder, the
+         * PQ index from the residual decoder */
+        rc = ix->codebook ? orc_residual_decode(ix, id, vec) : orc_sq_decode(ix, id, vec);
+        if (rc != 0) {
+            rc = rc == -4 ? -4 : -1;
             goto out;
         }
         if (ix->e4_codes) {

@@ -97,6 +97,12 @@ int orc_search(const OrcIndex *index, int scorer, const float *query,
                int pool_capacity, int num_out, int *out_ids,
                float *out_distances, int out_capacity, OrcStats *out_stats);
 
+/* The SQ index's reconstruction (retrieval_final_refine_decode_sq):
+ * sq_codes + residual; 0, -1 (bad parameters) or -4 (non-finite). */
+int orc_sq_decode(const OrcIndex *index, int id, float *out);
+
+/* With a PQ codebook the vectors come from the residual decoder, without
+ * one (an SQ index) from orc_sq_decode. */
 int orc_refine(const OrcIndex *index, const float *query, const int32_t *ids,
                int count, int output_k, const int32_t *graph_to_original,
                int32_t *out_ids, float *out_distances);

@@ -55,6 +55,9 @@ typedef struct RoverCtx {
     float *d_sq_res_qp;
     int sq_res_bits;
     float *d_split_bias;       /* SQMDBIAS payload for the split distance */
+    uint8_t *d_sq_e4_blocks;   /* SQERR4V1 code blocks and qparams        */
+    float *d_sq_e4_qparams;
+    int sq_e4_loaded, sq_e4_vpb, sq_e4_code_bytes;
     /* SQ traversal scratch: exact visited bitmaps and HBM frontiers per
        wave, and the cached live-path seed list */
     uint32_t *d_sqmd_vis;

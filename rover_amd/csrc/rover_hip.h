@@ -146,6 +146,10 @@ typedef struct RvSqmdBatch {
     int max_work;                /* > this many neighbors per turn: the
                                     query fails (the live path's limit 32)  */
     int correction;              /* 0: SQ code distance; 1: SQ + residual   */
+    int refine_candidates;       /* > 0: final refine of the first F pool
+                                    members from the SQ + residual decode
+                                    (retrieval_final_refine_decode_sq)      */
+    int refine_error4;           /* the refine adds the error4 level        */
     int32_t *out_ids;            /* [nq][out_stride]                        */
     float *out_dists;
     int32_t *out_counts;         /* count or RV_QUERY_* status              */
@@ -209,6 +213,11 @@ int rv_host_free(void *p);
 /* An event without timing data, and a device-side wait of `stream` for it. */
 int rv_event_create_untimed(void **ev);
 int rv_stream_wait_event(void *stream, void *ev);
+
+/* Decoded vectors d_out [count][dim] of the ids d_ids: sq = 0 the PQ +
+ * residual reconstruction, sq = 1 the SQ + residual one.  Asynchronous. */
+int rv_decode_vectors(const RvIndex *index, int sq, const int32_t *d_ids, int count,
+                      float *d_out, void *stream);
 
 /* Fixed per-device workspace for the no-allocation batch entry point: the
  * work counter at offset 0, then at most RV_NOALLOC_MAX_SEEDS seed ids at

@@ -152,6 +152,7 @@ AEEResult calculator_retrieval_unload(remote_handle64 h) {
     dfree(c->d_sqmd_seeds);
     dfree(c->d_split_bias);
     c->d_split_bias = NULL;
+    rover_sq_error4_unload(h);
     c->d_sq_graph = NULL;
     c->d_sq_codes = NULL;
     c->d_sq_res = NULL;
@@ -464,32 +465,29 @@ AEEResult rover_pure_pq_error4_unload(remote_handle64 h) {
 }
 
 /* The checks of refine_open in ERROR4 mode (retrieval_final_refine.c:
- * 128-160 and 181-186) on an in-memory image. */
-AEEResult rover_pure_pq_error4_load(remote_handle64 h, const uint8_t *a,
-                                    int64_t len, const uint8_t *expected_sha256) {
-    RoverCtx *c = ctx_of(h);
+ * 128-160 and 181-186) on an in-memory image with the given magic, for an
+ * index of n vectors of dim dimensions: header fields, section sizes, the
+ * whole-file digest when expected_sha256 is given, the payload digest, every
+ * (scale, zero) finite with scale >= 0.  Fills the block geometry. */
+static AEEResult check_error4_image(const uint8_t *a, int64_t len, const char *magic,
+                                    const char *other_magic, int n_vec, int dim,
+                                    const uint8_t *expected_sha256, int *out_vpb,
+                                    int *out_cb, uint64_t *out_qb, uint64_t *out_cbytes) {
     uint8_t digest[32];
-    uint64_t n, qb, cbytes;
-    int cb, vpb;
-    if (!c || !a) return ROVER_EBADPARM;
-    if (!c->pq_loaded) {
-        snprintf(g_err, sizeof(g_err), "error4_load: no PQ index loaded");
-        return ROVER_EBADPARM;
-    }
-    n = (uint64_t)c->n;
-    cb = (c->dim + 1) / 2;
-    vpb = RV_E4_BLOCK / cb;
+    const uint64_t n = (uint64_t)n_vec;
+    const int cb = (dim + 1) / 2;
+    const int vpb = RV_E4_BLOCK / cb;
+    uint64_t qb, cbytes;
     if (vpb <= 0 || len < 256) goto bad;
     qb = n * 8;
     cbytes = (n + (uint64_t)vpb - 1) / (uint64_t)vpb * RV_E4_BLOCK;
-    if (memcmp(a, "PQERR4V1", 8)) {
+    if (memcmp(a, magic, 8)) {
         snprintf(g_err, sizeof(g_err), "error4_load: %s",
-                 memcmp(a, "SQERR4V1", 8) ? "not an error4 artifact"
-                                          : "SQ artifact; the PQ refine needs PQERR4V1");
+                 memcmp(a, other_magic, 8) ? "not an error4 artifact"
+                                           : "artifact of the other index kind");
         return ROVER_EBADPARM;
     }
-    if (le32(a + 8) != 1 || le32(a + 12) != (uint32_t)n ||
-        le32(a + 16) != (uint32_t)c->dim ||
+    if (le32(a + 8) != 1 || le32(a + 12) != (uint32_t)n || le32(a + 16) != (uint32_t)dim ||
         (le32(a + 20) != 4 && le32(a + 20) != 8) ||
         (le32(a + 24) != 4 && le32(a + 24) != 8) || le32(a + 28) != RV_E4_BLOCK ||
         le32(a + 32) != 1 || le32(a + 36) != 1 || le64(a + 40) != qb ||
@@ -516,6 +514,30 @@ AEEResult rover_pure_pq_error4_load(remote_handle64 h, const uint8_t *a,
             return ROVER_EBADPARM;
         }
     }
+    *out_vpb = vpb;
+    *out_cb = cb;
+    *out_qb = qb;
+    *out_cbytes = cbytes;
+    return ROVER_SUCCESS;
+bad:
+    snprintf(g_err, sizeof(g_err), "error4_load: header or size mismatch");
+    return ROVER_EBADPARM;
+}
+
+AEEResult rover_pure_pq_error4_load(remote_handle64 h, const uint8_t *a,
+                                    int64_t len, const uint8_t *expected_sha256) {
+    RoverCtx *c = ctx_of(h);
+    uint64_t qb, cbytes;
+    int cb, vpb;
+    AEEResult rc;
+    if (!c || !a) return ROVER_EBADPARM;
+    if (!c->pq_loaded) {
+        snprintf(g_err, sizeof(g_err), "error4_load: no PQ index loaded");
+        return ROVER_EBADPARM;
+    }
+    rc = check_error4_image(a, len, "PQERR4V1", "SQERR4V1", c->n, c->dim, expected_sha256,
+                            &vpb, &cb, &qb, &cbytes);
+    if (rc != ROVER_SUCCESS) return rc;
     pthread_mutex_lock(&c->lock);
     rover_pure_pq_error4_unload(h);
     if (dalloc_copy((void **)&c->d_e4_qparams, a + 256, (size_t)qb) ||
@@ -531,9 +553,57 @@ AEEResult rover_pure_pq_error4_load(remote_handle64 h, const uint8_t *a,
     c->e4_loaded = 1;
     pthread_mutex_unlock(&c->lock);
     return ROVER_SUCCESS;
-bad:
-    snprintf(g_err, sizeof(g_err), "error4_load: header or size mismatch");
-    return ROVER_EBADPARM;
+}
+
+AEEResult rover_sq_error4_unload(remote_handle64 h) {
+    RoverCtx *c = ctx_of(h);
+    if (!c) return ROVER_EBADPARM;
+    dfree(c->d_sq_e4_blocks);
+    dfree(c->d_sq_e4_qparams);
+    c->d_sq_e4_blocks = NULL;
+    c->d_sq_e4_qparams = NULL;
+    c->sq_e4_loaded = 0;
+    return ROVER_SUCCESS;
+}
+
+/* SQERR4V1 (retrieval_final_refine_open without a codebook, the SQ path of
+ * retrieval_rpc_demo_pipeline.c:3026-3045): the same checks, against the
+ * loaded SQ index; the SQ index has no id maps, so the artifact must be
+ * bound to none (its +216 digest is that of no bytes). */
+AEEResult rover_sq_error4_load(remote_handle64 h, const uint8_t *a, int64_t len,
+                               const uint8_t *expected_sha256) {
+    RoverCtx *c = ctx_of(h);
+    uint8_t none[32];
+    uint64_t qb, cbytes;
+    int cb, vpb;
+    AEEResult rc;
+    if (!c || !a) return ROVER_EBADPARM;
+    if (!c->sq_loaded) {
+        snprintf(g_err, sizeof(g_err), "error4_load: no SQ index loaded");
+        return ROVER_EBADPARM;
+    }
+    rc = check_error4_image(a, len, "SQERR4V1", "PQERR4V1", c->sq_n, c->sq_dim,
+                            expected_sha256, &vpb, &cb, &qb, &cbytes);
+    if (rc != ROVER_SUCCESS) return rc;
+    rover_sha256(NULL, 0, none);
+    if (memcmp(a + 216, none, 32)) {
+        snprintf(g_err, sizeof(g_err), "error4_load: artifact is bound to id maps");
+        return ROVER_EBADPARM;
+    }
+    pthread_mutex_lock(&c->lock);
+    rover_sq_error4_unload(h);
+    if (dalloc_copy((void **)&c->d_sq_e4_qparams, a + 256, (size_t)qb) ||
+        dalloc_copy((void **)&c->d_sq_e4_blocks, a + 256 + qb, (size_t)cbytes)) {
+        rover_sq_error4_unload(h);
+        pthread_mutex_unlock(&c->lock);
+        set_err("error4_load");
+        return ROVER_ENOMEMORY;
+    }
+    c->sq_e4_vpb = vpb;
+    c->sq_e4_code_bytes = cb;
+    c->sq_e4_loaded = 1;
+    pthread_mutex_unlock(&c->lock);
+    return ROVER_SUCCESS;
 }
 
 AEEResult calculator_retrieval_load(remote_handle64 h, const int32_t *graph,
@@ -1028,9 +1098,12 @@ static AEEResult launch_sqmd(RoverCtx *c, const RoverSearchBatch *b, int max_poo
     int grid, front_lds = 0, words;
     if (b->num_queries < 0 || (b->num_queries > 0 && !b->queries) || b->expand_count <= 0 ||
         max_pool <= 0 || max_pool > ROVER_MAX_POOL || b->out_stride <= 0 || !b->out_ids ||
-        !b->out_dists || !b->out_counts || b->refine_candidates || b->learned_stop ||
+        !b->out_dists || !b->out_counts || b->refine_candidates < 0 || b->learned_stop ||
         (b->correction < 0 || b->correction > 2) || (dsp && b->correction) ||
-        b->entry_points <= 0)
+        b->entry_points <= 0 ||
+        (b->refine_mode != 0 && b->refine_mode != ROVER_REFINE_CURRENT &&
+         b->refine_mode != ROVER_REFINE_ERROR4) ||
+        (b->refine_mode == ROVER_REFINE_ERROR4 && b->refine_candidates <= 0))
         return ROVER_EBADPARM;
     /* Caller-given seed lists and expansions of more than 32 neighbors are
      * switched off for the SQ traversals: a GPU run that used them (with an
@@ -1050,7 +1123,8 @@ static AEEResult launch_sqmd(RoverCtx *c, const RoverSearchBatch *b, int max_poo
                  "search: the SQ traversals expand at most 32 neighbors per step");
         return ROVER_EBADPARM;
     }
-    if (b->correction) {
+    if (b->correction || b->refine_candidates > 0) {
+        /* the corrected distances and the final refine decode SQ + residual */
         if (!c->d_sq_res) {
             snprintf(g_err, sizeof(g_err), "search: no SQ residual loaded");
             return ROVER_EFAILED;
@@ -1069,7 +1143,19 @@ static AEEResult launch_sqmd(RoverCtx *c, const RoverSearchBatch *b, int max_poo
         }
         ix.split_bias = c->d_split_bias;
     }
+    if (b->refine_mode == ROVER_REFINE_ERROR4) {
+        if (!c->sq_e4_loaded) {
+            snprintf(g_err, sizeof(g_err), "search: no SQERR4V1 artifact loaded");
+            return ROVER_EFAILED;
+        }
+        ix.e4_blocks = c->d_sq_e4_blocks;
+        ix.e4_qparams = c->d_sq_e4_qparams;
+        ix.e4_vpb = c->sq_e4_vpb;
+        ix.e4_code_bytes = c->sq_e4_code_bytes;
+    }
     memset(&sb, 0, sizeof(sb));
+    sb.refine_candidates = b->refine_candidates;
+    sb.refine_error4 = b->refine_mode == ROVER_REFINE_ERROR4;
     if (dsp) {
         if (ensure_entries(c, ix.n, b->entry_points, ix.graph_entry, seed))
             return ROVER_EFAILED;
@@ -1820,6 +1906,155 @@ AEEResult rover_device_alloc(void **d_ptr, size_t bytes) {
     }
     return ROVER_SUCCESS;
 }
+/* Decoded vectors of `count` ids into out [count][dim] (host buffers). */
+static AEEResult decode_ids(remote_handle64 h, int sq, const int32_t *ids, int count,
+                            float *out) {
+    RoverCtx *c = ctx_of(h);
+    RvIndex ix;
+    int32_t *d_ids = NULL;
+    float *d_out = NULL;
+    AEEResult rc = ROVER_EFAILED;
+    int n, dim;
+    if (!c || count < 0 || (count > 0 && (!ids || !out))) return ROVER_EBADPARM;
+    pthread_mutex_lock(&c->lock);
+    if (sq) {
+        if (fill_index(c, ROVER_METHOD_SQ, 0, 0, &ix) || !c->d_sq_res) {
+            snprintf(g_err, sizeof(g_err), "decode: no SQ index with residuals loaded");
+            goto out;
+        }
+        ix.res_codes = c->d_sq_res;
+        ix.res_qparams = c->d_sq_res_qp;
+        ix.res_bits = c->sq_res_bits;
+        ix.res_code_bytes = c->sq_res_bits == 4 ? (c->sq_dim + 1) / 2 : c->sq_dim;
+    } else if (fill_index(c, ROVER_METHOD_PQ, 1, 0, &ix)) {
+        snprintf(g_err, sizeof(g_err), "decode: no PQ index with residuals loaded");
+        goto out;
+    }
+    n = ix.n;
+    dim = ix.dim;
+    for (int i = 0; i < count; i++)
+        if (ids[i] < 0 || ids[i] >= n) {
+            rc = ROVER_EBADPARM;
+            goto out;
+        }
+    if (count == 0) {
+        rc = ROVER_SUCCESS;
+        goto out;
+    }
+    if (rv_malloc((void **)&d_ids, (size_t)count * 4) ||
+        rv_malloc((void **)&d_out, (size_t)count * dim * 4) ||
+        rv_memcpy_h2d(d_ids, ids, (size_t)count * 4) ||
+        rv_decode_vectors(&ix, sq, d_ids, count, d_out, NULL) || rv_synchronize() ||
+        rv_memcpy_d2h(out, d_out, (size_t)count * dim * 4)) {
+        set_err("decode");
+        goto out;
+    }
+    rc = ROVER_SUCCESS;
+out:
+    pthread_mutex_unlock(&c->lock);
+    dfree(d_ids);
+    dfree(d_out);
+    return rc;
+}
+
+AEEResult rover_pure_pq_decode(remote_handle64 h, const int32_t *ids, int count,
+                               float *out) {
+    return decode_ids(h, 0, ids, count, out);
+}
+
+AEEResult rover_sq_decode(remote_handle64 h, const int32_t *ids, int count, float *out) {
+    return decode_ids(h, 1, ids, count, out);
+}
+
+/* ------------------------------------------------------------ traces */
+
+#define ROVER_TRACE_MAGIC 0x52565452u /* "RVTR" */
+
+typedef struct RoverTrace {
+    uint32_t magic;
+    FILE *f;
+    int format;
+} RoverTrace;
+
+AEEResult rover_trace_open(const char *path, int format, void **trace) {
+    RoverTrace *t;
+    if (!path || !trace || (format != ROVER_TRACE_TEXT && format != ROVER_TRACE_P15))
+        return ROVER_EBADPARM;
+    *trace = NULL;
+    t = calloc(1, sizeof(*t));
+    if (!t) return ROVER_ENOMEMORY;
+    /* the text trace is opened "w" (rpc_demo_pure_pq.c:535-538), the binary
+       one "wbx" and starts with its magic (pipeline.c:3047-3050) */
+    t->f = fopen(path, format == ROVER_TRACE_TEXT ? "w" : "wbx");
+    if (!t->f || (format == ROVER_TRACE_P15 && fwrite("P15TRC01", 1, 8, t->f) != 8)) {
+        if (t->f) fclose(t->f);
+        free(t);
+        snprintf(g_err, sizeof(g_err), "trace: cannot create %s", path);
+        return ROVER_EFAILED;
+    }
+    t->magic = ROVER_TRACE_MAGIC;
+    t->format = format;
+    *trace = t;
+    return ROVER_SUCCESS;
+}
+
+AEEResult rover_trace_query(void *trace, int entry_points, int pool_size, int query_index,
+                            int refine_budget, const int32_t *ids, const float *dists,
+                            int count, const float *decoded, int decoded_count, int dim,
+                            const int32_t *out_ids, const float *out_dists, int out_count) {
+    RoverTrace *t = trace;
+    int ok = 1;
+    if (!t || t->magic != ROVER_TRACE_MAGIC || count < 0 || out_count < 0 ||
+        (count > 0 && (!ids || !dists)) || (out_count > 0 && (!out_ids || !out_dists)))
+        return ROVER_EBADPARM;
+    if (t->format == ROVER_TRACE_TEXT) {
+        /* Q pool query count budget; C id dist per search result; V id and
+           the decoded vector for the first min(count, budget or top-k)
+           results; O id dist per output (rpc_demo_pure_pq.c:615-645) */
+        if (decoded_count < 0 || (decoded_count > 0 && (!decoded || dim <= 0)) ||
+            decoded_count > count)
+            return ROVER_EBADPARM;
+        ok = fprintf(t->f, "Q %d %d %d %d\n", pool_size, query_index, count, refine_budget) > 0;
+        for (int i = 0; ok && i < count; i++)
+            ok = fprintf(t->f, "C %d %a\n", ids[i], (double)dists[i]) > 0;
+        for (int i = 0; ok && i < decoded_count; i++) {
+            ok = fprintf(t->f, "V %d", ids[i]) > 0;
+            for (int d = 0; ok && d < dim; d++)
+                ok = fprintf(t->f, " %a", (double)decoded[(size_t)i * dim + d]) > 0;
+            ok = ok && fputc('\n', t->f) != EOF;
+        }
+        for (int i = 0; ok && i < out_count; i++)
+            ok = fprintf(t->f, "O %d %a\n", out_ids[i], (double)out_dists[i]) > 0;
+    } else {
+        /* u32 (entry points, pool, query, count), ids, distances; then
+           u32 n and the refined ids and distances (pipeline.c:3276-3308) */
+        const uint32_t hdr[4] = {(uint32_t)entry_points, (uint32_t)pool_size,
+                                 (uint32_t)query_index, (uint32_t)count};
+        const uint32_t n = (uint32_t)out_count;
+        ok = fwrite(hdr, sizeof(hdr), 1, t->f) == 1 &&
+             fwrite(ids, 4, (size_t)count, t->f) == (size_t)count &&
+             fwrite(dists, 4, (size_t)count, t->f) == (size_t)count &&
+             fwrite(&n, 4, 1, t->f) == 1 &&
+             fwrite(out_ids, 4, (size_t)n, t->f) == (size_t)n &&
+             fwrite(out_dists, 4, (size_t)n, t->f) == (size_t)n;
+    }
+    if (!ok || ferror(t->f)) {
+        snprintf(g_err, sizeof(g_err), "trace: write failed");
+        return ROVER_EFAILED;
+    }
+    return ROVER_SUCCESS;
+}
+
+AEEResult rover_trace_close(void *trace) {
+    RoverTrace *t = trace;
+    int rc;
+    if (!t || t->magic != ROVER_TRACE_MAGIC) return ROVER_EBADPARM;
+    rc = fclose(t->f);
+    t->magic = 0;
+    free(t);
+    return rc ? ROVER_EFAILED : ROVER_SUCCESS;
+}
+
 AEEResult rover_host_alloc(void **ptr, size_t bytes) {
     if (!ptr) return ROVER_EBADPARM;
     *ptr = NULL;

@@ -1405,6 +1405,52 @@ extern "C" int rv_fill_zero(void *p, size_t bytes, void *stream) {
     return rv_check(hipGetLastError(), "zero kernel");
 }
 
+// ------------------------------------------------------ vector decoding
+
+// One thread per (vector, dimension).  PQ: the residual decode
+// (retrieval_pq_residual.c:954-981), ((centroid + code * scale) + zero).
+// SQ: retrieval_final_refine_decode_sq (retrieval_final_refine.c:55-67),
+// ((l1 * scale) + zero) + ((res * rscale) + rzero).  Each step a rounded
+// f32.  out [count][dim].
+__global__ void rv_decode_kernel(RvIndex ix, int sq, const int32_t *ids, int count,
+                                 float *out) {
+    const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (t >= (size_t)count * ix.dim) return;
+    const int i = (int)(t / ix.dim), d = (int)(t % ix.dim);
+    const int id = ids[i];
+    const float s = ix.res_qparams[2 * (size_t)id], z = ix.res_qparams[2 * (size_t)id + 1];
+    const int rc = rv_res_code(ix.res_codes + (size_t)id * ix.res_code_bytes, ix.res_bits, d);
+    float v;
+    if (sq) {
+        const uint8_t *l1 = ix.sq_codes + (size_t)id * ix.sq_code_bytes;
+        const int lc = ix.sq_bits == 4 ? ((d & 1) ? (l1[d >> 1] >> 4) : (l1[d >> 1] & 15))
+                                       : l1[d];
+        const float lv = (float)lc * ix.sq_scale + ix.sq_zero;
+        const float rv = (float)rc * s + z;
+        v = lv + rv;
+    } else {
+        const int sub = d / ix.subdim;
+        const int code = rv_code_get(ix.codes + (size_t)id * ix.code_bytes, sub, ix.ks);
+        const float c = ix.codebook[((size_t)sub * ix.ks + code) * ix.subdim + d % ix.subdim];
+        v = rv_decode_value(c, rc, s, z);
+    }
+    out[t] = v;
+}
+
+extern "C" int rv_decode_vectors(const RvIndex *ix, int sq, const int32_t *d_ids, int count,
+                                 float *d_out, void *stream) {
+    if (count <= 0) return 0;
+    if (!ix || !d_ids || !d_out || !ix->res_codes || !ix->res_qparams ||
+        (sq ? !ix->sq_codes : (!ix->codes || !ix->codebook))) {
+        snprintf(g_rv_error, sizeof(g_rv_error), "rv_decode_vectors: bad arguments");
+        return -1;
+    }
+    const size_t total = (size_t)count * ix->dim;
+    rv_decode_kernel<<<(unsigned)((total + 255) / 256), 256, 0, (hipStream_t)stream>>>(
+        *ix, sq, d_ids, count, d_out);
+    return rv_check(hipGetLastError(), "rv_decode_kernel launch");
+}
+
 // ------------------------------------------- fixed no-allocation workspace
 
 // Workspace of the entry point that allocates nothing: the work counter in
@@ -2052,14 +2098,15 @@ __device__ __forceinline__ float rv_sq_split(const RvIndex &ix, const uint8_t *q
 }
 
 struct RvSqmdLayout {
-    int off_q, off_qsq, off_pool, off_wid, off_wd, off_front, total;
+    int off_q, off_qsq, off_pool, off_wid, off_wd, off_refine, off_front, total;
 };
 
-static RvSqmdLayout rv_sqmd_layout(int dim, int max_pool, int correction, int front_lds) {
+static RvSqmdLayout rv_sqmd_layout(int dim, int max_pool, int correction, int refine,
+                                   int front_lds) {
     RvSqmdLayout l;
     int off = 0;
     l.off_q = off;
-    if (correction == 1) off += rv_align(dim * 4, 16);
+    if (correction == 1 || refine > 0) off += rv_align(dim * 4, 16);
     l.off_qsq = off;
     off += rv_align(dim, 16);
     l.off_pool = off;
@@ -2068,6 +2115,8 @@ static RvSqmdLayout rv_sqmd_layout(int dim, int max_pool, int correction, int fr
     off += RV_WAVE * 4;
     l.off_wd = off;
     off += RV_WAVE * 4;
+    l.off_refine = off;
+    if (refine > 0) off += rv_align(8 * (refine < max_pool ? refine : max_pool), 16);
     l.off_front = off;
     off += front_lds;
     l.total = off;
@@ -2127,6 +2176,7 @@ __global__ __launch_bounds__(RV_WAVE) void rv_sqmd_kernel(RvIndex ix, RvSqmdBatc
             bool bad = false;
             for (int d = lane; d < ix.dim; d += RV_WAVE) {
                 const float x = qg[d];
+                if (b.refine_candidates > 0) qv[d] = x;
                 const double raw = ((double)x - (double)ix.sq_zero) / qs;
                 if (!isfinite(x) || !isfinite(raw)) bad = true;
                 long v = bad ? 0 : lround(raw);
@@ -2143,7 +2193,7 @@ __global__ __launch_bounds__(RV_WAVE) void rv_sqmd_kernel(RvIndex ix, RvSqmdBatc
             const int hi = ix.sq_bits == 4 ? 15 : 255;
             for (int d = lane; d < ix.dim; d += RV_WAVE) {
                 const float x = qg[d];
-                if (CORR == 1) qv[d] = x;
+                if (CORR == 1 || b.refine_candidates > 0) qv[d] = x;
                 int v = (int)roundf((x - ix.sq_zero) * inv);
                 v = v < 0 ? 0 : (v > hi ? hi : v);
                 qsq[d] = (uint8_t)v;
@@ -2277,14 +2327,84 @@ __global__ __launch_bounds__(RV_WAVE) void rv_sqmd_kernel(RvIndex ix, RvSqmdBatc
             }
             psize = rv_uniform(__shfl(psize, 0));
             __syncthreads();
-            const int cnt = K < psize ? K : psize;
             int32_t *oid = b.out_ids + (size_t)q * b.out_stride;
             float *odist = b.out_dists + (size_t)q * b.out_stride;
-            for (int i = lane; i < cnt; i += RV_WAVE) {
-                oid[i] = pool[i].id;
-                odist[i] = pool[i].d;
+            if (b.refine_candidates <= 0) {
+                const int cnt = K < psize ? K : psize;
+                for (int i = lane; i < cnt; i += RV_WAVE) {
+                    oid[i] = pool[i].id;
+                    odist[i] = pool[i].d;
+                }
+                if (lane == 0) b.out_counts[q] = cnt;
+            } else {
+                // final refine of the first F results (retrieval_final_refine.c:
+                // 256-292) from the SQ decode (:55-67): per dimension
+                // ((l1 * scale) + zero) + ((res * rscale) + rzero), plus the
+                // error4 level ((code * s4) + z4) in ERROR4 mode, each step a
+                // rounded f32; the distance a sequential f64 sum, ranked by
+                // (distance, original id)
+                const int F = b.refine_candidates < psize ? b.refine_candidates : psize;
+                uint64_t *dbits = reinterpret_cast<uint64_t *>(smem + lay.off_refine);
+                bool bad = false;
+                for (int i = lane; i < F; i += RV_WAVE) {
+                    const int id = pool[i].id;
+                    const uint8_t *l1 = ix.sq_codes + (size_t)id * ix.sq_code_bytes;
+                    const uint8_t *res = ix.res_codes + (size_t)id * ix.res_code_bytes;
+                    const float rs = ix.res_qparams[2 * (size_t)id];
+                    const float rz = ix.res_qparams[2 * (size_t)id + 1];
+                    const uint8_t *c4 = nullptr;
+                    float s4 = 0.0f, z4 = 0.0f;
+                    if (b.refine_error4) {
+                        c4 = ix.e4_blocks + (size_t)(id / ix.e4_vpb) * RV_E4_BLOCK +
+                             (size_t)(id % ix.e4_vpb) * ix.e4_code_bytes;
+                        s4 = ix.e4_qparams[2 * (size_t)id];
+                        z4 = ix.e4_qparams[2 * (size_t)id + 1];
+                    }
+                    if (!isfinite(rs) || rs < 0.0f || !isfinite(rz)) bad = true;
+                    double acc = 0.0;
+                    for (int d = 0; d < ix.dim; d++) {
+                        const int lc = ix.sq_bits == 4 ? ((d & 1) ? (l1[d >> 1] >> 4)
+                                                                  : (l1[d >> 1] & 15))
+                                                       : l1[d];
+                        const float lv = (float)lc * ix.sq_scale + ix.sq_zero;
+                        const float rv = (float)rv_res_code(res, ix.res_bits, d) * rs + rz;
+                        float v = lv + rv;
+                        if (!isfinite(v)) bad = true;
+                        if (c4) {
+                            const float level =
+                                (float)((c4[d >> 1] >> (4 * (d & 1))) & 15) * s4 + z4;
+                            v = v + level;
+                            if (!isfinite(v)) bad = true;
+                        }
+                        const double delta = (double)qv[d] - (double)v;
+                        acc += delta * delta;
+                    }
+                    if (!isfinite(acc) || acc >= (double)FLT_MAX) bad = true;
+                    dbits[i] = (uint64_t)__double_as_longlong(acc);
+                }
+                if (__ballot(bad) != 0ull) {
+                    if (lane == 0) b.out_counts[q] = RV_QUERY_NONFINITE;
+                } else {
+                    __syncthreads();
+                    for (int i = lane; i < F; i += RV_WAVE) {
+                        const uint64_t di = dbits[i];
+                        const int gi = pool[i].id;
+                        const int oi = ix.graph_to_original ? ix.graph_to_original[gi] : gi;
+                        int rank = 0;
+                        for (int j = 0; j < F; j++) {
+                            const uint64_t dj = dbits[j];
+                            const int gj = pool[j].id;
+                            const int oj = ix.graph_to_original ? ix.graph_to_original[gj] : gj;
+                            rank += (dj < di) || (dj == di && oj < oi);
+                        }
+                        if (rank < K) {
+                            oid[rank] = gi;
+                            odist[rank] = (float)__longlong_as_double((long long)di);
+                        }
+                    }
+                    if (lane == 0) b.out_counts[q] = K < F ? K : F;
+                }
             }
-            if (lane == 0) b.out_counts[q] = cnt;
         }
         if (b.out_stats && lane == 0) {
             uint64_t *o = b.out_stats + 4 * (size_t)q;
@@ -2313,10 +2433,13 @@ static int rv_sqmd_cus(void) {
 static RvSqmdLayout rv_sqmd_layout_for(const RvIndex *ix, const RvSqmdBatch *b,
                                        int *front_lds) {
     const long fmax = (long)b->fcap_entries + (long)b->max_pool * b->fcap_expand;
-    RvSqmdLayout l = rv_sqmd_layout(ix->dim, b->max_pool, b->correction, 0);
+    RvSqmdLayout l =
+        rv_sqmd_layout(ix->dim, b->max_pool, b->correction, b->refine_candidates, 0);
     const long with = l.total + fmax * 8;
     *front_lds = with <= RV_SQMD_LDS_BUDGET;
-    if (*front_lds) l = rv_sqmd_layout(ix->dim, b->max_pool, b->correction, (int)(fmax * 8));
+    if (*front_lds)
+        l = rv_sqmd_layout(ix->dim, b->max_pool, b->correction, b->refine_candidates,
+                           (int)(fmax * 8));
     return l;
 }
 
@@ -2359,7 +2482,8 @@ extern "C" int rv_sqmd_launch(const RvIndex *ix, const RvSqmdBatch *b, int grid_
     if (b->nq == 0) return 0;
     if (!ix->sq_codes || !ix->graph || !b->visited || b->visited_words % 4 ||
         b->visited_words < (ix->n + 31) / 32 || (!front_lds && !b->frontier) ||
-        (b->correction && (!ix->res_codes || !ix->res_qparams)) ||
+        ((b->correction || b->refine_candidates > 0) && (!ix->res_codes || !ix->res_qparams)) ||
+        (b->refine_error4 && (!ix->e4_blocks || !ix->e4_qparams || ix->e4_vpb <= 0)) ||
         (b->correction == 2 && (!ix->split_bias || ix->sq_bits != 4))) {
         snprintf(g_rv_error, sizeof(g_rv_error), "rv_sqmd_launch: bad arguments");
         return -1;

@@ -198,6 +198,49 @@ def encode_sq(x, bits, scale, zero):
     return out
 
 
+def decode_sq_residual(l1_codes, l1_bits, scale, zero, res_codes, res_qparams, res_bits, dim):
+    """The SQ index's reconstruction, float32 [n][dim]: per dimension
+    ((l1 * scale) + zero) + ((res * rscale) + rzero), each step rounded in
+    float32 (retrieval_final_refine_decode_sq, retrieval_final_refine.c:
+    46-67)."""
+    a = decode_sq(l1_codes, l1_bits, scale, zero, dim)
+    if res_bits == 8:
+        r = res_codes[:, :dim].astype(np.float32)
+    else:
+        r = np.zeros((res_codes.shape[0], dim), dtype=np.float32)
+        r[:, 0::2] = res_codes[:, :(dim + 1) // 2] & 15
+        r[:, 1::2] = res_codes[:, :dim // 2] >> 4
+    qp = np.asarray(res_qparams, dtype=np.float32)
+    level = ((r * qp[:, :1]).astype(np.float32) + qp[:, 1:2]).astype(np.float32)
+    return (a + level).astype(np.float32)
+
+
+def encode_error4(x, recon):
+    """Third-level (error4) codes of the error x - recon, as
+    retrieval_build_error4 forms them (retrieval_build_artifacts.c:25-66),
+    every step a float32 rounding: e = x - recon; lo, hi = min, max of e over
+    the row; scale = max(hi - lo, 1e-12) / 15; inv = 1 / scale;
+    code = clamp(floor((e - lo) * inv + 0.5), 0, 15).  Returns codes
+    [n][(dim+1)/2] (low nibble = even dimension) and (scale, lo) [n][2]."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    e = (x - np.asarray(recon, dtype=np.float32)).astype(np.float32)
+    if not np.isfinite(e).all():
+        raise ValueError("non-finite reconstruction error")
+    lo = e.min(axis=1).astype(np.float32)
+    hi = e.max(axis=1).astype(np.float32)
+    span = np.maximum((hi - lo).astype(np.float32), np.float32(1e-12)).astype(np.float32)
+    scale = (span / np.float32(15.0)).astype(np.float32)
+    inv = (np.float32(1.0) / scale).astype(np.float32)
+    t = ((e - lo[:, None]).astype(np.float32) * inv[:, None]).astype(np.float32)
+    t = (t + np.float32(0.5)).astype(np.float32)
+    code = np.clip(np.floor(t), 0, 15).astype(np.uint8)
+    n, dim = x.shape
+    out = np.zeros((n, (dim + 1) // 2), dtype=np.uint8)
+    out[:, :(dim + 1) // 2] |= code[:, 0::2]
+    out[:, :dim // 2] |= (code[:, 1::2] << 4).astype(np.uint8)
+    return out, np.stack([scale, lo], axis=1).astype(np.float32)
+
+
 def decode_sq(codes, bits, scale, zero, dim):
     """code * scale + zero per dimension in float32, two roundings
     (demo_quantize_l1_vector's dequantized vector)."""

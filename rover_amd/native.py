@@ -135,6 +135,13 @@ SIGNATURES = [
     ("rover_sq_split_bias_load", _I, [_H, _P, C.c_int64, _P]),
     ("rover_pure_pq_error4_load", _I, [_H, _P, C.c_int64, _P]),
     ("rover_pure_pq_error4_unload", _I, [_H]),
+    ("rover_sq_error4_load", _I, [_H, _P, C.c_int64, _P]),
+    ("rover_sq_error4_unload", _I, [_H]),
+    ("rover_pure_pq_decode", _I, [_H, _P, _I, _P]),
+    ("rover_sq_decode", _I, [_H, _P, _I, _P]),
+    ("rover_trace_open", _I, [C.c_char_p, _I, C.POINTER(_P)]),
+    ("rover_trace_query", _I, [_P, _I, _I, _I, _I, _P, _P, _I, _P, _I, _I, _P, _P, _I]),
+    ("rover_trace_close", _I, [_P]),
     ("rover_sha256", None, [_P, C.c_size_t, _P]),
     ("rover_pure_pq_router_load", _I, [_H, _P, _I, _P, _I]),
     ("rover_route_entries", _I, [_H, _P, _I, _I, _P, _P]),
@@ -327,6 +334,42 @@ def results_fingerprint(ids_rows, dist_rows, counts, h=FINGERPRINT_BASIS):
         if c > 0:
             h = result_fingerprint(ids_rows[q, :c], dist_rows[q, :c], h)
     return h
+
+
+TRACE_TEXT, TRACE_P15 = 1, 2
+
+
+class Trace:
+    """A refine trace file (rover_trace_open): TRACE_TEXT, the pure-PQ demo's
+    text trace, or TRACE_P15, the SQ pipeline's binary trace."""
+
+    def __init__(self, path, fmt):
+        p = C.c_void_p()
+        check(lib().rover_trace_open(str(path).encode(), fmt, C.byref(p)), "rover_trace_open")
+        self.ptr = p.value
+
+    def query(self, pool_size, query_index, ids, dists, out_ids, out_dists, refine_budget=0,
+              entry_points=0, decoded=None):
+        i = np.ascontiguousarray(ids, dtype=np.int32)
+        d = np.ascontiguousarray(dists, dtype=np.float32)
+        oi = np.ascontiguousarray(out_ids, dtype=np.int32)
+        od = np.ascontiguousarray(out_dists, dtype=np.float32)
+        v = None if decoded is None else np.ascontiguousarray(decoded, dtype=np.float32)
+        check(lib().rover_trace_query(
+            self.ptr, int(entry_points), int(pool_size), int(query_index), int(refine_budget),
+            ptr(i), ptr(d), len(i), ptr(v), 0 if v is None else v.shape[0],
+            0 if v is None else v.shape[1], ptr(oi), ptr(od), len(oi)), "rover_trace_query")
+
+    def close(self):
+        if self.ptr is not None:
+            check(lib().rover_trace_close(self.ptr), "rover_trace_close")
+        self.ptr = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 class Event:

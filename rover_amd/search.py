@@ -96,6 +96,35 @@ class Calculator:
         check(native.lib().rover_pure_pq_error4_load(self.h, ptr(a), a.size, ptr(e)),
               "rover_pure_pq_error4_load")
 
+    def sq_error4_load(self, artifact, expected_sha256=None):
+        """Loads an SQERR4V1 artifact (file image) for the SQ traversals'
+        final refine in ERROR4 mode."""
+        a = np.frombuffer(artifact, dtype=np.uint8) if isinstance(artifact, (bytes, bytearray)) \
+            else np.ascontiguousarray(artifact, dtype=np.uint8)
+        e = None if expected_sha256 is None else np.frombuffer(bytes(expected_sha256), dtype=np.uint8)
+        if e is not None and e.size != 32:
+            raise ValueError("expected_sha256 must be 32 bytes")
+        check(native.lib().rover_sq_error4_load(self.h, ptr(a), a.size, ptr(e)),
+              "rover_sq_error4_load")
+
+    def pure_pq_decode(self, ids):
+        """Reconstructed vectors [len(ids)][dim] (PQ + residual decode)."""
+        i = np.ascontiguousarray(ids, dtype=np.int32)
+        out = np.zeros((len(i), self.pq["dim"]), dtype=np.float32)
+        check(native.lib().rover_pure_pq_decode(self.h, ptr(i), len(i), ptr(out)),
+              "rover_pure_pq_decode")
+        return out
+
+    def sq_decode(self, ids):
+        """Reconstructed vectors [len(ids)][dim] (SQ + residual decode)."""
+        i = np.ascontiguousarray(ids, dtype=np.int32)
+        out = np.zeros((len(i), self.sq["dim"]), dtype=np.float32)
+        check(native.lib().rover_sq_decode(self.h, ptr(i), len(i), ptr(out)), "rover_sq_decode")
+        return out
+
+    def sq_error4_unload(self):
+        check(native.lib().rover_sq_error4_unload(self.h), "rover_sq_error4_unload")
+
     def pure_pq_error4_unload(self):
         check(native.lib().rover_pure_pq_error4_unload(self.h), "rover_pure_pq_error4_unload")
 

@@ -6,6 +6,7 @@ oracle/_ref/librover_ref.so from it):
     python tests/golden/make_golden.py           # every fixture
     python tests/golden/make_golden.py error4    # only the *_error4 files
     python tests/golden/make_golden.py r8        # only pq_d128_m32_r8 (+ error4)
+    python tests/golden/make_golden.py sq_refine # only sq_refine_d128
 
 Every expected output in the fixtures comes from the reference's own scalar
 C: retrieval_pure_pq_search_scalar_with_policy, retrieval_pq_build_lut,
@@ -448,6 +449,58 @@ def error4_case(name):
     print("wrote", path, os.path.getsize(path), "bytes")
 
 
+def sq_refine_case(name="sq_refine_d128", pq_name="pq_d128_m32", sq_name="sq_d128",
+                   sqmd_name="sqmd_d128"):
+    """The SQ final refine (SQERR4V1): for SQ8 with 4-bit residuals and SQ4
+    with 8-bit residuals, the reference builds the error4 artifact over its
+    SQ decoder (retrieval_build_error4, pq = 0) and refines the SQMD
+    traversal's outputs with retrieval_final_refine in CURRENT mode (also
+    with a permuted original-id map) and ERROR4 mode."""
+    ref = pyoracle.reference()
+    g = dict(np.load(os.path.join(HERE, pq_name + ".npz")))
+    s = dict(np.load(os.path.join(HERE, sq_name + ".npz")))
+    z = dict(np.load(os.path.join(HERE, sqmd_name + ".npz")))
+    base = (g["base_x16"].astype(np.float32) / np.float32(16.0))
+    queries = (g["queries_x16"].astype(np.float32) / np.float32(16.0))
+    n, dim = base.shape
+    rng = np.random.default_rng(5)
+    perm = rng.permutation(n).astype(np.int32)
+    out = dict(perm=perm)
+    for b in (8, 4):
+        codes = np.ascontiguousarray(s["sq%d_codes" % b])
+        res = np.ascontiguousarray(z["sq%d_res_codes" % b])
+        rqp = np.ascontiguousarray(z["sq%d_res_qparams" % b], dtype=np.float32)
+        rb = int(z["sq%d_res_bits" % b])
+        h = ref.ref_sq_refine_open(_p(base), _p(codes), b, float(s["sq%d_scale" % b]),
+                                   float(s["sq%d_zero" % b]), _p(res), rb, _p(rqp), n, dim)
+        assert h
+        size = ref.ref_sq_refine_artifact(h, None, 0)
+        art = np.zeros(size, dtype=np.uint8)
+        assert ref.ref_sq_refine_artifact(h, _p(art), size) == size
+        out["sq%d_artifact" % b] = art
+        cand = z["sq%d_c1_e32_x32_L64_ids" % b]
+        counts = z["sq%d_c1_e32_x32_L64_counts" % b]
+        for F in (20, 64):
+            for mode, tag, mp in ((1, "current", None), (1, "current_perm", perm),
+                                  (2, "error4", None)):
+                ids_all, d_all = [], []
+                for qi, q in enumerate(queries):
+                    c = min(int(counts[qi]), F)
+                    ids = np.ascontiguousarray(cand[qi][:c], dtype=np.int32).copy()
+                    d = np.zeros(c, dtype=np.float32)
+                    rc = ref.ref_sq_refine(h, mode, _p(q), _p(ids), _p(d), c, 10,
+                                           None if mp is None else _p(mp))
+                    assert rc == min(10, c), (b, tag, rc)
+                    ids_all.append(ids[:10])
+                    d_all.append(d[:10])
+                out["sq%d_F%d_%s_ids" % (b, F, tag)] = np.stack(ids_all)
+                out["sq%d_F%d_%s_dists" % (b, F, tag)] = np.stack(d_all)
+        ref.ref_sq_refine_close(h)
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
 def pq_r8_case():
     """The measured scorer's shape: M = 32 subspaces of 4 dims, 256
     centroids, 8-bit residuals, 32 neighbors per expansion (the two-lane
@@ -475,6 +528,9 @@ def main():
     if sys.argv[1:] == ["rnn"]:
         rnn_case("rnn_build")
         return
+    if sys.argv[1:] == ["sq_refine"]:
+        sq_refine_case()
+        return
     if sys.argv[1:] == ["r8"]:
         pq_r8_case()
         return
@@ -494,6 +550,7 @@ def main():
     for name in ("pq_d128_m32", "pq_d96_m48", "pq_d64_ks16"):
         error4_case(name)
     pq_r8_case()
+    sq_refine_case()
     sqmd_case("sqmd_d128")
     split_case("sqmd_split")
     graph_case("graph_build")

@@ -232,3 +232,66 @@ def test_mid_scale_refine_vs_oracle(mid_index, calc):
         assert np.array_equal(cnt, ocnt)
         assert np.array_equal(ids, oids)
         assert np.array_equal(bits(d), bits(od))
+
+
+@pytest.mark.parametrize("name", PQ_SETS)
+def test_decode_matches_reference(golden, calc, name):
+    """rover_pure_pq_decode equals the reference's residual decoder
+    (retrieval_pq_residual_query_decode) on the fixture's sampled ids, bit
+    for bit."""
+    g = golden(name)
+    say("case", name, "decode")
+    calc.load_pq_index(pq_data(g))
+    ids = g["adc_sample"][:16]
+    got = calc.pure_pq_decode(ids)
+    assert np.array_equal(bits(got), bits(g["decoded"]))
+    with pytest.raises(native.RoverError):
+        calc.pure_pq_decode([len(g["graph"])])
+
+
+def test_refine_text_trace(golden, calc, tmp_path):
+    """The pure-PQ demo's refine trace written from GPU results: Q/C lines
+    carry the search pool, V lines the decoded vectors of the refine
+    candidates, O lines the refined top-k; every number equals the oracle's
+    (search, residual decode, refine)."""
+    import pyoracle
+    g = golden("pq_d128_m32_r8")
+    calc.load_pq_index(pq_data(g))
+    n = len(g["graph"])
+    L, F, K = 64, 20, 10
+    ents = native.make_seed_ids(n, 32, int(g["entry"]))
+    ids, d, cnt = calc.search_batch(g["queries"], L, L, correction=1, entry_points=32)
+    rid, rd, rcnt = calc.search_batch(g["queries"], L, K, correction=1, entry_points=32,
+                                      refine_candidates=F)
+    path = tmp_path / "refine.trace"
+    with native.Trace(path, native.TRACE_TEXT) as t:
+        for qi in range(len(g["queries"])):
+            c = int(cnt[qi])
+            t.query(L, qi, ids[qi, :c], d[qi, :c], rid[qi, :K], rd[qi, :K], refine_budget=F,
+                    decoded=calc.pure_pq_decode(ids[qi, :min(c, F)]))
+    oix = pyoracle.Index.from_pq(rindex.PqIndexData(g["graph"], int(g["entry"]), g["codes"],
+                                                    g["codebook"], g["res_codes"],
+                                                    g["res_qparams"], int(g["res_bits"])))
+    lines = path.read_text().splitlines()
+    at = 0
+    for qi, q in enumerate(g["queries"]):
+        oi, od, _ = pyoracle.search(oix, pyoracle.SCORE_ONLINE, q, ents, 32, L, L)
+        head = lines[at].split()
+        assert head == ["Q", str(L), str(qi), str(len(oi)), str(F)]
+        at += 1
+        for i in range(len(oi)):
+            tag, a, b = lines[at].split()
+            assert tag == "C" and int(a) == oi[i] and float.fromhex(b) == float(od[i])
+            at += 1
+        for i in range(min(len(oi), F)):
+            f = lines[at].split()
+            want = pyoracle.residual_decode(oix, int(oi[i]))
+            assert f[0] == "V" and int(f[1]) == oi[i]
+            assert np.array_equal(np.array([float.fromhex(x) for x in f[2:]], np.float32), want)
+            at += 1
+        fi, fd = pyoracle.refine(oix, q, oi[:F], K)
+        for i in range(K):
+            tag, a, b = lines[at].split()
+            assert tag == "O" and int(a) == fi[i] and float.fromhex(b) == float(fd[i])
+            at += 1
+    assert at == len(lines)

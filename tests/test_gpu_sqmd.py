@@ -290,3 +290,60 @@ def test_retrieval_load_dma_equals_copy_load(fx):
                 c.h, g["graph"].ctypes.data, ptrs, 1, n - 1, n, dim, g["graph"].shape[1],
                 entry, float(sc), float(zr))
             assert rc == native.ROVER_EBADPARM
+
+
+@pytest.mark.parametrize("b", [8, 4])
+def test_sq_final_refine_matches_reference(fx, b):
+    """The SQ traversal's fused final refine (FP64 over the SQ + residual
+    decode, and with the SQERR4V1 level in ERROR4 mode) equals the
+    reference's retrieval_final_refine on the same candidates
+    (tests/golden/sq_refine_d128.npz), ids and distance bits; the loader
+    refuses a PQ artifact, a changed payload and a wrong file digest."""
+    import hashlib
+    g, s, z, queries = fx
+    r = load("sq_refine_d128")
+    dim = queries.shape[1]
+    art = r["sq%d_artifact" % b].tobytes()
+    with Calculator() as calc:
+        load_sq(calc, g, s, z, b, dim)
+        calc.sq_error4_load(art, hashlib.sha256(art).digest())
+        for F in (20, 64):
+            for mode, tag in ((native.REFINE_CURRENT, "current"),
+                              (native.REFINE_ERROR4, "error4")):
+                say("case sq refine sq%d F=%d %s" % (b, F, tag))
+                ids, d, cnt = calc.search_batch(queries, 64, 10, method=native.METHOD_SQ,
+                                                correction=1, entry_points=32,
+                                                expand_count=32, traversal=SQMD,
+                                                refine_candidates=F, refine_mode=mode)
+                key = "sq%d_F%d_%s" % (b, F, tag)
+                assert np.all(cnt == 10)
+                assert np.array_equal(ids, r[key + "_ids"])
+                assert np.array_equal(bits(d), bits(r[key + "_dists"]))
+        bad = bytearray(art)
+        bad[300] ^= 1
+        for image in (bytes(bad), b"PQERR4V1" + art[8:]):
+            with pytest.raises(native.RoverError) as e:
+                calc.sq_error4_load(image)
+            assert e.value.code == native.ROVER_EBADPARM
+        with pytest.raises(native.RoverError):
+            calc.sq_error4_load(art, b"\0" * 32)
+
+
+@pytest.mark.parametrize("b", [8, 4])
+def test_sq_decode_matches_oracle(fx, b):
+    """rover_sq_decode equals the oracle's SQ decoder (retrieval_final_refine_
+    decode_sq restated, itself pinned by the reference's SQERR4V1 build) on
+    every id, bit for bit."""
+    g, s, z, queries = fx
+    dim = queries.shape[1]
+    with Calculator() as calc:
+        load_sq(calc, g, s, z, b, dim)
+        ids = np.arange(len(g["graph"]), dtype=np.int32)
+        got = calc.sq_decode(ids)
+    oix = pyoracle.Index(g["graph"], sq_codes=s["sq%d_codes" % b], sq_bits=b,
+                         sq_scale=float(s["sq%d_scale" % b]), sq_zero=float(s["sq%d_zero" % b]),
+                         res_codes=z["sq%d_res_codes" % b],
+                         res_qparams=z["sq%d_res_qparams" % b],
+                         res_bits=int(z["sq%d_res_bits" % b]), dim=dim)
+    for i in range(0, len(ids), 7):
+        assert np.array_equal(bits(got[i]), bits(pyoracle.sq_decode(oix, i)))
