@@ -595,8 +595,8 @@ AEEResult rover_build_error4_encode(const float *d_x, const uint8_t *d_codes,
  * ties by id) merged with the node's random initial row (LCG from `seed`),
  * then the RNG occlusion rule (prune_mode 1) or the closest r (0), fill
  * from the remaining candidates, random fill, the row by (distance, id).
- * d_x [n][dim] and d_graph [n][r] are device pointers; r and candidates
- * <= 64, r < n; synchronous. */
+ * d_x [n][dim] and d_graph [n][r] are device pointers; r <= 64, r < n,
+ * r <= candidates <= 128; synchronous. */
 AEEResult rover_build_graph_exact(const float *d_x, int n, int dim, int r,
                                   int candidates, uint32_t seed, int prune_mode,
                                   int32_t *d_graph);

@@ -83,6 +83,36 @@ __device__ __forceinline__ uint64_t rb_merge64(uint64_t cur, uint64_t fresh) {
     return m;
 }
 
+// The bitonic sequence m (one key per lane) sorted ascending.
+__device__ __forceinline__ uint64_t rb_bitonic_sort64(uint64_t m) {
+    m = rv_bitonic_step<64, 32>(m);
+    m = rv_bitonic_step<64, 16>(m);
+    m = rv_bitonic_step<64, 8>(m);
+    m = rv_bitonic_step<64, 4>(m);
+    m = rv_bitonic_step<64, 2>(m);
+    m = rv_bitonic_step<64, 1>(m);
+    return m;
+}
+
+// Merges 64 new keys into the ascending 128-key list (lo: ranks 0..63, hi:
+// ranks 64..127, one key of each per lane), keeping the 128 smallest.  The
+// 64 smallest of lo and the new keys are the new lo (every hi key is at
+// least every lo key); the 64 largest of them, merged with hi, give the new
+// hi.
+__device__ __forceinline__ void rb_merge128(uint64_t &lo, uint64_t &hi, uint64_t fresh) {
+    fresh = rv_sort_wave(fresh);
+    const int src = 63 - __lane_id();
+    const uint64_t rev = ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(fresh >> 32), src) << 32) |
+                         (uint32_t)__shfl((int)(uint32_t)fresh, src);
+    const uint64_t a = lo < rev ? lo : rev;
+    const uint64_t spill = rb_bitonic_sort64(lo < rev ? rev : lo);
+    lo = rb_bitonic_sort64(a);
+    const uint64_t srev =
+        ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(spill >> 32), src) << 32) |
+        (uint32_t)__shfl((int)(uint32_t)spill, src);
+    hi = rb_bitonic_sort64(hi < srev ? hi : srev);
+}
+
 // Exact squared L2 of two rows in global memory, the reference's order.
 __device__ __forceinline__ float rb_l2(const float *x, int dim, int a, int b) {
     const float *va = x + (size_t)a * dim, *vb = x + (size_t)b * dim;
@@ -132,7 +162,8 @@ __device__ __forceinline__ void rb_stage(const float *x, int n, int dim, int r0,
 // LDS in chunks of RB_DC in ascending order, and each lane carries its
 // RB_ROWS partial sums from chunk to chunk, so every distance is still one
 // sequential sum over the dimensions.
-template <bool VEC>
+// KEEP = 64 or 128 closest per node (keys [n][KEEP]).
+template <bool VEC, int KEEP>
 __global__ __launch_bounds__(RB_WAVE *RB_WAVES) void rv_knn_exact_kernel(const float *x, int n,
                                                                          int dim,
                                                                          uint64_t *keys) {
@@ -146,9 +177,9 @@ __global__ __launch_bounds__(RB_WAVE *RB_WAVES) void rv_knn_exact_kernel(const f
     const int nchunk = (dim + RB_DC - 1) / RB_DC;
     const float *col = tile + lane * STRIDE;
     const float *mine = rows + wave * RB_ROWS * RB_DC;
-    uint64_t best[RB_ROWS];
+    uint64_t best[RB_ROWS], best_hi[RB_ROWS];
 #pragma unroll
-    for (int r = 0; r < RB_ROWS; r++) best[r] = RB_KEY_MAX;
+    for (int r = 0; r < RB_ROWS; r++) best[r] = best_hi[r] = RB_KEY_MAX;
     for (int c0 = 0; c0 < n; c0 += RB_TILE) {
         float acc[RB_ROWS];
 #pragma unroll
@@ -188,18 +219,25 @@ __global__ __launch_bounds__(RB_WAVE *RB_WAVES) void rv_knn_exact_kernel(const f
         for (int r = 0; r < RB_ROWS; r++) {
             const int u = u0 + wave * RB_ROWS + r;
             const uint64_t key = v < n && v != u ? rb_key(acc[r], v) : RB_KEY_MAX;
-            // the current 64th key is lane 63's; merge only when the tile
+            // the current last key is lane 63's; merge only when the tile
             // can improve the list
+            const uint64_t last = KEEP == 128 ? best_hi[r] : best[r];
             const uint64_t lim =
-                ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(best[r] >> 32), 63) << 32) |
-                (uint32_t)__shfl((int)(uint32_t)best[r], 63);
-            if (__ballot(key < lim)) best[r] = rb_merge64(best[r], key);
+                ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(last >> 32), 63) << 32) |
+                (uint32_t)__shfl((int)(uint32_t)last, 63);
+            if (__ballot(key < lim)) {
+                if (KEEP == 128) rb_merge128(best[r], best_hi[r], key);
+                else best[r] = rb_merge64(best[r], key);
+            }
         }
     }
 #pragma unroll
     for (int r = 0; r < RB_ROWS; r++) {
         const int u = u0 + wave * RB_ROWS + r;
-        if (u < n) keys[(size_t)u * RB_WAVE + lane] = best[r];
+        if (u < n) {
+            keys[(size_t)u * KEEP + lane] = best[r];
+            if (KEEP == 128) keys[(size_t)u * KEEP + RB_WAVE + lane] = best_hi[r];
+        }
     }
 }
 
@@ -238,13 +276,13 @@ __device__ static void rb_draw(int n, int r, uint32_t state, int u, int32_t *ids
 }
 
 __global__ __launch_bounds__(RB_WAVE) void rv_build_rows_kernel(const float *x, int n, int dim,
-                                                                int r, int cand, uint32_t seed,
-                                                                int prune_mode,
+                                                                int r, int cand, int keep,
+                                                                uint32_t seed, int prune_mode,
                                                                 const uint64_t *knn,
                                                                 int32_t *graph) {
     __shared__ int32_t s_ids[RB_WAVE];        // drawn ids (initial row / random fill)
-    __shared__ uint64_t s_lists[2 * RB_WAVE];  // exact candidates | initial row
-    __shared__ uint64_t s_items[2 * RB_WAVE];  // candidate keys, ascending
+    __shared__ uint64_t s_lists[3 * RB_WAVE];  // exact candidates [128] | initial row [64]
+    __shared__ uint64_t s_items[3 * RB_WAVE];  // candidate keys, ascending
     __shared__ uint64_t s_row[RB_WAVE];        // accepted keys
     __shared__ int s_count;
     const int lane = (int)__lane_id();
@@ -260,10 +298,13 @@ __global__ __launch_bounds__(RB_WAVE) void rv_build_rows_kernel(const float *x, 
         uint64_t init = lane < ni ? rb_key(rb_l2(x, dim, u, s_ids[lane]), s_ids[lane])
                                   : RB_KEY_MAX;
         init = rv_sort_wave(init);
-        const uint64_t prov = lane < cand ? knn[(size_t)u * RB_WAVE + lane] : RB_KEY_MAX;
+        const uint64_t prov = lane < cand ? knn[(size_t)u * keep + lane] : RB_KEY_MAX;
+        const uint64_t prov2 =
+            RB_WAVE + lane < cand ? knn[(size_t)u * keep + RB_WAVE + lane] : RB_KEY_MAX;
         __syncthreads();
         s_lists[lane] = prov;
-        s_lists[RB_WAVE + lane] = init;
+        s_lists[RB_WAVE + lane] = prov2;
+        s_lists[2 * RB_WAVE + lane] = init;
         __syncthreads();
         // ---- candidates: merge the two ascending lists, dropping the
         // initial row's ids that are exact candidates (same key)
@@ -271,7 +312,7 @@ __global__ __launch_bounds__(RB_WAVE) void rv_build_rows_kernel(const float *x, 
             int i = 0, j = 0, cnt = 0;
             while (i < cand || j < ni) {
                 const uint64_t a = i < cand ? s_lists[i] : RB_KEY_MAX;
-                const uint64_t b = j < ni ? s_lists[RB_WAVE + j] : RB_KEY_MAX;
+                const uint64_t b = j < ni ? s_lists[2 * RB_WAVE + j] : RB_KEY_MAX;
                 if (a == RB_KEY_MAX && b == RB_KEY_MAX) break;
                 if (a < b) {
                     s_items[cnt++] = a, i++;
@@ -342,24 +383,27 @@ __global__ __launch_bounds__(RB_WAVE) void rv_build_rows_kernel(const float *x, 
 extern "C" int rv_build_exact(const float *x, int n, int dim, int r, int cand, uint32_t seed,
                               int prune_mode, int32_t *graph, void *stream) {
     if (!x || !graph || n <= 1 || dim <= 0 || dim > 4096 || r <= 0 || r >= n ||
-        r > RB_WAVE || cand < r || cand > RB_WAVE || (prune_mode != 0 && prune_mode != 1)) {
+        r > RB_WAVE || cand < r || cand > 2 * RB_WAVE || (prune_mode != 0 && prune_mode != 1)) {
         rv_set_error("rv_build_exact: bad arguments");
         return -1;
     }
     hipStream_t s = (hipStream_t)stream;
     uint64_t *keys = nullptr;
-    if (rb_check(hipMalloc((void **)&keys, (size_t)n * RB_WAVE * sizeof(uint64_t)),
+    const int keep = cand > RB_WAVE ? 2 * RB_WAVE : RB_WAVE;
+    if (rb_check(hipMalloc((void **)&keys, (size_t)n * keep * sizeof(uint64_t)),
                  "hipMalloc knn"))
         return -1;
     const int blocks = (n + RB_WAVES * RB_ROWS - 1) / (RB_WAVES * RB_ROWS);
     int rc = 0;
-    if (dim % 4 == 0) {
-        rv_knn_exact_kernel<true><<<blocks, RB_WAVE * RB_WAVES, 0, s>>>(x, n, dim, keys);
-        rc = rb_check(hipGetLastError(), "rv_knn_exact_kernel");
-    } else {
-        rv_knn_exact_kernel<false><<<blocks, RB_WAVE * RB_WAVES, 0, s>>>(x, n, dim, keys);
-        rc = rb_check(hipGetLastError(), "rv_knn_exact_kernel");
-    }
+    if (dim % 4 == 0 && keep == RB_WAVE)
+        rv_knn_exact_kernel<true, 64><<<blocks, RB_WAVE * RB_WAVES, 0, s>>>(x, n, dim, keys);
+    else if (dim % 4 == 0)
+        rv_knn_exact_kernel<true, 128><<<blocks, RB_WAVE * RB_WAVES, 0, s>>>(x, n, dim, keys);
+    else if (keep == RB_WAVE)
+        rv_knn_exact_kernel<false, 64><<<blocks, RB_WAVE * RB_WAVES, 0, s>>>(x, n, dim, keys);
+    else
+        rv_knn_exact_kernel<false, 128><<<blocks, RB_WAVE * RB_WAVES, 0, s>>>(x, n, dim, keys);
+    rc = rb_check(hipGetLastError(), "rv_knn_exact_kernel");
     if (!rc) {
         int dev = 0, cus = 0;
         rc = rb_check(hipGetDevice(&dev), "hipGetDevice") ||
@@ -368,8 +412,8 @@ extern "C" int rv_build_exact(const float *x, int n, int dim, int r, int cand, u
         if (!rc) {
             int grid = cus * 16;
             if (grid > n) grid = n;
-            rv_build_rows_kernel<<<grid, RB_WAVE, 0, s>>>(x, n, dim, r, cand, seed, prune_mode,
-                                                          keys, graph);
+            rv_build_rows_kernel<<<grid, RB_WAVE, 0, s>>>(x, n, dim, r, cand, keep, seed,
+                                                          prune_mode, keys, graph);
             rc = rb_check(hipGetLastError(), "rv_build_rows_kernel");
         }
     }

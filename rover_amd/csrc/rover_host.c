@@ -2133,7 +2133,7 @@ AEEResult rover_build_graph_exact(const float *d_x, int n, int dim, int r,
                                   int candidates, uint32_t seed, int prune_mode,
                                   int32_t *d_graph) {
     if (!d_x || !d_graph || n <= 1 || dim <= 0 || dim > 4096 || r <= 0 || r >= n ||
-        r > 64 || candidates < r || candidates > 64 || (prune_mode != 0 && prune_mode != 1))
+        r > 64 || candidates < r || candidates > 128 || (prune_mode != 0 && prune_mode != 1))
         return ROVER_EBADPARM;
     if (rv_build_exact(d_x, n, dim, r, candidates, seed, prune_mode, d_graph, NULL)) {
         set_err("rover_build_graph_exact");

@@ -29,7 +29,9 @@ def test_device_build_matches_reference_fixture(golden):
 
 @pytest.mark.parametrize("spec", [(5000, 128, 32, 64, 1), (3000, 96, 48, 64, 1),
                                   (4000, 30, 16, 48, 0), (2000, 200, 32, 64, 1),
-                                  (1500, 131, 16, 32, 1), (777, 260, 24, 40, 0)])
+                                  (1500, 131, 16, 32, 1), (777, 260, 24, 40, 0),
+                                  (3000, 96, 48, 96, 1), (2500, 64, 32, 128, 1),
+                                  (1800, 100, 64, 100, 0)])
 def test_device_build_matches_oracle(spec):
     n, dim, r, cand, prune = spec
     say("case graph oracle", spec)

@@ -26,7 +26,8 @@ def test_oracle_matches_reference_fixture(golden):
 
 @pytest.mark.skipif(not pyoracle.ref_available(), reason="compiled reference not built")
 @pytest.mark.parametrize("spec", [(900, 24, 16, 40, 1, 7), (700, 12, 20, 20, 1, 3),
-                                  (500, 64, 32, 64, 0, 11)])
+                                  (500, 64, 32, 64, 0, 11), (800, 96, 48, 96, 1, 5),
+                                  (600, 40, 24, 128, 0, 9)])
 def test_oracle_matches_live_reference(spec):
     n, dim, r, cand, prune, seed = spec
     base, _ = data.workload(n, dim, 8, 8.0, 1, base_seed=seed, query_seed=seed + 1)
