@@ -601,6 +601,27 @@ AEEResult rover_build_graph_exact(const float *d_x, int n, int dim, int r,
                                   int candidates, uint32_t seed, int prune_mode,
                                   int32_t *d_graph);
 
+/* rover_build_graph_exact's graph (the same bits) for large n, computed over
+ * a clustering the caller supplies: d_assign [n] cluster ids in
+ * [0, num_clusters), d_centroids [num_clusters][dim] (any points; k-means
+ * centroids make it fast).  Each node's candidates come from its own
+ * cluster and from every other cluster that a triangle-inequality bound
+ * (with margins covering f32 rounding) cannot rule out, so the candidate
+ * lists, and the graph, equal the brute-force ones for any clustering.
+ * The reference has one provider (retrieval_search_build_core.c:314-361,
+ * an O(n^2) scan); this one keeps its results where n^2 is out of reach.
+ * stats (may be NULL) [4]: node blocks, cluster pairs screened in, (node,
+ * cluster) pairs searched, nodes with such pairs.  candidates < n;
+ * num_clusters <= ROVER_BUILD_MAX_CLUSTERS; ROVER_EFAILED when the
+ * clustering leaves more than 2^24 cluster pairs or 2^27 (node, cluster)
+ * pairs to search.  Synchronous. */
+#define ROVER_BUILD_MAX_CLUSTERS 32768
+AEEResult rover_build_graph_exact_clustered(const float *d_x, int n, int dim, int r,
+                                            int candidates, uint32_t seed, int prune_mode,
+                                            const int32_t *d_assign, const float *d_centroids,
+                                            int num_clusters, int32_t *d_graph,
+                                            int64_t *stats);
+
 /* RNN-Descent graph construction on the device, bit-identical to the
  * reference's retrieval_rnn_build_graph_ex
  * (retrieval_rnn_descent_core.c:1432-1548) followed by

@@ -380,6 +380,28 @@ __global__ __launch_bounds__(RB_WAVE) void rv_build_rows_kernel(const float *x, 
     }
 }
 
+// The rows of the search-build from precomputed exact candidate keys
+// (keys [n][keep], the first `cand` of each row used), on `stream`.
+extern "C" int rv_rows_from_keys(const float *x, int n, int dim, int r, int cand,
+                                 uint32_t seed, int prune_mode, const uint64_t *keys,
+                                 int keep, int32_t *graph, void *stream) {
+    int dev = 0, cus = 0;
+    if (cand > keep || cand > 2 * RB_WAVE || r > RB_WAVE || r > cand) {
+        rv_set_error("rv_rows_from_keys: bad arguments");
+        return -1;
+    }
+    if (rb_check(hipGetDevice(&dev), "hipGetDevice") ||
+        rb_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
+                 "hipDeviceGetAttribute"))
+        return -1;
+    int grid = cus * 16;
+    if (grid > n) grid = n;
+    rv_build_rows_kernel<<<grid, RB_WAVE, 0, (hipStream_t)stream>>>(x, n, dim, r, cand, keep,
+                                                                    seed, prune_mode, keys,
+                                                                    graph);
+    return rb_check(hipGetLastError(), "rv_build_rows_kernel");
+}
+
 extern "C" int rv_build_exact(const float *x, int n, int dim, int r, int cand, uint32_t seed,
                               int prune_mode, int32_t *graph, void *stream) {
     if (!x || !graph || n <= 1 || dim <= 0 || dim > 4096 || r <= 0 || r >= n ||

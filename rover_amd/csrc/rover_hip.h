@@ -178,11 +178,27 @@ int rv_sqmd_launch(const RvIndex *index, const RvSqmdBatch *batch, int grid_wave
  * retrieval_search_build_graph_ex with the exact provider and one iteration:
  * exact candidates (the `candidates` closest by (distance, id)), the
  * node's random initial row, RNG occlusion (prune_mode 1) or the closest r
- * (0), fill, random fill, rows by (distance, id).  r, candidates <= 64.
- * d_graph [n][r].  Asynchronous on `stream`; allocates a scratch of
- * n * 512 bytes (freed before returning, after a synchronize). */
+ * (0), fill, random fill, rows by (distance, id).  r <= 64, candidates
+ * <= 128.  d_graph [n][r].  Allocates a scratch of n * 512 (candidates <= 64)
+ * or n * 1024 bytes, freed after a synchronize of `stream`. */
 int rv_build_exact(const float *d_x, int n, int dim, int r, int candidates,
                    uint32_t seed, int prune_mode, int32_t *d_graph, void *stream);
+/* The rows of the build above from precomputed candidate keys d_keys
+ * [n][keep] ((distance bits << 32) | id, ascending; the first `candidates`
+ * used), enqueued on `stream`. */
+int rv_rows_from_keys(const float *d_x, int n, int dim, int r, int candidates,
+                      uint32_t seed, int prune_mode, const uint64_t *d_keys, int keep,
+                      int32_t *d_graph, void *stream);
+/* rv_build_exact's graph computed over a caller-supplied clustering (d_assign
+ * [n] cluster ids in [0, nc), d_centroids [nc][dim]): own-cluster candidate
+ * lists, then every other cluster that a triangle-inequality bound does not
+ * rule out, so the graph is the brute-force one for any clustering.
+ * out_stats (may be NULL) [4]: node blocks, cluster pairs screened in, (node,
+ * cluster) pairs searched, nodes with such pairs.  Synchronous. */
+int rv_build_exact_clustered(const float *d_x, int n, int dim, int r, int candidates,
+                             uint32_t seed, int prune_mode, const int32_t *d_assign,
+                             const float *d_centroids, int nc, int32_t *d_graph,
+                             int64_t *out_stats, void *stream);
 /* RNN-Descent on the device (synchronous): out d_graph [n][top_k]. */
 int rv_build_rnn(const float *d_x, int n, int dim, int s, int r, int t1, int t2,
                  uint32_t seed, int top_k, int32_t *d_graph, void *stream);

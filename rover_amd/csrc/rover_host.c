@@ -2142,6 +2142,24 @@ AEEResult rover_build_graph_exact(const float *d_x, int n, int dim, int r,
     return ROVER_SUCCESS;
 }
 
+AEEResult rover_build_graph_exact_clustered(const float *d_x, int n, int dim, int r,
+                                            int candidates, uint32_t seed, int prune_mode,
+                                            const int32_t *d_assign, const float *d_centroids,
+                                            int num_clusters, int32_t *d_graph,
+                                            int64_t *stats) {
+    if (!d_x || !d_graph || !d_assign || !d_centroids || n <= 1 || dim <= 0 || dim > 4096 ||
+        r <= 0 || r >= n || r > 64 || candidates < r || candidates > 128 || candidates >= n ||
+        num_clusters <= 0 || num_clusters > ROVER_BUILD_MAX_CLUSTERS ||
+        (prune_mode != 0 && prune_mode != 1))
+        return ROVER_EBADPARM;
+    if (rv_build_exact_clustered(d_x, n, dim, r, candidates, seed, prune_mode, d_assign,
+                                 d_centroids, num_clusters, d_graph, stats, NULL)) {
+        set_err("rover_build_graph_exact_clustered");
+        return ROVER_EFAILED;
+    }
+    return ROVER_SUCCESS;
+}
+
 AEEResult rover_build_graph_rnn(const float *d_x, int n, int dim, int s, int r, int t1,
                                 int t2, uint32_t seed, int top_k, int32_t *d_graph) {
     if (!d_x || !d_graph || n <= 1 || dim <= 0 || s <= 0 || r <= 0 || r > 64 || s > r ||

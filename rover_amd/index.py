@@ -20,6 +20,8 @@ neighbours sorted by (distance, id) (:314-361), RNG occlusion pruning
 `d(kept, c) < d(u, c)` (:904-930), fill from the remaining candidates in
 order (:932-947), rows sorted by (distance, id).
 """
+import ctypes as C
+
 import numpy as np
 
 
@@ -759,4 +761,65 @@ def build_graph_exact(x, R, candidates=64, seed=42, prune_mode=1):
     native.check(native.lib().rover_build_graph_exact(dx.ptr, n, dim, R, candidates, seed,
                                                       prune_mode, dg.ptr),
                  "rover_build_graph_exact")
+    return dg.download(np.int32, n * R).reshape(n, R)
+
+
+def cluster_for_build(x, clusters, iters=8, sample=262144, seed=7, device="cuda",
+                      chunk=65536):
+    """A k-means clustering of x for build_graph_exact_clustered: Lloyd's
+    iterations on a uniform-stride sample (float32 on the device), then every
+    row assigned to its nearest centroid.  Returns (assign int32 [n],
+    centroids float32 [clusters, dim]).  Any clustering gives the same graph;
+    a tight one leaves fewer clusters to search per node."""
+    import torch
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    n, dim = x.shape
+    clusters = max(1, min(int(clusters), n))
+    tr = torch.from_numpy(x[uniform_stride_sample(n, min(n, sample))]).to(device)
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    init = torch.randperm(tr.shape[0], generator=g)[:clusters].to(device)
+    cent = tr[init].clone()
+
+    def nearest(pts):
+        d = (cent * cent).sum(1)[None, :] - 2.0 * pts @ cent.T
+        return torch.argmin(d, dim=1)
+
+    for _ in range(iters):
+        a = torch.cat([nearest(tr[i:i + chunk]) for i in range(0, tr.shape[0], chunk)])
+        _segment_means(torch, tr, a, clusters, cent)
+    assign = np.empty(n, dtype=np.int32)
+    for i in range(0, n, chunk):
+        pts = torch.from_numpy(x[i:i + chunk]).to(device)
+        assign[i:i + chunk] = nearest(pts).to(torch.int32).cpu().numpy()
+    return assign, cent.cpu().numpy()
+
+
+def build_graph_exact_clustered(x, R, candidates=64, seed=42, prune_mode=1, clusters=None,
+                                assign=None, centroids=None, stats=None):
+    """build_graph_exact's graph, bit for bit, computed over a clustering
+    (rover_build_graph_exact_clustered): own-cluster candidates plus every
+    cluster a triangle-inequality bound does not rule out.  assign/centroids
+    default to cluster_for_build(x, clusters or sqrt(n)).  stats (a list)
+    receives [node blocks, cluster pairs, (node, cluster) pairs, nodes with
+    pairs]."""
+    from . import native
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    n, dim = x.shape
+    if assign is None:
+        k = clusters if clusters is not None else max(1, int(np.sqrt(n)))
+        assign, centroids = cluster_for_build(x, k)
+    assign = np.ascontiguousarray(assign, dtype=np.int32)
+    centroids = np.ascontiguousarray(centroids, dtype=np.float32)
+    if assign.shape != (n,) or centroids.ndim != 2 or centroids.shape[1] != dim:
+        raise ValueError("assign must be [n] and centroids [clusters, dim]")
+    dx = native.DeviceBuffer.from_array(x)
+    da = native.DeviceBuffer.from_array(assign)
+    dc = native.DeviceBuffer.from_array(centroids)
+    dg = native.DeviceBuffer(n * R * 4)
+    st = (C.c_int64 * 4)()
+    native.check(native.lib().rover_build_graph_exact_clustered(
+        dx.ptr, n, dim, R, candidates, seed, prune_mode, da.ptr, dc.ptr, centroids.shape[0],
+        dg.ptr, st), "rover_build_graph_exact_clustered")
+    if stats is not None:
+        stats[:] = list(st)
     return dg.download(np.int32, n * R).reshape(n, R)

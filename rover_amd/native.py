@@ -170,6 +170,8 @@ SIGNATURES = [
     ("rover_build_pq_encode", _I, [_P, _I, _I, _I, _I, _P, _P]),
     ("rover_build_residual_encode", _I, [_P, _P, _I, _I, _I, _I, _P, _I, _P, _P]),
     ("rover_build_graph_exact", _I, [_P, _I, _I, _I, _I, C.c_uint32, _I, _P]),
+    ("rover_build_graph_exact_clustered", _I,
+     [_P, _I, _I, _I, _I, C.c_uint32, _I, _P, _P, _I, _P, _P]),
     ("rover_build_graph_rnn", _I, [_P, _I, _I, _I, _I, _I, _I, C.c_uint32, _I, _P]),
     ("rover_build_error4_encode", _I,
      [_P, _P, _P, _P, _I, _I, _I, _I, _P, _I, _P, _P]),

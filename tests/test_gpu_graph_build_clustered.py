@@ -1,0 +1,133 @@
+"""The clustered exact build (rover_build_graph_exact_clustered) against the
+brute-force build and the reference's outputs, row for row: k-means
+clusterings, clusterings that fit the data badly (random assignments, one
+cluster, empty clusters), exact duplicates split across clusters (ties by
+id), 96 and 128 candidates, and a 200k-node mixture."""
+import ctypes as C
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import pyoracle
+from rover_amd import data, native
+from rover_amd import index as rindex
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def random_clustering(x, nc, seed):
+    rng = np.random.default_rng(seed)
+    assign = rng.integers(0, nc, size=x.shape[0]).astype(np.int32)
+    cent = x[rng.integers(0, x.shape[0], size=nc)].astype(np.float32)
+    return assign, cent
+
+
+def test_clustered_matches_reference_fixture(golden):
+    z = golden("graph_build")
+    for ci, (n, dim, r, cand, prune, seed) in enumerate(z["cases"]):
+        n, cand = int(n), int(cand)
+        if cand >= n:
+            continue
+        say("case clustered fixture", ci, n, int(dim), int(r), cand, int(prune))
+        base = z["base%d" % ci]
+        assign, cent = rindex.cluster_for_build(base, 6, iters=4)
+        g = rindex.build_graph_exact_clustered(base, int(r), cand, int(seed), int(prune),
+                                               assign=assign, centroids=cent)
+        assert np.array_equal(g, z["graph%d" % ci]), ci
+
+
+@pytest.mark.parametrize("spec", [(5000, 128, 32, 64, 1, "kmeans", 24),
+                                  (3000, 96, 48, 96, 1, "kmeans", 16),
+                                  (4000, 30, 16, 48, 0, "random", 7),
+                                  (2500, 64, 32, 128, 1, "random", 5),
+                                  (1800, 100, 64, 100, 0, "kmeans", 40),
+                                  (1500, 131, 16, 32, 1, "one", 1),
+                                  (2000, 96, 48, 96, 1, "empty", 64)])
+def test_clustered_matches_oracle(spec):
+    n, dim, r, cand, prune, kind, nc = spec
+    say("case clustered oracle", spec)
+    base, _ = data.workload(n, dim, 32, 8.0, 1, base_seed=n + dim, query_seed=1)
+    if kind == "kmeans":
+        assign, cent = rindex.cluster_for_build(base, nc, iters=6)
+    elif kind == "random":
+        assign, cent = random_clustering(base, nc, n)
+    elif kind == "one":
+        assign = np.zeros(n, np.int32)
+        cent = base[:1].copy()
+    else:
+        # k-means on 8 clusters, ids spread over 64 so most clusters are empty
+        a8, c8 = rindex.cluster_for_build(base, 8, iters=4)
+        assign = (a8 * 8 + 3).astype(np.int32)
+        cent = np.zeros((nc, dim), np.float32)
+        cent[np.arange(8) * 8 + 3] = c8
+    st = []
+    g = rindex.build_graph_exact_clustered(base, r, cand, 42, prune, assign=assign,
+                                           centroids=cent, stats=st)
+    say("  stats", st)
+    want = pyoracle.build_exact(base, r, cand, 42, prune)
+    assert np.array_equal(g, want)
+
+
+def test_clustered_duplicates_across_clusters():
+    """Every row appears three times; the copies go to different clusters, so
+    the equal-distance keys that decide the lists (ties by id) come from the
+    certified search of other clusters."""
+    base, _ = data.workload(900, 64, 16, 8.0, 1, base_seed=5, query_seed=1)
+    x = np.concatenate([base, base, base]).astype(np.float32)
+    n = x.shape[0]
+    assign = (np.arange(n) // 900).astype(np.int32)
+    cent = np.stack([x[assign == c].mean(0) for c in range(3)]).astype(np.float32)
+    say("case clustered duplicates", n)
+    for cand, r in ((64, 32), (128, 48)):
+        g = rindex.build_graph_exact_clustered(x, r, cand, 9, 1, assign=assign, centroids=cent)
+        assert np.array_equal(g, rindex.build_graph_exact(x, r, cand, 9, 1)), cand
+
+
+def test_clustered_errors():
+    base, _ = data.workload(500, 32, 8, 8.0, 1)
+    dx = native.DeviceBuffer.from_array(base)
+    dg = native.DeviceBuffer(500 * 16 * 4)
+    bad = np.zeros(500, np.int32)
+    bad[77] = 4
+    da = native.DeviceBuffer.from_array(bad)
+    dc = native.DeviceBuffer.from_array(base[:4].copy())
+    lib = native.lib()
+    say("case clustered errors")
+    rc = lib.rover_build_graph_exact_clustered(dx.ptr, 500, 32, 16, 32, 1, 1, da.ptr, dc.ptr,
+                                               4, dg.ptr, None)
+    assert rc == native.ROVER_EFAILED
+    assert "cluster id" in native.last_error()
+    ok = native.DeviceBuffer.from_array(np.zeros(500, np.int32))
+    # candidates >= n, too many clusters, r > candidates
+    for args in ((16, 500, 4), (16, 32, 40000), (40, 32, 4)):
+        r, cand, nc = args
+        rc = lib.rover_build_graph_exact_clustered(dx.ptr, 500, 32, r, cand, 1, 1, ok.ptr,
+                                                   dc.ptr, nc, dg.ptr, None)
+        assert rc == native.ROVER_EBADPARM, args
+
+
+def test_clustered_200k_mixture():
+    n, dim = 200_000, 96
+    say("case clustered 200k x 96, 256 components, sigma 6")
+    base = data.gaussian_mixture(n, dim, 256, 6.0, 61)
+    t = time.time()
+    want = rindex.build_graph_exact(base, 48, 96, 42, 1)
+    t_brute = time.time() - t
+    t = time.time()
+    assign, cent = rindex.cluster_for_build(base, 512, iters=6)
+    t_cluster = time.time() - t
+    st = []
+    t = time.time()
+    g = rindex.build_graph_exact_clustered(base, 48, 96, 42, 1, assign=assign, centroids=cent,
+                                           stats=st)
+    t_build = time.time() - t
+    say("  brute force %.2f s; k-means %.2f s + clustered build %.2f s; stats %s"
+        % (t_brute, t_cluster, t_build, st))
+    assert np.array_equal(g, want)
