@@ -436,7 +436,15 @@ def main():
                     help="entry-router centroids (k-means over the base)")
     ap.add_argument("--route", type=int, default=4,
                     help="entries per query: representatives of its nearest centroids")
-    ap.add_argument("--graph", choices=["backlinks", "reference"], default="reference",
+    ap.add_argument("--sigma", type=float, default=8.0,
+                    help="standard deviation of the mixture's components")
+    ap.add_argument("--build-clusters", type=int, default=0,
+                    help="k-means clusters for --graph clustered (default n / 1024)")
+    ap.add_argument("--uneven-L", default="",
+                    help="lo,hi: also search every query once with its own beam width "
+                         "uniform in [lo, hi], plain and learned stop, in one launch")
+    ap.add_argument("--graph", choices=["backlinks", "reference", "clustered"],
+                    default="reference",
                     help="backlinks: exact 64-NN + RNG prune + reverse edges (torch); "
                          "reference: the reference's exact search-build on the device "
                          "(random initial rows, RNG prune with fill)")
@@ -513,6 +521,7 @@ def main():
         ap.error("every beam width in --Ls must be >= k (%d)" % args.k)
 
     base = None  # kept for the reference's residual encoder (reference_cpu)
+    graph_build = None
     if args.load_index:
         z = np.load(args.load_index, allow_pickle=False)
         ix = index.PqIndexData(z["graph"], int(z["entry"]), z["codes"], z["codebook"],
@@ -527,13 +536,29 @@ def main():
         import torch
         dev = torch.device("cuda:0")
         log("device", torch.cuda.get_device_name(0))
-        base, queries = data.workload(args.n, args.dim, args.components, 8.0, args.nq)
+        base, queries = data.workload(args.n, args.dim, args.components, args.sigma, args.nq)
         log("data", base.shape, queries.shape)
         graph = None
         if args.graph == "reference":
             tg = time.time()
             graph = index.build_graph_exact(base, args.R, 2 * args.R, 42, 1)
+            graph_build = dict(build_s=round(time.time() - tg, 2))
             log("reference search-build graph on the device: %.2f s" % (time.time() - tg))
+        elif args.graph == "clustered":
+            tg = time.time()
+            if args.build_clusters <= 0:
+                args.build_clusters = max(1, args.n // 1024)
+            assign, cent = index.cluster_for_build(base, args.build_clusters)
+            tc = time.time() - tg
+            st = []
+            graph = index.build_graph_exact_clustered(base, args.R, 2 * args.R, 42, 1,
+                                                      assign=assign, centroids=cent, stats=st)
+            graph_build = dict(clusters=args.build_clusters, kmeans_s=round(tc, 2),
+                               build_s=round(time.time() - tg - tc, 2), node_blocks=st[0],
+                               cluster_pairs=st[1], block_cluster_pairs=st[2],
+                               blocks_with_pairs=st[3])
+            log("reference search-build graph, clustered exact candidates: %s" % graph_build)
+            del assign, cent
         ix = index.build_pq_index(base, m=args.m, R=args.R, res_bits=args.res_bits,
                                   iters=20, train_max=100000, device=dev,
                                   n_router=args.router, graph=graph)
@@ -763,7 +788,7 @@ def main():
     if not args.no_learned:
         if not args.load_model:
             train_q = data.gaussian_mixture(args.train_queries, args.dim, args.components,
-                                            8.0, 44, centers_seed=42)
+                                            args.sigma, 44, centers_seed=42)
             # the mixture's centres depend only on centers_seed, as for the base
             train_ents = entry_rows(train_q)
             log("training: %d queries at L=%d, entry rows chosen"
@@ -937,6 +962,50 @@ def main():
         tuned.clear()
         tuned.update(saved_tuned)
         d_rent.close()
+
+    # ---- uneven beam widths: every query once with its own L, plain and
+    # learned stop, in one launch (per-query pool sizes in HBM)
+    uneven = None
+    if args.uneven_L:
+        lo_L, hi_L = (int(v) for v in args.uneven_L.split(","))
+        lo_L = max(lo_L, k)
+        Lq = np.random.default_rng(4096).integers(lo_L, hi_L + 1, size=nq).astype(np.int32)
+        d_L = native.DeviceBuffer.from_array(Lq)
+        uneven = {"L": "per query uniform in [%d, %d], one launch" % (lo_L, hi_L),
+                  "mean_L": round(float(Lq.mean()), 1)}
+
+        def go(lrn):
+            calc.search_batch_device(dq.ptr, nq, hi_L, k, d_ids.ptr, d_d.ptr, d_cnt.ptr, k,
+                                     correction=1 if mode == "online" else 0,
+                                     refine_candidates=refine_of(mode, lo_L),
+                                     expand_count=args.R, learned_stop=1 if lrn else 0,
+                                     stop_threshold=0.9, d_pool_sizes=d_L.ptr,
+                                     d_entry_ids=es["ptr"], entry_count=es["S"],
+                                     entry_stride=es["S"])
+
+        for name, lrn in (("plain", False), ("learned", True)) if learned else \
+                (("plain", False),):
+            go(lrn)
+            native.synchronize()
+            cnt = d_cnt.download(np.int32, nq)
+            assert (cnt == k).all(), "uneven-L search failed: %s" % np.unique(cnt)
+            rec = data.recall_at_k(d_ids.download(np.int32, nq * k).reshape(nq, k), cnt, gt, k)
+            for _ in range(args.warmup):
+                go(lrn)
+            e0, e1 = native.Event(), native.Event()
+            e0.record()
+            for _ in range(args.steps):
+                go(lrn)
+            e1.record()
+            native.synchronize()
+            ms = e0.elapsed_ms(e1) / args.steps
+            uneven[name] = dict(recall10=round(rec, 4), qps=round(nq / (ms / 1e3), 1),
+                                ms_per_step=round(ms, 4))
+            log("uneven L %s: %s" % (name, uneven[name]))
+        if learned:
+            uneven["learned_speedup_vs_plain"] = round(
+                uneven["plain"]["ms_per_step"] / uneven["learned"]["ms_per_step"], 3)
+        d_L.close()
 
     # ---- refine leg: the top refine_factor*k pool members re-ranked in FP64
     # from PQ + residual (CURRENT) and with the error4 level (ERROR4)
@@ -1174,12 +1243,18 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "1M x 128 Gaussian mixture (1024 comps, sigma 8); PQ 32x256 + "
-                        "%d-bit residuals; R=32 graph (%s); %d queries, k=%d"
-                        % (args.res_bits,
-                           "exact 64-NN, RNG prune, backlinks" if args.graph == "backlinks"
-                           else "the reference's exact search-build: 64-NN, random initial "
-                                "rows, RNG prune with fill", nq, k),
+            "workload": "%s x %d Gaussian mixture (%d comps, sigma %g); PQ %dx256 + "
+                        "%d-bit residuals; R=%d graph (%s); %d queries, k=%d"
+                        % ("%gM" % (len(ix.graph) / 1e6), args.dim, args.components,
+                           args.sigma, args.m, args.res_bits, args.R,
+                           "exact %d-NN, RNG prune, backlinks" % (2 * args.R)
+                           if args.graph == "backlinks" else
+                           "the reference's exact search-build: %d-NN%s, random initial "
+                           "rows, RNG prune with fill"
+                           % (2 * args.R, " over %d k-means clusters, certified"
+                              % args.build_clusters if args.graph == "clustered" else ""),
+                           nq, k),
+            "graph_build": graph_build,
             "search": label, "mode": mode, "L": L,
             "refine_candidates": refine_of(mode, L),
             "entries": ("per query: representatives of its %d nearest of %d k-means "
@@ -1205,6 +1280,7 @@ def main():
             "sq": sq_leg,
             "reference_cpu": reference_cpu,
             "routed": routed,
+            "uneven_L": uneven,
             "launch_tuning": tuning,
             "uneven_batches": batching,
         },

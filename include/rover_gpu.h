@@ -610,11 +610,11 @@ AEEResult rover_build_graph_exact(const float *d_x, int n, int dim, int r,
  * lists, and the graph, equal the brute-force ones for any clustering.
  * The reference has one provider (retrieval_search_build_core.c:314-361,
  * an O(n^2) scan); this one keeps its results where n^2 is out of reach.
- * stats (may be NULL) [4]: node blocks, cluster pairs screened in, (node,
- * cluster) pairs searched, nodes with such pairs.  candidates < n;
- * num_clusters <= ROVER_BUILD_MAX_CLUSTERS; ROVER_EFAILED when the
- * clustering leaves more than 2^24 cluster pairs or 2^27 (node, cluster)
- * pairs to search.  Synchronous. */
+ * stats (may be NULL) [4]: node blocks (16 members of one cluster),
+ * cluster pairs screened in, (block, cluster) pairs searched, blocks with
+ * such pairs.  candidates < n; num_clusters <= ROVER_BUILD_MAX_CLUSTERS;
+ * ROVER_EFAILED when the clustering leaves more than 2^24 cluster pairs or
+ * 2^27 (block, cluster) pairs to search.  Synchronous. */
 #define ROVER_BUILD_MAX_CLUSTERS 32768
 AEEResult rover_build_graph_exact_clustered(const float *d_x, int n, int dim, int r,
                                             int candidates, uint32_t seed, int prune_mode,

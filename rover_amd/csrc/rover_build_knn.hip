@@ -4,29 +4,33 @@
 // n, computed over clusters and certified, so the result is the brute-force
 // one for any clustering the caller supplies:
 //
-//   1. nodes are grouped by cluster (counting sort on the device) and each
-//      cluster's radius r_c = max over its members of |x - mu_c| taken;
-//   2. every node's `keep` closest among its own cluster's members
-//      (rv_knn_range_kernel: 16 nodes per workgroup against the cluster's
+//   1. nodes are grouped by cluster (counting sort on the device), cut into
+//      blocks of 16 consecutive members of one cluster, and each cluster's
+//      radius r_c = max over its members of |x - mu_c| taken;
+//   2. every node's 128 closest among its own cluster's members
+//      (rv_knn_range_kernel: a block per workgroup against the cluster's
 //      members in 64-node tiles, the distances sequential f32 sums in the
 //      reference's order, keys (distance bits, original id));
 //   3. certification: a node y of another cluster c is at least
-//      (|x - mu_c| - r_c)^2 from x, so cluster c can hold one of x's
-//      candidates only if that bound, shrunk by a margin that covers the f32
+//      |x - mu_c| - r_c from x, so cluster c can hold one of x's candidates
+//      only if that bound squared, shrunk by a margin that covers the f32
 //      rounding of every quantity involved, is at most x's current
 //      candidates-th distance (an upper bound of the final one).  Cluster
 //      pairs are screened first with |mu_c' - mu_c| - r_c' - r_c
 //      (rv_kn_cpair_kernel), then the members of the pairs left, node by
-//      node (rv_kn_npair_kernel); every (node, cluster) pair not ruled out
-//      is listed;
-//   4. the listed clusters' members are merged into their node's list
-//      (rv_knn_extra_kernel, one wavefront per node).
+//      node (rv_kn_npair_kernel); a block is listed with cluster c when any
+//      of its nodes is not ruled out;
+//   4. the listed (block, cluster) pairs are sorted by block (device radix
+//      sort) and every block's lists are merged with all members of its
+//      listed clusters (rv_knn_range_kernel again, lists loaded from HBM).
 //
 // Every node outside the searched clusters is then provably farther than
-// the final candidates-th key, so the lists equal the brute-force lists key
-// for key (ties by id included).  A clustering that matches the data
+// the final candidates-th key, and searching more clusters than a node
+// needs only adds true candidates, so the lists equal the brute-force lists
+// key for key (ties by id included).  A clustering that matches the data
 // (k-means) leaves few pairs for step 4.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <float.h>
 #include <math.h>
@@ -142,21 +146,26 @@ __global__ void rv_kn_radius_kernel(const float *xp, const int32_t *offsets, int
     if (lane == 0) radius[c] = r;
 }
 
-// ------------------------------------------------- own-cluster lists
+// ------------------------------------------------- candidate lists
 
-// A block: nodes [node0, node0 + nodes) of the grouped order against the
-// columns [col0, col1) of the grouped order (the nodes' cluster).
+// A block: nodes [node0, node0 + nodes) of the grouped order, members of
+// one cluster whose columns are [col0, col1) of the grouped order.
 struct KnBlock {
     int node0, nodes, col0, col1;
 };
 
-// The kNN tile loop of the brute-force builder over one column range:
-// 16 nodes per workgroup (4 per wavefront), 64-column tiles staged through
-// LDS 128 dimensions at a time, sequential f32 sums carried across the
-// chunks; the best 128 keys of each node as two keys per lane.
+// The kNN tile loop of the brute-force builder over column ranges: 16 nodes
+// per workgroup (4 per wavefront), 64-column tiles staged through LDS 128
+// dimensions at a time, sequential f32 sums carried across the chunks; the
+// best 128 keys of each node as two keys per lane.  EXTRA = false: the
+// block's own cluster, lists start empty; EXTRA = true: the clusters listed
+// for the block (sorted pairs [pfirst, plast) of (block << 32 | cluster)),
+// lists loaded from `keys`.
+template <bool EXTRA>
 __global__ __launch_bounds__(KN_WAVE *KN_WAVES) void rv_knn_range_kernel(
     const float *xp, const int32_t *perm, int dim, const KnBlock *blocks, int nblocks,
-    uint64_t *keys) {
+    const int32_t *offsets, const uint64_t *pairs, const int32_t *pfirst,
+    const int32_t *plast, uint64_t *keys) {
     __shared__ __align__(16) float tile[KN_TILE * (KN_DC + 1)];
     __shared__ __align__(16) float rows[KN_NODES * KN_DC];
     constexpr int STRIDE = KN_DC + 1;
@@ -165,48 +174,65 @@ __global__ __launch_bounds__(KN_WAVE *KN_WAVES) void rv_knn_range_kernel(
     const int nchunk = (dim + KN_DC - 1) / KN_DC;
     for (int bi = blockIdx.x; bi < nblocks; bi += gridDim.x) {
         const KnBlock blk = blocks[bi];
+        const int p0 = EXTRA ? pfirst[bi] : 0, p1 = EXTRA ? plast[bi] : 1;
+        if (p0 >= p1) continue;
         uint64_t lo[KN_ROWS], hi[KN_ROWS];
 #pragma unroll
-        for (int r = 0; r < KN_ROWS; r++) lo[r] = hi[r] = KN_KEY_MAX;
-        for (int c0 = blk.col0; c0 < blk.col1; c0 += KN_TILE) {
-            float acc[KN_ROWS];
+        for (int r = 0; r < KN_ROWS; r++) {
+            lo[r] = hi[r] = KN_KEY_MAX;
+            if (EXTRA && wave * KN_ROWS + r < blk.nodes) {
+                const size_t u = (size_t)perm[blk.node0 + wave * KN_ROWS + r];
+                lo[r] = keys[u * 128 + lane];
+                hi[r] = keys[u * 128 + KN_WAVE + lane];
+            }
+        }
+        for (int p = p0; p < p1; p++) {
+            int col0 = blk.col0, col1 = blk.col1;
+            if (EXTRA) {
+                const int c = (int)(uint32_t)pairs[p];
+                col0 = offsets[c];
+                col1 = offsets[c + 1];
+            }
+            for (int c0 = col0; c0 < col1; c0 += KN_TILE) {
+                float acc[KN_ROWS];
 #pragma unroll
-            for (int r = 0; r < KN_ROWS; r++) acc[r] = 0.0f;
-            for (int k = 0; k < nchunk; k++) {
-                const int d0 = k * KN_DC, dc = min(KN_DC, dim - d0);
-                __syncthreads();
-                for (int i = threadIdx.x; i < KN_TILE * dc; i += blockDim.x) {
-                    const int rr = i / dc, e = i - rr * dc;
-                    const int v = c0 + rr;
-                    tile[rr * STRIDE + e] = v < blk.col1 ? xp[(size_t)v * dim + d0 + e] : 0.0f;
-                }
-                for (int i = threadIdx.x; i < KN_NODES * dc; i += blockDim.x) {
-                    const int rr = i / dc, e = i - rr * dc;
-                    rows[rr * KN_DC + e] =
-                        rr < blk.nodes ? xp[(size_t)(blk.node0 + rr) * dim + d0 + e] : 0.0f;
-                }
-                __syncthreads();
-                const float *col = tile + lane * STRIDE;
-                const float *mine = rows + wave * KN_ROWS * KN_DC;
-                for (int d = 0; d < dc; d++) {
-                    const float cv = col[d];
+                for (int r = 0; r < KN_ROWS; r++) acc[r] = 0.0f;
+                for (int k = 0; k < nchunk; k++) {
+                    const int d0 = k * KN_DC, dc = min(KN_DC, dim - d0);
+                    __syncthreads();
+                    for (int i = threadIdx.x; i < KN_TILE * dc; i += blockDim.x) {
+                        const int rr = i / dc, e = i - rr * dc;
+                        const int v = c0 + rr;
+                        tile[rr * STRIDE + e] = v < col1 ? xp[(size_t)v * dim + d0 + e] : 0.0f;
+                    }
+                    for (int i = threadIdx.x; i < KN_NODES * dc; i += blockDim.x) {
+                        const int rr = i / dc, e = i - rr * dc;
+                        rows[rr * KN_DC + e] =
+                            rr < blk.nodes ? xp[(size_t)(blk.node0 + rr) * dim + d0 + e] : 0.0f;
+                    }
+                    __syncthreads();
+                    const float *col = tile + lane * STRIDE;
+                    const float *mine = rows + wave * KN_ROWS * KN_DC;
+                    for (int d = 0; d < dc; d++) {
+                        const float cv = col[d];
 #pragma unroll
-                    for (int r = 0; r < KN_ROWS; r++) {
-                        const float t = mine[r * KN_DC + d] - cv;
-                        acc[r] = acc[r] + t * t;
+                        for (int r = 0; r < KN_ROWS; r++) {
+                            const float t = mine[r * KN_DC + d] - cv;
+                            acc[r] = acc[r] + t * t;
+                        }
                     }
                 }
-            }
-            const int v = c0 + lane;
-            const int vid = v < blk.col1 ? perm[v] : -1;
+                const int v = c0 + lane;
+                const int vid = v < col1 ? perm[v] : -1;
 #pragma unroll
-            for (int r = 0; r < KN_ROWS; r++) {
-                const int u = blk.node0 + wave * KN_ROWS + r;
-                const bool in = wave * KN_ROWS + r < blk.nodes;
-                const uint64_t key = in && v < blk.col1 && v != u ? kn_key(acc[r], vid)
-                                                                 : KN_KEY_MAX;
-                const uint64_t lim = kn_lane_key(hi[r], 63);
-                if (__ballot(key < lim)) kn_merge128(lo[r], hi[r], key);
+                for (int r = 0; r < KN_ROWS; r++) {
+                    const int u = blk.node0 + wave * KN_ROWS + r;
+                    const bool in = wave * KN_ROWS + r < blk.nodes;
+                    const uint64_t key = in && v < col1 && v != u ? kn_key(acc[r], vid)
+                                                                  : KN_KEY_MAX;
+                    const uint64_t lim = kn_lane_key(hi[r], 63);
+                    if (__ballot(key < lim)) kn_merge128(lo[r], hi[r], key);
+                }
             }
         }
 #pragma unroll
@@ -279,87 +305,66 @@ __global__ void rv_kn_cpair_kernel(const float *cent, int nc, int dim, const int
     }
 }
 
-// One workgroup per listed cluster pair (c, c'): each member x of c whose
-// own bound |x - mu_c'| - r_c' does not rule c' out gets the pair (x, c').
-__global__ void rv_kn_npair_kernel(const float *xp, const int32_t *perm, const int32_t *offsets,
-                                   int dim, const float *cent, const float *radius,
-                                   const uint64_t *keys, int cand, const int32_t *cpairs,
-                                   int ncp, int32_t *pairs, unsigned long long cap,
-                                   unsigned long long *count) {
+// One workgroup (256 threads) per listed cluster pair (c, c'): member x of
+// c needs c' when its own bound |x - mu_c'| - r_c' does not rule c' out.
+// The 16 lanes of a lane group cover one block of c (blocks start at the
+// cluster's first member and 256 is a multiple of 16); the group's first
+// lane lists (block, c') when any of them needs it.
+__global__ __launch_bounds__(256) void rv_kn_npair_kernel(
+    const float *xp, const int32_t *perm, const int32_t *offsets, const int32_t *bstart,
+    int dim, const float *cent, const float *radius, const uint64_t *keys, int cand,
+    const int32_t *cpairs, int ncp, uint64_t *pairs, unsigned long long cap,
+    unsigned long long *count) {
+    const int lane = (int)__lane_id();
     for (int pi = blockIdx.x; pi < ncp; pi += gridDim.x) {
         const int c = cpairs[2 * pi], c2 = cpairs[2 * pi + 1];
         const float *mu = cent + (size_t)c2 * dim;
-        for (int pos = offsets[c] + threadIdx.x; pos < offsets[c + 1]; pos += blockDim.x) {
-            const uint64_t k = keys[(size_t)perm[pos] * 128 + cand - 1];
-            bool need = k == KN_KEY_MAX;
-            if (!need) {
-                const float *xv = xp + (size_t)pos * dim;
-                float acc = 0.0f;
-                for (int d = 0; d < dim; d++) {
-                    const float s = xv[d] - mu[d];
-                    acc = acc + s * s;
+        const int first = offsets[c], end = offsets[c + 1];
+        for (int base = first; base < end; base += 256) {
+            const int pos = base + (int)threadIdx.x;
+            bool need = false;
+            if (pos < end) {
+                const uint64_t k = keys[(size_t)perm[pos] * 128 + cand - 1];
+                need = k == KN_KEY_MAX;
+                if (!need) {
+                    const float *xv = xp + (size_t)pos * dim;
+                    float acc = 0.0f;
+                    for (int d = 0; d < dim; d++) {
+                        const float s = xv[d] - mu[d];
+                        acc = acc + s * s;
+                    }
+                    need = kn_may_hold(sqrtf(acc) * (1.0f - 1e-3f) - radius[c2] * (1.0f + 1e-3f),
+                                       __uint_as_float((uint32_t)(k >> 32)));
                 }
-                need = kn_may_hold(sqrtf(acc) * (1.0f - 1e-3f) - radius[c2] * (1.0f + 1e-3f),
-                                   __uint_as_float((uint32_t)(k >> 32)));
             }
-            if (need) {
+            const uint64_t m = __ballot(need);
+            const bool any = ((m >> (16 * (lane >> 4))) & 0xffffull) != 0;
+            if (any && (lane & 15) == 0) {
                 const unsigned long long j = atomicAdd(count, 1ull);
-                if (j < cap) {
-                    pairs[2 * j] = pos;
-                    pairs[2 * j + 1] = c2;
-                }
+                if (j < cap)
+                    pairs[j] = ((uint64_t)(uint32_t)(bstart[c] + (pos - first) / KN_NODES) << 32) |
+                               (uint32_t)c2;
             }
         }
     }
 }
 
-// ----------------------------------------------- certified additions
-
-// One wavefront per listed node: its list [128 keys] merged with every
-// member of each of its listed clusters.  xs: the node's position in the
-// grouped order; ranges [pair0, pair1) of the pair list (sorted by node).
-__global__ __launch_bounds__(KN_WAVE) void rv_knn_extra_kernel(
-    const float *xp, const int32_t *perm, const int32_t *offsets, int dim,
-    const int32_t *pairs, const int32_t *node_start, int nodes, uint64_t *keys) {
-    __shared__ float q[4096];
-    const int lane = (int)__lane_id();
-    for (int w = blockIdx.x; w < nodes; w += gridDim.x) {
-        const int p0 = node_start[w], p1 = node_start[w + 1];
-        const int pos = pairs[2 * p0];
-        const int u = perm[pos];
-        for (int d = lane; d < dim; d += KN_WAVE) q[d] = xp[(size_t)pos * dim + d];
-        __syncthreads();
-        uint64_t lo = keys[(size_t)u * 128 + lane], hi = keys[(size_t)u * 128 + KN_WAVE + lane];
-        for (int p = p0; p < p1; p++) {
-            const int c = pairs[2 * p + 1];
-            for (int v0 = offsets[c]; v0 < offsets[c + 1]; v0 += KN_WAVE) {
-                const int v = v0 + lane;
-                uint64_t key = KN_KEY_MAX;
-                if (v < offsets[c + 1]) {
-                    const float *yv = xp + (size_t)v * dim;
-                    float acc = 0.0f;
-                    for (int d = 0; d < dim; d++) {
-                        const float s = q[d] - yv[d];
-                        acc = acc + s * s;
-                    }
-                    key = kn_key(acc, perm[v]);
-                }
-                const uint64_t lim = kn_lane_key(hi, 63);
-                if (__ballot(key < lim)) kn_merge128(lo, hi, key);
-            }
-        }
-        keys[(size_t)u * 128 + lane] = lo;
-        keys[(size_t)u * 128 + KN_WAVE + lane] = hi;
-        __syncthreads();
+// The sorted (block << 32 | cluster) pairs: each block's range [pfirst,
+// plast) (zero-filled beforehand, so blocks without pairs get an empty
+// range); `listed` counts the blocks with pairs.
+__global__ void rv_kn_mark_kernel(const uint64_t *sorted, int n, int32_t *pfirst,
+                                  int32_t *plast, int32_t *listed) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    const uint32_t b = (uint32_t)(sorted[i] >> 32);
+    if (i == 0 || (uint32_t)(sorted[i - 1] >> 32) != b) {
+        pfirst[b] = i;
+        atomicAdd(listed, 1);
     }
+    if (i == n - 1 || (uint32_t)(sorted[i + 1] >> 32) != b) plast[b] = i + 1;
 }
 
 // ------------------------------------------------------------ driver
-
-static int kn_cmp64(const void *a, const void *b) {
-    const int64_t x = *(const int64_t *)a, y = *(const int64_t *)b;
-    return (x > y) - (x < y);
-}
 
 extern "C" int rv_build_exact_clustered(const float *x, int n, int dim, int r, int cand,
                                         uint32_t seed, int prune_mode, const int32_t *assign,
@@ -367,42 +372,40 @@ extern "C" int rv_build_exact_clustered(const float *x, int n, int dim, int r, i
                                         int64_t *out_stats, void *stream) {
     if (!x || !graph || !assign || !centroids || n <= 1 || dim <= 0 || dim > 4096 ||
         r <= 0 || r >= n || r > KN_WAVE || cand < r || cand > 128 || cand >= n || nc <= 0 ||
-        nc > KN_MAX_CLUSTERS ||
-        (prune_mode != 0 && prune_mode != 1)) {
+        nc > KN_MAX_CLUSTERS || (prune_mode != 0 && prune_mode != 1)) {
         rv_set_error("rv_build_exact_clustered: bad arguments");
         return -1;
     }
     hipStream_t s = (hipStream_t)stream;
     int rc = -1;
-    int32_t *d_counts = nullptr, *d_cursor = nullptr, *d_perm = nullptr;
-    int32_t *d_off = nullptr, *d_bad = nullptr, *d_pairs = nullptr;
-    unsigned long long *d_npairs = nullptr, npairs64 = 0, ncpairs64 = 0;
-    int32_t *d_nstart = nullptr, *d_cpairs = nullptr;
+    int32_t *d_counts = nullptr, *d_cursor = nullptr, *d_perm = nullptr, *d_off = nullptr;
+    int32_t *d_bad = nullptr, *d_bstart = nullptr, *d_cpairs = nullptr;
+    int32_t *d_pfirst = nullptr, *d_plast = nullptr, *d_listed = nullptr;
     float *d_xp = nullptr, *d_rad = nullptr, *d_maxd = nullptr;
-    uint64_t *d_keys = nullptr;
+    uint64_t *d_keys = nullptr, *d_pairs = nullptr, *d_sorted = nullptr;
+    unsigned long long *d_count = nullptr, npairs64 = 0, ncpairs64 = 0;
+    void *d_tmp = nullptr;
+    size_t tmp_bytes = 0;
     KnBlock *d_blocks = nullptr;
-    int32_t *h_off = nullptr, *h_pairs = nullptr, *h_nstart = nullptr;
+    int32_t *h_off = nullptr, *h_bstart = nullptr;
     KnBlock *h_blocks = nullptr;
     int nblocks = 0, bad = 0, npairs = 0, ncpairs = 0, listed = 0;
-    // at most every other cluster for every node, and 2^27 pairs (1 GiB)
-    const size_t all_pairs = (size_t)n * (size_t)(nc - 1);
-    const size_t cap = all_pairs < ((size_t)1 << 27) ? (all_pairs > 0 ? all_pairs : 1)
-                                                     : ((size_t)1 << 27);
     const size_t ccap = (size_t)1 << 24;
     const unsigned g1 = (unsigned)((n + 255) / 256);
     if (kn_check(hipMalloc((void **)&d_counts, (size_t)nc * 4), "hipMalloc") ||
         kn_check(hipMalloc((void **)&d_cursor, (size_t)nc * 4), "hipMalloc") ||
         kn_check(hipMalloc((void **)&d_off, (size_t)(nc + 1) * 4), "hipMalloc") ||
+        kn_check(hipMalloc((void **)&d_bstart, (size_t)nc * 4), "hipMalloc") ||
         kn_check(hipMalloc((void **)&d_perm, (size_t)n * 4), "hipMalloc") ||
         kn_check(hipMalloc((void **)&d_bad, 8), "hipMalloc") ||
         kn_check(hipMalloc((void **)&d_xp, (size_t)n * dim * 4), "hipMalloc") ||
         kn_check(hipMalloc((void **)&d_rad, (size_t)nc * 4), "hipMalloc") ||
-        kn_check(hipMalloc((void **)&d_keys, (size_t)n * 128 * 8), "hipMalloc") ||
-        kn_check(hipMalloc((void **)&d_pairs, cap * 8), "hipMalloc"))
+        kn_check(hipMalloc((void **)&d_keys, (size_t)n * 128 * 8), "hipMalloc"))
         goto done;
     h_off = (int32_t *)malloc((size_t)(nc + 1) * 4);
-    if (!h_off) goto done;
-    // 1. grouping by cluster
+    h_bstart = (int32_t *)malloc((size_t)nc * 4);
+    if (!h_off || !h_bstart) goto done;
+    // 1. grouping by cluster, blocks of 16 members
     if (rv_fill_zero(d_counts, (size_t)nc * 4, s) || rv_fill_zero(d_bad, 8, s)) goto done;
     rv_kn_count_kernel<<<g1, 256, 0, s>>>(assign, n, nc, d_counts, d_bad);
     if (kn_check(hipGetLastError(), "rv_kn_count_kernel") ||
@@ -418,29 +421,28 @@ extern "C" int rv_build_exact_clustered(const float *x, int n, int dim, int r, i
     h_off[0] = 0;
     for (int c = 0; c < nc; c++) h_off[c + 1] += h_off[c];
     for (int c = 0; c < nc; c++) {
-        const int cnt = h_off[c + 1] - h_off[c];
-        nblocks += (cnt + KN_NODES - 1) / KN_NODES;
+        h_bstart[c] = nblocks;
+        nblocks += (h_off[c + 1] - h_off[c] + KN_NODES - 1) / KN_NODES;
     }
-    h_blocks = (KnBlock *)malloc((size_t)(nblocks > 0 ? nblocks : 1) * sizeof(KnBlock));
+    h_blocks = (KnBlock *)malloc((size_t)nblocks * sizeof(KnBlock));
     if (!h_blocks) goto done;
-    {
-        int b = 0;
-        for (int c = 0; c < nc; c++)
-            for (int v = h_off[c]; v < h_off[c + 1]; v += KN_NODES) {
-                h_blocks[b].node0 = v;
-                h_blocks[b].nodes = h_off[c + 1] - v < KN_NODES ? h_off[c + 1] - v : KN_NODES;
-                h_blocks[b].col0 = h_off[c];
-                h_blocks[b].col1 = h_off[c + 1];
-                b++;
-            }
-    }
-    if (kn_check(hipMalloc((void **)&d_blocks, (size_t)(nblocks > 0 ? nblocks : 1) *
-                                                   sizeof(KnBlock)),
+    for (int c = 0, b = 0; c < nc; c++)
+        for (int v = h_off[c]; v < h_off[c + 1]; v += KN_NODES, b++) {
+            h_blocks[b].node0 = v;
+            h_blocks[b].nodes = h_off[c + 1] - v < KN_NODES ? h_off[c + 1] - v : KN_NODES;
+            h_blocks[b].col0 = h_off[c];
+            h_blocks[b].col1 = h_off[c + 1];
+        }
+    if (kn_check(hipMalloc((void **)&d_blocks, (size_t)nblocks * sizeof(KnBlock)),
                  "hipMalloc") ||
+        kn_check(hipMalloc((void **)&d_pfirst, (size_t)nblocks * 4), "hipMalloc") ||
+        kn_check(hipMalloc((void **)&d_plast, (size_t)nblocks * 4), "hipMalloc") ||
         kn_check(hipMemcpyAsync(d_off, h_off, (size_t)(nc + 1) * 4, hipMemcpyHostToDevice, s),
                  "offsets") ||
         kn_check(hipMemcpyAsync(d_cursor, h_off, (size_t)nc * 4, hipMemcpyHostToDevice, s),
                  "cursor") ||
+        kn_check(hipMemcpyAsync(d_bstart, h_bstart, (size_t)nc * 4, hipMemcpyHostToDevice, s),
+                 "block starts") ||
         kn_check(hipMemcpyAsync(d_blocks, h_blocks, (size_t)nblocks * sizeof(KnBlock),
                                 hipMemcpyHostToDevice, s),
                  "blocks"))
@@ -453,27 +455,22 @@ extern "C" int rv_build_exact_clustered(const float *x, int n, int dim, int r, i
     rv_kn_radius_kernel<<<nc, KN_WAVE, 0, s>>>(d_xp, d_off, nc, dim, centroids, d_rad);
     if (kn_check(hipGetLastError(), "rv_kn_radius_kernel")) goto done;
     // 2. own-cluster lists
-    if (rv_fill_zero(d_keys, (size_t)n * 128 * 8, s)) goto done;
-    {
-        int grid = nblocks < 65536 ? nblocks : 65536;
-        if (grid > 0) {
-            rv_knn_range_kernel<<<grid, KN_WAVE * KN_WAVES, 0, s>>>(d_xp, d_perm, dim, d_blocks,
-                                                                     nblocks, d_keys);
-            if (kn_check(hipGetLastError(), "rv_knn_range_kernel")) goto done;
-        }
-    }
-    // 3. certification: cluster pairs, then nodes of the listed pairs
-    if (kn_check(hipMalloc((void **)&d_npairs, 8), "hipMalloc") ||
+    rv_knn_range_kernel<false><<<nblocks < 65536 ? nblocks : 65536, KN_WAVE * KN_WAVES, 0, s>>>(
+        d_xp, d_perm, dim, d_blocks, nblocks, d_off, nullptr, nullptr, nullptr, d_keys);
+    if (kn_check(hipGetLastError(), "rv_knn_range_kernel")) goto done;
+    // 3. certification: cluster pairs, then the blocks of the listed pairs
+    if (kn_check(hipMalloc((void **)&d_count, 8), "hipMalloc") ||
+        kn_check(hipMalloc((void **)&d_listed, 8), "hipMalloc") ||
         kn_check(hipMalloc((void **)&d_maxd, (size_t)nc * 4), "hipMalloc") ||
         kn_check(hipMalloc((void **)&d_cpairs, ccap * 8), "hipMalloc") ||
-        rv_fill_zero(d_npairs, 8, s))
+        rv_fill_zero(d_count, 8, s))
         goto done;
     rv_kn_maxd_kernel<<<nc, KN_WAVE, 0, s>>>(d_perm, d_off, nc, d_keys, cand, d_maxd);
     rv_kn_cpair_kernel<<<(unsigned)(((size_t)nc * nc + 255) / 256), 256, 0, s>>>(
-        centroids, nc, dim, d_off, d_rad, d_maxd, d_cpairs, ccap, d_npairs);
+        centroids, nc, dim, d_off, d_rad, d_maxd, d_cpairs, ccap, d_count);
     if (kn_check(hipGetLastError(), "rv_kn_cpair_kernel") ||
-        kn_check(hipMemcpyAsync(&ncpairs64, d_npairs, 8, hipMemcpyDeviceToHost, s),
-                 "ncpairs") ||
+        kn_check(hipMemcpyAsync(&ncpairs64, d_count, 8, hipMemcpyDeviceToHost, s),
+                 "cluster pairs") ||
         kn_check(hipStreamSynchronize(s), "sync"))
         goto done;
     if (ncpairs64 > ccap) {
@@ -483,85 +480,97 @@ extern "C" int rv_build_exact_clustered(const float *x, int n, int dim, int r, i
     }
     ncpairs = (int)ncpairs64;
     if (ncpairs > 0) {
-        if (rv_fill_zero(d_npairs, 8, s)) goto done;
+        // at most one pair per block of its first cluster for every listed
+        // cluster pair, and 2^27 pairs (1 GiB)
+        size_t cap = (size_t)ncpairs * (size_t)nblocks;
+        if (cap > ((size_t)1 << 27)) cap = (size_t)1 << 27;
+        if (kn_check(hipMalloc((void **)&d_pairs, cap * 8), "hipMalloc") ||
+            rv_fill_zero(d_count, 8, s))
+            goto done;
         rv_kn_npair_kernel<<<ncpairs < 65536 ? ncpairs : 65536, 256, 0, s>>>(
-            d_xp, d_perm, d_off, dim, centroids, d_rad, d_keys, cand, d_cpairs, ncpairs,
-            d_pairs, cap, d_npairs);
+            d_xp, d_perm, d_off, d_bstart, dim, centroids, d_rad, d_keys, cand, d_cpairs,
+            ncpairs, d_pairs, cap, d_count);
         if (kn_check(hipGetLastError(), "rv_kn_npair_kernel") ||
-            kn_check(hipMemcpyAsync(&npairs64, d_npairs, 8, hipMemcpyDeviceToHost, s),
-                     "npairs") ||
+            kn_check(hipMemcpyAsync(&npairs64, d_count, 8, hipMemcpyDeviceToHost, s),
+                     "pairs") ||
             kn_check(hipStreamSynchronize(s), "sync"))
             goto done;
-    }
-    if (npairs64 > cap) {
-        rv_set_error("rv_build_exact_clustered: too many uncertified (node, cluster) pairs; "
-                     "the clustering does not fit the data");
-        goto done;
-    }
-    npairs = (int)npairs64;
-    // 4. certified additions, pairs grouped by node
-    if (npairs > 0) {
-        h_pairs = (int32_t *)malloc((size_t)npairs * 8);
-        h_nstart = (int32_t *)malloc((size_t)(npairs + 1) * 4);
-        if (!h_pairs || !h_nstart ||
-            kn_check(hipMemcpy(h_pairs, d_pairs, (size_t)npairs * 8, hipMemcpyDeviceToHost),
-                     "pairs"))
+        if (npairs64 > cap) {
+            char msg[200];
+            snprintf(msg, sizeof(msg),
+                     "rv_build_exact_clustered: %llu (block, cluster) pairs left to search "
+                     "(of %d cluster pairs), more than %llu; the clustering does not fit "
+                     "the data",
+                     npairs64, ncpairs, (unsigned long long)cap);
+            rv_set_error(msg);
             goto done;
-        {
-            // the pairs by (node position, cluster)
-            int64_t *kv = (int64_t *)malloc((size_t)npairs * 8);
-            if (!kv) goto done;
-            for (int i = 0; i < npairs; i++)
-                kv[i] = ((int64_t)h_pairs[2 * i] << 32) | (uint32_t)h_pairs[2 * i + 1];
-            qsort(kv, (size_t)npairs, sizeof(int64_t), kn_cmp64);
-            for (int i = 0; i < npairs; i++) {
-                h_pairs[2 * i] = (int32_t)(kv[i] >> 32);
-                h_pairs[2 * i + 1] = (int32_t)(uint32_t)kv[i];
-                if (i == 0 || h_pairs[2 * i] != h_pairs[2 * i - 2]) h_nstart[listed++] = i;
-            }
-            h_nstart[listed] = npairs;
-            free(kv);
         }
-        if (kn_check(hipMalloc((void **)&d_nstart, (size_t)(listed + 1) * 4), "hipMalloc") ||
-            kn_check(hipMemcpy(d_nstart, h_nstart, (size_t)(listed + 1) * 4,
-                               hipMemcpyHostToDevice),
-                     "nstart") ||
-            kn_check(hipMemcpy(d_pairs, h_pairs, (size_t)npairs * 8, hipMemcpyHostToDevice),
-                     "pairs"))
+        npairs = (int)npairs64;
+    }
+    // 4. certified additions: the pairs sorted by block, each block's lists
+    // merged with its listed clusters
+    if (npairs > 0) {
+        // the key's high word is a block index below nblocks
+        int end_bit = 33;
+        while (end_bit < 64 && ((uint64_t)nblocks >> (end_bit - 32)) != 0) end_bit++;
+        if (kn_check(hipMalloc((void **)&d_sorted, (size_t)npairs * 8), "hipMalloc") ||
+            kn_check(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp_bytes, d_pairs, d_sorted,
+                                                       npairs, 0, end_bit, s),
+                     "radix sort size") ||
+            kn_check(hipMalloc(&d_tmp, tmp_bytes), "hipMalloc"))
             goto done;
-        rv_knn_extra_kernel<<<listed < 65536 ? listed : 65536, KN_WAVE, 0, s>>>(
-            d_xp, d_perm, d_off, dim, d_pairs, d_nstart, listed, d_keys);
-        if (kn_check(hipGetLastError(), "rv_knn_extra_kernel")) goto done;
+        if (kn_check(hipcub::DeviceRadixSort::SortKeys(d_tmp, tmp_bytes, d_pairs, d_sorted,
+                                                       npairs, 0, end_bit, s),
+                     "radix sort") ||
+            rv_fill_zero(d_pfirst, (size_t)nblocks * 4, s) ||
+            rv_fill_zero(d_plast, (size_t)nblocks * 4, s) || rv_fill_zero(d_listed, 8, s))
+            goto done;
+        rv_kn_mark_kernel<<<(unsigned)((npairs + 255) / 256), 256, 0, s>>>(d_sorted, npairs,
+                                                                           d_pfirst, d_plast,
+                                                                           d_listed);
+        if (kn_check(hipGetLastError(), "rv_kn_mark_kernel")) goto done;
+        rv_knn_range_kernel<true>
+            <<<nblocks < 65536 ? nblocks : 65536, KN_WAVE * KN_WAVES, 0, s>>>(
+                d_xp, d_perm, dim, d_blocks, nblocks, d_off, d_sorted, d_pfirst, d_plast,
+                d_keys);
+        if (kn_check(hipGetLastError(), "rv_knn_range_kernel") ||
+            kn_check(hipMemcpyAsync(&listed, d_listed, 4, hipMemcpyDeviceToHost, s),
+                     "listed"))
+            goto done;
     }
     // 5. the rows, as the brute-force builder forms them
     if (rv_rows_from_keys(x, n, dim, r, cand, seed, prune_mode, d_keys, 128, graph, s))
         goto done;
-    if (out_stats) {
+    rc = kn_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    if (!rc && out_stats) {
         out_stats[0] = nblocks;
         out_stats[1] = ncpairs;
         out_stats[2] = npairs;
         out_stats[3] = listed;
     }
-    rc = kn_check(hipStreamSynchronize(s), "hipStreamSynchronize");
 done:
     hipStreamSynchronize(s);
     hipFree(d_counts);
     hipFree(d_cursor);
     hipFree(d_off);
+    hipFree(d_bstart);
     hipFree(d_perm);
     hipFree(d_bad);
     hipFree(d_xp);
     hipFree(d_rad);
     hipFree(d_keys);
     hipFree(d_pairs);
-    hipFree(d_npairs);
-    hipFree(d_nstart);
-    hipFree(d_blocks);
+    hipFree(d_sorted);
+    hipFree(d_tmp);
+    hipFree(d_count);
+    hipFree(d_listed);
     hipFree(d_cpairs);
     hipFree(d_maxd);
+    hipFree(d_blocks);
+    hipFree(d_pfirst);
+    hipFree(d_plast);
     free(h_off);
-    free(h_pairs);
-    free(h_nstart);
+    free(h_bstart);
     free(h_blocks);
     return rc;
 }

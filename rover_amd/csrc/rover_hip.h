@@ -193,8 +193,8 @@ int rv_rows_from_keys(const float *d_x, int n, int dim, int r, int candidates,
  * [n] cluster ids in [0, nc), d_centroids [nc][dim]): own-cluster candidate
  * lists, then every other cluster that a triangle-inequality bound does not
  * rule out, so the graph is the brute-force one for any clustering.
- * out_stats (may be NULL) [4]: node blocks, cluster pairs screened in, (node,
- * cluster) pairs searched, nodes with such pairs.  Synchronous. */
+ * out_stats (may be NULL) [4]: node blocks, cluster pairs screened in,
+ * (block, cluster) pairs searched, blocks with such pairs.  Synchronous. */
 int rv_build_exact_clustered(const float *d_x, int n, int dim, int r, int candidates,
                              uint32_t seed, int prune_mode, const int32_t *d_assign,
                              const float *d_centroids, int nc, int32_t *d_graph,

@@ -53,7 +53,7 @@ def ground_truth(base, queries, k=100, device=None, margin=32):
         xb = torch.from_numpy(base).to(device)
         nb = (xb * xb).sum(1)
         out = np.empty((queries.shape[0], k), dtype=np.int32)
-        step = max(1, min(4096, (1 << 28) // max(n, 1)))
+        step = max(1, min(4096, (1 << 30) // max(n, 1)))
         for s in range(0, queries.shape[0], step):
             e = min(queries.shape[0], s + step)
             xq = torch.from_numpy(queries[s:e]).to(device)

@@ -524,8 +524,14 @@ def _backlinks_torch(x, graph, R, device, batch):
 def medoid(x):
     """Entry point: the base vector nearest the centroid (the standard
     stand-in for the medoid, which is quadratic to compute exactly)."""
-    c = x.astype(np.float64).mean(0)
-    d = ((x.astype(np.float64) - c) ** 2).sum(1)
+    n = x.shape[0]
+    step = 1 << 18
+    if n <= 1 << 22:
+        c = x.astype(np.float64).mean(0)
+    else:  # a float64 copy of a large set is avoided; the mean is summed by chunks
+        c = sum(x[s:s + step].astype(np.float64).sum(0) for s in range(0, n, step)) / n
+    d = np.concatenate([((x[s:s + step].astype(np.float64) - c) ** 2).sum(1)
+                        for s in range(0, n, step)])
     return int(np.argmin(d))
 
 
@@ -766,19 +772,32 @@ def build_graph_exact(x, R, candidates=64, seed=42, prune_mode=1):
 
 def cluster_for_build(x, clusters, iters=8, sample=262144, seed=7, device="cuda",
                       chunk=65536):
-    """A k-means clustering of x for build_graph_exact_clustered: Lloyd's
-    iterations on a uniform-stride sample (float32 on the device), then every
-    row assigned to its nearest centroid.  Returns (assign int32 [n],
-    centroids float32 [clusters, dim]).  Any clustering gives the same graph;
-    a tight one leaves fewer clusters to search per node."""
+    """A k-means clustering of x for build_graph_exact_clustered: k-means++
+    seeding (each next centre drawn with probability proportional to the
+    squared distance to the nearest centre so far) and Lloyd's iterations on
+    a uniform-stride sample (float32 on the device), then every row assigned
+    to its nearest centroid.  Returns (assign int32 [n], centroids float32
+    [clusters, dim]).  Any clustering gives the same graph; a tight one
+    leaves fewer clusters to search per node (seeding by squared distance
+    puts a centre in every well-separated group, where uniform seeding
+    leaves about a third of them to share a cluster with a neighbour)."""
     import torch
     x = np.ascontiguousarray(x, dtype=np.float32)
     n, dim = x.shape
     clusters = max(1, min(int(clusters), n))
-    tr = torch.from_numpy(x[uniform_stride_sample(n, min(n, sample))]).to(device)
-    g = torch.Generator(device="cpu").manual_seed(seed)
-    init = torch.randperm(tr.shape[0], generator=g)[:clusters].to(device)
-    cent = tr[init].clone()
+    tr = torch.from_numpy(x[uniform_stride_sample(n, min(n, max(sample, 64 * clusters)))]
+                          ).to(device)
+    rng = np.random.default_rng(seed)
+    pick = [int(rng.integers(tr.shape[0]))]
+    d2 = ((tr - tr[pick[0]]) ** 2).sum(1)
+    u = rng.random(clusters)
+    for i in range(1, clusters):
+        cs = torch.cumsum(d2.double(), 0)
+        j = int(torch.searchsorted(cs, cs[-1:] * float(u[i])).item())
+        j = min(j, tr.shape[0] - 1)
+        pick.append(j)
+        d2 = torch.minimum(d2, ((tr - tr[j]) ** 2).sum(1))
+    cent = tr[torch.tensor(pick, device=tr.device)].clone()
 
     def nearest(pts):
         d = (cent * cent).sum(1)[None, :] - 2.0 * pts @ cent.T
@@ -799,14 +818,14 @@ def build_graph_exact_clustered(x, R, candidates=64, seed=42, prune_mode=1, clus
     """build_graph_exact's graph, bit for bit, computed over a clustering
     (rover_build_graph_exact_clustered): own-cluster candidates plus every
     cluster a triangle-inequality bound does not rule out.  assign/centroids
-    default to cluster_for_build(x, clusters or sqrt(n)).  stats (a list)
-    receives [node blocks, cluster pairs, (node, cluster) pairs, nodes with
-    pairs]."""
+    default to cluster_for_build(x, clusters or n / 1024).  stats (a list)
+    receives [node blocks (16 members of one cluster), cluster pairs screened
+    in, (block, cluster) pairs searched, blocks with such pairs]."""
     from . import native
     x = np.ascontiguousarray(x, dtype=np.float32)
     n, dim = x.shape
     if assign is None:
-        k = clusters if clusters is not None else max(1, int(np.sqrt(n)))
+        k = clusters if clusters is not None else max(1, n // 1024)
         assign, centroids = cluster_for_build(x, k)
     assign = np.ascontiguousarray(assign, dtype=np.int32)
     centroids = np.ascontiguousarray(centroids, dtype=np.float32)

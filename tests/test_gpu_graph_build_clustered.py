@@ -2,7 +2,7 @@
 brute-force build and the reference's outputs, row for row: k-means
 clusterings, clusterings that fit the data badly (random assignments, one
 cluster, empty clusters), exact duplicates split across clusters (ties by
-id), 96 and 128 candidates, and a 200k-node mixture."""
+id), 96 and 128 candidates, and 200k- and 1M-node mixtures."""
 import ctypes as C
 import sys
 import time
@@ -130,4 +130,22 @@ def test_clustered_200k_mixture():
     t_build = time.time() - t
     say("  brute force %.2f s; k-means %.2f s + clustered build %.2f s; stats %s"
         % (t_brute, t_cluster, t_build, st))
+    assert np.array_equal(g, want)
+
+
+def test_clustered_1M_mixture():
+    """1M x 96 from 1024 components (sigma 6), the default clustering (k-means++
+    on n / 1024 clusters), against the brute-force build."""
+    n, dim = 1_000_000, 96
+    say("case clustered 1M x 96, 1024 components, sigma 6")
+    base = data.gaussian_mixture(n, dim, 1024, 6.0, 62)
+    t = time.time()
+    want = rindex.build_graph_exact(base, 48, 96, 42, 1)
+    t_brute = time.time() - t
+    st = []
+    t = time.time()
+    g = rindex.build_graph_exact_clustered(base, 48, 96, 42, 1, stats=st)
+    t_build = time.time() - t
+    say("  brute force %.2f s; k-means + clustered build %.2f s; stats %s"
+        % (t_brute, t_build, st))
     assert np.array_equal(g, want)
