@@ -156,7 +156,7 @@ def reference_leg(pyoracle, ix, base, queries, ents, gids, gd, L, k, args, threa
 
 
 def batching_leg(calc, queries, ents, S, mode, refine_of, args, cap=4096, kmax=100,
-                 lmax=512):
+                 lmax=512, refine_mode=0):
     """Uneven batches through one captured HIP graph: batch size uniform in
     1..4096, per-query k from {1, 10, 100} and beam width uniform in
     max(16, k)..512 (k <= L is required), queries drawn from the query set.
@@ -182,7 +182,7 @@ def batching_leg(calc, queries, ents, S, mode, refine_of, args, cap=4096, kmax=1
     g = calc.search_graph_create(dq.ptr, cap, lmax, kmax, ids.ptr, dd.ptr, cnt.ptr, kmax,
                                  stream.ptr, d_num_queries=live.ptr,
                                  correction=1 if mode == "online" else 0,
-                                 refine_candidates=refine_of(mode, lmax),
+                                 refine_candidates=refine_of(mode, lmax), refine_mode=refine_mode,
                                  expand_count=args.R, d_pool_sizes=dL.ptr, d_num_outs=dK.ptr,
                                  d_entry_ids=de.ptr, entry_count=S, entry_stride=S)
     e0, e1 = native.Event(), native.Event()
@@ -434,6 +434,9 @@ def main():
     ap.add_argument("--res-bits", type=int, default=8)
     ap.add_argument("--router", type=int, default=4096,
                     help="entry-router centroids (k-means over the base)")
+    ap.add_argument("--router-seeding", choices=["stride", "kmeans++"], default="stride",
+                    help="initial router centroids: uniform-stride sample rows or D^2 "
+                         "(k-means++) seeding")
     ap.add_argument("--route", type=int, default=4,
                     help="entries per query: representatives of its nearest centroids")
     ap.add_argument("--sigma", type=float, default=8.0,
@@ -466,6 +469,10 @@ def main():
     ap.add_argument("--no-sq", action="store_true",
                     help="skip the SQ leg (100k x 128 SQ8, L=64: the reference's SQ "
                          "traversal, plain and corrected, and the best-first SQ search)")
+    ap.add_argument("--final-refine", type=int, default=0,
+                    help="F > 0: every search of the sweeps and timings re-ranks its top "
+                         "F pool members in FP64 (--final-refine-mode) before the top k")
+    ap.add_argument("--final-refine-mode", choices=["current", "error4"], default="error4")
     ap.add_argument("--refine-factor", type=int, default=2,
                     help="refine leg: refine the top factor*k of the pool, with and "
                          "without the error4 level (0 skips the leg)")
@@ -561,13 +568,15 @@ def main():
             del assign, cent
         ix = index.build_pq_index(base, m=args.m, R=args.R, res_bits=args.res_bits,
                                   iters=20, train_max=100000, device=dev,
-                                  n_router=args.router, graph=graph)
+                                  n_router=args.router, graph=graph,
+                                  router_seeding=args.router_seeding)
         log("index built: graph", ix.graph.shape, "medoid", ix.entry, "router",
             ix.router_centroids.shape)
         gt = data.ground_truth(base, queries, k=100, device=dev)
         log("ground truth done")
         e4 = None
-        if args.refine_factor > 0:
+        if args.refine_factor > 0 or (args.final_refine > 0 and
+                                      args.final_refine_mode == "error4"):
             e4 = index.encode_error4_on_device(base, ix.codes, ix.codebook, ix.res_codes,
                                                ix.res_qparams, ix.res_bits)
             log("error4 codes encoded on the device")
@@ -618,7 +627,19 @@ def main():
     d_cnt = native.DeviceBuffer(nq * 4)
     d_st = native.DeviceBuffer(nq * 32)
 
+    final_rmode = 0
+    if args.final_refine > 0:
+        final_rmode = native.REFINE_ERROR4 if args.final_refine_mode == "error4" else \
+            native.REFINE_CURRENT
+        if final_rmode == native.REFINE_ERROR4:
+            calc.pure_pq_error4_load(artifacts.error4_image(e4[0], e4[1], args.dim, 8,
+                                                            args.res_bits))
+        log("final refine: top %d re-ranked in FP64 (%s)" % (args.final_refine,
+                                                              args.final_refine_mode))
+
     def refine_of(mode, L):
+        if args.final_refine > 0:
+            return min(args.final_refine, L)
         return 0 if mode == "online" else min(args.refine, L)
 
     tuned = {}  # (mode, L, learned) -> (hash_log2, probe_limit, grid_waves)
@@ -634,7 +655,7 @@ def main():
                                  correction=1 if mode == "online" else 0,
                                  refine_candidates=refine_of(mode, L) if refine is None
                                  else min(refine, L),
-                                 refine_mode=refine_mode,
+                                 refine_mode=final_rmode if refine is None else refine_mode,
                                  expand_count=args.R,
                                  d_stats=d_st.ptr if stats else None,
                                  learned_stop=1 if learned else 0,
@@ -647,7 +668,7 @@ def main():
                           correction=1 if mode == "online" else 0,
                           refine_candidates=refine_of(mode, L), expand_count=args.R,
                           d_entry_ids=es["ptr"], entry_count=es["S"], entry_stride=es["S"],
-                          learned_stop=1 if learned else 0)
+                          learned_stop=1 if learned else 0, refine_mode=final_rmode)
         tuned[(mode, L, learned)] = (t.hash_log2, t.probe_limit, t.grid_waves)
         info = dict(hash_log2=t.hash_log2, probe_limit=t.probe_limit,
                     grid_waves=t.grid_waves, ms=round(t.ms, 4),
@@ -737,7 +758,7 @@ def main():
                 pin_q.ptr, nq, L, k, native.METHOD_PQ, 1 if mode == "online" else 0,
                 refine_of(mode, L), 1, args.R, pin_ids.ptr, pin_d.ptr, pin_cnt.ptr, k, None,
                 h2, pl, gw, 1 if learned else 0, 0.9, None, None, native.ENTRY_POINT_SEED,
-                pin_e.ptr, S, S))
+                pin_e.ptr, S, S, 0, final_rmode))
         times = [[] for _ in configs]
         gc.collect()
         gc.disable()  # no collector pauses inside the timed calls
@@ -978,6 +999,7 @@ def main():
             calc.search_batch_device(dq.ptr, nq, hi_L, k, d_ids.ptr, d_d.ptr, d_cnt.ptr, k,
                                      correction=1 if mode == "online" else 0,
                                      refine_candidates=refine_of(mode, lo_L),
+                                     refine_mode=final_rmode,
                                      expand_count=args.R, learned_stop=1 if lrn else 0,
                                      stop_threshold=0.9, d_pool_sizes=d_L.ptr,
                                      d_entry_ids=es["ptr"], entry_count=es["S"],
@@ -1049,7 +1071,8 @@ def main():
 
     batching = None
     if args.batches > 0:
-        batching = batching_leg(calc, queries, ents, S, mode, refine_of, args)
+        batching = batching_leg(calc, queries, ents, S, mode, refine_of, args,
+                                refine_mode=final_rmode)
 
     # restore the main batch's results for the checks below
     run(mode, L_plain)

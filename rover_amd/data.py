@@ -44,8 +44,13 @@ def _exact_rerank(base, q, cand, k):
     return cand[order[:k]]
 
 
-def ground_truth(base, queries, k=100, device=None, margin=32):
-    """Exact top-k ids per query, int32 [nq, k]."""
+def ground_truth(base, queries, k=100, device=None, margin=32, verify=16):
+    """Exact top-k ids per query, int32 [nq, k]: float32 ||x||^2 - 2 q.x
+    screens the k + margin best per query, which are re-ranked exactly
+    (float64, ties by id).  On a device the base is screened in slices of
+    2^20 rows, 512 queries at a time (small products and top-k calls), and
+    `verify` queries spread over the set are checked against a float64 scan
+    of the whole base."""
     n = base.shape[0]
     kk = min(n, k + margin)
     if device is not None:
@@ -53,14 +58,33 @@ def ground_truth(base, queries, k=100, device=None, margin=32):
         xb = torch.from_numpy(base).to(device)
         nb = (xb * xb).sum(1)
         out = np.empty((queries.shape[0], k), dtype=np.int32)
-        step = max(1, min(4096, (1 << 30) // max(n, 1)))
+        cols, step = 1 << 20, 512
         for s in range(0, queries.shape[0], step):
             e = min(queries.shape[0], s + step)
             xq = torch.from_numpy(queries[s:e]).to(device)
-            d = nb[None, :] - 2.0 * (xq @ xb.T)
-            cand = torch.topk(d, kk, dim=1, largest=False).indices.cpu().numpy()
-            for i in range(e - s):
-                out[s + i] = _exact_rerank(base, queries[s + i], cand[i], k)
+            vals, idx = [], []
+            for c0 in range(0, n, cols):
+                c1 = min(n, c0 + cols)
+                d = nb[None, c0:c1] - 2.0 * (xq @ xb[c0:c1].T)
+                v, i = torch.topk(d, min(kk, c1 - c0), dim=1, largest=False)
+                vals.append(v)
+                idx.append(i + c0)
+            v, i = torch.cat(vals, 1), torch.cat(idx, 1)
+            sel = torch.topk(v, kk, dim=1, largest=False).indices
+            cand = torch.gather(i, 1, sel).cpu().numpy()
+            for j in range(e - s):
+                out[s + j] = _exact_rerank(base, queries[s + j], cand[j], k)
+        for qi in np.linspace(0, queries.shape[0] - 1, min(verify, queries.shape[0])).astype(int):
+            d = ((xb.double() - torch.from_numpy(queries[qi]).to(device).double()) ** 2).sum(1) \
+                if n <= (1 << 22) else torch.cat([
+                    ((xb[c0:c0 + (1 << 22)].double() -
+                      torch.from_numpy(queries[qi]).to(device).double()) ** 2).sum(1)
+                    for c0 in range(0, n, 1 << 22)])
+            dk = torch.topk(d, k, largest=False).values[-1].item()
+            got = ((base[out[qi]].astype(np.float64) - queries[qi].astype(np.float64)) ** 2
+                   ).sum(1)
+            if got.max() > dk * (1 + 1e-9) + 1e-9:
+                raise RuntimeError("ground truth check failed for query %d" % qi)
         return out
     nb = (base.astype(np.float32) ** 2).sum(1)
     out = np.empty((queries.shape[0], k), dtype=np.int32)

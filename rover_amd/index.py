@@ -553,20 +553,26 @@ def entry_list(x, count, entry, device=None, iters=10, sample=None):
     return np.array(out, dtype=np.int32)
 
 
-def kmeans_representatives(x, count, device=None, iters=10, sample=None):
+def kmeans_representatives(x, count, device=None, iters=10, sample=None, seeding="stride"):
     """(centroids float32 [count, dim], representative ids int32 [count]):
     Lloyd k-means on a uniform-stride sample, then for every centroid the
     base vector nearest to it.  Used by the entry router: each query starts
-    from the representatives of its nearest centroids."""
+    from the representatives of its nearest centroids.  seeding "stride":
+    the initial centroids are uniform-stride sample rows; "kmeans++" (device
+    only): D^2 seeding, which puts a centroid in every well-separated group
+    of the data."""
     n = x.shape[0]
     count = min(count, n)
     rows = uniform_stride_sample(n, min(n, sample or max(64 * count, 20000)))
     tr = np.ascontiguousarray(x[rows], dtype=np.float32)
     init = (np.arange(count, dtype=np.int64) * (len(tr) - 1)) // max(count - 1, 1)
+    if seeding not in ("stride", "kmeans++") or (seeding != "stride" and device is None):
+        raise ValueError("seeding: 'stride', or 'kmeans++' with a device")
     if device is not None:
         import torch
         t = torch.from_numpy(tr).to(device)
-        cent = t[torch.from_numpy(init).to(device)].clone()
+        cent = t[torch.from_numpy(init).to(device)].clone() if seeding == "stride" else \
+            _kmeanspp_seeds(torch, t, count, 11)
         for _ in range(iters):
             d = (t * t).sum(1, keepdim=True) - 2.0 * t @ cent.T + (cent * cent).sum(1)[None]
             a = torch.argmin(d, dim=1)
@@ -669,7 +675,7 @@ class PqIndexData:
 
 
 def build_pq_index(x, m, R, ks=256, res_bits=4, iters=20, train_max=100000,
-                   device=None, graph=None, n_entries=0, n_router=0):
+                   device=None, graph=None, n_entries=0, n_router=0, router_seeding="stride"):
     """n_entries > 0 adds a shared seed list of the medoid plus n_entries
     k-means representatives (see entry_list); n_router > 0 adds an entry
     router of n_router centroids (see kmeans_representatives)."""
@@ -683,7 +689,8 @@ def build_pq_index(x, m, R, ks=256, res_bits=4, iters=20, train_max=100000,
         graph = build_graph(x, R, device=device)
     entry = medoid(x)
     entries = entry_list(x, n_entries, entry, device=device) if n_entries > 0 else None
-    rc, ri = kmeans_representatives(x, n_router, device) if n_router > 0 else (None, None)
+    rc, ri = kmeans_representatives(x, n_router, device, seeding=router_seeding) \
+        if n_router > 0 else (None, None)
     return PqIndexData(graph, entry, codes, codebook, res, qp, res_bits, entries, rc, ri)
 
 
@@ -770,6 +777,24 @@ def build_graph_exact(x, R, candidates=64, seed=42, prune_mode=1):
     return dg.download(np.int32, n * R).reshape(n, R)
 
 
+def _kmeanspp_seeds(torch, tr, count, seed):
+    """k-means++ seeding on the device rows tr: the first centre uniform, each
+    next one drawn with probability proportional to the squared distance to
+    the nearest centre so far (numpy's generator from `seed` draws the
+    uniforms, so the seeds are reproducible)."""
+    rng = np.random.default_rng(seed)
+    pick = [int(rng.integers(tr.shape[0]))]
+    d2 = ((tr - tr[pick[0]]) ** 2).sum(1)
+    u = rng.random(count)
+    for i in range(1, count):
+        cs = torch.cumsum(d2.double(), 0)
+        j = int(torch.searchsorted(cs, cs[-1:] * float(u[i])).item())
+        j = min(j, tr.shape[0] - 1)
+        pick.append(j)
+        d2 = torch.minimum(d2, ((tr - tr[j]) ** 2).sum(1))
+    return tr[torch.tensor(pick, device=tr.device)].clone()
+
+
 def cluster_for_build(x, clusters, iters=8, sample=262144, seed=7, device="cuda",
                       chunk=65536):
     """A k-means clustering of x for build_graph_exact_clustered: k-means++
@@ -787,17 +812,7 @@ def cluster_for_build(x, clusters, iters=8, sample=262144, seed=7, device="cuda"
     clusters = max(1, min(int(clusters), n))
     tr = torch.from_numpy(x[uniform_stride_sample(n, min(n, max(sample, 64 * clusters)))]
                           ).to(device)
-    rng = np.random.default_rng(seed)
-    pick = [int(rng.integers(tr.shape[0]))]
-    d2 = ((tr - tr[pick[0]]) ** 2).sum(1)
-    u = rng.random(clusters)
-    for i in range(1, clusters):
-        cs = torch.cumsum(d2.double(), 0)
-        j = int(torch.searchsorted(cs, cs[-1:] * float(u[i])).item())
-        j = min(j, tr.shape[0] - 1)
-        pick.append(j)
-        d2 = torch.minimum(d2, ((tr - tr[j]) ** 2).sum(1))
-    cent = tr[torch.tensor(pick, device=tr.device)].clone()
+    cent = _kmeanspp_seeds(torch, tr, clusters, seed)
 
     def nearest(pts):
         d = (cent * cent).sum(1)[None, :] - 2.0 * pts @ cent.T

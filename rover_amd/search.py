@@ -357,13 +357,14 @@ class Calculator:
                  stride, stream=None, repeats=3, method=native.METHOD_PQ, correction=0,
                  refine_candidates=0, entry_points=1, expand_count=32,
                  d_entry_ids=None, entry_count=0, entry_stride=0,
-                 seed=native.ENTRY_POINT_SEED, learned_stop=0, stop_threshold=0.0):
+                 seed=native.ENTRY_POINT_SEED, learned_stop=0, stop_threshold=0.0,
+                 refine_mode=0):
         """rover_autotune on a device batch; returns native.RoverTuning."""
         b = self._batch_struct(d_queries, nq, pool_size, num_out, method, correction,
                                refine_candidates, entry_points, expand_count, d_ids,
                                d_dists, d_counts, stride, None, 0, 0, 0, learned_stop,
                                stop_threshold, None, None, seed, d_entry_ids, entry_count,
-                               entry_stride, 0)
+                               entry_stride, 0, refine_mode)
         t = native.RoverTuning()
         check(native.lib().rover_autotune(self.h, C.byref(b), stream, repeats, C.byref(t)),
               "rover_autotune")
