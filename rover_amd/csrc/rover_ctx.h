@@ -61,6 +61,8 @@ typedef struct RoverCtx {
     /* SQ traversal scratch: exact visited bitmaps and HBM frontiers per
        wave, and the cached live-path seed list */
     uint32_t *d_sqmd_vis;
+    uint64_t *d_visited;         /* exact visited sets of the search kernel  */
+    size_t visited_cap;
     size_t sqmd_vis_cap;
     void *d_sqmd_front;
     size_t sqmd_front_cap;
@@ -101,6 +103,13 @@ void rover_set_errf(const char *msg);
 void rover_dfree(void *p);
 /* Fills the kernel's index and batch descriptors for a batch request;
  * explicit_entries keeps the seed list already on the device. */
+/* The exact visited sets (RvBatch.visited) of a launch of `grid` waves in
+ * *buf, (re)allocated and zeroed on `stream` when too small. */
+int rover_attach_visited(uint64_t **buf, size_t *cap, const RvIndex *ix, RvBatch *rb,
+                         int grid, void *stream);
+/* rv_search_launch, with the session's exact visited sets attached first
+ * when the batch asks for them. */
+int rover_launch(RoverCtx *c, const RvIndex *ix, RvBatch *rb, int grid_waves, void *stream);
 AEEResult rover_prepare_launch(RoverCtx *c, const RoverSearchBatch *b,
                                int max_pool, int explicit_entries, RvIndex *ix,
                                RvBatch *rb);

@@ -96,6 +96,13 @@ typedef struct RvBatch {
     int refine_error4;           /* refine adds the error4 level            */
     int hash_log2;               /* visited hash has 1<<hash_log2 slots     */
     int probe_limit;
+    uint64_t *visited;           /* exact visited sets in HBM in place of the
+                                    LDS hash, or NULL: per persistent wave
+                                    (blockIdx.x) a stamp counter word, then
+                                    visited_words words of (query stamp << 32
+                                    | 32 visited bits)                      */
+    int visited_words;           /* ceil(n / 32)                            */
+    int visited_exact;           /* host: attach exact sets before launch   */
     int32_t *out_ids;            /* [nq][out_stride]                        */
     float *out_dists;            /* [nq][out_stride]                        */
     int32_t *out_counts;         /* [nq] count or RV_QUERY_* status         */

@@ -177,6 +177,7 @@ int calculator_close(remote_handle64 h) {
     dfree(c->d_route_dist);
     dfree(c->d_route_ent);
     dfree(c->d_sqmd_vis);
+    dfree(c->d_visited);
     dfree(c->d_sqmd_front);
     if (c->stream) {
         rv_stream_synchronize(c->stream);
@@ -861,6 +862,36 @@ static int fill_index(const RoverCtx *c, int method, int correction,
     return 0;
 }
 
+int rover_attach_visited(uint64_t **buf, size_t *cap, const RvIndex *ix, RvBatch *rb,
+                         int grid, void *stream) {
+    const size_t words = ((size_t)ix->n + 31) / 32;
+    const size_t bytes = (size_t)(grid > 0 ? grid : 1) * (words + 1) * sizeof(uint64_t);
+    if (bytes > *cap) {
+        dfree(*buf);
+        *buf = NULL;
+        *cap = 0;
+        if (rv_malloc((void **)buf, bytes)) return -1;
+        *cap = bytes;
+        /* stamp 0 marks a word no query has touched */
+        if (rv_fill_zero(*buf, bytes, stream)) return -1;
+    }
+    rb->visited = *buf;
+    rb->visited_words = (int)words;
+    return 0;
+}
+
+int rover_launch(RoverCtx *c, const RvIndex *ix, RvBatch *rb, int grid_waves, void *stream) {
+    if (rb->visited_exact && rb->nq > 0) {
+        int grid = grid_waves > 0 ? grid_waves : rv_search_prepare(ix, rb);
+        if (grid <= 0) return -1;
+        if (grid > rb->nq) grid = rb->nq;
+        if (rover_attach_visited(&c->d_visited, &c->visited_cap, ix, rb, grid, stream))
+            return -1;
+        grid_waves = grid;
+    }
+    return rv_search_launch(ix, rb, grid_waves, stream);
+}
+
 AEEResult rover_prepare_launch(RoverCtx *c, const RoverSearchBatch *b,
                                int max_pool, int explicit_entries, RvIndex *ix,
                                RvBatch *rb) {
@@ -868,6 +899,7 @@ AEEResult rover_prepare_launch(RoverCtx *c, const RoverSearchBatch *b,
     if (b->entry_ids && b->entry_count <= 0 && b->entry_stride <= 0)
         return ROVER_EBADPARM;
     if (b->num_queries < 0 || (!b->entry_ids && b->entry_points <= 0) ||
+        (b->hash_log2 < 0 && b->hash_log2 != ROVER_HASH_EXACT) ||
         b->expand_count <= 0 ||
         max_pool <= 0 || max_pool > ROVER_MAX_POOL || b->out_stride <= 0 ||
         !b->out_ids || !b->out_dists || !b->out_counts ||
@@ -919,7 +951,12 @@ AEEResult rover_prepare_launch(RoverCtx *c, const RoverSearchBatch *b,
                      : (b->correction ? RV_SCORE_ADC_ONLINE : RV_SCORE_ADC);
     rb->refine_candidates = b->refine_candidates;
     rb->refine_error4 = b->refine_mode == ROVER_REFINE_ERROR4;
-    rb->hash_log2 = b->hash_log2 > 0 ? b->hash_log2 : default_hash_log2(max_pool);
+    if (b->hash_log2 == ROVER_HASH_EXACT) {
+        rb->hash_log2 = 4; /* the LDS hash is not used */
+        rb->visited_exact = 1;
+    } else {
+        rb->hash_log2 = b->hash_log2 > 0 ? b->hash_log2 : default_hash_log2(max_pool);
+    }
     rb->probe_limit = b->probe_limit > 0 ? b->probe_limit : 8;
     rb->out_ids = b->out_ids;
     rb->out_dists = b->out_dists;
@@ -1270,7 +1307,7 @@ static AEEResult launch_batch_on(RoverCtx *c, const RoverSearchBatch *b_in,
     }
     rc = rover_prepare_launch(c, b, max_pool, explicit_entries, &ix, &rb);
     if (rc != ROVER_SUCCESS) return rc;
-    if (rv_search_launch(&ix, &rb, b->grid_waves, stream)) {
+    if (rover_launch(c, &ix, &rb, b->grid_waves, stream)) {
         set_err("search launch");
         return ROVER_EFAILED;
     }
@@ -1329,6 +1366,12 @@ AEEResult rover_search_batch_device_noalloc(remote_handle64 h, const RoverSearch
     pthread_mutex_lock(&c->lock);
     rc = rover_prepare_launch(c, b, b->pool_size, 1, &ix, &rb);
     if (rc != ROVER_SUCCESS) goto out;
+    if (rb.visited_exact) {
+        snprintf(g_err, sizeof(g_err),
+                 "search_noalloc: exact visited sets need a scratch; use an LDS hash");
+        rc = ROVER_EBADPARM;
+        goto out;
+    }
     rc = ROVER_EFAILED;
     rb.queue = work;
     if (!b->entry_ids) {
@@ -1361,6 +1404,8 @@ typedef struct RoverGraph {
     void *exec;         /* instantiated HIP graph                    */
     int32_t *d_queue;   /* the graph's own work counter              */
     int32_t *d_entries; /* copy of the LCG seed list, when used      */
+    uint64_t *d_visited; /* the graph's own exact visited sets       */
+    size_t visited_cap;
 } RoverGraph;
 
 static void graph_free(RoverGraph *g) {
@@ -1368,6 +1413,7 @@ static void graph_free(RoverGraph *g) {
     if (g->exec) rv_graph_destroy(g->exec);
     dfree(g->d_queue);
     dfree(g->d_entries);
+    dfree(g->d_visited);
     g->magic = 0;
     free(g);
 }
@@ -1413,6 +1459,13 @@ AEEResult rover_search_graph_create(remote_handle64 h, const RoverSearchBatch *b
         goto fail;
     }
     if (b->grid_waves > 0) grid = b->grid_waves;
+    if (grid > rb.nq) grid = rb.nq;
+    if (rb.visited_exact &&
+        (rover_attach_visited(&g->d_visited, &g->visited_cap, &ix, &rb, grid, stream) ||
+         rv_stream_synchronize(stream))) {
+        set_err("graph visited sets");
+        goto fail;
+    }
     if (rv_graph_begin(stream)) {
         set_err("graph capture");
         goto fail;
@@ -1455,19 +1508,20 @@ AEEResult rover_search_graph_destroy(void *graph) {
 /* ------------------------------------------------------------ autotuner */
 
 /* Mean launch time of `repeats` launches after one warm-up, or -1. */
-static float time_launches(const RvIndex *ix, const RvBatch *rb, int grid, void *stream,
-                           int repeats, void *e0, void *e1) {
+static float time_launches(RoverCtx *c, const RvIndex *ix, RvBatch *rb, int grid,
+                           void *stream, int repeats, void *e0, void *e1) {
     float ms = 0.0f;
-    if (rv_search_launch(ix, rb, grid, stream) || rv_event_record(e0, stream)) return -1.0f;
+    if (rover_launch(c, ix, rb, grid, stream) || rv_event_record(e0, stream)) return -1.0f;
     for (int r = 0; r < repeats; r++)
-        if (rv_search_launch(ix, rb, grid, stream)) return -1.0f;
+        if (rover_launch(c, ix, rb, grid, stream)) return -1.0f;
     if (rv_event_record(e1, stream) || rv_event_elapsed(e0, e1, &ms)) return -1.0f;
     return ms / (float)repeats;
 }
 
 AEEResult rover_autotune(remote_handle64 h, const RoverSearchBatch *b, void *stream,
                          int repeats, RoverTuning *out) {
-    static const int hashes[5] = {0, 9, 10, 12, 14};
+    /* 0: the default LDS hash; ROVER_HASH_EXACT: exact sets in HBM */
+    static const int hashes[6] = {0, 9, 10, 12, 14, ROVER_HASH_EXACT};
     static const int probes[3] = {4, 8, 16};
     static const int fill_num[3] = {4, 3, 2}; /* grid fill in quarters */
     RoverCtx *c = ctx_of(h);
@@ -1505,14 +1559,15 @@ AEEResult rover_autotune(remote_handle64 h, const RoverSearchBatch *b, void *str
         goto out;
     }
     /* the default configuration sets the reference results and time */
-    out->hash_log2 = rb.hash_log2;
+    out->hash_log2 = rb.visited_exact ? ROVER_HASH_EXACT : rb.hash_log2;
     out->probe_limit = rb.probe_limit;
     out->grid_waves = rv_search_prepare(&ix, &rb);
     if (out->grid_waves <= 0) {
         set_err("autotune prepare");
         goto out;
     }
-    out->default_ms = time_launches(&ix, &rb, out->grid_waves, stream, repeats, e0, e1);
+    if (out->grid_waves > b->num_queries) out->grid_waves = b->num_queries;
+    out->default_ms = time_launches(c, &ix, &rb, out->grid_waves, stream, repeats, e0, e1);
     if (out->default_ms < 0.0f || rv_stream_synchronize(stream) ||
         rv_memcpy_d2h(ref_ids, rb.out_ids, bytes_out) ||
         rv_memcpy_d2h(ref_d, rb.out_dists, bytes_out) ||
@@ -1522,22 +1577,29 @@ AEEResult rover_autotune(remote_handle64 h, const RoverSearchBatch *b, void *str
     }
     out->ms = out->default_ms;
     out->tried = 1;
-    for (int hi = 0; hi < 5; hi++) {
+    for (int hi = 0; hi < 6; hi++) {
         for (int pi = 0; pi < 3; pi++) {
             for (int fi = 0; fi < 3; fi++) {
                 int full, grid;
                 float ms;
+                const int exact = hashes[hi] == ROVER_HASH_EXACT;
+                if (exact && pi > 0) continue; /* no probes in an exact set */
                 cand = rb;
-                if (hashes[hi]) cand.hash_log2 = hashes[hi];
-                cand.probe_limit = probes[pi];
-                if (cand.hash_log2 == rb.hash_log2 && cand.probe_limit == rb.probe_limit &&
+                cand.visited_exact = exact;
+                if (exact) cand.hash_log2 = 4;
+                else if (hashes[hi]) cand.hash_log2 = hashes[hi];
+                else if (rb.visited_exact) continue; /* no LDS default to try */
+                cand.probe_limit = exact ? rb.probe_limit : probes[pi];
+                if (cand.visited_exact == rb.visited_exact &&
+                    cand.hash_log2 == rb.hash_log2 && cand.probe_limit == rb.probe_limit &&
                     fi == 0)
                     continue; /* the default, already timed */
                 full = rv_search_prepare(&ix, &cand);
                 if (full <= 0) continue; /* does not fit in LDS */
                 grid = full * fill_num[fi] / 4;
                 if (grid < 1) grid = 1;
-                ms = time_launches(&ix, &cand, grid, stream, repeats, e0, e1);
+                if (grid > b->num_queries) grid = b->num_queries;
+                ms = time_launches(c, &ix, &cand, grid, stream, repeats, e0, e1);
                 if (ms < 0.0f || rv_stream_synchronize(stream) ||
                     rv_memcpy_d2h(got_ids, cand.out_ids, bytes_out) ||
                     rv_memcpy_d2h(got_d, cand.out_dists, bytes_out) ||
@@ -1554,7 +1616,7 @@ AEEResult rover_autotune(remote_handle64 h, const RoverSearchBatch *b, void *str
                 }
                 if (ms < out->ms) {
                     out->ms = ms;
-                    out->hash_log2 = cand.hash_log2;
+                    out->hash_log2 = exact ? ROVER_HASH_EXACT : cand.hash_log2;
                     out->probe_limit = cand.probe_limit;
                     out->grid_waves = grid;
                 }
@@ -1563,9 +1625,10 @@ AEEResult rover_autotune(remote_handle64 h, const RoverSearchBatch *b, void *str
     }
     /* leave the chosen configuration's results in the output buffers */
     cand = rb;
-    cand.hash_log2 = out->hash_log2;
+    cand.visited_exact = out->hash_log2 == ROVER_HASH_EXACT;
+    cand.hash_log2 = cand.visited_exact ? 4 : out->hash_log2;
     cand.probe_limit = out->probe_limit;
-    if (rv_search_launch(&ix, &cand, out->grid_waves, stream) || rv_stream_synchronize(stream)) {
+    if (rover_launch(c, &ix, &cand, out->grid_waves, stream) || rv_stream_synchronize(stream)) {
         set_err("autotune final run");
         goto out;
     }

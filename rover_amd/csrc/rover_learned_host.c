@@ -185,7 +185,7 @@ AEEResult rover_search_trace_device(remote_handle64 h, const RoverSearchBatch *b
         rb.trace_steps = d_steps;
         rb.trace_meta = d_meta;
         rb.trace_cap = trace_cap;
-        if (rover_stream_turn(c, stream) || rv_search_launch(&ix, &rb, b->grid_waves, stream) ||
+        if (rover_stream_turn(c, stream) || rover_launch(c, &ix, &rb, b->grid_waves, stream) ||
             rover_stream_done(c, stream)) {
             rover_set_err("trace search");
             rc = ROVER_EFAILED;
@@ -279,7 +279,7 @@ AEEResult rover_learned_stop_train(remote_handle64 h, RoverLearnedStopTrain *t) 
     rb.trace_steps = d_steps;
     rb.trace_meta = d_meta;
     rb.trace_cap = t->trace_cap;
-    if (rover_stream_turn(c, NULL) || rv_search_launch(&ix, &rb, 0, NULL) ||
+    if (rover_stream_turn(c, NULL) || rover_launch(c, &ix, &rb, 0, NULL) ||
         rover_stream_done(c, NULL) || rv_synchronize()) {
         rover_set_err("learned stop: trace search");
         goto out;

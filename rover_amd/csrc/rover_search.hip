@@ -114,6 +114,25 @@ __device__ __forceinline__ bool rv_visit(uint32_t *table, int log2n,
     return true;
 }
 
+// Exact visited set in HBM: word i holds (stamp << 32) | the visited bits of
+// ids 32i..32i+31 for the query stamped `stamp`; a word with another stamp
+// counts as empty.  Test-and-set by compare-and-swap (lanes of one wave may
+// hit one word); the first read bypasses the non-coherent L1.  Returns true
+// when id was not visited by this query before.
+__device__ __forceinline__ bool rv_visit_exact(uint64_t *words, uint32_t stamp, uint32_t id) {
+    unsigned long long *w = reinterpret_cast<unsigned long long *>(words + (id >> 5));
+    const uint64_t bit = 1ull << (id & 31u);
+    uint64_t old = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (;;) {
+        const uint64_t bits = (uint32_t)(old >> 32) == stamp ? (old & 0xffffffffull) : 0ull;
+        if (bits & bit) return false;
+        const uint64_t want = ((uint64_t)stamp << 32) | bits | bit;
+        const uint64_t prev = atomicCAS(w, (unsigned long long)old, (unsigned long long)want);
+        if (prev == old) return true;
+        old = prev;
+    }
+}
+
 // ----------------------------------------------------------- distance ops
 
 __device__ __forceinline__ int rv_code_get(const uint8_t *code, int s, int ks) {
@@ -615,7 +634,7 @@ template <int SCORER, int STOP>
 // the same budget, the learned-stop instantiation spills into its feature
 // and predictor code and runs slower than at 3 waves per SIMD.
 __global__ __launch_bounds__(RV_WAVE)
-__attribute__((amdgpu_waves_per_eu(SCORER == RV_SCORE_ONLINE_PAIR32 && STOP == 0 ? 4 : 1)))
+__attribute__((amdgpu_waves_per_eu((SCORER == RV_SCORE_ONLINE_PAIR32 && STOP == 0) ? 4 : ((SCORER == RV_SCORE_ONLINE_PAIR32 || SCORER == RV_SCORE_ONLINE_PAIR32_R4) && STOP == 1) ? 3 : 1)))
 void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
     extern __shared__ __align__(16) unsigned char smem[];
     float *qv = reinterpret_cast<float *>(smem + lay.off_q);
@@ -643,6 +662,16 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
         nq = live < nq ? (live > 0 ? live : 0) : nq;
     }
     nq = rv_uniform(nq);
+    // exact visited sets: this wave's region and stamp counter (kept in a
+    // register while the wave runs, stored back when it leaves)
+    const bool gvis = b.visited != nullptr;
+    uint64_t *vwords = gvis ? b.visited + (size_t)blockIdx.x * (b.visited_words + 1) : nullptr;
+    uint32_t vstamp = gvis ? rv_uniform((int)(uint32_t)vwords[0]) : 0u;
+    if (gvis) vwords += 1;
+    auto visit = [&](uint32_t id) -> bool {
+        return gvis ? rv_visit_exact(vwords, vstamp, id)
+                    : rv_visit(hash, b.hash_log2, b.probe_limit, id);
+    };
     for (;;) {
         int q = 0;
         if (lane == 0) q = atomicAdd(b.queue, 1);
@@ -661,10 +690,21 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
             continue;
         }
 
-        // ---- per-query setup: query vector, table, visited hash
+        // ---- per-query setup: query vector, table, visited set
         const float *qg = b.queries + (size_t)q * ix.dim;
         for (int d = lane; d < ix.dim; d += RV_WAVE) qv[d] = qg[d];
-        for (int i = lane; i < hash_n; i += RV_WAVE) hash[i] = RV_EMPTY_SLOT;
+        if (gvis) {
+            // a fresh stamp empties the set; after 2^32 - 1 queries on this
+            // wave the stamps would repeat, so the words are cleared first
+            vstamp++;
+            if (vstamp == 0u) {
+                for (int i = lane; i < b.visited_words; i += RV_WAVE) vwords[i] = 0ull;
+                __threadfence();
+                vstamp = 1u;
+            }
+        } else {
+            for (int i = lane; i < hash_n; i += RV_WAVE) hash[i] = RV_EMPTY_SLOT;
+        }
         __syncthreads();
         if (SCORER == RV_SCORE_ADC) {
             const int entries = ix.m * ix.ks;
@@ -821,7 +861,7 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
                 if (owner && e < b.n_entries) {
                     id = ents[e];
                     if (id < 0 || id >= ix.n) bad = true;
-                    else if (!rv_visit(hash, b.hash_log2, b.probe_limit, (uint32_t)id)) dup = true;
+                    else if (!visit((uint32_t)id)) dup = true;
                     else fresh = 1;
                 }
                 id = __shfl(id, slot);
@@ -837,7 +877,7 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
                 const int id = ents[e];
                 if (id < 0 || id >= ix.n) {
                     bad = true;
-                } else if (!rv_visit(hash, b.hash_log2, b.probe_limit, (uint32_t)id)) {
+                } else if (!visit((uint32_t)id)) {
                     dup = true;
                 } else {
                     float d = 0.0f;
@@ -875,7 +915,7 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
         // search meets again is re-scored and rejected (its key is worse
         // than the pool's worst), so results do not change; only the
         // scored/dedup counters then differ from the reference's.
-        if (status == 0 && b.n_entries * 4 > hash_n) {
+        if (status == 0 && !gvis && b.n_entries * 4 > hash_n) {
             for (int i = lane; i < hash_n; i += RV_WAVE) hash[i] = RV_EMPTY_SLOT;
             __syncthreads();
             const uint64_t *k0 = pool_keys;
@@ -958,7 +998,7 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
                 // lane i < valid_n owns neighbor i's visited test
                 int fresh = 0;
                 if (lane < valid_n) {
-                    fresh = rv_visit(hash, b.hash_log2, b.probe_limit, (uint32_t)nb) ? 1 : 0;
+                    fresh = visit((uint32_t)nb) ? 1 : 0;
                     dup = !fresh;
                 }
                 const uint64_t fmask = __ballot(fresh != 0);
@@ -1004,7 +1044,7 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
                     __syncthreads();  // cids is reused by the next expansion
                 }
             } else if (lane < valid_n) {
-                if (!rv_visit(hash, b.hash_log2, b.probe_limit, (uint32_t)nb)) {
+                if (!visit((uint32_t)nb)) {
                     dup = true;
                 } else {
                     float d = 0.0f;
@@ -1162,6 +1202,7 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
         }
         __syncthreads();
     }
+    if (gvis && lane == 0) vwords[-1] = (uint64_t)vstamp;
 }
 
 __global__ void rv_zero_kernel(uint32_t *p, size_t words) {

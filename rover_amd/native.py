@@ -16,6 +16,7 @@ ROVER_EFAILED = 1
 ROVER_ENOMEMORY = 2
 ROVER_EBADPARM = 14
 METHOD_PQ = 0
+HASH_EXACT = -1  # RoverSearchBatch.hash_log2: exact visited sets in HBM
 METHOD_SQ = 1
 ENTRY_POINT_SEED = 42
 ABI_VERSION = 1

@@ -121,6 +121,44 @@ def test_graph_replays_match_eager(setup, correction, use_rows):
         calc.search_graph_destroy(g)
 
 
+def test_graph_exact_visited_sets(setup):
+    """A graph captured with exact visited sets (its own HBM sets) replays
+    the eager results while eager exact-set launches (the session's sets)
+    run between the replays; the allocation-free entry point refuses them."""
+    calc, ix, queries, rows = setup
+    say("case graph exact visited sets")
+    q1, r1 = queries[:NQ], rows[:NQ]
+    q2, r2 = queries[NQ:], rows[NQ:]
+    ref1 = eager(calc, q1, r1, 1, True)
+    ref2 = eager(calc, q2, r2, 1, True)
+    b = Bufs(q1, r1)
+    b2 = Bufs(q2, r2)
+    stream = native.Stream()
+    g = calc.search_graph_create(b.dq.ptr, NQ, L, K, b.ids.ptr, b.d.ptr, b.cnt.ptr, K,
+                                 stream.ptr, correction=1, d_entry_ids=b.de.ptr,
+                                 entry_count=4, entry_stride=4, hash_log2=native.HASH_EXACT)
+    try:
+        for rep in range(3):
+            b.clear()
+            b2.clear()
+            calc.search_batch_device(b2.dq.ptr, NQ, L, K, b2.ids.ptr, b2.d.ptr, b2.cnt.ptr, K,
+                                     correction=1, d_entry_ids=b2.de.ptr, entry_count=4,
+                                     entry_stride=4, hash_log2=native.HASH_EXACT,
+                                     grid_waves=7 + rep)
+            native.synchronize()
+            same(b2.get(), ref2)
+            calc.search_graph_launch(g, stream.ptr)
+            stream.synchronize()
+            same(b.get(), ref1)
+    finally:
+        calc.search_graph_destroy(g)
+    with pytest.raises(native.RoverError) as e:
+        calc.search_batch_device(b.dq.ptr, NQ, L, K, b.ids.ptr, b.d.ptr, b.cnt.ptr, K,
+                                 correction=1, d_entry_ids=b.de.ptr, entry_count=4,
+                                 entry_stride=4, hash_log2=native.HASH_EXACT, noalloc=True)
+    assert e.value.code == native.ROVER_EBADPARM
+
+
 def test_graph_uneven_batch_matches_eager(setup):
     """Per-query beam width and k from device arrays, captured once and
     replayed with a new batch (sizes, L, k, queries) in the same buffers."""

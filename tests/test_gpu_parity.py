@@ -79,6 +79,31 @@ def test_lossy_visited_hash_keeps_results(golden, calc, name):
 
 
 @pytest.mark.parametrize("name", PQ_SETS)
+def test_exact_visited_sets_match_reference(golden, calc, name):
+    """Exact visited sets in HBM (ROVER_HASH_EXACT): every counter equals the
+    reference's, as with a large LDS hash; launches of 3 and of the default
+    number of waves (each wave reusing its set under new stamps) and a
+    repeated launch give the same results."""
+    g = golden(name)
+    say("case", name, "exact visited sets")
+    calc.load_pq_index(pq_data(g))
+    for mode, ep, expand, L in g["search_cases"]:
+        key = "m%d_e%d_x%d_L%d" % (mode, ep, expand, L)
+        for grid in (0, 3, 0):
+            ids, d, cnt, st = calc.search_batch(g["queries"], int(L), int(L),
+                                                correction=int(mode), entry_points=int(ep),
+                                                expand_count=int(expand), stats=True,
+                                                hash_log2=native.HASH_EXACT, grid_waves=grid)
+            for qi in range(len(g["queries"])):
+                c = int(g[key + "_counts"][qi])
+                assert cnt[qi] == c, (key, qi, cnt[qi])
+                assert np.array_equal(ids[qi, :c], g[key + "_ids"][qi][:c]), (key, qi)
+                assert np.array_equal(bits(d[qi, :c]), bits(g[key + "_dists"][qi][:c]))
+                assert tuple(int(x) for x in st[qi]) == \
+                    tuple(int(x) for x in g[key + "_stats"][qi]), (key, qi, grid)
+
+
+@pytest.mark.parametrize("name", PQ_SETS)
 def test_refine_matches_reference(golden, calc, name):
     g = golden(name)
     say("case", name, "refine")
