@@ -647,10 +647,9 @@ def main():
     def run(mode, L, stats=False, learned=False, refine=None, refine_mode=0, exact_hash=False):
         h2, pl, gw = tuned.get((mode, L, learned), (0, 0, 0))
         if exact_hash:
-            # a visited hash of 2^14 slots probed up to 64 times drops no
-            # entry at these sizes, so the counters are the distinct nodes
-            # scored and expanded (what the oracle's stats count)
-            h2, pl = 14, 64
+            # exact visited sets: the counters are the distinct nodes scored
+            # and expanded (what the oracle's stats count)
+            h2, pl = native.HASH_EXACT, 0
         calc.search_batch_device(dq.ptr, nq, L, k, d_ids.ptr, d_d.ptr, d_cnt.ptr, k,
                                  correction=1 if mode == "online" else 0,
                                  refine_candidates=refine_of(mode, L) if refine is None
@@ -1212,7 +1211,7 @@ def main():
                                      expand_count=args.R, d_stats=mst.ptr,
                                      learned_stop=1 if lrn else 0, stop_threshold=0.9,
                                      d_entry_ids=d_ent.ptr, entry_count=S, entry_stride=S,
-                                     hash_log2=14, probe_limit=64)
+                                     hash_log2=native.HASH_EXACT)
             native.synchronize()
             mi = mids.download(np.int32, nq * Lx).reshape(nq, Lx)
             mdd = md.download(np.float32, nq * Lx).reshape(nq, Lx)

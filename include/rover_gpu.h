@@ -336,7 +336,9 @@ enum {
     ROVER_QUERY_BAD_PARAM = -5
 };
 
-/* RoverSearchBatch.hash_log2: exact visited sets in HBM */
+/* RoverSearchBatch.hash_log2: exact visited sets in HBM -- per persistent
+ * wave n/4 bytes of stamped words (query stamp, 32 visited bits), tested
+ * and set by compare-and-swap; nothing is cleared between queries */
 #define ROVER_HASH_EXACT (-1)
 
 typedef struct RoverSearchBatch {
@@ -369,8 +371,8 @@ typedef struct RoverSearchBatch {
     uint64_t *out_stats;         /* [num_queries][4] or NULL                */
     /* launch shape; 0 -> tuned default.  hash_log2: the LDS visited hash
        has 2^hash_log2 slots (4..16); ROVER_HASH_EXACT: exact visited sets
-       in HBM instead (n/4 bytes per persistent wave, kept by the session),
-       so no candidate is scored twice                                       */
+       in HBM instead (kept by the session), so no candidate is scored
+       twice                                                                 */
     int hash_log2;
     int probe_limit;
     int grid_waves;
@@ -454,7 +456,8 @@ AEEResult rover_search_graph_destroy(void *graph);
 
 /* Autotuner (no reference counterpart): times the device batch `b` under
  * launch configurations -- visited set (LDS hash of 2^9, 2^10, 2^12, 2^14 slots or
- * the default, or ROVER_HASH_EXACT), probe limit (4, 8, 16) and grid fill (all query slots of every
+ * the default, or ROVER_HASH_EXACT), probe limit (4, 8, 16) and grid fill (all
+ * query slots of every
  * CU, 3/4, 1/2) -- `repeats` launches each after one warm-up, keeps only
  * configurations whose ids, distances and counts equal the default
  * configuration's, and returns the fastest.  Results are left in b's output

@@ -63,6 +63,7 @@ typedef struct RoverCtx {
     uint32_t *d_sqmd_vis;
     uint64_t *d_visited;         /* exact visited sets of the search kernel  */
     size_t visited_cap;
+    size_t visited_layout;       /* the (mode, words) the sets were set up for */
     size_t sqmd_vis_cap;
     void *d_sqmd_front;
     size_t sqmd_front_cap;
@@ -105,8 +106,8 @@ void rover_dfree(void *p);
  * explicit_entries keeps the seed list already on the device. */
 /* The exact visited sets (RvBatch.visited) of a launch of `grid` waves in
  * *buf, (re)allocated and zeroed on `stream` when too small. */
-int rover_attach_visited(uint64_t **buf, size_t *cap, const RvIndex *ix, RvBatch *rb,
-                         int grid, void *stream);
+int rover_attach_visited(uint64_t **buf, size_t *cap, size_t *layout, const RvIndex *ix,
+                         RvBatch *rb, int grid, void *stream);
 /* rv_search_launch, with the session's exact visited sets attached first
  * when the batch asks for them. */
 int rover_launch(RoverCtx *c, const RvIndex *ix, RvBatch *rb, int grid_waves, void *stream);

@@ -862,18 +862,36 @@ static int fill_index(const RoverCtx *c, int method, int correction,
     return 0;
 }
 
-int rover_attach_visited(uint64_t **buf, size_t *cap, const RvIndex *ix, RvBatch *rb,
-                         int grid, void *stream) {
+static int grow(void **d, size_t *cap, size_t bytes) {
+    if (bytes <= *cap) return 0;
+    dfree(*d);
+    *d = NULL;
+    *cap = 0;
+    if (rv_malloc(d, bytes)) return -1;
+    *cap = bytes;
+    return 0;
+}
+
+/* Per persistent wave a counter word and one 64-bit word per 32 ids.  The
+ * sets rely on the buffer's contents (stamp 0 = untouched, a stamp counter
+ * per wave), so the buffer is zeroed whenever it is (re)allocated or was
+ * last laid out for another index size. */
+int rover_attach_visited(uint64_t **buf, size_t *cap, size_t *layout, const RvIndex *ix,
+                         RvBatch *rb, int grid, void *stream) {
     const size_t words = ((size_t)ix->n + 31) / 32;
     const size_t bytes = (size_t)(grid > 0 ? grid : 1) * (words + 1) * sizeof(uint64_t);
+    const size_t key = words + 1;
     if (bytes > *cap) {
         dfree(*buf);
         *buf = NULL;
         *cap = 0;
+        *layout = 0;
         if (rv_malloc((void **)buf, bytes)) return -1;
         *cap = bytes;
-        /* stamp 0 marks a word no query has touched */
-        if (rv_fill_zero(*buf, bytes, stream)) return -1;
+    }
+    if (*layout != key) {
+        if (rv_fill_zero(*buf, *cap, stream)) return -1;
+        *layout = key;
     }
     rb->visited = *buf;
     rb->visited_words = (int)words;
@@ -885,7 +903,8 @@ int rover_launch(RoverCtx *c, const RvIndex *ix, RvBatch *rb, int grid_waves, vo
         int grid = grid_waves > 0 ? grid_waves : rv_search_prepare(ix, rb);
         if (grid <= 0) return -1;
         if (grid > rb->nq) grid = rb->nq;
-        if (rover_attach_visited(&c->d_visited, &c->visited_cap, ix, rb, grid, stream))
+        if (rover_attach_visited(&c->d_visited, &c->visited_cap, &c->visited_layout, ix, rb,
+                                 grid, stream))
             return -1;
         grid_waves = grid;
     }
@@ -987,15 +1006,6 @@ AEEResult rover_prepare_launch(RoverCtx *c, const RoverSearchBatch *b,
     return ROVER_SUCCESS;
 }
 
-static int grow(void **d, size_t *cap, size_t bytes) {
-    if (bytes <= *cap) return 0;
-    dfree(*d);
-    *d = NULL;
-    *cap = 0;
-    if (rv_malloc(d, bytes)) return -1;
-    *cap = bytes;
-    return 0;
-}
 
 /* The device router runs at the sizes its GPU tests cover: at most
  * ROVER_ROUTER_MAX_CENTROIDS centroids, queries in launches of at most
@@ -1405,7 +1415,7 @@ typedef struct RoverGraph {
     int32_t *d_queue;   /* the graph's own work counter              */
     int32_t *d_entries; /* copy of the LCG seed list, when used      */
     uint64_t *d_visited; /* the graph's own exact visited sets       */
-    size_t visited_cap;
+    size_t visited_cap, visited_layout;
 } RoverGraph;
 
 static void graph_free(RoverGraph *g) {
@@ -1461,7 +1471,8 @@ AEEResult rover_search_graph_create(remote_handle64 h, const RoverSearchBatch *b
     if (b->grid_waves > 0) grid = b->grid_waves;
     if (grid > rb.nq) grid = rb.nq;
     if (rb.visited_exact &&
-        (rover_attach_visited(&g->d_visited, &g->visited_cap, &ix, &rb, grid, stream) ||
+        (rover_attach_visited(&g->d_visited, &g->visited_cap, &g->visited_layout, &ix, &rb,
+                              grid, stream) ||
          rv_stream_synchronize(stream))) {
         set_err("graph visited sets");
         goto fail;
