@@ -488,6 +488,11 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="time the default launch configuration instead of the tuned one")
+    ap.add_argument("--plain-tuning", default="",
+                    help="H,P,G: visited hash (-1 exact), probe limit and grid waves for "
+                         "every plain launch, in place of the autotuner")
+    ap.add_argument("--learned-tuning", default="",
+                    help="H,P,G as --plain-tuning, for the learned-stop launches")
     ap.add_argument("--batches", type=int, default=48,
                     help="uneven batches replayed through one captured HIP graph "
                          "(0: skip)")
@@ -643,9 +648,12 @@ def main():
         return 0 if mode == "online" else min(args.refine, L)
 
     tuned = {}  # (mode, L, learned) -> (hash_log2, probe_limit, grid_waves)
+    # fixed launch configurations (profiler runs): every L of the mode
+    fixed = {lrn: tuple(int(v) for v in spec.split(","))
+             for lrn, spec in ((False, args.plain_tuning), (True, args.learned_tuning)) if spec}
 
     def run(mode, L, stats=False, learned=False, refine=None, refine_mode=0, exact_hash=False):
-        h2, pl, gw = tuned.get((mode, L, learned), (0, 0, 0))
+        h2, pl, gw = tuned.get((mode, L, learned), fixed.get(learned, (0, 0, 0)))
         if exact_hash:
             # exact visited sets: the counters are the distinct nodes scored
             # and expanded (what the oracle's stats count)
@@ -752,7 +760,7 @@ def main():
         per step and whether the results equal the device-resident run's."""
         structs = []
         for mode, L, learned in configs:
-            h2, pl, gw = tuned.get((mode, L, learned), (0, 0, 0))
+            h2, pl, gw = tuned.get((mode, L, learned), fixed.get(learned, (0, 0, 0)))
             structs.append(calc._batch_struct(
                 pin_q.ptr, nq, L, k, native.METHOD_PQ, 1 if mode == "online" else 0,
                 refine_of(mode, L), 1, args.R, pin_ids.ptr, pin_d.ptr, pin_cnt.ptr, k, None,
@@ -900,11 +908,13 @@ def main():
     # device-resident kernel time is measured too and sets the roofline.
     log("timing plain %s at L=%d: %d warmup + %d steps" % (mode, L_plain, args.warmup,
                                                             args.steps))
-    tuning = {"plain": autotune(mode, L_plain, False) if not args.no_autotune else None}
+    tuning = {"plain": autotune(mode, L_plain, False)
+              if not (args.no_autotune or False in fixed) else None}
     ms_plain_dev = timed(mode, L_plain, False)
     if learned:
         L_l = learned["chosen"]["L"]
-        tuning["learned"] = autotune(mode, L_l, True) if not args.no_autotune else None
+        tuning["learned"] = autotune(mode, L_l, True) \
+            if not (args.no_autotune or True in fixed) else None
         ms_l_dev = timed(mode, L_l, True)
         (ms_plain, same_plain, med_plain), (ms_l, same_l, med_l) = timed_host(
             [(mode, L_plain, False), (mode, L_l, True)])
@@ -1092,7 +1102,7 @@ def main():
                 "traffic": None, "kernel": kernel,
                 "algorithmic_bytes_per_launch": int(algo),
                 "kernel_ms": round(kernel_ms, 4),
-                "counters": "distinct scored / expanded nodes (near-exact visited hash); "
+                "counters": "distinct scored / expanded nodes (exact visited sets); "
                             "the plain search's equal the oracle's PurePqSearchStats"}
     # HBM bytes per launch come from the profiler's FETCH_SIZE / WRITE_SIZE
     # passes over this same workload (calibrated against a copy kernel),

@@ -429,9 +429,10 @@ AEEResult rover_search_batch_device(remote_handle64 h,
  * ROVER_NOALLOC_SCRATCH_BYTES large, held until the process ends.  Calls on
  * one device share that array, and it keeps the last call's contents:
  * calls that overlap on one GPU (two streams, two threads, two handles)
- * race and must be ordered by the caller.  Best-first traversal only: routed
- * entries (route_entries > 0) and the SQ heap traversals return
- * ROVER_EBADPARM, as do more entry points than the array holds. */
+ * race and must be ordered by the caller.  Best-first traversal with the
+ * LDS visited hash only: routed entries (route_entries > 0), exact visited
+ * sets (ROVER_HASH_EXACT) and the SQ heap traversals return ROVER_EBADPARM,
+ * as do more entry points than the array holds. */
 #define ROVER_NOALLOC_MAX_SEEDS 16384
 #define ROVER_NOALLOC_SCRATCH_BYTES (256 + 4 * ROVER_NOALLOC_MAX_SEEDS)
 AEEResult rover_search_batch_device_noalloc(remote_handle64 h,
