@@ -72,6 +72,9 @@ def oracle():
         L.orc_fingerprint.restype = C.c_uint64
         L.orc_search_batch.argtypes = [P, I, P, I, P, I, I, I, P, P, I, I, I, P,
                                        P, P, P]
+        L.orc_search_window.argtypes = [P, I, P, P, I, I, I, I, I, P, P, I, P, P]
+        L.orc_search_batch_window.argtypes = [P, I, P, I, P, I, I, I, P, P, I, I, I,
+                                              I, P, P, P, P]
         L.orc_build_exact.argtypes = [P, I, I, I, I, C.c_uint32, I, P]
         L.orc_rnn_build.argtypes = [P, I, I, I, I, I, I, C.c_uint32, I, P]
         L.orc_l2.argtypes = [P, I, I, I]
@@ -121,6 +124,8 @@ def reference():
         L.ref_search.argtypes = [P, I, I, P, I, P, I, I, I, P, P, I, I, I, I, I,
                                  P, P, I, F, F, P, P, I, P]
         L.ref_refine.argtypes = [P, I, I, P, P, P, I, I, P]
+        L.ref_pipeline_search.argtypes = [P, I, I, P, I, P, I, I, I, P, P, I, I, I, I,
+                                          I, I, I, P, P, P, I, P]
         L.ref_error4_build.argtypes = [P, P, P, I, P, I, I, I, P, P, P]
         L.ref_error4_file.argtypes = [P, P, C.c_long]
         L.ref_error4_file.restype = C.c_long
@@ -207,10 +212,30 @@ def search(index, scorer, query, entries, expand, L, num_out):
     return ids[:n].copy(), d[:n].copy(), stats
 
 
+def search_window(index, scorer, query, entries, expand, L, num_out, window):
+    """The live pipeline with `window` source turns in flight; returns ids,
+    distances, stats (scored, corrected, turns, dedup) and the high water."""
+    q = np.ascontiguousarray(query, dtype=np.float32)
+    e = np.ascontiguousarray(entries, dtype=np.int32)
+    ids = np.zeros(max(num_out, 1), dtype=np.int32)
+    d = np.zeros(max(num_out, 1), dtype=np.float32)
+    st = OrcStats()
+    hw = C.c_int(0)
+    n = oracle().orc_search_window(C.byref(index.struct), scorer, _p(q), _p(e), len(e),
+                                   expand, L, num_out, window, _p(ids), _p(d), num_out,
+                                   C.byref(st), C.byref(hw))
+    stats = (st.pq_scored_candidates, st.corrected_candidates,
+             st.expanded_nodes, st.visited_dedup_skips)
+    if n < 0:
+        return None, None, stats, 0
+    return ids[:n].copy(), d[:n].copy(), stats, hw.value
+
+
 def search_batch(index, scorer, queries, entries, expand, L, k, refine=0,
-                 threads=None):
+                 threads=None, window=1):
     """Many queries on host threads; L and k scalars or per-query arrays;
-    entries a shared list or an [nq, S] array of per-query rows.
+    entries a shared list or an [nq, S] array of per-query rows; window > 1
+    searches with that many source turns in flight (search_window).
     Returns ids [nq, stride], dists, counts, stats [nq, 4]."""
     q = np.ascontiguousarray(queries, dtype=np.float32)
     nq = q.shape[0]
@@ -224,10 +249,10 @@ def search_batch(index, scorer, queries, entries, expand, L, k, refine=0,
     st = (OrcStats * max(nq, 1))()
     threads = threads or os.cpu_count() or 1
     estride = e.shape[1] if e.ndim == 2 else 0
-    oracle().orc_search_batch(C.byref(index.struct), scorer, _p(q), nq, _p(e),
-                              e.shape[1] if estride else len(e), estride, expand,
-                              _p(Ls), _p(ks), stride, refine, threads, _p(ids),
-                              _p(d), _p(cnt), st)
+    oracle().orc_search_batch_window(C.byref(index.struct), scorer, _p(q), nq, _p(e),
+                                     e.shape[1] if estride else len(e), estride, expand,
+                                     _p(Ls), _p(ks), stride, refine, window, threads,
+                                     _p(ids), _p(d), _p(cnt), st)
     stats = np.array([(s.pq_scored_candidates, s.corrected_candidates,
                        s.expanded_nodes, s.visited_dedup_skips) for s in st[:nq]],
                      dtype=np.uint64).reshape(nq, 4)

@@ -353,6 +353,128 @@ int ref_search(const int32_t *graph, int degree, int n, const uint8_t *codes,
     return rc;
 }
 
+/* The reference's live pipeline for one query with `window` source turns
+ * in flight: pure_pq_cpu_search_create/reset/seed, then
+ * pure_pq_source_pipeline_step until it reports the search done, with the
+ * DSP side (pure_pq_sqmd_process_buffer: ADC of each turn's work) run on
+ * this thread between steps on every buffer the CPU side has filled -- the
+ * demo's per-query driver (rpc_demo_pure_pq.c:283-321) with its seeds
+ * scored by ADC first (:286-290).  mode 0 = correction OFF, 1 = ONLINE
+ * (residual handle).  The pool is returned sorted (pure_pq_cpu_search_finish)
+ * and cut to num_out; out_stats = (turns dispatched, work items, high water,
+ * steps). */
+int ref_pipeline_search(const int32_t *graph, int degree, int n, const uint8_t *codes,
+                        int chunk_vectors, const float *codebook, int dim, int m,
+                        int ks, const float *query, const int *entries,
+                        int entry_count, int expand_count, int pool, int num_out,
+                        int window, int lanes, int mode, void *residual,
+                        int *out_ids, float *out_dists, int out_cap,
+                        uint64_t *out_stats) {
+    const uint8_t *chunks[PURE_PQ_MAX_CODE_CHUNKS];
+    int counts[PURE_PQ_MAX_CODE_CHUNKS];
+    int nchunks = (n + chunk_vectors - 1) / chunk_vectors;
+    static const uint8_t uuid[16] = {0x52, 0x6f, 0x76, 0x65, 0x72, 0x2d, 0x47, 0x50,
+                                     0x55, 0x2d, 0x70, 0x69, 0x70, 0x65, 0x2d, 0x31};
+    PurePqQueryLutHeader hdr;
+    PurePqSqmdSharedControl *control = NULL;
+    PurePqSqmdAdcContext ctx;
+    PurePqCpuSearchState *state = NULL;
+    PurePqDistancePolicy policy;
+    PurePqSourcePipeline pl;
+    float *lut = NULL, *seed_d = NULL, *all_d = NULL;
+    int *all_ids = NULL;
+    uint64_t steps = 0;
+    int rc = -2, count;
+    if (nchunks > PURE_PQ_MAX_CODE_CHUNKS || num_out <= 0 || num_out > pool) return -2;
+    for (int c = 0; c < nchunks; c++) {
+        chunks[c] = codes + (size_t)c * chunk_vectors * retrieval_pq_code_bytes(m, ks);
+        counts[c] = n - c * chunk_vectors < chunk_vectors ? n - c * chunk_vectors
+                                                          : chunk_vectors;
+    }
+    lut = malloc((size_t)m * ks * sizeof(float));
+    seed_d = malloc((size_t)entry_count * sizeof(float));
+    all_ids = malloc((size_t)pool * sizeof(int));
+    all_d = malloc((size_t)pool * sizeof(float));
+    control = aligned_alloc(PURE_PQ_SQMD_CACHE_LINE_BYTES,
+                            PURE_PQ_SQMD_ALIGN_UP(sizeof(*control),
+                                                  PURE_PQ_SQMD_CACHE_LINE_BYTES));
+    if (!lut || !seed_d || !all_ids || !all_d || !control) goto out;
+    if (retrieval_pq_build_lut(query, codebook, m, ks, dim / m, lut)) goto out;
+    memset(&hdr, 0, sizeof(hdr));
+    hdr.m = m;
+    hdr.ks = ks;
+    hdr.lut_count = m * ks;
+    memcpy(hdr.index_uuid, uuid, 16);
+    hdr.lut_gen = 1;
+    hdr.query_gen = 1;
+    pure_pq_sqmd_control_init(control, lanes);
+    control->query_gen = 1;
+    memset(&ctx, 0, sizeof(ctx));
+    ctx.code_chunks = chunks;
+    ctx.chunk_vector_counts = counts;
+    ctx.chunk_count = nchunks;
+    ctx.chunk_vectors = chunk_vectors;
+    ctx.num_vectors = n;
+    ctx.m = m;
+    ctx.ks = ks;
+    ctx.index_uuid = uuid;
+    ctx.lut_header = &hdr;
+    ctx.lut = lut;
+    if (retrieval_pq_adc_ids_batch_scalar(chunks, counts, nchunks, chunk_vectors,
+                                          (const int32_t *)entries, entry_count, lut,
+                                          m, ks, seed_d))
+        goto out;
+    memset(&policy, 0, sizeof(policy));
+    if (mode == 1) {
+        RefResidual *r = residual;
+        if (!r || retrieval_pq_residual_query_bind(r->query, query)) goto out;
+        policy.mode = PURE_PQ_CORRECTION_ONLINE;
+        policy.evaluator = ref_online_evaluator;
+        policy.context = r;
+    }
+    rc = -1;
+    if (pure_pq_cpu_search_create(n, pool, expand_count, &state) ||
+        pure_pq_cpu_search_reset(state, n, pool, expand_count) ||
+        pure_pq_cpu_search_set_distance_policy(state, mode ? &policy : NULL) ||
+        pure_pq_cpu_search_seed(state, entries, seed_d, entry_count) ||
+        pure_pq_source_pipeline_init(&pl, window))
+        goto out;
+    for (;;) {
+        int step = pure_pq_source_pipeline_step(&pl, state, graph, degree, control, 1);
+        steps++;
+        if (step < 0) goto out;
+        if (step == 1) break;
+        for (int lane = 0; lane < control->lane_count; lane++)
+            for (int bi = 0; bi < PURE_PQ_SQMD_BUFFERS_PER_LANE; bi++) {
+                PurePqSqmdLaneBuffer *buf = &control->lanes[lane].buffers[bi];
+                if (buf->state == PURE_PQ_SQMD_BUFFER_CPU_READY &&
+                    pure_pq_sqmd_process_buffer(buf, buf->turn_id, 1, &ctx))
+                    goto out;
+            }
+        if (steps > 100000000ull) goto out;
+    }
+    count = pure_pq_cpu_search_finish(state, all_ids, all_d, pool);
+    if (count < 0) goto out;
+    rc = count < num_out ? count : num_out;
+    if (rc > out_cap) rc = out_cap;
+    memcpy(out_ids, all_ids, (size_t)rc * sizeof(int));
+    memcpy(out_dists, all_d, (size_t)rc * sizeof(float));
+    if (out_stats) {
+        out_stats[0] = pl.jobs;
+        out_stats[1] = pl.work_items;
+        out_stats[2] = (uint64_t)pl.high_water;
+        out_stats[3] = steps;
+    }
+out:
+    pure_pq_cpu_search_destroy(state);
+    free(lut);
+    free(seed_d);
+    free(all_ids);
+    free(all_d);
+    free(control);
+    return rc;
+}
+
 static int ref_refine_decode(void *provider, int id, float *out) {
     RefResidual *r = provider;
     return retrieval_pq_residual_query_decode(r->query, id, out);

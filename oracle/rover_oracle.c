@@ -618,6 +618,177 @@ done:
     return result;
 }
 
+/* The live pipeline's source window (window 1 or 2 sources in flight):
+ * pure_pq_source_pipeline_step (retrieval_sqmd_pure_pq.c:722-762) over
+ * pure_pq_cpu_search_dispatch (:495-594) and pure_pq_cpu_search_collect
+ * (:596-706), seeded by pure_pq_cpu_search_seed (:442-472), as the demo
+ * drives one query (rpc_demo_pure_pq.c:283-321).  Each step first
+ * dispatches until `window` turns are pending -- pop the frontier to the
+ * best source still in the pool (stopping, without popping, when the pool
+ * is full and the frontier top is worse than its worst), mark the source's
+ * unvisited neighbours visited and queue them as the turn's work -- and then
+ * collects the oldest turn: its work is scored, sorted by (distance, id)
+ * and inserted.  The search ends when nothing is pending and nothing can be
+ * dispatched.  With window 1 every turn is collected before the next
+ * dispatch, which is orc_search's order.  The reference's work items are
+ * scored by ADC on the DSP and then passed through the distance policy on
+ * collect; the scorer does both here.  The DSP lane buffers bound a turn's
+ * work at 32 items per lane; this restatement has no lanes and no bound.
+ * Stats: expanded_nodes counts dispatched turns, pq_scored_candidates and
+ * corrected_candidates the seeds and work items scored (corrected only with
+ * a correcting scorer), visited_dedup_skips the seeds and neighbours found
+ * visited.  *high_water receives the most turns pending at once. */
+int orc_search_window(const OrcIndex *ix, int scorer, const float *query,
+                      const int *entry_ids, int entry_count, int expand_count,
+                      int pool_capacity, int num_out, int window, int *out_ids,
+                      float *out_distances, int out_capacity, OrcStats *out_stats,
+                      int *high_water) {
+    OrcCand *pool = NULL, *front = NULL;
+    uint32_t *visited = NULL;
+    float *lut = NULL;
+    int *turn_ids[2] = {NULL, NULL};
+    int turn_n[2] = {0, 0};
+    OrcCand *items = NULL;
+    int pool_size = 0, front_size = 0, result = -1;
+    int head = 0, pending = 0, hw = 0;
+    OrcStats st;
+    OrcScorer sc;
+    memset(&st, 0, sizeof(st));
+    if (out_stats) memset(out_stats, 0, sizeof(*out_stats));
+    if (high_water) *high_water = 0;
+    if (!ix || !query || !entry_ids || entry_count <= 0 || expand_count <= 0 ||
+        pool_capacity <= 0 || num_out <= 0 || num_out > pool_capacity ||
+        window < 1 || window > 2 || !out_ids || !out_distances ||
+        out_capacity <= 0 || !ix->graph || ix->degree <= 0 || ix->num_vectors <= 0)
+        return -1;
+    if (scorer != ORC_SCORE_SQ &&
+        (!ix->codes || !ix->codebook || ix->m <= 0 ||
+         (ix->ks != 16 && ix->ks != 256) || ix->subdim <= 0 ||
+         ix->m * ix->subdim != ix->dim))
+        return -1;
+    if (scorer == ORC_SCORE_ADC_ONLINE && (!ix->res_codes || !ix->res_qparams))
+        return -1;
+    if (scorer == ORC_SCORE_SQ &&
+        (!ix->sq_codes || ix->dim > 4096 || (ix->sq_bits != 4 && ix->sq_bits != 8)))
+        return -1;
+    if (expand_count > ix->degree) expand_count = ix->degree;
+    memset(&sc, 0, sizeof(sc));
+    sc.ix = ix;
+    sc.scorer = scorer;
+    sc.query = query;
+    pool = malloc((size_t)pool_capacity * sizeof(*pool));
+    front = malloc((size_t)ix->num_vectors * sizeof(*front));
+    visited = calloc(((size_t)ix->num_vectors + 31) / 32, sizeof(*visited));
+    turn_ids[0] = malloc((size_t)expand_count * sizeof(int));
+    turn_ids[1] = malloc((size_t)expand_count * sizeof(int));
+    items = malloc((size_t)expand_count * sizeof(*items));
+    if (!pool || !front || !visited || !turn_ids[0] || !turn_ids[1] || !items) goto done;
+    if (scorer == ORC_SCORE_SQ) {
+        if (orc_sq_quantize_query(query, ix->dim, ix->sq_bits, ix->sq_scale,
+                                  ix->sq_zero, sc.qsq) != 0)
+            goto done;
+    } else {
+        lut = malloc((size_t)ix->m * ix->ks * sizeof(*lut));
+        if (!lut || orc_build_lut(query, ix->codebook, ix->m, ix->ks,
+                                  ix->subdim, lut) != 0)
+            goto done;
+        sc.lut = lut;
+    }
+
+    /* seeds (pure_pq_cpu_search_seed): duplicates skipped, each scored */
+    for (int e = 0; e < entry_count; e++) {
+        int id = entry_ids[e];
+        float dist;
+        if (id < 0 || id >= ix->num_vectors) goto done;
+        if (visited[id >> 5] & (1u << (id & 31))) {
+            st.visited_dedup_skips++;
+            continue;
+        }
+        visited[id >> 5] |= 1u << (id & 31);
+        if (orc_score(&sc, id, &dist) != 0) goto done;
+        st.pq_scored_candidates++;
+        if (scorer != ORC_SCORE_ADC) st.corrected_candidates++;
+        if (orc_pool_insert(pool, &pool_size, pool_capacity, id, dist))
+            orc_front_push(front, &front_size, id, dist);
+    }
+    if (pool_size == 0) goto done;
+
+    for (;;) {
+        /* dispatch up to the window */
+        while (pending < window) {
+            OrcCand src;
+            int found = 0, valid, slot, nb = 0;
+            const int32_t *row;
+            while (front_size > 0) {
+                if (pool_size == pool_capacity &&
+                    orc_order(front[0].distance, front[0].id, pool[0].distance,
+                              pool[0].id) > 0)
+                    break;
+                src = orc_front_pop(front, &front_size);
+                if (orc_pool_contains(pool, pool_size, src.id, src.distance)) {
+                    found = 1;
+                    break;
+                }
+            }
+            if (!found) break;
+            row = ix->graph + (size_t)src.id * ix->degree;
+            valid = orc_row_count(row, expand_count, ix->num_vectors);
+            if (valid < 0) goto done;
+            slot = (head + pending) % 2;
+            for (int j = 0; j < valid; j++) {
+                int id = row[j];
+                if (visited[id >> 5] & (1u << (id & 31))) {
+                    st.visited_dedup_skips++;
+                    continue;
+                }
+                visited[id >> 5] |= 1u << (id & 31);
+                turn_ids[slot][nb++] = id;
+            }
+            turn_n[slot] = nb;
+            st.expanded_nodes++;
+            pending++;
+            if (pending > hw) hw = pending;
+        }
+        if (pending == 0) break;
+        /* collect the oldest turn: score, sort, insert */
+        {
+            int nb = turn_n[head];
+            for (int j = 0; j < nb; j++) {
+                items[j].id = turn_ids[head][j];
+                if (orc_score(&sc, items[j].id, &items[j].distance) != 0) goto done;
+            }
+            st.pq_scored_candidates += (uint64_t)nb;
+            if (scorer != ORC_SCORE_ADC) st.corrected_candidates += (uint64_t)nb;
+            qsort(items, (size_t)nb, sizeof(*items), orc_cand_cmp);
+            for (int j = 0; j < nb; j++)
+                if (orc_pool_insert(pool, &pool_size, pool_capacity, items[j].id,
+                                    items[j].distance))
+                    orc_front_push(front, &front_size, items[j].id, items[j].distance);
+            head = (head + 1) % 2;
+            pending--;
+        }
+    }
+
+    qsort(pool, (size_t)pool_size, sizeof(*pool), orc_cand_cmp);
+    result = num_out < pool_size ? num_out : pool_size;
+    if (result > out_capacity) result = out_capacity;
+    for (int i = 0; i < result; i++) {
+        out_ids[i] = pool[i].id;
+        out_distances[i] = pool[i].distance;
+    }
+    if (out_stats) *out_stats = st;
+    if (high_water) *high_water = hw;
+done:
+    free(pool);
+    free(front);
+    free(visited);
+    free(lut);
+    free(turn_ids[0]);
+    free(turn_ids[1]);
+    free(items);
+    return result;
+}
+
 typedef struct {
     int graph, original;
     double distance;
@@ -1232,6 +1403,7 @@ typedef struct {
     OrcStats *stats;
     int next;
     int failed;
+    int window;
 } OrcBatch;
 
 static void *orc_batch_worker(void *arg) {
@@ -1261,8 +1433,11 @@ static void *orc_batch_worker(void *arg) {
         OrcStats st;
         const int *ents = b->entries + (b->entry_stride > 0 ? (size_t)q * b->entry_stride : 0);
         int ne = b->entry_stride > 0 ? b->entry_stride : b->n_entries;
-        int n = orc_search(b->ix, b->scorer, qv, ents, ne, b->expand, L, want, ids,
-                           dists, cap, &st);
+        int n = b->window > 1
+                    ? orc_search_window(b->ix, b->scorer, qv, ents, ne, b->expand, L,
+                                        want, b->window, ids, dists, cap, &st, NULL)
+                    : orc_search(b->ix, b->scorer, qv, ents, ne, b->expand, L, want,
+                                 ids, dists, cap, &st);
         if (n >= 0 && b->refine > 0) {
             n = orc_refine(b->ix, qv, ids, n, b->num_out[q], NULL, ids, dists);
         }
@@ -1285,9 +1460,21 @@ int orc_search_batch(const OrcIndex *ix, int scorer, const float *queries,
                      const int *num_out, int stride, int refine, int threads,
                      int *out_ids, float *out_d, int *out_counts,
                      OrcStats *stats) {
+    return orc_search_batch_window(ix, scorer, queries, nq, entries, n_entries,
+                                   entry_stride, expand, pool, num_out, stride,
+                                   refine, 1, threads, out_ids, out_d, out_counts,
+                                   stats);
+}
+
+int orc_search_batch_window(const OrcIndex *ix, int scorer, const float *queries,
+                            int nq, const int *entries, int n_entries,
+                            int entry_stride, int expand, const int *pool,
+                            const int *num_out, int stride, int refine, int window,
+                            int threads, int *out_ids, float *out_d, int *out_counts,
+                            OrcStats *stats) {
     OrcBatch b = {ix, scorer, queries, nq, entries, n_entries, entry_stride,
                   expand, pool, num_out, stride, refine, out_ids, out_d,
-                  out_counts, stats, 0, 0};
+                  out_counts, stats, 0, 0, window};
     pthread_t tid[256];
     if (threads < 1) threads = 1;
     if (threads > 256) threads = 256;

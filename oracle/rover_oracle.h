@@ -97,6 +97,17 @@ int orc_search(const OrcIndex *index, int scorer, const float *query,
                int pool_capacity, int num_out, int *out_ids,
                float *out_distances, int out_capacity, OrcStats *out_stats);
 
+/* The live pipeline with `window` (1 or 2) source turns in flight
+ * (pure_pq_source_pipeline_step, retrieval_sqmd_pure_pq.c:722-762): a turn
+ * is dispatched from the pool as it stands before the previous turn's work
+ * is inserted.  window 1 gives orc_search's results.  expanded_nodes counts
+ * dispatched turns; *high_water (may be NULL) the most pending at once. */
+int orc_search_window(const OrcIndex *index, int scorer, const float *query,
+                      const int *entry_ids, int entry_count, int expand_count,
+                      int pool_capacity, int num_out, int window, int *out_ids,
+                      float *out_distances, int out_capacity, OrcStats *out_stats,
+                      int *high_water);
+
 /* The SQ index's reconstruction (retrieval_final_refine_decode_sq):
  * sq_codes + residual; 0, -1 (bad parameters) or -4 (non-finite). */
 int orc_sq_decode(const OrcIndex *index, int id, float *out);
@@ -176,6 +187,14 @@ int orc_search_batch(const OrcIndex *index, int scorer, const float *queries,
                      const int *num_out, int out_stride, int refine_candidates,
                      int threads, int *out_ids, float *out_distances,
                      int *out_counts, OrcStats *out_stats);
+/* The same with every query searched by orc_search_window at `window`. */
+int orc_search_batch_window(const OrcIndex *index, int scorer, const float *queries,
+                            int num_queries, const int *entry_ids, int entry_count,
+                            int entry_stride, int expand_count,
+                            const int *pool_capacity, const int *num_out,
+                            int out_stride, int refine_candidates, int window,
+                            int threads, int *out_ids, float *out_distances,
+                            int *out_counts, OrcStats *out_stats);
 
 #ifdef __cplusplus
 }

@@ -7,9 +7,11 @@ oracle/_ref/librover_ref.so from it):
     python tests/golden/make_golden.py error4    # only the *_error4 files
     python tests/golden/make_golden.py r8        # only pq_d128_m32_r8 (+ error4)
     python tests/golden/make_golden.py sq_refine # only sq_refine_d128
+    python tests/golden/make_golden.py w2        # only the *_w2 files
 
 Every expected output in the fixtures comes from the reference's own scalar
-C: retrieval_pure_pq_search_scalar_with_policy, retrieval_pq_build_lut,
+C: retrieval_pure_pq_search_scalar_with_policy, the live pipeline with two
+source turns in flight (pure_pq_source_pipeline_step), retrieval_pq_build_lut,
 retrieval_pq_adc_ids_batch_scalar, retrieval_pq_train/encode_batch, the
 PQRES01 residual writer/reader and its evaluator/decoder,
 retrieval_final_refine (CURRENT and ERROR4), retrieval_build_error4, the
@@ -511,6 +513,59 @@ def pq_r8_case():
     error4_case("pq_d128_m32_r8")
 
 
+def pipeline_case(pq_name):
+    """The live pipeline with two source turns in flight
+    (pure_pq_source_pipeline_step, window 2) on a PQ fixture's index: whole
+    sorted pools, counts and (turns dispatched, work items, high water,
+    steps) for every search case of the fixture, written to
+    <pq_name>_w2.npz.  The residual handle is reopened from the fixture's
+    base vectors; its residual codes must equal the fixture's."""
+    ref = pyoracle.reference()
+    z = np.load(os.path.join(HERE, pq_name + ".npz"))
+    base = z["base_x16"].astype(np.float32) / 16.0
+    queries = z["queries_x16"].astype(np.float32) / 16.0
+    cb, codes, graph = z["codebook"], z["codes"], z["graph"]
+    res_bits = int(z["res_bits"])
+    n, dim = base.shape
+    m, ks, _ = cb.shape
+    R = graph.shape[1]
+    res = np.zeros_like(z["res_codes"])
+    qp = np.zeros_like(z["res_qparams"])
+    handle = ref.ref_residual_open(_p(base), _p(codes), _p(cb), n, dim, m, ks,
+                                   res_bits, _p(res), _p(qp))
+    assert handle
+    assert np.array_equal(res, z["res_codes"]) and np.array_equal(qp, z["res_qparams"])
+    entry = int(z["entry"])
+    out = {}
+    for mode, ep, expand, L in z["search_cases"]:
+        mode, ep, expand, L = int(mode), int(ep), int(expand), int(L)
+        entries = pyoracle.seed_ids(n, ep, entry, 42)
+        key = "m%d_e%d_x%d_L%d" % (mode, ep, expand, L)
+        ids_all, d_all, st_all, cnt_all = [], [], [], []
+        for q in queries:
+            ids = np.zeros(L, dtype=np.int32)
+            d = np.zeros(L, dtype=np.float32)
+            st = np.zeros(4, dtype=np.uint64)
+            rc = ref.ref_pipeline_search(_p(graph), R, n, _p(codes), 300, _p(cb), dim,
+                                         m, ks, _p(q), _p(entries), len(entries),
+                                         expand, L, L, 2, 1, mode, handle, _p(ids),
+                                         _p(d), L, _p(st))
+            assert rc >= 0, (key, rc)
+            ids_all.append(ids)
+            d_all.append(d)
+            st_all.append(st)
+            cnt_all.append(rc)
+        out[key + "_ids"] = np.stack(ids_all)
+        out[key + "_dists"] = np.stack(d_all)
+        out[key + "_stats"] = np.stack(st_all)
+        out[key + "_counts"] = np.array(cnt_all, dtype=np.int32)
+    out["search_cases"] = z["search_cases"]
+    ref.ref_residual_close(handle)
+    path = os.path.join(HERE, pq_name + "_w2.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
 def main():
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "all"],
                    check=True)
@@ -534,6 +589,10 @@ def main():
     if sys.argv[1:] == ["r8"]:
         pq_r8_case()
         return
+    if sys.argv[1:] == ["w2"]:
+        for name in ("pq_d128_m32", "pq_d128_m32_r8"):
+            pipeline_case(name)
+        return
     if sys.argv[1:] == ["error4"]:
         for name in ("pq_d128_m32", "pq_d96_m48", "pq_d64_ks16"):
             error4_case(name)
@@ -550,6 +609,8 @@ def main():
     for name in ("pq_d128_m32", "pq_d96_m48", "pq_d64_ks16"):
         error4_case(name)
     pq_r8_case()
+    for name in ("pq_d128_m32", "pq_d128_m32_r8"):
+        pipeline_case(name)
     sq_refine_case()
     sqmd_case("sqmd_d128")
     split_case("sqmd_split")
