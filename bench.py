@@ -455,6 +455,8 @@ def main():
                     help="routed: per-query rows from the k-means router; medoid: "
                          "the reference's seeds (medoid + LCG ids, --entry-points)")
     ap.add_argument("--entry-points", type=int, default=32)
+    ap.add_argument("--no-window2-leg", dest="window2_leg", action="store_false",
+                    help="skip the source-window-2 (live pipeline) plain / learned leg")
     ap.add_argument("--no-routed-leg", action="store_true",
                     help="skip the plain / learned comparison with routed entry rows")
     ap.add_argument("--Ls", default="10,12,16,20,24,32,48,64,96,128,192,256,384,512,768,1024",
@@ -511,6 +513,9 @@ def main():
                     help="write the built index, queries and ground truth (.npz)")
     ap.add_argument("--load-index", default="",
                     help="reuse an index written by --save-index (no rebuild)")
+    ap.add_argument("--dump-query-stats", default="",
+                    help="write the per-query counters of the timed plain and headline "
+                         "searches to this .npz")
     ap.add_argument("--save-model", default="",
                     help="write the chosen learned-stop model (.npz)")
     ap.add_argument("--load-model", default="",
@@ -647,13 +652,16 @@ def main():
             return min(args.final_refine, L)
         return 0 if mode == "online" else min(args.refine, L)
 
-    tuned = {}  # (mode, L, learned) -> (hash_log2, probe_limit, grid_waves)
+    tuned = {}  # (mode, L, learned, window) -> (hash_log2, probe_limit, grid_waves)
+    # source window of every launch below (1: the scalar search; 2: the live
+    # pipeline's two turns in flight, the window-2 leg)
+    win = {"w": 1}
     # fixed launch configurations (profiler runs): every L of the mode
     fixed = {lrn: tuple(int(v) for v in spec.split(","))
              for lrn, spec in ((False, args.plain_tuning), (True, args.learned_tuning)) if spec}
 
     def run(mode, L, stats=False, learned=False, refine=None, refine_mode=0, exact_hash=False):
-        h2, pl, gw = tuned.get((mode, L, learned), fixed.get(learned, (0, 0, 0)))
+        h2, pl, gw = tuned.get((mode, L, learned, win["w"]), fixed.get(learned, (0, 0, 0)))
         if exact_hash:
             # exact visited sets: the counters are the distinct nodes scored
             # and expanded (what the oracle's stats count)
@@ -668,15 +676,16 @@ def main():
                                  learned_stop=1 if learned else 0,
                                  stop_threshold=0.9, d_entry_ids=es["ptr"],
                                  entry_count=es["S"], entry_stride=es["S"], hash_log2=h2,
-                                 probe_limit=pl, grid_waves=gw)
+                                 probe_limit=pl, grid_waves=gw, source_window=win["w"])
 
     def autotune(mode, L, learned):
         t = calc.autotune(dq.ptr, nq, L, k, d_ids.ptr, d_d.ptr, d_cnt.ptr, k, repeats=3,
                           correction=1 if mode == "online" else 0,
                           refine_candidates=refine_of(mode, L), expand_count=args.R,
                           d_entry_ids=es["ptr"], entry_count=es["S"], entry_stride=es["S"],
-                          learned_stop=1 if learned else 0, refine_mode=final_rmode)
-        tuned[(mode, L, learned)] = (t.hash_log2, t.probe_limit, t.grid_waves)
+                          learned_stop=1 if learned else 0, refine_mode=final_rmode,
+                          source_window=win["w"])
+        tuned[(mode, L, learned, win["w"])] = (t.hash_log2, t.probe_limit, t.grid_waves)
         info = dict(hash_log2=t.hash_log2, probe_limit=t.probe_limit,
                     grid_waves=t.grid_waves, ms=round(t.ms, 4),
                     default_ms=round(t.default_ms, 4), tried=t.tried, rejected=t.rejected)
@@ -760,7 +769,7 @@ def main():
         per step and whether the results equal the device-resident run's."""
         structs = []
         for mode, L, learned in configs:
-            h2, pl, gw = tuned.get((mode, L, learned), fixed.get(learned, (0, 0, 0)))
+            h2, pl, gw = tuned.get((mode, L, learned, 1), fixed.get(learned, (0, 0, 0)))
             structs.append(calc._batch_struct(
                 pin_q.ptr, nq, L, k, native.METHOD_PQ, 1 if mode == "online" else 0,
                 refine_of(mode, L), 1, args.R, pin_ids.ptr, pin_d.ptr, pin_cnt.ptr, k, None,
@@ -949,6 +958,77 @@ def main():
     run(mode, L_plain, stats=True, exact_hash=True)
     native.synchronize()
     plain_exact_stats = d_st.download(np.uint64, nq * 4).reshape(nq, 4)
+    if args.dump_query_stats:
+        # per-query counters (scored, corrected, expanded, dedup) of the two
+        # timed searches, for scheduling studies
+        np.savez(args.dump_query_stats, plain=plain_exact_stats, headline=stats,
+                 plain_L=L_plain, headline_L=L, headline_search=label)
+        log("per-query counters written to", args.dump_query_stats)
+
+    # ---- window-2 leg: the live pipeline with two source turns in flight
+    # (pure_pq_source_pipeline_step, SQMD_MAX_INFLIGHT_SOURCES=2): each
+    # source is chosen before the previous turn's candidates are merged, so
+    # the merge overlaps the next neighbor-row load.  Its results are the
+    # reference pipeline's, not the scalar search's; plain and learned stop
+    # (the same model) are swept to the target and timed device-resident.
+    window2 = None
+    if args.window2_leg:
+        win["w"] = 2
+        window2 = {"what": "source window 2 (the live pipeline's two turns in flight), "
+                           "the same seeds and model; device-resident launches"}
+        w_rows, wch = sweep(mode, False)
+        if wch:
+            if not args.no_autotune:
+                window2["plain_tuning"] = autotune(mode, wch["L"], False)
+            ms_wp = timed(mode, wch["L"], False)
+            window2["plain"] = dict(L=wch["L"], recall10=round(wch["recall"], 4),
+                                    qps=round(nq / (ms_wp / 1e3), 1),
+                                    ms_per_step=round(ms_wp, 4),
+                                    mean_expanded=round(float(wch["stats"][:, 2].mean()), 2),
+                                    vs_window1_plain=round(ms_plain_dev / ms_wp, 3))
+            log("window 2, timed plain: %s" % window2["plain"])
+            window2["plain_sweep"] = w_rows
+            if learned:
+                l_rows, wlch = sweep(mode, True)
+                if wlch:
+                    if not args.no_autotune:
+                        window2["learned_tuning"] = autotune(mode, wlch["L"], True)
+                    ms_wl = timed(mode, wlch["L"], True)
+                    window2["learned"] = dict(
+                        L=wlch["L"], recall10=round(wlch["recall"], 4),
+                        qps=round(nq / (ms_wl / 1e3), 1), ms_per_step=round(ms_wl, 4),
+                        mean_expanded=round(float(wlch["stats"][:, 2].mean()), 2),
+                        speedup_vs_window2_plain=round(ms_wp / ms_wl, 3),
+                        speedup_vs_window1_plain=round(ms_plain_dev / ms_wl, 3))
+                    log("window 2, timed learned: %s" % window2["learned"])
+                window2["learned_sweep"] = l_rows
+            if not args.no_cpu:
+                # the plain window-2 results against the oracle's restatement
+                # of the pipeline (orc_search_window) on a sample of queries
+                sys.path.insert(0, os.path.join(ROOT, "oracle"))
+                import pyoracle
+                ns = min(nq, 2000)
+                run(mode, wch["L"], stats=True, exact_hash=True)
+                native.synchronize()
+                gids = d_ids.download(np.int32, nq * k).reshape(nq, k)[:ns]
+                gd = d_d.download(np.float32, nq * k).reshape(nq, k)[:ns]
+                gst = d_st.download(np.uint64, nq * 4).reshape(nq, 4)[:ns]
+                oix_w = pyoracle.Index.from_pq(ix)
+                sc_w = pyoracle.SCORE_ONLINE if mode == "online" else pyoracle.SCORE_ADC
+                oi, od, oc, ost = pyoracle.search_batch(oix_w, sc_w, queries[:ns], ents[:ns],
+                                                        args.R, wch["L"], k,
+                                                        refine=refine_of(mode, wch["L"]),
+                                                        threads=16, window=2)
+                window2["oracle_identical_frac"] = float(
+                    ((gids == oi).all(axis=1) &
+                     (gd.view(np.uint32) == od.view(np.uint32)).all(axis=1)).mean())
+                window2["oracle_stats_identical_frac"] = float(
+                    (gst.astype(np.int64) == ost.astype(np.int64)).all(axis=1).mean())
+                window2["oracle_sample"] = ns
+                log("window 2: GPU equals the oracle's pipeline on %.4f of %d queries "
+                    "(counters %.4f)" % (window2["oracle_identical_frac"], ns,
+                                         window2["oracle_stats_identical_frac"]))
+        win["w"] = 1
 
     # ---- routed leg: the same plain / learned comparison with per-query entry
     # rows from the k-means router (representatives of the centroids nearest
@@ -1312,6 +1392,7 @@ def main():
             "sq": sq_leg,
             "reference_cpu": reference_cpu,
             "routed": routed,
+            "window2": window2,
             "uneven_L": uneven,
             "launch_tuning": tuning,
             "uneven_batches": batching,

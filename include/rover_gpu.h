@@ -387,6 +387,14 @@ typedef struct RoverSearchBatch {
     /* traversal (ROVER_TRAVERSAL_*): 0 = the method's reference traversal
        (PQ: best-first in (distance, id) order; SQ: the live SQ traversal)  */
     int traversal;
+    /* source window of the best-first traversal: 0 or 1 = the scalar
+       search (retrieval_sqmd_pure_pq.c:793-969); 2 = the live pipeline with
+       two source turns in flight (pure_pq_source_pipeline_step,
+       retrieval_sqmd_pure_pq.c:722-762, SQMD_MAX_INFLIGHT_SOURCES=2 at
+       rpc_demo_pure_pq.c:422-431): each source is chosen before the previous
+       turn's candidates are inserted, which gives that pipeline's results
+       (not the scalar search's).  Not with the SQ heap traversals.          */
+    int source_window;
 } RoverSearchBatch;
 
 enum { ROVER_REFINE_CURRENT = 1, ROVER_REFINE_ERROR4 = 2 };

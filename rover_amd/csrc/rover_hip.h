@@ -127,6 +127,9 @@ typedef struct RvBatch {
                                     final top-K members were scored, ascending
                                     (min(K, pool) values, padded with the last) */
     int trace_cap;
+    int window;                  /* source turns in flight: 0/1, or 2 (the
+                                    live pipeline's lookahead); not with
+                                    trace recording                         */
 } RvBatch;
 
 /* The reference's SQ traversal with distance-only heaps: the live SQ path

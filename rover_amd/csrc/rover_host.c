@@ -931,7 +931,8 @@ AEEResult rover_prepare_launch(RoverCtx *c, const RoverSearchBatch *b,
         /* the SQ traversals run through launch_sqmd, never here */
         (b->method == ROVER_METHOD_SQ && b->traversal != ROVER_TRAVERSAL_BEST_FIRST) ||
         (b->method == ROVER_METHOD_PQ && b->traversal != ROVER_TRAVERSAL_DEFAULT &&
-         b->traversal != ROVER_TRAVERSAL_BEST_FIRST))
+         b->traversal != ROVER_TRAVERSAL_BEST_FIRST) ||
+        b->source_window < 0 || b->source_window > 2)
         return ROVER_EBADPARM;
     if (b->refine_mode == ROVER_REFINE_ERROR4) {
         if (!c->e4_loaded) {
@@ -965,6 +966,7 @@ AEEResult rover_prepare_launch(RoverCtx *c, const RoverSearchBatch *b,
     rb->n_entries = b->entry_ids ? (rb->entry_stride ? rb->entry_stride : b->entry_count)
                                  : c->entries_count;
     rb->expand_count = b->expand_count;
+    rb->window = b->source_window == 2 ? 2 : 1;
     rb->scorer = b->method == ROVER_METHOD_SQ
                      ? RV_SCORE_SQ
                      : (b->correction ? RV_SCORE_ADC_ONLINE : RV_SCORE_ADC);
@@ -1292,6 +1294,10 @@ static AEEResult launch_batch_on(RoverCtx *c, const RoverSearchBatch *b_in,
         if (b_in->traversal != ROVER_TRAVERSAL_DEFAULT &&
             b_in->traversal != ROVER_TRAVERSAL_SQMD && b_in->traversal != ROVER_TRAVERSAL_SQ_DSP)
             return ROVER_EBADPARM;
+        if (b_in->source_window > 1) {
+            set_err("search: the SQ heap traversals have no source window");
+            return ROVER_EBADPARM;
+        }
         return launch_sqmd(c, b_in, max_pool,
                            b_in->traversal == ROVER_TRAVERSAL_SQ_DSP ? ROVER_TRAVERSAL_SQ_DSP
                                                                      : ROVER_TRAVERSAL_SQMD,

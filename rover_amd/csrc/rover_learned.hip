@@ -124,10 +124,13 @@ __global__ __launch_bounds__(THREADS) void rv_mlp_fwd_bwd_kernel(
             if (s < n) {
                 const int src = idx ? idx[s] : s;
                 const float yy = y[src], w = wt[src];
-                // stable BCE with logits: max(a,0) - a*y + log(1+exp(-|a|))
-                l = w * (fmaxf(a, 0.f) - a * yy + log1pf(expf(-fabsf(a))));
-                const float p = 1.f / (1.f + expf(-a));
-                dout = w * (p - yy);
+                // stable BCE with logits: max(a,0) - a*y + log(1+exp(-|a|));
+                // a record of weight 0 contributes exactly nothing
+                if (w != 0.f) {
+                    l = w * (fmaxf(a, 0.f) - a * yy + log1pf(expf(-fabsf(a))));
+                    const float p = 1.f / (1.f + expf(-a));
+                    dout = w * (p - yy);
+                }
                 if (out) out[s] = a;
             }
             DO[t] = dout;

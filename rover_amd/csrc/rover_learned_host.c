@@ -229,7 +229,12 @@ AEEResult rover_learned_stop_train(remote_handle64 h, RoverLearnedStopTrain *t) 
         rv_malloc((void **)&d_meta, (size_t)t->num_queries * RV_TRACE_META * sizeof(int32_t)) ||
         rv_malloc((void **)&d_oid, (size_t)t->num_queries * sizeof(int32_t)) ||
         rv_malloc((void **)&d_od, (size_t)t->num_queries * sizeof(float)) ||
-        rv_malloc((void **)&d_cnt, (size_t)t->num_queries * sizeof(int32_t))) {
+        rv_malloc((void **)&d_cnt, (size_t)t->num_queries * sizeof(int32_t)) ||
+        /* the trace search writes only the records it keeps; the others are
+           sampled with weight 0 and must hold finite values (0 * NaN would
+           poison the gradient), so start from zeros */
+        rv_fill_zero(d_feat, nrec * NF * sizeof(float), NULL) ||
+        rv_fill_zero(d_steps, nrec * sizeof(int32_t), NULL)) {
         rover_set_err("learned stop: trace buffers");
         goto out;
     }

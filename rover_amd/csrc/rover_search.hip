@@ -627,8 +627,14 @@ __device__ float rv_mlp_logit(const RvMlp &mlp_in, float f_lane, float *hbuf) {
 
 // STOP is the learned-stop mode (b.stop_mode): 0 plain, 1 predict and stop,
 // 2 record traces.  Separate instantiations keep the plain search free of
-// the predictor's code and registers.
-template <int SCORER, int STOP>
+// the predictor's code and registers.  WIN is the source window (b.window):
+// 1 expands the best unexpanded pool member and merges its candidates before
+// choosing the next (the scalar search); 2 is the live pipeline with two
+// source turns in flight (pure_pq_source_pipeline_step,
+// retrieval_sqmd_pure_pq.c:722-762): the next source is chosen, and its
+// neighbor row requested, before the previous turn's candidates are merged,
+// so the merge runs while the row load is in flight.
+template <int SCORER, int STOP, int WIN>
 // The plain two-lane online instantiation is held to 128 VGPRs (4 waves per
 // SIMD); the few values that spill live outside the scoring loop.  Held to
 // the same budget, the learned-stop instantiation spills into its feature
@@ -937,6 +943,19 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
         int scan = 0;
         int step = 0;
         bool stopped = false;
+        // window 2: the last dispatched turn's candidate keys wait here and
+        // are merged once the next source has been chosen
+        uint64_t pend_key = RV_KEY_MAX;
+        int pend_span = 0;
+        bool pending = false;
+        auto after_merge = [&](int first_new) {
+            if (STOP != 0) {
+                ss.improve_ema =
+                    0.7f * ss.improve_ema + (first_new < b.stop_label_k ? 0.3f : 0.0f);
+                if (first_new < b.stop_label_k) ss.last_change = step;
+                if (first_new == 0) ss.last_top1 = step;
+            }
+        };
         while (status == 0) {
             const uint64_t *keys = pool_keys;
             uint8_t *flags = pool_flags;
@@ -952,7 +971,16 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
                 scan += RV_WAVE;
             }
             next = rv_uniform(next);
-            if (next < 0) break;
+            if (next < 0) {
+                if (WIN == 1 || !pending) break;
+                // nothing to dispatch: collect the pending turn and look again
+                int first_new, admitted, unused;
+                merge(pend_key, pend_span, -1, &first_new, &admitted, &unused);
+                pending = false;
+                after_merge(first_new);
+                scan = first_new < scan ? first_new : scan;
+                continue;
+            }
 
             // learned stop: features of the current state
             if (STOP != 0 && step > 0 && step % b.stop_every == 0) {
@@ -983,6 +1011,18 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
             // neighbor row and its validity (retrieval_sqmd_pure_pq.c:59-87)
             int nb = -1;
             if (lane < expand) nb = ix.graph[(size_t)src * ix.degree + lane];
+            int src_pos = next;
+            if (WIN == 2 && pending) {
+                // collect the previous turn while the row is in flight; the
+                // source (flagged) moves with the pool
+                int first_new, admitted;
+                merge(pend_key, pend_span, next, &first_new, &admitted, &src_pos);
+                pending = false;
+                after_merge(first_new);
+                scan = first_new < src_pos + 1 ? first_new : src_pos + 1;
+            } else if (WIN == 2) {
+                scan = next + 1;
+            }
             const uint64_t pad = __ballot(lane < expand && nb == -1);
             const int valid_n = pad ? (__ffsll((unsigned long long)pad) - 1) : expand;
             const uint64_t after = __ballot(lane >= valid_n && lane < expand && nb != -1);
@@ -1071,19 +1111,26 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
             st_dedup += __popcll(__ballot(dup));
             step++;
 
-            int first_new, admitted, src_pos;
+            if (WIN == 2) {
+                // this turn's candidates wait for the next dispatch
+                pend_key = key;
+                pend_span = span;
+                pending = true;
+                continue;
+            }
+            int first_new, admitted;
             born_step = step < 65535 ? step : 65535;
             merge(key, span, next, &first_new, &admitted, &src_pos);
             (void)admitted;
-            if (STOP != 0) {
-                ss.improve_ema =
-                    0.7f * ss.improve_ema + (first_new < b.stop_label_k ? 0.3f : 0.0f);
-                if (first_new < b.stop_label_k) ss.last_change = step;
-                if (first_new == 0) ss.last_top1 = step;
-            }
+            after_merge(first_new);
             // everything left of the expanded source is expanded, except the
             // new candidates; resume the scan at the first of either
             scan = first_new < src_pos + 1 ? first_new : src_pos + 1;
+        }
+        if (WIN == 2 && pending && status == 0) {
+            // a stopped search still takes the scores of its last turn
+            int first_new, admitted, unused;
+            merge(pend_key, pend_span, -1, &first_new, &admitted, &unused);
         }
         (void)stopped;
 
@@ -1568,21 +1615,22 @@ static RvLayout rv_layout_for(const RvIndex *ix, const RvBatch *b) {
 }
 
 
-// One instantiation per (scorer, learned-stop mode).
-template <int S, int T>
+// One instantiation per (scorer, learned-stop mode, source window); traces
+// are recorded with window 1 only.
+template <int S, int T, int W>
 static int rv_occupancy_t(int lds) {
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rv_search_kernel<S, T>,
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, rv_search_kernel<S, T, W>,
                                                      RV_WAVE, lds) != hipSuccess)
         return 0;
     return blocks;
 }
 
-template <int S, int T>
+template <int S, int T, int W>
 static int rv_configure_t() {
     static int configured = 0;
     if (!configured) {
-        if (rv_check(hipFuncSetAttribute((const void *)rv_search_kernel<S, T>,
+        if (rv_check(hipFuncSetAttribute((const void *)rv_search_kernel<S, T, W>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize,
                                          160 * 1024),
                      "hipFuncSetAttribute"))
@@ -1592,32 +1640,34 @@ static int rv_configure_t() {
     return 0;
 }
 
-template <int S, int T>
+template <int S, int T, int W>
 static int rv_launch_t(const RvIndex *ix, const RvBatch *b, int grid, RvLayout lay,
                        hipStream_t s) {
-    rv_search_kernel<S, T><<<grid, RV_WAVE, lay.total, s>>>(*ix, *b, lay);
+    rv_search_kernel<S, T, W><<<grid, RV_WAVE, lay.total, s>>>(*ix, *b, lay);
     return rv_check(hipGetLastError(), "rv_search_kernel launch");
 }
 
 // op 0: configure attributes, 1: occupancy (blocks per CU), 2: launch
+template <int S, int T, int W>
+static int rv_op_t(int op, const RvIndex *ix, const RvBatch *b, int grid, RvLayout lay,
+                   hipStream_t s) {
+    return op == 0 ? rv_configure_t<S, T, W>()
+                   : op == 1 ? rv_occupancy_t<S, T, W>(lay.total)
+                             : rv_launch_t<S, T, W>(ix, b, grid, lay, s);
+}
+
 template <int S>
 static int rv_dispatch_stop(int op, const RvIndex *ix, const RvBatch *b, int grid,
                             RvLayout lay, hipStream_t s) {
-    switch (b->stop_mode) {
-    case 0:
-        return op == 0 ? rv_configure_t<S, 0>()
-                       : op == 1 ? rv_occupancy_t<S, 0>(lay.total)
-                                 : rv_launch_t<S, 0>(ix, b, grid, lay, s);
-    case 1:
-        return op == 0 ? rv_configure_t<S, 1>()
-                       : op == 1 ? rv_occupancy_t<S, 1>(lay.total)
-                                 : rv_launch_t<S, 1>(ix, b, grid, lay, s);
-    case 2:
-        return op == 0 ? rv_configure_t<S, 2>()
-                       : op == 1 ? rv_occupancy_t<S, 2>(lay.total)
-                                 : rv_launch_t<S, 2>(ix, b, grid, lay, s);
-    }
-    snprintf(g_rv_error, sizeof(g_rv_error), "rv_search: unknown stop mode");
+    const int w = b->window == 2 ? 2 : 1;
+    if (b->stop_mode == 0)
+        return w == 2 ? rv_op_t<S, 0, 2>(op, ix, b, grid, lay, s)
+                      : rv_op_t<S, 0, 1>(op, ix, b, grid, lay, s);
+    if (b->stop_mode == 1)
+        return w == 2 ? rv_op_t<S, 1, 2>(op, ix, b, grid, lay, s)
+                      : rv_op_t<S, 1, 1>(op, ix, b, grid, lay, s);
+    if (b->stop_mode == 2 && w == 1) return rv_op_t<S, 2, 1>(op, ix, b, grid, lay, s);
+    snprintf(g_rv_error, sizeof(g_rv_error), "rv_search: unknown stop mode / window");
     return -1;
 }
 
@@ -1642,6 +1692,7 @@ static int rv_resolve(const RvIndex *ix, const RvBatch *b_in, RvBatch *bb, RvLay
     if (!ix || !b_in || !b_in->queue || b_in->nq < 0 || b_in->hash_log2 < 4 ||
         b_in->hash_log2 > 16 || b_in->max_pool <= 0 || b_in->probe_limit <= 0 ||
         b_in->n_entries <= 0 || b_in->expand_count <= 0 || ix->degree > RV_WAVE ||
+        b_in->window < 0 || b_in->window > 2 || (b_in->window == 2 && b_in->stop_mode == 2) ||
         ix->dim <= 0 || ix->dim > 4096 || b_in->stop_mode < 0 || b_in->stop_mode > 2 ||
         (b_in->stop_mode && (b_in->stop_every <= 0 || b_in->stop_label_k <= 0)) ||
         (b_in->stop_mode == 2 &&

@@ -76,7 +76,8 @@ class RoverSearchBatch(C.Structure):
                 ("out_stats", C.c_void_p), ("hash_log2", C.c_int),
                 ("probe_limit", C.c_int), ("grid_waves", C.c_int),
                 ("learned_stop", C.c_int), ("stop_threshold", C.c_float),
-                ("refine_mode", C.c_int), ("traversal", C.c_int)]
+                ("refine_mode", C.c_int), ("traversal", C.c_int),
+                ("source_window", C.c_int)]
 
 
 REFINE_CURRENT, REFINE_ERROR4 = 1, 2

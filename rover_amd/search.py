@@ -231,8 +231,10 @@ class Calculator:
                       out_counts, stride, out_stats, hash_log2, probe_limit,
                       grid_waves, learned_stop, stop_threshold, pool_sizes,
                       num_outs, seed, entry_ids=None, entry_count=0,
-                      entry_stride=0, route_entries=0, refine_mode=0, traversal=0):
+                      entry_stride=0, route_entries=0, refine_mode=0, traversal=0,
+                      source_window=0):
         b = native.RoverSearchBatch()
+        b.source_window = int(source_window)
         b.refine_mode = int(refine_mode)
         b.traversal = int(traversal)
         b.entry_ids = entry_ids
@@ -268,7 +270,7 @@ class Calculator:
                      expand_count=32, stats=False, hash_log2=0, probe_limit=0,
                      grid_waves=0, learned_stop=0, stop_threshold=0.0,
                      seed=native.ENTRY_POINT_SEED, entry_ids=None,
-                     route_entries=0, refine_mode=0, traversal=0):
+                     route_entries=0, refine_mode=0, traversal=0, source_window=0):
         """Host arrays in and out.  entry_ids: explicit seeds, a list shared
         by all queries or a [nq, S] array of per-query rows (otherwise graph
         entry + LCG seeds); route_entries=S: per-query seeds from the router.  pool_size / num_out: int or per-query
@@ -292,7 +294,7 @@ class Calculator:
                                hash_log2, probe_limit, grid_waves, learned_stop,
                                stop_threshold, ptr(Ls), ptr(Ks), seed, ptr(e),
                                0 if e is None else (stride or len(e)), stride,
-                               route_entries, refine_mode, traversal)
+                               route_entries, refine_mode, traversal, source_window)
         check(native.lib().rover_search_batch(self.h, C.byref(b)), "rover_search_batch")
         return (ids, d, cnt, st) if stats else (ids, d, cnt)
 
@@ -305,7 +307,7 @@ class Calculator:
                             d_num_outs=None, stream=None,
                             seed=native.ENTRY_POINT_SEED, d_entry_ids=None,
                             entry_count=0, entry_stride=0, route_entries=0,
-                            refine_mode=0, traversal=0, noalloc=False):
+                            refine_mode=0, traversal=0, noalloc=False, source_window=0):
         """Device pointers in and out; asynchronous.  With per-query device
         arrays, pool_size must be their maximum.  noalloc=True calls
         rover_search_batch_device_noalloc (fixed per-device workspace)."""
@@ -315,7 +317,8 @@ class Calculator:
                                d_stats, hash_log2, probe_limit, grid_waves,
                                learned_stop, stop_threshold, d_pool_sizes,
                                d_num_outs, seed, d_entry_ids, entry_count,
-                               entry_stride, route_entries, refine_mode, traversal)
+                               entry_stride, route_entries, refine_mode, traversal,
+                               source_window)
         fn = native.lib().rover_search_batch_device_noalloc if noalloc else \
             native.lib().rover_search_batch_device
         check(fn(self.h, C.byref(b), stream),
@@ -329,7 +332,8 @@ class Calculator:
                             probe_limit=0, grid_waves=0, learned_stop=0,
                             stop_threshold=0.0, d_pool_sizes=None, d_num_outs=None,
                             seed=native.ENTRY_POINT_SEED, d_entry_ids=None,
-                            entry_count=0, entry_stride=0, refine_mode=0):
+                            entry_count=0, entry_stride=0, refine_mode=0,
+                            source_window=0):
         """Captures a device batch in a HIP graph (rover_search_graph_create);
         returns the graph handle for search_graph_launch."""
         b = self._batch_struct(d_queries, capacity, pool_size, num_out, method,
@@ -338,7 +342,7 @@ class Calculator:
                                d_stats, hash_log2, probe_limit, grid_waves,
                                learned_stop, stop_threshold, d_pool_sizes,
                                d_num_outs, seed, d_entry_ids, entry_count,
-                               entry_stride, 0, refine_mode)
+                               entry_stride, 0, refine_mode, 0, source_window)
         g = C.c_void_p()
         check(native.lib().rover_search_graph_create(self.h, C.byref(b), d_num_queries,
                                                      stream, C.byref(g)),
@@ -358,13 +362,13 @@ class Calculator:
                  refine_candidates=0, entry_points=1, expand_count=32,
                  d_entry_ids=None, entry_count=0, entry_stride=0,
                  seed=native.ENTRY_POINT_SEED, learned_stop=0, stop_threshold=0.0,
-                 refine_mode=0):
+                 refine_mode=0, source_window=0):
         """rover_autotune on a device batch; returns native.RoverTuning."""
         b = self._batch_struct(d_queries, nq, pool_size, num_out, method, correction,
                                refine_candidates, entry_points, expand_count, d_ids,
                                d_dists, d_counts, stride, None, 0, 0, 0, learned_stop,
                                stop_threshold, None, None, seed, d_entry_ids, entry_count,
-                               entry_stride, 0, refine_mode)
+                               entry_stride, 0, refine_mode, 0, source_window)
         t = native.RoverTuning()
         check(native.lib().rover_autotune(self.h, C.byref(b), stream, repeats, C.byref(t)),
               "rover_autotune")

@@ -143,3 +143,35 @@ def test_cfg3_learned_stop_at_full_size(big):
         (L, rp, ep, rl, el))
     assert rl >= rp - 0.02
     assert el <= 0.7 * ep
+    # the two-turn pipeline: a threshold no probability reaches gives the
+    # plain window-2 search bit for bit
+    w = calc.search_batch(q, L, K, correction=1, entry_points=32, stats=True,
+                          source_window=2)
+    wl = calc.search_batch(q, L, K, correction=1, entry_points=32, stats=True,
+                           learned_stop=1, stop_threshold=1.5, source_window=2)
+    assert np.array_equal(w[0], wl[0])
+    assert np.array_equal(bits(w[1]), bits(wl[1]))
+    assert np.array_equal(w[3], wl[3])
+
+
+def test_window2_at_full_size(big):
+    """The live pipeline's two source turns in flight at 1M x 128: ids,
+    distance bits and counters equal the oracle's window-2 search on 1,000
+    queries (exact visited sets), L = 64 and 256."""
+    calc, ixs, base, queries, gt, seeds = big
+    ix = ixs[8]
+    calc.load_pq_index(ix, with_router=False)
+    oix = pyoracle.Index.from_pq(ix)
+    q = queries[:1000]
+    for L in (64, 256):
+        say("case window 2 at 1M, L=%d" % L)
+        ids, d, cnt, st = calc.search_batch(q, L, K, correction=1, entry_points=32,
+                                            stats=True, hash_log2=native.HASH_EXACT,
+                                            source_window=2)
+        oids, od, ocnt, ost = pyoracle.search_batch(oix, pyoracle.SCORE_ONLINE, q, seeds, 32,
+                                                    L, K, threads=16, window=2)
+        assert np.array_equal(cnt, ocnt)
+        assert np.array_equal(ids, oids)
+        assert np.array_equal(bits(d), bits(od))
+        assert np.array_equal(st.astype(np.int64), ost.astype(np.int64))
+        say("  recall@10 %.4f" % data.recall_at_k(ids, cnt, gt[:1000], K))

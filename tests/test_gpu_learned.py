@@ -237,3 +237,41 @@ def test_fused_predictor_matches_cpu_forward(trained):
     say("fused predictor: %d of %d queries checked, %d stopped early" % (checked, nq, early))
     assert checked >= 0.9 * nq
     assert early >= 0.2 * checked
+
+
+def test_training_ignores_stale_device_memory(trained):
+    """The trace search writes only the records it keeps; the training must
+    not read the rest.  Device memory freed with NaN bit patterns in it (as
+    recycled allocations may hold) must not reach the weights."""
+    calc, ix, queries, gt, t0 = trained
+    say("case training over recycled NaN memory")
+    q = np.ascontiguousarray(data.gaussian_mixture(800, 128, 64, 8.0, 45, centers_seed=42))
+    t = native.RoverLearnedStopTrain()
+    t.queries = q.ctypes.data
+    t.num_queries = q.shape[0]
+    t.pool_size = 96
+    t.correction = 1
+    t.label_k = 10
+    t.entry_points = 8
+    t.expand_count = 32
+    t.stop_every = 4
+    t.min_steps = 4
+    t.trace_cap = 128
+    t.steps = 60
+    t.batch = 2048
+    t.lr = 3e-3
+    t.seed = 91
+    nrec = q.shape[0] * t.trace_cap
+    junk = [native.DeviceBuffer.from_array(np.full(nbytes, 0xFF, dtype=np.uint8))
+            for nbytes in (nrec * 16 * 4, nrec * 4, nrec * 4, nrec * 4)]
+    native.synchronize()
+    for j in junk:
+        j.close()
+    native.check(native.lib().rover_learned_stop_train(calc.h, C.byref(t)), "train")
+    P = native.lib().rover_mlp_param_count()
+    m = [np.zeros(P, np.float32), np.zeros(16, np.float32), np.zeros(16, np.float32)]
+    native.check(native.lib().rover_learned_stop_get(calc.h, m[0].ctypes.data, P,
+                                                     m[1].ctypes.data, m[2].ctypes.data),
+                 "rover_learned_stop_get")
+    assert np.isfinite(t.final_loss)
+    assert all(np.isfinite(a).all() for a in m)
