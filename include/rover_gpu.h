@@ -395,6 +395,16 @@ typedef struct RoverSearchBatch {
        turn's candidates are inserted, which gives that pipeline's results
        (not the scalar search's).  Not with the SQ heap traversals.          */
     int source_window;
+    /* waves per query (no reference counterpart): 0 = one wavefront per
+       query with adaptive scoring (the default kernel); 1, 2 or 4 = the
+       multi-wave kernel (online correction on 32 x 4-dim PQ with at most 32
+       neighbors per expansion, source window 1): a workgroup of that many
+       wavefronts shares one query's pool, wave 0 expands and merges, and all
+       waves score the unvisited neighbors, candidate_tile (16, 32 or 64; 0 =
+       32) of them per pass.  Results are identical to the default kernel's.
+       grid_waves then counts workgroups.                                    */
+    int waves_per_query;
+    int candidate_tile;
 } RoverSearchBatch;
 
 enum { ROVER_REFINE_CURRENT = 1, ROVER_REFINE_ERROR4 = 2 };
@@ -466,11 +476,13 @@ AEEResult rover_search_graph_destroy(void *graph);
 /* Autotuner (no reference counterpart): times the device batch `b` under
  * launch configurations -- visited set (LDS hash of 2^9, 2^10, 2^12, 2^14 slots or
  * the default, or ROVER_HASH_EXACT), probe limit (4, 8, 16) and grid fill (all
- * query slots of every
- * CU, 3/4, 1/2) -- `repeats` launches each after one warm-up, keeps only
- * configurations whose ids, distances and counts equal the default
- * configuration's, and returns the fastest.  Results are left in b's output
- * buffers. */
+ * query slots of every CU, 3/4, 1/2); then, with the best of those and where
+ * the batch allows it (b->waves_per_query == 0, online correction on
+ * 32 x 4-dim PQ, source window 1), waves per query (1, 2, 4) x candidate
+ * tile (16, 32, 64) of the multi-wave kernel -- `repeats` launches each
+ * after one warm-up, keeps only configurations whose ids, distances and
+ * counts equal the default configuration's, and returns the fastest.
+ * Results are left in b's output buffers. */
 typedef struct RoverTuning {
     int hash_log2;
     int probe_limit;
@@ -479,6 +491,8 @@ typedef struct RoverTuning {
     float default_ms;  /* the same for the default configuration       */
     int tried;         /* configurations timed                         */
     int rejected;      /* configurations whose results differed        */
+    int waves_per_query;  /* 0: the one-wave kernel; else 1, 2 or 4     */
+    int candidate_tile;   /* with waves_per_query > 0: 16, 32 or 64     */
 } RoverTuning;
 AEEResult rover_autotune(remote_handle64 h, const RoverSearchBatch *b, void *stream,
                          int repeats, RoverTuning *out);

@@ -130,6 +130,10 @@ typedef struct RvBatch {
     int window;                  /* source turns in flight: 0/1, or 2 (the
                                     live pipeline's lookahead); not with
                                     trace recording                         */
+    int waves_per_query;         /* 0: one wave per query, adaptive scoring;
+                                    1, 2, 4: the multi-wave kernel           */
+    int cand_tile;               /* multi-wave kernel: unvisited neighbors
+                                    scored per pass (16, 32, 64)             */
 } RvBatch;
 
 /* The reference's SQ traversal with distance-only heaps: the live SQ path

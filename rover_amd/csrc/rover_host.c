@@ -932,7 +932,11 @@ AEEResult rover_prepare_launch(RoverCtx *c, const RoverSearchBatch *b,
         (b->method == ROVER_METHOD_SQ && b->traversal != ROVER_TRAVERSAL_BEST_FIRST) ||
         (b->method == ROVER_METHOD_PQ && b->traversal != ROVER_TRAVERSAL_DEFAULT &&
          b->traversal != ROVER_TRAVERSAL_BEST_FIRST) ||
-        b->source_window < 0 || b->source_window > 2)
+        b->source_window < 0 || b->source_window > 2 ||
+        (b->waves_per_query != 0 && b->waves_per_query != 1 && b->waves_per_query != 2 &&
+         b->waves_per_query != 4) ||
+        (b->candidate_tile != 0 && b->candidate_tile != 16 && b->candidate_tile != 32 &&
+         b->candidate_tile != 64))
         return ROVER_EBADPARM;
     if (b->refine_mode == ROVER_REFINE_ERROR4) {
         if (!c->e4_loaded) {
@@ -967,6 +971,8 @@ AEEResult rover_prepare_launch(RoverCtx *c, const RoverSearchBatch *b,
                                  : c->entries_count;
     rb->expand_count = b->expand_count;
     rb->window = b->source_window == 2 ? 2 : 1;
+    rb->waves_per_query = b->waves_per_query;
+    rb->cand_tile = b->candidate_tile ? b->candidate_tile : 32;
     rb->scorer = b->method == ROVER_METHOD_SQ
                      ? RV_SCORE_SQ
                      : (b->correction ? RV_SCORE_ADC_ONLINE : RV_SCORE_ADC);
@@ -1640,11 +1646,61 @@ AEEResult rover_autotune(remote_handle64 h, const RoverSearchBatch *b, void *str
             }
         }
     }
+    /* waves per query x candidate tile (the multi-wave kernel) with the
+       visited set chosen above, every CU filled */
+    out->waves_per_query = rb.waves_per_query;
+    out->candidate_tile = rb.waves_per_query ? rb.cand_tile : 0;
+    if (rb.waves_per_query == 0) {
+        static const int waves[3] = {1, 2, 4};
+        static const int tiles[3] = {16, 32, 64};
+        for (int wi = 0; wi < 3; wi++) {
+            for (int ti = 0; ti < 3; ti++) {
+                int grid;
+                float ms;
+                cand = rb;
+                cand.visited_exact = out->hash_log2 == ROVER_HASH_EXACT;
+                cand.hash_log2 = cand.visited_exact ? 4 : out->hash_log2;
+                cand.probe_limit = out->probe_limit;
+                cand.waves_per_query = waves[wi];
+                cand.cand_tile = tiles[ti];
+                grid = rv_search_prepare(&ix, &cand);
+                if (grid <= 0) {
+                    /* not this batch's shape (or no fit): the axis does not apply */
+                    rv_set_error("");
+                    continue;
+                }
+                if (grid > b->num_queries) grid = b->num_queries;
+                ms = time_launches(c, &ix, &cand, grid, stream, repeats, e0, e1);
+                if (ms < 0.0f || rv_stream_synchronize(stream) ||
+                    rv_memcpy_d2h(got_ids, cand.out_ids, bytes_out) ||
+                    rv_memcpy_d2h(got_d, cand.out_dists, bytes_out) ||
+                    rv_memcpy_d2h(got_cnt, cand.out_counts,
+                                  (size_t)b->num_queries * sizeof(int32_t))) {
+                    set_err("autotune run");
+                    goto out;
+                }
+                out->tried++;
+                if (memcmp(got_ids, ref_ids, bytes_out) || memcmp(got_d, ref_d, bytes_out) ||
+                    memcmp(got_cnt, ref_cnt, (size_t)b->num_queries * sizeof(int32_t))) {
+                    out->rejected++;
+                    continue;
+                }
+                if (ms < out->ms) {
+                    out->ms = ms;
+                    out->grid_waves = grid;
+                    out->waves_per_query = waves[wi];
+                    out->candidate_tile = tiles[ti];
+                }
+            }
+        }
+    }
     /* leave the chosen configuration's results in the output buffers */
     cand = rb;
     cand.visited_exact = out->hash_log2 == ROVER_HASH_EXACT;
     cand.hash_log2 = cand.visited_exact ? 4 : out->hash_log2;
     cand.probe_limit = out->probe_limit;
+    cand.waves_per_query = out->waves_per_query;
+    cand.cand_tile = out->candidate_tile ? out->candidate_tile : 32;
     if (rover_launch(c, &ix, &cand, out->grid_waves, stream) || rv_stream_synchronize(stream)) {
         set_err("autotune final run");
         goto out;

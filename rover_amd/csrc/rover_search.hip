@@ -366,14 +366,21 @@ __device__ __forceinline__ bool rv_online_group(const RvIndex &ix, const float *
     const float s = qp.x, z = qp.y;
     const uint8_t *crow = ix.codes + (size_t)id * 32 + sub * MG;
     uint32_t cw[MG >= 4 ? MG / 4 : 1];
-    if (MG == 2) {
+    if constexpr (MG == 2) {
         cw[0] = *reinterpret_cast<const uint16_t *>(crow);
-    } else if (MG == 4) {
+    } else if constexpr (MG == 4) {
         cw[0] = *reinterpret_cast<const uint32_t *>(crow);
-    } else {
+    } else if constexpr (MG == 8) {
         const uint2 v = *reinterpret_cast<const uint2 *>(crow);
         cw[0] = v.x;
-        cw[MG / 4 - 1] = v.y;
+        cw[1] = v.y;
+    } else {  // MG == 16 (two lanes per candidate)
+        static_assert(MG == 16, "group size");
+        const uint4 v = *reinterpret_cast<const uint4 *>(crow);
+        cw[0] = v.x;
+        cw[1] = v.y;
+        cw[2] = v.z;
+        cw[3] = v.w;
     }
     const uint8_t *rrow = ix.res_codes + (size_t)id * (16 * RB) + sub * RBY;
     uint32_t rw[RBY / 4];
@@ -427,6 +434,43 @@ __device__ __forceinline__ bool rv_online_group(const RvIndex &ix, const float *
     return isfinite(s) && isfinite(z) && s > 0.0f;
 }
 
+// One lane per candidate for the 32-subspace x 4-dim shape, loading as it
+// goes (few registers; the multi-wave kernel's 64-candidate tile).  Same
+// sequential order as rv_online_fixed.
+template <int RB>
+__device__ __forceinline__ bool rv_online_lane(const RvIndex &ix, const float *q, int id,
+                                               float *out) {
+    const float2 qp = *reinterpret_cast<const float2 *>(ix.res_qparams + 2 * (size_t)id);
+    const float s = qp.x, z = qp.y;
+    const uint4 *crow = reinterpret_cast<const uint4 *>(ix.codes + (size_t)id * 32);
+    const uint8_t *rrow = ix.res_codes + (size_t)id * (16 * RB);
+    float acc = 0.0f;
+    for (int w = 0; w < 2; w++) {
+        const uint4 cv = crow[w];
+        const uint32_t cws[4] = {cv.x, cv.y, cv.z, cv.w};
+#pragma unroll 2
+        for (int m = 0; m < 16; m++) {
+            const int sub = 16 * w + m;
+            const int code = (cws[m >> 2] >> (8 * (m & 3))) & 255;
+            const float4 c4 = *reinterpret_cast<const float4 *>(
+                ix.codebook + ((size_t)sub * 256 + code) * 4);
+            const float cs[4] = {c4.x, c4.y, c4.z, c4.w};
+            uint32_t rw;
+            if (RB == 8) rw = *reinterpret_cast<const uint32_t *>(rrow + 4 * sub);
+            else rw = *reinterpret_cast<const uint16_t *>(rrow + 2 * sub);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int rc = RB == 8 ? (int)((rw >> (8 * k)) & 255) : (int)((rw >> (4 * k)) & 15);
+                const float rec = rv_decode_value(cs[k], rc, s, z);
+                const float diff = q[4 * sub + k] - rec;
+                acc += diff * diff;
+            }
+        }
+    }
+    *out = acc;
+    return isfinite(s) && isfinite(z) && s > 0.0f;
+}
+
 // Two-lane online scoring is used for 32-subspace x 4-dim PQ with at most
 // 32 neighbors per expansion.
 __host__ __device__ inline bool rv_online_pairable(const RvIndex &ix, int expand) {
@@ -472,7 +516,9 @@ struct RvLayout {
     int off_born;    // trace mode: step each pool entry was scored
     int off_refine;  // final refine: f64 distance bits of the refined members
     int off_stop;              // learned-stop state of the current query
-    int off_cand, off_hash, total;
+    int off_cand, off_hash;
+    int off_mw;      // several waves per query: neighbor list, keys, control
+    int total;
 };
 
 __host__ __device__ static inline int rv_align(int v, int a) {
@@ -482,7 +528,7 @@ __host__ __device__ static inline int rv_align(int v, int a) {
 __host__ __device__ static inline RvLayout rv_layout(int dim, int scorer, int m,
                                                     int ks, int max_pool,
                                                     int hash_log2, bool trace,
-                                                    int max_refine) {
+                                                    int max_refine, bool multi_wave) {
     RvLayout l;
     int off = 0;
     l.off_q = off;
@@ -510,6 +556,11 @@ __host__ __device__ static inline RvLayout rv_layout(int dim, int scorer, int m,
     off = rv_align(off + 8 * RV_WAVE, 16);
     l.off_hash = off;
     off = rv_align(off + 4 * (1 << hash_log2), 16);
+    l.off_mw = -1;
+    if (multi_wave) {
+        l.off_mw = off;
+        off = rv_align(off + 12 * RV_WAVE + 16, 16);
+    }
     l.total = off;
     return l;
 }
@@ -578,6 +629,17 @@ __device__ __forceinline__ float rv_lane_value(float v, int src) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
 }
 
+// Orders the LDS accesses of one wavefront's lanes (a wavefront-scope fence
+// and a wave barrier: no s_barrier, no wait on other waves).  The search
+// kernels whose workgroup is one wavefront use __syncthreads for this; code
+// that one wave of a larger workgroup runs alone uses this instead.
+__device__ __forceinline__ void rv_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <bool BLOCK_SYNC = true>
 __device__ float rv_mlp_logit(const RvMlp &mlp_in, float f_lane, float *hbuf) {
     const int j = rv_lane();
     // The weights are re-read at every evaluation: the empty asm hides the
@@ -602,7 +664,8 @@ __device__ float rv_mlp_logit(const RvMlp &mlp_in, float f_lane, float *hbuf) {
     }
     h1 = h1 > 0.0f ? h1 : 0.0f;
     hbuf[j] = h1;
-    __syncthreads();
+    if (BLOCK_SYNC) __syncthreads();
+    else rv_wave_sync();
     float h2 = mlp.b2[j];
     const float4 *h4 = reinterpret_cast<const float4 *>(hbuf);
     const uint2 *w2 = reinterpret_cast<const uint2 *>(mlp.w2);
@@ -615,7 +678,8 @@ __device__ float rv_mlp_logit(const RvMlp &mlp_in, float f_lane, float *hbuf) {
         h2 += rv_bf16_lo(v.y) * hv.z;
         h2 += rv_bf16_hi(v.y) * hv.w;
     }
-    __syncthreads();  // hbuf is reused by the next merge
+    if (BLOCK_SYNC) __syncthreads();  // hbuf is reused by the next merge
+    else rv_wave_sync();
     h2 = h2 > 0.0f ? h2 : 0.0f;
     float o = rv_bf16(mlp.w3[j]) * h2;
 #pragma unroll
@@ -1252,6 +1316,418 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
     if (gvis && lane == 0) vwords[-1] = (uint64_t)vstamp;
 }
 
+// ------------------------------------------------ several waves per query
+
+// The best-first search of rv_search_kernel with WPQ wavefronts per query (a
+// workgroup of 64 * WPQ lanes sharing one LDS pool), for the online
+// correction on the 32-subspace x 4-dim shape with 8- or 4-bit residuals.
+// Wave 0 owns the pool and the visited set: it picks the source, reads its
+// row, tests the neighbors and writes the unvisited ones to a compact LDS
+// list; then every wave scores that list, b.cand_tile candidates per pass
+// with 64 * WPQ / cand_tile lanes each (the grouped sequential sum, bit-
+// identical to one lane's), and writes the keys back to LDS; wave 0 merges
+// them.  Expansion order, ids, distances and counters are those of
+// rv_search_kernel.  Control flow between the workgroup barriers is decided
+// by wave 0 and read by all waves from LDS, so every wave passes the same
+// barriers.
+template <int RB, int STOP, int WPQ>
+__global__ __launch_bounds__(RV_WAVE * WPQ)
+void rv_search_mw_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    float *qv = reinterpret_cast<float *>(smem + lay.off_q);
+    uint64_t *pool_keys = reinterpret_cast<uint64_t *>(smem + lay.off_keys);
+    uint8_t *pool_flags = smem + lay.off_flags;
+    uint64_t *cand = reinterpret_cast<uint64_t *>(smem + lay.off_cand);
+    uint32_t *hash = reinterpret_cast<uint32_t *>(smem + lay.off_hash);
+    int32_t *cids = reinterpret_cast<int32_t *>(smem + lay.off_mw);
+    uint64_t *ckeys = reinterpret_cast<uint64_t *>(smem + lay.off_mw + 4 * RV_WAVE);
+    // control words: [0] query, [1] source position (-1: the search ends),
+    // [2] unvisited neighbors to score, [3] a score was not finite
+    int32_t *ctl = reinterpret_cast<int32_t *>(smem + lay.off_mw + 12 * RV_WAVE);
+    constexpr int NT = RV_WAVE * WPQ;
+    const int gl = threadIdx.x;
+    const int lane = rv_lane();
+    const int wave = gl / RV_WAVE;
+    const int hash_n = 1 << b.hash_log2;
+    const int expand = b.expand_count < ix.degree ? b.expand_count : ix.degree;
+    const int tile = b.cand_tile;
+    const int G = NT / tile;  // lanes per candidate: 1, 2, 4, 8 or 16
+
+    int nq = b.nq;
+    if (b.nq_dev) {
+        const int live = *b.nq_dev;
+        nq = live < nq ? (live > 0 ? live : 0) : nq;
+    }
+    nq = rv_uniform(nq);
+    const bool gvis = b.visited != nullptr;
+    uint64_t *vwords = gvis ? b.visited + (size_t)blockIdx.x * (b.visited_words + 1) : nullptr;
+    uint32_t vstamp = (gvis && wave == 0) ? rv_uniform((int)(uint32_t)vwords[0]) : 0u;
+    if (gvis) vwords += 1;
+    auto visit = [&](uint32_t id) -> bool {
+        return gvis ? rv_visit_exact(vwords, vstamp, id)
+                    : rv_visit(hash, b.hash_log2, b.probe_limit, id);
+    };
+    for (;;) {
+        if (gl == 0) ctl[0] = atomicAdd(b.queue, 1);
+        __syncthreads();
+        const int q = ctl[0];
+        if (q >= nq) break;
+        const int L = b.pool_sizes ? b.pool_sizes[q] : b.pool_size;
+        const int K = b.num_outs ? b.num_outs[q] : b.num_out;
+        if (L <= 0 || L > b.max_pool || K <= 0 || K > L || K > b.out_stride) {
+            if (gl == 0) b.out_counts[q] = RV_QUERY_BAD_PARAM;
+            __syncthreads();
+            continue;
+        }
+        const float *qg = b.queries + (size_t)q * ix.dim;
+        for (int d = gl; d < ix.dim; d += NT) qv[d] = qg[d];
+        if (wave == 0) {
+            if (gvis) {
+                vstamp++;
+                if (vstamp == 0u) {
+                    for (int i = lane; i < b.visited_words; i += RV_WAVE) vwords[i] = 0ull;
+                    __threadfence();
+                    vstamp = 1u;
+                }
+            } else {
+                for (int i = lane; i < hash_n; i += RV_WAVE) hash[i] = RV_EMPTY_SLOT;
+            }
+        }
+        __syncthreads();
+
+        // ---- wave 0: seeds, then the pool (held in wave 0's registers)
+        int n = 0, status = 0;
+        uint64_t st_scored = 0, st_expanded = 0, st_dedup = 0;
+        RvStopState &ss = *reinterpret_cast<RvStopState *>(smem + lay.off_stop);
+        auto merge = [&](uint64_t key, int track, int *first_new, int *tracked) {
+            if (n == L && key != RV_KEY_MAX && key >= pool_keys[L - 1]) key = RV_KEY_MAX;
+            *first_new = 0x7fffffff;
+            *tracked = track;
+            if (__ballot(key != RV_KEY_MAX) == 0ull) return;
+            key = rv_sort_halves(key);
+            if (__ballot(lane >= 32 && key != RV_KEY_MAX) != 0ull) {
+                // keys came in the upper half (seeds): move them down
+                const uint64_t up = rv_xor_lane64<32>(key);
+                key = lane < 32 ? up : RV_KEY_MAX;
+            }
+            const uint64_t prev = rv_prev_lane64(key);
+            bool valid = key != RV_KEY_MAX && !(lane > 0 && prev == key);
+            int lb = 0;
+            if (valid) {
+                lb = rv_lower_bound(pool_keys, n, key);
+                if (lb < n && pool_keys[lb] == key) valid = false;
+            }
+            const uint64_t vmask = __ballot(valid);
+            const int cv = __popcll(vmask);
+            if (cv == 0) return;
+            const int rank = __popcll(vmask & ((1ull << lane) - 1ull));
+            if (track >= 0 && track < n) {
+                const uint64_t tk = pool_keys[track];
+                *tracked = track + rv_uniform(__popcll(__ballot(valid && key < tk)));
+            }
+            // in-place move of the pool (see rv_search_kernel's merge)
+            const int lb_min =
+                __builtin_amdgcn_readlane(lb, __ffsll((unsigned long long)vmask) - 1);
+            uint64_t rem = vmask;
+            int above = 0;
+            for (int j0 = ((n - 1) >> 6) << 6; j0 >= 0 && j0 + RV_WAVE > lb_min;
+                 j0 -= RV_WAVE) {
+                const int j = j0 + lane;
+                int right = above;
+                while (rem) {
+                    const int top = 63 - __clzll((long long)rem);
+                    const int lbc = __builtin_amdgcn_readlane(lb, top);
+                    if (lbc < j0) break;
+                    right += lbc > j ? 1 : 0;
+                    above++;
+                    rem &= ~(1ull << top);
+                }
+                const int shift = cv - right;
+                if (j < n && shift > 0) {
+                    const uint64_t pk = pool_keys[j];
+                    const uint8_t pf = pool_flags[j];
+                    const int np = j + shift;
+                    if (np < L) {
+                        pool_keys[np] = pk;
+                        pool_flags[np] = pf;
+                    }
+                }
+            }
+            int pos = 0x7fffffff;
+            if (valid) {
+                pos = lb + rank;
+                if (pos < L) {
+                    pool_keys[pos] = key;
+                    pool_flags[pos] = 0;
+                } else {
+                    pos = 0x7fffffff;
+                }
+            }
+            *first_new = __builtin_amdgcn_readlane(pos, __ffsll((unsigned long long)vmask) - 1);
+            n = (n + cv) < L ? (n + cv) : L;
+            rv_wave_sync();
+        };
+        if (wave == 0) {
+            if (STOP != 0) {
+                ss.d_entry = 0.0f;
+                ss.d_entry2 = 0.0f;
+                ss.top1_0 = 0.0f;
+                ss.topk_0 = 0.0f;
+                ss.improve_ema = 1.0f;
+                ss.last_change = 0;
+                ss.last_top1 = 0;
+            }
+            // seeds (rpc_demo_pure_pq.c:192-208): 32 per round, lane slot
+            // owns the entry, lane slot + 32 its key
+            const int32_t *ents =
+                b.entries + (b.entry_stride > 0 ? (size_t)q * b.entry_stride : 0);
+            const int slot = lane & 31;
+            for (int e0 = 0; e0 < b.n_entries && status == 0; e0 += 32) {
+                const int e = e0 + slot;
+                uint64_t key = RV_KEY_MAX;
+                bool bad = false, nonfinite = false, dup = false;
+                int id = -1, fresh = 0;
+                if (lane < 32 && e < b.n_entries) {
+                    id = ents[e];
+                    if (id < 0 || id >= ix.n) bad = true;
+                    else if (!visit((uint32_t)id)) dup = true;
+                    else fresh = 1;
+                }
+                id = __shfl(id, slot);
+                fresh = __shfl(fresh, slot);
+                const bool active = fresh != 0;
+                float d = 0.0f;
+                const bool ok = rv_online_pair32<RB>(ix, qv, id, active, lane >= 32, &d);
+                if (lane >= 32 && active) {
+                    if (!ok || !isfinite(d) || d == FLT_MAX) nonfinite = true;
+                    else key = rv_key(d + 0.0f, id);
+                }
+                if (__ballot(bad) != 0ull) status = RV_QUERY_BAD_ENTRY;
+                else if (__ballot(nonfinite) != 0ull) status = RV_QUERY_NONFINITE;
+                if (status) break;
+                st_scored += __popcll(__ballot(key != RV_KEY_MAX));
+                st_dedup += __popcll(__ballot(dup));
+                if (STOP != 0 && e0 == 0 && lane == 32 && key != RV_KEY_MAX)
+                    ss.d_entry = rv_key_dist(key);
+                if (STOP != 0 && e0 == 0 && lane == 33 && key != RV_KEY_MAX)
+                    ss.d_entry2 = rv_key_dist(key);
+                int fn, tr;
+                merge(key, -1, &fn, &tr);
+            }
+            if (status == 0 && n == 0) status = RV_QUERY_BAD_ENTRY;
+            if (status == 0 && !gvis && b.n_entries * 4 > hash_n) {
+                for (int i = lane; i < hash_n; i += RV_WAVE) hash[i] = RV_EMPTY_SLOT;
+                rv_wave_sync();
+                for (int i = lane; i < n; i += RV_WAVE)
+                    rv_visit(hash, b.hash_log2, b.probe_limit, (uint32_t)rv_key_id(pool_keys[i]));
+                rv_wave_sync();
+            }
+            if (STOP != 0 && status == 0) {
+                ss.d_entry = rv_lane_value(ss.d_entry, 32);
+                ss.d_entry2 = rv_lane_value(ss.d_entry2, 33);
+                ss.top1_0 = rv_key_dist(pool_keys[0]);
+                ss.topk_0 = rv_key_dist(pool_keys[(n < b.stop_label_k ? n : b.stop_label_k) - 1]);
+            }
+        }
+
+        // ---- main loop
+        int scan = 0, step = 0, next = -1;
+        for (;;) {
+            if (wave == 0) {
+                next = -1;
+                if (status == 0) {
+                    while (scan < n) {
+                        const int p = scan + lane;
+                        const bool open = p < n && pool_flags[p] == 0;
+                        const uint64_t m = __ballot(open);
+                        if (m) {
+                            next = scan + __ffsll((unsigned long long)m) - 1;
+                            break;
+                        }
+                        scan += RV_WAVE;
+                    }
+                    next = rv_uniform(next);
+                }
+                if (STOP == 1 && next >= 0 && step > 0 && step % b.stop_every == 0 &&
+                    step >= b.stop_min_steps) {
+                    const float f =
+                        rv_feature(ss, pool_keys, n, b.stop_label_k, step, next, lane);
+                    const float logit =
+                        rv_mlp_logit<false>(b.mlp, f, reinterpret_cast<float *>(cand));
+                    const float p = 1.0f / (1.0f + __expf(-logit));
+                    if (rv_uniform(p >= b.stop_threshold ? 1 : 0)) next = -1;
+                }
+                int nf = 0;
+                if (next >= 0) {
+                    const int src = rv_key_id(pool_keys[next]);
+                    if (lane == 0) pool_flags[next] = 1;
+                    int nb = -1;
+                    if (lane < expand) nb = ix.graph[(size_t)src * ix.degree + lane];
+                    const uint64_t pad = __ballot(lane < expand && nb == -1);
+                    const int valid_n = pad ? (__ffsll((unsigned long long)pad) - 1) : expand;
+                    const uint64_t after = __ballot(lane >= valid_n && lane < expand && nb != -1);
+                    const uint64_t oob = __ballot(lane < valid_n && (nb < 0 || nb >= ix.n));
+                    if (after | oob) {
+                        status = RV_QUERY_BAD_ROW;
+                        next = -1;
+                    } else {
+                        int fresh = 0;
+                        bool dup = false;
+                        if (lane < valid_n) {
+                            fresh = visit((uint32_t)nb) ? 1 : 0;
+                            dup = !fresh;
+                        }
+                        const uint64_t fmask = __ballot(fresh != 0);
+                        nf = __popcll(fmask);
+                        if (fresh) cids[__popcll(fmask & ((1ull << lane) - 1ull))] = nb;
+                        st_expanded++;
+                        st_scored += nf;
+                        st_dedup += __popcll(__ballot(dup));
+                        step++;
+                    }
+                }
+                if (lane == 0) {
+                    ctl[1] = next;
+                    ctl[2] = nf;
+                    ctl[3] = 0;
+                }
+            }
+            __syncthreads();
+            if (ctl[1] < 0) break;
+            const int nf = ctl[2];
+            // every wave scores the list: pass `base` covers candidates
+            // base .. base + tile - 1, candidate r on lanes r*G .. r*G+G-1
+            for (int base = 0; base < nf; base += tile) {
+                const int r = gl / G, sub = gl % G;
+                const int idx = base + r;
+                const bool active = idx < nf;
+                const int id = active ? cids[idx] : 0;
+                float d = 0.0f;
+                bool ok;
+                switch (G) {
+                case 1: ok = rv_online_lane<RB>(ix, qv, id, &d); break;
+                case 2: ok = rv_online_group<2, RB>(ix, qv, id, active, sub, &d); break;
+                case 4: ok = rv_online_group<4, RB>(ix, qv, id, active, sub, &d); break;
+                case 8: ok = rv_online_group<8, RB>(ix, qv, id, active, sub, &d); break;
+                default: ok = rv_online_group<16, RB>(ix, qv, id, active, sub, &d); break;
+                }
+                if (active && sub == G - 1) {
+                    if (!ok || !isfinite(d) || d == FLT_MAX) ctl[3] = 1;
+                    else ckeys[idx] = rv_key(d + 0.0f, id);
+                }
+            }
+            __syncthreads();
+            if (wave == 0) {
+                if (ctl[3]) {
+                    status = RV_QUERY_NONFINITE;
+                } else {
+                    const uint64_t key = lane < nf ? ckeys[lane] : RV_KEY_MAX;
+                    int first_new, src_pos;
+                    merge(key, next, &first_new, &src_pos);
+                    if (STOP != 0) {
+                        ss.improve_ema = 0.7f * ss.improve_ema +
+                                         (first_new < b.stop_label_k ? 0.3f : 0.0f);
+                        if (first_new < b.stop_label_k) ss.last_change = step;
+                        if (first_new == 0) ss.last_top1 = step;
+                    }
+                    scan = first_new < src_pos + 1 ? first_new : src_pos + 1;
+                }
+            }
+        }
+
+        // ---- output (wave 0)
+        if (wave == 0) {
+            int32_t *oid = b.out_ids + (size_t)q * b.out_stride;
+            float *odist = b.out_dists + (size_t)q * b.out_stride;
+            const int refine = b.refine_candidates;
+            if (status != 0) {
+                if (lane == 0) b.out_counts[q] = status;
+            } else if (refine == 0) {
+                const int cnt = K < n ? K : n;
+                for (int i = lane; i < cnt; i += RV_WAVE) {
+                    oid[i] = rv_key_id(pool_keys[i]);
+                    odist[i] = rv_key_dist(pool_keys[i]);
+                }
+                if (lane == 0) b.out_counts[q] = cnt;
+            } else {
+                // final refine of the first F pool members, as rv_search_kernel
+                const int F = refine < n ? refine : n;
+                const bool e4 = b.refine_error4 != 0;
+                uint64_t *dbits = reinterpret_cast<uint64_t *>(smem + lay.off_refine);
+                bool bad = false;
+                for (int i = lane; i < F; i += RV_WAVE) {
+                    const int id = rv_key_id(pool_keys[i]);
+                    const uint8_t *pq = ix.codes + (size_t)id * ix.code_bytes;
+                    const uint8_t *res = ix.res_codes + (size_t)id * ix.res_code_bytes;
+                    const float sc = ix.res_qparams[2 * (size_t)id];
+                    const float zc = ix.res_qparams[2 * (size_t)id + 1];
+                    const uint8_t *c4 = nullptr;
+                    float s4 = 0.0f, z4 = 0.0f;
+                    if (e4) {
+                        c4 = ix.e4_blocks + (size_t)(id / ix.e4_vpb) * RV_E4_BLOCK +
+                             (size_t)(id % ix.e4_vpb) * ix.e4_code_bytes;
+                        s4 = ix.e4_qparams[2 * (size_t)id];
+                        z4 = ix.e4_qparams[2 * (size_t)id + 1];
+                    }
+                    double acc = 0.0;
+                    if (!isfinite(sc) || !isfinite(zc) || !(sc > 0.0f)) bad = true;
+                    int d = 0;
+                    for (int sub = 0; sub < ix.m; sub++) {
+                        const float *cent = ix.codebook +
+                            ((size_t)sub * ix.ks + rv_code_get(pq, sub, ix.ks)) * ix.subdim;
+                        for (int t = 0; t < ix.subdim; t++, d++) {
+                            float v = rv_decode_value(cent[t], rv_res_code(res, ix.res_bits, d),
+                                                      sc, zc);
+                            if (!isfinite(v)) bad = true;
+                            if (e4) {
+                                const float level =
+                                    (float)((c4[d >> 1] >> (4 * (d & 1))) & 15) * s4 + z4;
+                                v = v + level;
+                                if (!isfinite(v)) bad = true;
+                            }
+                            double delta = (double)qv[d] - (double)v;
+                            acc += delta * delta;
+                        }
+                    }
+                    if (!isfinite(acc) || acc >= (double)FLT_MAX) bad = true;
+                    dbits[i] = (uint64_t)__double_as_longlong(acc);
+                }
+                if (__ballot(bad) != 0ull) {
+                    if (lane == 0) b.out_counts[q] = RV_QUERY_NONFINITE;
+                } else {
+                    rv_wave_sync();
+                    for (int i = lane; i < F; i += RV_WAVE) {
+                        const uint64_t di = dbits[i];
+                        const int gi = rv_key_id(pool_keys[i]);
+                        const int oi = ix.graph_to_original ? ix.graph_to_original[gi] : gi;
+                        int rank = 0;
+                        for (int j = 0; j < F; j++) {
+                            const uint64_t dj = dbits[j];
+                            const int gj = rv_key_id(pool_keys[j]);
+                            const int oj = ix.graph_to_original ? ix.graph_to_original[gj] : gj;
+                            rank += (dj < di) || (dj == di && oj < oi);
+                        }
+                        if (rank < K) {
+                            oid[rank] = gi;
+                            odist[rank] = (float)__longlong_as_double((long long)di);
+                        }
+                    }
+                    if (lane == 0) b.out_counts[q] = K < F ? K : F;
+                }
+            }
+            if (b.out_stats && lane == 0) {
+                uint64_t *o = b.out_stats + 4 * (size_t)q;
+                o[0] = st_scored;
+                o[1] = st_scored;
+                o[2] = st_expanded;
+                o[3] = st_dedup;
+            }
+        }
+        __syncthreads();
+    }
+    if (gvis && gl == 0) vwords[-1] = (uint64_t)vstamp;
+}
+
 __global__ void rv_zero_kernel(uint32_t *p, size_t words) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < words;
          i += (size_t)gridDim.x * blockDim.x)
@@ -1611,7 +2087,7 @@ extern "C" int rv_seed_list(int32_t *d_out, int n, int count, int graph_entry,
 
 static RvLayout rv_layout_for(const RvIndex *ix, const RvBatch *b) {
     return rv_layout(ix->dim, b->scorer, ix->m, ix->ks, b->max_pool, b->hash_log2,
-                     b->stop_mode == 2, b->refine_candidates);
+                     b->stop_mode == 2, b->refine_candidates, b->waves_per_query > 0);
 }
 
 
@@ -1671,8 +2147,59 @@ static int rv_dispatch_stop(int op, const RvIndex *ix, const RvBatch *b, int gri
     return -1;
 }
 
+// The multi-wave kernel: one instantiation per (residual bits, stop mode,
+// waves per query).
+template <int RB, int T, int W>
+static int rv_mw_op_t(int op, const RvIndex *ix, const RvBatch *b, int grid, RvLayout lay,
+                      hipStream_t s) {
+    if (op == 0) {
+        static int configured = 0;
+        if (!configured) {
+            if (rv_check(hipFuncSetAttribute((const void *)rv_search_mw_kernel<RB, T, W>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             160 * 1024),
+                         "hipFuncSetAttribute"))
+                return -1;
+            configured = 1;
+        }
+        return 0;
+    }
+    if (op == 1) {
+        int blocks = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &blocks, rv_search_mw_kernel<RB, T, W>, RV_WAVE * W, lay.total) != hipSuccess)
+            return 0;
+        return blocks;
+    }
+    rv_search_mw_kernel<RB, T, W><<<grid, RV_WAVE * W, lay.total, s>>>(*ix, *b, lay);
+    return rv_check(hipGetLastError(), "rv_search_mw_kernel launch");
+}
+
+template <int RB, int T>
+static int rv_mw_dispatch_w(int op, const RvIndex *ix, const RvBatch *b, int grid,
+                            RvLayout lay, hipStream_t s) {
+    switch (b->waves_per_query) {
+    case 1: return rv_mw_op_t<RB, T, 1>(op, ix, b, grid, lay, s);
+    case 2: return rv_mw_op_t<RB, T, 2>(op, ix, b, grid, lay, s);
+    case 4: return rv_mw_op_t<RB, T, 4>(op, ix, b, grid, lay, s);
+    }
+    snprintf(g_rv_error, sizeof(g_rv_error), "rv_search: waves per query 1, 2 or 4");
+    return -1;
+}
+
+template <int RB>
+static int rv_mw_dispatch(int op, const RvIndex *ix, const RvBatch *b, int grid,
+                          RvLayout lay, hipStream_t s) {
+    return b->stop_mode == 1 ? rv_mw_dispatch_w<RB, 1>(op, ix, b, grid, lay, s)
+                             : rv_mw_dispatch_w<RB, 0>(op, ix, b, grid, lay, s);
+}
+
 static int rv_dispatch(int op, const RvIndex *ix, const RvBatch *b, int grid, RvLayout lay,
                        hipStream_t s) {
+    if (b->waves_per_query > 0)
+        return b->scorer == RV_SCORE_ONLINE_PAIR32_R4
+                   ? rv_mw_dispatch<4>(op, ix, b, grid, lay, s)
+                   : rv_mw_dispatch<8>(op, ix, b, grid, lay, s);
     switch (b->scorer) {
     case RV_SCORE_ADC: return rv_dispatch_stop<RV_SCORE_ADC>(op, ix, b, grid, lay, s);
     case RV_SCORE_ADC_ONLINE:
@@ -1706,6 +2233,17 @@ static int rv_resolve(const RvIndex *ix, const RvBatch *b_in, RvBatch *bb, RvLay
     if (bb->scorer == RV_SCORE_ADC_ONLINE &&
         rv_online_pairable(*ix, bb->expand_count < ix->degree ? bb->expand_count : ix->degree))
         bb->scorer = ix->res_bits == 4 ? RV_SCORE_ONLINE_PAIR32_R4 : RV_SCORE_ONLINE_PAIR32;
+    if (bb->waves_per_query > 0 &&
+        ((bb->waves_per_query != 1 && bb->waves_per_query != 2 && bb->waves_per_query != 4) ||
+         (bb->cand_tile != 16 && bb->cand_tile != 32 && bb->cand_tile != 64) ||
+         (bb->scorer != RV_SCORE_ONLINE_PAIR32 && bb->scorer != RV_SCORE_ONLINE_PAIR32_R4) ||
+         bb->stop_mode > 1 || bb->window > 1)) {
+        snprintf(g_rv_error, sizeof(g_rv_error),
+                 "rv_search_launch: several waves per query need the online correction on "
+                 "32 x 4-dim PQ (<= 32 neighbors), plain or learned stop, source window 1, "
+                 "waves 1/2/4 and candidate tile 16/32/64");
+        return -1;
+    }
     *lay = rv_layout_for(ix, bb);
     if (lay->total > 160 * 1024) {
         snprintf(g_rv_error, sizeof(g_rv_error),

@@ -77,7 +77,8 @@ class RoverSearchBatch(C.Structure):
                 ("probe_limit", C.c_int), ("grid_waves", C.c_int),
                 ("learned_stop", C.c_int), ("stop_threshold", C.c_float),
                 ("refine_mode", C.c_int), ("traversal", C.c_int),
-                ("source_window", C.c_int)]
+                ("source_window", C.c_int), ("waves_per_query", C.c_int),
+                ("candidate_tile", C.c_int)]
 
 
 REFINE_CURRENT, REFINE_ERROR4 = 1, 2
@@ -102,7 +103,8 @@ class RoverLearnedStopTrain(C.Structure):
 class RoverTuning(C.Structure):
     _fields_ = [("hash_log2", C.c_int), ("probe_limit", C.c_int),
                 ("grid_waves", C.c_int), ("ms", C.c_float), ("default_ms", C.c_float),
-                ("tried", C.c_int), ("rejected", C.c_int)]
+                ("tried", C.c_int), ("rejected", C.c_int),
+                ("waves_per_query", C.c_int), ("candidate_tile", C.c_int)]
 
 
 # (name, restype, argtypes); every symbol include/rover_gpu.h declares.

@@ -232,9 +232,11 @@ class Calculator:
                       grid_waves, learned_stop, stop_threshold, pool_sizes,
                       num_outs, seed, entry_ids=None, entry_count=0,
                       entry_stride=0, route_entries=0, refine_mode=0, traversal=0,
-                      source_window=0):
+                      source_window=0, waves_per_query=0, candidate_tile=0):
         b = native.RoverSearchBatch()
         b.source_window = int(source_window)
+        b.waves_per_query = int(waves_per_query)
+        b.candidate_tile = int(candidate_tile)
         b.refine_mode = int(refine_mode)
         b.traversal = int(traversal)
         b.entry_ids = entry_ids
@@ -270,7 +272,8 @@ class Calculator:
                      expand_count=32, stats=False, hash_log2=0, probe_limit=0,
                      grid_waves=0, learned_stop=0, stop_threshold=0.0,
                      seed=native.ENTRY_POINT_SEED, entry_ids=None,
-                     route_entries=0, refine_mode=0, traversal=0, source_window=0):
+                     route_entries=0, refine_mode=0, traversal=0, source_window=0,
+                     waves_per_query=0, candidate_tile=0):
         """Host arrays in and out.  entry_ids: explicit seeds, a list shared
         by all queries or a [nq, S] array of per-query rows (otherwise graph
         entry + LCG seeds); route_entries=S: per-query seeds from the router.  pool_size / num_out: int or per-query
@@ -294,7 +297,8 @@ class Calculator:
                                hash_log2, probe_limit, grid_waves, learned_stop,
                                stop_threshold, ptr(Ls), ptr(Ks), seed, ptr(e),
                                0 if e is None else (stride or len(e)), stride,
-                               route_entries, refine_mode, traversal, source_window)
+                               route_entries, refine_mode, traversal, source_window,
+                               waves_per_query, candidate_tile)
         check(native.lib().rover_search_batch(self.h, C.byref(b)), "rover_search_batch")
         return (ids, d, cnt, st) if stats else (ids, d, cnt)
 
@@ -307,7 +311,8 @@ class Calculator:
                             d_num_outs=None, stream=None,
                             seed=native.ENTRY_POINT_SEED, d_entry_ids=None,
                             entry_count=0, entry_stride=0, route_entries=0,
-                            refine_mode=0, traversal=0, noalloc=False, source_window=0):
+                            refine_mode=0, traversal=0, noalloc=False, source_window=0,
+                            waves_per_query=0, candidate_tile=0):
         """Device pointers in and out; asynchronous.  With per-query device
         arrays, pool_size must be their maximum.  noalloc=True calls
         rover_search_batch_device_noalloc (fixed per-device workspace)."""
@@ -318,7 +323,7 @@ class Calculator:
                                learned_stop, stop_threshold, d_pool_sizes,
                                d_num_outs, seed, d_entry_ids, entry_count,
                                entry_stride, route_entries, refine_mode, traversal,
-                               source_window)
+                               source_window, waves_per_query, candidate_tile)
         fn = native.lib().rover_search_batch_device_noalloc if noalloc else \
             native.lib().rover_search_batch_device
         check(fn(self.h, C.byref(b), stream),
