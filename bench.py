@@ -491,8 +491,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="time the default launch configuration instead of the tuned one")
     ap.add_argument("--plain-tuning", default="",
-                    help="H,P,G: visited hash (-1 exact), probe limit and grid waves for "
-                         "every plain launch, in place of the autotuner")
+                    help="H,P,G[,W,T]: visited hash (-1 exact), probe limit, grid "
+                         "(waves, or workgroups with W), waves per query and candidate "
+                         "tile for every plain launch, in place of the autotuner")
     ap.add_argument("--learned-tuning", default="",
                     help="H,P,G as --plain-tuning, for the learned-stop launches")
     ap.add_argument("--batches", type=int, default=48,
@@ -652,16 +653,19 @@ def main():
             return min(args.final_refine, L)
         return 0 if mode == "online" else min(args.refine, L)
 
-    tuned = {}  # (mode, L, learned, window) -> (hash_log2, probe_limit, grid_waves)
+    # (mode, L, learned, window) -> (hash_log2, probe_limit, grid_waves,
+    # waves_per_query, candidate_tile)
+    tuned = {}
     # source window of every launch below (1: the scalar search; 2: the live
     # pipeline's two turns in flight, the window-2 leg)
     win = {"w": 1}
     # fixed launch configurations (profiler runs): every L of the mode
-    fixed = {lrn: tuple(int(v) for v in spec.split(","))
+    fixed = {lrn: (tuple(int(v) for v in spec.split(",")) + (0, 0))[:5]
              for lrn, spec in ((False, args.plain_tuning), (True, args.learned_tuning)) if spec}
 
     def run(mode, L, stats=False, learned=False, refine=None, refine_mode=0, exact_hash=False):
-        h2, pl, gw = tuned.get((mode, L, learned, win["w"]), fixed.get(learned, (0, 0, 0)))
+        h2, pl, gw, wq, tl = tuned.get((mode, L, learned, win["w"]),
+                                       fixed.get(learned, (0, 0, 0, 0, 0)))
         if exact_hash:
             # exact visited sets: the counters are the distinct nodes scored
             # and expanded (what the oracle's stats count)
@@ -676,7 +680,8 @@ def main():
                                  learned_stop=1 if learned else 0,
                                  stop_threshold=0.9, d_entry_ids=es["ptr"],
                                  entry_count=es["S"], entry_stride=es["S"], hash_log2=h2,
-                                 probe_limit=pl, grid_waves=gw, source_window=win["w"])
+                                 probe_limit=pl, grid_waves=gw, source_window=win["w"],
+                                 waves_per_query=wq, candidate_tile=tl)
 
     def autotune(mode, L, learned):
         t = calc.autotune(dq.ptr, nq, L, k, d_ids.ptr, d_d.ptr, d_cnt.ptr, k, repeats=3,
@@ -685,9 +690,11 @@ def main():
                           d_entry_ids=es["ptr"], entry_count=es["S"], entry_stride=es["S"],
                           learned_stop=1 if learned else 0, refine_mode=final_rmode,
                           source_window=win["w"])
-        tuned[(mode, L, learned, win["w"])] = (t.hash_log2, t.probe_limit, t.grid_waves)
+        tuned[(mode, L, learned, win["w"])] = (t.hash_log2, t.probe_limit, t.grid_waves,
+                                              t.waves_per_query, t.candidate_tile)
         info = dict(hash_log2=t.hash_log2, probe_limit=t.probe_limit,
-                    grid_waves=t.grid_waves, ms=round(t.ms, 4),
+                    grid_waves=t.grid_waves, waves_per_query=t.waves_per_query,
+                    candidate_tile=t.candidate_tile, ms=round(t.ms, 4),
                     default_ms=round(t.default_ms, 4), tried=t.tried, rejected=t.rejected)
         log("autotuned %s%s L=%d: %s" % (mode, "+learned" if learned else "", L, info))
         return info
@@ -769,12 +776,13 @@ def main():
         per step and whether the results equal the device-resident run's."""
         structs = []
         for mode, L, learned in configs:
-            h2, pl, gw = tuned.get((mode, L, learned, 1), fixed.get(learned, (0, 0, 0)))
+            h2, pl, gw, wq, tl = tuned.get((mode, L, learned, 1),
+                                           fixed.get(learned, (0, 0, 0, 0, 0)))
             structs.append(calc._batch_struct(
                 pin_q.ptr, nq, L, k, native.METHOD_PQ, 1 if mode == "online" else 0,
                 refine_of(mode, L), 1, args.R, pin_ids.ptr, pin_d.ptr, pin_cnt.ptr, k, None,
                 h2, pl, gw, 1 if learned else 0, 0.9, None, None, native.ENTRY_POINT_SEED,
-                pin_e.ptr, S, S, 0, final_rmode))
+                pin_e.ptr, S, S, 0, final_rmode, 0, 0, wq, tl))
         times = [[] for _ in configs]
         gc.collect()
         gc.disable()  # no collector pauses inside the timed calls
@@ -1176,7 +1184,13 @@ def main():
     # instantiation (scorer 3); ADC ranking is scorer 0
     scorer = (3 if args.m * 4 == args.dim and args.m == 32 and args.R <= 32 else 1) \
         if mode == "online" else 0
-    kernel = "rv_search_kernel<%d, %d>" % (scorer, 1 if label == "learned-stop" else 0)
+    tl_ = tuned.get((mode, L, label == "learned-stop", 1), (0, 0, 0, 0, 0))
+    if tl_[3]:
+        kernel = "rv_search_mw_kernel<%d, %d, %d>" % (args.res_bits,
+                                                     1 if label == "learned-stop" else 0,
+                                                     tl_[3])
+    else:
+        kernel = "rv_search_kernel<%d, %d, 1>" % (scorer, 1 if label == "learned-stop" else 0)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                 "traffic": None, "kernel": kernel,
