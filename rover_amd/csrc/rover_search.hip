@@ -340,7 +340,8 @@ __device__ __forceinline__ bool rv_online_pair32(const RvIndex &ix, const float 
     float acc = 0.0f;
 #pragma unroll
     for (int d = 0; d < DH; d++) acc += t[d];
-    float dist = __shfl_xor(acc, 32);  // the high lane receives terms 0..63
+    // the high lane receives terms 0..63
+    float dist = __uint_as_float(rv_low_half(__float_as_uint(acc)));
 #pragma unroll
     for (int d = 0; d < DH; d++) dist += t[d];
     *out = dist;
@@ -419,10 +420,12 @@ __device__ __forceinline__ bool rv_online_group(const RvIndex &ix, const float *
             t[d] = diff * diff;
         }
     }
+    // a group lies inside one 16-lane row (G divides 16), so the running
+    // sum moves up by a row shift; the group's first lane ignores it
     float acc = 0.0f;
 #pragma unroll
     for (int j = 0; j < G; j++) {
-        const float below = __shfl_up(acc, 1);
+        const float below = rv_row_prev(acc);
         if (sub == j) {
             float a = j == 0 ? 0.0f : below;
 #pragma unroll
@@ -934,8 +937,8 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
                     else if (!visit((uint32_t)id)) dup = true;
                     else fresh = 1;
                 }
-                id = __shfl(id, slot);
-                fresh = __shfl(fresh, slot);
+                id = (int)rv_low_half((uint32_t)id);
+                fresh = (int)rv_low_half((uint32_t)fresh);
                 const bool active = fresh != 0;
                 float d = 0.0f;
                 const bool ok = rv_online_pair32<pair_rb>(ix, qv, id, active, lane >= 32, &d);
@@ -1108,11 +1111,10 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
                 const uint64_t fmask = __ballot(fresh != 0);
                 const int nf = __popcll(fmask);
                 if (nf > 8) {
-                    // neighbor `slot` is scored by lanes slot and slot + 32;
-                    // the high lane holds the key
-                    const int slot = lane & 31;
-                    const int nbs = __shfl(nb, slot);
-                    const bool active = __shfl(fresh, slot) != 0;
+                    // neighbor i is scored by lanes i and i + 32; the high
+                    // lane holds the key
+                    const int nbs = (int)rv_low_half((uint32_t)nb);
+                    const bool active = rv_low_half((uint32_t)fresh) != 0u;
                     float d = 0.0f;
                     const bool ok =
                         rv_online_pair32<pair_rb>(ix, qv, nbs, active, lane >= 32, &d);
@@ -1493,8 +1495,8 @@ void rv_search_mw_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
                     else if (!visit((uint32_t)id)) dup = true;
                     else fresh = 1;
                 }
-                id = __shfl(id, slot);
-                fresh = __shfl(fresh, slot);
+                id = (int)rv_low_half((uint32_t)id);
+                fresh = (int)rv_low_half((uint32_t)fresh);
                 const bool active = fresh != 0;
                 float d = 0.0f;
                 const bool ok = rv_online_pair32<RB>(ix, qv, id, active, lane >= 32, &d);

@@ -3,12 +3,33 @@
 // Lane exchanges inside a quad use DPP quad permutes (a VALU modifier, no
 // LDS round trip); exchanges across 4..16 lanes use ds_swizzle in bitmask
 // mode (lane i reads lane i ^ j inside its 32-lane group, no address
-// register); only the 32-lane exchange uses ds_bpermute.
+// register); the 32-lane exchange uses v_permlane32_swap.
 #ifndef RV_WAVE_H
 #define RV_WAVE_H
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+// v_permlane32_swap with v in both operands exchanges the upper 32 lanes of
+// the first with the lower 32 lanes of the second: the first result holds
+// lanes 0..31 in both halves, the second lanes 32..63 in both halves.
+// Value of lane (lane & 31): the lower half copied to the upper one.
+__device__ __forceinline__ uint32_t rv_low_half(uint32_t v) {
+    return __builtin_amdgcn_permlane32_swap(v, v, false, false)[0];
+}
+
+// Value of lane (lane ^ 32).
+__device__ __forceinline__ uint32_t rv_swap_halves(uint32_t v) {
+    const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return __builtin_amdgcn_inverse_ballot_w64(0xffffffffull) ? r[1] : r[0];
+}
+
+// Value of lane - 1 inside each 16-lane row; the first lane of a row
+// receives 0 (DPP row_shr:1, a VALU operand modifier).
+__device__ __forceinline__ float rv_row_prev(float v) {
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xF, 0xF, false));
+}
 
 // Value of lane (lane ^ J) for J in {1, 2, 4, 8, 16, 32}.
 template <int J>
@@ -16,7 +37,7 @@ __device__ __forceinline__ uint32_t rv_xor_lane(uint32_t v) {
     static_assert(J == 1 || J == 2 || J == 4 || J == 8 || J == 16 || J == 32, "xor");
     if (J == 1) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);
     if (J == 2) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false);
-    if (J == 32) return (uint32_t)__shfl_xor((int)v, 32);
+    if (J == 32) return rv_swap_halves(v);
     return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (J << 10));
 }
 
