@@ -491,9 +491,14 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="time the default launch configuration instead of the tuned one")
     ap.add_argument("--plain-tuning", default="",
-                    help="H,P,G[,W,T]: visited hash (-1 exact), probe limit, grid "
-                         "(waves, or workgroups with W), waves per query and candidate "
-                         "tile for every plain launch, in place of the autotuner")
+                    help="H,P,G[,W,T[,R]]: visited hash (-1 exact), probe limit, grid "
+                         "(waves, or workgroups with W), waves per query, candidate "
+                         "tile and reorder step for every plain launch, in place of the "
+                         "autotuner and the reorder timing")
+    ap.add_argument("--reorder-steps", default="0,16,24,32,48",
+                    help="suspension steps timed for each headline launch "
+                         "(RoverSearchBatch.reorder_step; 0 = one pass); the fastest is "
+                         "kept (empty: no reordering)")
     ap.add_argument("--learned-tuning", default="",
                     help="H,P,G as --plain-tuning, for the learned-stop launches")
     ap.add_argument("--batches", type=int, default=48,
@@ -654,18 +659,21 @@ def main():
         return 0 if mode == "online" else min(args.refine, L)
 
     # (mode, L, learned, window) -> (hash_log2, probe_limit, grid_waves,
-    # waves_per_query, candidate_tile)
+    # waves_per_query, candidate_tile, reorder_step)
     tuned = {}
+    # a learned-stop predictor is loaded in the session (the reorder timing
+    # needs one to order the resumed searches)
+    have_model = {"set": False}
     # source window of every launch below (1: the scalar search; 2: the live
     # pipeline's two turns in flight, the window-2 leg)
     win = {"w": 1}
     # fixed launch configurations (profiler runs): every L of the mode
-    fixed = {lrn: (tuple(int(v) for v in spec.split(",")) + (0, 0))[:5]
+    fixed = {lrn: (tuple(int(v) for v in spec.split(",")) + (0, 0, 0))[:6]
              for lrn, spec in ((False, args.plain_tuning), (True, args.learned_tuning)) if spec}
 
     def run(mode, L, stats=False, learned=False, refine=None, refine_mode=0, exact_hash=False):
-        h2, pl, gw, wq, tl = tuned.get((mode, L, learned, win["w"]),
-                                       fixed.get(learned, (0, 0, 0, 0, 0)))
+        h2, pl, gw, wq, tl, rs = tuned.get((mode, L, learned, win["w"]),
+                                           fixed.get(learned, (0, 0, 0, 0, 0, 0)))
         if exact_hash:
             # exact visited sets: the counters are the distinct nodes scored
             # and expanded (what the oracle's stats count)
@@ -681,7 +689,7 @@ def main():
                                  stop_threshold=0.9, d_entry_ids=es["ptr"],
                                  entry_count=es["S"], entry_stride=es["S"], hash_log2=h2,
                                  probe_limit=pl, grid_waves=gw, source_window=win["w"],
-                                 waves_per_query=wq, candidate_tile=tl)
+                                 waves_per_query=wq, candidate_tile=tl, reorder_step=rs)
 
     def autotune(mode, L, learned):
         t = calc.autotune(dq.ptr, nq, L, k, d_ids.ptr, d_d.ptr, d_cnt.ptr, k, repeats=3,
@@ -691,13 +699,44 @@ def main():
                           learned_stop=1 if learned else 0, refine_mode=final_rmode,
                           source_window=win["w"])
         tuned[(mode, L, learned, win["w"])] = (t.hash_log2, t.probe_limit, t.grid_waves,
-                                              t.waves_per_query, t.candidate_tile)
+                                              t.waves_per_query, t.candidate_tile, 0)
         info = dict(hash_log2=t.hash_log2, probe_limit=t.probe_limit,
                     grid_waves=t.grid_waves, waves_per_query=t.waves_per_query,
                     candidate_tile=t.candidate_tile, ms=round(t.ms, 4),
                     default_ms=round(t.default_ms, 4), tried=t.tried, rejected=t.rejected)
         log("autotuned %s%s L=%d: %s" % (mode, "+learned" if learned else "", L, info))
         return info
+
+    def choose_reorder(mode, L, learned):
+        """Times the tuned launch suspended after each of --reorder-steps
+        expansions (0: one pass) and keeps the fastest.  The searches resume
+        ordered by the learned-stop predictor's probability (for the plain
+        search the predictor only orders them), so the results are the same
+        for every step; needs exact visited sets and a predictor."""
+        key = (mode, L, learned, win["w"])
+        cfg = tuned.get(key)
+        steps = [int(v) for v in args.reorder_steps.split(",") if v.strip()]
+        if not cfg or cfg[0] != native.HASH_EXACT or cfg[3] or not have_model["set"] or \
+                not any(steps):
+            return None
+        table = {}
+        for rs in steps:
+            tuned[key] = cfg[:5] + (rs,)
+            for _ in range(3):
+                run(mode, L, learned=learned)
+            native.synchronize()
+            e0, e1 = native.Event(), native.Event()
+            e0.record()
+            for _ in range(5):
+                run(mode, L, learned=learned)
+            e1.record()
+            native.synchronize()
+            table[rs] = round(e0.elapsed_ms(e1) / 5, 4)
+        best = min(table, key=table.get)
+        tuned[key] = cfg[:5] + (best,)
+        log("reorder %s%s L=%d: ms by suspension step %s; kept %d" %
+            (mode, "+learned" if learned else "", L, table, best))
+        return dict(reorder_step=best, ms_by_step={str(k): v for k, v in table.items()})
 
     def probe(mode, L, learned, rows, bisect=False):
         run(mode, L, stats=True, learned=learned)
@@ -776,13 +815,13 @@ def main():
         per step and whether the results equal the device-resident run's."""
         structs = []
         for mode, L, learned in configs:
-            h2, pl, gw, wq, tl = tuned.get((mode, L, learned, 1),
-                                           fixed.get(learned, (0, 0, 0, 0, 0)))
+            h2, pl, gw, wq, tl, rs = tuned.get((mode, L, learned, 1),
+                                               fixed.get(learned, (0, 0, 0, 0, 0, 0)))
             structs.append(calc._batch_struct(
                 pin_q.ptr, nq, L, k, native.METHOD_PQ, 1 if mode == "online" else 0,
                 refine_of(mode, L), 1, args.R, pin_ids.ptr, pin_d.ptr, pin_cnt.ptr, k, None,
                 h2, pl, gw, 1 if learned else 0, 0.9, None, None, native.ENTRY_POINT_SEED,
-                pin_e.ptr, S, S, 0, final_rmode, 0, 0, wq, tl))
+                pin_e.ptr, S, S, 0, final_rmode, 0, 0, wq, tl, reorder_step=rs))
         times = [[] for _ in configs]
         gc.collect()
         gc.disable()  # no collector pauses inside the timed calls
@@ -844,6 +883,7 @@ def main():
             native.check(native.lib().rover_learned_stop_set(
                 calc.h, m[0].ctypes.data, P, m[1].ctypes.data, m[2].ctypes.data,
                 args.stop_every, args.label_k, args.stop_every), "rover_learned_stop_set")
+            have_model["set"] = True
 
         def train_model(slack, tq, tents, tS):
             t = native.RoverLearnedStopTrain()
@@ -927,11 +967,15 @@ def main():
                                                             args.steps))
     tuning = {"plain": autotune(mode, L_plain, False)
               if not (args.no_autotune or False in fixed) else None}
+    if tuning["plain"]:
+        tuning["plain"]["reorder"] = choose_reorder(mode, L_plain, False)
     ms_plain_dev = timed(mode, L_plain, False)
     if learned:
         L_l = learned["chosen"]["L"]
         tuning["learned"] = autotune(mode, L_l, True) \
             if not (args.no_autotune or True in fixed) else None
+        if tuning["learned"]:
+            tuning["learned"]["reorder"] = choose_reorder(mode, L_l, True)
         ms_l_dev = timed(mode, L_l, True)
         (ms_plain, same_plain, med_plain), (ms_l, same_l, med_l) = timed_host(
             [(mode, L_plain, False), (mode, L_l, True)])
@@ -1184,11 +1228,16 @@ def main():
     # instantiation (scorer 3); ADC ranking is scorer 0
     scorer = (3 if args.m * 4 == args.dim and args.m == 32 and args.R <= 32 else 1) \
         if mode == "online" else 0
-    tl_ = tuned.get((mode, L, label == "learned-stop", 1), (0, 0, 0, 0, 0))
+    tl_ = tuned.get((mode, L, label == "learned-stop", 1),
+                    fixed.get(label == "learned-stop", (0, 0, 0, 0, 0, 0)))
     if tl_[3]:
         kernel = "rv_search_mw_kernel<%d, %d, %d>" % (args.res_bits,
                                                      1 if label == "learned-stop" else 0,
                                                      tl_[3])
+    elif tl_[5]:
+        # suspended and resumed: two dispatches per step (the searches up to
+        # the suspension step, then the resumed ones) and the ordering kernel
+        kernel = "rv_search_kernel<%d, 3, 1>" % scorer
     else:
         kernel = "rv_search_kernel<%d, %d, 1>" % (scorer, 1 if label == "learned-stop" else 0)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,

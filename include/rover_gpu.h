@@ -405,6 +405,18 @@ typedef struct RoverSearchBatch {
        grid_waves then counts workgroups.                                    */
     int waves_per_query;
     int candidate_tile;
+    /* resumption order (no reference counterpart): 0 = off; N > 0 = every
+       search still running after N expansions is suspended (its pool, flags,
+       counters and learned-stop state saved), and the suspended searches
+       resume in order of the learned-stop predictor's stop probability at
+       that point, lowest (the longest expected) first, so that long searches
+       do not start last.  Results and counters are those of the unsuspended
+       search.  Needs exact visited sets (hash_log2 = ROVER_HASH_EXACT; one
+       set per query of the batch, ceil(n / 32) + 1 words of 8 bytes each),
+       source window 1, one wave per query and a learned-stop model
+       (rover_learned_stop_set or _train); with learned_stop = 0 the
+       predictor only orders the resumption and stops no search.            */
+    int reorder_step;
 } RoverSearchBatch;
 
 enum { ROVER_REFINE_CURRENT = 1, ROVER_REFINE_ERROR4 = 2 };

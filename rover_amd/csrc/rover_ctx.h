@@ -64,6 +64,8 @@ typedef struct RoverCtx {
     uint64_t *d_visited;         /* exact visited sets of the search kernel  */
     size_t visited_cap;
     size_t visited_layout;       /* the (mode, words) the sets were set up for */
+    void *d_susp;                /* suspended searches' state (one block)    */
+    size_t susp_cap;
     size_t sqmd_vis_cap;
     void *d_sqmd_front;
     size_t sqmd_front_cap;

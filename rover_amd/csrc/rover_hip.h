@@ -79,6 +79,37 @@ typedef struct RvMlp {
     const float *feat_inv_std; /* [16]       */
 } RvMlp;
 
+/* Suspended searches (RvBatch.phase): the state a query saves when it is
+ * suspended and resumes from, per query of the batch, in one block of
+ * rv_susp_layout(nq, max_pool).bytes at RvBatch.susp:
+ *   keys  [nq][max_pool] u64  pool keys
+ *   flags [nq][max_pool] u8   expanded flags
+ *   meta  [nq][RV_SUSP_META] i32: pool size, scan position, step, scored /
+ *         expanded / duplicate counts, learned-stop state
+ *   prob  [nq] f32  predictor's stop probability when suspended
+ *   list  [nq] i32  suspended ids in suspension order
+ *   order [nq] i32  resumption order
+ *   count i32       queries suspended */
+#define RV_SUSP_META 16
+typedef struct RvSuspLayout {
+    size_t flags, meta, prob, list, order, count, bytes;
+} RvSuspLayout;
+static inline
+#ifdef __HIPCC__
+__host__ __device__
+#endif
+RvSuspLayout rv_susp_layout(size_t nq, size_t max_pool) {
+    RvSuspLayout l;
+    l.flags = nq * max_pool * 8;
+    l.meta = (l.flags + nq * max_pool + 15) / 16 * 16;
+    l.prob = l.meta + nq * RV_SUSP_META * 4;
+    l.list = l.prob + nq * 4;
+    l.order = l.list + nq * 4;
+    l.count = l.order + nq * 4;
+    l.bytes = l.count + 16;
+    return l;
+}
+
 typedef struct RvBatch {
     const float *queries;        /* [nq][dim]                               */
     int nq;
@@ -134,6 +165,18 @@ typedef struct RvBatch {
                                     1, 2, 4: the multi-wave kernel           */
     int cand_tile;               /* multi-wave kernel: unvisited neighbors
                                     scored per pass (16, 32, 64)             */
+    int phase;                   /* 0: whole searches.  1: a query still
+                                    running after suspend_step expansions
+                                    saves its state in susp and leaves; 2:
+                                    the saved queries resume in susp.order
+                                    (queue length *susp.count).  Phases 1
+                                    and 2 keep one exact visited set per
+                                    query: visited holds nq sets            */
+    int suspend_step;
+    int stop_probe_only;         /* learned stop: the predictor runs only at
+                                    the suspension (its probability orders
+                                    the resumption) and stops no search     */
+    unsigned char *susp;         /* suspended searches' state block         */
 } RvBatch;
 
 /* The reference's SQ traversal with distance-only heaps: the live SQ path
@@ -260,6 +303,10 @@ int rv_decode_vectors(const RvIndex *index, int sq, const int32_t *d_ids, int co
 int rv_noalloc_scratch(int device, void **d_scratch);
 /* The graph entry + LCG seed list (rpc_demo_pure_pq.c:192-208) of `count`
  * (<= n) ids, written to d_out by a kernel on `stream`. */
+/* Resumption order of the suspended queries: ascending stop probability
+ * (the longest expected first), 4096 probability buckets. */
+int rv_susp_order(unsigned char *susp, int nq, int max_pool, void *stream);
+
 int rv_seed_list(int32_t *d_out, int n, int count, int graph_entry, uint32_t seed,
                  void *stream);
 

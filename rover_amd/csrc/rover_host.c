@@ -178,6 +178,7 @@ int calculator_close(remote_handle64 h) {
     dfree(c->d_route_ent);
     dfree(c->d_sqmd_vis);
     dfree(c->d_visited);
+    dfree(c->d_susp);
     dfree(c->d_sqmd_front);
     if (c->stream) {
         rv_stream_synchronize(c->stream);
@@ -898,11 +899,40 @@ int rover_attach_visited(uint64_t **buf, size_t *cap, size_t *layout, const RvIn
     return 0;
 }
 
+/* Suspend and resume (RoverSearchBatch.reorder_step): one exact visited set
+ * per query, the state block carved into the RvSusp arrays, then phase 1
+ * (searches still running at the step save their state), the resumption
+ * order, and phase 2 (the saved searches, longest expected first). */
+static int rover_launch_reordered(RoverCtx *c, const RvIndex *ix, RvBatch *rb, int grid,
+                                  void *stream) {
+    const RvSuspLayout sl = rv_susp_layout((size_t)rb->nq, (size_t)rb->max_pool);
+    int32_t *count;
+    if (rover_attach_visited(&c->d_visited, &c->visited_cap, &c->visited_layout, ix, rb,
+                             rb->nq, stream))
+        return -1;
+    if (grow(&c->d_susp, &c->susp_cap, sl.bytes)) return -1;
+    rb->susp = (unsigned char *)c->d_susp;
+    count = (int32_t *)(rb->susp + sl.count);
+    if (rv_fill_zero(count, sizeof(int32_t), stream)) return -1;
+    rb->phase = 1;
+    if (rv_search_launch(ix, rb, grid, stream)) return -1;
+    if (rv_susp_order(rb->susp, rb->nq, rb->max_pool, stream)) return -1;
+    {
+        RvBatch r2 = *rb;
+        r2.phase = 2;
+        r2.nq_dev = count;
+        if (rv_search_launch(ix, &r2, grid, stream)) return -1;
+    }
+    rb->phase = 0;
+    return 0;
+}
+
 int rover_launch(RoverCtx *c, const RvIndex *ix, RvBatch *rb, int grid_waves, void *stream) {
     if (rb->visited_exact && rb->nq > 0) {
         int grid = grid_waves > 0 ? grid_waves : rv_search_prepare(ix, rb);
         if (grid <= 0) return -1;
         if (grid > rb->nq) grid = rb->nq;
+        if (rb->suspend_step > 0) return rover_launch_reordered(c, ix, rb, grid, stream);
         if (rover_attach_visited(&c->d_visited, &c->visited_cap, &c->visited_layout, ix, rb,
                                  grid, stream))
             return -1;
@@ -936,7 +966,11 @@ AEEResult rover_prepare_launch(RoverCtx *c, const RoverSearchBatch *b,
         (b->waves_per_query != 0 && b->waves_per_query != 1 && b->waves_per_query != 2 &&
          b->waves_per_query != 4) ||
         (b->candidate_tile != 0 && b->candidate_tile != 16 && b->candidate_tile != 32 &&
-         b->candidate_tile != 64))
+         b->candidate_tile != 64) ||
+        b->reorder_step < 0 ||
+        (b->reorder_step > 0 &&
+         (b->hash_log2 != ROVER_HASH_EXACT || b->source_window > 1 || b->waves_per_query ||
+          b->method != ROVER_METHOD_PQ)))
         return ROVER_EBADPARM;
     if (b->refine_mode == ROVER_REFINE_ERROR4) {
         if (!c->e4_loaded) {
@@ -993,12 +1027,15 @@ AEEResult rover_prepare_launch(RoverCtx *c, const RoverSearchBatch *b,
     rb->queue = c->d_queue;
     rb->stop_label_k = c->learned.label_k > 0 ? c->learned.label_k : 10;
     rb->stop_every = c->learned.stop_every > 0 ? c->learned.stop_every : 1;
-    if (b->learned_stop) {
+    if (b->learned_stop || b->reorder_step > 0) {
         if (!c->learned.ready) {
             snprintf(g_err, sizeof(g_err), "search: learned stop model not set");
             return ROVER_EFAILED;
         }
         rb->stop_mode = 1;
+        /* without the learned stop the predictor only orders the resumption */
+        rb->stop_probe_only = !b->learned_stop;
+        rb->suspend_step = b->reorder_step;
         rb->stop_threshold = b->stop_threshold > 0.0f ? b->stop_threshold : 0.9f;
         rb->stop_min_steps = c->learned.min_steps;
         rb->mlp.w1 = c->learned.d_w;
@@ -1607,6 +1644,7 @@ AEEResult rover_autotune(remote_handle64 h, const RoverSearchBatch *b, void *str
                 float ms;
                 const int exact = hashes[hi] == ROVER_HASH_EXACT;
                 if (exact && pi > 0) continue; /* no probes in an exact set */
+                if (!exact && rb.suspend_step > 0) continue; /* reordering needs exact sets */
                 cand = rb;
                 cand.visited_exact = exact;
                 if (exact) cand.hash_log2 = 4;

@@ -582,6 +582,7 @@ struct RvStopState {
     float d_entry, d_entry2, top1_0, topk_0;
     float improve_ema;  // decaying rate of steps that changed the top-K
     int last_change, last_top1;
+    float suspend_p;    // predictor probability when the search was suspended
 };
 
 // Feature i (0..15) of the search state; lane i computes feature i, so the
@@ -693,7 +694,9 @@ __device__ float rv_mlp_logit(const RvMlp &mlp_in, float f_lane, float *hbuf) {
 // ------------------------------------------------------------- the kernel
 
 // STOP is the learned-stop mode (b.stop_mode): 0 plain, 1 predict and stop,
-// 2 record traces.  Separate instantiations keep the plain search free of
+// 2 record traces, 3 as 1 with suspension and resumption (b.phase, for
+// RoverSearchBatch.reorder_step; also the plain search, the predictor then
+// only ordering the resumption).  Separate instantiations keep the plain search free of
 // the predictor's code and registers.  WIN is the source window (b.window):
 // 1 expands the best unexpanded pool member and merges its candidates before
 // choosing the next (the scalar search); 2 is the live pipeline with two
@@ -707,7 +710,7 @@ template <int SCORER, int STOP, int WIN>
 // the same budget, the learned-stop instantiation spills into its feature
 // and predictor code and runs slower than at 3 waves per SIMD.
 __global__ __launch_bounds__(RV_WAVE)
-__attribute__((amdgpu_waves_per_eu((SCORER == RV_SCORE_ONLINE_PAIR32 && STOP == 0) ? 4 : ((SCORER == RV_SCORE_ONLINE_PAIR32 || SCORER == RV_SCORE_ONLINE_PAIR32_R4) && STOP == 1) ? 3 : 1)))
+__attribute__((amdgpu_waves_per_eu((SCORER == RV_SCORE_ONLINE_PAIR32 && STOP == 0) ? 4 : ((SCORER == RV_SCORE_ONLINE_PAIR32 || SCORER == RV_SCORE_ONLINE_PAIR32_R4) && (STOP == 1 || STOP == 3)) ? 3 : 1)))
 void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
     extern __shared__ __align__(16) unsigned char smem[];
     float *qv = reinterpret_cast<float *>(smem + lay.off_q);
@@ -718,6 +721,9 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
     uint64_t *pool_keys = reinterpret_cast<uint64_t *>(smem + lay.off_keys);
     uint8_t *pool_flags = smem + lay.off_flags;
     constexpr bool tracing = STOP == 2;
+    // STOP 3: the learned-stop search that can be suspended and resumed
+    // (RvBatch.phase); the other instantiations carry none of that code
+    constexpr bool susp = STOP == 3;
     uint16_t *pool_born = tracing ? reinterpret_cast<uint16_t *>(smem + lay.off_born) : nullptr;
     uint64_t *cand = reinterpret_cast<uint64_t *>(smem + lay.off_cand);
     uint32_t *hash = reinterpret_cast<uint32_t *>(smem + lay.off_hash);
@@ -738,9 +744,14 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
     // exact visited sets: this wave's region and stamp counter (kept in a
     // register while the wave runs, stored back when it leaves)
     const bool gvis = b.visited != nullptr;
-    uint64_t *vwords = gvis ? b.visited + (size_t)blockIdx.x * (b.visited_words + 1) : nullptr;
-    uint32_t vstamp = gvis ? rv_uniform((int)(uint32_t)vwords[0]) : 0u;
-    if (gvis) vwords += 1;
+    // suspended and resumed searches (always launched in phase 1 or 2)
+    // keep one set per query (the region of query q, its stamp in the first
+    // word); the others one per wave
+    constexpr bool vq = susp;
+    uint64_t *vwords = gvis && !vq ? b.visited + (size_t)blockIdx.x * (b.visited_words + 1)
+                                   : nullptr;
+    uint32_t vstamp = gvis && !vq ? rv_uniform((int)(uint32_t)vwords[0]) : 0u;
+    if (gvis && !vq) vwords += 1;
     auto visit = [&](uint32_t id) -> bool {
         return gvis ? rv_visit_exact(vwords, vstamp, id)
                     : rv_visit(hash, b.hash_log2, b.probe_limit, id);
@@ -750,6 +761,11 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
         if (lane == 0) q = atomicAdd(b.queue, 1);
         q = rv_uniform(__shfl(q, 0));
         if (q >= nq) break;
+        const bool resume = susp && b.phase == 2;
+        if (resume) {
+            const RvSuspLayout sl = rv_susp_layout((size_t)b.nq, (size_t)b.max_pool);
+            q = rv_uniform(reinterpret_cast<const int32_t *>(b.susp + sl.order)[q]);
+        }
 
         const int L = b.pool_sizes ? b.pool_sizes[q] : b.pool_size;
         const int K = b.num_outs ? b.num_outs[q] : b.num_out;
@@ -766,16 +782,25 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
         // ---- per-query setup: query vector, table, visited set
         const float *qg = b.queries + (size_t)q * ix.dim;
         for (int d = lane; d < ix.dim; d += RV_WAVE) qv[d] = qg[d];
-        if (gvis) {
+        if (gvis && vq) {
+            // the query's own set: a new stamp when it starts, the saved
+            // one when it resumes
+            vwords = b.visited + (size_t)q * (b.visited_words + 1);
+            vstamp = rv_uniform((int)(uint32_t)vwords[0]);
+            vwords += 1;
+        }
+        if (gvis && !resume) {
             // a fresh stamp empties the set; after 2^32 - 1 queries on this
-            // wave the stamps would repeat, so the words are cleared first
+            // wave (or this query's region) the stamps would repeat, so the
+            // words are cleared first
             vstamp++;
             if (vstamp == 0u) {
                 for (int i = lane; i < b.visited_words; i += RV_WAVE) vwords[i] = 0ull;
                 __threadfence();
                 vstamp = 1u;
             }
-        } else {
+            if (vq && lane == 0) vwords[-1] = (uint64_t)vstamp;
+        } else if (!gvis) {
             for (int i = lane; i < hash_n; i += RV_WAVE) hash[i] = RV_EMPTY_SLOT;
         }
         __syncthreads();
@@ -915,6 +940,37 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
             __syncthreads();
         };
 
+        // ---- a suspended search resumes: pool, flags, counters, scan
+        // position, step and learned-stop state as it left them
+        int r_scan = 0, r_step = 0;
+        if (resume) {
+            const RvSuspLayout sl = rv_susp_layout((size_t)b.nq, (size_t)b.max_pool);
+            const int32_t *mt =
+                reinterpret_cast<const int32_t *>(b.susp + sl.meta) + (size_t)q * RV_SUSP_META;
+            n = rv_uniform(mt[0]);
+            r_scan = rv_uniform(mt[1]);
+            r_step = rv_uniform(mt[2]);
+            st_scored = (uint32_t)rv_uniform(mt[3]);
+            st_expanded = (uint32_t)rv_uniform(mt[4]);
+            st_dedup = (uint32_t)rv_uniform(mt[5]);
+            const uint64_t *sk = reinterpret_cast<const uint64_t *>(b.susp) + (size_t)q * b.max_pool;
+            const uint8_t *sf = b.susp + sl.flags + (size_t)q * b.max_pool;
+            for (int i = lane; i < n; i += RV_WAVE) {
+                pool_keys[i] = sk[i];
+                pool_flags[i] = sf[i];
+            }
+            if (STOP != 0) {
+                ss.d_entry = __int_as_float(mt[6]);
+                ss.d_entry2 = __int_as_float(mt[7]);
+                ss.top1_0 = __int_as_float(mt[8]);
+                ss.topk_0 = __int_as_float(mt[9]);
+                ss.improve_ema = __int_as_float(mt[10]);
+                ss.last_change = mt[11];
+                ss.last_top1 = mt[12];
+            }
+            __syncthreads();
+        }
+
         // ---- seeds (rpc_demo_pure_pq.c:192-208 seeds; :843-873 scoring);
         // one shared list, or a row of entry_stride ids per query
         const int32_t *ents =
@@ -925,7 +981,7 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
         const int slot = pairable ? (lane & 31) : lane;
         const bool owner = !pairable || lane < 32;
         const int key_lane0 = pairable ? 32 : 0;
-        for (int e0 = 0; e0 < b.n_entries && status == 0; e0 += per_round) {
+        for (int e0 = 0; !resume && e0 < b.n_entries && status == 0; e0 += per_round) {
             const int e = e0 + slot;
             uint64_t key = RV_KEY_MAX;
             bool bad = false, nonfinite = false, dup = false;
@@ -982,7 +1038,7 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
             int fn, adm, tr;
             merge(key, pairable ? 1 : 2, -1, &fn, &adm, &tr);
         }
-        if (status == 0 && n == 0) status = RV_QUERY_BAD_ENTRY;
+        if (!resume && status == 0 && n == 0) status = RV_QUERY_BAD_ENTRY;
         // A long seed list would fill the visited hash with far-away seeds.
         // Keep only the pool members then: a seed outside the pool that the
         // search meets again is re-scored and rejected (its key is worse
@@ -996,7 +1052,7 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
                 rv_visit(hash, b.hash_log2, b.probe_limit, (uint32_t)rv_key_id(k0[i]));
             __syncthreads();
         }
-        if (STOP != 0 && status == 0) {
+        if (STOP != 0 && status == 0 && !resume) {
             // the first two entries' distances were taken in the lanes that
             // held their keys; every lane computes features from them
             ss.d_entry = rv_lane_value(ss.d_entry, key_lane0);
@@ -1007,9 +1063,11 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
         }
 
         // ---- main loop: expand the best unexpanded pool member
-        int scan = 0;
-        int step = 0;
+        int scan = r_scan;
+        int step = r_step;
         bool stopped = false;
+        // status while the search is suspended (internal; never output)
+        constexpr int suspended_status = 1;
         // window 2: the last dispatched turn's candidate keys wait here and
         // are merged once the next source has been chosen
         uint64_t pend_key = RV_KEY_MAX;
@@ -1050,25 +1108,37 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
             }
 
             // learned stop: features of the current state
-            if (STOP != 0 && step > 0 && step % b.stop_every == 0) {
+            const bool suspend_now = susp && b.phase == 1 && step >= b.suspend_step;
+            float p_now = -1.0f;
+            if (STOP == 2 && step > 0 && step % b.stop_every == 0) {
                 const float f = rv_feature(ss, keys, n, b.stop_label_k, step, next, lane);
-                if (STOP == 2) {
-                    if (trace_n < b.trace_cap) {
-                        if (lane < RV_MLP_IN)
-                            b.trace_feats[((size_t)q * b.trace_cap + trace_n) * RV_MLP_IN + lane] = f;
-                        if (lane == 0) b.trace_steps[(size_t)q * b.trace_cap + trace_n] = step;
-                        trace_n++;
-                    }
-                } else if (step >= b.stop_min_steps) {
+                if (trace_n < b.trace_cap) {
+                    if (lane < RV_MLP_IN)
+                        b.trace_feats[((size_t)q * b.trace_cap + trace_n) * RV_MLP_IN + lane] = f;
+                    if (lane == 0) b.trace_steps[(size_t)q * b.trace_cap + trace_n] = step;
+                    trace_n++;
+                }
+            } else if (STOP == 1 || STOP == 3) {
+                const bool regular = !(susp && b.stop_probe_only) && step > 0 &&
+                                     step % b.stop_every == 0 && step >= b.stop_min_steps;
+                if (regular || suspend_now) {
                     // the reduction leaves the same logit in every lane
+                    const float f = rv_feature(ss, keys, n, b.stop_label_k, step, next, lane);
                     const float logit =
                         rv_mlp_logit(b.mlp, f, reinterpret_cast<float *>(cand));
-                    const float p = 1.0f / (1.0f + __expf(-logit));
-                    if (rv_uniform(p >= b.stop_threshold ? 1 : 0)) {
+                    p_now = 1.0f / (1.0f + __expf(-logit));
+                    if (regular && rv_uniform(p_now >= b.stop_threshold ? 1 : 0)) {
                         stopped = true;
                         break;
                     }
                 }
+            }
+            if (suspend_now) {
+                // leave the search as it stands (saved below); it resumes
+                // in phase 2
+                ss.suspend_p = p_now;
+                status = suspended_status;
+                break;
             }
 
             const uint64_t src_key = keys[next];
@@ -1227,6 +1297,37 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
             }
         }
 
+        if (susp && status == suspended_status) {
+            const RvSuspLayout sl = rv_susp_layout((size_t)b.nq, (size_t)b.max_pool);
+            uint64_t *sk = reinterpret_cast<uint64_t *>(b.susp) + (size_t)q * b.max_pool;
+            uint8_t *sf = b.susp + sl.flags + (size_t)q * b.max_pool;
+            for (int i = lane; i < n; i += RV_WAVE) {
+                sk[i] = pool_keys[i];
+                sf[i] = pool_flags[i];
+            }
+            int32_t *mt = reinterpret_cast<int32_t *>(b.susp + sl.meta) + (size_t)q * RV_SUSP_META;
+            if (lane == 0) {
+                mt[0] = n;
+                mt[1] = scan;
+                mt[2] = step;
+                mt[3] = (int32_t)st_scored;
+                mt[4] = (int32_t)st_expanded;
+                mt[5] = (int32_t)st_dedup;
+                mt[6] = __float_as_int(ss.d_entry);
+                mt[7] = __float_as_int(ss.d_entry2);
+                mt[8] = __float_as_int(ss.top1_0);
+                mt[9] = __float_as_int(ss.topk_0);
+                mt[10] = __float_as_int(ss.improve_ema);
+                mt[11] = ss.last_change;
+                mt[12] = ss.last_top1;
+                reinterpret_cast<float *>(b.susp + sl.prob)[q] = ss.suspend_p;
+                const int at = atomicAdd(reinterpret_cast<int32_t *>(b.susp + sl.count), 1);
+                reinterpret_cast<int32_t *>(b.susp + sl.list)[at] = q;
+            }
+            __syncthreads();
+            continue;
+        }
+
         // ---- output
         const uint64_t *keys = pool_keys;
         if (status != 0) {
@@ -1315,7 +1416,7 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
         }
         __syncthreads();
     }
-    if (gvis && lane == 0) vwords[-1] = (uint64_t)vstamp;
+    if (gvis && !vq && lane == 0) vwords[-1] = (uint64_t)vstamp;
 }
 
 // ------------------------------------------------ several waves per query
@@ -2141,6 +2242,8 @@ static int rv_dispatch_stop(int op, const RvIndex *ix, const RvBatch *b, int gri
     if (b->stop_mode == 0)
         return w == 2 ? rv_op_t<S, 0, 2>(op, ix, b, grid, lay, s)
                       : rv_op_t<S, 0, 1>(op, ix, b, grid, lay, s);
+    if (b->stop_mode == 1 && b->suspend_step > 0 && w == 1)
+        return rv_op_t<S, 3, 1>(op, ix, b, grid, lay, s);
     if (b->stop_mode == 1)
         return w == 2 ? rv_op_t<S, 1, 2>(op, ix, b, grid, lay, s)
                       : rv_op_t<S, 1, 1>(op, ix, b, grid, lay, s);
@@ -2239,11 +2342,18 @@ static int rv_resolve(const RvIndex *ix, const RvBatch *b_in, RvBatch *bb, RvLay
         ((bb->waves_per_query != 1 && bb->waves_per_query != 2 && bb->waves_per_query != 4) ||
          (bb->cand_tile != 16 && bb->cand_tile != 32 && bb->cand_tile != 64) ||
          (bb->scorer != RV_SCORE_ONLINE_PAIR32 && bb->scorer != RV_SCORE_ONLINE_PAIR32_R4) ||
-         bb->stop_mode > 1 || bb->window > 1)) {
+         bb->stop_mode > 1 || bb->window > 1 || bb->phase != 0)) {
         snprintf(g_rv_error, sizeof(g_rv_error),
                  "rv_search_launch: several waves per query need the online correction on "
                  "32 x 4-dim PQ (<= 32 neighbors), plain or learned stop, source window 1, "
                  "waves 1/2/4 and candidate tile 16/32/64");
+        return -1;
+    }
+    if (bb->phase < 0 || bb->phase > 2 ||
+        (bb->phase != 0 && (bb->window > 1 || bb->stop_mode == 2 || bb->suspend_step <= 0)) ||
+        (bb->stop_probe_only && bb->stop_mode != 1)) {
+        snprintf(g_rv_error, sizeof(g_rv_error),
+                 "rv_search_launch: suspended searches need source window 1 and no traces");
         return -1;
     }
     *lay = rv_layout_for(ix, bb);
@@ -2297,11 +2407,80 @@ extern "C" int rv_search_launch(const RvIndex *ix, const RvBatch *b_in, int grid
     RvLayout lay;
     if (rv_resolve(ix, b_in, &bb, &lay)) return -1;
     if (bb.nq == 0) return 0;
+    const bool reorder = bb.stop_mode == 1 && bb.suspend_step > 0 && bb.window == 1;
+    if ((bb.phase != 0) != reorder ||
+        (reorder &&
+         (!bb.visited || !bb.susp ||
+          (bb.phase == 2 &&
+           bb.nq_dev != reinterpret_cast<const int32_t *>(
+                            bb.susp + rv_susp_layout((size_t)bb.nq, (size_t)bb.max_pool).count))))) {
+        snprintf(g_rv_error, sizeof(g_rv_error),
+                 "rv_search_launch: suspended searches need per-query visited sets and state");
+        return -1;
+    }
     if (rv_dispatch(0, ix, &bb, 0, lay, nullptr)) return -1;
     if (grid_waves <= 0 && (grid_waves = rv_default_grid(ix, &bb, lay)) < 0) return -1;
     if (grid_waves > bb.nq) grid_waves = bb.nq;
     if (rv_fill_zero(bb.queue, sizeof(int32_t), stream)) return -1;
     return rv_dispatch(2, ix, &bb, grid_waves, lay, (hipStream_t)stream);
+}
+
+// Resumption order of the suspended queries: ascending stop probability in
+// 4096 buckets (a probability below 0, none taken, counts as 0), so the
+// searches the predictor expects to run longest resume first.  One
+// workgroup: bucket counts, an exclusive scan, a scatter (the order inside
+// a bucket is arbitrary; it changes when searches run, not their results).
+#define RV_ORDER_BUCKETS 4096
+__global__ __launch_bounds__(1024) void rv_susp_order_kernel(unsigned char *blk, int nq,
+                                                              int max_pool) {
+    __shared__ int cnt[RV_ORDER_BUCKETS];
+    __shared__ int part[1024];
+    const RvSuspLayout sl = rv_susp_layout((size_t)nq, (size_t)max_pool);
+    const float *prob = reinterpret_cast<const float *>(blk + sl.prob);
+    const int32_t *list = reinterpret_cast<const int32_t *>(blk + sl.list);
+    int32_t *order = reinterpret_cast<int32_t *>(blk + sl.order);
+    const int t = threadIdx.x;
+    int total = *reinterpret_cast<const int32_t *>(blk + sl.count);
+    total = total < nq ? total : nq;
+    for (int i = t; i < RV_ORDER_BUCKETS; i += 1024) cnt[i] = 0;
+    __syncthreads();
+    auto bucket = [&](int i) {
+        const float p = prob[list[i]];
+        int k = (int)(p * (float)RV_ORDER_BUCKETS);
+        return k < 0 ? 0 : (k >= RV_ORDER_BUCKETS ? RV_ORDER_BUCKETS - 1 : k);
+    };
+    for (int i = t; i < total; i += 1024) atomicAdd(&cnt[bucket(i)], 1);
+    __syncthreads();
+    // exclusive scan: four buckets per thread, then over the threads
+    int local = 0;
+    for (int j = 0; j < RV_ORDER_BUCKETS / 1024; j++) local += cnt[t * (RV_ORDER_BUCKETS / 1024) + j];
+    part[t] = local;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = part[t] - local;
+    for (int j = 0; j < RV_ORDER_BUCKETS / 1024; j++) {
+        const int k = t * (RV_ORDER_BUCKETS / 1024) + j;
+        const int c = cnt[k];
+        cnt[k] = run;
+        run += c;
+    }
+    __syncthreads();
+    for (int i = t; i < total; i += 1024) order[atomicAdd(&cnt[bucket(i)], 1)] = list[i];
+}
+
+extern "C" int rv_susp_order(unsigned char *susp, int nq, int max_pool, void *stream) {
+    if (!susp || nq < 0 || max_pool <= 0) {
+        snprintf(g_rv_error, sizeof(g_rv_error), "rv_susp_order: bad arguments");
+        return -1;
+    }
+    if (nq == 0) return 0;
+    rv_susp_order_kernel<<<1, 1024, 0, (hipStream_t)stream>>>(susp, nq, max_pool);
+    return rv_check(hipGetLastError(), "rv_susp_order_kernel launch");
 }
 
 extern "C" int rv_graph_begin(void *stream) {

@@ -78,7 +78,7 @@ class RoverSearchBatch(C.Structure):
                 ("learned_stop", C.c_int), ("stop_threshold", C.c_float),
                 ("refine_mode", C.c_int), ("traversal", C.c_int),
                 ("source_window", C.c_int), ("waves_per_query", C.c_int),
-                ("candidate_tile", C.c_int)]
+                ("candidate_tile", C.c_int), ("reorder_step", C.c_int)]
 
 
 REFINE_CURRENT, REFINE_ERROR4 = 1, 2

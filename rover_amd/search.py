@@ -232,8 +232,10 @@ class Calculator:
                       grid_waves, learned_stop, stop_threshold, pool_sizes,
                       num_outs, seed, entry_ids=None, entry_count=0,
                       entry_stride=0, route_entries=0, refine_mode=0, traversal=0,
-                      source_window=0, waves_per_query=0, candidate_tile=0):
+                      source_window=0, waves_per_query=0, candidate_tile=0,
+                      reorder_step=0):
         b = native.RoverSearchBatch()
+        b.reorder_step = int(reorder_step)
         b.source_window = int(source_window)
         b.waves_per_query = int(waves_per_query)
         b.candidate_tile = int(candidate_tile)
@@ -273,7 +275,7 @@ class Calculator:
                      grid_waves=0, learned_stop=0, stop_threshold=0.0,
                      seed=native.ENTRY_POINT_SEED, entry_ids=None,
                      route_entries=0, refine_mode=0, traversal=0, source_window=0,
-                     waves_per_query=0, candidate_tile=0):
+                     waves_per_query=0, candidate_tile=0, reorder_step=0):
         """Host arrays in and out.  entry_ids: explicit seeds, a list shared
         by all queries or a [nq, S] array of per-query rows (otherwise graph
         entry + LCG seeds); route_entries=S: per-query seeds from the router.  pool_size / num_out: int or per-query
@@ -298,7 +300,7 @@ class Calculator:
                                stop_threshold, ptr(Ls), ptr(Ks), seed, ptr(e),
                                0 if e is None else (stride or len(e)), stride,
                                route_entries, refine_mode, traversal, source_window,
-                               waves_per_query, candidate_tile)
+                               waves_per_query, candidate_tile, reorder_step)
         check(native.lib().rover_search_batch(self.h, C.byref(b)), "rover_search_batch")
         return (ids, d, cnt, st) if stats else (ids, d, cnt)
 
@@ -312,7 +314,7 @@ class Calculator:
                             seed=native.ENTRY_POINT_SEED, d_entry_ids=None,
                             entry_count=0, entry_stride=0, route_entries=0,
                             refine_mode=0, traversal=0, noalloc=False, source_window=0,
-                            waves_per_query=0, candidate_tile=0):
+                            waves_per_query=0, candidate_tile=0, reorder_step=0):
         """Device pointers in and out; asynchronous.  With per-query device
         arrays, pool_size must be their maximum.  noalloc=True calls
         rover_search_batch_device_noalloc (fixed per-device workspace)."""
@@ -323,7 +325,7 @@ class Calculator:
                                learned_stop, stop_threshold, d_pool_sizes,
                                d_num_outs, seed, d_entry_ids, entry_count,
                                entry_stride, route_entries, refine_mode, traversal,
-                               source_window, waves_per_query, candidate_tile)
+                               source_window, waves_per_query, candidate_tile, reorder_step)
         fn = native.lib().rover_search_batch_device_noalloc if noalloc else \
             native.lib().rover_search_batch_device
         check(fn(self.h, C.byref(b), stream),
@@ -367,13 +369,14 @@ class Calculator:
                  refine_candidates=0, entry_points=1, expand_count=32,
                  d_entry_ids=None, entry_count=0, entry_stride=0,
                  seed=native.ENTRY_POINT_SEED, learned_stop=0, stop_threshold=0.0,
-                 refine_mode=0, source_window=0):
+                 refine_mode=0, source_window=0, reorder_step=0):
         """rover_autotune on a device batch; returns native.RoverTuning."""
         b = self._batch_struct(d_queries, nq, pool_size, num_out, method, correction,
                                refine_candidates, entry_points, expand_count, d_ids,
                                d_dists, d_counts, stride, None, 0, 0, 0, learned_stop,
                                stop_threshold, None, None, seed, d_entry_ids, entry_count,
-                               entry_stride, 0, refine_mode, 0, source_window)
+                               entry_stride, 0, refine_mode, 0, source_window,
+                               reorder_step=reorder_step)
         t = native.RoverTuning()
         check(native.lib().rover_autotune(self.h, C.byref(b), stream, repeats, C.byref(t)),
               "rover_autotune")

@@ -77,6 +77,15 @@ def test_cfg1_plain_search_matches_oracle(big, res_bits):
             assert np.array_equal(bits(d), bits(od))
             assert np.array_equal(st.astype(np.int64), ost.astype(np.int64))
             say("  recall@10 %.4f" % data.recall_at_k(ids, cnt, gt[:1000], K))
+            if mode == 1 and L in (64, 256):
+                # exact visited sets in HBM
+                ids, d, cnt, st = calc.search_batch(q, L, K, correction=mode,
+                                                    entry_points=32, stats=True,
+                                                    hash_log2=native.HASH_EXACT)
+                assert np.array_equal(cnt, ocnt)
+                assert np.array_equal(ids, oids)
+                assert np.array_equal(bits(d), bits(od))
+                assert np.array_equal(st.astype(np.int64), ost.astype(np.int64))
 
 
 def test_cfg2_uneven_batches_match_oracle(big):
@@ -95,12 +104,16 @@ def test_cfg2_uneven_batches_match_oracle(big):
                                             stats=True, **EXACT)
         oids, od, ocnt, ost = pyoracle.search_batch(oix, pyoracle.SCORE_ONLINE, q, seeds, 32,
                                                     Ls, ks, threads=16)
-        assert np.array_equal(cnt, ocnt) and np.array_equal(cnt, ks)
-        for i in range(size):
-            c = int(cnt[i])
-            assert np.array_equal(ids[i, :c], oids[i, :c]), i
-            assert np.array_equal(bits(d[i, :c]), bits(od[i, :c])), i
-        assert np.array_equal(st.astype(np.int64), ost.astype(np.int64))
+        for h in (EXACT, dict(hash_log2=native.HASH_EXACT)):
+            if h is not EXACT:
+                ids, d, cnt, st = calc.search_batch(q, Ls, ks, correction=1, entry_points=32,
+                                                    stats=True, **h)
+            assert np.array_equal(cnt, ocnt) and np.array_equal(cnt, ks)
+            for i in range(size):
+                c = int(cnt[i])
+                assert np.array_equal(ids[i, :c], oids[i, :c]), i
+                assert np.array_equal(bits(d[i, :c]), bits(od[i, :c])), i
+            assert np.array_equal(st.astype(np.int64), ost.astype(np.int64))
 
 
 def test_cfg3_learned_stop_at_full_size(big):
@@ -143,6 +156,20 @@ def test_cfg3_learned_stop_at_full_size(big):
         (L, rp, ep, rl, el))
     assert rl >= rp - 0.02
     assert el <= 0.7 * ep
+    # exact visited sets in HBM: the same stops, ids and bits; suspended
+    # after 48 expansions and resumed in predicted order: the same again,
+    # counters included
+    ce = calc.search_batch(q, L, K, correction=1, entry_points=32, stats=True,
+                           learned_stop=1, stop_threshold=0.9, hash_log2=native.HASH_EXACT)
+    assert np.array_equal(c[0], ce[0])
+    assert np.array_equal(bits(c[1]), bits(ce[1]))
+    assert np.array_equal(c[3][:, 2], ce[3][:, 2])
+    cr = calc.search_batch(q, L, K, correction=1, entry_points=32, stats=True,
+                           learned_stop=1, stop_threshold=0.9, hash_log2=native.HASH_EXACT,
+                           reorder_step=48)
+    assert np.array_equal(ce[0], cr[0])
+    assert np.array_equal(bits(ce[1]), bits(cr[1]))
+    assert np.array_equal(ce[3], cr[3])
     # the two-turn pipeline: a threshold no probability reaches gives the
     # plain window-2 search bit for bit
     w = calc.search_batch(q, L, K, correction=1, entry_points=32, stats=True,
