@@ -1247,6 +1247,10 @@ def main():
                 "kernel_ms": round(kernel_ms, 4),
                 "counters": "distinct scored / expanded nodes (exact visited sets); "
                             "the plain search's equal the oracle's PurePqSearchStats"}
+    if tl_[5] and not tl_[3]:
+        roofline["launch"] = ("per step: %s twice (searches up to %d expansions, then the "
+                              "suspended ones resumed) and rv_susp_order_kernel; kernel_ms "
+                              "and traffic are per step" % (kernel, tl_[5]))
     # HBM bytes per launch come from the profiler's FETCH_SIZE / WRITE_SIZE
     # passes over this same workload (calibrated against a copy kernel),
     # recorded next to the profile summary; used when kernel and L match
