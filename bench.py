@@ -510,8 +510,8 @@ def main():
     ap.add_argument("--label-slacks", default="0,1",
                     help="label slacks to train (label 1 once all but `slack` of the "
                          "final top-K are found); the best at the target is kept")
-    ap.add_argument("--stop-every", type=int, default=12)
-    ap.add_argument("--trace-cap", type=int, default=24,
+    ap.add_argument("--stop-every", type=int, default=16)
+    ap.add_argument("--trace-cap", type=int, default=18,
                     help="trace records per training query (steps up to "
                          "trace-cap * stop-every)")
     ap.add_argument("--train-steps", type=int, default=2000)
