@@ -686,8 +686,14 @@ __device__ float rv_mlp_logit(const RvMlp &mlp_in, float f_lane, float *hbuf) {
     else rv_wave_sync();
     h2 = h2 > 0.0f ? h2 : 0.0f;
     float o = rv_bf16(mlp.w3[j]) * h2;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) o += __shfl_xor(o, off);
+    // butterfly sum over the wave (partners lane ^ 32, 16, ..., 1, in that
+    // order) by permlane, swizzle and DPP exchanges
+    o += __uint_as_float(rv_xor_lane<32>(__float_as_uint(o)));
+    o += __uint_as_float(rv_xor_lane<16>(__float_as_uint(o)));
+    o += __uint_as_float(rv_xor_lane<8>(__float_as_uint(o)));
+    o += __uint_as_float(rv_xor_lane<4>(__float_as_uint(o)));
+    o += __uint_as_float(rv_xor_lane<2>(__float_as_uint(o)));
+    o += __uint_as_float(rv_xor_lane<1>(__float_as_uint(o)));
     return o + mlp.b3[0];
 }
 
@@ -887,37 +893,51 @@ void rv_search_kernel(RvIndex ix, RvBatch b, RvLayout lay) {
             // Pool entry j moves right by #{valid c : lb_c <= j}: no
             // candidate equals a pool key, so candidate c sorts below entry
             // j exactly when lb_c <= j.  New positions increase with j, so
-            // moving the pool in place, last 64-entry chunk first and each
+            // moving the pool in place, last 128-entry chunk first and each
             // chunk read before it is written, never overwrites an entry
             // that has not moved yet; entries left of the smallest lb stay.
-            // The valid lanes hold ascending keys and lbs; walking them from
-            // the top, `above` counts those right of the current chunk.
+            // Lane l moves entries j0 + 2l and j0 + 2l + 1, read with one
+            // 16-byte load (the pair's flags with one 2-byte load).  The
+            // valid lanes hold ascending keys and lbs; walking them from the
+            // top, `above` counts those right of the current chunk.
             const int lb_min =
                 __builtin_amdgcn_readlane(lb, __ffsll((unsigned long long)vmask) - 1);
             uint64_t rem = vmask;
             int above = 0;
-            for (int j0 = ((n - 1) >> 6) << 6; j0 >= 0 && j0 + RV_WAVE > lb_min;
-                 j0 -= RV_WAVE) {
-                const int j = j0 + lane;
-                int right = above;
+            for (int j0 = ((n - 1) >> 7) << 7; j0 >= 0 && j0 + 2 * RV_WAVE > lb_min;
+                 j0 -= 2 * RV_WAVE) {
+                const int ja = j0 + 2 * lane;
+                int right_a = above, right_b = above;
                 while (rem) {
                     const int top = 63 - __clzll((long long)rem);
                     const int lbc = __builtin_amdgcn_readlane(lb, top);
                     if (lbc < j0) break;
-                    right += lbc > j ? 1 : 0;
+                    right_a += lbc > ja ? 1 : 0;
+                    right_b += lbc > ja + 1 ? 1 : 0;
                     above++;
                     rem &= ~(1ull << top);
                 }
-                const int shift = cv - right;
-                if (j < n && shift > 0) {
-                    const uint64_t pk = pool_keys[j];
-                    const uint8_t pf = pool_flags[j];
-                    const uint16_t pb = tracing ? pool_born[j] : 0;
-                    const int np = j + shift;
-                    if (np < L) {
-                        pool_keys[np] = pk;
-                        pool_flags[np] = pf;
-                        if (tracing) pool_born[np] = pb;
+                // shift_b >= shift_a (entry ja + 1 has every candidate of
+                // entry ja below it, and those landing between them)
+                const int shift_a = cv - right_a, shift_b = cv - right_b;
+                if (ja < n && shift_b > 0) {
+                    // the pair is read whole; indices up to max_pool stay
+                    // inside the LDS layout (each array is padded to 16
+                    // bytes and followed by another)
+                    const ulonglong2 pk = *reinterpret_cast<const ulonglong2 *>(pool_keys + ja);
+                    const uint16_t pf = *reinterpret_cast<const uint16_t *>(pool_flags + ja);
+                    const uint32_t pb =
+                        tracing ? *reinterpret_cast<const uint32_t *>(pool_born + ja) : 0u;
+                    const int npa = ja + shift_a, npb = ja + 1 + shift_b;
+                    if (shift_a > 0 && npa < L) {
+                        pool_keys[npa] = pk.x;
+                        pool_flags[npa] = (uint8_t)pf;
+                        if (tracing) pool_born[npa] = (uint16_t)pb;
+                    }
+                    if (ja + 1 < n && npb < L) {
+                        pool_keys[npb] = pk.y;
+                        pool_flags[npb] = (uint8_t)(pf >> 8);
+                        if (tracing) pool_born[npb] = (uint16_t)(pb >> 16);
                     }
                 }
             }
