@@ -36,7 +36,6 @@
 // Bound: VALU (the exact distances are sequential per pair and cannot use
 // the matrix cores without changing their rounding).
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <float.h>
 #include <stdint.h>
@@ -703,11 +702,8 @@ extern "C" int rv_build_rnn(const float *x, int n, int dim, int s, int r, int t1
              rb_check(hipMalloc((void **)&pk2, slots * 8), "hipMalloc rnn keys") ||
              rb_check(hipMalloc((void **)&pv, slots * 4), "hipMalloc rnn values") ||
              rb_check(hipMalloc((void **)&pv2, slots * 4), "hipMalloc rnn values");
-    if (!rc)
-        rc = rb_check(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, pk, pk2, pv, pv2,
-                                                         (int)slots, 0, 64, st),
-                      "radix sort size");
-    if (!rc) rc = rb_check(hipMalloc(&tmp, tmp_bytes > 0 ? tmp_bytes : 16), "hipMalloc sort");
+    tmp_bytes = rv_radix_scratch_bytes(slots, 1);
+    if (!rc) rc = rb_check(hipMalloc(&tmp, tmp_bytes), "hipMalloc sort");
     int grid = 0;
     if (!rc) {
         int dev = 0, cus = 0;
@@ -718,10 +714,8 @@ extern "C" int rv_build_rnn(const float *x, int n, int dim, int s, int r, int t1
     }
     // sorts the pending insertions and applies them per source node
     auto apply = [&]() -> int {
-        size_t b = tmp_bytes;
-        if (rb_check(hipcub::DeviceRadixSort::SortPairs(tmp, b, pk, pk2, pv, pv2, (int)slots, 0,
-                                                        64, st),
-                     "radix sort"))
+        if (rv_radix_sort(pk, pk2, reinterpret_cast<const uint32_t *>(pv),
+                          reinterpret_cast<uint32_t *>(pv2), slots, 0, 64, tmp, tmp_bytes, st))
             return -1;
         rv_rnn_apply_kernel<<<grid, RB_WAVE, 0, st>>>(n, r, ids, dists, fresh, deg, pk2, pv2,
                                                       slots);

@@ -30,7 +30,6 @@
 // key for key (ties by id included).  A clustering that matches the data
 // (k-means) leaves few pairs for step 4.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <float.h>
 #include <math.h>
@@ -513,15 +512,12 @@ extern "C" int rv_build_exact_clustered(const float *x, int n, int dim, int r, i
         // the key's high word is a block index below nblocks
         int end_bit = 33;
         while (end_bit < 64 && ((uint64_t)nblocks >> (end_bit - 32)) != 0) end_bit++;
+        tmp_bytes = rv_radix_scratch_bytes((size_t)npairs, 0);
         if (kn_check(hipMalloc((void **)&d_sorted, (size_t)npairs * 8), "hipMalloc") ||
-            kn_check(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp_bytes, d_pairs, d_sorted,
-                                                       npairs, 0, end_bit, s),
-                     "radix sort size") ||
             kn_check(hipMalloc(&d_tmp, tmp_bytes), "hipMalloc"))
             goto done;
-        if (kn_check(hipcub::DeviceRadixSort::SortKeys(d_tmp, tmp_bytes, d_pairs, d_sorted,
-                                                       npairs, 0, end_bit, s),
-                     "radix sort") ||
+        if (rv_radix_sort(d_pairs, d_sorted, nullptr, nullptr, (size_t)npairs, 0, end_bit, d_tmp,
+                          tmp_bytes, s) ||
             rv_fill_zero(d_pfirst, (size_t)nblocks * 4, s) ||
             rv_fill_zero(d_plast, (size_t)nblocks * 4, s) || rv_fill_zero(d_listed, 8, s))
             goto done;

@@ -307,6 +307,15 @@ int rv_noalloc_scratch(int device, void **d_scratch);
  * (the longest expected first), 4096 probability buckets. */
 int rv_susp_order(unsigned char *susp, int nq, int max_pool, void *stream);
 
+/* Stable LSD radix sort of n 64-bit keys on bits [begin_bit, end_bit)
+ * (bits outside are ignored for the order), with optional 32-bit values;
+ * the inputs are left as they are.  scratch: rv_radix_scratch_bytes(n,
+ * values given) bytes of device memory. */
+size_t rv_radix_scratch_bytes(size_t n, int with_values);
+int rv_radix_sort(const uint64_t *keys_in, uint64_t *keys_out, const uint32_t *vals_in,
+                  uint32_t *vals_out, size_t n, int begin_bit, int end_bit, void *scratch,
+                  size_t scratch_bytes, void *stream);
+
 int rv_seed_list(int32_t *d_out, int n, int count, int graph_entry, uint32_t seed,
                  void *stream);
 
