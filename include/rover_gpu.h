@@ -39,6 +39,11 @@ typedef uint64_t remote_handle64;
 
 #define ROVER_ABI_VERSION 1        /* search-only, as calculator_rover_configure */
 #define ROVER_MAX_POOL 2000        /* rover_limits.h:40 DSP_SEARCH_MAX_POOL */
+/* Graph rows of up to this many slots load (the reference checks none);
+   a search reads min(expand_count, degree) of them per expansion, at most
+   64 (one wavefront's lanes): a wider expansion is refused with
+   ROVER_EBADPARM, the SQ heap traversals take at most 32. */
+#define ROVER_MAX_DEGREE 1024
 #define ROVER_ENTRY_POINT_SEED 42u /* rover_limits.h:38, config.c:18 */
 
 /* ---------------------------------------------------------------- 1. IDL */

@@ -265,7 +265,7 @@ AEEResult rover_pure_pq_graph_load(remote_handle64 h, const int32_t *graph,
     RoverCtx *c = ctx_of(h);
     if (!c) return ROVER_EBADPARM;
     if (!c->pq_loaded || !graph || num_vectors != c->n || graph_k <= 0 ||
-        graph_k > 64 || graphLen != (int64_t)num_vectors * graph_k ||
+        graph_k > ROVER_MAX_DEGREE || graphLen != (int64_t)num_vectors * graph_k ||
         entry >= num_vectors)
         return ROVER_EBADPARM;
     dfree(c->d_graph);
@@ -619,7 +619,7 @@ AEEResult calculator_retrieval_load(remote_handle64 h, const int32_t *graph,
     if (!c) return ROVER_EBADPARM;
     cb = c->l1_bits == 4 ? (dim + 1) / 2 : dim;
     if (!graph || !l1_codes || num_vectors <= 0 || dim <= 0 || dim > 4096 ||
-        graph_k <= 0 || graph_k > 64 ||
+        graph_k <= 0 || graph_k > ROVER_MAX_DEGREE ||
         (int64_t)graphLen != (int64_t)num_vectors * graph_k ||
         (int64_t)l1_codesLen != (int64_t)num_vectors * cb ||
         !isfinite(l1_scale) || !(l1_scale > 0.0f) || !isfinite(l1_zero) ||
@@ -662,7 +662,7 @@ AEEResult calculator_retrieval_load_dma(remote_handle64 h, const int32_t *graph,
     size_t cb;
     if (!c) return ROVER_EBADPARM;
     calculator_retrieval_unload(h);
-    if (num_vectors <= 0 || dim <= 0 || dim > 4096 || graph_k <= 0 || graph_k > 64 ||
+    if (num_vectors <= 0 || dim <= 0 || dim > 4096 || graph_k <= 0 || graph_k > ROVER_MAX_DEGREE ||
         !l1_codes || l1_codesLen < 1 || l1_codesLen > ROVER_MAX_L1_CHUNKS ||
         l1_chunk_vectors <= 0 ||
         (int64_t)l1_codesLen * l1_chunk_vectors < (int64_t)num_vectors ||
@@ -722,7 +722,7 @@ AEEResult rover_sq_graph_load(remote_handle64 h, const int32_t *graph, int64_t g
     RoverCtx *c = ctx_of(h);
     if (!c) return ROVER_EBADPARM;
     if (!c->sq_loaded) return ROVER_EFAILED;
-    if (!graph || num_vectors != c->sq_n || graph_k <= 0 || graph_k > 64 ||
+    if (!graph || num_vectors != c->sq_n || graph_k <= 0 || graph_k > ROVER_MAX_DEGREE ||
         graphLen != (int64_t)num_vectors * graph_k || graph_entry_point >= num_vectors)
         return ROVER_EBADPARM;
     dfree(c->d_sq_graph);
@@ -987,6 +987,14 @@ AEEResult rover_prepare_launch(RoverCtx *c, const RoverSearchBatch *b,
     if (fill_index(c, b->method, b->correction, b->refine_candidates, ix)) {
         snprintf(g_err, sizeof(g_err), "search: index or its graph not loaded");
         return ROVER_EFAILED;
+    }
+    if ((b->expand_count < ix->degree ? b->expand_count : ix->degree) > 64) {
+        /* one wavefront reads and scores one expansion's neighbors */
+        snprintf(g_err, sizeof(g_err),
+                 "search: at most 64 neighbors per expansion (expand_count %d, graph "
+                 "degree %d)",
+                 b->expand_count, ix->degree);
+        return ROVER_EBADPARM;
     }
     if (!explicit_entries && !b->entry_ids &&
         ensure_entries(c, ix->n, b->entry_points, ix->graph_entry, seed))
@@ -1943,7 +1951,7 @@ int retrieval_pure_pq_search_scalar_with_policy(
     if (out_stats) memset(out_stats, 0, sizeof(*out_stats));
     /* retrieval_sqmd_pure_pq.c:251-278 (pure_pq_validate_search) */
     if (!ix || !rq || !out_ids || !out_distances || out_capacity <= 0 ||
-        !ix->graph || ix->graph_degree <= 0 || ix->graph_degree > 64 ||
+        !ix->graph || ix->graph_degree <= 0 || ix->graph_degree > ROVER_MAX_DEGREE ||
         ix->num_vectors <= 0 || !ix->code_chunks || !ix->chunk_vector_counts ||
         ix->chunk_count <= 0 || ix->chunk_vectors <= 0 || !ix->codebook ||
         ix->dim <= 0 || ix->m <= 0 || (ix->ks != 16 && ix->ks != 256) ||

@@ -2343,7 +2343,8 @@ static int rv_dispatch(int op, const RvIndex *ix, const RvBatch *b, int grid, Rv
 static int rv_resolve(const RvIndex *ix, const RvBatch *b_in, RvBatch *bb, RvLayout *lay) {
     if (!ix || !b_in || !b_in->queue || b_in->nq < 0 || b_in->hash_log2 < 4 ||
         b_in->hash_log2 > 16 || b_in->max_pool <= 0 || b_in->probe_limit <= 0 ||
-        b_in->n_entries <= 0 || b_in->expand_count <= 0 || ix->degree > RV_WAVE ||
+        b_in->n_entries <= 0 || b_in->expand_count <= 0 || ix->degree <= 0 ||
+        (b_in->expand_count < ix->degree ? b_in->expand_count : ix->degree) > RV_WAVE ||
         b_in->window < 0 || b_in->window > 2 || (b_in->window == 2 && b_in->stop_mode == 2) ||
         ix->dim <= 0 || ix->dim > 4096 || b_in->stop_mode < 0 || b_in->stop_mode > 2 ||
         (b_in->stop_mode && (b_in->stop_every <= 0 || b_in->stop_label_k <= 0)) ||
