@@ -511,6 +511,8 @@ def main():
                     help="label slacks to train (label 1 once all but `slack` of the "
                          "final top-K are found); the best at the target is kept")
     ap.add_argument("--stop-every", type=int, default=16)
+    ap.add_argument("--stop-threshold", type=float, default=0.9,
+                    help="stop probability at which the learned stop ends a search")
     ap.add_argument("--trace-cap", type=int, default=18,
                     help="trace records per training query (steps up to "
                          "trace-cap * stop-every)")
@@ -686,7 +688,7 @@ def main():
                                  expand_count=args.R,
                                  d_stats=d_st.ptr if stats else None,
                                  learned_stop=1 if learned else 0,
-                                 stop_threshold=0.9, d_entry_ids=es["ptr"],
+                                 stop_threshold=args.stop_threshold, d_entry_ids=es["ptr"],
                                  entry_count=es["S"], entry_stride=es["S"], hash_log2=h2,
                                  probe_limit=pl, grid_waves=gw, source_window=win["w"],
                                  waves_per_query=wq, candidate_tile=tl, reorder_step=rs)
@@ -696,7 +698,8 @@ def main():
                           correction=1 if mode == "online" else 0,
                           refine_candidates=refine_of(mode, L), expand_count=args.R,
                           d_entry_ids=es["ptr"], entry_count=es["S"], entry_stride=es["S"],
-                          learned_stop=1 if learned else 0, refine_mode=final_rmode,
+                          learned_stop=1 if learned else 0,
+                          stop_threshold=args.stop_threshold, refine_mode=final_rmode,
                           source_window=win["w"])
         tuned[(mode, L, learned, win["w"])] = (t.hash_log2, t.probe_limit, t.grid_waves,
                                               t.waves_per_query, t.candidate_tile, 0)
@@ -820,7 +823,8 @@ def main():
             structs.append(calc._batch_struct(
                 pin_q.ptr, nq, L, k, native.METHOD_PQ, 1 if mode == "online" else 0,
                 refine_of(mode, L), 1, args.R, pin_ids.ptr, pin_d.ptr, pin_cnt.ptr, k, None,
-                h2, pl, gw, 1 if learned else 0, 0.9, None, None, native.ENTRY_POINT_SEED,
+                h2, pl, gw, 1 if learned else 0, args.stop_threshold, None, None,
+                native.ENTRY_POINT_SEED,
                 pin_e.ptr, S, S, 0, final_rmode, 0, 0, wq, tl, reorder_step=rs))
         times = [[] for _ in configs]
         gc.collect()
@@ -913,7 +917,7 @@ def main():
                             stop_every=args.stop_every, trace_cap=args.trace_cap,
                             samples=int(t.samples), positive_frac=round(t.positive_frac, 4),
                             final_loss=round(t.final_loss, 5),
-                            seconds=round(time.time() - tt, 2), threshold=0.9)
+                            seconds=round(time.time() - tt, 2), threshold=args.stop_threshold)
             log("learned stop trained:", training)
             model = [np.zeros(P, np.float32), np.zeros(16, np.float32),
                      np.zeros(16, np.float32)]
@@ -1142,7 +1146,7 @@ def main():
                                      refine_candidates=refine_of(mode, lo_L),
                                      refine_mode=final_rmode,
                                      expand_count=args.R, learned_stop=1 if lrn else 0,
-                                     stop_threshold=0.9, d_pool_sizes=d_L.ptr,
+                                     stop_threshold=args.stop_threshold, d_pool_sizes=d_L.ptr,
                                      d_entry_ids=es["ptr"], entry_count=es["S"],
                                      entry_stride=es["S"])
 
@@ -1366,7 +1370,8 @@ def main():
             calc.search_batch_device(dq.ptr, nq, Lx, Lx, mids.ptr, md.ptr, mcnt.ptr, Lx,
                                      correction=1 if mode == "online" else 0,
                                      expand_count=args.R, d_stats=mst.ptr,
-                                     learned_stop=1 if lrn else 0, stop_threshold=0.9,
+                                     learned_stop=1 if lrn else 0,
+                                     stop_threshold=args.stop_threshold,
                                      d_entry_ids=d_ent.ptr, entry_count=S, entry_stride=S,
                                      hash_log2=native.HASH_EXACT)
             native.synchronize()
