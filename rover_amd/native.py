@@ -209,6 +209,15 @@ def lib():
             raise RoverError(
                 "librover.so not built (%s): run __graft_entry__.build()"
                 % LIB_PATH)
+        # One HIP runtime per process: librover.so needs libamdhip64.so.7
+        # by name, and a library of that name already loaded (torch's, when
+        # torch is imported first) serves it.  Loading librover.so first
+        # would map the system runtime, and torch would then load its own
+        # copy beside it; importing torch here keeps torch's the only one.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, res, args in SIGNATURES:
             if not hasattr(L, name):
